@@ -1,0 +1,117 @@
+/* vgpu.h -- the extern "C" boundary of the vectis MI355X engine (libvgpu.so).
+ *
+ * Plain pointers and sizes only. Every function returns VGPU_OK (0) on success and VGPU_ERROR (1) on
+ * failure, with the reason in vgpu_last_error() (thread-local). Unless a comment says otherwise, pointers
+ * named *_dev / columns are DEVICE pointers, and `stream` is a hipStream_t (NULL = the null stream); calls
+ * are ordered on that stream and return once their work is queued, except where they say they
+ * synchronize (calls that must read a size back).
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the reference tree).
+ * The C++ drop-in headers (gpu_engine.hpp and companions, same declarations as cuda_engine/include) sit on
+ * top of the same library; this C layer is what ctypes / cgo / JNI callers and the parity tests bind.
+ */
+#ifndef VGPU_H
+#define VGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VGPU_OK 0
+#define VGPU_ERROR 1
+
+/* Value types: numbered exactly like vector_gpu::GpuValueType (cuda_engine/include/gpu_engine.hpp:29-38). */
+enum vgpu_type {
+	VGPU_INT16 = 0,
+	VGPU_INT32 = 1,
+	VGPU_INT64 = 2,
+	VGPU_FLOAT32 = 3,
+	VGPU_FLOAT64 = 4,
+	VGPU_BOOLEAN = 5,
+	VGPU_DICTIONARY_STRING = 6,
+	VGPU_HUGEINT = 7
+};
+
+/* Aggregate kinds: numbered like vector_gpu::GpuAggregateKind (cuda_engine/include/gpu_engine.hpp:85-92). */
+enum vgpu_agg_kind { VGPU_COUNT_STAR = 0, VGPU_COUNT = 1, VGPU_SUM = 2, VGPU_MIN = 3, VGPU_MAX = 4, VGPU_AVG = 5 };
+
+typedef struct vgpu_column {
+	const void *data; /* device pointer, tightly packed */
+	int32_t type;     /* enum vgpu_type */
+	uint64_t rows;
+} vgpu_column;
+
+typedef struct vgpu_agg {
+	int32_t kind;        /* enum vgpu_agg_kind */
+	int32_t input;       /* index into the value columns, -1 for COUNT(*) */
+	int32_t output_type; /* enum vgpu_type: COUNT -> INT64, SUM(int) -> HUGEINT, SUM(float) -> FLOAT64,
+	                        MIN/MAX -> input type, AVG -> FLOAT64 */
+} vgpu_agg;
+
+/* ---------------------------------------------------------------- library */
+
+/* Reason for the last VGPU_ERROR returned on this thread (empty string if none). */
+const char *vgpu_last_error(void);
+/* ABI version of this header (bumped on any signature change). */
+int vgpu_abi_version(void);
+/* Makes `device` current for this thread and initializes the runtime; replaces the context bring-up in
+ * RmmPool::EnsureInitialized (cuda_engine/src/rmm_pool.cpp:90-150). */
+int vgpu_device_init(int device);
+
+/* ---------------------------------------------------------------- GROUP BY (radix sort + segmented reduce)
+ * Replaces ExecuteGroupByNode (cuda_engine/src/operators/gpu_groupby.cu:303-592) for device-resident
+ * columns. Keys: 0-4 integer/boolean columns (0 = one global group). Outputs: one device buffer per key
+ * (key type) and per aggregate (output_type), each with room for `capacity` rows; the group count is
+ * written to *groups (host). Synchronizes. Fails (without writing) when `capacity` is below the bound that
+ * vgpu_group_by_bound returns. */
+int vgpu_group_by_bound(const vgpu_column *keys, int32_t nkeys, uint64_t rows, uint64_t *bound, void *stream);
+int vgpu_group_by(const vgpu_column *keys, int32_t nkeys, const vgpu_column *values, int32_t nvalues,
+                  const vgpu_agg *aggs, int32_t naggs, uint64_t rows, void *const *key_out, void *const *agg_out,
+                  uint64_t capacity, uint64_t *groups, void *stream);
+
+/* Stable sort of one integer key column: keys_out receives the sorted keys, perm_out (uint32, may be NULL)
+ * the source row of each output row. Replaces thrust::sequence + stable_sort_by_key
+ * (cuda_engine/src/operators/gpu_groupby.cu:352-376, cuda_engine/src/operators/gpu_hash_join.cu:205-206).
+ * Requires rows < 2^30. */
+int vgpu_sort_column(const vgpu_column *key, void *keys_out, uint32_t *perm_out, void *stream);
+
+/* ---------------------------------------------------------------- primitives */
+
+/* Order-preserving copy of in[i] > threshold into out; *count_host receives the kept count (synchronizes).
+ * Replaces FilterGreaterThanInt32 (cuda_engine/include/gpu_filter_ops.hpp:32,
+ * cuda_engine/src/operators/gpu_filter.cu:36-85). */
+int vgpu_filter_gt_i32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_host,
+                       void *stream);
+
+/* out[i] = src[idx[i]] for elements of `width` bytes (1/2/4/8/16); idx is uint32 (idx64 = 0) or uint64.
+ * Replaces GatherColumn / MaterializeColumn (cuda_engine/src/gpu_executor.cu:144-252). */
+int vgpu_gather(const void *src, int32_t width, const void *idx, int32_t idx64, uint64_t n, void *out,
+                void *stream);
+
+/* Seeded synthetic column: dist 0 uniform int [lo, hi], 1 uniform double [0,1), 2 normal(0,1), 3 Zipf
+ * (cdf/perm device tables, cdf_len entries), 4 sequence lo + i. No reference counterpart (test/bench data). */
+int vgpu_generate_column(void *out, int32_t type, uint64_t n, int32_t dist, int64_t lo, int64_t hi,
+                         uint64_t seed, uint64_t stream_id, const double *cdf, uint64_t cdf_len,
+                         const int64_t *perm, void *stream);
+
+/* ---------------------------------------------------------------- measurement */
+
+/* When on, every kernel launch is bracketed by HIP events on its own stream. */
+int vgpu_profile_enable(int32_t on);
+int vgpu_profile_reset(void);
+/* JSON array [{"name":..,"ms":..,"launches":..,"bytes":..}, ...] into buf (synchronizes). Returns the
+ * needed size in *needed when buf is too small. */
+int vgpu_profile_read(char *buf, size_t cap, size_t *needed);
+
+/* Device pool statistics (bytes). */
+int vgpu_pool_stats(uint64_t *in_use, uint64_t *cached, uint64_t *peak);
+int vgpu_pool_trim(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VGPU_H */

@@ -1,0 +1,211 @@
+"""GROUP BY parity: the HIP radix-sort + segmented-reduce path (through the C ABI) against the oracle.
+
+Bar: bit-exact for keys, COUNT, SUM(int) (HUGEINT), MIN, MAX; DOUBLE SUM/AVG within 1e-12 relative of the
+correctly rounded sum (tolerance stated in the north-star contract; the engine's double-double accumulation
+is normally exact).
+"""
+import numpy as np
+import pytest
+
+from vectisdb_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+AGGS7 = [
+    (capi.COUNT_STAR, -1, capi.INT64),
+    (capi.SUM, 0, capi.HUGEINT),
+    (capi.MIN, 0, capi.INT64),
+    (capi.MAX, 0, capi.INT64),
+    (capi.SUM, 1, capi.FLOAT64),
+    (capi.AVG, 1, capi.FLOAT64),
+]
+
+
+def _hugeint_to_ints(t):
+    a = t.cpu().numpy().reshape(-1, 2)
+    return [int(lo) % (1 << 64) + (int(hi) << 64) for lo, hi in a]
+
+
+def _run(dev, key, v1, v2):
+    import torch
+    k = torch.from_numpy(key).to(dev)
+    a = torch.from_numpy(v1).to(dev)
+    b = torch.from_numpy(v2).to(dev)
+    keys, aggs, g = capi.group_by([k], [a, b], AGGS7, len(key))
+    return keys, aggs, g
+
+
+def _check(dev, key, v1, v2):
+    keys, aggs, g = _run(dev, key, v1, v2)
+    ref = capi_ref = None
+    import oracle
+    ref = oracle.group_by_i64(key, v1, v2)
+    assert g == len(ref["key"])
+    gk = keys[0].cpu().numpy()
+    order = np.argsort(gk, kind="stable")  # groups come out in ascending key order; sort defensively
+    np.testing.assert_array_equal(gk[order], ref["key"])
+    np.testing.assert_array_equal(aggs[0].cpu().numpy()[order], ref["count"])
+    s1 = _hugeint_to_ints(aggs[1])
+    assert [s1[i] for i in order] == ref["sum_v1"]
+    np.testing.assert_array_equal(aggs[2].cpu().numpy()[order], ref["min_v1"])
+    np.testing.assert_array_equal(aggs[3].cpu().numpy()[order], ref["max_v1"])
+    s2 = aggs[4].cpu().numpy()[order]
+    avg = aggs[5].cpu().numpy()[order]
+    tol = 1e-12
+    np.testing.assert_allclose(s2, ref["sum_v2"], rtol=tol, atol=0)
+    np.testing.assert_allclose(avg, ref["avg_v2"], rtol=tol, atol=0)
+    return g
+
+
+@pytest.mark.parametrize("n,distinct", [(1, 1), (7, 3), (2048, 17), (2049, 2049), (100_000, 1000),
+                                        (1_000_003, 50_000), (3_000_000, 1_000_000)])
+def test_groupby_uniform_keys(dev, n, distinct):
+    rng = np.random.default_rng(n)
+    key = rng.integers(0, distinct, n, dtype=np.int64)
+    v1 = rng.integers(-1_000_000, 1_000_001, n, dtype=np.int64)
+    v2 = rng.standard_normal(n)
+    _check(dev, key, v1, v2)
+
+
+def test_groupby_zipf_skew(dev):
+    """A few keys own most rows: groups span many tiles, exercising the tail/lead fix-up."""
+    rng = np.random.default_rng(7)
+    n = 4_000_000
+    ranks = rng.zipf(1.1, n)
+    ranks = ranks[ranks <= 1_000_000]
+    perm = rng.permutation(1_000_000).astype(np.int64)
+    key = perm[ranks - 1]
+    n = len(key)
+    v1 = rng.integers(-1_000_000, 1_000_001, n, dtype=np.int64)
+    v2 = rng.standard_normal(n)
+    _check(dev, key, v1, v2)
+
+
+def test_groupby_single_key_all_rows(dev):
+    n = 1_500_000
+    key = np.full(n, 42, np.int64)
+    rng = np.random.default_rng(3)
+    _check(dev, key, rng.integers(-5, 5, n, dtype=np.int64), rng.standard_normal(n))
+
+
+def test_groupby_negative_and_wide_keys(dev):
+    rng = np.random.default_rng(11)
+    n = 500_000
+    key = rng.choice(np.array([-2**62, -1, 0, 1, 2**40, 2**62 - 1], np.int64), n)
+    key[::7] = rng.integers(-2**63, 2**63 - 1, len(key[::7]), dtype=np.int64)
+    _check(dev, key, rng.integers(-10**6, 10**6, n, dtype=np.int64), rng.standard_normal(n))
+
+
+def test_groupby_sorted_and_reverse_input(dev):
+    n = 300_000
+    key = np.repeat(np.arange(1000, dtype=np.int64), n // 1000)
+    rng = np.random.default_rng(5)
+    v1 = rng.integers(-10**6, 10**6, len(key), dtype=np.int64)
+    v2 = rng.standard_normal(len(key))
+    _check(dev, key, v1, v2)
+    _check(dev, key[::-1].copy(), v1, v2)
+
+
+def test_groupby_global_aggregate(dev):
+    import torch
+    rng = np.random.default_rng(9)
+    n = 777_777
+    v1 = rng.integers(-10**6, 10**6, n, dtype=np.int64)
+    v2 = rng.standard_normal(n)
+    a = torch.from_numpy(v1).to(dev)
+    b = torch.from_numpy(v2).to(dev)
+    _, aggs, g = capi.group_by([], [a, b], AGGS7, n)
+    assert g == 1
+    assert int(aggs[0].cpu()[0]) == n
+    assert _hugeint_to_ints(aggs[1])[0] == int(v1.sum())
+    assert int(aggs[2].cpu()[0]) == v1.min() and int(aggs[3].cpu()[0]) == v1.max()
+    import math
+    exact = math.fsum(v2)
+    assert abs(float(aggs[4].cpu()[0]) - exact) <= 1e-12 * abs(exact)
+
+
+def test_groupby_int32_keys_and_narrow_minmax(dev):
+    import torch
+    rng = np.random.default_rng(13)
+    n = 200_000
+    key = rng.integers(-50, 50, n).astype(np.int32)
+    v = rng.integers(-30000, 30000, n).astype(np.int16)
+    keys, aggs, g = capi.group_by([torch.from_numpy(key).to(dev)], [torch.from_numpy(v).to(dev)],
+                                  [(capi.MIN, 0, capi.INT16), (capi.MAX, 0, capi.INT16), (capi.COUNT, 0, capi.INT64),
+                                   (capi.SUM, 0, capi.HUGEINT)], n)
+    gk = keys[0].cpu().numpy()
+    uk = np.unique(key)
+    np.testing.assert_array_equal(np.sort(gk), uk)
+    mins = aggs[0].cpu().numpy()
+    maxs = aggs[1].cpu().numpy()
+    cnts = aggs[2].cpu().numpy()
+    sums = _hugeint_to_ints(aggs[3])
+    for i, k in enumerate(gk):
+        sel = v[key == k]
+        assert mins[i] == sel.min() and maxs[i] == sel.max() and cnts[i] == len(sel)
+        assert sums[i] == int(sel.astype(np.int64).sum())
+
+
+def test_groupby_two_key_columns(dev):
+    import torch
+    rng = np.random.default_rng(17)
+    n = 400_000
+    k1 = rng.integers(0, 40, n).astype(np.int32)
+    k2 = rng.integers(-1000, 1000, n).astype(np.int64)
+    v = rng.integers(-10**6, 10**6, n).astype(np.int64)
+    keys, aggs, g = capi.group_by([torch.from_numpy(k1).to(dev), torch.from_numpy(k2).to(dev)],
+                                  [torch.from_numpy(v).to(dev)],
+                                  [(capi.COUNT_STAR, -1, capi.INT64), (capi.SUM, 0, capi.HUGEINT)], n)
+    got = {}
+    c = aggs[0].cpu().numpy()
+    s = _hugeint_to_ints(aggs[1])
+    for a, b, cc, ss in zip(keys[0].cpu().numpy(), keys[1].cpu().numpy(), c, s):
+        got[(int(a), int(b))] = (int(cc), ss)
+    exp = {}
+    for a, b, vv in zip(k1.tolist(), k2.tolist(), v.tolist()):
+        cc, ss = exp.get((a, b), (0, 0))
+        exp[(a, b)] = (cc + 1, ss + vv)
+    assert got == exp
+    # groups come out in lexicographic (k1, k2) order
+    pairs = list(zip(keys[0].cpu().numpy().tolist(), keys[1].cpu().numpy().tolist()))
+    assert pairs == sorted(pairs)
+
+
+def test_groupby_empty_input(dev):
+    import torch
+    k = torch.empty(0, dtype=torch.int64, device=dev)
+    v = torch.empty(0, dtype=torch.int64, device=dev)
+    keys, aggs, g = capi.group_by([k], [v], [(capi.COUNT_STAR, -1, capi.INT64)], 0, capacity=1)
+    assert g == 0
+
+
+def test_groupby_rejects_float_key_and_bad_types(dev):
+    import torch
+    f = torch.zeros(10, dtype=torch.float64, device=dev)
+    i = torch.zeros(10, dtype=torch.int64, device=dev)
+    with pytest.raises(capi.VgpuError, match="integer/boolean"):
+        capi.group_by([f], [i], [(capi.COUNT_STAR, -1, capi.INT64)], 10, capacity=10)
+    with pytest.raises(capi.VgpuError, match="MIN/MAX output type"):
+        capi.group_by([i], [i], [(capi.MIN, 0, capi.INT32)], 10, capacity=10)
+    with pytest.raises(capi.VgpuError, match="AVG must produce DOUBLE"):
+        capi.group_by([i], [i], [(capi.AVG, 0, capi.INT64)], 10, capacity=10)
+
+
+def test_sort_column_matches_stable_argsort(dev):
+    import torch
+    rng = np.random.default_rng(21)
+    for n, hi in [(1, 5), (1000, 3), (65_537, 1 << 20), (2_000_000, 1 << 40)]:
+        key = rng.integers(-hi, hi, n, dtype=np.int64)
+        out, perm = capi.sort_column(torch.from_numpy(key).to(dev))
+        order = np.argsort(key, kind="stable")
+        np.testing.assert_array_equal(out.cpu().numpy(), key[order])
+        np.testing.assert_array_equal(perm.cpu().numpy().astype(np.int64), order)
+
+
+def test_filter_gt_i32(dev):
+    import torch
+    rng = np.random.default_rng(23)
+    for n in [0, 1, 2047, 2048, 1_000_001]:
+        x = rng.integers(-100, 100, n).astype(np.int32)
+        got = capi.filter_gt_i32(torch.from_numpy(x).to(dev), 10)
+        np.testing.assert_array_equal(got.cpu().numpy(), x[x > 10])

@@ -1,0 +1,202 @@
+"""ctypes binding of libvgpu.so (include/vgpu.h).
+
+This is the same stub a maintainer would add on the reference side to call the engine from Python; the
+tests and bench.py drive the HIP path only through it. Loading fails loudly: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libvgpu.so"
+
+INT16, INT32, INT64, FLOAT32, FLOAT64, BOOLEAN, DICTIONARY_STRING, HUGEINT = range(8)
+COUNT_STAR, COUNT, SUM, MIN, MAX, AVG = range(6)
+TYPE_WIDTH = {INT16: 2, INT32: 4, INT64: 8, FLOAT32: 4, FLOAT64: 8, BOOLEAN: 1, HUGEINT: 16}
+
+
+class VgpuError(RuntimeError):
+    pass
+
+
+class Column(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("type", C.c_int32), ("rows", C.c_uint64)]
+
+
+class Agg(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("input", C.c_int32), ("output_type", C.c_int32)]
+
+
+# (name, restype, argtypes); every function returns int status unless noted
+_PROTOS = {
+    "vgpu_last_error": (C.c_char_p, []),
+    "vgpu_abi_version": (C.c_int, []),
+    "vgpu_device_init": (C.c_int, [C.c_int]),
+    "vgpu_group_by_bound": (C.c_int, [C.POINTER(Column), C.c_int32, C.c_uint64, C.POINTER(C.c_uint64), C.c_void_p]),
+    "vgpu_group_by": (C.c_int, [C.POINTER(Column), C.c_int32, C.POINTER(Column), C.c_int32, C.POINTER(Agg), C.c_int32,
+                                C.c_uint64, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_uint64,
+                                C.POINTER(C.c_uint64), C.c_void_p]),
+    "vgpu_sort_column": (C.c_int, [C.POINTER(Column), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vgpu_filter_gt_i32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
+    "vgpu_gather": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "vgpu_generate_column": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_int32, C.c_int64, C.c_int64, C.c_uint64,
+                                       C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "vgpu_profile_enable": (C.c_int, [C.c_int32]),
+    "vgpu_profile_reset": (C.c_int, []),
+    "vgpu_profile_read": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vgpu_pool_stats": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "vgpu_pool_trim": (C.c_int, []),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libvgpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m vectisdb_amd.build` first")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_PROTOS)
+
+
+def check(status: int):
+    if status != 0:
+        raise VgpuError(load().vgpu_last_error().decode())
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(stream.cuda_stream)
+
+
+def _tensor_type(t):
+    import torch
+    return {torch.int16: INT16, torch.int32: INT32, torch.int64: INT64, torch.float32: FLOAT32,
+            torch.float64: FLOAT64, torch.bool: BOOLEAN, torch.uint8: BOOLEAN}[t.dtype]
+
+
+def column(t, vtype=None) -> Column:
+    """Column descriptor of a contiguous 1-D device tensor."""
+    assert t.is_contiguous() and t.dim() == 1
+    return Column(t.data_ptr(), _tensor_type(t) if vtype is None else vtype, t.numel())
+
+
+def torch_dtype(vtype):
+    import torch
+    return {INT16: torch.int16, INT32: torch.int32, INT64: torch.int64, FLOAT32: torch.float32,
+            FLOAT64: torch.float64, BOOLEAN: torch.uint8}[vtype]
+
+
+def empty_column(vtype, rows, device="cuda"):
+    """Device buffer for `rows` values of `vtype` (HUGEINT as int64 pairs)."""
+    import torch
+    if vtype == HUGEINT:
+        return torch.empty(max(rows, 1) * 2, dtype=torch.int64, device=device)
+    return torch.empty(max(rows, 1), dtype=torch_dtype(vtype), device=device)
+
+
+def group_by_bound(keys, rows, stream=None) -> int:
+    lib = load()
+    kc = (Column * max(len(keys), 1))(*keys)
+    out = C.c_uint64(0)
+    check(lib.vgpu_group_by_bound(kc, len(keys), rows, C.byref(out), _stream_ptr(stream)))
+    return out.value
+
+
+def group_by(keys, values, aggs, rows, capacity=None, stream=None):
+    """GROUP BY over device columns.
+
+    keys/values: lists of device tensors; aggs: list of (kind, input_index, output_type).
+    Returns (key_tensors, agg_tensors, groups) with tensors trimmed to `groups` rows.
+    """
+    lib = load()
+    key_cols = [column(k) for k in keys]
+    val_cols = [column(v) for v in values]
+    if capacity is None:
+        capacity = group_by_bound(key_cols, rows, stream)
+    key_out = [empty_column(kc.type, capacity) for kc in key_cols]
+    agg_out = [empty_column(a[2], capacity) for a in aggs]
+    kc = (Column * max(len(key_cols), 1))(*key_cols)
+    vc = (Column * max(len(val_cols), 1))(*val_cols)
+    ac = (Agg * max(len(aggs), 1))(*[Agg(*a) for a in aggs])
+    kp = (C.c_void_p * max(len(key_out), 1))(*[t.data_ptr() for t in key_out])
+    ap = (C.c_void_p * max(len(agg_out), 1))(*[t.data_ptr() for t in agg_out])
+    groups = C.c_uint64(0)
+    check(lib.vgpu_group_by(kc, len(key_cols), vc, len(val_cols), ac, len(aggs), rows, kp, ap, capacity,
+                            C.byref(groups), _stream_ptr(stream)))
+    g = groups.value
+    trimmed_aggs = [t[: 2 * g] if a[2] == HUGEINT else t[:g] for t, a in zip(agg_out, aggs)]
+    return [t[:g] for t in key_out], trimmed_aggs, g
+
+
+def sort_column(key, stream=None):
+    import torch
+    lib = load()
+    kc = column(key)
+    out = torch.empty_like(key)
+    perm = torch.empty(key.numel(), dtype=torch.int32, device=key.device)
+    check(lib.vgpu_sort_column(C.byref(kc), C.c_void_p(out.data_ptr()), C.c_void_p(perm.data_ptr()),
+                               _stream_ptr(stream)))
+    return out, perm
+
+
+def filter_gt_i32(x, threshold, stream=None):
+    import torch
+    lib = load()
+    out = torch.empty(max(x.numel(), 1), dtype=torch.int32, device=x.device)
+    count = C.c_uint64(0)
+    check(lib.vgpu_filter_gt_i32(C.c_void_p(x.data_ptr()), x.numel(), threshold, C.c_void_p(out.data_ptr()),
+                                 C.byref(count), _stream_ptr(stream)))
+    return out[: count.value]
+
+
+def generate_column(out, vtype, dist, lo=0, hi=0, seed=0, stream_id=0, cdf=None, perm=None, stream=None):
+    lib = load()
+    check(lib.vgpu_generate_column(C.c_void_p(out.data_ptr()), vtype, out.numel() if vtype != HUGEINT else out.numel() // 2,
+                                   dist, lo, hi, seed, stream_id,
+                                   C.c_void_p(cdf.data_ptr() if cdf is not None else 0),
+                                   cdf.numel() if cdf is not None else 0,
+                                   C.c_void_p(perm.data_ptr() if perm is not None else 0), _stream_ptr(stream)))
+
+
+def profile_enable(on=True):
+    check(load().vgpu_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    check(load().vgpu_profile_reset())
+
+
+def profile_read():
+    lib = load()
+    needed = C.c_size_t(0)
+    buf = C.create_string_buffer(1 << 16)
+    st = lib.vgpu_profile_read(buf, len(buf), C.byref(needed))
+    if st != 0:
+        buf = C.create_string_buffer(needed.value)
+        check(lib.vgpu_profile_read(buf, len(buf), C.byref(needed)))
+    return json.loads(buf.value.decode())
+
+
+def pool_stats():
+    a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    check(load().vgpu_pool_stats(C.byref(a), C.byref(b), C.byref(c)))
+    return {"in_use": a.value, "cached": b.value, "peak": c.value}

@@ -1,0 +1,289 @@
+// agg_state.h -- partial aggregate state shared by the segmented reduction and the join combiner.
+//
+// A group's state is kMaxComponents slots of two 64-bit words. Components (see kernels.h CompKind):
+//   COUNT     a = rows
+//   SUM_I128  (a, b) = two's-complement int128 (low, high); exact, so any combine order gives the same bits
+//   SUM_DD    (a, b) = double-double (hi, lo); error-free TwoSum accumulation, rounded once at finalize, so
+//             the result is the correctly rounded sum except under extreme cancellation
+//   MIN/MAX   a = widened int64, or double bits with DuckDB's order (NaN greatest)
+#pragma once
+
+#include "common.h"
+#include "kernels.h"
+
+namespace vgpu {
+
+struct CompState {
+	u64 a[kMaxComponents];
+	u64 b[kMaxComponents];
+};
+
+__device__ __forceinline__ double AsDouble(u64 bits) {
+	return __longlong_as_double((long long)bits);
+}
+__device__ __forceinline__ u64 AsBits(double v) {
+	return (u64)__double_as_longlong(v);
+}
+
+//! Error-free transformation: s + e == a + b exactly (round-to-nearest, no contraction).
+__device__ __forceinline__ void TwoSum(double a, double b, double &s, double &e) {
+	s = a + b;
+	double bb = s - a;
+	e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ void DdAdd(double &hi, double &lo, double x) {
+	double s, e;
+	TwoSum(hi, x, s, e);
+	e += lo;
+	hi = s + e;
+	lo = e - (hi - s);
+}
+
+__device__ __forceinline__ void DdAddDd(double &hi, double &lo, double hi2, double lo2) {
+	double s, e;
+	TwoSum(hi, hi2, s, e);
+	e += lo + lo2;
+	hi = s + e;
+	lo = e - (hi - s);
+}
+
+__device__ __forceinline__ double DdValue(double hi, double lo) {
+	if (!isfinite(hi)) {
+		return hi;
+	}
+	return hi + lo;
+}
+
+__device__ __forceinline__ void I128AddI64(u64 &lo, u64 &hi, long long x) {
+	u64 nlo = lo + (u64)x;
+	u64 carry = nlo < lo ? 1ull : 0ull;
+	hi = hi + (x < 0 ? ~0ull : 0ull) + carry;
+	lo = nlo;
+}
+
+__device__ __forceinline__ void I128AddI128(u64 &lo, u64 &hi, u64 lo2, u64 hi2) {
+	u64 nlo = lo + lo2;
+	u64 carry = nlo < lo ? 1ull : 0ull;
+	hi = hi + hi2 + carry;
+	lo = nlo;
+}
+
+//! DuckDB orders NaN above every other double.
+__device__ __forceinline__ bool DuckLess(double x, double y) {
+	if (isnan(y)) {
+		return !isnan(x);
+	}
+	return !isnan(x) && x < y;
+}
+
+struct CompDesc {
+	int ncomp;
+	int kind[kMaxComponents];
+	int payload[kMaxComponents];
+	int in_type[kMaxComponents];
+};
+
+__device__ __forceinline__ void StateIdentity(const CompDesc &d, CompState &s) {
+#pragma unroll
+	for (int c = 0; c < kMaxComponents; c++) {
+		s.a[c] = 0;
+		s.b[c] = 0;
+		if (c < d.ncomp) {
+			switch (d.kind[c]) {
+			case COMP_MIN_I64:
+				s.a[c] = (u64)0x7fffffffffffffffll;
+				break;
+			case COMP_MAX_I64:
+				s.a[c] = (u64)0x8000000000000000ull;
+				break;
+			case COMP_MIN_F64:
+				s.a[c] = AsBits(__builtin_nan(""));
+				break;
+			case COMP_MAX_F64:
+				s.a[c] = AsBits(-__builtin_inf());
+				break;
+			default:
+				break;
+			}
+		}
+	}
+}
+
+//! Value of a payload element as int64 (integers, sign-extended; BOOLEAN 0/1) -- raw bits in `raw`.
+__device__ __forceinline__ long long RawAsI64(u64 raw, int vtype) {
+	switch (vtype) {
+	case VT_INT16:
+		return (long long)(short)(unsigned short)raw;
+	case VT_INT32:
+		return (long long)(int)(unsigned)raw;
+	case VT_BOOLEAN:
+		return (long long)((raw & 0xff) != 0);
+	default:
+		return (long long)raw;
+	}
+}
+
+__device__ __forceinline__ double RawAsF64(u64 raw, int vtype) {
+	if (vtype == VT_FLOAT32) {
+		return (double)__uint_as_float((unsigned)raw);
+	}
+	if (vtype == VT_FLOAT64) {
+		return AsDouble(raw);
+	}
+	return (double)RawAsI64(raw, vtype);
+}
+
+//! Folds one row into the state. `vals[p]` holds the raw bits of payload column p at this row.
+template <int NP>
+__device__ __forceinline__ void StateAddRow(const CompDesc &d, CompState &s, const u64 (&vals)[NP]) {
+#pragma unroll
+	for (int c = 0; c < kMaxComponents; c++) {
+		if (c < d.ncomp) {
+			const int p = d.payload[c];
+			u64 raw = 0;
+#pragma unroll
+			for (int q = 0; q < NP; q++) {
+				if (q == p) {
+					raw = vals[q];
+				}
+			}
+			switch (d.kind[c]) {
+			case COMP_COUNT:
+				s.a[c] += 1;
+				break;
+			case COMP_SUM_I128:
+				I128AddI64(s.a[c], s.b[c], RawAsI64(raw, d.in_type[c]));
+				break;
+			case COMP_SUM_DD: {
+				double hi = AsDouble(s.a[c]), lo = AsDouble(s.b[c]);
+				DdAdd(hi, lo, RawAsF64(raw, d.in_type[c]));
+				s.a[c] = AsBits(hi);
+				s.b[c] = AsBits(lo);
+				break;
+			}
+			case COMP_MIN_I64: {
+				long long v = RawAsI64(raw, d.in_type[c]);
+				if (v < (long long)s.a[c]) {
+					s.a[c] = (u64)v;
+				}
+				break;
+			}
+			case COMP_MAX_I64: {
+				long long v = RawAsI64(raw, d.in_type[c]);
+				if (v > (long long)s.a[c]) {
+					s.a[c] = (u64)v;
+				}
+				break;
+			}
+			case COMP_MIN_F64: {
+				double v = RawAsF64(raw, d.in_type[c]);
+				if (DuckLess(v, AsDouble(s.a[c]))) {
+					s.a[c] = AsBits(v);
+				}
+				break;
+			}
+			case COMP_MAX_F64: {
+				double v = RawAsF64(raw, d.in_type[c]);
+				if (DuckLess(AsDouble(s.a[c]), v)) {
+					s.a[c] = AsBits(v);
+				}
+				break;
+			}
+			default:
+				break;
+			}
+		}
+	}
+}
+
+//! s = s (earlier rows) combined with t (later rows).
+__device__ __forceinline__ void StateCombine(const CompDesc &d, CompState &s, const CompState &t) {
+#pragma unroll
+	for (int c = 0; c < kMaxComponents; c++) {
+		if (c < d.ncomp) {
+			switch (d.kind[c]) {
+			case COMP_COUNT:
+				s.a[c] += t.a[c];
+				break;
+			case COMP_SUM_I128:
+				I128AddI128(s.a[c], s.b[c], t.a[c], t.b[c]);
+				break;
+			case COMP_SUM_DD: {
+				double hi = AsDouble(s.a[c]), lo = AsDouble(s.b[c]);
+				DdAddDd(hi, lo, AsDouble(t.a[c]), AsDouble(t.b[c]));
+				s.a[c] = AsBits(hi);
+				s.b[c] = AsBits(lo);
+				break;
+			}
+			case COMP_MIN_I64:
+				if ((long long)t.a[c] < (long long)s.a[c]) {
+					s.a[c] = t.a[c];
+				}
+				break;
+			case COMP_MAX_I64:
+				if ((long long)t.a[c] > (long long)s.a[c]) {
+					s.a[c] = t.a[c];
+				}
+				break;
+			case COMP_MIN_F64:
+				if (DuckLess(AsDouble(t.a[c]), AsDouble(s.a[c]))) {
+					s.a[c] = t.a[c];
+				}
+				break;
+			case COMP_MAX_F64:
+				if (DuckLess(AsDouble(s.a[c]), AsDouble(t.a[c]))) {
+					s.a[c] = t.a[c];
+				}
+				break;
+			default:
+				break;
+			}
+		}
+	}
+}
+
+__device__ __forceinline__ void StateShflUp(const CompDesc &d, CompState &dst, const CompState &src, int delta) {
+#pragma unroll
+	for (int c = 0; c < kMaxComponents; c++) {
+		if (c < d.ncomp) {
+			dst.a[c] = __shfl_up(src.a[c], delta);
+			dst.b[c] = __shfl_up(src.b[c], delta);
+		}
+	}
+}
+
+__device__ __forceinline__ void StateShflXor(const CompDesc &d, CompState &dst, const CompState &src, int mask) {
+#pragma unroll
+	for (int c = 0; c < kMaxComponents; c++) {
+		if (c < d.ncomp) {
+			dst.a[c] = __shfl_xor(src.a[c], mask);
+			dst.b[c] = __shfl_xor(src.b[c], mask);
+		}
+	}
+}
+
+__device__ __forceinline__ void StateStore(const CompDesc &d, u64 *dst, const CompState &s) {
+#pragma unroll
+	for (int c = 0; c < kMaxComponents; c++) {
+		if (c < d.ncomp) {
+			dst[2 * c] = s.a[c];
+			dst[2 * c + 1] = s.b[c];
+		}
+	}
+}
+
+__device__ __forceinline__ void StateLoad(const CompDesc &d, const u64 *src, CompState &s) {
+#pragma unroll
+	for (int c = 0; c < kMaxComponents; c++) {
+		if (c < d.ncomp) {
+			s.a[c] = src[2 * c];
+			s.b[c] = src[2 * c + 1];
+		} else {
+			s.a[c] = 0;
+			s.b[c] = 0;
+		}
+	}
+}
+
+} // namespace vgpu

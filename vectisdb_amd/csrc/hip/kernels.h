@@ -1,0 +1,225 @@
+// kernels.h -- host-side launch interface of the vectis HIP kernels (internal to the library).
+//
+// The C++ engine (host/) and the extern "C" boundary (capi.cpp) call these; nothing here is visible to
+// the DuckDB extension. All pointers are device pointers unless stated otherwise; every launcher is
+// asynchronous on `stream` unless its comment says it synchronizes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace vgpu {
+
+//! Value types, numbered exactly like vector_gpu::GpuValueType (gpu_engine.hpp).
+enum VType : int {
+	VT_INT16 = 0,
+	VT_INT32 = 1,
+	VT_INT64 = 2,
+	VT_FLOAT32 = 3,
+	VT_FLOAT64 = 4,
+	VT_BOOLEAN = 5,
+	VT_DICTIONARY_STRING = 6,
+	VT_HUGEINT = 7,
+};
+
+int TypeWidth(int vtype); // bytes per element on the device; -1 for DICTIONARY_STRING
+bool TypeIsSigned(int vtype);
+bool TypeIsInteger(int vtype); // INT16/32/64 and BOOLEAN
+
+//! Thrown by launchers; the engine turns it into success=false.
+struct HipError : std::runtime_error {
+	using std::runtime_error::runtime_error;
+};
+
+void Check(hipError_t status, const char *what);
+#define VGPU_HIP(call) ::vgpu::Check((call), #call)
+
+//! Scratch allocator the launchers draw temporaries from. The engine passes its per-plan arena; the C ABI
+//! passes a pooled arena. Freed when the allocator is destroyed (after the caller's stream work is
+//! ordered, see DevicePool).
+struct Scratch {
+	virtual ~Scratch() = default;
+	virtual void *Alloc(size_t bytes) = 0;
+};
+
+// ------------------------------------------------------------------ profiling (HIP events per launch)
+
+//! When enabled, every launcher brackets each kernel with hipEvents on its launch stream; totals are read
+//! with ProfileSnapshot. Used by the bench to time the dominant kernel on its own stream.
+void ProfileEnable(bool on);
+bool ProfileEnabled();
+struct ProfileEntry {
+	std::string name;
+	double total_ms = 0;
+	uint64_t launches = 0;
+	double bytes = 0; // algorithmic bytes the launcher attributes to these launches
+};
+std::vector<ProfileEntry> ProfileSnapshot(); // synchronizes the device
+void ProfileReset();
+
+//! RAII bracket used by launchers.
+class ProfileScope {
+public:
+	ProfileScope(const char *name, hipStream_t stream, double bytes);
+	~ProfileScope();
+
+private:
+	const char *name_;
+	hipStream_t stream_;
+	double bytes_;
+	hipEvent_t start_ = nullptr;
+	hipEvent_t stop_ = nullptr;
+};
+
+// ------------------------------------------------------------------ key packing
+
+constexpr int kMaxKeyCols = 4;
+constexpr int kMaxPayload = 8;
+constexpr int kMaxComponents = 8;
+
+//! Packs up to kMaxKeyCols integer key columns into one order-preserving unsigned key:
+//! packed = sum_c (Ordered(col_c[row]) - min_c) << shift_c. The first column is most significant.
+struct KeyPack {
+	int ncols = 0;
+	const void *col[kMaxKeyCols] = {};
+	int width[kMaxKeyCols] = {};
+	int is_signed[kMaxKeyCols] = {};
+	uint64_t min[kMaxKeyCols] = {};
+	int shift[kMaxKeyCols] = {};
+	int bits[kMaxKeyCols] = {};
+	int total_bits = 0; // sum of bits; <= 64
+};
+
+//! Ordered min/max of each key column (synchronizes: the host needs them to plan the passes).
+void KeyRanges(const void *const *cols, const int *vtypes, int ncols, uint64_t n, uint64_t *min_out,
+               uint64_t *max_out, Scratch &scratch, hipStream_t stream);
+
+//! Builds the packing from the ranges. Returns false when the packed key needs more than 64 bits.
+bool PlanKeyPack(const void *const *cols, const int *vtypes, int ncols, const uint64_t *mins, const uint64_t *maxs,
+                 KeyPack &pack);
+
+// ------------------------------------------------------------------ radix sort
+
+struct SortTuning {
+	int radix_bits = 8;      // 8 or 11 (digit width upper bound)
+	int passes_override = 0; // 0 = derive from total_bits
+};
+
+//! Output of SortPacked: the compact keys (uint32 when total_bits <= 32, else uint64) in ascending order and
+//! each payload column permuted identically (stable). Buffers come from `scratch`.
+struct SortedRun {
+	void *keys = nullptr;
+	bool keys64 = false;
+	void *payload[kMaxPayload] = {};
+	int payload_width[kMaxPayload] = {};
+	int npayload = 0;
+	int passes = 0;
+	int radix_bits = 0;
+};
+
+//! LSD radix sort of the packed key with payload. payload_in[i] == nullptr with width 4 means "generate
+//! row ids" (a uint32 permutation). Requires n < 2^30.
+void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
+                uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream);
+
+// ------------------------------------------------------------------ segmented multi-aggregate reduction
+
+//! Component kinds of a group's partial state. Aggregates map onto components (deduplicated), so
+//! SUM(v)/AVG(v) share one SUM_DD(v) and every COUNT shares the one COUNT.
+enum CompKind : int {
+	COMP_COUNT = 0,   // uint64
+	COMP_SUM_I128 = 1, // int128 over an integer column
+	COMP_SUM_DD = 2,   // double-double over a FLOAT/DOUBLE column
+	COMP_MIN_I64 = 3,  // signed min over an integer column (widened)
+	COMP_MAX_I64 = 4,
+	COMP_MIN_F64 = 5,  // floating min (DuckDB order: NaN is greatest)
+	COMP_MAX_F64 = 6,
+};
+
+struct Component {
+	int kind = COMP_COUNT;
+	int payload = -1; // index into the sorted payload columns
+	int in_type = VT_INT64;
+};
+
+//! Output aggregate kinds, numbered like vector_gpu::GpuAggregateKind.
+enum AggKind : int { AGG_COUNT_STAR = 0, AGG_COUNT = 1, AGG_SUM = 2, AGG_MIN = 3, AGG_MAX = 4, AGG_AVG = 5 };
+
+struct AggOut {
+	int kind = AGG_COUNT_STAR;
+	int comp = 0;        // main component
+	int out_type = VT_INT64;
+	void *out = nullptr; // device output column (capacity rows)
+};
+
+struct ReduceSpec {
+	int ncomp = 0;
+	Component comp[kMaxComponents];
+	int naggs = 0;
+	AggOut aggs[16];
+	// group key reconstruction
+	KeyPack pack;
+	void *key_out[kMaxKeyCols] = {};
+};
+
+//! Reduces sorted keys/payload into one row per distinct key. `keys` may be nullptr: every row is then one
+//! group (global aggregate). Writes the group count to *groups_device (device uint64).
+void SegmentedReduce(const SortedRun &run, uint64_t n, const ReduceSpec &spec, uint64_t *groups_device,
+                     Scratch &scratch, hipStream_t stream);
+
+// ------------------------------------------------------------------ join
+
+//! Per-key state of one side after its own reduce-by-key: unique compact keys + component columns.
+struct KeyedStates {
+	void *keys = nullptr; // compact keys, ascending, unique
+	bool keys64 = false;
+	uint64_t groups = 0;
+	int ncomp = 0;
+	Component comp[kMaxComponents];
+	void *state = nullptr; // groups x ncomp x 16 bytes (component slots)
+};
+
+//! Aggregate-only INNER join: merge-path over two unique-key state runs, accumulating
+//! COUNT(*) = sum ca*cb and the per-side components; results land in host `result` (16 bytes per output).
+struct JoinAggSpec {
+	int naggs = 0;
+	struct {
+		int kind;     // AggKind
+		int side;     // 0 = left (probe/children[0]), 1 = right, -1 = COUNT_STAR
+		int comp;     // component index on that side
+		int out_type; // VType
+	} aggs[16];
+};
+
+// ------------------------------------------------------------------ misc primitives
+
+//! out[i] = src[idx[i]] for fixed-width elements (1/2/4/8/16 bytes); idx is uint32 or uint64.
+void Gather(const void *src, int width, const void *idx, bool idx64, uint64_t n, void *out, hipStream_t stream);
+
+//! Dense byte validity from an Arrow LSB-first bitmap.
+void UnpackValidity(const uint8_t *packed, uint8_t *out, uint64_t n, hipStream_t stream);
+
+//! Zero-fill / value-fill without hipMemset (safe inside captured graphs).
+void FillU32(uint32_t *dst, uint32_t value, uint64_t n, hipStream_t stream);
+void FillU64(uint64_t *dst, uint64_t value, uint64_t n, hipStream_t stream);
+
+//! Left-major cross product expansion of one column: dst[i] = src[from_left ? i / m : i % m].
+void CrossExpand(const void *src, int width, uint64_t m, bool from_left, uint64_t total, void *dst,
+                 hipStream_t stream);
+
+//! Seeded counter-based column generator (dist: 0 uniform int [lo, hi], 1 uniform double [0,1), 2 normal,
+//! 3 Zipf via `cdf`/`perm` tables on the device, 4 sequence lo + i).
+void GenerateColumn(void *out, int vtype, uint64_t n, int dist, long long lo, long long hi, uint64_t seed,
+                    uint64_t stream_id, const double *cdf, uint64_t cdf_len, const long long *perm,
+                    hipStream_t stream);
+
+//! Order-preserving compaction of x > threshold (the FilterGreaterThanInt32 primitive).
+void FilterGtI32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_device,
+                 Scratch &scratch, hipStream_t stream);
+
+} // namespace vgpu

@@ -1,0 +1,350 @@
+// misc.hip -- small device primitives: fills, gathers, validity unpack, cross-product expansion, the
+// order-preserving int32 threshold filter, type helpers and per-launch HIP-event profiling.
+#include "common.h"
+#include "kernels.h"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace vgpu {
+
+int TypeWidth(int vtype) {
+	switch (vtype) {
+	case VT_INT16:
+		return 2;
+	case VT_INT32:
+		return 4;
+	case VT_INT64:
+		return 8;
+	case VT_FLOAT32:
+		return 4;
+	case VT_FLOAT64:
+		return 8;
+	case VT_BOOLEAN:
+		return 1;
+	case VT_HUGEINT:
+		return 16;
+	default:
+		return -1;
+	}
+}
+
+bool TypeIsSigned(int vtype) {
+	return vtype == VT_INT16 || vtype == VT_INT32 || vtype == VT_INT64 || vtype == VT_HUGEINT;
+}
+
+bool TypeIsInteger(int vtype) {
+	return vtype == VT_INT16 || vtype == VT_INT32 || vtype == VT_INT64 || vtype == VT_BOOLEAN;
+}
+
+void Check(hipError_t status, const char *what) {
+	if (status != hipSuccess) {
+		throw HipError(std::string(what) + " failed: " + hipGetErrorString(status));
+	}
+}
+
+// ------------------------------------------------------------------ profiling
+
+namespace {
+struct ProfRecord {
+	std::string name;
+	hipEvent_t start;
+	hipEvent_t stop;
+	double bytes;
+};
+std::mutex g_prof_mutex;
+bool g_prof_on = false;
+std::vector<ProfRecord> g_prof_pending;
+std::map<std::string, ProfileEntry> g_prof_totals;
+} // namespace
+
+void ProfileEnable(bool on) {
+	std::lock_guard<std::mutex> lock(g_prof_mutex);
+	g_prof_on = on;
+}
+
+bool ProfileEnabled() {
+	std::lock_guard<std::mutex> lock(g_prof_mutex);
+	return g_prof_on;
+}
+
+ProfileScope::ProfileScope(const char *name, hipStream_t stream, double bytes)
+    : name_(name), stream_(stream), bytes_(bytes) {
+	if (!ProfileEnabled()) {
+		return;
+	}
+	VGPU_HIP(hipEventCreate(&start_));
+	VGPU_HIP(hipEventCreate(&stop_));
+	VGPU_HIP(hipEventRecord(start_, stream_));
+}
+
+ProfileScope::~ProfileScope() {
+	if (start_ == nullptr) {
+		return;
+	}
+	if (hipEventRecord(stop_, stream_) != hipSuccess) {
+		(void)0;
+		(void)hipEventDestroy(start_);
+		(void)hipEventDestroy(stop_);
+		return;
+	}
+	std::lock_guard<std::mutex> lock(g_prof_mutex);
+	g_prof_pending.push_back({name_, start_, stop_, bytes_});
+}
+
+std::vector<ProfileEntry> ProfileSnapshot() {
+	std::lock_guard<std::mutex> lock(g_prof_mutex);
+	for (auto &rec : g_prof_pending) {
+		(void)hipEventSynchronize(rec.stop);
+		float ms = 0;
+		(void)hipEventElapsedTime(&ms, rec.start, rec.stop);
+		auto &entry = g_prof_totals[rec.name];
+		entry.name = rec.name;
+		entry.total_ms += ms;
+		entry.launches += 1;
+		entry.bytes += rec.bytes;
+		(void)hipEventDestroy(rec.start);
+		(void)hipEventDestroy(rec.stop);
+	}
+	g_prof_pending.clear();
+	std::vector<ProfileEntry> out;
+	for (auto &kv : g_prof_totals) {
+		out.push_back(kv.second);
+	}
+	return out;
+}
+
+void ProfileReset() {
+	ProfileSnapshot();
+	std::lock_guard<std::mutex> lock(g_prof_mutex);
+	g_prof_totals.clear();
+}
+
+// ------------------------------------------------------------------ fills
+
+namespace {
+unsigned GridFor(uint64_t n, unsigned block = 256, unsigned cap = 8192) {
+	uint64_t g = (n + block - 1) / block;
+	if (g == 0) {
+		g = 1;
+	}
+	return (unsigned)std::min<uint64_t>(g, cap);
+}
+} // namespace
+
+void FillU32(uint32_t *dst, uint32_t value, uint64_t n, hipStream_t stream) {
+	if (n == 0) {
+		return;
+	}
+	hipLaunchKernelGGL(FillKernel<uint32_t>, dim3(GridFor(n)), dim3(256), 0, stream, dst, value, n);
+	VGPU_HIP(hipGetLastError());
+}
+
+void FillU64(uint64_t *dst, uint64_t value, uint64_t n, hipStream_t stream) {
+	if (n == 0) {
+		return;
+	}
+	hipLaunchKernelGGL(FillKernel<u64>, dim3(GridFor(n)), dim3(256), 0, stream, (u64 *)dst, (u64)value, n);
+	VGPU_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ gather
+
+namespace {
+struct alignas(16) U128 {
+	u64 lo;
+	u64 hi;
+};
+
+template <typename T, typename I>
+__global__ void GatherKernel(const T *src, const I *idx, uint64_t n, T *out) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		out[i] = src[idx[i]];
+	}
+}
+
+template <typename T>
+void GatherT(const void *src, const void *idx, bool idx64, uint64_t n, void *out, hipStream_t stream) {
+	if (idx64) {
+		hipLaunchKernelGGL((GatherKernel<T, u64>), dim3(GridFor(n)), dim3(256), 0, stream, (const T *)src,
+		                   (const u64 *)idx, n, (T *)out);
+	} else {
+		hipLaunchKernelGGL((GatherKernel<T, uint32_t>), dim3(GridFor(n)), dim3(256), 0, stream, (const T *)src,
+		                   (const uint32_t *)idx, n, (T *)out);
+	}
+	VGPU_HIP(hipGetLastError());
+}
+} // namespace
+
+void Gather(const void *src, int width, const void *idx, bool idx64, uint64_t n, void *out, hipStream_t stream) {
+	if (n == 0) {
+		return;
+	}
+	ProfileScope prof("gather", stream, (double)n * (2.0 * width + (idx64 ? 8 : 4)));
+	switch (width) {
+	case 1:
+		GatherT<uint8_t>(src, idx, idx64, n, out, stream);
+		break;
+	case 2:
+		GatherT<uint16_t>(src, idx, idx64, n, out, stream);
+		break;
+	case 4:
+		GatherT<uint32_t>(src, idx, idx64, n, out, stream);
+		break;
+	case 8:
+		GatherT<u64>(src, idx, idx64, n, out, stream);
+		break;
+	case 16:
+		GatherT<U128>(src, idx, idx64, n, out, stream);
+		break;
+	default:
+		throw HipError("Gather: unsupported element width");
+	}
+}
+
+// ------------------------------------------------------------------ validity
+
+namespace {
+__global__ void UnpackValidityKernel(const uint8_t *packed, uint8_t *out, uint64_t n) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		out[i] = (packed[i >> 3] >> (i & 7)) & 1u;
+	}
+}
+} // namespace
+
+void UnpackValidity(const uint8_t *packed, uint8_t *out, uint64_t n, hipStream_t stream) {
+	if (n == 0) {
+		return;
+	}
+	hipLaunchKernelGGL(UnpackValidityKernel, dim3(GridFor(n)), dim3(256), 0, stream, packed, out, n);
+	VGPU_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ cross product expansion
+
+namespace {
+//! Left-major pairing: output row i is (left i / m, right i % m). A left column broadcasts one value to m
+//! consecutive outputs; a right column repeats the whole (cache-resident) right side.
+template <typename T>
+__global__ void CrossExpandKernel(const T *src, uint64_t m, bool from_left, uint64_t total, T *dst) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+		dst[i] = src[from_left ? i / m : i % m];
+	}
+}
+template <typename T>
+void CrossT(const void *src, uint64_t m, bool from_left, uint64_t total, void *dst, hipStream_t stream) {
+	hipLaunchKernelGGL(CrossExpandKernel<T>, dim3(GridFor(total, 256, 16384)), dim3(256), 0, stream,
+	                   (const T *)src, m, from_left, total, (T *)dst);
+	VGPU_HIP(hipGetLastError());
+}
+} // namespace
+
+void CrossExpand(const void *src, int width, uint64_t m, bool from_left, uint64_t total, void *dst,
+                 hipStream_t stream) {
+	if (total == 0) {
+		return;
+	}
+	ProfileScope prof("cross_expand", stream, (double)total * width);
+	switch (width) {
+	case 1:
+		CrossT<uint8_t>(src, m, from_left, total, dst, stream);
+		break;
+	case 2:
+		CrossT<uint16_t>(src, m, from_left, total, dst, stream);
+		break;
+	case 4:
+		CrossT<uint32_t>(src, m, from_left, total, dst, stream);
+		break;
+	case 8:
+		CrossT<u64>(src, m, from_left, total, dst, stream);
+		break;
+	case 16:
+		CrossT<U128>(src, m, from_left, total, dst, stream);
+		break;
+	default:
+		throw HipError("CrossExpand: unsupported element width");
+	}
+}
+
+// ------------------------------------------------------------------ FilterGreaterThanInt32
+
+namespace {
+constexpr int kFgBlock = 256;
+constexpr int kFgItems = 8;
+
+//! Order-preserving stream compaction of in[i] > threshold: per-tile wave ballots + block scan, global
+//! offset by decoupled look-back.
+__global__ void __launch_bounds__(kFgBlock) FilterGtKernel(const int32_t *in, uint64_t n, int32_t threshold,
+                                                          int32_t *out, u64 *state, uint32_t *ticket,
+                                                          u64 *count_out, uint64_t ntiles) {
+	__shared__ uint32_t s_tile;
+	__shared__ u64 s_scan[kFgBlock / kWave + 1];
+	__shared__ u64 s_prefix;
+	const int tid = threadIdx.x;
+	if (tid == 0) {
+		s_tile = FetchAddAgent(ticket, 1u);
+	}
+	__syncthreads();
+	const uint32_t tile = s_tile;
+	const uint64_t base = (uint64_t)tile * kFgBlock * kFgItems;
+	int32_t v[kFgItems];
+	bool keep[kFgItems];
+	u64 count = 0;
+#pragma unroll
+	for (int k = 0; k < kFgItems; k++) {
+		uint64_t row = base + (uint64_t)tid * kFgItems + k;
+		v[k] = row < n ? in[row] : 0;
+		keep[k] = row < n && v[k] > threshold;
+		count += keep[k];
+	}
+	u64 total;
+	u64 excl = BlockExclusiveScan<u64, kFgBlock>(count, s_scan, &total);
+	if (tid == 0) {
+		StoreAgent(state + tile, LbPack(tile == 0 ? kLbInclusive : kLbAggregate, total));
+	}
+	if (tid < kWave) {
+		u64 prefix = tile == 0 ? 0ull : LookbackExclusiveWave(state, tile);
+		if (tid == 0) {
+			if (tile > 0) {
+				StoreAgent(state + tile, LbPack(kLbInclusive, prefix + total));
+			}
+			s_prefix = prefix;
+			if (tile == ntiles - 1) {
+				*count_out = prefix + total;
+			}
+		}
+	}
+	__syncthreads();
+	u64 pos = s_prefix + excl;
+#pragma unroll
+	for (int k = 0; k < kFgItems; k++) {
+		if (keep[k]) {
+			out[pos++] = v[k];
+		}
+	}
+}
+} // namespace
+
+void FilterGtI32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_device,
+                 Scratch &scratch, hipStream_t stream) {
+	if (n == 0) {
+		FillU64(count_device, 0ull, 1, stream);
+		return;
+	}
+	const uint64_t tile = (uint64_t)kFgBlock * kFgItems;
+	const uint64_t ntiles = (n + tile - 1) / tile;
+	auto *state = static_cast<u64 *>(scratch.Alloc((ntiles + 1) * sizeof(u64)));
+	FillU64(state, 0ull, ntiles + 1, stream);
+	ProfileScope prof("filter_gt_i32", stream, (double)n * 4.0);
+	hipLaunchKernelGGL(FilterGtKernel, dim3((unsigned)ntiles), dim3(kFgBlock), 0, stream, in, n, threshold, out,
+	                   state, reinterpret_cast<uint32_t *>(state + ntiles), (u64 *)count_device, ntiles);
+	VGPU_HIP(hipGetLastError());
+}
+
+} // namespace vgpu

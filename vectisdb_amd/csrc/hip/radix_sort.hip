@@ -1,0 +1,576 @@
+// radix_sort.hip -- LSD radix sort for gfx950: key-range pre-pass, one-read multi-digit histogram, and a
+// onesweep digit pass (LDS-staged, wave64 multisplit ranking, decoupled look-back on per-digit counts).
+//
+// Replaces the reference's `thrust::sequence` + `thrust::stable_sort_by_key` per key column
+// (cuda_engine/src/operators/gpu_groupby.cu:352-376) and the right-side `thrust::sort_by_key` of the join
+// (cuda_engine/src/operators/gpu_hash_join.cu:205-206). Differences by design:
+//   * the key is the order-preserving packing of ALL key columns, minus their minimum, so only the bits
+//     that vary are sorted (a 20-bit key range takes 2-3 digit passes, not 8);
+//   * payload columns (the aggregate inputs) travel with the key through LDS, so no per-aggregate gather
+//     through a permutation is needed afterwards;
+//   * one kernel per digit pass: each tile ranks its keys in LDS, publishes its per-digit counts, and
+//     learns its global offsets by looking back over predecessor tiles (agent-scope atomics, visible
+//     across all 8 XCDs), so keys are read once and written once per pass.
+#include "common.h"
+#include "kernels.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace vgpu {
+namespace {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortWaves = kSortBlock / kWave;
+constexpr uint32_t kStFlagAgg = 1u << 30;
+constexpr uint32_t kStFlagInc = 2u << 30;
+constexpr uint32_t kStValue = (1u << 30) - 1;
+
+struct PayloadArgs {
+	int n = 0;
+	const void *in[kMaxPayload] = {};
+	void *out[kMaxPayload] = {};
+	int width[kMaxPayload] = {};
+};
+
+struct PassList {
+	int npasses = 0;
+	int shift[8] = {};
+	int bits[8] = {};
+};
+
+__device__ __forceinline__ u64 PackRow(const KeyPack &p, uint64_t row) {
+	u64 packed = 0;
+#pragma unroll
+	for (int c = 0; c < kMaxKeyCols; c++) {
+		if (c < p.ncols && p.bits[c] > 0) {
+			u64 raw = LoadRawBits(p.col[c], row, p.width[c]);
+			u64 ord = OrderedKey(raw, p.width[c], p.is_signed[c] != 0);
+			packed |= (ord - p.min[c]) << p.shift[c];
+		}
+	}
+	return packed;
+}
+
+struct RangeCols {
+	int ncols;
+	const void *col[kMaxKeyCols];
+	int width[kMaxKeyCols];
+	int is_signed[kMaxKeyCols];
+};
+
+__global__ void __launch_bounds__(256) KeyRangeKernel(RangeCols cols, uint64_t n, u64 *mins, u64 *maxs) {
+	u64 lo[kMaxKeyCols], hi[kMaxKeyCols];
+#pragma unroll
+	for (int c = 0; c < kMaxKeyCols; c++) {
+		lo[c] = ~0ull;
+		hi[c] = 0;
+	}
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n; row += stride) {
+#pragma unroll
+		for (int c = 0; c < kMaxKeyCols; c++) {
+			if (c < cols.ncols) {
+				u64 ord = OrderedKey(LoadRawBits(cols.col[c], row, cols.width[c]), cols.width[c],
+				                     cols.is_signed[c] != 0);
+				lo[c] = ord < lo[c] ? ord : lo[c];
+				hi[c] = ord > hi[c] ? ord : hi[c];
+			}
+		}
+	}
+#pragma unroll
+	for (int c = 0; c < kMaxKeyCols; c++) {
+		if (c < cols.ncols) {
+			for (int off = 32; off >= 1; off >>= 1) {
+				u64 a = __shfl_xor(lo[c], off);
+				u64 b = __shfl_xor(hi[c], off);
+				lo[c] = a < lo[c] ? a : lo[c];
+				hi[c] = b > hi[c] ? b : hi[c];
+			}
+			if (LaneId() == 0) {
+				MinAgent(mins + c, lo[c]);
+				MaxAgent(maxs + c, hi[c]);
+			}
+		}
+	}
+}
+
+//! One read of the keys builds the digit histogram of every pass.
+template <int R>
+__global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, PassList passes, uint32_t *hist) {
+	constexpr int RADIX = 1 << R;
+	constexpr int PMAX = (64 + R - 1) / R;
+	__shared__ uint32_t s_hist[PMAX * RADIX];
+	for (int i = threadIdx.x; i < PMAX * RADIX; i += blockDim.x) {
+		s_hist[i] = 0;
+	}
+	__syncthreads();
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n; row += stride) {
+		u64 key = PackRow(pack, row);
+		for (int p = 0; p < passes.npasses; p++) {
+			uint32_t d = (uint32_t)(key >> passes.shift[p]) & ((1u << passes.bits[p]) - 1);
+			atomicAdd(&s_hist[p * RADIX + d], 1u);
+		}
+	}
+	__syncthreads();
+	for (int i = threadIdx.x; i < passes.npasses * RADIX; i += blockDim.x) {
+		uint32_t v = s_hist[i];
+		if (v) {
+			FetchAddAgent(hist + i, v);
+		}
+	}
+}
+
+template <int R, int ITEMS, typename KeyT, bool FIRST>
+__global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const KeyT *keys_in, KeyT *keys_out,
+                                                             PayloadArgs pay, uint64_t n, int digit_shift,
+                                                             int digit_bits, const uint32_t *ghist,
+                                                             uint32_t *tile_state, uint32_t *tile_ticket) {
+	constexpr int RADIX = 1 << R;
+	constexpr int TILE = kSortBlock * ITEMS;
+	constexpr int DPT = (RADIX + kSortBlock - 1) / kSortBlock;
+	__shared__ KeyT s_keys[TILE];
+	__shared__ union {
+		u64 pay[TILE];
+		uint32_t wave_count[kSortWaves * RADIX];
+	} s_u;
+	__shared__ uint32_t s_digit_base[RADIX];
+	__shared__ uint32_t s_scan[kSortWaves + 1];
+	__shared__ uint32_t s_tile;
+
+	const int tid = threadIdx.x;
+	const int lane = tid & (kWave - 1);
+	const int wave = tid / kWave;
+	if (tid == 0) {
+		s_tile = FetchAddAgent(tile_ticket, 1u);
+	}
+	for (int i = tid; i < kSortWaves * RADIX; i += kSortBlock) {
+		s_u.wave_count[i] = 0;
+	}
+	__syncthreads();
+	const uint32_t tile = s_tile;
+	const uint64_t tile_base = (uint64_t)tile * TILE;
+	const uint64_t wave_base = tile_base + (uint64_t)wave * ITEMS * kWave;
+	const uint32_t digit_mask = (1u << digit_bits) - 1;
+
+	KeyT key[ITEMS];
+	uint32_t rank[ITEMS];
+#pragma unroll
+	for (int k = 0; k < ITEMS; k++) {
+		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
+		if (row < n) {
+			if constexpr (FIRST) {
+				key[k] = (KeyT)PackRow(pack, row);
+			} else {
+				key[k] = keys_in[row];
+			}
+		} else {
+			key[k] = 0;
+		}
+	}
+
+	// Wave-level multisplit: peers = lanes of this round holding the same digit. Rank = earlier rounds'
+	// count of that digit in this wave + earlier peers in this round, which keeps the sort stable.
+	uint32_t *my_counts = s_u.wave_count + wave * RADIX;
+#pragma unroll
+	for (int k = 0; k < ITEMS; k++) {
+		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
+		bool valid = row < n;
+		uint32_t d = (uint32_t)(key[k] >> digit_shift) & digit_mask;
+		u64 peers = __ballot(valid);
+		for (int b = 0; b < digit_bits; b++) {
+			bool bit = (d >> b) & 1u;
+			u64 bal = __ballot(bit);
+			peers &= bit ? bal : ~bal;
+		}
+		uint32_t r = (uint32_t)Popc64(peers & LaneMaskLt());
+		uint32_t base = valid ? my_counts[d] : 0u;
+		__builtin_amdgcn_wave_barrier();
+		if (valid && r == 0) {
+			my_counts[d] = base + (uint32_t)Popc64(peers);
+		}
+		__builtin_amdgcn_wave_barrier();
+		rank[k] = base + r;
+	}
+	__syncthreads();
+
+	// Per-digit block totals; wave_count becomes the exclusive offset of each wave within the digit.
+	uint32_t dtotal[DPT];
+#pragma unroll
+	for (int j = 0; j < DPT; j++) {
+		int d = tid * DPT + j;
+		dtotal[j] = 0;
+		if (d < RADIX) {
+			uint32_t run = 0;
+#pragma unroll
+			for (int w = 0; w < kSortWaves; w++) {
+				uint32_t c = s_u.wave_count[w * RADIX + d];
+				s_u.wave_count[w * RADIX + d] = run;
+				run += c;
+			}
+			dtotal[j] = run;
+			StoreAgent(tile_state + (uint64_t)tile * RADIX + d, (tile == 0 ? kStFlagInc : kStFlagAgg) | run);
+		}
+	}
+	uint32_t tsum = 0;
+	uint32_t gh[DPT];
+	uint32_t gsum = 0;
+#pragma unroll
+	for (int j = 0; j < DPT; j++) {
+		int d = tid * DPT + j;
+		tsum += dtotal[j];
+		gh[j] = d < RADIX ? ghist[d] : 0u;
+		gsum += gh[j];
+	}
+	uint32_t block_total;
+	uint32_t texcl = BlockExclusiveScan<uint32_t, kSortBlock>(tsum, s_scan, &block_total);
+	uint32_t gtotal;
+	uint32_t gexcl = BlockExclusiveScan<uint32_t, kSortBlock>(gsum, s_scan, &gtotal);
+
+	uint32_t bstart = texcl;
+	uint32_t goff = gexcl;
+#pragma unroll
+	for (int j = 0; j < DPT; j++) {
+		int d = tid * DPT + j;
+		if (d < RADIX) {
+			uint32_t excl = 0;
+			if (tile > 0) {
+				int64_t t = (int64_t)tile - 1;
+				while (t >= 0) {
+					uint32_t word = LoadAgent(tile_state + (uint64_t)t * RADIX + d);
+					uint32_t flag = word & (3u << 30);
+					if (flag == 0) {
+						__builtin_amdgcn_s_sleep(1);
+						continue;
+					}
+					excl += word & kStValue;
+					if (flag == kStFlagInc) {
+						break;
+					}
+					t--;
+				}
+				StoreAgent(tile_state + (uint64_t)tile * RADIX + d, kStFlagInc | (excl + dtotal[j]));
+			}
+			s_digit_base[d] = goff + excl - bstart;
+#pragma unroll
+			for (int w = 0; w < kSortWaves; w++) {
+				s_u.wave_count[w * RADIX + d] += bstart;
+			}
+		}
+		bstart += dtotal[j];
+		goff += gh[j];
+	}
+	__syncthreads();
+
+	uint32_t lpos[ITEMS];
+#pragma unroll
+	for (int k = 0; k < ITEMS; k++) {
+		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
+		if (row < n) {
+			uint32_t d = (uint32_t)(key[k] >> digit_shift) & digit_mask;
+			lpos[k] = my_counts[d] + rank[k];
+		} else {
+			lpos[k] = 0;
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int k = 0; k < ITEMS; k++) {
+		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
+		if (row < n) {
+			s_keys[lpos[k]] = key[k];
+		}
+	}
+	__syncthreads();
+
+	const uint32_t tile_count = (uint32_t)((n - tile_base) < (uint64_t)TILE ? (n - tile_base) : (uint64_t)TILE);
+	uint32_t dst[ITEMS];
+#pragma unroll
+	for (int j = 0; j < ITEMS; j++) {
+		uint32_t i = tid + j * kSortBlock;
+		dst[j] = 0;
+		if (i < tile_count) {
+			KeyT k2 = s_keys[i];
+			uint32_t d = (uint32_t)(k2 >> digit_shift) & digit_mask;
+			dst[j] = s_digit_base[d] + i;
+			keys_out[dst[j]] = k2;
+		}
+	}
+
+	for (int p = 0; p < pay.n; p++) {
+		const int width = pay.width[p];
+		const void *src = pay.in[p];
+#define VGPU_STAGE_PAYLOAD(T)                                                                                  \
+	{                                                                                                          \
+		T *lds = reinterpret_cast<T *>(s_u.pay);                                                               \
+		_Pragma("unroll") for (int k = 0; k < ITEMS; k++) {                                                    \
+			uint64_t row = wave_base + (uint64_t)k * kWave + lane;                                             \
+			if (row < n) {                                                                                     \
+				T v;                                                                                           \
+				if (FIRST && src == nullptr) {                                                                 \
+					v = (T)row;                                                                                \
+				} else {                                                                                       \
+					v = reinterpret_cast<const T *>(src)[row];                                                 \
+				}                                                                                              \
+				lds[lpos[k]] = v;                                                                              \
+			}                                                                                                  \
+		}                                                                                                      \
+		__syncthreads();                                                                                       \
+		T *out = reinterpret_cast<T *>(pay.out[p]);                                                            \
+		_Pragma("unroll") for (int j = 0; j < ITEMS; j++) {                                                    \
+			uint32_t i = tid + j * kSortBlock;                                                                 \
+			if (i < tile_count) {                                                                              \
+				out[dst[j]] = lds[i];                                                                          \
+			}                                                                                                  \
+		}                                                                                                      \
+		__syncthreads();                                                                                       \
+	}
+		if (width == 8) {
+			VGPU_STAGE_PAYLOAD(u64)
+		} else if (width == 4) {
+			VGPU_STAGE_PAYLOAD(uint32_t)
+		} else if (width == 2) {
+			VGPU_STAGE_PAYLOAD(uint16_t)
+		} else {
+			VGPU_STAGE_PAYLOAD(uint8_t)
+		}
+#undef VGPU_STAGE_PAYLOAD
+	}
+}
+
+template <int R, typename KeyT>
+void LaunchPass(bool first, const KeyPack &pack, const void *keys_in, void *keys_out, const PayloadArgs &pay,
+                uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state, uint32_t *ticket,
+                hipStream_t stream) {
+	constexpr int ITEMS = (R <= 8) ? 16 : 16;
+	constexpr int TILE = kSortBlock * ITEMS;
+	uint64_t tiles = (n + TILE - 1) / TILE;
+	double bytes = 0;
+	{
+		double key_in = first ? 0.0 : (double)sizeof(KeyT);
+		if (first) {
+			for (int c = 0; c < pack.ncols; c++) {
+				key_in += pack.width[c];
+			}
+		}
+		double pay_bytes = 0;
+		for (int p = 0; p < pay.n; p++) {
+			pay_bytes += (first && pay.in[p] == nullptr ? 0.0 : pay.width[p]) + pay.width[p];
+		}
+		bytes = (double)n * (key_in + sizeof(KeyT) + pay_bytes);
+	}
+	ProfileScope prof(first ? "radix_pass_first" : "radix_pass", stream, bytes);
+	if (first) {
+		hipLaunchKernelGGL((OnesweepKernel<R, ITEMS, KeyT, true>), dim3((unsigned)tiles), dim3(kSortBlock), 0,
+		                   stream, pack, (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state,
+		                   ticket);
+	} else {
+		hipLaunchKernelGGL((OnesweepKernel<R, ITEMS, KeyT, false>), dim3((unsigned)tiles), dim3(kSortBlock), 0,
+		                   stream, pack, (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state,
+		                   ticket);
+	}
+	VGPU_HIP(hipGetLastError());
+}
+
+template <int R>
+uint64_t TilesFor(uint64_t n) {
+	constexpr int TILE = kSortBlock * 16;
+	return (n + TILE - 1) / TILE;
+}
+
+} // namespace
+
+void KeyRanges(const void *const *cols, const int *vtypes, int ncols, uint64_t n, uint64_t *min_out,
+               uint64_t *max_out, Scratch &scratch, hipStream_t stream) {
+	if (ncols <= 0 || ncols > kMaxKeyCols) {
+		throw HipError("KeyRanges: between 1 and 4 key columns are supported");
+	}
+	RangeCols rc {};
+	rc.ncols = ncols;
+	for (int c = 0; c < ncols; c++) {
+		rc.col[c] = cols[c];
+		rc.width[c] = TypeWidth(vtypes[c]);
+		rc.is_signed[c] = TypeIsSigned(vtypes[c]) ? 1 : 0;
+	}
+	auto *dev = static_cast<u64 *>(scratch.Alloc(2 * kMaxKeyCols * sizeof(u64)));
+	FillU64(dev, ~0ull, kMaxKeyCols, stream);
+	FillU64(dev + kMaxKeyCols, 0ull, kMaxKeyCols, stream);
+	if (n > 0) {
+		uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
+		double bytes = 0;
+		for (int c = 0; c < ncols; c++) {
+			bytes += (double)n * rc.width[c];
+		}
+		ProfileScope prof("key_range", stream, bytes);
+		hipLaunchKernelGGL(KeyRangeKernel, dim3((unsigned)blocks), dim3(256), 0, stream, rc, n, dev,
+		                   dev + kMaxKeyCols);
+		VGPU_HIP(hipGetLastError());
+	}
+	u64 host[2 * kMaxKeyCols];
+	VGPU_HIP(hipMemcpyAsync(host, dev, sizeof(host), hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	for (int c = 0; c < ncols; c++) {
+		min_out[c] = host[c];
+		max_out[c] = host[kMaxKeyCols + c];
+	}
+}
+
+bool PlanKeyPack(const void *const *cols, const int *vtypes, int ncols, const uint64_t *mins, const uint64_t *maxs,
+                 KeyPack &pack) {
+	pack = KeyPack {};
+	pack.ncols = ncols;
+	int total = 0;
+	for (int c = 0; c < ncols; c++) {
+		pack.col[c] = cols[c];
+		pack.width[c] = TypeWidth(vtypes[c]);
+		pack.is_signed[c] = TypeIsSigned(vtypes[c]) ? 1 : 0;
+		pack.min[c] = mins[c];
+		uint64_t range = maxs[c] >= mins[c] ? maxs[c] - mins[c] : 0;
+		int bits = 0;
+		while (bits < 64 && (range >> bits) != 0) {
+			bits++;
+		}
+		pack.bits[c] = bits;
+		total += bits;
+	}
+	if (total > 64) {
+		return false;
+	}
+	pack.total_bits = total;
+	int shift = 0;
+	for (int c = ncols - 1; c >= 0; c--) {
+		pack.shift[c] = pack.bits[c] > 0 ? shift : 0;
+		shift += pack.bits[c];
+	}
+	return true;
+}
+
+void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
+                uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream) {
+	if (n >= (1ull << 30)) {
+		throw HipError("radix sort: at most 2^30 - 1 rows per sort call");
+	}
+	if (npayload > kMaxPayload) {
+		throw HipError("radix sort: too many payload columns");
+	}
+	out = SortedRun {};
+	out.npayload = npayload;
+	const bool keys64 = pack.total_bits > 32;
+	out.keys64 = keys64;
+	const size_t kw = keys64 ? 8 : 4;
+	if (n == 0) {
+		out.keys = scratch.Alloc(kw);
+		for (int p = 0; p < npayload; p++) {
+			out.payload[p] = scratch.Alloc(8);
+			out.payload_width[p] = payload_width[p];
+		}
+		return;
+	}
+
+	// Digit plan: the fewest passes of at most `radix_bits` bits covering the varying bits, split evenly.
+	int B = std::max(pack.total_bits, 1);
+	int rmax = tuning.radix_bits >= 11 ? 11 : 8;
+	int P = tuning.passes_override > 0 ? tuning.passes_override : (B + rmax - 1) / rmax;
+	if (P < 1) {
+		P = 1;
+	}
+	if (P > 8) {
+		P = 8;
+	}
+	PassList passes;
+	passes.npasses = P;
+	int shift = 0;
+	int widest = 0;
+	for (int p = 0; p < P; p++) {
+		int bits = B / P + (p < B % P ? 1 : 0);
+		passes.shift[p] = shift;
+		passes.bits[p] = bits;
+		shift += bits;
+		widest = std::max(widest, bits);
+	}
+	const int R = widest <= 8 ? 8 : 11;
+	if (widest > 11) {
+		throw HipError("radix sort: digit wider than 11 bits requested");
+	}
+	out.passes = P;
+	out.radix_bits = widest;
+	const int RADIX = 1 << R;
+	const uint64_t tiles = R == 8 ? TilesFor<8>(n) : TilesFor<11>(n);
+
+	// Scratch: ping-pong keys and payloads; histogram; per-pass tile state + ticket.
+	void *kbuf[2] = {scratch.Alloc(n * kw), P > 1 ? scratch.Alloc(n * kw) : nullptr};
+	void *pbuf[kMaxPayload][2] = {};
+	for (int p = 0; p < npayload; p++) {
+		pbuf[p][0] = scratch.Alloc(n * payload_width[p]);
+		pbuf[p][1] = P > 1 ? scratch.Alloc(n * payload_width[p]) : nullptr;
+	}
+	auto *hist = static_cast<uint32_t *>(scratch.Alloc((size_t)P * RADIX * sizeof(uint32_t)));
+	const uint64_t state_words = (uint64_t)P * tiles * RADIX + P;
+	auto *state = static_cast<uint32_t *>(scratch.Alloc(state_words * sizeof(uint32_t)));
+	FillU32(hist, 0u, (uint64_t)P * RADIX, stream);
+	FillU32(state, 0u, state_words, stream);
+	uint32_t *tickets = state + (uint64_t)P * tiles * RADIX;
+
+	{
+		double bytes = 0;
+		for (int c = 0; c < pack.ncols; c++) {
+			bytes += (double)n * pack.width[c];
+		}
+		ProfileScope prof("radix_hist", stream, bytes);
+		uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 2048);
+		if (R == 8) {
+			hipLaunchKernelGGL(HistKernel<8>, dim3((unsigned)blocks), dim3(256), 0, stream, pack, n, passes, hist);
+		} else {
+			hipLaunchKernelGGL(HistKernel<11>, dim3((unsigned)blocks), dim3(256), 0, stream, pack, n, passes, hist);
+		}
+		VGPU_HIP(hipGetLastError());
+	}
+
+	const void *cur_keys = nullptr;
+	const void *cur_pay[kMaxPayload];
+	for (int p = 0; p < npayload; p++) {
+		cur_pay[p] = payload_in[p];
+	}
+	for (int pass = 0; pass < P; pass++) {
+		int slot = pass & 1;
+		PayloadArgs pa;
+		pa.n = npayload;
+		for (int p = 0; p < npayload; p++) {
+			pa.in[p] = cur_pay[p];
+			pa.out[p] = pbuf[p][slot];
+			pa.width[p] = payload_width[p];
+		}
+		uint32_t *st = state + (uint64_t)pass * tiles * RADIX;
+		const uint32_t *gh = hist + (uint64_t)pass * RADIX;
+		bool first = pass == 0;
+		if (R == 8) {
+			if (keys64) {
+				LaunchPass<8, u64>(first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass], passes.bits[pass],
+				                   gh, st, tickets + pass, stream);
+			} else {
+				LaunchPass<8, uint32_t>(first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass],
+				                        passes.bits[pass], gh, st, tickets + pass, stream);
+			}
+		} else {
+			if (keys64) {
+				LaunchPass<11, u64>(first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass],
+				                    passes.bits[pass], gh, st, tickets + pass, stream);
+			} else {
+				LaunchPass<11, uint32_t>(first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass],
+				                         passes.bits[pass], gh, st, tickets + pass, stream);
+			}
+		}
+		cur_keys = kbuf[slot];
+		for (int p = 0; p < npayload; p++) {
+			cur_pay[p] = pbuf[p][slot];
+		}
+	}
+	out.keys = const_cast<void *>(cur_keys);
+	for (int p = 0; p < npayload; p++) {
+		out.payload[p] = const_cast<void *>(cur_pay[p]);
+		out.payload_width[p] = payload_width[p];
+	}
+}
+
+} // namespace vgpu

@@ -1,0 +1,502 @@
+// segreduce.hip -- one-pass segmented multi-aggregate reduction over a key-sorted run (gfx950).
+//
+// Replaces the reference's head flags + inclusive_scan group ids + group-count readback + one
+// `thrust::reduce_by_key` (preceded by a `GatherColumn`) PER AGGREGATE
+// (cuda_engine/src/operators/gpu_groupby.cu:380-576). Here every aggregate of the query is folded in the
+// same pass over the sorted keys and payload:
+//   * each thread reduces 8 consecutive rows serially; groups that start and end inside the thread are
+//     finished right there;
+//   * open segments are stitched across the 256 threads with a wave64 segmented scan (shuffles) plus one
+//     LDS step across the 4 waves;
+//   * a group's output index is its head's global rank, which each tile learns by decoupled look-back on
+//     per-tile head counts (one 64-bit atomic word per tile);
+//   * a group that continues past its tile leaves a tail partial; a small fix-up kernel folds the leading
+//     partials of the following tiles into it (one wave per tail, 64 tiles per step).
+// SUM over integers accumulates in exact int128 (DuckDB's HUGEINT), SUM/AVG over doubles in double-double
+// so the result is rounded once.
+#include "agg_state.h"
+#include "common.h"
+#include "kernels.h"
+
+#include <algorithm>
+
+namespace vgpu {
+namespace {
+
+constexpr int kRedBlock = 256;
+constexpr int kRedItems = 8;
+constexpr int kRedTile = kRedBlock * kRedItems;
+constexpr int kRedWaves = kRedBlock / kWave;
+
+struct PayloadPtrs {
+	int n = 0;
+	const void *ptr[kMaxPayload] = {};
+	int width[kMaxPayload] = {};
+};
+
+struct AggOutDesc {
+	int naggs = 0;
+	int count_comp = 0;
+	int kind[16] = {};
+	int comp[16] = {};
+	int comp_kind[16] = {};
+	int out_type[16] = {};
+	void *out[16] = {};
+};
+
+struct KeyOutDesc {
+	KeyPack pack;
+	void *out[kMaxKeyCols] = {};
+};
+
+__device__ __forceinline__ double I128ToDouble(u64 lo, u64 hi) {
+	bool neg = (long long)hi < 0;
+	if (neg) {
+		lo = ~lo + 1;
+		hi = ~hi + (lo == 0 ? 1 : 0);
+	}
+	double v = (double)hi * 18446744073709551616.0 + (double)lo;
+	return neg ? -v : v;
+}
+
+__device__ __forceinline__ void StoreTyped(void *out, u64 gid, int vtype, long long iv, double dv, bool is_float) {
+	switch (vtype) {
+	case VT_INT16:
+		((short *)out)[gid] = (short)iv;
+		break;
+	case VT_INT32:
+		((int *)out)[gid] = (int)iv;
+		break;
+	case VT_INT64:
+		((long long *)out)[gid] = is_float ? (long long)dv : iv;
+		break;
+	case VT_BOOLEAN:
+		((unsigned char *)out)[gid] = (unsigned char)(iv != 0);
+		break;
+	case VT_FLOAT32:
+		((float *)out)[gid] = is_float ? (float)dv : (float)iv;
+		break;
+	default:
+		((double *)out)[gid] = is_float ? dv : (double)iv;
+		break;
+	}
+}
+
+//! Register-array element select with a runtime index, unrolled so the state never leaves VGPRs.
+__device__ __forceinline__ u64 Pick(const u64 (&arr)[kMaxComponents], int index) {
+	u64 v = 0;
+#pragma unroll
+	for (int c = 0; c < kMaxComponents; c++) {
+		if (c == index) {
+			v = arr[c];
+		}
+	}
+	return v;
+}
+
+__device__ __forceinline__ void WriteGroup(const AggOutDesc &o, const CompState &s, u64 gid) {
+	const u64 count = Pick(s.a, o.count_comp);
+	for (int i = 0; i < o.naggs; i++) {
+		const int c = o.comp[i];
+		const u64 sa = Pick(s.a, c);
+		const u64 sb = Pick(s.b, c);
+		void *out = o.out[i];
+		switch (o.kind[i]) {
+		case AGG_COUNT_STAR:
+		case AGG_COUNT:
+			((long long *)out)[gid] = (long long)count;
+			break;
+		case AGG_SUM:
+			if (o.comp_kind[i] == COMP_SUM_I128) {
+				if (o.out_type[i] == VT_HUGEINT) {
+					((u64 *)out)[2 * gid] = sa;
+					((u64 *)out)[2 * gid + 1] = sb;
+				} else if (o.out_type[i] == VT_INT64) {
+					((long long *)out)[gid] = (long long)sa;
+				} else {
+					StoreTyped(out, gid, o.out_type[i], 0, I128ToDouble(sa, sb), true);
+				}
+			} else {
+				StoreTyped(out, gid, o.out_type[i], 0, DdValue(AsDouble(sa), AsDouble(sb)), true);
+			}
+			break;
+		case AGG_MIN:
+		case AGG_MAX:
+			if (o.comp_kind[i] == COMP_MIN_F64 || o.comp_kind[i] == COMP_MAX_F64) {
+				StoreTyped(out, gid, o.out_type[i], 0, AsDouble(sa), true);
+			} else {
+				StoreTyped(out, gid, o.out_type[i], (long long)sa, 0.0, false);
+			}
+			break;
+		case AGG_AVG: {
+			double sum = o.comp_kind[i] == COMP_SUM_I128 ? I128ToDouble(sa, sb) : DdValue(AsDouble(sa), AsDouble(sb));
+			StoreTyped(out, gid, o.out_type[i], 0, sum / (double)count, true);
+			break;
+		}
+		default:
+			break;
+		}
+	}
+}
+
+__device__ __forceinline__ void WriteKeys(const KeyOutDesc &ko, u64 key, u64 gid) {
+	for (int c = 0; c < ko.pack.ncols; c++) {
+		const int bits = ko.pack.bits[c];
+		u64 field = bits == 0 ? 0ull : ((key >> ko.pack.shift[c]) & (bits >= 64 ? ~0ull : ((1ull << bits) - 1)));
+		u64 raw = UnorderedKey(field + ko.pack.min[c], ko.pack.width[c], ko.pack.is_signed[c] != 0);
+		switch (ko.pack.width[c]) {
+		case 1:
+			((uint8_t *)ko.out[c])[gid] = (uint8_t)raw;
+			break;
+		case 2:
+			((uint16_t *)ko.out[c])[gid] = (uint16_t)raw;
+			break;
+		case 4:
+			((uint32_t *)ko.out[c])[gid] = (uint32_t)raw;
+			break;
+		default:
+			((u64 *)ko.out[c])[gid] = raw;
+			break;
+		}
+	}
+}
+
+template <int NP>
+__device__ __forceinline__ void LoadRowPayload(const PayloadPtrs &pay, uint64_t row, u64 (&vals)[NP]) {
+#pragma unroll
+	for (int p = 0; p < NP; p++) {
+		vals[p] = 0;
+		if (p < pay.n) {
+			vals[p] = LoadRawBits(pay.ptr[p], row, pay.width[p]);
+		}
+	}
+}
+
+template <typename KeyT, int NP>
+__global__ void __launch_bounds__(kRedBlock)
+    SegReduceKernel(const KeyT *keys, uint64_t n, PayloadPtrs pay, CompDesc d, AggOutDesc o, KeyOutDesc ko,
+                    u64 *tile_state, uint32_t *ticket, uint32_t *tile_heads, u64 *lead_state, u64 *tail_state,
+                    u64 *tail_gid, u64 *groups_out, uint64_t ntiles) {
+	__shared__ uint32_t s_tile;
+	__shared__ u64 s_scan[kRedWaves + 1];
+	__shared__ u64 s_hb;
+	__shared__ KeyT s_lastkey[kRedWaves];
+	__shared__ u64 s_wave_state[kRedWaves][2 * kMaxComponents];
+	__shared__ int s_wave_flag[kRedWaves];
+
+	const int tid = threadIdx.x;
+	const int lane = tid & (kWave - 1);
+	const int wave = tid / kWave;
+	if (tid == 0) {
+		s_tile = FetchAddAgent(ticket, 1u);
+	}
+	__syncthreads();
+	const uint32_t tile = s_tile;
+	const uint64_t tile_base = (uint64_t)tile * kRedTile;
+	const uint64_t base = tile_base + (uint64_t)tid * kRedItems;
+
+	KeyT k[kRedItems];
+#pragma unroll
+	for (int i = 0; i < kRedItems; i++) {
+		k[i] = 0;
+		if (keys != nullptr && base + i < n) {
+			k[i] = keys[base + i];
+		}
+	}
+	// Key of the row just before this thread's first row.
+	KeyT prev = __shfl_up(k[kRedItems - 1], 1);
+	if (lane == kWave - 1) {
+		s_lastkey[wave] = k[kRedItems - 1];
+	}
+	__syncthreads();
+	if (lane == 0) {
+		if (wave > 0) {
+			prev = s_lastkey[wave - 1];
+		} else if (base > 0 && keys != nullptr) {
+			prev = keys[base - 1];
+		} else {
+			prev = 0;
+		}
+	}
+	bool head[kRedItems];
+	u64 hc = 0;
+#pragma unroll
+	for (int i = 0; i < kRedItems; i++) {
+		uint64_t row = base + i;
+		KeyT before = i == 0 ? prev : k[i - 1];
+		head[i] = row < n && (row == 0 || k[i] != before);
+		hc += head[i] ? 1 : 0;
+	}
+	u64 H;
+	u64 hb_excl = BlockExclusiveScan<u64, kRedBlock>(hc, s_scan, &H);
+
+	if (tid == 0) {
+		StoreAgent(tile_state + tile, LbPack(tile == 0 ? kLbInclusive : kLbAggregate, H));
+		tile_heads[tile] = (uint32_t)H;
+	}
+	if (wave == 0) {
+		u64 excl = tile == 0 ? 0ull : LookbackExclusiveWave(tile_state, tile);
+		if (lane == 0) {
+			if (tile > 0) {
+				StoreAgent(tile_state + tile, LbPack(kLbInclusive, excl + H));
+			}
+			s_hb = excl;
+		}
+	}
+	__syncthreads();
+	const u64 hb_global = s_hb;
+
+	// Serial pass over this thread's rows.
+	CompState S, lead;
+	StateIdentity(d, S);
+	StateIdentity(d, lead);
+	bool seen = false;
+	u64 gid = hb_global + hb_excl;
+	u64 cur_gid = 0;
+#pragma unroll
+	for (int i = 0; i < kRedItems; i++) {
+		uint64_t row = base + i;
+		if (row < n) {
+			u64 vals[NP];
+			LoadRowPayload<NP>(pay, row, vals);
+			if (head[i]) {
+				if (seen) {
+					WriteGroup(o, S, cur_gid);
+				} else {
+					lead = S;
+					seen = true;
+				}
+				cur_gid = gid++;
+				WriteKeys(ko, (u64)k[i], cur_gid);
+				StateIdentity(d, S);
+			}
+			StateAddRow<NP>(d, S, vals);
+		}
+	}
+	if (!seen) {
+		lead = S;
+	}
+
+	// Segmented inclusive scan of (seen, S) across the wave: X covers from the latest head at or before this
+	// thread's last row.
+	CompState X = S;
+	bool F = seen;
+	for (int off = 1; off < kWave; off <<= 1) {
+		CompState up;
+		StateShflUp(d, up, X, off);
+		bool fup = __shfl_up((int)F, off) != 0;
+		if (lane >= off) {
+			if (!F) {
+				StateCombine(d, up, X);
+				X = up;
+			}
+			F = F || fup;
+		}
+	}
+	if (lane == kWave - 1) {
+		StateStore(d, s_wave_state[wave], X);
+		s_wave_flag[wave] = F ? 1 : 0;
+	}
+	__syncthreads();
+	// Prefix from earlier waves.
+	CompState P;
+	StateIdentity(d, P);
+	for (int w = 0; w < wave; w++) {
+		CompState t;
+		StateLoad(d, s_wave_state[w], t);
+		if (s_wave_flag[w]) {
+			P = t;
+		} else {
+			StateCombine(d, P, t);
+		}
+	}
+	if (!F) {
+		CompState t = P;
+		StateCombine(d, t, X);
+		X = t;
+	}
+	// Exclusive value for this thread: inclusive X of the previous thread.
+	CompState E;
+	StateShflUp(d, E, X, 1);
+	if (lane == 0) {
+		E = P;
+	}
+
+	if (seen) {
+		CompState V = E;
+		StateCombine(d, V, lead);
+		if (hb_excl > 0) {
+			WriteGroup(o, V, hb_global + hb_excl - 1);
+		} else {
+			StateStore(d, lead_state + (uint64_t)tile * 2 * d.ncomp, V);
+		}
+	}
+	if (tid == kRedBlock - 1) {
+		uint64_t tile_end = tile_base + kRedTile;
+		if (H > 0) {
+			u64 gid_last = hb_global + H - 1;
+			bool ends = tile_end >= n;
+			if (!ends) {
+				KeyT next = keys != nullptr ? keys[tile_end] : (KeyT)0;
+				ends = next != k[kRedItems - 1];
+			}
+			if (ends) {
+				WriteGroup(o, X, gid_last);
+				tail_gid[tile] = ~0ull;
+			} else {
+				StateStore(d, tail_state + (uint64_t)tile * 2 * d.ncomp, X);
+				tail_gid[tile] = gid_last;
+			}
+		} else {
+			StateStore(d, lead_state + (uint64_t)tile * 2 * d.ncomp, X);
+			tail_gid[tile] = ~0ull;
+		}
+		if (tile == ntiles - 1) {
+			*groups_out = hb_global + H;
+		}
+	}
+}
+
+__global__ void __launch_bounds__(64) SegFixupKernel(CompDesc d, AggOutDesc o, const uint32_t *tile_heads,
+                                                     const u64 *lead_state, const u64 *tail_state,
+                                                     const u64 *tail_gid, uint64_t ntiles) {
+	const uint64_t T = blockIdx.x;
+	const u64 g = tail_gid[T];
+	if (g == ~0ull) {
+		return;
+	}
+	const int lane = threadIdx.x;
+	CompState acc;
+	StateLoad(d, tail_state + T * 2 * d.ncomp, acc);
+	for (uint64_t U = T + 1; U < ntiles; U += kWave) {
+		uint64_t u = U + lane;
+		bool in_range = u < ntiles;
+		uint32_t h = in_range ? tile_heads[u] : 1u;
+		u64 stops = __ballot(h > 0);
+		int stop_lane = stops ? (__ffsll((long long)stops) - 1) : kWave;
+		CompState v;
+		if (in_range && lane <= stop_lane) {
+			StateLoad(d, lead_state + u * 2 * d.ncomp, v);
+		} else {
+			StateIdentity(d, v);
+		}
+		// Ordered tree: lanes combine left-to-right pairwise.
+		for (int off = 1; off < kWave; off <<= 1) {
+			CompState other;
+			StateShflXor(d, other, v, off);
+			if ((lane & off) == 0) {
+				StateCombine(d, v, other);
+			} else {
+				StateCombine(d, other, v);
+				v = other;
+			}
+		}
+		StateCombine(d, acc, v);
+		if (stops) {
+			break;
+		}
+	}
+	if (lane == 0) {
+		WriteGroup(o, acc, g);
+	}
+}
+
+template <typename KeyT, int NP>
+void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, const CompDesc &cd,
+                  const AggOutDesc &od, const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch,
+                  hipStream_t stream) {
+	const uint64_t ntiles = (n + kRedTile - 1) / kRedTile;
+	const size_t state_words = 2 * (size_t)std::max(cd.ncomp, 1);
+	auto *lb = static_cast<u64 *>(scratch.Alloc((ntiles + 1) * sizeof(u64)));
+	auto *ticket = reinterpret_cast<uint32_t *>(lb + ntiles);
+	auto *tile_heads = static_cast<uint32_t *>(scratch.Alloc(ntiles * sizeof(uint32_t)));
+	auto *lead_state = static_cast<u64 *>(scratch.Alloc(ntiles * state_words * sizeof(u64)));
+	auto *tail_state = static_cast<u64 *>(scratch.Alloc(ntiles * state_words * sizeof(u64)));
+	auto *tail_gid = static_cast<u64 *>(scratch.Alloc(ntiles * sizeof(u64)));
+	FillU64(lb, 0ull, ntiles + 1, stream);
+	double bytes = (double)n * (run.keys ? (run.keys64 ? 8.0 : 4.0) : 0.0);
+	for (int p = 0; p < pay.n; p++) {
+		bytes += (double)n * pay.width[p];
+	}
+	{
+		ProfileScope prof("seg_reduce", stream, bytes);
+		hipLaunchKernelGGL((SegReduceKernel<KeyT, NP>), dim3((unsigned)ntiles), dim3(kRedBlock), 0, stream,
+		                   (const KeyT *)run.keys, n, pay, cd, od, ko, lb, ticket, tile_heads, lead_state,
+		                   tail_state, tail_gid, groups_device, ntiles);
+		VGPU_HIP(hipGetLastError());
+	}
+	{
+		ProfileScope prof("seg_fixup", stream, 0.0);
+		hipLaunchKernelGGL(SegFixupKernel, dim3((unsigned)ntiles), dim3(64), 0, stream, cd, od, tile_heads,
+		                   lead_state, tail_state, tail_gid, ntiles);
+		VGPU_HIP(hipGetLastError());
+	}
+}
+
+template <typename KeyT>
+void DispatchNP(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, const CompDesc &cd, const AggOutDesc &od,
+                const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch, hipStream_t stream) {
+	if (pay.n <= 1) {
+		LaunchReduce<KeyT, 1>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+	} else if (pay.n <= 2) {
+		LaunchReduce<KeyT, 2>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+	} else if (pay.n <= 4) {
+		LaunchReduce<KeyT, 4>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+	} else {
+		LaunchReduce<KeyT, 8>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+	}
+}
+
+} // namespace
+
+void SegmentedReduce(const SortedRun &run, uint64_t n, const ReduceSpec &spec, uint64_t *groups_device,
+                     Scratch &scratch, hipStream_t stream) {
+	if (spec.ncomp > kMaxComponents || spec.naggs > 16) {
+		throw HipError("segmented reduce: too many aggregates");
+	}
+	if (n == 0) {
+		FillU64(groups_device, 0ull, 1, stream);
+		return;
+	}
+	PayloadPtrs pay;
+	pay.n = run.npayload;
+	for (int p = 0; p < run.npayload; p++) {
+		pay.ptr[p] = run.payload[p];
+		pay.width[p] = run.payload_width[p];
+	}
+	CompDesc cd {};
+	cd.ncomp = spec.ncomp;
+	for (int c = 0; c < spec.ncomp; c++) {
+		cd.kind[c] = spec.comp[c].kind;
+		cd.payload[c] = spec.comp[c].payload;
+		cd.in_type[c] = spec.comp[c].in_type;
+	}
+	AggOutDesc od;
+	od.naggs = spec.naggs;
+	od.count_comp = 0;
+	for (int c = 0; c < spec.ncomp; c++) {
+		if (spec.comp[c].kind == COMP_COUNT) {
+			od.count_comp = c;
+			break;
+		}
+	}
+	for (int i = 0; i < spec.naggs; i++) {
+		od.kind[i] = spec.aggs[i].kind;
+		od.comp[i] = spec.aggs[i].comp;
+		od.comp_kind[i] = spec.comp[spec.aggs[i].comp].kind;
+		od.out_type[i] = spec.aggs[i].out_type;
+		od.out[i] = spec.aggs[i].out;
+	}
+	KeyOutDesc ko;
+	ko.pack = spec.pack;
+	for (int c = 0; c < spec.pack.ncols; c++) {
+		ko.out[c] = spec.key_out[c];
+	}
+	if (run.keys64) {
+		DispatchNP<u64>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+	} else {
+		DispatchNP<uint32_t>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+	}
+}
+
+} // namespace vgpu

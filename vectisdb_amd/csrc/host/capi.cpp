@@ -1,0 +1,198 @@
+// capi.cpp -- extern "C" entry points declared in include/vgpu.h. Each wraps a C++ operator in a
+// try/catch so no exception crosses the C boundary; errors land in a thread-local message.
+#include "../../../include/vgpu.h"
+#include "capi_util.h"
+#include "device_pool.h"
+#include "operators.h"
+
+#include <cstdio>
+#include <cstring>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace vgpu {
+namespace {
+thread_local std::string g_last_error;
+} // namespace
+
+void SetLastError(const std::string &message) {
+	g_last_error = message;
+}
+
+std::vector<DevCol> ToDevCols(const vgpu_column *cols, int32_t n) {
+	std::vector<DevCol> out;
+	for (int32_t i = 0; i < n; i++) {
+		DevCol c;
+		c.data = cols[i].data;
+		c.type = cols[i].type;
+		c.rows = cols[i].rows;
+		out.push_back(c);
+	}
+	return out;
+}
+
+} // namespace vgpu
+
+using namespace vgpu;
+
+extern "C" {
+
+const char *vgpu_last_error(void) {
+	return g_last_error.c_str();
+}
+
+int vgpu_abi_version(void) {
+	return 1;
+}
+
+int vgpu_device_init(int device) {
+	return Guard([&] {
+		VGPU_HIP(hipSetDevice(device));
+		VGPU_HIP(hipFree(nullptr));
+	});
+}
+
+int vgpu_group_by_bound(const vgpu_column *keys, int32_t nkeys, uint64_t rows, uint64_t *bound, void *stream) {
+	return Guard([&] {
+		auto s = static_cast<hipStream_t>(stream);
+		PoolArena arena(s);
+		KeyPack pack;
+		*bound = PlanGroupBy(ToDevCols(keys, nkeys), rows, pack, arena, s);
+	});
+}
+
+int vgpu_group_by(const vgpu_column *keys, int32_t nkeys, const vgpu_column *values, int32_t nvalues,
+                  const vgpu_agg *aggs, int32_t naggs, uint64_t rows, void *const *key_out, void *const *agg_out,
+                  uint64_t capacity, uint64_t *groups, void *stream) {
+	return Guard([&] {
+		auto s = static_cast<hipStream_t>(stream);
+		PoolArena arena(s);
+		auto kc = ToDevCols(keys, nkeys);
+		auto vc = ToDevCols(values, nvalues);
+		std::vector<AggReq> reqs;
+		for (int32_t i = 0; i < naggs; i++) {
+			reqs.push_back({aggs[i].kind, aggs[i].input, aggs[i].output_type});
+		}
+		KeyPack pack;
+		uint64_t bound = PlanGroupBy(kc, rows, pack, arena, s);
+		if (bound > capacity) {
+			throw std::runtime_error("vgpu_group_by: output capacity " + std::to_string(capacity) +
+			                         " is below the group bound " + std::to_string(bound));
+		}
+		// Outputs are the caller's buffers, handed out in order.
+		std::vector<void *> fixed;
+		for (int32_t c = 0; c < nkeys; c++) {
+			fixed.push_back(key_out[c]);
+		}
+		for (int32_t i = 0; i < naggs; i++) {
+			fixed.push_back(agg_out[i]);
+		}
+		size_t next = 0;
+		OutAlloc alloc = [&](size_t) { return fixed.at(next++); };
+		std::vector<void *> ko, ao;
+		*groups = GroupByDevice(kc, vc, reqs, rows, pack, capacity, alloc, ko, ao, TuneSort(pack.total_bits, rows),
+		                        arena, s, true);
+	});
+}
+
+int vgpu_sort_column(const vgpu_column *key, void *keys_out, uint32_t *perm_out, void *stream) {
+	return Guard([&] {
+		auto s = static_cast<hipStream_t>(stream);
+		PoolArena arena(s);
+		if (!TypeIsInteger(key->type)) {
+			throw std::runtime_error("vgpu_sort_column: integer/boolean key required");
+		}
+		const uint64_t n = key->rows;
+		if (n == 0) {
+			return;
+		}
+		const void *cols[1] = {key->data};
+		int types[1] = {key->type};
+		uint64_t mn[1], mx[1];
+		KeyRanges(cols, types, 1, n, mn, mx, arena, s);
+		KeyPack pack;
+		PlanKeyPack(cols, types, 1, mn, mx, pack);
+		const void *pin[2] = {nullptr, key->data};
+		int pw[2] = {4, TypeWidth(key->type)};
+		SortedRun run;
+		SortPacked(pack, pin, pw, 2, n, TuneSort(pack.total_bits, n), run, arena, s);
+		VGPU_HIP(hipMemcpyAsync(keys_out, run.payload[1], n * pw[1], hipMemcpyDeviceToDevice, s));
+		if (perm_out != nullptr) {
+			VGPU_HIP(hipMemcpyAsync(perm_out, run.payload[0], n * 4, hipMemcpyDeviceToDevice, s));
+		}
+	});
+}
+
+int vgpu_filter_gt_i32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_host,
+                       void *stream) {
+	return Guard([&] {
+		auto s = static_cast<hipStream_t>(stream);
+		PoolArena arena(s);
+		auto *count = static_cast<uint64_t *>(arena.Alloc(sizeof(uint64_t)));
+		FilterGtI32(in, n, threshold, out, count, arena, s);
+		VGPU_HIP(hipMemcpyAsync(count_host, count, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+		VGPU_HIP(hipStreamSynchronize(s));
+	});
+}
+
+int vgpu_gather(const void *src, int32_t width, const void *idx, int32_t idx64, uint64_t n, void *out,
+                void *stream) {
+	return Guard([&] { Gather(src, width, idx, idx64 != 0, n, out, static_cast<hipStream_t>(stream)); });
+}
+
+int vgpu_generate_column(void *out, int32_t type, uint64_t n, int32_t dist, int64_t lo, int64_t hi, uint64_t seed,
+                         uint64_t stream_id, const double *cdf, uint64_t cdf_len, const int64_t *perm,
+                         void *stream) {
+	return Guard([&] {
+		GenerateColumn(out, type, n, dist, (long long)lo, (long long)hi, seed, stream_id, cdf, cdf_len,
+		               reinterpret_cast<const long long *>(perm), static_cast<hipStream_t>(stream));
+	});
+}
+
+int vgpu_profile_enable(int32_t on) {
+	return Guard([&] { ProfileEnable(on != 0); });
+}
+
+int vgpu_profile_reset(void) {
+	return Guard([&] { ProfileReset(); });
+}
+
+int vgpu_profile_read(char *buf, size_t cap, size_t *needed) {
+	return Guard([&] {
+		auto entries = ProfileSnapshot();
+		std::ostringstream os;
+		os << "[";
+		for (size_t i = 0; i < entries.size(); i++) {
+			char line[512];
+			std::snprintf(line, sizeof(line), "%s{\"name\":\"%s\",\"ms\":%.6f,\"launches\":%llu,\"bytes\":%.1f}",
+			              i ? "," : "", entries[i].name.c_str(), entries[i].total_ms,
+			              (unsigned long long)entries[i].launches, entries[i].bytes);
+			os << line;
+		}
+		os << "]";
+		std::string text = os.str();
+		if (needed != nullptr) {
+			*needed = text.size() + 1;
+		}
+		if (buf == nullptr || cap < text.size() + 1) {
+			throw std::runtime_error("vgpu_profile_read: buffer too small");
+		}
+		std::memcpy(buf, text.c_str(), text.size() + 1);
+	});
+}
+
+int vgpu_pool_stats(uint64_t *in_use, uint64_t *cached, uint64_t *peak) {
+	return Guard([&] {
+		auto &pool = DevicePool::Instance();
+		*in_use = pool.BytesInUse();
+		*cached = pool.BytesCached();
+		*peak = pool.PeakBytesInUse();
+	});
+}
+
+int vgpu_pool_trim(void) {
+	return Guard([&] { DevicePool::Instance().Trim(); });
+}
+
+} // extern "C"

@@ -1,0 +1,167 @@
+#include "device_pool.h"
+
+#include <algorithm>
+#include <string>
+
+namespace vgpu {
+
+namespace {
+constexpr size_t kSmallGranule = 256;
+constexpr size_t kLargeGranule = 2u << 20;
+constexpr size_t kLargeThreshold = 1u << 20;
+} // namespace
+
+DevicePool &DevicePool::Instance() {
+	// Leaked on purpose: destroying it at exit would race the HIP runtime's own teardown.
+	static DevicePool *pool = new DevicePool();
+	return *pool;
+}
+
+DevicePool::~DevicePool() = default;
+
+size_t DevicePool::RoundSize(size_t bytes) {
+	if (bytes == 0) {
+		bytes = 1;
+	}
+	if (bytes < kLargeThreshold) {
+		return (bytes + kSmallGranule - 1) / kSmallGranule * kSmallGranule;
+	}
+	return (bytes + kLargeGranule - 1) / kLargeGranule * kLargeGranule;
+}
+
+void *DevicePool::TakeCachedLocked(size_t size, hipStream_t stream) {
+	// Accept blocks up to 2x (small) or size + 25% (large) so a huge block is not burned on a tiny request.
+	size_t limit = size < kLargeThreshold ? size * 2 : size + size / 4 + kLargeGranule;
+	for (auto it = free_.lower_bound(size); it != free_.end() && it->first <= limit; ++it) {
+		Block &b = it->second;
+		bool usable = b.stream == stream;
+		if (!usable && b.ready != nullptr) {
+			usable = hipEventQuery(b.ready) == hipSuccess;
+		}
+		if (!usable) {
+			continue;
+		}
+		void *ptr = b.ptr;
+		size_t bsize = b.size;
+		if (b.ready != nullptr) {
+			spare_events_.push_back(b.ready);
+		}
+		free_.erase(it);
+		cached_ -= bsize;
+		live_[ptr] = bsize;
+		in_use_ += bsize;
+		peak_ = std::max(peak_, in_use_);
+		return ptr;
+	}
+	return nullptr;
+}
+
+void *DevicePool::Alloc(size_t bytes, hipStream_t stream) {
+	const size_t size = RoundSize(bytes);
+	{
+		std::lock_guard<std::mutex> lock(mutex_);
+		if (void *p = TakeCachedLocked(size, stream)) {
+			return p;
+		}
+	}
+	void *ptr = nullptr;
+	hipError_t st = hipMalloc(&ptr, size);
+	if (st != hipSuccess) {
+		(void)hipGetLastError();
+		Trim();
+		st = hipMalloc(&ptr, size);
+		if (st != hipSuccess) {
+			(void)hipGetLastError();
+			throw HipError("device pool: hipMalloc(" + std::to_string(size) +
+			               ") failed: " + hipGetErrorString(st));
+		}
+	}
+	std::lock_guard<std::mutex> lock(mutex_);
+	live_[ptr] = size;
+	in_use_ += size;
+	peak_ = std::max(peak_, in_use_);
+	return ptr;
+}
+
+void DevicePool::Free(void *ptr, hipStream_t stream) {
+	if (ptr == nullptr) {
+		return;
+	}
+	std::lock_guard<std::mutex> lock(mutex_);
+	auto it = live_.find(ptr);
+	if (it == live_.end()) {
+		return; // not ours; never free foreign memory
+	}
+	Block b;
+	b.ptr = ptr;
+	b.size = it->second;
+	b.stream = stream;
+	if (!spare_events_.empty()) {
+		b.ready = spare_events_.back();
+		spare_events_.pop_back();
+	} else if (hipEventCreateWithFlags(&b.ready, hipEventDisableTiming) != hipSuccess) {
+		b.ready = nullptr;
+	}
+	if (b.ready != nullptr && hipEventRecord(b.ready, stream) != hipSuccess) {
+		(void)hipEventDestroy(b.ready);
+		b.ready = nullptr;
+	}
+	live_.erase(it);
+	in_use_ -= b.size;
+	cached_ += b.size;
+	free_.emplace(b.size, b);
+}
+
+void DevicePool::Trim() {
+	(void)hipDeviceSynchronize();
+	std::lock_guard<std::mutex> lock(mutex_);
+	for (auto &kv : free_) {
+		(void)hipFree(kv.second.ptr);
+		if (kv.second.ready != nullptr) {
+			(void)hipEventDestroy(kv.second.ready);
+		}
+	}
+	free_.clear();
+	cached_ = 0;
+}
+
+size_t DevicePool::BytesInUse() const {
+	std::lock_guard<std::mutex> lock(mutex_);
+	return in_use_;
+}
+
+size_t DevicePool::BytesCached() const {
+	std::lock_guard<std::mutex> lock(mutex_);
+	return cached_;
+}
+
+size_t DevicePool::PeakBytesInUse() const {
+	std::lock_guard<std::mutex> lock(mutex_);
+	return peak_;
+}
+
+void DevicePool::ResetPeak() {
+	std::lock_guard<std::mutex> lock(mutex_);
+	peak_ = in_use_;
+}
+
+PoolArena::~PoolArena() {
+	for (void *p : blocks_) {
+		DevicePool::Instance().Free(p, stream_);
+	}
+}
+
+void *PoolArena::Alloc(size_t bytes) {
+	void *p = DevicePool::Instance().Alloc(bytes, stream_);
+	blocks_.push_back(p);
+	return p;
+}
+
+void PoolArena::Release(void *ptr) {
+	auto it = std::find(blocks_.begin(), blocks_.end(), ptr);
+	if (it != blocks_.end()) {
+		blocks_.erase(it);
+	}
+}
+
+} // namespace vgpu

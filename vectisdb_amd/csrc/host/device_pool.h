@@ -1,0 +1,85 @@
+// device_pool.h -- stream-aware caching device allocator sized for one MI355X (288 GB HBM3E).
+//
+// hipMalloc/hipFree serialize the device, so the engine never calls them per operator: every plan
+// temporary comes from this pool and goes back to it. A freed block is reusable at once on the stream
+// that freed it (stream order protects it) and on any other stream once the event recorded at free time
+// has completed. Large blocks are rounded to 2 MiB so repeated shapes (the same query over the same
+// tables) find exact-size blocks again.
+#pragma once
+
+#include "../hip/kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+namespace vgpu {
+
+class DevicePool {
+public:
+	static DevicePool &Instance();
+
+	//! Throws HipError when the device is out of memory even after releasing the cache.
+	void *Alloc(size_t bytes, hipStream_t stream);
+	//! Returns a block to the cache; `stream` is the last stream that used it.
+	void Free(void *ptr, hipStream_t stream);
+
+	//! Releases every cached (free) block back to the driver. Synchronizes the device.
+	void Trim();
+
+	size_t BytesInUse() const;
+	size_t BytesCached() const;
+	size_t PeakBytesInUse() const;
+	void ResetPeak();
+
+private:
+	DevicePool() = default;
+	~DevicePool();
+	DevicePool(const DevicePool &) = delete;
+	DevicePool &operator=(const DevicePool &) = delete;
+
+	struct Block {
+		void *ptr = nullptr;
+		size_t size = 0;
+		hipStream_t stream = nullptr;
+		hipEvent_t ready = nullptr; // recorded at free; block reusable elsewhere once complete
+	};
+	static size_t RoundSize(size_t bytes);
+	void *TakeCachedLocked(size_t size, hipStream_t stream);
+
+	mutable std::mutex mutex_;
+	std::multimap<size_t, Block> free_;             // by size
+	std::unordered_map<void *, size_t> live_;        // ptr -> rounded size
+	std::vector<hipEvent_t> spare_events_;
+	size_t in_use_ = 0;
+	size_t cached_ = 0;
+	size_t peak_ = 0;
+};
+
+//! Scratch arena over the pool: everything allocated is returned (tagged with `stream`) on destruction.
+class PoolArena : public Scratch {
+public:
+	explicit PoolArena(hipStream_t stream) : stream_(stream) {
+	}
+	~PoolArena() override;
+	PoolArena(const PoolArena &) = delete;
+	PoolArena &operator=(const PoolArena &) = delete;
+
+	void *Alloc(size_t bytes) override;
+	//! Hands ownership of `ptr` (allocated by this arena) to the caller; it is no longer freed here.
+	void Release(void *ptr);
+	hipStream_t stream() const {
+		return stream_;
+	}
+
+private:
+	hipStream_t stream_;
+	std::vector<void *> blocks_;
+};
+
+} // namespace vgpu

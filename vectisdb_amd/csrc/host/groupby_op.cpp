@@ -1,0 +1,243 @@
+// groupby_op.cpp -- host side of the sort-based GROUP BY: validation, key packing, component planning,
+// then radix sort (keys + aggregate inputs as payload) and the one-pass segmented reduction.
+//
+// Semantics follow cuda_engine/src/operators/gpu_groupby.cu:224-274 (ValidateAggregate) and :303-592
+// (ExecuteGroupByNode): integer/boolean keys only, output = group keys then aggregates, COUNT(*) and
+// COUNT(col) INT64, MIN/MAX keep the input type, AVG DOUBLE. Extensions: SUM over integer columns
+// produces HUGEINT (what DuckDB returns and what the reference refuses at :242-247), and MIN/MAX over
+// FLOAT/DOUBLE order NaN above every value as DuckDB does.
+#include "operators.h"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace vgpu {
+
+namespace {
+bool IsFloat(int t) {
+	return t == VT_FLOAT32 || t == VT_FLOAT64;
+}
+bool IsNumeric(int t) {
+	return t == VT_INT16 || t == VT_INT32 || t == VT_INT64 || IsFloat(t);
+}
+} // namespace
+
+void ValidateAggregateReq(const AggReq &agg, const std::vector<DevCol> &values) {
+	if (agg.kind == AGG_COUNT_STAR) {
+		if (agg.out_type != VT_INT64) {
+			throw std::runtime_error("gpu_groupby: COUNT(*) must produce INT64");
+		}
+		return;
+	}
+	if (agg.input < 0 || agg.input >= (int)values.size()) {
+		throw std::runtime_error("gpu_groupby: aggregate input column index out of range");
+	}
+	const int in = values[agg.input].type;
+	switch (agg.kind) {
+	case AGG_COUNT:
+		if (agg.out_type != VT_INT64) {
+			throw std::runtime_error("gpu_groupby: COUNT must produce INT64");
+		}
+		return;
+	case AGG_SUM:
+		if (IsFloat(in)) {
+			if (agg.out_type != VT_FLOAT64) {
+				throw std::runtime_error("gpu_groupby: SUM over FLOAT/DOUBLE must produce DOUBLE");
+			}
+			return;
+		}
+		if (in == VT_INT16 || in == VT_INT32 || in == VT_INT64) {
+			if (agg.out_type != VT_HUGEINT && agg.out_type != VT_INT64 && agg.out_type != VT_FLOAT64) {
+				throw std::runtime_error("gpu_groupby: SUM over an integer column must produce HUGEINT "
+				                         "(DuckDB's result type), INT64 or DOUBLE");
+			}
+			return;
+		}
+		throw std::runtime_error("gpu_groupby: SUM supports numeric columns only");
+	case AGG_AVG:
+		if (agg.out_type != VT_FLOAT64) {
+			throw std::runtime_error("gpu_groupby: AVG must produce DOUBLE");
+		}
+		if (!IsNumeric(in)) {
+			throw std::runtime_error("gpu_groupby: AVG supports numeric columns only");
+		}
+		return;
+	case AGG_MIN:
+	case AGG_MAX:
+		if (agg.out_type != in) {
+			throw std::runtime_error("gpu_groupby: MIN/MAX output type must match its input column type");
+		}
+		if (!(TypeIsInteger(in) || IsFloat(in))) {
+			throw std::runtime_error("gpu_groupby: MIN/MAX supports integer, boolean and floating columns");
+		}
+		return;
+	default:
+		throw std::runtime_error("gpu_groupby: unknown aggregate kind");
+	}
+}
+
+uint64_t PlanGroupBy(const std::vector<DevCol> &keys, uint64_t rows, KeyPack &pack, Scratch &scratch,
+                     hipStream_t stream) {
+	pack = KeyPack {};
+	if (keys.empty()) {
+		return 1;
+	}
+	if (keys.size() > (size_t)kMaxKeyCols) {
+		throw std::runtime_error("gpu_groupby: at most 4 group key columns are supported");
+	}
+	std::vector<const void *> cols;
+	std::vector<int> types;
+	for (auto &k : keys) {
+		if (!TypeIsInteger(k.type)) {
+			throw std::runtime_error("gpu_groupby: group key is not an integer/boolean column, which this GPU "
+			                         "path requires");
+		}
+		cols.push_back(k.data);
+		types.push_back(k.type);
+	}
+	uint64_t mins[kMaxKeyCols] = {}, maxs[kMaxKeyCols] = {};
+	if (rows == 0) {
+		PlanKeyPack(cols.data(), types.data(), (int)keys.size(), mins, mins, pack);
+		return 0;
+	}
+	KeyRanges(cols.data(), types.data(), (int)keys.size(), rows, mins, maxs, scratch, stream);
+	if (!PlanKeyPack(cols.data(), types.data(), (int)keys.size(), mins, maxs, pack)) {
+		throw std::runtime_error("gpu_groupby: the group keys' value ranges need more than 64 bits once packed");
+	}
+	// Bound = min(rows, prod(range + 1)), saturating.
+	long double product = 1.0L;
+	for (size_t c = 0; c < keys.size(); c++) {
+		product *= (long double)(maxs[c] - mins[c]) + 1.0L;
+	}
+	if (product < (long double)rows) {
+		return (uint64_t)product;
+	}
+	return rows;
+}
+
+uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol> &values,
+                       const std::vector<AggReq> &aggs, uint64_t rows, const KeyPack &pack, uint64_t capacity,
+                       const OutAlloc &out_alloc, std::vector<void *> &key_out, std::vector<void *> &agg_out,
+                       const SortTuning &tuning, Scratch &scratch, hipStream_t stream, bool read_groups,
+                       uint64_t *groups_device) {
+	if (aggs.size() > 16) {
+		throw std::runtime_error("gpu_groupby: at most 16 aggregates per GROUP BY");
+	}
+	for (auto &a : aggs) {
+		ValidateAggregateReq(a, values);
+	}
+	for (auto &k : keys) {
+		if (k.rows != rows) {
+			throw std::runtime_error("gpu_groupby: group key row count mismatch");
+		}
+	}
+	for (auto &v : values) {
+		if (v.rows != rows) {
+			throw std::runtime_error("gpu_groupby: aggregate input column row count mismatch");
+		}
+	}
+
+	// Payload = the distinct value columns the aggregates read; components deduplicated per column.
+	ReduceSpec spec;
+	std::vector<int> payload_of_value(values.size(), -1);
+	std::vector<const void *> payload_in;
+	std::vector<int> payload_width;
+	auto payload_for = [&](int value_index) {
+		if (payload_of_value[value_index] < 0) {
+			if ((int)payload_in.size() >= kMaxPayload) {
+				throw std::runtime_error("gpu_groupby: aggregates read more than 8 distinct columns");
+			}
+			payload_of_value[value_index] = (int)payload_in.size();
+			payload_in.push_back(values[value_index].data);
+			payload_width.push_back(TypeWidth(values[value_index].type));
+		}
+		return payload_of_value[value_index];
+	};
+	spec.ncomp = 1;
+	spec.comp[0].kind = COMP_COUNT;
+	auto comp_for = [&](int kind, int value_index) {
+		int p = payload_for(value_index);
+		for (int c = 0; c < spec.ncomp; c++) {
+			if (spec.comp[c].kind == kind && spec.comp[c].payload == p) {
+				return c;
+			}
+		}
+		if (spec.ncomp >= kMaxComponents) {
+			throw std::runtime_error("gpu_groupby: too many distinct aggregate states (max 8)");
+		}
+		int c = spec.ncomp++;
+		spec.comp[c].kind = kind;
+		spec.comp[c].payload = p;
+		spec.comp[c].in_type = values[value_index].type;
+		return c;
+	};
+	spec.naggs = (int)aggs.size();
+	for (size_t i = 0; i < aggs.size(); i++) {
+		const auto &a = aggs[i];
+		int comp = 0;
+		switch (a.kind) {
+		case AGG_COUNT_STAR:
+		case AGG_COUNT:
+			comp = 0;
+			break;
+		case AGG_SUM:
+		case AGG_AVG:
+			comp = comp_for(IsFloat(values[a.input].type) ? COMP_SUM_DD : COMP_SUM_I128, a.input);
+			break;
+		case AGG_MIN:
+			comp = comp_for(IsFloat(values[a.input].type) ? COMP_MIN_F64 : COMP_MIN_I64, a.input);
+			break;
+		case AGG_MAX:
+			comp = comp_for(IsFloat(values[a.input].type) ? COMP_MAX_F64 : COMP_MAX_I64, a.input);
+			break;
+		}
+		spec.aggs[i].kind = a.kind;
+		spec.aggs[i].comp = comp;
+		spec.aggs[i].out_type = a.out_type;
+	}
+
+	// Outputs.
+	const uint64_t cap = std::max<uint64_t>(capacity, 1);
+	key_out.clear();
+	agg_out.clear();
+	for (size_t c = 0; c < keys.size(); c++) {
+		key_out.push_back(out_alloc(cap * TypeWidth(keys[c].type)));
+		spec.key_out[c] = key_out.back();
+	}
+	for (size_t i = 0; i < aggs.size(); i++) {
+		agg_out.push_back(out_alloc(cap * TypeWidth(aggs[i].out_type)));
+		spec.aggs[i].out = agg_out.back();
+	}
+	spec.pack = pack;
+
+	auto *groups_dev = groups_device;
+	if (groups_dev == nullptr) {
+		groups_dev = static_cast<uint64_t *>(scratch.Alloc(sizeof(uint64_t)));
+	}
+	if (rows == 0) {
+		FillU64(groups_dev, 0ull, 1, stream);
+	} else if (keys.empty()) {
+		SortedRun run;
+		run.keys = nullptr;
+		run.npayload = (int)payload_in.size();
+		for (int p = 0; p < run.npayload; p++) {
+			run.payload[p] = const_cast<void *>(payload_in[p]);
+			run.payload_width[p] = payload_width[p];
+		}
+		SegmentedReduce(run, rows, spec, groups_dev, scratch, stream);
+	} else {
+		SortedRun run;
+		SortPacked(pack, payload_in.data(), payload_width.data(), (int)payload_in.size(), rows, tuning, run, scratch,
+		           stream);
+		SegmentedReduce(run, rows, spec, groups_dev, scratch, stream);
+	}
+	if (!read_groups) {
+		return 0;
+	}
+	uint64_t groups = 0;
+	VGPU_HIP(hipMemcpyAsync(&groups, groups_dev, sizeof(groups), hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	return groups;
+}
+
+} // namespace vgpu
