@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=20_000_000, help="rows in the CPU baseline sample")
     ap.add_argument("--only", default="", help="comma list of extra operator lines (none by default)")
+    ap.add_argument("--sweep", action="store_true", help="also time each sort tuning candidate (stderr)")
     return ap.parse_args()
 
 
@@ -64,7 +65,78 @@ def make_groupby_table(torch, capi, dev, rows, seed=1):
     return key, v1, v2
 
 
-AGGS7 = None
+def kernel_stats(prof, name):
+    for e in prof:
+        if e["name"] == name:
+            avg = e["ms"] / max(e["launches"], 1)
+            b = e["bytes"] / max(e["launches"], 1)
+            return avg, b
+    return None, None
+
+
+def bench_filter_project(torch, capi, dev, steps, copies=16, rows=1_048_576):
+    """fused scan-filter-project over 1,048,576 rows; `copies` table copies are rotated so the
+    27 MB working set is not served from the 256 MB Infinity Cache."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from exprs import cfg0_exprs
+    preds, projs = cfg0_exprs()
+    tables = []
+    for c in range(copies):
+        a = torch.empty(rows, dtype=torch.int32, device=dev)
+        b = torch.empty(rows, dtype=torch.int64, device=dev)
+        x = torch.empty(rows, dtype=torch.float64, device=dev)
+        capi.generate_column(a, capi.INT32, 0, lo=0, hi=999_999, seed=c, stream_id=1)
+        capi.generate_column(b, capi.INT64, 0, lo=-(1 << 40), hi=(1 << 40) - 1, seed=c, stream_id=2)
+        capi.generate_column(x, capi.FLOAT64, 1, seed=c, stream_id=3)
+        tables.append((a, b, x))
+    torch.cuda.synchronize()
+    for t in tables[:2]:
+        capi.filter_project(list(t), preds, projs)
+    capi.profile_reset()
+    capi.profile_enable(True)
+    kept = 0
+    t0 = time.perf_counter()
+    for i in range(steps):
+        _, _, _, kept = capi.filter_project(list(tables[i % copies]), preds, projs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    capi.profile_enable(False)
+    avg, inb = kernel_stats(capi.profile_read(), "fused_filter_project")
+    # algorithmic bytes: read a, b, c (20 B/row) + write kept rows' two 8-byte outputs
+    alg = rows * 20 + kept * 16
+    gbs = alg / (avg * 1e-3) / 1e9
+    return {"operator": "fused filter/project (configs[0])", "rows": rows, "kept": kept,
+            "kernel_ms": round(avg, 5), "rows_per_s": round(rows / (avg * 1e-3), 1),
+            "call_ms": round((t1 - t0) * 1e3 / steps, 4), "algorithmic_bytes": alg, "gbs": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def bench_cross(torch, capi, dev, steps):
+    """configs[2] cross product: 4,096 x 65,536 INT32 pairs WHERE x + y < 2^30, materialized (x, y)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from exprs import col, copy_body, out
+    x = torch.empty(4096, dtype=torch.int32, device=dev)
+    y = torch.empty(65536, dtype=torch.int32, device=dev)
+    capi.generate_column(x, capi.INT32, 0, lo=0, hi=(1 << 30) - 1, seed=11, stream_id=1)
+    capi.generate_column(y, capi.INT32, 0, lo=0, hi=(1 << 30) - 1, seed=11, stream_id=2)
+    pred = (out(f"(({col(0, capi.INT32)} + {col(1, capi.INT32)}) < 1073741824)", capi.BOOLEAN), [0, 1],
+            capi.BOOLEAN)
+    projs = [(copy_body(capi.INT32), [0], capi.INT32), (copy_body(capi.INT32), [1], capi.INT32)]
+    capi.cross_filter([x], [y], [pred], projs)
+    capi.profile_reset()
+    capi.profile_enable(True)
+    kept = 0
+    for i in range(steps):
+        _, kept = capi.cross_filter([x], [y], [pred], projs)
+    torch.cuda.synchronize()
+    capi.profile_enable(False)
+    avg, _ = kernel_stats(capi.profile_read(), "fused_cross_filter")
+    pairs = 4096 * 65536
+    alg = 4 * (4096 + 65536) + 8 * kept
+    gbs = alg / (avg * 1e-3) / 1e9
+    return {"operator": "cross product + filter (configs[2])", "pairs": pairs, "kept": kept,
+            "kernel_ms": round(avg, 4), "rows_per_s": round(pairs / (avg * 1e-3), 1),
+            "algorithmic_bytes": alg, "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def roofline_from_profile(prof, prefer=None):
@@ -131,6 +203,40 @@ def main():
     # algorithmic bytes for the whole GROUP BY step (sum of every kernel's attributed bytes)
     step_bytes = sum(e["bytes"] for e in prof) / args.steps
 
+    if args.sweep:
+        for rb in ("8", "11"):
+            os.environ["VGPU_RADIX_BITS"] = rb
+            capi.group_by([key], [v1, v2], aggs, rows, capacity=cap)
+            capi.profile_reset()
+            capi.profile_enable(True)
+            torch.cuda.synchronize()
+            s0 = time.perf_counter()
+            for i in range(3):
+                capi.group_by([key], [v1, v2], aggs, rows, capacity=cap)
+            torch.cuda.synchronize()
+            s1 = time.perf_counter()
+            capi.profile_enable(False)
+            sp = capi.profile_read()
+            print(f"[sweep] radix_bits={rb}: {(s1 - s0) * 1e3 / 3:.3f} ms/step", file=sys.stderr, flush=True)
+            for e in sorted(sp, key=lambda e: -e["ms"]):
+                avg = e["ms"] / max(e["launches"], 1)
+                gbs = (e["bytes"] / max(e["launches"], 1)) / (avg * 1e-3) / 1e9 if avg > 0 else 0
+                print(f"[sweep]   {e['name']:<20} {avg:8.4f} ms/launch x{e['launches']:<4} {gbs:8.1f} GB/s",
+                      file=sys.stderr, flush=True)
+        os.environ.pop("VGPU_RADIX_BITS", None)
+
+    operators = []
+    extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "cross"]
+    for name in extra:
+        try:
+            if name == "filter":
+                operators.append(bench_filter_project(torch, capi, dev, 50))
+            elif name == "cross":
+                operators.append(bench_cross(torch, capi, dev, 10))
+        except Exception as exc:  # an operator line must not hide the headline number
+            operators.append({"operator": name, "error": str(exc)[:300]})
+        print(f"[bench] operator {operators[-1]}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -164,6 +270,7 @@ def main():
                    "step_gbs": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "operators": operators,
     }
     print(json.dumps(line), flush=True)
 

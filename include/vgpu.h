@@ -78,6 +78,39 @@ int vgpu_group_by(const vgpu_column *keys, int32_t nkeys, const vgpu_column *val
  * Requires rows < 2^30. */
 int vgpu_sort_column(const vgpu_column *key, void *keys_out, uint32_t *perm_out, void *stream);
 
+/* ---------------------------------------------------------------- fused filter / projection (hipRTC)
+ * Expression bodies are GpuExpr::generated_cuda_source text (cuda_engine/include/gpu_engine.hpp:56-61):
+ * they read ((const T*)inputs[k])[row] and write ((U*)out)[idx]. inputs[k] of an expression is column
+ * `inputs[k]` of the column array. */
+typedef struct vgpu_expr {
+	const char *body;
+	const int32_t *inputs;
+	int32_t ninputs;
+	int32_t output_type; /* enum vgpu_type; BOOLEAN for predicates */
+} vgpu_expr;
+
+/* FILTER + PROJECTION in one kernel. Replaces ExecuteFilter + ExecuteProjection
+ * (cuda_engine/src/gpu_executor.cu:645-746) and the per-expression NVRTC kernels they launch. Rows where
+ * every predicate is true (a NULL predicate counts as false, as in SQL WHERE) are written in row order
+ * through every projection into outs[e] (capacity: rows). col_valid[c]: dense byte validity (1 = valid)
+ * or NULL (no NULLs); may itself be NULL. out_valid[e]: NULL or a byte buffer receiving output validity
+ * (AND of the expression's inputs). sel_out (optional) receives the kept row ids. *kept receives the
+ * output row count (synchronizes). With npreds == 0 every row is projected. */
+int vgpu_filter_project(const vgpu_column *cols, const uint8_t *const *col_valid, int32_t ncols,
+                        const vgpu_expr *preds, int32_t npreds, const vgpu_expr *projs, int32_t nprojs,
+                        void *const *outs, uint8_t *const *out_valid, uint32_t *sel_out, uint64_t *kept,
+                        void *stream);
+
+/* CROSS_PRODUCT with an optional fused FILTER. Replaces ExecuteCrossProductNode
+ * (cuda_engine/src/operators/gpu_cross_join.cu:111-192) plus a FILTER above it. Columns 0..nleft-1 are
+ * the left side (left_rows rows), nleft..ncols-1 the right side (right_rows rows); pairs are enumerated
+ * left-major (pair f = (f / right_rows, f % right_rows)), so output order matches the reference's. Kept
+ * pairs are written through the projections (outs capacity: left_rows * right_rows when npreds == 0,
+ * else the caller's bound `capacity`). Fails when more than `capacity` pairs are kept. Synchronizes. */
+int vgpu_cross_filter(const vgpu_column *cols, int32_t ncols, int32_t nleft, uint64_t left_rows, uint64_t right_rows,
+                      const vgpu_expr *preds, int32_t npreds, const vgpu_expr *projs, int32_t nprojs,
+                      void *const *outs, uint64_t capacity, uint64_t *kept, void *stream);
+
 /* ---------------------------------------------------------------- primitives */
 
 /* Order-preserving copy of in[i] > threshold into out; *count_host receives the kept count (synchronizes).

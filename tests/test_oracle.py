@@ -1,0 +1,175 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker. CPU only.
+
+Each oracle function is checked against something independent of its own formula: Python's math.fsum
+(Shewchuk's exact summation, a library routine), exact rational arithmetic (fractions.Fraction), plain
+Python brute force on small inputs, closed forms, and the reference's own known answers (tests/golden).
+"""
+import json
+import math
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def test_exact_sum_matches_fsum_and_fractions():
+    rng = np.random.default_rng(0)
+    cases = [rng.standard_normal(1000), rng.standard_normal(10) * 1e300, np.array([1e16, 1.0, -1e16]),
+             np.array([0.1] * 10), np.array([1.0, 1e100, 1.0, -1e100]), np.array([]),
+             np.array([2.0 ** -1074] * 3), rng.uniform(-1, 1, 5000) * 10.0 ** rng.integers(-20, 20, 5000)]
+    for x in cases:
+        got = oracle.exact_sum(x)
+        assert got == math.fsum(x)
+        if len(x) and np.all(np.isfinite(x)):
+            exact = sum((Fraction(float(v)) for v in x), Fraction(0))
+            assert got == float(exact)  # Fraction -> float is correctly rounded
+
+
+def test_exact_sum_specials():
+    assert math.isinf(oracle.exact_sum(np.array([1.0, np.inf, 2.0])))
+    assert math.isnan(oracle.exact_sum(np.array([np.inf, -np.inf])))
+
+
+def _brute_group_by(key, v1, v2):
+    groups = {}
+    for k, a, b in zip(key.tolist(), v1.tolist(), v2.tolist()):
+        g = groups.setdefault(k, [0, 0, None, None, []])
+        g[0] += 1
+        g[1] += a
+        g[2] = a if g[2] is None else min(g[2], a)
+        g[3] = a if g[3] is None else max(g[3], a)
+        g[4].append(Fraction(b))
+    out = {}
+    for k, (c, s, mn, mx, fr) in groups.items():
+        exact = sum(fr, Fraction(0))
+        out[k] = (c, s, mn, mx, float(exact), float(exact / c) if False else float(exact) / c)
+    return out
+
+
+@pytest.mark.parametrize("n,distinct", [(1, 1), (50, 3), (2000, 37), (5000, 5000)])
+def test_group_by_matches_brute_force(n, distinct):
+    rng = np.random.default_rng(n)
+    key = rng.integers(-distinct, distinct, n).astype(np.int64)
+    v1 = rng.integers(-10**6, 10**6, n).astype(np.int64)
+    v2 = rng.standard_normal(n)
+    res = oracle.group_by_i64(key, v1, v2)
+    exp = _brute_group_by(key, v1, v2)
+    assert list(res["key"]) == sorted(exp)
+    for i, k in enumerate(res["key"].tolist()):
+        c, s, mn, mx, s2, avg = exp[k]
+        assert res["count"][i] == c and res["sum_v1"][i] == s
+        assert res["min_v1"][i] == mn and res["max_v1"][i] == mx
+        assert res["sum_v2"][i] == s2
+        assert res["avg_v2"][i] == avg
+
+
+def test_group_by_int128_sum_no_overflow():
+    key = np.zeros(4, np.int64)
+    v1 = np.array([2**62, 2**62, 2**62, 2**62], np.int64)
+    res = oracle.group_by_i64(key, v1, np.zeros(4))
+    assert res["sum_v1"][0] == 2**64
+
+
+def test_group_by_wide_key_range_uses_sort_path():
+    key = np.array([2**62, -2**62, 5, 2**62, 5], np.int64)
+    res = oracle.group_by_i64(key, np.arange(5, dtype=np.int64), np.ones(5))
+    assert list(res["key"]) == [-2**62, 5, 2**62]
+    assert list(res["count"]) == [1, 2, 2]
+
+
+def test_filter_project_cfg0_matches_numpy():
+    rng = np.random.default_rng(1)
+    n = 100_000
+    a = rng.integers(0, 10**6, n).astype(np.int32)
+    b = rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+    c = rng.random(n)
+    o1, o2 = oracle.filter_project_cfg0(a, b, c)
+    keep = (a < 500000) & (c > 0.25)
+    np.testing.assert_array_equal(o1, (a[keep] * 2).astype(np.int64) + b[keep])
+    np.testing.assert_array_equal(o2, c[keep] * 1.5 + a[keep].astype(np.float64))
+    assert abs(keep.mean() - 0.375) < 0.01
+
+
+def test_join_sum_count_closed_form_and_brute_force():
+    rng = np.random.default_rng(2)
+    nb = 1 << 12
+    build = rng.permutation(nb).astype(np.int64)
+    payload = rng.integers(0, 1000, nb).astype(np.int32)
+    fk = np.where(rng.random(50_000) < 0.9, rng.integers(0, nb, 50_000), rng.integers(nb, 2 * nb, 50_000))
+    fk = fk.astype(np.int64)
+    s, c = oracle.join_sum_count(build, payload, fk)
+    # closed form: keys are a permutation of [0, nb) -> inverse permutation gives each key's payload
+    inv = np.empty(nb, np.int64)
+    inv[build] = np.arange(nb)
+    hit = fk < nb
+    assert c == int(hit.sum())
+    assert s == int(payload[inv[fk[hit]]].astype(np.int64).sum())
+    # duplicates on the build side: brute force
+    b2 = rng.integers(0, 50, 300).astype(np.int64)
+    p2 = rng.integers(0, 1000, 300).astype(np.int32)
+    f2 = rng.integers(0, 60, 400).astype(np.int64)
+    s2, c2 = oracle.join_sum_count(b2, p2, f2)
+    es = ec = 0
+    for f in f2.tolist():
+        m = b2 == f
+        ec += int(m.sum())
+        es += int(p2[m].astype(np.int64).sum())
+    assert (s2, c2) == (es, ec)
+
+
+def test_join_pairs_brute_force():
+    rng = np.random.default_rng(3)
+    left = rng.integers(0, 20, 200).astype(np.int64)
+    right = rng.integers(0, 20, 150).astype(np.int64)
+    l, r = oracle.join_pairs(left, right)
+    exp = sorted((i, j) for i in range(len(left)) for j in range(len(right)) if left[i] == right[j])
+    assert sorted(zip(l.tolist(), r.tolist())) == exp
+
+
+def test_cross_filter_closed_form_vs_brute_force():
+    rng = np.random.default_rng(4)
+    x = rng.integers(0, 1 << 30, 300).astype(np.int32)
+    y = rng.integers(0, 1 << 30, 700).astype(np.int32)
+    n, sx, sy = oracle.cross_filter_count(x, y, 1 << 30)
+    ox, oy = oracle.cross_filter_pairs(x, y, 1 << 30)
+    m = (x[:, None].astype(np.int64) + y[None, :]) < (1 << 30)
+    assert n == int(m.sum()) == len(ox)
+    assert sx == int((x[:, None].astype(np.int64) * m).sum()) and sy == int((y[None, :].astype(np.int64) * m).sum())
+    # left-major order
+    ii, jj = np.nonzero(m)
+    np.testing.assert_array_equal(ox, x[ii])
+    np.testing.assert_array_equal(oy, y[jj])
+
+
+def test_generator_is_counter_based_and_in_range():
+    a = oracle.gen_uniform_i64(1000, -5, 5, 7, 1)
+    assert a.min() >= -5 and a.max() <= 5
+    # row i depends only on (seed, stream, i)
+    b = oracle.gen_uniform_i64(10, -5, 5, 7, 1)
+    np.testing.assert_array_equal(a[:10], b)
+    u = oracle.gen_uniform_f64(100_000, 3, 3)
+    assert 0.0 <= u.min() and u.max() < 1.0 and abs(u.mean() - 0.5) < 0.01
+    # the mixer is splitmix64's finalizer over (seed ^ stream * C1) + (i + 1) * golden
+    z = ((7 ^ (1 * 0xD1B54A32D192ED03)) + 1 * 0x9E3779B97F4A7C15) & (2**64 - 1)
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2**64 - 1)
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2**64 - 1)
+    z ^= z >> 31
+    assert oracle.mix_counter(7, 1, 0) == z
+
+
+def test_reference_known_answers_group_by():
+    """GROUP BY cases of the reference's executor tests (cuda_engine/tests/test_gpu_executor.cpp), restated
+    as data in tests/golden/reference_groupby.json."""
+    cases = json.loads((GOLDEN / "reference_groupby.json").read_text())
+    for case in cases:
+        key = np.array(case["key"], np.int64)
+        v = np.array(case["value"], np.int64)
+        res = oracle.group_by_i64(key, v, v.astype(np.float64))
+        got = {int(k): (int(c), int(s)) for k, c, s in zip(res["key"], res["count"], res["sum_v1"])}
+        exp = {int(k): tuple(x) for k, x in case["expected"].items()}
+        assert got == exp, case["name"]

@@ -48,7 +48,7 @@ def _compile(src: Path) -> Path:
     if src.suffix == ".hip":
         cmd = [HIPCC, "-c", *COMMON, *DEVICE, "-x", "hip", str(src), "-o", str(obj)]
     else:
-        cmd = [HIPCC, "-c", *COMMON, "-ffp-contract=off", str(src), "-o", str(obj)]
+        cmd = [HIPCC, "-c", *COMMON, *DEVICE, str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         noise = ("unknown target feature", "packed-fp32")

@@ -29,6 +29,11 @@ class Agg(C.Structure):
     _fields_ = [("kind", C.c_int32), ("input", C.c_int32), ("output_type", C.c_int32)]
 
 
+class Expr(C.Structure):
+    _fields_ = [("body", C.c_char_p), ("inputs", C.POINTER(C.c_int32)), ("ninputs", C.c_int32),
+                ("output_type", C.c_int32)]
+
+
 # (name, restype, argtypes); every function returns int status unless noted
 _PROTOS = {
     "vgpu_last_error": (C.c_char_p, []),
@@ -43,6 +48,12 @@ _PROTOS = {
     "vgpu_gather": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p]),
     "vgpu_generate_column": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_int32, C.c_int64, C.c_int64, C.c_uint64,
                                        C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "vgpu_filter_project": (C.c_int, [C.POINTER(Column), C.c_void_p, C.c_int32, C.POINTER(Expr), C.c_int32,
+                                      C.POINTER(Expr), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.POINTER(C.c_uint64), C.c_void_p]),
+    "vgpu_cross_filter": (C.c_int, [C.POINTER(Column), C.c_int32, C.c_int32, C.c_uint64, C.c_uint64,
+                                    C.POINTER(Expr), C.c_int32, C.POINTER(Expr), C.c_int32, C.c_void_p, C.c_uint64,
+                                    C.POINTER(C.c_uint64), C.c_void_p]),
     "vgpu_profile_enable": (C.c_int, [C.c_int32]),
     "vgpu_profile_reset": (C.c_int, []),
     "vgpu_profile_read": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -175,6 +186,72 @@ def generate_column(out, vtype, dist, lo=0, hi=0, seed=0, stream_id=0, cdf=None,
                                    C.c_void_p(cdf.data_ptr() if cdf is not None else 0),
                                    cdf.numel() if cdf is not None else 0,
                                    C.c_void_p(perm.data_ptr() if perm is not None else 0), _stream_ptr(stream)))
+
+
+def _exprs(exprs):
+    """exprs: list of (body, [input column indices], output_type) -> ctypes array (kept alive by caller)."""
+    keep = []
+    arr = (Expr * max(len(exprs), 1))()
+    for i, (body, inputs, otype) in enumerate(exprs):
+        ins = (C.c_int32 * max(len(inputs), 1))(*inputs)
+        b = body.encode() if isinstance(body, str) else body
+        keep += [ins, b]
+        arr[i] = Expr(b, ins, len(inputs), otype)
+    return arr, keep
+
+
+def filter_project(cols, preds, projs, valid=None, out_valid=False, emit_selection=False, stream=None):
+    """Fused FILTER + PROJECTION over device tensors.
+
+    cols: list of 1-D device tensors; preds/projs: list of (body, [col indices], output_type);
+    valid: optional list (per column) of uint8 device tensors or None. Returns (outputs, out_valid, sel, kept).
+    """
+    import torch
+    lib = load()
+    rows = cols[0].numel() if cols else 0
+    cc = (Column * max(len(cols), 1))(*[column(c) for c in cols])
+    vptr = None
+    if valid is not None:
+        vptr = (C.c_void_p * len(cols))(*[(v.data_ptr() if v is not None else 0) for v in valid])
+    pa, k1 = _exprs(preds)
+    ja, k2 = _exprs(projs)
+    outs = [empty_column(p[2], rows) for p in projs]
+    op = (C.c_void_p * max(len(outs), 1))(*[o.data_ptr() for o in outs])
+    ov = None
+    ovt = []
+    if out_valid:
+        ovt = [torch.empty(max(rows, 1), dtype=torch.uint8, device=cols[0].device) for _ in projs]
+        ov = (C.c_void_p * max(len(ovt), 1))(*[o.data_ptr() for o in ovt])
+    sel = torch.empty(max(rows, 1), dtype=torch.int32, device=cols[0].device) if emit_selection else None
+    kept = C.c_uint64(0)
+    check(lib.vgpu_filter_project(cc, vptr, len(cols), pa, len(preds), ja, len(projs), op, ov,
+                                  C.c_void_p(sel.data_ptr() if sel is not None else 0), C.byref(kept),
+                                  _stream_ptr(stream)))
+    k = kept.value
+    return ([o[:k] for o in outs], [o[:k] for o in ovt], sel[:k] if sel is not None else None, k)
+
+
+def cross_filter(left, right, preds, projs, capacity=None, stream=None):
+    """Left-major cross product of `left` x `right` column lists with a fused filter.
+
+    Column index i < len(left) is left[i], else right[i - len(left)]. Returns (outputs, kept).
+    """
+    lib = load()
+    cols = list(left) + list(right)
+    nl = left[0].numel()
+    nr = right[0].numel()
+    if capacity is None:
+        capacity = nl * nr
+    cc = (Column * len(cols))(*[column(c) for c in cols])
+    pa, k1 = _exprs(preds)
+    ja, k2 = _exprs(projs)
+    outs = [empty_column(p[2], capacity) for p in projs]
+    op = (C.c_void_p * max(len(outs), 1))(*[o.data_ptr() for o in outs])
+    kept = C.c_uint64(0)
+    check(lib.vgpu_cross_filter(cc, len(cols), len(left), nl, nr, pa, len(preds), ja, len(projs), op, capacity,
+                                C.byref(kept), _stream_ptr(stream)))
+    k = kept.value
+    return [o[:k] for o in outs], k
 
 
 def profile_enable(on=True):

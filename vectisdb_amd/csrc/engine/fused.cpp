@@ -1,0 +1,411 @@
+// fused.cpp -- source generator and launcher for the fused filter/projection kernels (see fused.h).
+#include "fused.h"
+
+#include "../../../include/kernel_cache.hpp"
+#include "../hip/kernels.h"
+
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+
+namespace vgpu {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kVec = 4;
+constexpr int kRounds = 2;
+constexpr int kTile = kBlock * kVec * kRounds; // 2048 rows (or pairs, or selection entries)
+
+const char *CType(int vtype) {
+	switch (vtype) {
+	case VT_INT16:
+		return "short";
+	case VT_INT32:
+		return "int";
+	case VT_INT64:
+		return "long long";
+	case VT_FLOAT32:
+		return "float";
+	case VT_FLOAT64:
+		return "double";
+	case VT_BOOLEAN:
+		return "bool";
+	default:
+		throw std::runtime_error("fused kernel: unsupported column type (dictionary strings and HUGEINT are not "
+		                         "usable in a fused expression)");
+	}
+}
+
+// Helpers compiled into every generated kernel. DuckDB functions that have no device builtin, the wave
+// look-back over 64-bit [flag:2 | value:62] words (agent scope: the tiles of one launch run on all 8
+// XCDs), and a 64-lane inclusive scan.
+const char *kPreamble = R"VGPU(
+typedef unsigned long long vg_u64;
+__device__ __forceinline__ double __vgpu_radians(double d) { return d * 0.017453292519943295; }
+__device__ __forceinline__ double __vgpu_degrees(double r) { return r * 57.29577951308232; }
+struct VArgs {
+  const void* cols[16];
+  const unsigned char* valid[16];
+  void* outs[16];
+  unsigned char* out_valid[16];
+  vg_u64 n;
+  vg_u64 m;
+  const unsigned int* sel;
+  unsigned int* sel_out;
+  vg_u64* state;
+  vg_u64* count;
+  vg_u64 ntiles;
+  vg_u64 cap;
+};
+__device__ __forceinline__ vg_u64 vg_load(const vg_u64* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void vg_store(vg_u64* p, vg_u64 v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ vg_u64 vg_lookback(const vg_u64* state, long long tile) {
+  const int lane = __lane_id();
+  vg_u64 excl = 0;
+  long long end = tile;
+  while (end > 0) {
+    long long t = end - 1 - lane;
+    vg_u64 word = 0, flag = 2;
+    if (t >= 0) {
+      do { word = vg_load(state + t); flag = word >> 62; if (flag == 0) __builtin_amdgcn_s_sleep(1); } while (flag == 0);
+    }
+    vg_u64 value = t >= 0 ? (word & ((1ull << 62) - 1)) : 0ull;
+    vg_u64 inc = __ballot(flag == 2);
+    int stop = inc ? (__ffsll((long long)inc) - 1) : 64;
+    vg_u64 c = lane <= stop ? value : 0ull;
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+    excl += c;
+    if (inc) break;
+    end -= 64;
+  }
+  return excl;
+}
+__device__ __forceinline__ unsigned int vg_wave_incl(unsigned int v) {
+  const int lane = __lane_id();
+  for (int o = 1; o < 64; o <<= 1) { unsigned int t = __shfl_up(v, o); if (lane >= o) v += t; }
+  return v;
+}
+)VGPU";
+
+void EmitExprScope(std::ostringstream &o, const FusedSpec &s, const FusedExpr &e, int r, const std::string &out_ptr,
+                   const std::string &idx_expr, bool predicate) {
+	o << "        {\n";
+	size_t k = e.slots.size();
+	o << "          const void* inputs[" << (k ? k : 1) << "] = {";
+	for (size_t i = 0; i < k; i++) {
+		o << (i ? ", " : "") << "(const void*)&__v" << e.slots[i] << "[" << r << "][__j]";
+	}
+	if (k == 0) {
+		o << "0";
+	}
+	o << "};\n";
+	o << "          (void)inputs;\n";
+	o << "          const vg_u64 row = 0; (void)row;\n";
+	o << "          const vg_u64 idx = " << idx_expr << "; (void)idx;\n";
+	o << "          void* out = " << out_ptr << "; (void)out;\n";
+	o << "          " << e.body << "\n";
+	o << "        }\n";
+	(void)s;
+	(void)predicate;
+}
+
+std::string ValidExpr(const FusedSpec &s, const FusedExpr &e, int r) {
+	std::string v;
+	for (int slot : e.slots) {
+		if (s.cols[slot].has_valid) {
+			v += (v.empty() ? "" : " && ") + std::string("__m") + std::to_string(slot) + "[" + std::to_string(r) +
+			     "][__j]";
+		}
+	}
+	return v;
+}
+
+} // namespace
+
+std::string GenerateFusedSource(const FusedSpec &s) {
+	if (s.cols.size() > (size_t)kFusedMaxCols || s.projections.size() > (size_t)kFusedMaxOuts) {
+		throw std::runtime_error("fused kernel: at most 16 input columns and 16 outputs");
+	}
+	for (auto *list : {&s.predicates, &s.projections}) {
+		for (auto &e : *list) {
+			for (int slot : e.slots) {
+				if (slot < 0 || slot >= (int)s.cols.size()) {
+					throw std::runtime_error("fused kernel: expression input slot out of range");
+				}
+			}
+		}
+	}
+	const bool filter = !s.predicates.empty();
+	std::ostringstream o;
+	o << kPreamble;
+	o << "extern \"C\" __global__ void __launch_bounds__(" << kBlock << ") vectis_fused(VArgs a) {\n";
+	o << "  const int tid = threadIdx.x; const int lane = tid & 63; const int wave = tid >> 6; (void)lane;\n";
+	if (filter) {
+		o << "  __shared__ unsigned int s_tile; __shared__ vg_u64 s_wtot[4]; __shared__ vg_u64 s_prefix;\n";
+		o << "  if (tid == 0) s_tile = __hip_atomic_fetch_add((unsigned int*)(a.state + a.ntiles), 1u, "
+		     "__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
+		o << "  __syncthreads();\n";
+		o << "  const vg_u64 tile = s_tile;\n";
+	} else {
+		o << "  const vg_u64 tile = blockIdx.x;\n";
+	}
+	o << "  const vg_u64 tile_base = tile * " << kTile << "ull;\n";
+	for (size_t c = 0; c < s.cols.size(); c++) {
+		const char *t = CType(s.cols[c].vtype);
+		o << "  const " << t << "* __c" << c << " = (const " << t << "*)a.cols[" << c << "];\n";
+		o << "  " << t << " __v" << c << "[" << kRounds << "][" << kVec << "];\n";
+		if (s.cols[c].has_valid) {
+			o << "  const unsigned char* __vc" << c << " = a.valid[" << c << "];\n";
+			o << "  bool __m" << c << "[" << kRounds << "][" << kVec << "];\n";
+		}
+	}
+	o << "  vg_u64 __i0[" << kRounds << "];\n";
+	o << "  vg_u64 __rowid[" << kRounds << "][" << kVec << "];\n";
+	o << "  unsigned int __keep[" << kRounds << "];\n";
+	o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) {\n";
+	o << "    const vg_u64 i0 = tile_base + (vg_u64)wave * " << (kVec * kRounds * 64) << "ull + (vg_u64)__r * "
+	  << (kVec * 64) << "ull + (vg_u64)lane * " << kVec << "ull;\n";
+	o << "    __i0[__r] = i0;\n";
+	// Row numbers (and cross-product coordinates) of this lane's 4 entries.
+	if (s.cross) {
+		o << "    vg_u64 __li = i0 / a.m; vg_u64 __rj = i0 - __li * a.m;\n";
+		o << "    vg_u64 __lrow[" << kVec << "], __rrow[" << kVec << "];\n";
+		o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) {\n";
+		o << "      __lrow[__j] = __li; __rrow[__j] = __rj; __rowid[__r][__j] = i0 + __j;\n";
+		o << "      __rj++; while (__rj >= a.m) { __rj -= a.m; __li++; }\n";
+		o << "    }\n";
+	} else if (s.selection_in) {
+		o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) {\n";
+		o << "      __rowid[__r][__j] = (i0 + __j < a.n) ? (vg_u64)a.sel[i0 + __j] : 0ull;\n";
+		o << "    }\n";
+	} else {
+		o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) __rowid[__r][__j] = i0 + __j;\n";
+	}
+	o << "    const bool __full = i0 + " << (kVec - 1) << " < a.n;\n";
+	// Loads: 128-bit (or 64/32-bit for narrow types) when the lane's 4 rows are contiguous and aligned.
+	for (size_t c = 0; c < s.cols.size(); c++) {
+		const char *t = CType(s.cols[c].vtype);
+		std::string cn = "__c" + std::to_string(c);
+		std::string vn = "__v" + std::to_string(c) + "[__r]";
+		if (s.cross) {
+			const char *rr = s.cols[c].side == 0 ? "__lrow" : "__rrow";
+			o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) " << vn << "[__j] = (i0 + __j < a.n) ? "
+			  << cn << "[" << rr << "[__j]] : (" << t << ")0;\n";
+		} else if (s.selection_in) {
+			o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) " << vn << "[__j] = (i0 + __j < a.n) ? "
+			  << cn << "[__rowid[__r][__j]] : (" << t << ")0;\n";
+		} else {
+			o << "    if (__full && ((vg_u64)(" << cn << " + i0) & " << (kVec * 1) << " * sizeof(" << t << ") - 1) == 0) {\n";
+			o << "      const void* __ap = __builtin_assume_aligned((const void*)(" << cn << " + i0), " << kVec
+			  << " * sizeof(" << t << "));\n";
+			o << "      __builtin_memcpy(&" << vn << "[0], __ap, " << kVec << " * sizeof(" << t << "));\n";
+			o << "    } else {\n";
+			o << "#pragma unroll\n      for (int __j = 0; __j < " << kVec << "; __j++) " << vn << "[__j] = (i0 + __j < a.n) ? "
+			  << cn << "[i0 + __j] : (" << t << ")0;\n";
+			o << "    }\n";
+		}
+		if (s.cols[c].has_valid) {
+			std::string mn = "__m" + std::to_string(c) + "[__r]";
+			std::string idx = s.cross ? (s.cols[c].side == 0 ? "__lrow[__j]" : "__rrow[__j]") : "__rowid[__r][__j]";
+			o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) " << mn
+			  << "[__j] = (i0 + __j < a.n) ? (__vc" << c << " == 0 || __vc" << c << "[" << idx << "] != 0) : false;\n";
+		}
+	}
+	// Predicates.
+	o << "    unsigned int __k = 0;\n";
+	o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) {\n";
+	o << "      bool __ok = i0 + __j < a.n;\n";
+	for (size_t p = 0; p < s.predicates.size(); p++) {
+		auto &e = s.predicates[p];
+		o << "      bool __p" << p << " = false;\n";
+		for (int r = 0; r < 1; r++) {
+			(void)r;
+		}
+		o << "      if (__ok) {\n";
+		std::ostringstream body;
+		EmitExprScope(body, s, e, -1, "(void*)&__p" + std::to_string(p), "0", true);
+		std::string text = body.str();
+		// The register tile index is the loop variable __r here.
+		size_t pos;
+		while ((pos = text.find("[-1]")) != std::string::npos) {
+			text.replace(pos, 4, "[__r]");
+		}
+		o << text;
+		std::string v = ValidExpr(s, e, 0);
+		while ((pos = v.find("[0]")) != std::string::npos) {
+			v.replace(pos, 3, "[__r]");
+		}
+		o << "      }\n";
+		o << "      __ok = __ok && __p" << p << (v.empty() ? "" : " && (" + v + ")") << ";\n";
+	}
+	o << "      if (__ok) __k |= 1u << __j;\n";
+	o << "    }\n";
+	o << "    __keep[__r] = __k;\n";
+	o << "  }\n";
+
+	// Output positions.
+	if (filter) {
+		o << "  unsigned int __cnt[" << kRounds << "], __excl[" << kRounds << "], __rtot[" << kRounds << "];\n";
+		o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) {\n";
+		o << "    __cnt[__r] = __popc(__keep[__r]);\n";
+		o << "    unsigned int __inc = vg_wave_incl(__cnt[__r]);\n";
+		o << "    __excl[__r] = __inc - __cnt[__r];\n";
+		o << "    __rtot[__r] = __shfl(__inc, 63);\n";
+		o << "  }\n";
+		o << "  vg_u64 __wtot = 0;\n";
+		o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) __wtot += __rtot[__r];\n";
+		o << "  if (lane == 0) s_wtot[wave] = __wtot;\n";
+		o << "  __syncthreads();\n";
+		o << "  vg_u64 __wexcl = 0, __ttot = 0;\n";
+		o << "  for (int w = 0; w < 4; w++) { if (w < wave) __wexcl += s_wtot[w]; __ttot += s_wtot[w]; }\n";
+		o << "  if (tid == 0) vg_store(a.state + tile, ((tile == 0 ? 2ull : 1ull) << 62) | __ttot);\n";
+		o << "  if (wave == 0) {\n";
+		o << "    vg_u64 __pre = tile == 0 ? 0ull : vg_lookback(a.state, (long long)tile);\n";
+		o << "    if (lane == 0) {\n";
+		o << "      if (tile > 0) vg_store(a.state + tile, (2ull << 62) | (__pre + __ttot));\n";
+		o << "      s_prefix = __pre;\n";
+		o << "      if (tile == a.ntiles - 1) *a.count = __pre + __ttot;\n";
+		o << "    }\n";
+		o << "  }\n";
+		o << "  __syncthreads();\n";
+		o << "  vg_u64 __base = s_prefix + __wexcl;\n";
+	}
+	// Writes.
+	o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) {\n";
+	if (filter) {
+		o << "    vg_u64 __pos = __base + __excl[__r];\n";
+		o << "    for (int __q = 0; __q < __r; __q++) __pos += __rtot[__q];\n";
+	}
+	o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) {\n";
+	if (filter) {
+		o << "      if (((__keep[__r] >> __j) & 1u) && __pos < a.cap) {\n";
+		o << "        const vg_u64 __o = __pos++;\n";
+	} else {
+		o << "      if (__i0[__r] + __j < a.n) {\n";
+		o << "        const vg_u64 __o = __i0[__r] + __j;\n";
+	}
+	if (s.emit_selection) {
+		o << "        a.sel_out[__o] = (unsigned int)__rowid[__r][__j];\n";
+	}
+	for (size_t e = 0; e < s.projections.size(); e++) {
+		auto &ex = s.projections[e];
+		std::ostringstream body;
+		EmitExprScope(body, s, ex, -1, "a.outs[" + std::to_string(e) + "]", "__o", false);
+		std::string text = body.str();
+		size_t pos;
+		while ((pos = text.find("[-1]")) != std::string::npos) {
+			text.replace(pos, 4, "[__r]");
+		}
+		o << text;
+		std::string v = ValidExpr(s, ex, 0);
+		while ((pos = v.find("[0]")) != std::string::npos) {
+			v.replace(pos, 3, "[__r]");
+		}
+		if (!v.empty()) {
+			o << "        if (a.out_valid[" << e << "] != 0) a.out_valid[" << e << "][__o] = (" << v << ") ? 1 : 0;\n";
+		}
+	}
+	o << "      }\n";
+	o << "    }\n";
+	o << "  }\n";
+	if (!filter) {
+		o << "  (void)__keep;\n";
+	}
+	o << "}\n";
+	return o.str();
+}
+
+FusedKernel::~FusedKernel() {
+	DestroyRtcKernel(rtc_);
+}
+
+size_t FusedKernel::StateBytes(uint64_t n) {
+	uint64_t tiles = (n + kTile - 1) / kTile;
+	return (tiles + 1) * sizeof(uint64_t);
+}
+
+namespace {
+struct VArgsHost {
+	const void *cols[16];
+	const unsigned char *valid[16];
+	void *outs[16];
+	unsigned char *out_valid[16];
+	uint64_t n;
+	uint64_t m;
+	const uint32_t *sel;
+	uint32_t *sel_out;
+	uint64_t *state;
+	uint64_t *count;
+	uint64_t ntiles;
+	uint64_t cap;
+};
+} // namespace
+
+void FusedKernel::Launch(const FusedLaunch &args, void *scratch_state, size_t scratch_bytes, hipStream_t stream) const {
+	if (args.n == 0) {
+		if (!spec_.predicates.empty() && args.count_device != nullptr) {
+			FillU64(args.count_device, 0ull, 1, stream);
+		}
+		return;
+	}
+	VArgsHost h {};
+	for (size_t c = 0; c < spec_.cols.size(); c++) {
+		h.cols[c] = args.cols[c];
+		h.valid[c] = args.valid != nullptr ? args.valid[c] : nullptr;
+	}
+	for (size_t e = 0; e < spec_.projections.size(); e++) {
+		h.outs[e] = args.outs[e];
+		h.out_valid[e] = args.out_valid != nullptr ? args.out_valid[e] : nullptr;
+	}
+	h.n = args.n;
+	h.m = args.right_rows == 0 ? 1 : args.right_rows;
+	h.sel = args.sel;
+	h.sel_out = args.sel_out;
+	const uint64_t tiles = (args.n + kTile - 1) / kTile;
+	h.ntiles = tiles;
+	h.cap = args.capacity == 0 ? args.n : args.capacity;
+	h.count = args.count_device;
+	if (!spec_.predicates.empty()) {
+		if (scratch_state == nullptr || scratch_bytes < StateBytes(args.n)) {
+			throw std::runtime_error("fused kernel: look-back state buffer too small");
+		}
+		if (args.count_device == nullptr) {
+			throw std::runtime_error("fused kernel: a filtering launch needs a device count slot");
+		}
+		h.state = static_cast<uint64_t *>(scratch_state);
+		FillU64(h.state, 0ull, tiles + 1, stream);
+	}
+	void *kargs[] = {&h};
+	double bytes = 0;
+	for (auto &c : spec_.cols) {
+		bytes += (double)args.n * TypeWidth(c.vtype);
+	}
+	ProfileScope prof(spec_.cross ? "fused_cross_filter" : "fused_filter_project", stream, bytes);
+	VGPU_HIP(hipModuleLaunchKernel(rtc_->function, (unsigned)tiles, 1, 1, kBlock, 1, 1, 0, stream, kargs, nullptr));
+}
+
+std::shared_ptr<FusedKernel> GetFusedKernel(const FusedSpec &spec) {
+	static std::mutex mutex;
+	static std::map<std::string, std::shared_ptr<FusedKernel>> cache;
+	std::string source = GenerateFusedSource(spec);
+	std::string key = vector_gpu::KernelCache::HashSource(source + (spec.fast_math ? "#fast" : "#strict"));
+	std::lock_guard<std::mutex> lock(mutex);
+	auto it = cache.find(key);
+	if (it != cache.end()) {
+		return it->second;
+	}
+	auto k = std::shared_ptr<FusedKernel>(new FusedKernel());
+	k->spec_ = spec;
+	k->source = source;
+	k->rtc_ = CompileRtc(source, "vectis_fused", spec.fast_math);
+	cache.emplace(key, k);
+	return k;
+}
+
+} // namespace vgpu

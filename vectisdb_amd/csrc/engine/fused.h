@@ -1,0 +1,90 @@
+// fused.h -- generated fused filter/projection kernels (hipRTC, gfx950).
+//
+// One kernel evaluates every predicate of a FILTER (AND-ed; a NULL predicate drops the row) and every
+// projection above it, compacts the surviving rows in order (wave64 ballots + per-lane prefix, global
+// offset by decoupled look-back) and writes dense outputs -- one read of each input column and one write
+// of each output, instead of the reference's mask kernel per predicate, AND kernel, copy_if, narrowing copy
+// and one kernel per projection reading through a selection vector
+// (cuda_engine/src/gpu_executor.cu:645-746).
+//
+// Expression bodies are the extension's text unchanged (they read ((const T*)inputs[k])[row] and write
+// ((U*)out)[idx]); the generator gives each body its own inputs[] of pointers into per-row registers, so
+// the same text runs over 128-bit-loaded register tiles, over a gathered selection, or over the two sides
+// of a cross product.
+#pragma once
+
+#include "rtc.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace vgpu {
+
+struct FusedColumn {
+	int vtype = 1;          // VType of the column
+	bool has_valid = false; // dense byte validity supplied at launch
+	int side = 0;           // cross-product mode: 0 = left (broadcast), 1 = right
+};
+
+struct FusedExpr {
+	std::string body;
+	std::vector<int> slots; // inputs[k] = column slots[k]
+	int out_type = 4;
+};
+
+struct FusedSpec {
+	std::vector<FusedColumn> cols;
+	std::vector<FusedExpr> predicates;
+	std::vector<FusedExpr> projections;
+	bool selection_in = false;   // rows are read through a uint32 selection vector
+	bool emit_selection = false; // write kept row ids (uint32, base-row numbering)
+	bool cross = false;          // rows are (left i, right j) pairs of a left-major cross product
+	bool fast_math = false;
+};
+
+struct FusedLaunch {
+	const void *const *cols = nullptr;        // host array of device column pointers (cols.size())
+	const uint8_t *const *valid = nullptr;    // host array (cols.size()) of dense validity or nullptr
+	void *const *outs = nullptr;              // host array of projection outputs
+	uint8_t *const *out_valid = nullptr;      // host array (projections) or nullptr entries
+	uint64_t n = 0;                           // rows (or selection length, or left rows x right rows)
+	uint64_t right_rows = 0;                  // cross mode: m
+	const uint32_t *sel = nullptr;            // selection_in
+	uint32_t *sel_out = nullptr;              // emit_selection
+	uint64_t *count_device = nullptr;         // kept rows (device); required when predicates exist
+	uint64_t capacity = 0;                    // output rows available (0 = n); writes past it are dropped
+};
+
+class FusedKernel {
+public:
+	~FusedKernel();
+	//! Launches on `stream`. Pointer tables are passed by value in the kernel arguments (no device-side
+	//! scratch, so concurrent launches on different streams never share state).
+	void Launch(const FusedLaunch &args, void *scratch_state, size_t scratch_bytes, hipStream_t stream) const;
+	//! Device bytes of look-back state a launch over `n` rows needs.
+	static size_t StateBytes(uint64_t n);
+	const FusedSpec &spec() const {
+		return spec_;
+	}
+	std::string source;
+
+private:
+	friend std::shared_ptr<FusedKernel> GetFusedKernel(const FusedSpec &spec);
+	FusedSpec spec_;
+	RtcKernel *rtc_ = nullptr;
+};
+
+//! Builds (or fetches from the process cache, keyed by SHA-256 of the generated source) the kernel.
+std::shared_ptr<FusedKernel> GetFusedKernel(const FusedSpec &spec);
+
+//! The generated source (exposed for tests and the disk-cache key).
+std::string GenerateFusedSource(const FusedSpec &spec);
+
+constexpr int kFusedMaxCols = 16;
+constexpr int kFusedMaxOuts = 16;
+
+} // namespace vgpu

@@ -103,11 +103,25 @@ void KeyRanges(const void *const *cols, const int *vtypes, int ncols, uint64_t n
 bool PlanKeyPack(const void *const *cols, const int *vtypes, int ncols, const uint64_t *mins, const uint64_t *maxs,
                  KeyPack &pack);
 
+//! Single integer key column, one read: ordered min/max plus the 256-bin histogram of each of the eight
+//! byte-aligned digits of the ordered key (hist8_dev: 8 x 256 uint32 on the device, zeroed here).
+//! Synchronizes (min/max are read back).
+void KeyRangeHist8(const void *col, int vtype, uint64_t n, uint64_t *min_out, uint64_t *max_out,
+                   uint32_t *hist8_dev, hipStream_t stream);
+
+//! Packing for KeyRangeHist8: the varying span of the ordered key rounded up to whole bytes, minus the
+//! common prefix, so digit p of the packed key is byte p of the ordered key and hist8 row p is exactly
+//! that pass's histogram.
+void PlanKeyPackAligned(const void *col, int vtype, uint64_t omin, uint64_t omax, KeyPack &pack);
+
 // ------------------------------------------------------------------ radix sort
 
 struct SortTuning {
 	int radix_bits = 8;      // 8 or 11 (digit width upper bound)
 	int passes_override = 0; // 0 = derive from total_bits
+	//! Optional precomputed per-byte histograms (KeyRangeHist8) for a byte-aligned pack: skips the
+	//! histogram pass.
+	const uint32_t *hist8 = nullptr;
 };
 
 //! Output of SortPacked: the compact keys (uint32 when total_bits <= 32, else uint64) in ascending order and

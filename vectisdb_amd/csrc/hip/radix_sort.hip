@@ -66,15 +66,26 @@ __global__ void __launch_bounds__(256) KeyRangeKernel(RangeCols cols, uint64_t n
 		lo[c] = ~0ull;
 		hi[c] = 0;
 	}
+	// Four strided rows per step so each thread keeps four loads in flight.
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-	for (uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n; row += stride) {
+	for (uint64_t row0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; row0 < n; row0 += 4 * stride) {
 #pragma unroll
 		for (int c = 0; c < kMaxKeyCols; c++) {
 			if (c < cols.ncols) {
-				u64 ord = OrderedKey(LoadRawBits(cols.col[c], row, cols.width[c]), cols.width[c],
-				                     cols.is_signed[c] != 0);
-				lo[c] = ord < lo[c] ? ord : lo[c];
-				hi[c] = ord > hi[c] ? ord : hi[c];
+				u64 raw[4];
+#pragma unroll
+				for (int u = 0; u < 4; u++) {
+					uint64_t row = row0 + u * stride;
+					raw[u] = row < n ? LoadRawBits(cols.col[c], row, cols.width[c]) : 0ull;
+				}
+#pragma unroll
+				for (int u = 0; u < 4; u++) {
+					if (row0 + u * stride < n) {
+						u64 ord = OrderedKey(raw[u], cols.width[c], cols.is_signed[c] != 0);
+						lo[c] = ord < lo[c] ? ord : lo[c];
+						hi[c] = ord > hi[c] ? ord : hi[c];
+					}
+				}
 			}
 		}
 	}
@@ -95,6 +106,76 @@ __global__ void __launch_bounds__(256) KeyRangeKernel(RangeCols cols, uint64_t n
 	}
 }
 
+//! Min/max of the ordered key and the histograms of all 8 byte digits in one read. A digit that is the
+//! same on every active lane (the constant high bytes of a narrow key range, or a Zipf head value) is
+//! counted with one LDS atomic per wave instead of 64 conflicting ones.
+__global__ void __launch_bounds__(256) KeyRangeHist8Kernel(const void *col, int width, int is_signed, uint64_t n,
+                                                           u64 *mins, u64 *maxs, uint32_t *hist8) {
+	__shared__ uint32_t s_hist[8 * 256];
+	for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) {
+		s_hist[i] = 0;
+	}
+	__syncthreads();
+	const int lane = LaneId();
+	u64 lo = ~0ull, hi = 0;
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	const uint64_t start = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const uint64_t wave_start = start - lane;
+	for (uint64_t row0 = start, w0 = wave_start; w0 < n; row0 += 4 * stride, w0 += 4 * stride) {
+		u64 raw[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			uint64_t row = row0 + u * stride;
+			raw[u] = row < n ? LoadRawBits(col, row, width) : 0ull;
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			uint64_t row = row0 + u * stride;
+			const bool valid = row < n;
+			const u64 vmask = __ballot(valid);
+			if (vmask == 0) {
+				continue;
+			}
+			const u64 ord = OrderedKey(raw[u], width, is_signed != 0);
+			if (valid) {
+				lo = ord < lo ? ord : lo;
+				hi = ord > hi ? ord : hi;
+			}
+			const int leader = __ffsll((long long)vmask) - 1;
+#pragma unroll
+			for (int d = 0; d < 8; d++) {
+				uint32_t dg = (uint32_t)(ord >> (8 * d)) & 255u;
+				uint32_t ref = __shfl(dg, leader);
+				u64 same = __ballot(valid && dg == ref);
+				if (same == vmask) {
+					if (lane == leader) {
+						atomicAdd(&s_hist[d * 256 + ref], (uint32_t)Popc64(vmask));
+					}
+				} else if (valid) {
+					atomicAdd(&s_hist[d * 256 + dg], 1u);
+				}
+			}
+		}
+	}
+	__syncthreads();
+	for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) {
+		uint32_t v = s_hist[i];
+		if (v) {
+			FetchAddAgent(hist8 + i, v);
+		}
+	}
+	for (int off = 32; off >= 1; off >>= 1) {
+		u64 a = __shfl_xor(lo, off);
+		u64 b = __shfl_xor(hi, off);
+		lo = a < lo ? a : lo;
+		hi = b > hi ? b : hi;
+	}
+	if (lane == 0) {
+		MinAgent(mins, lo);
+		MaxAgent(maxs, hi);
+	}
+}
+
 //! One read of the keys builds the digit histogram of every pass.
 template <int R>
 __global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, PassList passes, uint32_t *hist) {
@@ -106,11 +187,21 @@ __global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, Pass
 	}
 	__syncthreads();
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-	for (uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n; row += stride) {
-		u64 key = PackRow(pack, row);
-		for (int p = 0; p < passes.npasses; p++) {
-			uint32_t d = (uint32_t)(key >> passes.shift[p]) & ((1u << passes.bits[p]) - 1);
-			atomicAdd(&s_hist[p * RADIX + d], 1u);
+	for (uint64_t row0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; row0 < n; row0 += 4 * stride) {
+		u64 key[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			uint64_t row = row0 + u * stride;
+			key[u] = row < n ? PackRow(pack, row) : 0ull;
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			if (row0 + u * stride < n) {
+				for (int p = 0; p < passes.npasses; p++) {
+					uint32_t d = (uint32_t)(key[u] >> passes.shift[p]) & ((1u << passes.bits[p]) - 1);
+					atomicAdd(&s_hist[p * RADIX + d], 1u);
+				}
+			}
 		}
 	}
 	__syncthreads();
@@ -122,19 +213,27 @@ __global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, Pass
 	}
 }
 
-template <int R, int ITEMS, typename KeyT, bool FIRST>
+//! One digit pass. Template: R = digit table width (8 or 11 bits), KeyT = compact key (u32/u64), FIRST =
+//! the pass packs the raw key columns, NP = payload columns prefetched into registers (0..4).
+//!
+//! Tile = 256 threads x 16 items, wave-striped (lane-contiguous 64-item rows, so every global load is a
+//! coalesced 64 x width-byte line). Payload values are loaded at kernel entry so their latency overlaps the
+//! ranking and the look-back. One LDS buffer (TILE x 4 bytes per round) stages first the per-wave digit
+//! counters, then the locally sorted keys, then each payload column 32 bits at a time, so the scatter to
+//! global memory writes runs of consecutive addresses per digit bucket.
+template <int R, typename KeyT, bool FIRST, int NP>
 __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const KeyT *keys_in, KeyT *keys_out,
                                                              PayloadArgs pay, uint64_t n, int digit_shift,
                                                              int digit_bits, const uint32_t *ghist,
                                                              uint32_t *tile_state, uint32_t *tile_ticket) {
+	constexpr int ITEMS = 16;
 	constexpr int RADIX = 1 << R;
 	constexpr int TILE = kSortBlock * ITEMS;
 	constexpr int DPT = (RADIX + kSortBlock - 1) / kSortBlock;
-	__shared__ KeyT s_keys[TILE];
-	__shared__ union {
-		u64 pay[TILE];
-		uint32_t wave_count[kSortWaves * RADIX];
-	} s_u;
+	constexpr int LDS_WORDS_KEYS = TILE * (int)(sizeof(KeyT) / 4);
+	constexpr int LDS_WORDS_COUNT = kSortWaves * RADIX;
+	constexpr int LDS_WORDS = LDS_WORDS_KEYS > LDS_WORDS_COUNT ? LDS_WORDS_KEYS : LDS_WORDS_COUNT;
+	__shared__ uint32_t s_buf[LDS_WORDS];
 	__shared__ uint32_t s_digit_base[RADIX];
 	__shared__ uint32_t s_scan[kSortWaves + 1];
 	__shared__ uint32_t s_tile;
@@ -146,7 +245,7 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 		s_tile = FetchAddAgent(tile_ticket, 1u);
 	}
 	for (int i = tid; i < kSortWaves * RADIX; i += kSortBlock) {
-		s_u.wave_count[i] = 0;
+		s_buf[i] = 0;
 	}
 	__syncthreads();
 	const uint32_t tile = s_tile;
@@ -155,7 +254,6 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 	const uint32_t digit_mask = (1u << digit_bits) - 1;
 
 	KeyT key[ITEMS];
-	uint32_t rank[ITEMS];
 #pragma unroll
 	for (int k = 0; k < ITEMS; k++) {
 		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
@@ -169,20 +267,45 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 			key[k] = 0;
 		}
 	}
+	// Payload prefetch (raw bits), issued before the ranking so its latency is hidden.
+	u64 pv[NP > 0 ? NP : 1][ITEMS];
+#pragma unroll
+	for (int p = 0; p < NP; p++) {
+		if (p < pay.n) {
+			const int width = pay.width[p];
+			const void *src = pay.in[p];
+#pragma unroll
+			for (int k = 0; k < ITEMS; k++) {
+				uint64_t row = wave_base + (uint64_t)k * kWave + lane;
+				pv[p][k] = 0;
+				if (row < n) {
+					if (FIRST && src == nullptr) {
+						pv[p][k] = row;
+					} else {
+						pv[p][k] = LoadRawBits(src, row, width);
+					}
+				}
+			}
+		}
+	}
 
 	// Wave-level multisplit: peers = lanes of this round holding the same digit. Rank = earlier rounds'
 	// count of that digit in this wave + earlier peers in this round, which keeps the sort stable.
-	uint32_t *my_counts = s_u.wave_count + wave * RADIX;
+	uint32_t *my_counts = s_buf + wave * RADIX;
+	uint32_t pos[ITEMS];
 #pragma unroll
 	for (int k = 0; k < ITEMS; k++) {
 		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
 		bool valid = row < n;
 		uint32_t d = (uint32_t)(key[k] >> digit_shift) & digit_mask;
 		u64 peers = __ballot(valid);
-		for (int b = 0; b < digit_bits; b++) {
-			bool bit = (d >> b) & 1u;
-			u64 bal = __ballot(bit);
-			peers &= bit ? bal : ~bal;
+#pragma unroll
+		for (int b = 0; b < R; b++) {
+			if (b < digit_bits) {
+				bool bit = (d >> b) & 1u;
+				u64 bal = __ballot(bit);
+				peers &= bit ? bal : ~bal;
+			}
 		}
 		uint32_t r = (uint32_t)Popc64(peers & LaneMaskLt());
 		uint32_t base = valid ? my_counts[d] : 0u;
@@ -191,11 +314,11 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 			my_counts[d] = base + (uint32_t)Popc64(peers);
 		}
 		__builtin_amdgcn_wave_barrier();
-		rank[k] = base + r;
+		pos[k] = base + r;
 	}
 	__syncthreads();
 
-	// Per-digit block totals; wave_count becomes the exclusive offset of each wave within the digit.
+	// Per-digit block totals; the counters become each wave's exclusive offset within the digit.
 	uint32_t dtotal[DPT];
 #pragma unroll
 	for (int j = 0; j < DPT; j++) {
@@ -205,17 +328,16 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 			uint32_t run = 0;
 #pragma unroll
 			for (int w = 0; w < kSortWaves; w++) {
-				uint32_t c = s_u.wave_count[w * RADIX + d];
-				s_u.wave_count[w * RADIX + d] = run;
+				uint32_t c = s_buf[w * RADIX + d];
+				s_buf[w * RADIX + d] = run;
 				run += c;
 			}
 			dtotal[j] = run;
 			StoreAgent(tile_state + (uint64_t)tile * RADIX + d, (tile == 0 ? kStFlagInc : kStFlagAgg) | run);
 		}
 	}
-	uint32_t tsum = 0;
+	uint32_t tsum = 0, gsum = 0;
 	uint32_t gh[DPT];
-	uint32_t gsum = 0;
 #pragma unroll
 	for (int j = 0; j < DPT; j++) {
 		int d = tid * DPT + j;
@@ -228,34 +350,64 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 	uint32_t gtotal;
 	uint32_t gexcl = BlockExclusiveScan<uint32_t, kSortBlock>(gsum, s_scan, &gtotal);
 
+	// Look-back: all of this thread's digits advance together, so their loads are issued in parallel.
+	uint32_t excl[DPT];
+	bool done[DPT];
+#pragma unroll
+	for (int j = 0; j < DPT; j++) {
+		excl[j] = 0;
+		done[j] = tile == 0 || tid * DPT + j >= RADIX;
+	}
+	int64_t t = (int64_t)tile - 1;
+	while (true) {
+		bool all = true;
+#pragma unroll
+		for (int j = 0; j < DPT; j++) {
+			all = all && done[j];
+		}
+		if (all) {
+			break;
+		}
+		uint32_t word[DPT];
+#pragma unroll
+		for (int j = 0; j < DPT; j++) {
+			word[j] = done[j] ? 0u : LoadAgent(tile_state + (uint64_t)t * RADIX + tid * DPT + j);
+		}
+		bool any_pending = false;
+#pragma unroll
+		for (int j = 0; j < DPT; j++) {
+			if (!done[j] && (word[j] & (3u << 30)) == 0) {
+				any_pending = true;
+			}
+		}
+		if (any_pending) {
+			__builtin_amdgcn_s_sleep(1);
+			continue; // re-read the same predecessor
+		}
+#pragma unroll
+		for (int j = 0; j < DPT; j++) {
+			if (!done[j]) {
+				excl[j] += word[j] & kStValue;
+				if ((word[j] & (3u << 30)) == kStFlagInc) {
+					done[j] = true;
+				}
+			}
+		}
+		t--;
+	}
 	uint32_t bstart = texcl;
 	uint32_t goff = gexcl;
 #pragma unroll
 	for (int j = 0; j < DPT; j++) {
 		int d = tid * DPT + j;
 		if (d < RADIX) {
-			uint32_t excl = 0;
 			if (tile > 0) {
-				int64_t t = (int64_t)tile - 1;
-				while (t >= 0) {
-					uint32_t word = LoadAgent(tile_state + (uint64_t)t * RADIX + d);
-					uint32_t flag = word & (3u << 30);
-					if (flag == 0) {
-						__builtin_amdgcn_s_sleep(1);
-						continue;
-					}
-					excl += word & kStValue;
-					if (flag == kStFlagInc) {
-						break;
-					}
-					t--;
-				}
-				StoreAgent(tile_state + (uint64_t)tile * RADIX + d, kStFlagInc | (excl + dtotal[j]));
+				StoreAgent(tile_state + (uint64_t)tile * RADIX + d, kStFlagInc | (excl[j] + dtotal[j]));
 			}
-			s_digit_base[d] = goff + excl - bstart;
+			s_digit_base[d] = goff + excl[j] - bstart;
 #pragma unroll
 			for (int w = 0; w < kSortWaves; w++) {
-				s_u.wave_count[w * RADIX + d] += bstart;
+				s_buf[w * RADIX + d] += bstart;
 			}
 		}
 		bstart += dtotal[j];
@@ -263,23 +415,21 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 	}
 	__syncthreads();
 
-	uint32_t lpos[ITEMS];
 #pragma unroll
 	for (int k = 0; k < ITEMS; k++) {
 		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
 		if (row < n) {
 			uint32_t d = (uint32_t)(key[k] >> digit_shift) & digit_mask;
-			lpos[k] = my_counts[d] + rank[k];
-		} else {
-			lpos[k] = 0;
+			pos[k] += my_counts[d];
 		}
 	}
 	__syncthreads();
+	KeyT *s_keys = reinterpret_cast<KeyT *>(s_buf);
 #pragma unroll
 	for (int k = 0; k < ITEMS; k++) {
 		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
 		if (row < n) {
-			s_keys[lpos[k]] = key[k];
+			s_keys[pos[k]] = key[k];
 		}
 	}
 	__syncthreads();
@@ -298,85 +448,104 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 		}
 	}
 
-	for (int p = 0; p < pay.n; p++) {
-		const int width = pay.width[p];
-		const void *src = pay.in[p];
-#define VGPU_STAGE_PAYLOAD(T)                                                                                  \
-	{                                                                                                          \
-		T *lds = reinterpret_cast<T *>(s_u.pay);                                                               \
-		_Pragma("unroll") for (int k = 0; k < ITEMS; k++) {                                                    \
-			uint64_t row = wave_base + (uint64_t)k * kWave + lane;                                             \
-			if (row < n) {                                                                                     \
-				T v;                                                                                           \
-				if (FIRST && src == nullptr) {                                                                 \
-					v = (T)row;                                                                                \
-				} else {                                                                                       \
-					v = reinterpret_cast<const T *>(src)[row];                                                 \
-				}                                                                                              \
-				lds[lpos[k]] = v;                                                                              \
-			}                                                                                                  \
-		}                                                                                                      \
-		__syncthreads();                                                                                       \
-		T *out = reinterpret_cast<T *>(pay.out[p]);                                                            \
-		_Pragma("unroll") for (int j = 0; j < ITEMS; j++) {                                                    \
-			uint32_t i = tid + j * kSortBlock;                                                                 \
-			if (i < tile_count) {                                                                              \
-				out[dst[j]] = lds[i];                                                                          \
-			}                                                                                                  \
-		}                                                                                                      \
-		__syncthreads();                                                                                       \
-	}
-		if (width == 8) {
-			VGPU_STAGE_PAYLOAD(u64)
-		} else if (width == 4) {
-			VGPU_STAGE_PAYLOAD(uint32_t)
-		} else if (width == 2) {
-			VGPU_STAGE_PAYLOAD(uint16_t)
-		} else {
-			VGPU_STAGE_PAYLOAD(uint8_t)
+	// Payload: staged through LDS 32 bits at a time (two rounds for 8-byte columns).
+#pragma unroll
+	for (int p = 0; p < NP; p++) {
+		if (p < pay.n) {
+			const int width = pay.width[p];
+			const int rounds = width == 8 ? 2 : 1;
+			for (int half = 0; half < rounds; half++) {
+				__syncthreads();
+#pragma unroll
+				for (int k = 0; k < ITEMS; k++) {
+					uint64_t row = wave_base + (uint64_t)k * kWave + lane;
+					if (row < n) {
+						s_buf[pos[k]] = (uint32_t)(pv[p][k] >> (32 * half));
+					}
+				}
+				__syncthreads();
+				void *out = pay.out[p];
+#pragma unroll
+				for (int j = 0; j < ITEMS; j++) {
+					uint32_t i = tid + j * kSortBlock;
+					if (i < tile_count) {
+						uint32_t v = s_buf[i];
+						switch (width) {
+						case 8:
+							reinterpret_cast<uint32_t *>(out)[2 * (u64)dst[j] + half] = v;
+							break;
+						case 4:
+							reinterpret_cast<uint32_t *>(out)[dst[j]] = v;
+							break;
+						case 2:
+							reinterpret_cast<uint16_t *>(out)[dst[j]] = (uint16_t)v;
+							break;
+						default:
+							reinterpret_cast<uint8_t *>(out)[dst[j]] = (uint8_t)v;
+							break;
+						}
+					}
+				}
+			}
 		}
-#undef VGPU_STAGE_PAYLOAD
 	}
 }
+
+template <int R, typename KeyT, int NP>
+void LaunchPassNP(bool first, const KeyPack &pack, const void *keys_in, void *keys_out, const PayloadArgs &pay,
+                  uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state, uint32_t *ticket,
+                  hipStream_t stream, unsigned tiles) {
+	if (first) {
+		hipLaunchKernelGGL((OnesweepKernel<R, KeyT, true, NP>), dim3(tiles), dim3(kSortBlock), 0, stream, pack,
+		                   (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state, ticket);
+	} else {
+		hipLaunchKernelGGL((OnesweepKernel<R, KeyT, false, NP>), dim3(tiles), dim3(kSortBlock), 0, stream, pack,
+		                   (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state, ticket);
+	}
+}
+
+constexpr int kSortTile = kSortBlock * 16;
 
 template <int R, typename KeyT>
 void LaunchPass(bool first, const KeyPack &pack, const void *keys_in, void *keys_out, const PayloadArgs &pay,
                 uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state, uint32_t *ticket,
                 hipStream_t stream) {
-	constexpr int ITEMS = (R <= 8) ? 16 : 16;
-	constexpr int TILE = kSortBlock * ITEMS;
-	uint64_t tiles = (n + TILE - 1) / TILE;
-	double bytes = 0;
-	{
-		double key_in = first ? 0.0 : (double)sizeof(KeyT);
-		if (first) {
-			for (int c = 0; c < pack.ncols; c++) {
-				key_in += pack.width[c];
-			}
-		}
-		double pay_bytes = 0;
-		for (int p = 0; p < pay.n; p++) {
-			pay_bytes += (first && pay.in[p] == nullptr ? 0.0 : pay.width[p]) + pay.width[p];
-		}
-		bytes = (double)n * (key_in + sizeof(KeyT) + pay_bytes);
-	}
-	ProfileScope prof(first ? "radix_pass_first" : "radix_pass", stream, bytes);
+	const unsigned tiles = (unsigned)((n + kSortTile - 1) / kSortTile);
+	double key_in = first ? 0.0 : (double)sizeof(KeyT);
 	if (first) {
-		hipLaunchKernelGGL((OnesweepKernel<R, ITEMS, KeyT, true>), dim3((unsigned)tiles), dim3(kSortBlock), 0,
-		                   stream, pack, (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state,
-		                   ticket);
-	} else {
-		hipLaunchKernelGGL((OnesweepKernel<R, ITEMS, KeyT, false>), dim3((unsigned)tiles), dim3(kSortBlock), 0,
-		                   stream, pack, (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state,
-		                   ticket);
+		for (int c = 0; c < pack.ncols; c++) {
+			key_in += pack.width[c];
+		}
+	}
+	double pay_bytes = 0;
+	for (int p = 0; p < pay.n; p++) {
+		pay_bytes += (first && pay.in[p] == nullptr ? 0.0 : pay.width[p]) + pay.width[p];
+	}
+	ProfileScope prof(first ? "radix_pass_first" : "radix_pass", stream,
+	                  (double)n * (key_in + sizeof(KeyT) + pay_bytes));
+	switch (pay.n) {
+	case 0:
+		LaunchPassNP<R, KeyT, 0>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket, stream, tiles);
+		break;
+	case 1:
+		LaunchPassNP<R, KeyT, 1>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket, stream, tiles);
+		break;
+	case 2:
+		LaunchPassNP<R, KeyT, 2>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket, stream, tiles);
+		break;
+	case 3:
+	case 4:
+		LaunchPassNP<R, KeyT, 4>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket, stream, tiles);
+		break;
+	default:
+		throw HipError("radix sort: at most 4 payload columns per pass");
 	}
 	VGPU_HIP(hipGetLastError());
 }
 
 template <int R>
 uint64_t TilesFor(uint64_t n) {
-	constexpr int TILE = kSortBlock * 16;
-	return (n + TILE - 1) / TILE;
+	return (n + kSortTile - 1) / kSortTile;
 }
 
 } // namespace
@@ -416,6 +585,49 @@ void KeyRanges(const void *const *cols, const int *vtypes, int ncols, uint64_t n
 	}
 }
 
+void KeyRangeHist8(const void *col, int vtype, uint64_t n, uint64_t *min_out, uint64_t *max_out,
+                   uint32_t *hist8_dev, hipStream_t stream) {
+	// hist8_dev is followed by 2 words of min/max scratch (callers allocate 8 * 256 + 4 uint32).
+	u64 *mm = reinterpret_cast<u64 *>(hist8_dev + 8 * 256);
+	FillU32(hist8_dev, 0u, 8 * 256, stream);
+	FillU64(mm, ~0ull, 1, stream);
+	FillU64(mm + 1, 0ull, 1, stream);
+	if (n > 0) {
+		const int width = TypeWidth(vtype);
+		uint64_t blocks = std::min<uint64_t>((n + 1023) / 1024, 2048);
+		ProfileScope prof("key_range_hist", stream, (double)n * width);
+		hipLaunchKernelGGL(KeyRangeHist8Kernel, dim3((unsigned)blocks), dim3(256), 0, stream, col, width,
+		                   TypeIsSigned(vtype) ? 1 : 0, n, mm, mm + 1, hist8_dev);
+		VGPU_HIP(hipGetLastError());
+	}
+	u64 host[2];
+	VGPU_HIP(hipMemcpyAsync(host, mm, sizeof(host), hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	*min_out = host[0];
+	*max_out = host[1];
+}
+
+void PlanKeyPackAligned(const void *col, int vtype, uint64_t omin, uint64_t omax, KeyPack &pack) {
+	pack = KeyPack {};
+	pack.ncols = 1;
+	pack.col[0] = col;
+	pack.width[0] = TypeWidth(vtype);
+	pack.is_signed[0] = TypeIsSigned(vtype) ? 1 : 0;
+	uint64_t diff = omin ^ omax;
+	int span = 0;
+	while (span < 64 && (diff >> span) != 0) {
+		span++;
+	}
+	int bits = ((span + 7) / 8) * 8;
+	if (bits == 0) {
+		bits = 8; // one (trivial) pass keeps the sort path uniform
+	}
+	pack.bits[0] = bits;
+	pack.shift[0] = 0;
+	pack.total_bits = bits;
+	pack.min[0] = bits >= 64 ? 0ull : (omin & ~((1ull << bits) - 1));
+}
+
 bool PlanKeyPack(const void *const *cols, const int *vtypes, int ncols, const uint64_t *mins, const uint64_t *maxs,
                  KeyPack &pack) {
 	pack = KeyPack {};
@@ -451,8 +663,8 @@ void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *p
 	if (n >= (1ull << 30)) {
 		throw HipError("radix sort: at most 2^30 - 1 rows per sort call");
 	}
-	if (npayload > kMaxPayload) {
-		throw HipError("radix sort: too many payload columns");
+	if (npayload > 4) {
+		throw HipError("radix sort: at most 4 payload columns");
 	}
 	out = SortedRun {};
 	out.npayload = npayload;
@@ -505,14 +717,19 @@ void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *p
 		pbuf[p][0] = scratch.Alloc(n * payload_width[p]);
 		pbuf[p][1] = P > 1 ? scratch.Alloc(n * payload_width[p]) : nullptr;
 	}
-	auto *hist = static_cast<uint32_t *>(scratch.Alloc((size_t)P * RADIX * sizeof(uint32_t)));
+	bool aligned8 = R == 8 && tuning.hist8 != nullptr;
+	for (int p = 0; p < P; p++) {
+		aligned8 = aligned8 && passes.bits[p] == 8 && passes.shift[p] == 8 * p;
+	}
+	uint32_t *hist = aligned8 ? const_cast<uint32_t *>(tuning.hist8)
+	                          : static_cast<uint32_t *>(scratch.Alloc((size_t)P * RADIX * sizeof(uint32_t)));
 	const uint64_t state_words = (uint64_t)P * tiles * RADIX + P;
 	auto *state = static_cast<uint32_t *>(scratch.Alloc(state_words * sizeof(uint32_t)));
-	FillU32(hist, 0u, (uint64_t)P * RADIX, stream);
 	FillU32(state, 0u, state_words, stream);
 	uint32_t *tickets = state + (uint64_t)P * tiles * RADIX;
 
-	{
+	if (!aligned8) {
+		FillU32(hist, 0u, (uint64_t)P * RADIX, stream);
 		double bytes = 0;
 		for (int c = 0; c < pack.ncols; c++) {
 			bytes += (double)n * pack.width[c];

@@ -161,28 +161,71 @@ __device__ __forceinline__ void WriteKeys(const KeyOutDesc &ko, u64 key, u64 gid
 	}
 }
 
-template <int NP>
-__device__ __forceinline__ void LoadRowPayload(const PayloadPtrs &pay, uint64_t row, u64 (&vals)[NP]) {
+//! Copies `nbytes` from global `src` into LDS `dst` with the whole block: 16-byte vector loads when both
+//! ends allow it, 4-byte otherwise, bytes for any ragged tail. Consecutive threads read consecutive
+//! vectors, so the copy is one coalesced sweep of the tile.
+template <int MAXV>
+__device__ __forceinline__ void StageToLds(uint8_t *dst, const uint8_t *src, uint32_t nbytes) {
+	uint32_t done = 0;
+	if ((((uintptr_t)src) & 15) == 0) {
+		// All of this thread's vectors are loaded before any is stored, so MAXV loads are in flight at once.
+		const uint32_t vecs = nbytes / 16;
+		uint4 v[MAXV];
 #pragma unroll
-	for (int p = 0; p < NP; p++) {
-		vals[p] = 0;
-		if (p < pay.n) {
-			vals[p] = LoadRawBits(pay.ptr[p], row, pay.width[p]);
+		for (int j = 0; j < MAXV; j++) {
+			uint32_t i = threadIdx.x + j * kRedBlock;
+			if (i < vecs) {
+				v[j] = reinterpret_cast<const uint4 *>(src)[i];
+			}
 		}
+#pragma unroll
+		for (int j = 0; j < MAXV; j++) {
+			uint32_t i = threadIdx.x + j * kRedBlock;
+			if (i < vecs) {
+				reinterpret_cast<uint4 *>(dst)[i] = v[j];
+			}
+		}
+		done = vecs * 16;
+	} else if ((((uintptr_t)src) & 3) == 0) {
+		const uint32_t words = nbytes / 4;
+		for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) {
+			reinterpret_cast<uint32_t *>(dst)[i] = reinterpret_cast<const uint32_t *>(src)[i];
+		}
+		done = words * 4;
+	}
+	for (uint32_t i = done + threadIdx.x; i < nbytes; i += blockDim.x) {
+		dst[i] = src[i];
 	}
 }
 
+__device__ __forceinline__ u64 LdsRawBits(const uint8_t *base, uint32_t row, int width) {
+	switch (width) {
+	case 1:
+		return base[row];
+	case 2:
+		return reinterpret_cast<const uint16_t *>(base)[row];
+	case 4:
+		return reinterpret_cast<const uint32_t *>(base)[row];
+	default:
+		return reinterpret_cast<const u64 *>(base)[row];
+	}
+}
+
+//! Reduce tile: 256 threads x 8 consecutive rows. Keys and payload columns are staged into LDS with one
+//! coalesced sweep each, then every thread folds its 8 rows serially from LDS.
 template <typename KeyT, int NP>
 __global__ void __launch_bounds__(kRedBlock)
     SegReduceKernel(const KeyT *keys, uint64_t n, PayloadPtrs pay, CompDesc d, AggOutDesc o, KeyOutDesc ko,
                     u64 *tile_state, uint32_t *ticket, uint32_t *tile_heads, u64 *lead_state, u64 *tail_state,
-                    u64 *tail_gid, u64 *groups_out, uint64_t ntiles) {
+                    u64 *tail_gid, uint32_t *tail_list, uint32_t *tail_count, u64 *groups_out, uint64_t ntiles) {
 	__shared__ uint32_t s_tile;
 	__shared__ u64 s_scan[kRedWaves + 1];
 	__shared__ u64 s_hb;
-	__shared__ KeyT s_lastkey[kRedWaves];
 	__shared__ u64 s_wave_state[kRedWaves][2 * kMaxComponents];
 	__shared__ int s_wave_flag[kRedWaves];
+	__shared__ __align__(16) KeyT s_keys[kRedTile];
+	__shared__ KeyT s_prev;
+	__shared__ __align__(16) uint8_t s_pay[NP > 0 ? NP : 1][kRedTile * 8];
 
 	const int tid = threadIdx.x;
 	const int lane = tid & (kWave - 1);
@@ -193,38 +236,38 @@ __global__ void __launch_bounds__(kRedBlock)
 	__syncthreads();
 	const uint32_t tile = s_tile;
 	const uint64_t tile_base = (uint64_t)tile * kRedTile;
-	const uint64_t base = tile_base + (uint64_t)tid * kRedItems;
+	const uint32_t tile_count = (uint32_t)((n - tile_base) < (uint64_t)kRedTile ? (n - tile_base) : kRedTile);
 
-	KeyT k[kRedItems];
-#pragma unroll
-	for (int i = 0; i < kRedItems; i++) {
-		k[i] = 0;
-		if (keys != nullptr && base + i < n) {
-			k[i] = keys[base + i];
+	// Stage: s_keys[r] = row r of the tile (16-byte aligned for vector stores); s_prev = key before it.
+	if (keys != nullptr) {
+		StageToLds<4>(reinterpret_cast<uint8_t *>(s_keys), reinterpret_cast<const uint8_t *>(keys + tile_base),
+		              tile_count * (uint32_t)sizeof(KeyT));
+		if (tid == 0) {
+			s_prev = tile_base > 0 ? keys[tile_base - 1] : (KeyT)0;
 		}
 	}
-	// Key of the row just before this thread's first row.
-	KeyT prev = __shfl_up(k[kRedItems - 1], 1);
-	if (lane == kWave - 1) {
-		s_lastkey[wave] = k[kRedItems - 1];
+#pragma unroll
+	for (int p = 0; p < NP; p++) {
+		if (p < pay.n) {
+			const int w = pay.width[p];
+			StageToLds<4>(s_pay[p], reinterpret_cast<const uint8_t *>(pay.ptr[p]) + tile_base * w, tile_count * w);
+		}
 	}
 	__syncthreads();
-	if (lane == 0) {
-		if (wave > 0) {
-			prev = s_lastkey[wave - 1];
-		} else if (base > 0 && keys != nullptr) {
-			prev = keys[base - 1];
-		} else {
-			prev = 0;
-		}
-	}
+
+	const uint32_t local0 = (uint32_t)tid * kRedItems;
+	const uint64_t base = tile_base + local0;
+	KeyT k[kRedItems];
 	bool head[kRedItems];
 	u64 hc = 0;
+	KeyT before = keys == nullptr ? (KeyT)0 : (local0 == 0 ? s_prev : s_keys[local0 - 1]);
 #pragma unroll
 	for (int i = 0; i < kRedItems; i++) {
+		uint32_t lr = local0 + i;
+		k[i] = (keys != nullptr && lr < tile_count) ? s_keys[lr] : (KeyT)0;
 		uint64_t row = base + i;
-		KeyT before = i == 0 ? prev : k[i - 1];
-		head[i] = row < n && (row == 0 || k[i] != before);
+		head[i] = lr < tile_count && (row == 0 || k[i] != before);
+		before = k[i];
 		hc += head[i] ? 1 : 0;
 	}
 	u64 H;
@@ -255,10 +298,13 @@ __global__ void __launch_bounds__(kRedBlock)
 	u64 cur_gid = 0;
 #pragma unroll
 	for (int i = 0; i < kRedItems; i++) {
-		uint64_t row = base + i;
-		if (row < n) {
-			u64 vals[NP];
-			LoadRowPayload<NP>(pay, row, vals);
+		uint32_t lr = local0 + i;
+		if (lr < tile_count) {
+			u64 vals[NP > 0 ? NP : 1];
+#pragma unroll
+			for (int p = 0; p < (NP > 0 ? NP : 1); p++) {
+				vals[p] = (p < pay.n) ? LdsRawBits(s_pay[p], lr, pay.width[p]) : 0ull;
+			}
 			if (head[i]) {
 				if (seen) {
 					WriteGroup(o, S, cur_gid);
@@ -270,7 +316,7 @@ __global__ void __launch_bounds__(kRedBlock)
 				WriteKeys(ko, (u64)k[i], cur_gid);
 				StateIdentity(d, S);
 			}
-			StateAddRow<NP>(d, S, vals);
+			StateAddRow<(NP > 0 ? NP : 1)>(d, S, vals);
 		}
 	}
 	if (!seen) {
@@ -298,7 +344,6 @@ __global__ void __launch_bounds__(kRedBlock)
 		s_wave_flag[wave] = F ? 1 : 0;
 	}
 	__syncthreads();
-	// Prefix from earlier waves.
 	CompState P;
 	StateIdentity(d, P);
 	for (int w = 0; w < wave; w++) {
@@ -315,7 +360,6 @@ __global__ void __launch_bounds__(kRedBlock)
 		StateCombine(d, t, X);
 		X = t;
 	}
-	// Exclusive value for this thread: inclusive X of the previous thread.
 	CompState E;
 	StateShflUp(d, E, X, 1);
 	if (lane == 0) {
@@ -338,7 +382,7 @@ __global__ void __launch_bounds__(kRedBlock)
 			bool ends = tile_end >= n;
 			if (!ends) {
 				KeyT next = keys != nullptr ? keys[tile_end] : (KeyT)0;
-				ends = next != k[kRedItems - 1];
+				ends = next != s_keys[kRedTile - 1];
 			}
 			if (ends) {
 				WriteGroup(o, X, gid_last);
@@ -346,6 +390,8 @@ __global__ void __launch_bounds__(kRedBlock)
 			} else {
 				StateStore(d, tail_state + (uint64_t)tile * 2 * d.ncomp, X);
 				tail_gid[tile] = gid_last;
+				uint32_t slot = FetchAddAgent(tail_count, 1u);
+				tail_list[slot] = tile;
 			}
 		} else {
 			StateStore(d, lead_state + (uint64_t)tile * 2 * d.ncomp, X);
@@ -357,30 +403,42 @@ __global__ void __launch_bounds__(kRedBlock)
 	}
 }
 
-__global__ void __launch_bounds__(64) SegFixupKernel(CompDesc d, AggOutDesc o, const uint32_t *tile_heads,
-                                                     const u64 *lead_state, const u64 *tail_state,
-                                                     const u64 *tail_gid, uint64_t ntiles) {
-	const uint64_t T = blockIdx.x;
+constexpr int kFixBlock = 256;
+constexpr int kFixWaves = kFixBlock / kWave;
+
+//! Finishes one group that continues past tile T: folds the leading partials of the following tiles (up to
+//! and including the first tile that starts a new group) into T's tail partial. 256 following tiles are
+//! examined per step, combined left to right (wave xor-trees in lane order, then waves in order) so the
+//! result does not depend on scheduling. Called by the whole block.
+__device__ __forceinline__ void FixupOne(const CompDesc &d, const AggOutDesc &o, const uint32_t *tile_heads,
+                                         const u64 *lead_state, const u64 *tail_state, const u64 *tail_gid,
+                                         uint64_t T, uint64_t ntiles, u64 (*s_state)[2 * kMaxComponents],
+                                         int *s_stop) {
 	const u64 g = tail_gid[T];
-	if (g == ~0ull) {
-		return;
-	}
-	const int lane = threadIdx.x;
+	const int tid = threadIdx.x;
+	const int lane = tid & (kWave - 1);
+	const int wave = tid / kWave;
 	CompState acc;
 	StateLoad(d, tail_state + T * 2 * d.ncomp, acc);
-	for (uint64_t U = T + 1; U < ntiles; U += kWave) {
-		uint64_t u = U + lane;
+	for (uint64_t U = T + 1; U < ntiles; U += kFixBlock) {
+		uint64_t u = U + tid;
 		bool in_range = u < ntiles;
 		uint32_t h = in_range ? tile_heads[u] : 1u;
 		u64 stops = __ballot(h > 0);
-		int stop_lane = stops ? (__ffsll((long long)stops) - 1) : kWave;
+		if (lane == 0) {
+			s_stop[wave] = stops ? (wave * kWave + __ffsll((long long)stops) - 1) : kFixBlock;
+		}
+		__syncthreads();
+		int stop = kFixBlock;
+		for (int w = 0; w < kFixWaves; w++) {
+			stop = s_stop[w] < stop ? s_stop[w] : stop;
+		}
 		CompState v;
-		if (in_range && lane <= stop_lane) {
+		if (in_range && tid <= stop) {
 			StateLoad(d, lead_state + u * 2 * d.ncomp, v);
 		} else {
 			StateIdentity(d, v);
 		}
-		// Ordered tree: lanes combine left-to-right pairwise.
 		for (int off = 1; off < kWave; off <<= 1) {
 			CompState other;
 			StateShflXor(d, other, v, off);
@@ -391,13 +449,39 @@ __global__ void __launch_bounds__(64) SegFixupKernel(CompDesc d, AggOutDesc o, c
 				v = other;
 			}
 		}
-		StateCombine(d, acc, v);
-		if (stops) {
+		if (lane == 0) {
+			StateStore(d, s_state[wave], v);
+		}
+		__syncthreads();
+		if (tid == 0) {
+			for (int w = 0; w < kFixWaves; w++) {
+				CompState t;
+				StateLoad(d, s_state[w], t);
+				StateCombine(d, acc, t);
+			}
+		}
+		__syncthreads();
+		if (stop < kFixBlock) {
 			break;
 		}
 	}
-	if (lane == 0) {
+	if (tid == 0) {
 		WriteGroup(o, acc, g);
+	}
+	__syncthreads(); // s_state / s_stop are reused by the block's next tail
+}
+
+//! Grid-stride over the compact list of tiles whose last group continues (appended by SegReduceKernel),
+//! so the launch is bounded no matter how many tiles the input has.
+__global__ void __launch_bounds__(kFixBlock) SegFixupKernel(CompDesc d, AggOutDesc o, const uint32_t *tile_heads,
+                                                            const u64 *lead_state, const u64 *tail_state,
+                                                            const u64 *tail_gid, const uint32_t *tail_list,
+                                                            const uint32_t *tail_count, uint64_t ntiles) {
+	__shared__ u64 s_state[kFixWaves][2 * kMaxComponents];
+	__shared__ int s_stop[kFixWaves];
+	const uint32_t ntails = *tail_count;
+	for (uint32_t item = blockIdx.x; item < ntails; item += gridDim.x) {
+		FixupOne(d, o, tile_heads, lead_state, tail_state, tail_gid, tail_list[item], ntiles, s_state, s_stop);
 	}
 }
 
@@ -413,6 +497,8 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 	auto *lead_state = static_cast<u64 *>(scratch.Alloc(ntiles * state_words * sizeof(u64)));
 	auto *tail_state = static_cast<u64 *>(scratch.Alloc(ntiles * state_words * sizeof(u64)));
 	auto *tail_gid = static_cast<u64 *>(scratch.Alloc(ntiles * sizeof(u64)));
+	auto *tail_list = static_cast<uint32_t *>(scratch.Alloc(ntiles * sizeof(uint32_t)));
+	auto *tail_count = ticket + 1; // second half of the zeroed ticket word
 	FillU64(lb, 0ull, ntiles + 1, stream);
 	double bytes = (double)n * (run.keys ? (run.keys64 ? 8.0 : 4.0) : 0.0);
 	for (int p = 0; p < pay.n; p++) {
@@ -422,13 +508,14 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 		ProfileScope prof("seg_reduce", stream, bytes);
 		hipLaunchKernelGGL((SegReduceKernel<KeyT, NP>), dim3((unsigned)ntiles), dim3(kRedBlock), 0, stream,
 		                   (const KeyT *)run.keys, n, pay, cd, od, ko, lb, ticket, tile_heads, lead_state,
-		                   tail_state, tail_gid, groups_device, ntiles);
+		                   tail_state, tail_gid, tail_list, tail_count, groups_device, ntiles);
 		VGPU_HIP(hipGetLastError());
 	}
 	{
 		ProfileScope prof("seg_fixup", stream, 0.0);
-		hipLaunchKernelGGL(SegFixupKernel, dim3((unsigned)ntiles), dim3(64), 0, stream, cd, od, tile_heads,
-		                   lead_state, tail_state, tail_gid, ntiles);
+		const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, 2048);
+		hipLaunchKernelGGL(SegFixupKernel, dim3(grid), dim3(kFixBlock), 0, stream, cd, od, tile_heads, lead_state,
+		                   tail_state, tail_gid, tail_list, tail_count, ntiles);
 		VGPU_HIP(hipGetLastError());
 	}
 }
@@ -436,14 +523,16 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 template <typename KeyT>
 void DispatchNP(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, const CompDesc &cd, const AggOutDesc &od,
                 const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch, hipStream_t stream) {
-	if (pay.n <= 1) {
+	if (pay.n == 0) {
+		LaunchReduce<KeyT, 0>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+	} else if (pay.n == 1) {
 		LaunchReduce<KeyT, 1>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
-	} else if (pay.n <= 2) {
+	} else if (pay.n == 2) {
 		LaunchReduce<KeyT, 2>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
 	} else if (pay.n <= 4) {
 		LaunchReduce<KeyT, 4>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
 	} else {
-		LaunchReduce<KeyT, 8>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		throw HipError("segmented reduce: at most 4 payload columns");
 	}
 }
 

@@ -75,7 +75,8 @@ int vgpu_group_by(const vgpu_column *keys, int32_t nkeys, const vgpu_column *val
 			reqs.push_back({aggs[i].kind, aggs[i].input, aggs[i].output_type});
 		}
 		KeyPack pack;
-		uint64_t bound = PlanGroupBy(kc, rows, pack, arena, s);
+		const uint32_t *hist8 = nullptr;
+		uint64_t bound = PlanGroupBy(kc, rows, pack, arena, s, &hist8);
 		if (bound > capacity) {
 			throw std::runtime_error("vgpu_group_by: output capacity " + std::to_string(capacity) +
 			                         " is below the group bound " + std::to_string(bound));
@@ -91,8 +92,9 @@ int vgpu_group_by(const vgpu_column *keys, int32_t nkeys, const vgpu_column *val
 		size_t next = 0;
 		OutAlloc alloc = [&](size_t) { return fixed.at(next++); };
 		std::vector<void *> ko, ao;
-		*groups = GroupByDevice(kc, vc, reqs, rows, pack, capacity, alloc, ko, ao, TuneSort(pack.total_bits, rows),
-		                        arena, s, true);
+		SortTuning tuning = TuneSort(pack.total_bits, rows);
+		tuning.hist8 = hist8;
+		*groups = GroupByDevice(kc, vc, reqs, rows, pack, capacity, alloc, ko, ao, tuning, arena, s, true);
 	});
 }
 

@@ -77,8 +77,23 @@ void ValidateAggregateReq(const AggReq &agg, const std::vector<DevCol> &values) 
 }
 
 uint64_t PlanGroupBy(const std::vector<DevCol> &keys, uint64_t rows, KeyPack &pack, Scratch &scratch,
-                     hipStream_t stream) {
+                     hipStream_t stream, const uint32_t **hist8) {
 	pack = KeyPack {};
+	if (hist8 != nullptr) {
+		*hist8 = nullptr;
+	}
+	if (keys.size() == 1 && rows > 0 && TypeIsInteger(keys[0].type) && TuneSort(64, rows).radix_bits == 8) {
+		// One read: min/max and every byte-digit histogram.
+		auto *h = static_cast<uint32_t *>(scratch.Alloc((8 * 256 + 4) * sizeof(uint32_t)));
+		uint64_t omin = 0, omax = 0;
+		KeyRangeHist8(keys[0].data, keys[0].type, rows, &omin, &omax, h, stream);
+		PlanKeyPackAligned(keys[0].data, keys[0].type, omin, omax, pack);
+		if (hist8 != nullptr) {
+			*hist8 = h;
+		}
+		uint64_t range = omax - omin;
+		return range < rows ? range + 1 : rows;
+	}
 	if (keys.empty()) {
 		return 1;
 	}

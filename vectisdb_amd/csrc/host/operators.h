@@ -34,8 +34,10 @@ void ValidateAggregateReq(const AggReq &agg, const std::vector<DevCol> &values);
 
 //! Upper bound on the number of groups (min of rows and the product of key ranges); also returns the key
 //! packing. Synchronizes (reads key ranges back). Throws when the keys need more than 64 packed bits.
+//! For a single key column the range pass also builds the byte-digit histograms (*hist8, from `scratch`),
+//! which the sort then reuses instead of reading the keys again.
 uint64_t PlanGroupBy(const std::vector<DevCol> &keys, uint64_t rows, KeyPack &pack, Scratch &scratch,
-                     hipStream_t stream);
+                     hipStream_t stream, const uint32_t **hist8 = nullptr);
 
 //! Sort-based GROUP BY. Key outputs (one per key column, key type) and aggregate outputs (out_type) are
 //! allocated through `out_alloc` with `capacity` rows each. Returns the group count when `read_groups`
