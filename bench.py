@@ -111,6 +111,46 @@ def bench_filter_project(torch, capi, dev, steps, copies=16, rows=1_048_576):
             "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
+    """configs[2] join: build 2^24 unique keys (permutation) + INT32 payload, probe 2^28 fk (90% hits);
+    SELECT SUM(payload), COUNT(*) FROM probe JOIN build ON fk = key."""
+    import numpy as np
+    build = torch.from_numpy(np.random.RandomState(21).permutation(nb).astype(np.int64)).to(dev)
+    payload = torch.empty(nb, dtype=torch.int32, device=dev)
+    capi.generate_column(payload, capi.INT32, 0, lo=0, hi=999, seed=21, stream_id=1)
+    hit = torch.empty(nprobe, dtype=torch.int64, device=dev)
+    miss = torch.empty(nprobe, dtype=torch.int64, device=dev)
+    sel = torch.empty(nprobe, dtype=torch.int64, device=dev)
+    capi.generate_column(hit, capi.INT64, 0, lo=0, hi=nb - 1, seed=21, stream_id=2)
+    capi.generate_column(miss, capi.INT64, 0, lo=nb, hi=2 * nb - 1, seed=21, stream_id=3)
+    capi.generate_column(sel, capi.INT64, 0, lo=0, hi=9, seed=21, stream_id=4)
+    body = ("((long long*)out)[idx] = (((const long long*)inputs[2])[row] == 0LL) ? "
+            "((const long long*)inputs[1])[row] : ((const long long*)inputs[0])[row];")
+    outs, _, _, _ = capi.filter_project([hit, miss, sel], [], [(body, [0, 1, 2], capi.INT64)])
+    fk = outs[0]
+    del hit, miss, sel
+    torch.cuda.synchronize()
+    aggs = [(capi.SUM, 1, 0, capi.HUGEINT), (capi.COUNT_STAR, -1, -1, capi.INT64)]
+    res = capi.join_aggregate(fk, [], build, [payload], aggs)
+    capi.profile_reset()
+    capi.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = capi.join_aggregate(fk, [], build, [payload], aggs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    capi.profile_enable(False)
+    prof = capi.profile_read()
+    ms = (t1 - t0) * 1e3 / steps
+    kernels = {e["name"]: round(e["ms"] / max(e["launches"], 1) * (e["launches"] / steps), 4) for e in prof}
+    alg = sum(e["bytes"] for e in prof) / steps
+    return {"operator": "sort-merge join + aggregate (configs[2])", "probe_rows": nprobe, "build_rows": nb,
+            "sum_payload": int(res[0]), "count": int(res[1]), "ms": round(ms, 3),
+            "rows_per_s": round((nprobe + nb) / (ms * 1e-3), 1), "algorithmic_bytes": alg,
+            "gbs": round(alg / (ms * 1e-3) / 1e9, 1), "kernel_ms_per_step": kernels}
+
+
 def bench_cross(torch, capi, dev, steps):
     """configs[2] cross product: 4,096 x 65,536 INT32 pairs WHERE x + y < 2^30, materialized (x, y)."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -226,13 +266,15 @@ def main():
         os.environ.pop("VGPU_RADIX_BITS", None)
 
     operators = []
-    extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "cross"]
+    extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "cross", "join"]
     for name in extra:
         try:
             if name == "filter":
                 operators.append(bench_filter_project(torch, capi, dev, 50))
             elif name == "cross":
                 operators.append(bench_cross(torch, capi, dev, 10))
+            elif name == "join":
+                operators.append(bench_join(torch, capi, dev, 3))
         except Exception as exc:  # an operator line must not hide the headline number
             operators.append({"operator": name, "error": str(exc)[:300]})
         print(f"[bench] operator {operators[-1]}", file=sys.stderr, flush=True)

@@ -111,6 +111,31 @@ int vgpu_cross_filter(const vgpu_column *cols, int32_t ncols, int32_t nleft, uin
                       const vgpu_expr *preds, int32_t npreds, const vgpu_expr *projs, int32_t nprojs,
                       void *const *outs, uint64_t capacity, uint64_t *kept, void *stream);
 
+/* ---------------------------------------------------------------- sort-merge equi-join
+ * Side A = children[0] (build_keys), side B = children[1] (probe_keys), one integer key of the same type on
+ * both sides, INNER semantics (cuda_engine/src/operators/gpu_hash_join.cu:25-38). */
+typedef struct vgpu_join_agg {
+	int32_t kind;        /* enum vgpu_agg_kind */
+	int32_t side;        /* 0 = A, 1 = B; ignored for COUNT(*) */
+	int32_t input;       /* index into that side's value columns */
+	int32_t output_type; /* COUNT -> INT64, SUM(int) -> HUGEINT, SUM(float) -> FLOAT64, AVG -> FLOAT64,
+	                        MIN/MAX -> input type (BIGINT/DOUBLE) */
+} vgpu_join_agg;
+
+/* Global aggregates over A JOIN B ON akey = bkey without materializing the join (both sides radix-sorted
+ * and reduced to distinct keys, then merge-path matched). Replaces a GROUP_BY_AGGREGATE without keys over
+ * a HASH_JOIN (cuda_engine/src/operators/gpu_hash_join.cu:138-267 followed by gpu_groupby.cu). results:
+ * naggs x 16 host bytes, each the aggregate in its output_type (HUGEINT = {lower, upper}). Synchronizes. */
+int vgpu_join_aggregate(const vgpu_column *akey, const vgpu_column *avals, int32_t navals, const vgpu_column *bkey,
+                        const vgpu_column *bvals, int32_t nbvals, const vgpu_join_agg *aggs, int32_t naggs,
+                        void *results, void *stream);
+
+/* Materialized join as row-id pairs (uint32), ordered by key, then A row, then B row. *count receives the
+ * pair count; fails without writing when it exceeds `capacity` (call again with enough room). Replaces
+ * the lower/upper_bound + scan + ExpandMatchesKernel steps of gpu_hash_join.cu:196-254. Synchronizes. */
+int vgpu_join_pairs(const vgpu_column *akey, const vgpu_column *bkey, uint32_t *a_rows, uint32_t *b_rows,
+                    uint64_t capacity, uint64_t *count, void *stream);
+
 /* ---------------------------------------------------------------- primitives */
 
 /* Order-preserving copy of in[i] > threshold into out; *count_host receives the kept count (synchronizes).

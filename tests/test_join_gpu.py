@@ -1,0 +1,117 @@
+"""Sort-merge equi-join parity (aggregate-only and materialized) through the C ABI, against the oracle's
+join and the closed form for permutation build keys. Integer results bit-exact; double sums within 1e-12."""
+import numpy as np
+import pytest
+
+from vectisdb_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def config2_data(nb, np_, seed=0):
+    rng = np.random.default_rng(seed)
+    build = rng.permutation(nb).astype(np.int64)
+    payload = rng.integers(0, 1000, nb).astype(np.int32)
+    hit = rng.random(np_) < 0.9
+    fk = np.where(hit, rng.integers(0, nb, np_), rng.integers(nb, 2 * nb, np_)).astype(np.int64)
+    return build, payload, fk
+
+
+@pytest.mark.parametrize("nb,np_", [(1, 1), (100, 1000), (1 << 16, 1 << 20), (1 << 20, 1 << 23)])
+def test_join_sum_count_config2_shape(dev, nb, np_):
+    import oracle
+    build, payload, fk = config2_data(nb, np_, seed=nb)
+    # A = probe (fk), B = build (key, payload): SELECT SUM(payload), COUNT(*)
+    res = capi.join_aggregate(_dev(fk, dev), [], _dev(build, dev), [_dev(payload, dev)],
+                              [(capi.SUM, 1, 0, capi.HUGEINT), (capi.COUNT_STAR, -1, -1, capi.INT64)])
+    s, c = oracle.join_sum_count(build, payload, fk)
+    assert res == [s, c]
+    # closed form: build keys are a permutation of [0, nb)
+    inv = np.empty(nb, np.int64)
+    inv[build] = np.arange(nb)
+    m = fk < nb
+    assert c == int(m.sum())
+    assert s == int(payload[inv[fk[m]]].astype(np.int64).sum())
+
+
+def test_join_aggregate_duplicates_both_sides_and_probe_side_sums(dev):
+    rng = np.random.default_rng(1)
+    a = rng.integers(-50, 50, 20_000).astype(np.int64)
+    b = rng.integers(-40, 60, 15_000).astype(np.int64)
+    av = rng.integers(-10**6, 10**6, len(a)).astype(np.int64)
+    bv = rng.standard_normal(len(b))
+    res = capi.join_aggregate(_dev(a, dev), [_dev(av, dev)], _dev(b, dev), [_dev(bv, dev)],
+                              [(capi.COUNT_STAR, -1, -1, capi.INT64), (capi.SUM, 0, 0, capi.HUGEINT),
+                               (capi.SUM, 1, 0, capi.FLOAT64), (capi.MIN, 0, 0, capi.INT64),
+                               (capi.MAX, 1, 0, capi.FLOAT64), (capi.AVG, 0, 0, capi.FLOAT64)])
+    # brute force through per-key counts
+    ka, ca = np.unique(a, return_counts=True)
+    kb, cb = np.unique(b, return_counts=True)
+    common = np.intersect1d(ka, kb)
+    cnt = 0
+    s_av = 0
+    s_bv = []
+    mn = None
+    mx = None
+    for k in common.tolist():
+        na_ = int((a == k).sum())
+        nb_ = int((b == k).sum())
+        cnt += na_ * nb_
+        s_av += int(av[a == k].sum()) * nb_
+        s_bv += list(bv[b == k]) * na_
+        mn = min(mn, int(av[a == k].min())) if mn is not None else int(av[a == k].min())
+        mx = max(mx, float(bv[b == k].max())) if mx is not None else float(bv[b == k].max())
+    import math
+    exact_bv = math.fsum(s_bv)
+    assert res[0] == cnt
+    assert res[1] == s_av
+    assert abs(res[2] - exact_bv) <= 1e-12 * abs(exact_bv)
+    assert res[3] == mn and res[4] == mx
+    assert abs(res[5] - s_av / cnt) <= 1e-12 * abs(s_av / cnt)
+
+
+def test_join_aggregate_empty_and_disjoint(dev):
+    a = np.arange(10, dtype=np.int64)
+    b = np.arange(100, 110, dtype=np.int64)
+    res = capi.join_aggregate(_dev(a, dev), [], _dev(b, dev), [], [(capi.COUNT_STAR, -1, -1, capi.INT64)])
+    assert res == [0]
+
+
+@pytest.mark.parametrize("seed,na,nb,hi", [(0, 1, 1, 1), (1, 1000, 800, 50), (2, 50_000, 40_000, 10_000),
+                                           (3, 200_000, 3, 4)])
+def test_join_pairs_match_oracle(dev, seed, na, nb, hi):
+    import oracle
+    rng = np.random.default_rng(seed)
+    a = rng.integers(-hi, hi, na).astype(np.int64)
+    b = rng.integers(-hi, hi, nb).astype(np.int64)
+    ga, gb = capi.join_pairs(_dev(a, dev), _dev(b, dev))
+    el, er = oracle.join_pairs(a, b)
+    got = sorted(zip(ga.cpu().numpy().tolist(), gb.cpu().numpy().tolist()))
+    assert got == sorted(zip(el.tolist(), er.tolist()))
+    # output is ordered by key, then A row, then B row
+    ka = a[ga.cpu().numpy()]
+    assert np.all(np.diff(ka) >= 0)
+
+
+def test_join_pairs_skewed_key(dev):
+    """One key with 3,000 x 2,000 = 6M pairs is expanded across the device, not by one thread."""
+    a = np.concatenate([np.full(3000, 7, np.int64), np.arange(100, 200, dtype=np.int64)])
+    b = np.concatenate([np.full(2000, 7, np.int64), np.arange(150, 160, dtype=np.int64)])
+    ga, gb = capi.join_pairs(_dev(a, dev), _dev(b, dev))
+    assert len(ga) == 3000 * 2000 + 10
+    assert np.all(a[ga.cpu().numpy()] == b[gb.cpu().numpy()])
+
+
+def test_join_key_type_rules(dev):
+    a = np.arange(10, dtype=np.int64)
+    b = np.arange(10, dtype=np.int32)
+    f = np.arange(10, dtype=np.float64)
+    with pytest.raises(capi.VgpuError, match="same type"):
+        capi.join_pairs(_dev(a, dev), _dev(b, dev))
+    with pytest.raises(capi.VgpuError, match="integer/boolean"):
+        capi.join_pairs(_dev(f, dev), _dev(f, dev))

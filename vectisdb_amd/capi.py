@@ -29,6 +29,10 @@ class Agg(C.Structure):
     _fields_ = [("kind", C.c_int32), ("input", C.c_int32), ("output_type", C.c_int32)]
 
 
+class JoinAgg(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("side", C.c_int32), ("input", C.c_int32), ("output_type", C.c_int32)]
+
+
 class Expr(C.Structure):
     _fields_ = [("body", C.c_char_p), ("inputs", C.POINTER(C.c_int32)), ("ninputs", C.c_int32),
                 ("output_type", C.c_int32)]
@@ -54,6 +58,11 @@ _PROTOS = {
     "vgpu_cross_filter": (C.c_int, [C.POINTER(Column), C.c_int32, C.c_int32, C.c_uint64, C.c_uint64,
                                     C.POINTER(Expr), C.c_int32, C.POINTER(Expr), C.c_int32, C.c_void_p, C.c_uint64,
                                     C.POINTER(C.c_uint64), C.c_void_p]),
+    "vgpu_join_aggregate": (C.c_int, [C.POINTER(Column), C.POINTER(Column), C.c_int32, C.POINTER(Column),
+                                      C.POINTER(Column), C.c_int32, C.POINTER(JoinAgg), C.c_int32, C.c_void_p,
+                                      C.c_void_p]),
+    "vgpu_join_pairs": (C.c_int, [C.POINTER(Column), C.POINTER(Column), C.c_void_p, C.c_void_p, C.c_uint64,
+                                  C.POINTER(C.c_uint64), C.c_void_p]),
     "vgpu_profile_enable": (C.c_int, [C.c_int32]),
     "vgpu_profile_reset": (C.c_int, []),
     "vgpu_profile_read": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -252,6 +261,54 @@ def cross_filter(left, right, preds, projs, capacity=None, stream=None):
                                 C.byref(kept), _stream_ptr(stream)))
     k = kept.value
     return [o[:k] for o in outs], k
+
+
+def join_aggregate(akey, avals, bkey, bvals, aggs, stream=None):
+    """Global aggregates over akey = bkey. aggs: list of (kind, side, input, output_type).
+    Returns Python values (int for COUNT/SUM(int)/MIN/MAX ints, float otherwise)."""
+    import struct
+    lib = load()
+    ak = column(akey)
+    bk = column(bkey)
+    av = (Column * max(len(avals), 1))(*[column(v) for v in avals])
+    bv = (Column * max(len(bvals), 1))(*[column(v) for v in bvals])
+    ja = (JoinAgg * max(len(aggs), 1))(*[JoinAgg(*a) for a in aggs])
+    res = C.create_string_buffer(16 * max(len(aggs), 1))
+    check(lib.vgpu_join_aggregate(C.byref(ak), av, len(avals), C.byref(bk), bv, len(bvals), ja, len(aggs), res,
+                                  _stream_ptr(stream)))
+    out = []
+    raw = res.raw
+    for i, (kind, side, inp, otype) in enumerate(aggs):
+        chunk = raw[16 * i: 16 * i + 16]
+        if otype == HUGEINT:
+            lo, hi = struct.unpack("<Qq", chunk)
+            out.append((hi << 64) + lo)
+        elif otype in (FLOAT64, FLOAT32):
+            out.append(struct.unpack("<d", chunk[:8])[0])
+        else:
+            out.append(struct.unpack("<q", chunk[:8])[0])
+    return out
+
+
+def join_pairs(akey, bkey, capacity=None, stream=None):
+    """(a_rows, b_rows) int32 device tensors of the INNER equi-join."""
+    import torch
+    lib = load()
+    ak = column(akey)
+    bk = column(bkey)
+    count = C.c_uint64(0)
+    if capacity is None:
+        st = lib.vgpu_join_pairs(C.byref(ak), C.byref(bk), None, None, 0, C.byref(count), _stream_ptr(stream))
+        if st == 0:
+            capacity = 0
+        else:
+            capacity = count.value
+    a = torch.empty(max(capacity, 1), dtype=torch.int32, device=akey.device)
+    b = torch.empty(max(capacity, 1), dtype=torch.int32, device=akey.device)
+    check(lib.vgpu_join_pairs(C.byref(ak), C.byref(bk), C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()),
+                              capacity, C.byref(count), _stream_ptr(stream)))
+    n = count.value
+    return a[:n], b[:n]
 
 
 def profile_enable(on=True):

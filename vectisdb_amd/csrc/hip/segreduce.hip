@@ -380,9 +380,9 @@ __global__ void __launch_bounds__(kRedBlock)
 		if (H > 0) {
 			u64 gid_last = hb_global + H - 1;
 			bool ends = tile_end >= n;
-			if (!ends) {
-				KeyT next = keys != nullptr ? keys[tile_end] : (KeyT)0;
-				ends = next != s_keys[kRedTile - 1];
+			if (!ends && keys != nullptr) {
+				// keys == nullptr: one global group, which only ends with the input
+				ends = keys[tile_end] != s_keys[kRedTile - 1];
 			}
 			if (ends) {
 				WriteGroup(o, X, gid_last);

@@ -1,0 +1,209 @@
+// join_op.cpp -- host side of the sort-merge INNER equi-join.
+//
+// Semantics follow cuda_engine/src/operators/gpu_hash_join.cu:138-267: INNER, exactly one equality key,
+// both key columns the same integer/boolean type, output columns selected by name, at most 2^32 output
+// rows. The algorithm is sort-merge end to end (both sides radix-sorted, then merged), see join.hip.
+#include "../hip/join_kernels.h"
+#include "operators.h"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace vgpu {
+
+namespace {
+
+void ValidateJoinKeys(const DevCol &a, const DevCol &b) {
+	if (!TypeIsInteger(a.type) || !TypeIsInteger(b.type)) {
+		throw std::runtime_error("gpu_hash_join: join key is not an integer/boolean column, which this GPU path "
+		                         "requires");
+	}
+	if (a.type != b.type) {
+		throw std::runtime_error("gpu_hash_join: join key columns must have the same type on both sides");
+	}
+}
+
+//! One side reduced to its distinct keys: keys (original type, ascending), per-key row counts, optional
+//! per-key aggregate columns, and (materialized form) the stable row permutation and run starts.
+struct SideRuns {
+	uint64_t groups = 0;
+	void *keys = nullptr;
+	long long *counts = nullptr;
+	std::vector<void *> aggs;
+	uint32_t *perm = nullptr;
+	uint64_t *starts = nullptr;
+};
+
+SideRuns ReduceSide(const DevCol &key, const std::vector<DevCol> &values, const std::vector<AggReq> &aggs,
+                    bool with_perm, Scratch &scratch, hipStream_t stream) {
+	SideRuns s;
+	const uint64_t rows = key.rows;
+	KeyPack pack;
+	const uint32_t *hist8 = nullptr;
+	uint64_t bound = PlanGroupBy({key}, rows, pack, scratch, stream, &hist8);
+	SortTuning tuning = TuneSort(pack.total_bits, rows);
+	tuning.hist8 = hist8;
+	if (!with_perm) {
+		std::vector<AggReq> all = {{AGG_COUNT_STAR, -1, VT_INT64}};
+		all.insert(all.end(), aggs.begin(), aggs.end());
+		std::vector<void *> ko, ao;
+		OutAlloc alloc = [&](size_t bytes) { return scratch.Alloc(bytes); };
+		s.groups = GroupByDevice({key}, values, all, rows, pack, bound, alloc, ko, ao, tuning, scratch, stream, true);
+		s.keys = ko[0];
+		s.counts = static_cast<long long *>(ao[0]);
+		s.aggs.assign(ao.begin() + 1, ao.end());
+		return s;
+	}
+	// Materialized form: sort with a row-id payload, then reduce to (key, count).
+	SortedRun run;
+	const void *pin[1] = {nullptr};
+	int pw[1] = {4};
+	SortPacked(pack, pin, pw, 1, rows, tuning, run, scratch, stream);
+	s.perm = static_cast<uint32_t *>(run.payload[0]);
+	const uint64_t cap = std::max<uint64_t>(bound, 1);
+	ReduceSpec spec;
+	spec.ncomp = 1;
+	spec.comp[0].kind = COMP_COUNT;
+	spec.naggs = 1;
+	spec.aggs[0].kind = AGG_COUNT_STAR;
+	spec.aggs[0].comp = 0;
+	spec.aggs[0].out_type = VT_INT64;
+	spec.aggs[0].out = scratch.Alloc(cap * 8);
+	spec.pack = pack;
+	spec.key_out[0] = scratch.Alloc(cap * TypeWidth(key.type));
+	// the reduce reads no payload: hide the permutation from it
+	SortedRun keys_only = run;
+	keys_only.npayload = 0;
+	auto *groups_dev = static_cast<uint64_t *>(scratch.Alloc(8));
+	SegmentedReduce(keys_only, rows, spec, groups_dev, scratch, stream);
+	VGPU_HIP(hipMemcpyAsync(&s.groups, groups_dev, 8, hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	s.keys = spec.key_out[0];
+	s.counts = static_cast<long long *>(spec.aggs[0].out);
+	s.starts = static_cast<uint64_t *>(scratch.Alloc(std::max<uint64_t>(s.groups, 1) * 8));
+	ExclusiveScanU64(reinterpret_cast<const uint64_t *>(s.counts), s.starts, s.groups, nullptr, scratch, stream);
+	return s;
+}
+
+} // namespace
+
+std::vector<JoinAggValue> JoinAggregateDevice(const DevCol &akey, const std::vector<DevCol> &avals, const DevCol &bkey,
+                                              const std::vector<DevCol> &bvals, const std::vector<JoinAggReq> &aggs,
+                                              Scratch &scratch, hipStream_t stream) {
+	ValidateJoinKeys(akey, bkey);
+	if (aggs.size() > (size_t)kJoinAggMax) {
+		throw std::runtime_error("gpu_hash_join: at most 8 aggregates over a join");
+	}
+	// Per-side aggregate requests.
+	std::vector<AggReq> side_aggs[2];
+	std::vector<int> side_slot(aggs.size(), -1);
+	JoinAggArgs args;
+	args.naggs = (int)aggs.size();
+	for (size_t q = 0; q < aggs.size(); q++) {
+		const auto &a = aggs[q];
+		const std::vector<DevCol> &vals = a.side == 0 ? avals : bvals;
+		if (a.kind == AGG_COUNT_STAR || a.kind == AGG_COUNT) {
+			args.kind[q] = JA_COUNT;
+			continue;
+		}
+		if (a.side != 0 && a.side != 1) {
+			throw std::runtime_error("gpu_hash_join: aggregate side must be 0 or 1");
+		}
+		if (a.input < 0 || a.input >= (int)vals.size()) {
+			throw std::runtime_error("gpu_hash_join: aggregate input out of range");
+		}
+		const int in = vals[a.input].type;
+		const bool flt = in == VT_FLOAT32 || in == VT_FLOAT64;
+		AggReq r;
+		r.input = a.input;
+		args.side[q] = a.side;
+		switch (a.kind) {
+		case AGG_SUM:
+		case AGG_AVG:
+			r.kind = AGG_SUM;
+			r.out_type = flt ? VT_FLOAT64 : VT_HUGEINT;
+			args.kind[q] = flt ? JA_SUM_F64 : JA_SUM_I128;
+			break;
+		case AGG_MIN:
+		case AGG_MAX:
+			if (!(in == VT_INT64 || in == VT_FLOAT64)) {
+				throw std::runtime_error("gpu_hash_join: MIN/MAX over a join supports BIGINT and DOUBLE inputs");
+			}
+			r.kind = a.kind;
+			r.out_type = in;
+			args.kind[q] = flt ? (a.kind == AGG_MIN ? JA_MIN_F64 : JA_MAX_F64)
+			                   : (a.kind == AGG_MIN ? JA_MIN_I64 : JA_MAX_I64);
+			break;
+		default:
+			throw std::runtime_error("gpu_hash_join: unsupported aggregate over a join");
+		}
+		side_slot[q] = (int)side_aggs[a.side].size();
+		side_aggs[a.side].push_back(r);
+	}
+	std::vector<JoinAggValue> result(aggs.size());
+	if (akey.rows == 0 || bkey.rows == 0) {
+		return result; // zero rows: COUNT 0, SUM 0 (the caller applies SQL NULL rules for empty input)
+	}
+	SideRuns A = ReduceSide(akey, avals, side_aggs[0], false, scratch, stream);
+	SideRuns B = ReduceSide(bkey, bvals, side_aggs[1], false, scratch, stream);
+	auto *match = static_cast<long long *>(scratch.Alloc(std::max<uint64_t>(A.groups, 1) * 8));
+	MergeMatch(A.keys, A.groups, B.keys, B.groups, akey.type, match, stream);
+	args.na = A.groups;
+	args.match = match;
+	args.ca = A.counts;
+	args.cb = B.counts;
+	for (size_t q = 0; q < aggs.size(); q++) {
+		if (side_slot[q] >= 0) {
+			args.per_key[q] = (aggs[q].side == 0 ? A : B).aggs[side_slot[q]];
+		}
+	}
+	auto *res_dev = static_cast<uint64_t *>(scratch.Alloc(kJoinAggMax * 2 * 8));
+	JoinAggregate(args, res_dev, scratch, stream);
+	uint64_t host[kJoinAggMax * 2];
+	VGPU_HIP(hipMemcpyAsync(host, res_dev, sizeof(host), hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	for (size_t q = 0; q < aggs.size(); q++) {
+		result[q].lo = host[2 * q];
+		result[q].hi = host[2 * q + 1];
+		result[q].kind = args.kind[q];
+	}
+	return result;
+}
+
+uint64_t JoinMaterializeDevice(const DevCol &akey, const DevCol &bkey, uint32_t **left_rows, uint32_t **right_rows,
+                               Scratch &scratch, hipStream_t stream) {
+	ValidateJoinKeys(akey, bkey);
+	*left_rows = nullptr;
+	*right_rows = nullptr;
+	if (akey.rows == 0 || bkey.rows == 0) {
+		return 0;
+	}
+	SideRuns A = ReduceSide(akey, {}, {}, true, scratch, stream);
+	SideRuns B = ReduceSide(bkey, {}, {}, true, scratch, stream);
+	auto *match = static_cast<long long *>(scratch.Alloc(std::max<uint64_t>(A.groups, 1) * 8));
+	MergeMatch(A.keys, A.groups, B.keys, B.groups, akey.type, match, stream);
+	auto *pairs = static_cast<uint64_t *>(scratch.Alloc(std::max<uint64_t>(A.groups, 1) * 8));
+	PairCounts(match, A.counts, B.counts, A.groups, pairs, stream);
+	auto *offsets = static_cast<uint64_t *>(scratch.Alloc(std::max<uint64_t>(A.groups, 1) * 8));
+	auto *total_dev = static_cast<uint64_t *>(scratch.Alloc(8));
+	ExclusiveScanU64(pairs, offsets, A.groups, total_dev, scratch, stream);
+	uint64_t total = 0;
+	VGPU_HIP(hipMemcpyAsync(&total, total_dev, 8, hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	constexpr uint64_t kMaxJoinOutputRows = 1ull << 32;
+	if (total > kMaxJoinOutputRows) {
+		throw std::runtime_error("gpu_hash_join: join result of " + std::to_string(total) +
+		                         " rows is too large for GPU execution");
+	}
+	if (total == 0) {
+		return 0;
+	}
+	*left_rows = static_cast<uint32_t *>(scratch.Alloc(total * 4));
+	*right_rows = static_cast<uint32_t *>(scratch.Alloc(total * 4));
+	ExpandPairs(offsets, A.groups, match, B.counts, reinterpret_cast<const long long *>(A.starts),
+	            reinterpret_cast<const long long *>(B.starts), A.perm, B.perm, total, *left_rows, *right_rows, stream);
+	return total;
+}
+
+} // namespace vgpu

@@ -48,6 +48,34 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
                        const SortTuning &tuning, Scratch &scratch, hipStream_t stream, bool read_groups,
                        uint64_t *groups_device = nullptr);
 
+// ------------------------------------------------------------------ join
+
+struct JoinAggReq {
+	int kind = AGG_COUNT_STAR; // AggKind
+	int side = -1;             // 0 = A (children[0]), 1 = B (children[1]); -1 for COUNT(*)
+	int input = -1;            // index into that side's value columns
+	int out_type = VT_INT64;
+};
+
+//! One aggregate over the join, as its raw two-word state: COUNT -> (lo, hi) uint128; SUM int -> int128;
+//! SUM float -> double-double (hi word = bits of the high double); MIN/MAX -> lo.
+struct JoinAggValue {
+	uint64_t lo = 0;
+	uint64_t hi = 0;
+	int kind = 0; // JoinAggKind
+};
+
+//! Aggregate-only INNER equi-join (global aggregate over the join, nothing materialized). Synchronizes.
+std::vector<JoinAggValue> JoinAggregateDevice(const DevCol &akey, const std::vector<DevCol> &avals, const DevCol &bkey,
+                                              const std::vector<DevCol> &bvals, const std::vector<JoinAggReq> &aggs,
+                                              Scratch &scratch, hipStream_t stream);
+
+//! Materialized INNER equi-join: returns the number of output rows and, in *left_rows / *right_rows (from
+//! `scratch`), the source row of each side per output row, ordered by key, then left row, then right row.
+//! Synchronizes. Throws past 2^32 output rows.
+uint64_t JoinMaterializeDevice(const DevCol &akey, const DevCol &bkey, uint32_t **left_rows, uint32_t **right_rows,
+                               Scratch &scratch, hipStream_t stream);
+
 //! Autotuner: radix digit width per key width / row count, from measured sweeps (see tuning.cpp).
 SortTuning TuneSort(int total_bits, uint64_t rows);
 
