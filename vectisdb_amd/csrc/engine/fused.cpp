@@ -4,6 +4,7 @@
 #include "../../../include/kernel_cache.hpp"
 #include "../hip/kernels.h"
 
+#include <functional>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -146,6 +147,8 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 	o << kPreamble;
 	o << "extern \"C\" __global__ void __launch_bounds__(" << kBlock << ") vectis_fused(VArgs a) {\n";
 	o << "  const int tid = threadIdx.x; const int lane = tid & 63; const int wave = tid >> 6; (void)lane;\n";
+	// Output staging buffer: each output (and the selection) leaves the block as one contiguous run.
+	o << "  __shared__ vg_u64 s_stage[" << kTile << "];\n";
 	if (filter) {
 		o << "  __shared__ unsigned int s_tile; __shared__ vg_u64 s_wtot[4]; __shared__ vg_u64 s_prefix;\n";
 		o << "  if (tid == 0) s_tile = __hip_atomic_fetch_add((unsigned int*)(a.state + a.ntiles), 1u, "
@@ -168,17 +171,28 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 	o << "  vg_u64 __i0[" << kRounds << "];\n";
 	o << "  vg_u64 __rowid[" << kRounds << "][" << kVec << "];\n";
 	o << "  unsigned int __keep[" << kRounds << "];\n";
+	if (s.cross) {
+		// Fast path: the right side is a multiple of the tile, so a tile is one left row x contiguous right
+		// rows -- one division per block, vector loads on the right, a broadcast on the left.
+		o << "  const bool __aligned_cross = (a.m % " << kTile << "ull) == 0;\n";
+		o << "  const vg_u64 __tli = __aligned_cross ? tile_base / a.m : 0;\n";
+		o << "  const vg_u64 __trj = __aligned_cross ? tile_base - __tli * a.m : 0;\n";
+	}
 	o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) {\n";
 	o << "    const vg_u64 i0 = tile_base + (vg_u64)wave * " << (kVec * kRounds * 64) << "ull + (vg_u64)__r * "
 	  << (kVec * 64) << "ull + (vg_u64)lane * " << kVec << "ull;\n";
 	o << "    __i0[__r] = i0;\n";
-	// Row numbers (and cross-product coordinates) of this lane's 4 entries.
+	o << "    const bool __full = i0 + " << (kVec - 1) << " < a.n;\n";
 	if (s.cross) {
-		o << "    vg_u64 __li = i0 / a.m; vg_u64 __rj = i0 - __li * a.m;\n";
 		o << "    vg_u64 __lrow[" << kVec << "], __rrow[" << kVec << "];\n";
-		o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) {\n";
-		o << "      __lrow[__j] = __li; __rrow[__j] = __rj; __rowid[__r][__j] = i0 + __j;\n";
-		o << "      __rj++; while (__rj >= a.m) { __rj -= a.m; __li++; }\n";
+		o << "    if (__aligned_cross) {\n";
+		o << "#pragma unroll\n      for (int __j = 0; __j < " << kVec << "; __j++) { __lrow[__j] = __tli; __rrow[__j] = __trj + (i0 - tile_base) + __j; __rowid[__r][__j] = i0 + __j; }\n";
+		o << "    } else {\n";
+		o << "      vg_u64 __li = i0 / a.m; vg_u64 __rj = i0 - __li * a.m;\n";
+		o << "#pragma unroll\n      for (int __j = 0; __j < " << kVec << "; __j++) {\n";
+		o << "        __lrow[__j] = __li; __rrow[__j] = __rj; __rowid[__r][__j] = i0 + __j;\n";
+		o << "        __rj++; while (__rj >= a.m) { __rj -= a.m; __li++; }\n";
+		o << "      }\n";
 		o << "    }\n";
 	} else if (s.selection_in) {
 		o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) {\n";
@@ -187,28 +201,41 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 	} else {
 		o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) __rowid[__r][__j] = i0 + __j;\n";
 	}
-	o << "    const bool __full = i0 + " << (kVec - 1) << " < a.n;\n";
 	// Loads: 128-bit (or 64/32-bit for narrow types) when the lane's 4 rows are contiguous and aligned.
+	auto emit_vector_load = [&](const std::string &cn, const std::string &vn, const char *t, const std::string &first,
+	                            const std::string &indent) {
+		o << indent << "if (__full && ((vg_u64)(" << cn << " + " << first << ") & " << kVec << " * sizeof(" << t
+		  << ") - 1) == 0) {\n";
+		o << indent << "  const void* __ap = __builtin_assume_aligned((const void*)(" << cn << " + " << first << "), "
+		  << kVec << " * sizeof(" << t << "));\n";
+		o << indent << "  __builtin_memcpy(&" << vn << "[0], __ap, " << kVec << " * sizeof(" << t << "));\n";
+		o << indent << "} else {\n";
+		o << indent << "#pragma unroll\n";
+		o << indent << "  for (int __j = 0; __j < " << kVec << "; __j++) " << vn << "[__j] = (i0 + __j < a.n) ? " << cn
+		  << "[" << first << " + __j] : (" << t << ")0;\n";
+		o << indent << "}\n";
+	};
 	for (size_t c = 0; c < s.cols.size(); c++) {
 		const char *t = CType(s.cols[c].vtype);
 		std::string cn = "__c" + std::to_string(c);
 		std::string vn = "__v" + std::to_string(c) + "[__r]";
 		if (s.cross) {
-			const char *rr = s.cols[c].side == 0 ? "__lrow" : "__rrow";
-			o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) " << vn << "[__j] = (i0 + __j < a.n) ? "
-			  << cn << "[" << rr << "[__j]] : (" << t << ")0;\n";
+			if (s.cols[c].side == 0) {
+				o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) " << vn
+				  << "[__j] = (i0 + __j < a.n) ? " << cn << "[__lrow[__j]] : (" << t << ")0;\n";
+			} else {
+				o << "    if (__aligned_cross) {\n";
+				emit_vector_load(cn, vn, t, "__rrow[0]", "      ");
+				o << "    } else {\n";
+				o << "#pragma unroll\n      for (int __j = 0; __j < " << kVec << "; __j++) " << vn
+				  << "[__j] = (i0 + __j < a.n) ? " << cn << "[__rrow[__j]] : (" << t << ")0;\n";
+				o << "    }\n";
+			}
 		} else if (s.selection_in) {
 			o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) " << vn << "[__j] = (i0 + __j < a.n) ? "
 			  << cn << "[__rowid[__r][__j]] : (" << t << ")0;\n";
 		} else {
-			o << "    if (__full && ((vg_u64)(" << cn << " + i0) & " << (kVec * 1) << " * sizeof(" << t << ") - 1) == 0) {\n";
-			o << "      const void* __ap = __builtin_assume_aligned((const void*)(" << cn << " + i0), " << kVec
-			  << " * sizeof(" << t << "));\n";
-			o << "      __builtin_memcpy(&" << vn << "[0], __ap, " << kVec << " * sizeof(" << t << "));\n";
-			o << "    } else {\n";
-			o << "#pragma unroll\n      for (int __j = 0; __j < " << kVec << "; __j++) " << vn << "[__j] = (i0 + __j < a.n) ? "
-			  << cn << "[i0 + __j] : (" << t << ")0;\n";
-			o << "    }\n";
+			emit_vector_load(cn, vn, t, "i0", "    ");
 		}
 		if (s.cols[c].has_valid) {
 			std::string mn = "__m" + std::to_string(c) + "[__r]";
@@ -224,14 +251,10 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 	for (size_t p = 0; p < s.predicates.size(); p++) {
 		auto &e = s.predicates[p];
 		o << "      bool __p" << p << " = false;\n";
-		for (int r = 0; r < 1; r++) {
-			(void)r;
-		}
 		o << "      if (__ok) {\n";
 		std::ostringstream body;
 		EmitExprScope(body, s, e, -1, "(void*)&__p" + std::to_string(p), "0", true);
 		std::string text = body.str();
-		// The register tile index is the loop variable __r here.
 		size_t pos;
 		while ((pos = text.find("[-1]")) != std::string::npos) {
 			text.replace(pos, 4, "[__r]");
@@ -249,22 +272,24 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 	o << "    __keep[__r] = __k;\n";
 	o << "  }\n";
 
-	// Output positions.
+	// Tile-local output slot of every kept entry (__lpos), and the tile's global base (__gbase).
+	o << "  unsigned int __lpos0[" << kRounds << "];\n";
 	if (filter) {
-		o << "  unsigned int __cnt[" << kRounds << "], __excl[" << kRounds << "], __rtot[" << kRounds << "];\n";
+		o << "  unsigned int __rtot[" << kRounds << "];\n";
+		o << "  unsigned int __wrun = 0;\n";
 		o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) {\n";
-		o << "    __cnt[__r] = __popc(__keep[__r]);\n";
-		o << "    unsigned int __inc = vg_wave_incl(__cnt[__r]);\n";
-		o << "    __excl[__r] = __inc - __cnt[__r];\n";
+		o << "    unsigned int __cnt = __popc(__keep[__r]);\n";
+		o << "    unsigned int __inc = vg_wave_incl(__cnt);\n";
+		o << "    __lpos0[__r] = __wrun + __inc - __cnt;\n";
 		o << "    __rtot[__r] = __shfl(__inc, 63);\n";
+		o << "    __wrun += __rtot[__r];\n";
 		o << "  }\n";
-		o << "  vg_u64 __wtot = 0;\n";
-		o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) __wtot += __rtot[__r];\n";
-		o << "  if (lane == 0) s_wtot[wave] = __wtot;\n";
+		o << "  if (lane == 0) s_wtot[wave] = __wrun;\n";
 		o << "  __syncthreads();\n";
-		o << "  vg_u64 __wexcl = 0, __ttot = 0;\n";
-		o << "  for (int w = 0; w < 4; w++) { if (w < wave) __wexcl += s_wtot[w]; __ttot += s_wtot[w]; }\n";
-		o << "  if (tid == 0) vg_store(a.state + tile, ((tile == 0 ? 2ull : 1ull) << 62) | __ttot);\n";
+		o << "  unsigned int __wexcl = 0, __ttot = 0;\n";
+		o << "  for (int w = 0; w < 4; w++) { if (w < wave) __wexcl += (unsigned int)s_wtot[w]; __ttot += (unsigned int)s_wtot[w]; }\n";
+		o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) __lpos0[__r] += __wexcl;\n";
+		o << "  if (tid == 0) vg_store(a.state + tile, ((tile == 0 ? 2ull : 1ull) << 62) | (vg_u64)__ttot);\n";
 		o << "  if (wave == 0) {\n";
 		o << "    vg_u64 __pre = tile == 0 ? 0ull : vg_lookback(a.state, (long long)tile);\n";
 		o << "    if (lane == 0) {\n";
@@ -274,48 +299,66 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 		o << "    }\n";
 		o << "  }\n";
 		o << "  __syncthreads();\n";
-		o << "  vg_u64 __base = s_prefix + __wexcl;\n";
-	}
-	// Writes.
-	o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) {\n";
-	if (filter) {
-		o << "    vg_u64 __pos = __base + __excl[__r];\n";
-		o << "    for (int __q = 0; __q < __r; __q++) __pos += __rtot[__q];\n";
-	}
-	o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) {\n";
-	if (filter) {
-		o << "      if (((__keep[__r] >> __j) & 1u) && __pos < a.cap) {\n";
-		o << "        const vg_u64 __o = __pos++;\n";
+		o << "  const vg_u64 __gbase = s_prefix;\n";
 	} else {
-		o << "      if (__i0[__r] + __j < a.n) {\n";
-		o << "        const vg_u64 __o = __i0[__r] + __j;\n";
+		o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) __lpos0[__r] = (unsigned int)(__i0[__r] - tile_base);\n";
+		o << "  const unsigned int __ttot = (unsigned int)((a.n - tile_base) < " << kTile << "ull ? (a.n - tile_base) : "
+		  << kTile << "ull);\n";
+		o << "  const vg_u64 __gbase = tile_base;\n";
 	}
+	// Writes: every output goes through the LDS stage, then out as one coalesced run of __ttot entries.
+	auto emit_stage_pass = [&](const std::string &stage_type, const std::string &write_stmt_prefix,
+	                           const std::string &out_expr, const std::function<void()> &emit_value) {
+		o << "  {\n";
+		o << "    " << stage_type << "* __st = (" << stage_type << "*)s_stage;\n";
+		o << "#pragma unroll\n    for (int __r = 0; __r < " << kRounds << "; __r++) {\n";
+		o << "      unsigned int __lp = __lpos0[__r];\n";
+		o << "#pragma unroll\n      for (int __j = 0; __j < " << kVec << "; __j++) {\n";
+		o << "        if ((__keep[__r] >> __j) & 1u) {\n";
+		o << "          const vg_u64 __o = __lp++;\n";
+		emit_value();
+		o << "        }\n";
+		o << "      }\n";
+		o << "    }\n";
+		o << "    __syncthreads();\n";
+		o << "    for (unsigned int __i = tid; __i < __ttot; __i += " << kBlock << ") {\n";
+		o << "      if (__gbase + __i < a.cap) " << write_stmt_prefix << out_expr << "[__gbase + __i] = __st[__i];\n";
+		o << "    }\n";
+		o << "    __syncthreads();\n";
+		o << "  }\n";
+		(void)write_stmt_prefix;
+	};
 	if (s.emit_selection) {
-		o << "        a.sel_out[__o] = (unsigned int)__rowid[__r][__j];\n";
+		emit_stage_pass("unsigned int", "", "a.sel_out", [&]() {
+			o << "          __st[__o] = (unsigned int)__rowid[__r][__j];\n";
+		});
 	}
 	for (size_t e = 0; e < s.projections.size(); e++) {
 		auto &ex = s.projections[e];
-		std::ostringstream body;
-		EmitExprScope(body, s, ex, -1, "a.outs[" + std::to_string(e) + "]", "__o", false);
-		std::string text = body.str();
-		size_t pos;
-		while ((pos = text.find("[-1]")) != std::string::npos) {
-			text.replace(pos, 4, "[__r]");
-		}
-		o << text;
+		const char *ot = CType(ex.out_type);
+		std::string out_cast = std::string("((") + ot + "*)a.outs[" + std::to_string(e) + "])";
+		emit_stage_pass(ot, "", out_cast, [&]() {
+			std::ostringstream body;
+			EmitExprScope(body, s, ex, -1, "(void*)__st", "__o", false);
+			std::string text = body.str();
+			size_t pos;
+			while ((pos = text.find("[-1]")) != std::string::npos) {
+				text.replace(pos, 4, "[__r]");
+			}
+			o << text;
+		});
 		std::string v = ValidExpr(s, ex, 0);
-		while ((pos = v.find("[0]")) != std::string::npos) {
-			v.replace(pos, 3, "[__r]");
-		}
 		if (!v.empty()) {
-			o << "        if (a.out_valid[" << e << "] != 0) a.out_valid[" << e << "][__o] = (" << v << ") ? 1 : 0;\n";
+			size_t pos;
+			while ((pos = v.find("[0]")) != std::string::npos) {
+				v.replace(pos, 3, "[__r]");
+			}
+			o << "  if (a.out_valid[" << e << "] != 0) {\n";
+			emit_stage_pass("unsigned char", "", "a.out_valid[" + std::to_string(e) + "]", [&]() {
+				o << "          __st[__o] = (" << v << ") ? 1 : 0;\n";
+			});
+			o << "  }\n";
 		}
-	}
-	o << "      }\n";
-	o << "    }\n";
-	o << "  }\n";
-	if (!filter) {
-		o << "  (void)__keep;\n";
 	}
 	o << "}\n";
 	return o.str();

@@ -52,6 +52,37 @@ __device__ __forceinline__ u64 PackRow(const KeyPack &p, uint64_t row) {
 	return packed;
 }
 
+//! Loads the wave-striped items (row = wave_base + k * 64 + lane) of one column with the width switch
+//! hoisted out of the item loop, so all ITEMS loads issue back to back.
+template <int ITEMS, typename T>
+__device__ __forceinline__ void LoadWaveItemsT(const T *src, uint64_t wave_base, int lane, uint64_t n,
+                                               u64 (&out)[ITEMS]) {
+#pragma unroll
+	for (int k = 0; k < ITEMS; k++) {
+		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
+		out[k] = row < n ? (u64)src[row] : 0ull;
+	}
+}
+
+template <int ITEMS>
+__device__ __forceinline__ void LoadWaveItems(const void *src, int width, uint64_t wave_base, int lane, uint64_t n,
+                                              u64 (&out)[ITEMS]) {
+	switch (width) {
+	case 8:
+		LoadWaveItemsT<ITEMS>((const u64 *)src, wave_base, lane, n, out);
+		break;
+	case 4:
+		LoadWaveItemsT<ITEMS>((const uint32_t *)src, wave_base, lane, n, out);
+		break;
+	case 2:
+		LoadWaveItemsT<ITEMS>((const uint16_t *)src, wave_base, lane, n, out);
+		break;
+	default:
+		LoadWaveItemsT<ITEMS>((const uint8_t *)src, wave_base, lane, n, out);
+		break;
+	}
+}
+
 struct RangeCols {
 	int ncols;
 	const void *col[kMaxKeyCols];
@@ -109,7 +140,8 @@ __global__ void __launch_bounds__(256) KeyRangeKernel(RangeCols cols, uint64_t n
 //! Min/max of the ordered key and the histograms of all 8 byte digits in one read. A digit that is the
 //! same on every active lane (the constant high bytes of a narrow key range, or a Zipf head value) is
 //! counted with one LDS atomic per wave instead of 64 conflicting ones.
-__global__ void __launch_bounds__(256) KeyRangeHist8Kernel(const void *col, int width, int is_signed, uint64_t n,
+template <typename T>
+__global__ void __launch_bounds__(256) KeyRangeHist8Kernel(const T *col, int width, int is_signed, uint64_t n,
                                                            u64 *mins, u64 *maxs, uint32_t *hist8) {
 	__shared__ uint32_t s_hist[8 * 256];
 	for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) {
@@ -126,7 +158,7 @@ __global__ void __launch_bounds__(256) KeyRangeHist8Kernel(const void *col, int 
 #pragma unroll
 		for (int u = 0; u < 4; u++) {
 			uint64_t row = row0 + u * stride;
-			raw[u] = row < n ? LoadRawBits(col, row, width) : 0ull;
+			raw[u] = row < n ? (u64)col[row] : 0ull;
 		}
 #pragma unroll
 		for (int u = 0; u < 4; u++) {
@@ -145,7 +177,10 @@ __global__ void __launch_bounds__(256) KeyRangeHist8Kernel(const void *col, int 
 #pragma unroll
 			for (int d = 0; d < 8; d++) {
 				uint32_t dg = (uint32_t)(ord >> (8 * d)) & 255u;
-				uint32_t ref = __shfl(dg, leader);
+				// every lane is active here, so readfirstlane reads lane 0; if lane 0 is past the end the
+				// uniform test below simply fails and the lanes count themselves
+				uint32_t ref = (uint32_t)__builtin_amdgcn_readfirstlane((int)dg);
+				(void)leader;
 				u64 same = __ballot(valid && dg == ref);
 				if (same == vmask) {
 					if (lane == leader) {
@@ -254,17 +289,28 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 	const uint32_t digit_mask = (1u << digit_bits) - 1;
 
 	KeyT key[ITEMS];
+	if constexpr (FIRST) {
+		if (pack.ncols == 1) {
+			// Common case: one key column. Width dispatch once, then 16 straight-line coalesced loads.
+			u64 raw[ITEMS];
+			LoadWaveItems<ITEMS>(pack.col[0], pack.width[0], wave_base, lane, n, raw);
 #pragma unroll
-	for (int k = 0; k < ITEMS; k++) {
-		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
-		if (row < n) {
-			if constexpr (FIRST) {
-				key[k] = (KeyT)PackRow(pack, row);
-			} else {
-				key[k] = keys_in[row];
+			for (int k = 0; k < ITEMS; k++) {
+				u64 ord = OrderedKey(raw[k], pack.width[0], pack.is_signed[0] != 0);
+				key[k] = (KeyT)((ord - pack.min[0]) << pack.shift[0]);
 			}
 		} else {
-			key[k] = 0;
+#pragma unroll
+			for (int k = 0; k < ITEMS; k++) {
+				uint64_t row = wave_base + (uint64_t)k * kWave + lane;
+				key[k] = row < n ? (KeyT)PackRow(pack, row) : (KeyT)0;
+			}
+		}
+	} else {
+#pragma unroll
+		for (int k = 0; k < ITEMS; k++) {
+			uint64_t row = wave_base + (uint64_t)k * kWave + lane;
+			key[k] = row < n ? keys_in[row] : (KeyT)0;
 		}
 	}
 	// Payload prefetch (raw bits), issued before the ranking so its latency is hidden.
@@ -272,19 +318,13 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 #pragma unroll
 	for (int p = 0; p < NP; p++) {
 		if (p < pay.n) {
-			const int width = pay.width[p];
-			const void *src = pay.in[p];
+			if (FIRST && pay.in[p] == nullptr) {
 #pragma unroll
-			for (int k = 0; k < ITEMS; k++) {
-				uint64_t row = wave_base + (uint64_t)k * kWave + lane;
-				pv[p][k] = 0;
-				if (row < n) {
-					if (FIRST && src == nullptr) {
-						pv[p][k] = row;
-					} else {
-						pv[p][k] = LoadRawBits(src, row, width);
-					}
+				for (int k = 0; k < ITEMS; k++) {
+					pv[p][k] = wave_base + (uint64_t)k * kWave + lane;
 				}
+			} else {
+				LoadWaveItems<ITEMS>(pay.in[p], pay.width[p], wave_base, lane, n, pv[p]);
 			}
 		}
 	}
@@ -596,8 +636,25 @@ void KeyRangeHist8(const void *col, int vtype, uint64_t n, uint64_t *min_out, ui
 		const int width = TypeWidth(vtype);
 		uint64_t blocks = std::min<uint64_t>((n + 1023) / 1024, 2048);
 		ProfileScope prof("key_range_hist", stream, (double)n * width);
-		hipLaunchKernelGGL(KeyRangeHist8Kernel, dim3((unsigned)blocks), dim3(256), 0, stream, col, width,
-		                   TypeIsSigned(vtype) ? 1 : 0, n, mm, mm + 1, hist8_dev);
+		const int sg = TypeIsSigned(vtype) ? 1 : 0;
+		switch (width) {
+		case 8:
+			hipLaunchKernelGGL(KeyRangeHist8Kernel<u64>, dim3((unsigned)blocks), dim3(256), 0, stream,
+			                   (const u64 *)col, width, sg, n, mm, mm + 1, hist8_dev);
+			break;
+		case 4:
+			hipLaunchKernelGGL(KeyRangeHist8Kernel<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, stream,
+			                   (const uint32_t *)col, width, sg, n, mm, mm + 1, hist8_dev);
+			break;
+		case 2:
+			hipLaunchKernelGGL(KeyRangeHist8Kernel<uint16_t>, dim3((unsigned)blocks), dim3(256), 0, stream,
+			                   (const uint16_t *)col, width, sg, n, mm, mm + 1, hist8_dev);
+			break;
+		default:
+			hipLaunchKernelGGL(KeyRangeHist8Kernel<uint8_t>, dim3((unsigned)blocks), dim3(256), 0, stream,
+			                   (const uint8_t *)col, width, sg, n, mm, mm + 1, hist8_dev);
+			break;
+		}
 		VGPU_HIP(hipGetLastError());
 	}
 	u64 host[2];

@@ -10,8 +10,9 @@
 //     LDS step across the 4 waves;
 //   * a group's output index is its head's global rank, which each tile learns by decoupled look-back on
 //     per-tile head counts (one 64-bit atomic word per tile);
-//   * a group that continues past its tile leaves a tail partial; a small fix-up kernel folds the leading
-//     partials of the following tiles into it (one wave per tail, 64 tiles per step).
+//   * a group that continues past its tile leaves a tail partial; a two-level fix-up (a reverse segmented
+//     scan of tile leads per 256-tile chunk, then one fold per open tail plus one per spanned chunk)
+//     finishes it, so even the head group of a Zipf key (thousands of tiles) costs a few dependent loads.
 // SUM over integers accumulates in exact int128 (DuckDB's HUGEINT), SUM/AVG over doubles in double-double
 // so the result is rounded once.
 #include "agg_state.h"
@@ -211,6 +212,27 @@ __device__ __forceinline__ u64 LdsRawBits(const uint8_t *base, uint32_t row, int
 	}
 }
 
+constexpr int kSlotStride = kRedBlock + 8; // 264 = 8 (mod 64): conflict-free transposed staging
+
+//! Row r of the tile -> dst[(r % 8) * kSlotStride + r / 8]. Each thread loads rows tid + 256 j (coalesced
+//! across the wave), all loads first, then the stores.
+template <typename T, typename D>
+__device__ __forceinline__ void StageTransposed(D *dst, const T *src, uint32_t count) {
+	T v[kRedItems];
+#pragma unroll
+	for (int j = 0; j < kRedItems; j++) {
+		uint32_t r = threadIdx.x + j * kRedBlock;
+		v[j] = r < count ? src[r] : (T)0;
+	}
+#pragma unroll
+	for (int j = 0; j < kRedItems; j++) {
+		uint32_t r = threadIdx.x + j * kRedBlock;
+		if (r < count) {
+			dst[(r % kRedItems) * kSlotStride + r / kRedItems] = (D)v[j];
+		}
+	}
+}
+
 //! Reduce tile: 256 threads x 8 consecutive rows. Keys and payload columns are staged into LDS with one
 //! coalesced sweep each, then every thread folds its 8 rows serially from LDS.
 template <typename KeyT, int NP>
@@ -223,9 +245,9 @@ __global__ void __launch_bounds__(kRedBlock)
 	__shared__ u64 s_hb;
 	__shared__ u64 s_wave_state[kRedWaves][2 * kMaxComponents];
 	__shared__ int s_wave_flag[kRedWaves];
-	__shared__ __align__(16) KeyT s_keys[kRedTile];
+	__shared__ KeyT s_keys[kRedItems * kSlotStride];
 	__shared__ KeyT s_prev;
-	__shared__ __align__(16) uint8_t s_pay[NP > 0 ? NP : 1][kRedTile * 8];
+	__shared__ u64 s_pay[NP > 0 ? NP : 1][kRedItems * kSlotStride]; // raw bits, zero-extended
 
 	const int tid = threadIdx.x;
 	const int lane = tid & (kWave - 1);
@@ -238,10 +260,12 @@ __global__ void __launch_bounds__(kRedBlock)
 	const uint64_t tile_base = (uint64_t)tile * kRedTile;
 	const uint32_t tile_count = (uint32_t)((n - tile_base) < (uint64_t)kRedTile ? (n - tile_base) : kRedTile);
 
-	// Stage: s_keys[r] = row r of the tile (16-byte aligned for vector stores); s_prev = key before it.
+	// Stage, transposed: row r of the tile goes to slot (r % 8) * kSlotStride + r / 8, so a thread's 8
+	// consecutive rows are 8 slots one stride apart and a wave reads 64 consecutive slots per step; the
+	// stride (264 = 8 mod 64) keeps the staging writes conflict-free too. Loads are coalesced per wave and
+	// all issued before the first LDS store. s_prev = key before the tile.
 	if (keys != nullptr) {
-		StageToLds<4>(reinterpret_cast<uint8_t *>(s_keys), reinterpret_cast<const uint8_t *>(keys + tile_base),
-		              tile_count * (uint32_t)sizeof(KeyT));
+		StageTransposed<KeyT>(s_keys, keys + tile_base, tile_count);
 		if (tid == 0) {
 			s_prev = tile_base > 0 ? keys[tile_base - 1] : (KeyT)0;
 		}
@@ -250,7 +274,23 @@ __global__ void __launch_bounds__(kRedBlock)
 	for (int p = 0; p < NP; p++) {
 		if (p < pay.n) {
 			const int w = pay.width[p];
-			StageToLds<4>(s_pay[p], reinterpret_cast<const uint8_t *>(pay.ptr[p]) + tile_base * w, tile_count * w);
+			switch (w) {
+			case 8:
+				StageTransposed<u64>(s_pay[p], reinterpret_cast<const u64 *>(pay.ptr[p]) + tile_base, tile_count);
+				break;
+			case 4:
+				StageTransposed<uint32_t>(s_pay[p], reinterpret_cast<const uint32_t *>(pay.ptr[p]) + tile_base,
+				                          tile_count);
+				break;
+			case 2:
+				StageTransposed<uint16_t>(s_pay[p], reinterpret_cast<const uint16_t *>(pay.ptr[p]) + tile_base,
+				                          tile_count);
+				break;
+			default:
+				StageTransposed<uint8_t>(s_pay[p], reinterpret_cast<const uint8_t *>(pay.ptr[p]) + tile_base,
+				                         tile_count);
+				break;
+			}
 		}
 	}
 	__syncthreads();
@@ -260,11 +300,11 @@ __global__ void __launch_bounds__(kRedBlock)
 	KeyT k[kRedItems];
 	bool head[kRedItems];
 	u64 hc = 0;
-	KeyT before = keys == nullptr ? (KeyT)0 : (local0 == 0 ? s_prev : s_keys[local0 - 1]);
+	KeyT before = keys == nullptr ? (KeyT)0 : (tid == 0 ? s_prev : s_keys[(kRedItems - 1) * kSlotStride + tid - 1]);
 #pragma unroll
 	for (int i = 0; i < kRedItems; i++) {
 		uint32_t lr = local0 + i;
-		k[i] = (keys != nullptr && lr < tile_count) ? s_keys[lr] : (KeyT)0;
+		k[i] = (keys != nullptr && lr < tile_count) ? s_keys[i * kSlotStride + tid] : (KeyT)0;
 		uint64_t row = base + i;
 		head[i] = lr < tile_count && (row == 0 || k[i] != before);
 		before = k[i];
@@ -303,7 +343,7 @@ __global__ void __launch_bounds__(kRedBlock)
 			u64 vals[NP > 0 ? NP : 1];
 #pragma unroll
 			for (int p = 0; p < (NP > 0 ? NP : 1); p++) {
-				vals[p] = (p < pay.n) ? LdsRawBits(s_pay[p], lr, pay.width[p]) : 0ull;
+				vals[p] = (p < pay.n) ? s_pay[p][i * kSlotStride + tid] : 0ull;
 			}
 			if (head[i]) {
 				if (seen) {
@@ -382,7 +422,7 @@ __global__ void __launch_bounds__(kRedBlock)
 			bool ends = tile_end >= n;
 			if (!ends && keys != nullptr) {
 				// keys == nullptr: one global group, which only ends with the input
-				ends = keys[tile_end] != s_keys[kRedTile - 1];
+				ends = keys[tile_end] != s_keys[(kRedItems - 1) * kSlotStride + kRedBlock - 1];
 			}
 			if (ends) {
 				WriteGroup(o, X, gid_last);
@@ -406,82 +446,88 @@ __global__ void __launch_bounds__(kRedBlock)
 constexpr int kFixBlock = 256;
 constexpr int kFixWaves = kFixBlock / kWave;
 
-//! Finishes one group that continues past tile T: folds the leading partials of the following tiles (up to
-//! and including the first tile that starts a new group) into T's tail partial. 256 following tiles are
-//! examined per step, combined left to right (wave xor-trees in lane order, then waves in order) so the
-//! result does not depend on scheduling. Called by the whole block.
-__device__ __forceinline__ void FixupOne(const CompDesc &d, const AggOutDesc &o, const uint32_t *tile_heads,
-                                         const u64 *lead_state, const u64 *tail_state, const u64 *tail_gid,
-                                         uint64_t T, uint64_t ntiles, u64 (*s_state)[2 * kMaxComponents],
-                                         int *s_stop) {
-	const u64 g = tail_gid[T];
+//! Level 1 of the fix-up. For each tile u of a 256-tile chunk: run[u] = lead[u] o lead[u+1] o ... up to
+//! and including the first tile v >= u that starts a group (its lead is the part before that group), and
+//! open[u] = no such tile inside the chunk (the run continues into the next chunk). A reverse segmented
+//! scan: wave shuffles, then waves right to left through LDS. Tiles past the input act as a stop.
+__global__ void __launch_bounds__(kFixBlock) ChunkRunKernel(CompDesc d, const uint32_t *tile_heads,
+                                                            const u64 *lead_state, u64 *run_state,
+                                                            uint8_t *run_open, uint64_t ntiles) {
+	__shared__ u64 s_state[kFixWaves][2 * kMaxComponents];
+	__shared__ int s_flag[kFixWaves];
 	const int tid = threadIdx.x;
 	const int lane = tid & (kWave - 1);
 	const int wave = tid / kWave;
-	CompState acc;
-	StateLoad(d, tail_state + T * 2 * d.ncomp, acc);
-	for (uint64_t U = T + 1; U < ntiles; U += kFixBlock) {
-		uint64_t u = U + tid;
-		bool in_range = u < ntiles;
-		uint32_t h = in_range ? tile_heads[u] : 1u;
-		u64 stops = __ballot(h > 0);
-		if (lane == 0) {
-			s_stop[wave] = stops ? (wave * kWave + __ffsll((long long)stops) - 1) : kFixBlock;
-		}
-		__syncthreads();
-		int stop = kFixBlock;
-		for (int w = 0; w < kFixWaves; w++) {
-			stop = s_stop[w] < stop ? s_stop[w] : stop;
-		}
-		CompState v;
-		if (in_range && tid <= stop) {
-			StateLoad(d, lead_state + u * 2 * d.ncomp, v);
-		} else {
-			StateIdentity(d, v);
-		}
-		for (int off = 1; off < kWave; off <<= 1) {
-			CompState other;
-			StateShflXor(d, other, v, off);
-			if ((lane & off) == 0) {
-				StateCombine(d, v, other);
-			} else {
-				StateCombine(d, other, v);
-				v = other;
+	const uint64_t u = (uint64_t)blockIdx.x * kFixBlock + tid;
+	const bool in_range = u < ntiles;
+	bool F = !in_range || tile_heads[u] > 0; // stop flag
+	CompState X;
+	if (in_range) {
+		StateLoad(d, lead_state + u * 2 * d.ncomp, X);
+	} else {
+		StateIdentity(d, X);
+	}
+	for (int off = 1; off < kWave; off <<= 1) {
+		CompState dn;
+#pragma unroll
+		for (int c = 0; c < kMaxComponents; c++) {
+			if (c < d.ncomp) {
+				dn.a[c] = __shfl_down(X.a[c], off);
+				dn.b[c] = __shfl_down(X.b[c], off);
 			}
 		}
-		if (lane == 0) {
-			StateStore(d, s_state[wave], v);
+		bool fd = __shfl_down((int)F, off) != 0;
+		if (lane + off < kWave) {
+			if (!F) {
+				StateCombine(d, X, dn);
+			}
+			F = F || fd;
 		}
-		__syncthreads();
-		if (tid == 0) {
-			for (int w = 0; w < kFixWaves; w++) {
-				CompState t;
-				StateLoad(d, s_state[w], t);
-				StateCombine(d, acc, t);
+	}
+	if (lane == 0) {
+		StateStore(d, s_state[wave], X);
+		s_flag[wave] = F ? 1 : 0;
+	}
+	__syncthreads();
+	if (!F) {
+		for (int w = wave + 1; w < kFixWaves; w++) {
+			CompState t;
+			StateLoad(d, s_state[w], t);
+			StateCombine(d, X, t);
+			if (s_flag[w]) {
+				F = true;
+				break;
 			}
 		}
-		__syncthreads();
-		if (stop < kFixBlock) {
-			break;
-		}
 	}
-	if (tid == 0) {
-		WriteGroup(o, acc, g);
+	if (in_range) {
+		StateStore(d, run_state + u * 2 * d.ncomp, X);
+		run_open[u] = F ? 0 : 1;
 	}
-	__syncthreads(); // s_state / s_stop are reused by the block's next tail
 }
 
-//! Grid-stride over the compact list of tiles whose last group continues (appended by SegReduceKernel),
-//! so the launch is bounded no matter how many tiles the input has.
-__global__ void __launch_bounds__(kFixBlock) SegFixupKernel(CompDesc d, AggOutDesc o, const uint32_t *tile_heads,
-                                                            const u64 *lead_state, const u64 *tail_state,
-                                                            const u64 *tail_gid, const uint32_t *tail_list,
-                                                            const uint32_t *tail_count, uint64_t ntiles) {
-	__shared__ u64 s_state[kFixWaves][2 * kMaxComponents];
-	__shared__ int s_stop[kFixWaves];
+//! Level 2: each open tail T folds run[T+1] and, while the run stays open, the first-tile runs of the
+//! following chunks (a few steps even for a group spanning thousands of tiles), then writes the group.
+__global__ void __launch_bounds__(kFixBlock) TailFinishKernel(CompDesc d, AggOutDesc o, const u64 *run_state,
+                                                              const uint8_t *run_open, const u64 *tail_state,
+                                                              const u64 *tail_gid, const uint32_t *tail_list,
+                                                              const uint32_t *tail_count, uint64_t ntiles) {
 	const uint32_t ntails = *tail_count;
-	for (uint32_t item = blockIdx.x; item < ntails; item += gridDim.x) {
-		FixupOne(d, o, tile_heads, lead_state, tail_state, tail_gid, tail_list[item], ntiles, s_state, s_stop);
+	for (uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < ntails; item += gridDim.x * blockDim.x) {
+		const uint64_t T = tail_list[item];
+		CompState acc;
+		StateLoad(d, tail_state + T * 2 * d.ncomp, acc);
+		uint64_t u = T + 1;
+		while (u < ntiles) {
+			CompState r;
+			StateLoad(d, run_state + u * 2 * d.ncomp, r);
+			StateCombine(d, acc, r);
+			if (!run_open[u]) {
+				break;
+			}
+			u = (u / kFixBlock + 1) * kFixBlock; // first tile of the next chunk
+		}
+		WriteGroup(o, acc, tail_gid[T]);
 	}
 }
 
@@ -513,9 +559,14 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 	}
 	{
 		ProfileScope prof("seg_fixup", stream, 0.0);
-		const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, 2048);
-		hipLaunchKernelGGL(SegFixupKernel, dim3(grid), dim3(kFixBlock), 0, stream, cd, od, tile_heads, lead_state,
-		                   tail_state, tail_gid, tail_list, tail_count, ntiles);
+		const uint64_t nchunks = (ntiles + kFixBlock - 1) / kFixBlock;
+		auto *run_state = static_cast<u64 *>(scratch.Alloc(ntiles * state_words * sizeof(u64)));
+		auto *run_open = static_cast<uint8_t *>(scratch.Alloc(ntiles));
+		hipLaunchKernelGGL(ChunkRunKernel, dim3((unsigned)nchunks), dim3(kFixBlock), 0, stream, cd, tile_heads,
+		                   lead_state, run_state, run_open, ntiles);
+		VGPU_HIP(hipGetLastError());
+		hipLaunchKernelGGL(TailFinishKernel, dim3((unsigned)nchunks), dim3(kFixBlock), 0, stream, cd, od, run_state,
+		                   run_open, tail_state, tail_gid, tail_list, tail_count, ntiles);
 		VGPU_HIP(hipGetLastError());
 	}
 }

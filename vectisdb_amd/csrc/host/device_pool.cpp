@@ -112,6 +112,24 @@ void DevicePool::Free(void *ptr, hipStream_t stream) {
 	free_.emplace(b.size, b);
 }
 
+bool DevicePool::Owns(void *ptr) const {
+	std::lock_guard<std::mutex> lock(mutex_);
+	return live_.count(ptr) != 0;
+}
+
+void DevicePool::FreeAny(void *ptr) {
+	if (ptr == nullptr) {
+		return;
+	}
+	if (Owns(ptr)) {
+		// The null stream synchronizes with every blocking stream, so the event recorded there is only
+		// complete once earlier work on them is.
+		Free(ptr, nullptr);
+		return;
+	}
+	(void)hipFree(ptr);
+}
+
 void DevicePool::Trim() {
 	(void)hipDeviceSynchronize();
 	std::lock_guard<std::mutex> lock(mutex_);

@@ -28,6 +28,10 @@ public:
 	void *Alloc(size_t bytes, hipStream_t stream);
 	//! Returns a block to the cache; `stream` is the last stream that used it.
 	void Free(void *ptr, hipStream_t stream);
+	//! Frees memory of unknown origin: a pool block goes back to the pool (after a device sync point is
+	//! not needed: it is tagged with the null stream), anything else is released with hipFree.
+	void FreeAny(void *ptr);
+	bool Owns(void *ptr) const;
 
 	//! Releases every cached (free) block back to the driver. Synchronizes the device.
 	void Trim();
