@@ -75,7 +75,7 @@ def kernel_stats(prof, name):
 
 
 def bench_filter_project(torch, capi, dev, steps, copies=16, rows=1_048_576):
-    """fused scan-filter-project over 1,048,576 rows; `copies` table copies are rotated so the
+    """Fused scan-filter-project over 1,048,576 rows (SELECT a*2+b, c*1.5+a WHERE a < 500000 AND c > 0.25); `copies` table copies are rotated so the
     27 MB working set is not served from the 256 MB Infinity Cache."""
     sys.path.insert(0, str(ROOT / "tests"))
     from exprs import cfg0_exprs
@@ -105,14 +105,14 @@ def bench_filter_project(torch, capi, dev, steps, copies=16, rows=1_048_576):
     # algorithmic bytes: read a, b, c (20 B/row) + write kept rows' two 8-byte outputs
     alg = rows * 20 + kept * 16
     gbs = alg / (avg * 1e-3) / 1e9
-    return {"operator": "fused filter/project (configs[0])", "rows": rows, "kept": kept,
+    return {"operator": "fused filter/project, 1,048,576 rows", "rows": rows, "kept": kept,
             "kernel_ms": round(avg, 5), "rows_per_s": round(rows / (avg * 1e-3), 1),
             "call_ms": round((t1 - t0) * 1e3 / steps, 4), "algorithmic_bytes": alg, "gbs": round(gbs, 1),
             "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
-    """configs[2] join: build 2^24 unique keys (permutation) + INT32 payload, probe 2^28 fk (90% hits);
+    """Join + aggregate: build 2^24 unique keys (permutation) + INT32 payload, probe 2^28 fk (90% hits);
     SELECT SUM(payload), COUNT(*) FROM probe JOIN build ON fk = key."""
     import numpy as np
     build = torch.from_numpy(np.random.RandomState(21).permutation(nb).astype(np.int64)).to(dev)
@@ -145,14 +145,14 @@ def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
     ms = (t1 - t0) * 1e3 / steps
     kernels = {e["name"]: round(e["ms"] / max(e["launches"], 1) * (e["launches"] / steps), 4) for e in prof}
     alg = sum(e["bytes"] for e in prof) / steps
-    return {"operator": "sort-merge join + aggregate (configs[2])", "probe_rows": nprobe, "build_rows": nb,
+    return {"operator": "sort-merge join + aggregate, 2^24 build x 2^28 probe", "probe_rows": nprobe, "build_rows": nb,
             "sum_payload": int(res[0]), "count": int(res[1]), "ms": round(ms, 3),
             "rows_per_s": round((nprobe + nb) / (ms * 1e-3), 1), "algorithmic_bytes": alg,
             "gbs": round(alg / (ms * 1e-3) / 1e9, 1), "kernel_ms_per_step": kernels}
 
 
 def bench_cross(torch, capi, dev, steps):
-    """configs[2] cross product: 4,096 x 65,536 INT32 pairs WHERE x + y < 2^30, materialized (x, y)."""
+    """Cross product + filter: 4,096 x 65,536 INT32 pairs WHERE x + y < 2^30, materialized (x, y)."""
     sys.path.insert(0, str(ROOT / "tests"))
     from exprs import col, copy_body, out
     x = torch.empty(4096, dtype=torch.int32, device=dev)
@@ -174,7 +174,7 @@ def bench_cross(torch, capi, dev, steps):
     pairs = 4096 * 65536
     alg = 4 * (4096 + 65536) + 8 * kept
     gbs = alg / (avg * 1e-3) / 1e9
-    return {"operator": "cross product + filter (configs[2])", "pairs": pairs, "kept": kept,
+    return {"operator": "cross product + filter, 4,096 x 65,536", "pairs": pairs, "kept": kept,
             "kernel_ms": round(avg, 4), "rows_per_s": round(pairs / (avg * 1e-3), 1),
             "algorithmic_bytes": alg, "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
@@ -200,8 +200,9 @@ def roofline_from_profile(prof, prefer=None):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
-                                   "(the multi-GPU sharding design is described in DESIGN.md)"}))
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus N > 1 is not supported: GROUP BY "
+                                   "would shard by key range with one all-to-all of per-GPU partial "
+                                   "aggregates, which is not built"}))
         sys.exit(2)
     import numpy as np
     import torch
@@ -307,7 +308,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int64+f64",
         "data": "synthetic (seeded device generator: Zipf(1.1) keys over 1M values, uniform int64, normal f64)",
-        "config": {"workload": "sort-based GROUP BY, 100M rows, 7 aggregates (configs[1])", "rows": rows,
+        "config": {"workload": "sort-based GROUP BY, 100M rows, Zipf(1.1) keys over 1M values, 7 aggregates", "rows": rows,
                    "groups": groups, "step_algorithmic_bytes": step_bytes,
                    "step_gbs": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1)},
         "roofline": roof,

@@ -155,6 +155,92 @@ int vgpu_generate_column(void *out, int32_t type, uint64_t n, int32_t dist, int6
                          uint64_t seed, uint64_t stream_id, const double *cdf, uint64_t cdf_len,
                          const int64_t *perm, void *stream);
 
+/* ---------------------------------------------------------------- plans
+ * A plan is a GpuPlanNode tree (cuda_engine/include/gpu_engine.hpp:135-185) written as JSON:
+ *   {"op": "SCAN"|"FILTER"|"PROJECTION"|"HASH_JOIN"|"GROUP_BY_AGGREGATE"|"CROSS_PRODUCT",
+ *    "estimated_rows", "estimated_row_width_bytes", "table": {"catalog","schema","table"}, "column_names",
+ *    "expressions": [{"source", "inputs", "output_type", "ops"}], "build_keys", "probe_keys",
+ *    "output_columns", "group_keys", "aggregate_exprs", "aggregates": [{"kind", "input", "output_type"}],
+ *    "children": [...]}
+ * with types and kinds by name ("INT64", "SUM") or number. Host columns are HOST memory here. */
+typedef struct vgpu_host_column {
+	const char *name;
+	int32_t type;             /* enum vgpu_type */
+	const void *data;         /* host, tightly packed */
+	const uint8_t *validity;  /* Arrow bitmap (bit r%8 of byte r/8, 1 = valid) or NULL */
+	uint64_t rows;
+} vgpu_host_column;
+
+/* GpuExecuteOptions (cuda_engine/include/gpu_engine.hpp:357-385). */
+typedef struct vgpu_execute_options {
+	int32_t stage_scan_to_cache;
+	int64_t cache_chunk_index; /* -1 = off */
+	int32_t serve_whole_scan_from_cache;
+} vgpu_execute_options;
+
+typedef struct vgpu_result vgpu_result; /* GpuExecutionResult, owned by the library until vgpu_result_free */
+
+/* GpuEngine::Instance().ExecutePlan (cuda_engine/include/gpu_engine.hpp:389-402,
+ * cuda_engine/src/gpu_engine.cpp:222-259). Returns VGPU_ERROR only when the plan text cannot be parsed;
+ * execution failures come back in the result (vgpu_result_success == 0 plus vgpu_result_error), exactly as
+ * ExecutePlan reports them. `options` may be NULL. Synchronizes. */
+int vgpu_execute_plan(const char *plan_json, const vgpu_host_column *inputs, int32_t ninputs,
+                      const vgpu_execute_options *options, vgpu_result **result);
+int32_t vgpu_result_success(const vgpu_result *result);
+const char *vgpu_result_error(const vgpu_result *result);
+int32_t vgpu_result_num_columns(const vgpu_result *result);
+/* Column `index` of the result; its pointers stay valid until vgpu_result_free. */
+int vgpu_result_column(const vgpu_result *result, int32_t index, vgpu_host_column *out);
+void vgpu_result_free(vgpu_result *result);
+
+/* ---------------------------------------------------------------- routing (no device needed)
+ * AnalyzePlanShape, IsRowIndependent, EstimateWorkingSet, EstimateColumnsSize, DetermineExecutionMode
+ * (cuda_engine/include/gpu_engine.hpp:248, :329-348; cuda_engine/src/gpu_cost_model.cpp). */
+typedef struct vgpu_plan_shape {
+	uint64_t scanned_rows;
+	uint64_t scanned_bytes;
+	uint64_t max_rows;
+	double total_ops;
+	double ops_per_scanned_row;
+	double ops_per_scanned_byte;
+	int32_t density; /* 0 INSUFFICIENT, 1 INCONCLUSIVE, 2 SUFFICIENT */
+	double amplification;
+} vgpu_plan_shape;
+int vgpu_analyze_plan_shape(const char *plan_json, vgpu_plan_shape *out);
+int vgpu_is_row_independent(const char *plan_json, int32_t *out);
+/* largest_operator receives the operator name (NUL-terminated, truncated to `cap`). */
+int vgpu_estimate_working_set(const char *plan_json, uint64_t max_chunk_rows, uint64_t *total_bytes,
+                              int32_t *chunk_capped, uint64_t *largest_bytes, char *largest_operator, size_t cap);
+int vgpu_estimate_columns_size(const char *plan_json, uint64_t *out);
+/* *mode: 0 PIPELINE_STREAM_CACHED, 1 PIPELINE_STREAM_UNCACHED. */
+int vgpu_determine_execution_mode(const char *plan_json, uint64_t vram_budget, double threshold, int32_t *mode);
+
+/* KernelCache::HashSource (cuda_engine/include/kernel_cache.hpp:56): lowercase hex SHA-256 into hex[65]. */
+int vgpu_hash_source(const char *text, size_t len, char *hex);
+/* IsGpuFastMathEnabled / SetGpuFastMathEnabled (cuda_engine/include/code_generator.hpp:14-16). */
+int32_t vgpu_fast_math_enabled(void);
+void vgpu_set_fast_math(int32_t on);
+
+/* ---------------------------------------------------------------- column cache and device budget
+ * GpuColumnCache (cuda_engine/include/gpu_column_cache.hpp:130-175) and RmmPool
+ * (cuda_engine/include/rmm_pool.hpp:66-95)
+ * operations the extension calls, for plans staged with stage_scan_to_cache. */
+int vgpu_cache_begin_staging(const char *catalog, const char *schema, const char *table);
+int vgpu_cache_commit_staging(const char *catalog, const char *schema, const char *table, uint64_t scanned_rows);
+int vgpu_cache_abort_staging(const char *catalog, const char *schema, const char *table);
+int vgpu_cache_invalidate_table(const char *catalog, const char *schema, const char *table);
+int vgpu_cache_invalidate_all(void);
+int vgpu_cache_current_bytes(uint64_t *out);
+int vgpu_cache_table_generation(const char *catalog, const char *schema, const char *table, uint64_t *out);
+/* Chunk row counts when every named column is resident (*nchunks = 0 otherwise); fails when more than
+ * `cap` chunks. */
+int vgpu_cache_replayable_chunk_rows(const char *catalog, const char *schema, const char *table,
+                                     const char *const *columns, int32_t ncolumns, uint64_t *rows, size_t cap,
+                                     size_t *nchunks);
+int vgpu_vram_budget_init(double fraction_of_vram);
+int vgpu_vram_has_headroom(uint64_t bytes, int32_t *out);
+int vgpu_vram_budget_bytes(uint64_t *total, uint64_t *allocated);
+
 /* ---------------------------------------------------------------- measurement */
 
 /* When on, every kernel launch is bracketed by HIP events on its own stream. */

@@ -38,6 +38,22 @@ class Expr(C.Structure):
                 ("output_type", C.c_int32)]
 
 
+class HostColumn(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("type", C.c_int32), ("data", C.c_void_p), ("validity", C.c_void_p),
+                ("rows", C.c_uint64)]
+
+
+class ExecuteOptions(C.Structure):
+    _fields_ = [("stage_scan_to_cache", C.c_int32), ("cache_chunk_index", C.c_int64),
+                ("serve_whole_scan_from_cache", C.c_int32)]
+
+
+class PlanShape(C.Structure):
+    _fields_ = [("scanned_rows", C.c_uint64), ("scanned_bytes", C.c_uint64), ("max_rows", C.c_uint64),
+                ("total_ops", C.c_double), ("ops_per_scanned_row", C.c_double),
+                ("ops_per_scanned_byte", C.c_double), ("density", C.c_int32), ("amplification", C.c_double)]
+
+
 # (name, restype, argtypes); every function returns int status unless noted
 _PROTOS = {
     "vgpu_last_error": (C.c_char_p, []),
@@ -68,6 +84,35 @@ _PROTOS = {
     "vgpu_profile_read": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vgpu_pool_stats": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "vgpu_pool_trim": (C.c_int, []),
+    "vgpu_execute_plan": (C.c_int, [C.c_char_p, C.POINTER(HostColumn), C.c_int32, C.POINTER(ExecuteOptions),
+                                    C.POINTER(C.c_void_p)]),
+    "vgpu_result_success": (C.c_int32, [C.c_void_p]),
+    "vgpu_result_error": (C.c_char_p, [C.c_void_p]),
+    "vgpu_result_num_columns": (C.c_int32, [C.c_void_p]),
+    "vgpu_result_column": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(HostColumn)]),
+    "vgpu_result_free": (None, [C.c_void_p]),
+    "vgpu_analyze_plan_shape": (C.c_int, [C.c_char_p, C.POINTER(PlanShape)]),
+    "vgpu_is_row_independent": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32)]),
+    "vgpu_estimate_working_set": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_uint64), C.c_char_p, C.c_size_t]),
+    "vgpu_estimate_columns_size": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint64)]),
+    "vgpu_determine_execution_mode": (C.c_int, [C.c_char_p, C.c_uint64, C.c_double, C.POINTER(C.c_int32)]),
+    "vgpu_hash_source": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p]),
+    "vgpu_fast_math_enabled": (C.c_int32, []),
+    "vgpu_set_fast_math": (None, [C.c_int32]),
+    "vgpu_cache_begin_staging": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p]),
+    "vgpu_cache_commit_staging": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64]),
+    "vgpu_cache_abort_staging": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p]),
+    "vgpu_cache_invalidate_table": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p]),
+    "vgpu_cache_invalidate_all": (C.c_int, []),
+    "vgpu_cache_current_bytes": (C.c_int, [C.POINTER(C.c_uint64)]),
+    "vgpu_cache_table_generation": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_uint64)]),
+    "vgpu_cache_replayable_chunk_rows": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p),
+                                                   C.c_int32, C.POINTER(C.c_uint64), C.c_size_t,
+                                                   C.POINTER(C.c_size_t)]),
+    "vgpu_vram_budget_init": (C.c_int, [C.c_double]),
+    "vgpu_vram_has_headroom": (C.c_int, [C.c_uint64, C.POINTER(C.c_int32)]),
+    "vgpu_vram_budget_bytes": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
 }
 
 _lib = None

@@ -1,0 +1,1674 @@
+// executor.cpp -- GpuEngine::ExecutePlan: the recursive plan executor behind gpu_engine.hpp.
+//
+// Behaviour follows cuda_engine/src/gpu_engine.cpp:222-259 (ExecutePlan) and
+// cuda_engine/src/gpu_executor.cu:368-967 (ExecuteScan, FILTER, PROJECTION, ExecuteNode, ExecuteGpuPlan)
+// with the operators of cuda_engine/src/operators/*.cu: SCAN leaves consume `inputs` in traversal order
+// (children[0] before children[1]); projection outputs are named col<i>, aggregates agg_<i>, group keys
+// keep their names, join and cross outputs follow output_columns; NULLs are modelled only by
+// row-independent plans (a NULL predicate drops the row, a projection is NULL where any referenced input
+// is); every failure becomes success=false with the reference's message.
+//
+// Execution is shaped for the device instead of mirrored:
+//   * Every intermediate is dense. A FILTER is one generated kernel that evaluates all its predicates and
+//     writes the surviving rows of the columns its consumer reads (wave64 ballot compaction, decoupled
+//     look-back); a PROJECTION over a FILTER (or over a CROSS_PRODUCT, or a FILTER over one) is the same
+//     single kernel, so no mask, index vector, selection or per-expression pass is ever materialized.
+//   * Columns no consumer reads are pruned: they keep their name and type (so name resolution and its
+//     errors are unchanged) but are never uploaded, gathered or written.
+//   * A global aggregate over an INNER join runs as a sort-merge join-aggregate without materializing a
+//     single pair.
+//   * Device memory comes from a stream-ordered caching pool, not one hipMalloc/hipFree per buffer.
+//   * NULL validity is packed into the Arrow bitmap on the device; only the bitmap crosses PCIe.
+#include "../../../include/code_generator.hpp"
+#include "../../../include/gpu_column_cache.hpp"
+#include "../../../include/gpu_engine.hpp"
+#include "../../../include/gpu_logger.hpp"
+#include "../../../include/pinned_buffer_pool.hpp"
+#include "../../../include/rmm_pool.hpp"
+#include "../host/device_pool.h"
+#include "../host/operators.h"
+#include "fused.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace vector_gpu {
+
+namespace {
+
+using NameSet = std::set<std::string>;
+
+size_t TypeSize(GpuValueType type) {
+	switch (type) {
+	case GpuValueType::INT16:
+		return 2;
+	case GpuValueType::INT32:
+		return 4;
+	case GpuValueType::INT64:
+		return 8;
+	case GpuValueType::HUGEINT:
+		return 16;
+	case GpuValueType::FLOAT32:
+		return 4;
+	case GpuValueType::FLOAT64:
+		return 8;
+	case GpuValueType::BOOLEAN:
+		return 1;
+	case GpuValueType::DICTIONARY_STRING:
+		throw std::runtime_error("gpu_executor: dictionary-encoded string columns cannot be executed on GPU");
+	}
+	throw std::runtime_error("gpu_executor: unknown GpuValueType");
+}
+
+//! A column resident in device memory while a plan executes. Always dense. `valid` is one byte per row
+//! (non-zero = valid) or nullptr when the column has no NULLs. A pruned column carries name, type and row
+//! count only: no operator above it reads its values.
+struct DeviceColumn {
+	std::string name;
+	GpuValueType type = GpuValueType::FLOAT64;
+	void *data = nullptr;
+	uint64_t rows = 0;
+	uint8_t *valid = nullptr;
+	bool pruned = false;
+};
+using Columns = std::vector<DeviceColumn>;
+
+struct ExecContext {
+	const std::vector<GpuColumn> &inputs;
+	size_t cursor = 0;
+	vgpu::PoolArena &arena;
+	hipStream_t stream = nullptr;
+	int64_t cache_chunk_index = -1;
+	bool serve_whole_scan_from_cache = false;
+	bool stage_scan_to_cache = false;
+	bool nulls_supported = false;
+	bool fast_math = false;
+	//! Page-locked slots read or written by transfers on `stream`; released only after the final sync.
+	std::vector<PinnedBufferSlot> pinned;
+
+	void *Alloc(size_t bytes) {
+		return arena.Alloc(bytes == 0 ? 1 : bytes);
+	}
+};
+
+const DeviceColumn &FindColumn(const Columns &columns, const std::string &name) {
+	for (auto &column : columns) {
+		if (column.name == name) {
+			if (column.pruned) {
+				throw std::logic_error("gpu_executor: column '" + name + "' was pruned but is read");
+			}
+			return column;
+		}
+	}
+	throw std::runtime_error("gpu_executor: expression references column '" + name +
+	                         "' which is not produced by its input operator");
+}
+
+bool HasColumn(const Columns &columns, const std::string &name) {
+	for (auto &column : columns) {
+		if (column.name == name) {
+			return true;
+		}
+	}
+	return false;
+}
+
+//! Resolves an output name against the two inputs of a join or cross product (same errors as
+//! cuda_engine/src/gpu_executor.cu:116-141).
+const DeviceColumn &ResolveSideColumn(const std::string &name, const Columns &left, const Columns &right,
+                                      bool &from_left) {
+	const DeviceColumn *found = nullptr;
+	for (auto &column : left) {
+		if (column.name == name) {
+			found = &column;
+			from_left = true;
+			break;
+		}
+	}
+	for (auto &column : right) {
+		if (column.name == name) {
+			if (found != nullptr) {
+				throw std::runtime_error("gpu_executor: output column '" + name +
+				                         "' is produced by both inputs, so selecting it by name is ambiguous");
+			}
+			found = &column;
+			from_left = false;
+			break;
+		}
+	}
+	if (found == nullptr) {
+		throw std::runtime_error("gpu_executor: output column '" + name + "' is produced by neither input");
+	}
+	return *found;
+}
+
+bool Needed(const NameSet *need, const std::string &name) {
+	return need == nullptr || need->count(name) != 0;
+}
+
+void AddInputs(NameSet &set, const std::vector<GpuExpr> &exprs) {
+	for (auto &e : exprs) {
+		set.insert(e.input_columns.begin(), e.input_columns.end());
+	}
+}
+
+DeviceColumn PrunedLike(const DeviceColumn &column, uint64_t rows) {
+	DeviceColumn p;
+	p.name = column.name;
+	p.type = column.type;
+	p.rows = rows;
+	p.pruned = true;
+	return p;
+}
+
+DeviceColumn EmptyColumn(const std::string &name, GpuValueType type, ExecContext &ctx) {
+	DeviceColumn c;
+	c.name = name;
+	c.type = type;
+	c.rows = 0;
+	c.data = ctx.Alloc(0);
+	return c;
+}
+
+const char *CTypeName(GpuValueType type) {
+	switch (type) {
+	case GpuValueType::INT16:
+		return "short";
+	case GpuValueType::INT32:
+		return "int";
+	case GpuValueType::INT64:
+		return "long long";
+	case GpuValueType::FLOAT32:
+		return "float";
+	case GpuValueType::FLOAT64:
+		return "double";
+	case GpuValueType::BOOLEAN:
+		return "bool";
+	default:
+		return nullptr;
+	}
+}
+
+bool FusableType(GpuValueType type) {
+	return CTypeName(type) != nullptr;
+}
+
+//! The body that copies `column` unchanged, in the same text contract as the extension's expressions.
+GpuExpr IdentityExpr(const DeviceColumn &column) {
+	if (!FusableType(column.type)) {
+		throw std::runtime_error("gpu_executor: column '" + column.name +
+		                         "' has a type no generated expression can read");
+	}
+	GpuExpr e;
+	const std::string t = CTypeName(column.type);
+	e.generated_cuda_source = "((" + t + "*)out)[idx] = ((const " + t + "*)inputs[0])[row];";
+	e.input_columns = {column.name};
+	e.output_type = column.type;
+	return e;
+}
+
+// ------------------------------------------------------------------ host <-> device transfers
+
+//! Host to device through page-locked staging. Large transfers are split into pieces that alternate
+//! between two slots, so the host copy of one piece overlaps the DMA of the previous one.
+void UploadH2D(void *dst, const void *src, size_t bytes, ExecContext &ctx) {
+	constexpr size_t kPiece = 16u << 20;
+	if (bytes <= kPiece) {
+		auto slot = PinnedBufferPool::Instance().Acquire(bytes);
+		std::memcpy(slot.data(), src, bytes);
+		VGPU_HIP(hipMemcpyAsync(dst, slot.data(), bytes, hipMemcpyHostToDevice, ctx.stream));
+		ctx.pinned.push_back(std::move(slot));
+		return;
+	}
+	PinnedBufferSlot slots[2] = {PinnedBufferPool::Instance().Acquire(kPiece),
+	                             PinnedBufferPool::Instance().Acquire(kPiece)};
+	hipEvent_t done[2] = {nullptr, nullptr};
+	VGPU_HIP(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+	VGPU_HIP(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+	bool used[2] = {false, false};
+	try {
+		size_t offset = 0;
+		for (int piece = 0; offset < bytes; piece++) {
+			const int s = piece & 1;
+			const size_t len = std::min(kPiece, bytes - offset);
+			if (used[s]) {
+				VGPU_HIP(hipEventSynchronize(done[s]));
+			}
+			std::memcpy(slots[s].data(), static_cast<const uint8_t *>(src) + offset, len);
+			VGPU_HIP(hipMemcpyAsync(static_cast<uint8_t *>(dst) + offset, slots[s].data(), len, hipMemcpyHostToDevice,
+			                        ctx.stream));
+			VGPU_HIP(hipEventRecord(done[s], ctx.stream));
+			used[s] = true;
+			offset += len;
+		}
+	} catch (...) {
+		(void)hipEventDestroy(done[0]);
+		(void)hipEventDestroy(done[1]);
+		throw;
+	}
+	(void)hipEventDestroy(done[0]);
+	(void)hipEventDestroy(done[1]);
+	ctx.pinned.push_back(std::move(slots[0]));
+	ctx.pinned.push_back(std::move(slots[1]));
+}
+
+// ------------------------------------------------------------------ SCAN
+
+Columns ExecuteScan(const GpuPlanNode &node, ExecContext &ctx, const NameSet *need) {
+	Columns columns;
+	columns.reserve(node.column_names.size());
+
+	if (ctx.cache_chunk_index >= 0) {
+		const auto chunk_index = static_cast<uint64_t>(ctx.cache_chunk_index);
+		for (auto &column_name : node.column_names) {
+			GpuColumnChunk cached;
+			if (!GpuColumnCache::Instance().TryGetChunk(node.table, column_name, chunk_index, cached)) {
+				throw std::runtime_error("gpu_executor: chunk " + std::to_string(chunk_index) + " of column '" +
+				                         column_name + "' of table '" + node.table.ToString() +
+				                         "' was expected to be cache-resident but is no longer present "
+				                         "(likely invalidated by a concurrent write) -- retry the query");
+			}
+			if (cached.device_validity != nullptr && !ctx.nulls_supported) {
+				throw std::runtime_error("gpu_executor: cached column '" + column_name +
+				                         "' contains NULLs, which this operator does not model");
+			}
+			DeviceColumn column;
+			column.name = column_name;
+			column.type = cached.type;
+			column.rows = cached.row_count;
+			column.data = cached.device_data;
+			column.valid = static_cast<uint8_t *>(cached.device_validity);
+			column.pruned = !Needed(need, column_name);
+			VGPU_LOG(LogLevel::DEBUG, "cache_hit",
+			         "\"table\":" + GpuLogger::Quote(node.table.ToString()) + ",\"column\":" +
+			             GpuLogger::Quote(column_name) + ",\"chunk\":" + std::to_string(chunk_index) +
+			             ",\"rows\":" + std::to_string(column.rows));
+			columns.push_back(std::move(column));
+		}
+		return columns;
+	}
+
+	if (ctx.serve_whole_scan_from_cache) {
+		auto chunk_rows = GpuColumnCache::Instance().ReplayableChunkRows(node.table, node.column_names);
+		if (chunk_rows.empty()) {
+			throw std::runtime_error("gpu_executor: table '" + node.table.ToString() +
+			                         "' was expected to be fully cache-resident but is no longer present "
+			                         "(likely invalidated by a concurrent write) -- retry the query");
+		}
+		uint64_t total_rows = 0;
+		for (auto rows : chunk_rows) {
+			total_rows += rows;
+		}
+		for (auto &column_name : node.column_names) {
+			std::vector<GpuColumnChunk> chunks;
+			bool any_validity = false;
+			for (uint64_t index = 0; index < chunk_rows.size(); index++) {
+				GpuColumnChunk cached;
+				if (!GpuColumnCache::Instance().TryGetChunk(node.table, column_name, index, cached)) {
+					throw std::runtime_error("gpu_executor: chunk " + std::to_string(index) + " of column '" +
+					                         column_name + "' of table '" + node.table.ToString() +
+					                         "' disappeared from the cache mid-gather -- retry the query");
+				}
+				if (cached.device_validity != nullptr) {
+					if (!ctx.nulls_supported) {
+						throw std::runtime_error("gpu_executor: cached column '" + column_name +
+						                         "' contains NULLs, which this operator does not model");
+					}
+					any_validity = true;
+				}
+				chunks.push_back(cached);
+			}
+			const size_t width = TypeSize(chunks.front().type);
+			DeviceColumn column;
+			column.name = column_name;
+			column.type = chunks.front().type;
+			column.rows = total_rows;
+			if (!Needed(need, column_name)) {
+				column.pruned = true;
+				columns.push_back(std::move(column));
+				continue;
+			}
+			for (auto &cached : chunks) {
+				if (cached.data_bytes != cached.row_count * width) {
+					throw std::runtime_error("gpu_executor: cached chunk of column '" + column_name + "' holds " +
+					                         std::to_string(cached.data_bytes) + " bytes for " +
+					                         std::to_string(cached.row_count) +
+					                         " rows, which does not match its declared type");
+				}
+			}
+			if (chunks.size() == 1) {
+				// The whole column is one resident chunk: borrow it. Nothing evicts while a plan is served
+				// from the cache, and the borrow ends with this execution.
+				column.data = chunks[0].device_data;
+				column.valid = static_cast<uint8_t *>(chunks[0].device_validity);
+			} else {
+				column.data = ctx.Alloc(total_rows * width);
+				if (any_validity) {
+					column.valid = static_cast<uint8_t *>(ctx.Alloc(total_rows));
+					vgpu::FillU8(column.valid, 1, total_rows, ctx.stream);
+				}
+				uint64_t offset = 0;
+				for (auto &cached : chunks) {
+					VGPU_HIP(hipMemcpyAsync(static_cast<uint8_t *>(column.data) + offset * width, cached.device_data,
+					                        cached.row_count * width, hipMemcpyDeviceToDevice, ctx.stream));
+					if (any_validity && cached.device_validity != nullptr) {
+						VGPU_HIP(hipMemcpyAsync(column.valid + offset, cached.device_validity, cached.row_count,
+						                        hipMemcpyDeviceToDevice, ctx.stream));
+					}
+					offset += cached.row_count;
+				}
+			}
+			VGPU_LOG(LogLevel::INFO, "cache_gather",
+			         "\"table\":" + GpuLogger::Quote(node.table.ToString()) + ",\"column\":" +
+			             GpuLogger::Quote(column_name) + ",\"chunks\":" + std::to_string(chunks.size()) +
+			             ",\"rows\":" + std::to_string(total_rows));
+			columns.push_back(std::move(column));
+		}
+		return columns;
+	}
+
+	for (size_t i = 0; i < node.column_names.size(); i++) {
+		if (ctx.cursor >= ctx.inputs.size()) {
+			throw std::runtime_error("gpu_executor: plan expects more scanned columns than were supplied");
+		}
+		const GpuColumn &host = ctx.inputs[ctx.cursor++];
+		if (host.validity != nullptr && !ctx.nulls_supported) {
+			throw std::runtime_error("gpu_executor: column '" + host.name +
+			                         "' contains NULLs, which this operator does not model");
+		}
+		DeviceColumn column;
+		column.name = host.name;
+		column.type = host.type;
+		column.rows = host.row_count;
+		const size_t bytes = static_cast<size_t>(host.row_count) * TypeSize(host.type);
+		// Staging offers every column to the cache, read or not, so the cached table is complete.
+		if (!ctx.stage_scan_to_cache && !Needed(need, host.name)) {
+			column.pruned = true;
+			columns.push_back(std::move(column));
+			continue;
+		}
+		// Drawn from the pool directly rather than through the plan arena: the buffer may end up owned by
+		// the column cache. Exactly one owner results either way (the cache, or the arena adopting it).
+		column.data = vgpu::DevicePool::Instance().Alloc(bytes == 0 ? 1 : bytes, ctx.stream);
+		if (bytes > 0 && host.data != nullptr) {
+			auto start = std::chrono::steady_clock::now();
+			UploadH2D(column.data, host.data, bytes, ctx);
+			VGPU_LOG(LogLevel::INFO, "h2d",
+			         "\"column\":" + GpuLogger::Quote(host.name) + ",\"bytes\":" + std::to_string(bytes) + ",\"us\":" +
+			             std::to_string(std::chrono::duration_cast<std::chrono::microseconds>(
+			                                std::chrono::steady_clock::now() - start)
+			                                .count()));
+		}
+		uint64_t validity_bytes = 0;
+		if (host.validity != nullptr) {
+			const size_t packed_bytes = (static_cast<size_t>(host.row_count) + 7) / 8;
+			auto *packed = static_cast<uint8_t *>(ctx.Alloc(packed_bytes));
+			if (packed_bytes > 0) {
+				UploadH2D(packed, host.validity, packed_bytes, ctx);
+			}
+			validity_bytes = host.row_count;
+			column.valid = static_cast<uint8_t *>(
+			    vgpu::DevicePool::Instance().Alloc(validity_bytes == 0 ? 1 : validity_bytes, ctx.stream));
+			vgpu::UnpackValidity(packed, column.valid, host.row_count, ctx.stream);
+		}
+		bool cached = false;
+		if (ctx.stage_scan_to_cache) {
+			GpuColumnChunk chunk;
+			chunk.device_data = column.data;
+			chunk.device_validity = column.valid;
+			chunk.row_count = column.rows;
+			chunk.data_bytes = bytes;
+			chunk.validity_bytes = validity_bytes;
+			chunk.type = column.type;
+			cached = GpuColumnCache::Instance().StageChunk(node.table, host.name, chunk);
+		}
+		if (!cached) {
+			ctx.arena.Adopt(column.data);
+			if (column.valid != nullptr) {
+				ctx.arena.Adopt(column.valid);
+			}
+		}
+		column.pruned = !Needed(need, host.name);
+		columns.push_back(std::move(column));
+	}
+	return columns;
+}
+
+// ------------------------------------------------------------------ fused FILTER / PROJECTION kernels
+
+//! One output of a fused kernel: an expression, or (`copy_of`) an unchanged copy of a column whose type no
+//! generated kernel handles (HUGEINT), which is gathered instead.
+struct NamedExpr {
+	std::string name;
+	const GpuExpr *expr = nullptr;
+	const DeviceColumn *copy_of = nullptr;
+};
+
+//! Generated kernel(s) over dense `child` columns: AND of `preds` (a NULL predicate drops the row), then
+//! every output of `outs` over the surviving rows, written densely. An output whose inputs are all
+//! non-NULL carries no validity; otherwise its validity is the AND of the referenced inputs'. One kernel
+//! when the argument tables allow (16 columns, 16 outputs, every output an expression); otherwise one
+//! predicate kernel emits a row selection and the outputs are produced through it in groups.
+Columns RunFused(const Columns &child, uint64_t rows, const std::vector<GpuExpr> &preds,
+                 const std::vector<NamedExpr> &outs, ExecContext &ctx, uint64_t *kept_out = nullptr) {
+	std::vector<const DeviceColumn *> slot_cols;
+	std::map<std::string, int> slot_of;
+	auto resolve = [&](const GpuExpr &e) {
+		vgpu::FusedExpr f;
+		f.body = e.generated_cuda_source;
+		f.out_type = static_cast<int>(e.output_type);
+		for (auto &name : e.input_columns) {
+			const DeviceColumn &c = FindColumn(child, name);
+			if (c.rows != rows) {
+				throw std::runtime_error("gpu_executor: input column '" + name + "' row count mismatch");
+			}
+			auto it = slot_of.find(name);
+			int slot;
+			if (it == slot_of.end()) {
+				slot = static_cast<int>(slot_cols.size());
+				slot_of.emplace(name, slot);
+				slot_cols.push_back(&c);
+			} else {
+				slot = it->second;
+			}
+			f.slots.push_back(slot);
+		}
+		return f;
+	};
+	std::vector<vgpu::FusedExpr> predicates;
+	for (auto &p : preds) {
+		predicates.push_back(resolve(p));
+		if (p.output_type != GpuValueType::BOOLEAN) {
+			throw std::runtime_error("gpu_executor: FILTER predicate did not evaluate to BOOLEAN");
+		}
+	}
+	std::vector<vgpu::FusedExpr> projections(outs.size());
+	bool all_expr = true;
+	for (size_t e = 0; e < outs.size(); e++) {
+		if (outs[e].expr != nullptr) {
+			projections[e] = resolve(*outs[e].expr);
+		} else {
+			all_expr = false;
+		}
+	}
+
+	Columns result;
+	for (auto &o : outs) {
+		DeviceColumn c;
+		c.name = o.name;
+		c.type = o.expr != nullptr ? o.expr->output_type : o.copy_of->type;
+		result.push_back(std::move(c));
+	}
+	if (kept_out != nullptr) {
+		*kept_out = 0;
+	}
+	if (rows == 0) {
+		for (auto &c : result) {
+			c.data = ctx.Alloc(0);
+		}
+		return result;
+	}
+	for (auto *c : slot_cols) {
+		if (!FusableType(c->type)) {
+			throw std::runtime_error("gpu_executor: column '" + c->name +
+			                         "' has a type no generated expression can read");
+		}
+	}
+	for (auto &o : outs) {
+		if (o.expr != nullptr && !FusableType(o.expr->output_type)) {
+			throw std::runtime_error("gpu_executor: expression output type is not supported by generated kernels");
+		}
+	}
+	auto fused_col = [&](int slot) {
+		vgpu::FusedColumn fc;
+		fc.vtype = static_cast<int>(slot_cols[slot]->type);
+		fc.has_valid = slot_cols[slot]->valid != nullptr;
+		return fc;
+	};
+	auto nullable = [&](const vgpu::FusedExpr &f) {
+		for (int s : f.slots) {
+			if (slot_cols[s]->valid != nullptr) {
+				return true;
+			}
+		}
+		return false;
+	};
+	//! A kernel spec over a subset of the slots: re-slots `exprs` densely and returns the pointer tables.
+	struct SubSpec {
+		vgpu::FusedSpec spec;
+		std::vector<const void *> ptrs;
+		std::vector<const uint8_t *> valid;
+		std::map<int, int> remap;
+		void Add(std::vector<vgpu::FusedExpr> &dst, vgpu::FusedExpr f, const std::function<vgpu::FusedColumn(int)> &col,
+		         const std::vector<const DeviceColumn *> &cols) {
+			for (auto &s : f.slots) {
+				auto it = remap.find(s);
+				if (it == remap.end()) {
+					const int ns = static_cast<int>(spec.cols.size());
+					remap.emplace(s, ns);
+					spec.cols.push_back(col(s));
+					ptrs.push_back(cols[s]->data);
+					valid.push_back(cols[s]->valid);
+					s = ns;
+				} else {
+					s = it->second;
+				}
+			}
+			dst.push_back(std::move(f));
+		}
+		size_t NewCols(const vgpu::FusedExpr &f) const {
+			std::set<int> fresh;
+			for (int s : f.slots) {
+				if (!remap.count(s)) {
+					fresh.insert(s);
+				}
+			}
+			return fresh.size();
+		}
+	};
+
+	const bool filtering = !predicates.empty();
+	auto *count_dev = filtering ? static_cast<uint64_t *>(ctx.Alloc(8)) : nullptr;
+	const size_t state_bytes = filtering ? vgpu::FusedKernel::StateBytes(rows) : 0;
+	void *state = filtering ? ctx.Alloc(state_bytes) : nullptr;
+	uint64_t kept = rows;
+	auto read_kept = [&]() {
+		VGPU_HIP(hipMemcpyAsync(&kept, count_dev, 8, hipMemcpyDeviceToHost, ctx.stream));
+		VGPU_HIP(hipStreamSynchronize(ctx.stream));
+	};
+
+	if (all_expr && slot_cols.size() <= (size_t)vgpu::kFusedMaxCols && outs.size() <= (size_t)vgpu::kFusedMaxOuts) {
+		vgpu::FusedSpec spec;
+		spec.fast_math = ctx.fast_math;
+		std::vector<const void *> ptrs;
+		std::vector<const uint8_t *> valid;
+		for (size_t s = 0; s < slot_cols.size(); s++) {
+			spec.cols.push_back(fused_col(static_cast<int>(s)));
+			ptrs.push_back(slot_cols[s]->data);
+			valid.push_back(slot_cols[s]->valid);
+		}
+		spec.predicates = predicates;
+		spec.projections = projections;
+		auto kernel = vgpu::GetFusedKernel(spec);
+		std::vector<void *> out_ptrs;
+		std::vector<uint8_t *> out_valid;
+		for (size_t e = 0; e < outs.size(); e++) {
+			result[e].data = ctx.Alloc(rows * TypeSize(result[e].type));
+			if (nullable(projections[e])) {
+				result[e].valid = static_cast<uint8_t *>(ctx.Alloc(rows));
+			}
+			out_ptrs.push_back(result[e].data);
+			out_valid.push_back(result[e].valid);
+		}
+		vgpu::FusedLaunch la;
+		la.cols = ptrs.data();
+		la.valid = valid.data();
+		la.outs = out_ptrs.data();
+		la.out_valid = out_valid.data();
+		la.n = rows;
+		la.count_device = count_dev;
+		kernel->Launch(la, state, state_bytes, ctx.stream);
+		if (filtering) {
+			read_kept();
+		}
+		for (auto &c : result) {
+			c.rows = kept;
+		}
+		if (kept_out != nullptr) {
+			*kept_out = kept;
+		}
+		return result;
+	}
+
+	if (rows >= (1ull << 32)) {
+		throw std::runtime_error("gpu_executor: a FILTER/PROJECTION of this width is limited to 2^32 input rows");
+	}
+	const uint32_t *sel = nullptr;
+	if (filtering) {
+		SubSpec ps;
+		ps.spec.fast_math = ctx.fast_math;
+		ps.spec.emit_selection = true;
+		for (auto &f : predicates) {
+			ps.Add(ps.spec.predicates, f, fused_col, slot_cols);
+		}
+		if (ps.spec.cols.size() > (size_t)vgpu::kFusedMaxCols) {
+			throw std::runtime_error("gpu_executor: FILTER predicates read more than 16 columns");
+		}
+		auto *selection = static_cast<uint32_t *>(ctx.Alloc(rows * 4));
+		vgpu::FusedLaunch la;
+		la.cols = ps.ptrs.data();
+		la.valid = ps.valid.data();
+		la.n = rows;
+		la.sel_out = selection;
+		la.count_device = count_dev;
+		vgpu::GetFusedKernel(ps.spec)->Launch(la, state, state_bytes, ctx.stream);
+		read_kept();
+		sel = selection;
+	}
+	for (auto &c : result) {
+		c.rows = kept;
+		c.data = ctx.Alloc(kept * TypeSize(c.type));
+	}
+	if (kept_out != nullptr) {
+		*kept_out = kept;
+	}
+	if (kept == 0) {
+		return result;
+	}
+	// Unchanged copies of columns no generated kernel reads.
+	for (size_t e = 0; e < outs.size(); e++) {
+		const DeviceColumn *src = outs[e].copy_of;
+		if (src == nullptr) {
+			continue;
+		}
+		const int width = static_cast<int>(TypeSize(src->type));
+		if (sel != nullptr) {
+			vgpu::Gather(src->data, width, sel, false, kept, result[e].data, ctx.stream);
+		} else {
+			VGPU_HIP(hipMemcpyAsync(result[e].data, src->data, kept * width, hipMemcpyDeviceToDevice, ctx.stream));
+		}
+		if (src->valid != nullptr) {
+			result[e].valid = static_cast<uint8_t *>(ctx.Alloc(kept));
+			if (sel != nullptr) {
+				vgpu::Gather(src->valid, 1, sel, false, kept, result[e].valid, ctx.stream);
+			} else {
+				VGPU_HIP(hipMemcpyAsync(result[e].valid, src->valid, kept, hipMemcpyDeviceToDevice, ctx.stream));
+			}
+		}
+	}
+	// Expression outputs in groups that fit the argument tables.
+	for (size_t e = 0; e < outs.size();) {
+		if (outs[e].expr == nullptr) {
+			e++;
+			continue;
+		}
+		SubSpec os;
+		os.spec.fast_math = ctx.fast_math;
+		os.spec.selection_in = sel != nullptr;
+		std::vector<void *> out_ptrs;
+		std::vector<uint8_t *> out_valid;
+		for (; e < outs.size() && os.spec.projections.size() < (size_t)vgpu::kFusedMaxOuts; e++) {
+			if (outs[e].expr == nullptr) {
+				continue;
+			}
+			if (os.spec.cols.size() + os.NewCols(projections[e]) > (size_t)vgpu::kFusedMaxCols) {
+				if (os.spec.projections.empty()) {
+					throw std::runtime_error("gpu_executor: an expression reads more than 16 columns");
+				}
+				break;
+			}
+			os.Add(os.spec.projections, projections[e], fused_col, slot_cols);
+			if (nullable(projections[e])) {
+				result[e].valid = static_cast<uint8_t *>(ctx.Alloc(kept));
+			}
+			out_ptrs.push_back(result[e].data);
+			out_valid.push_back(result[e].valid);
+		}
+		vgpu::FusedLaunch la;
+		la.cols = os.ptrs.data();
+		la.valid = os.valid.data();
+		la.outs = out_ptrs.data();
+		la.out_valid = out_valid.data();
+		la.n = kept;
+		la.sel = sel;
+		vgpu::GetFusedKernel(os.spec)->Launch(la, nullptr, 0, ctx.stream);
+	}
+	return result;
+}
+
+// ------------------------------------------------------------------ plan walk
+
+Columns ExecuteNode(const GpuPlanNode &node, ExecContext &ctx, const NameSet *need);
+
+//! CROSS_PRODUCT, optionally with the FILTER (`preds`) and/or PROJECTION (`projection`) directly above it
+//! fused into the pair enumeration (cuda_engine/src/operators/gpu_cross_join.cu:111-192 for the
+//! operator's own checks and left-major order).
+Columns ExecuteCross(const GpuPlanNode &node, ExecContext &ctx, const NameSet *need, const std::vector<GpuExpr> *preds,
+                     const GpuPlanNode *projection) {
+	if (node.children.size() != 2) {
+		throw std::runtime_error("gpu_executor: CROSS_PRODUCT expects exactly two children");
+	}
+	// Names the fused consumer reads from the cross product's outputs.
+	NameSet reads;
+	if (projection != nullptr) {
+		AddInputs(reads, projection->expressions);
+	} else {
+		for (auto &name : node.output_columns) {
+			if (Needed(need, name)) {
+				reads.insert(name);
+			}
+		}
+	}
+	if (preds != nullptr) {
+		AddInputs(reads, *preds);
+	}
+	NameSet side_need;
+	for (auto &name : node.output_columns) {
+		if (reads.count(name)) {
+			side_need.insert(name);
+		}
+	}
+	Columns left = ExecuteNode(*node.children[0], ctx, &side_need);
+	Columns right = ExecuteNode(*node.children[1], ctx, &side_need);
+
+	if (node.output_columns.empty()) {
+		throw std::runtime_error("gpu_cross_join: node has no output columns");
+	}
+	if (left.empty() || right.empty()) {
+		throw std::runtime_error("gpu_cross_join: both inputs must produce at least one column");
+	}
+	std::vector<const DeviceColumn *> sources;
+	std::vector<bool> from_left;
+	for (auto &name : node.output_columns) {
+		bool l = false;
+		sources.push_back(&ResolveSideColumn(name, left, right, l));
+		from_left.push_back(l);
+	}
+	const uint64_t left_rows = left.front().rows;
+	const uint64_t right_rows = right.front().rows;
+	for (auto &c : left) {
+		if (c.rows != left_rows) {
+			throw std::runtime_error("gpu_cross_join: left input column '" + c.name + "' row count mismatch");
+		}
+	}
+	for (auto &c : right) {
+		if (c.rows != right_rows) {
+			throw std::runtime_error("gpu_cross_join: right input column '" + c.name + "' row count mismatch");
+		}
+	}
+	constexpr uint64_t kMaxCrossProductOutputRows = 1ull << 32;
+	uint64_t total = 0;
+	if (left_rows != 0 && right_rows != 0) {
+		if (left_rows > kMaxCrossProductOutputRows / right_rows) {
+			throw std::runtime_error("gpu_cross_join: cross product of " + std::to_string(left_rows) + " x " +
+			                         std::to_string(right_rows) + " rows is too large for GPU execution");
+		}
+		total = left_rows * right_rows;
+		if (total > kMaxCrossProductOutputRows) {
+			throw std::runtime_error("gpu_cross_join: cross product result of " + std::to_string(total) +
+			                         " rows is too large for GPU execution");
+		}
+	}
+
+	// The cross product's own output columns (what a FILTER or PROJECTION above it sees).
+	Columns cross_out;
+	for (size_t i = 0; i < sources.size(); i++) {
+		DeviceColumn c = *sources[i];
+		c.name = node.output_columns[i];
+		c.rows = total;
+		cross_out.push_back(c);
+	}
+
+	std::vector<NamedExpr> outs;
+	std::vector<GpuExpr> identity;
+	std::vector<std::string> proj_names;
+	if (projection != nullptr) {
+		for (size_t i = 0; i < projection->expressions.size(); i++) {
+			proj_names.push_back("col" + std::to_string(i));
+		}
+		for (size_t i = 0; i < projection->expressions.size(); i++) {
+			outs.push_back({proj_names[i], &projection->expressions[i]});
+		}
+	} else if (preds != nullptr) {
+		identity.reserve(cross_out.size());
+		for (auto &c : cross_out) {
+			if (!c.pruned && Needed(need, c.name)) {
+				identity.push_back(IdentityExpr(c));
+			}
+		}
+		size_t k = 0;
+		for (auto &c : cross_out) {
+			if (!c.pruned && Needed(need, c.name)) {
+				outs.push_back({c.name, &identity[k++]});
+			}
+		}
+	}
+
+	if (total == 0) {
+		if (projection != nullptr) {
+			Columns r;
+			for (size_t i = 0; i < projection->expressions.size(); i++) {
+				r.push_back(EmptyColumn(proj_names[i], projection->expressions[i].output_type, ctx));
+			}
+			return r;
+		}
+		Columns r;
+		for (auto &c : cross_out) {
+			r.push_back(c.pruned || !Needed(need, c.name) ? PrunedLike(c, 0) : EmptyColumn(c.name, c.type, ctx));
+		}
+		return r;
+	}
+
+	if (preds == nullptr && projection == nullptr) {
+		// Plain expansion, one write-bound pass per requested column.
+		Columns r;
+		for (size_t i = 0; i < cross_out.size(); i++) {
+			const DeviceColumn &c = cross_out[i];
+			if (c.pruned || !Needed(need, c.name)) {
+				r.push_back(PrunedLike(c, total));
+				continue;
+			}
+			DeviceColumn o;
+			o.name = c.name;
+			o.type = c.type;
+			o.rows = total;
+			o.data = ctx.Alloc(total * TypeSize(c.type));
+			vgpu::CrossExpand(c.data, static_cast<int>(TypeSize(c.type)), right_rows, from_left[i], total, o.data,
+			                  ctx.stream);
+			r.push_back(std::move(o));
+		}
+		return r;
+	}
+
+	// Fused pair enumeration: slot per distinct source column, tagged with its side.
+	vgpu::FusedSpec spec;
+	spec.cross = true;
+	spec.fast_math = ctx.fast_math;
+	std::vector<const void *> ptrs;
+	std::map<std::string, int> slot_of;
+	auto resolve = [&](const GpuExpr &e) {
+		vgpu::FusedExpr f;
+		f.body = e.generated_cuda_source;
+		f.out_type = static_cast<int>(e.output_type);
+		for (auto &name : e.input_columns) {
+			size_t idx = 0;
+			for (; idx < cross_out.size(); idx++) {
+				if (cross_out[idx].name == name) {
+					break;
+				}
+			}
+			if (idx == cross_out.size()) {
+				throw std::runtime_error("gpu_executor: expression references column '" + name +
+				                         "' which is not produced by its input operator");
+			}
+			const DeviceColumn &c = cross_out[idx];
+			if (!FusableType(c.type)) {
+				throw std::runtime_error("gpu_executor: column '" + name +
+				                         "' has a type no generated expression can read");
+			}
+			auto it = slot_of.find(name);
+			int slot;
+			if (it == slot_of.end()) {
+				slot = static_cast<int>(spec.cols.size());
+				slot_of.emplace(name, slot);
+				vgpu::FusedColumn fc;
+				fc.vtype = static_cast<int>(c.type);
+				fc.side = from_left[idx] ? 0 : 1;
+				spec.cols.push_back(fc);
+				ptrs.push_back(c.data);
+			} else {
+				slot = it->second;
+			}
+			f.slots.push_back(slot);
+		}
+		return f;
+	};
+	if (preds != nullptr) {
+		for (auto &p : *preds) {
+			auto f = resolve(p);
+			if (p.output_type != GpuValueType::BOOLEAN) {
+				throw std::runtime_error("gpu_executor: FILTER predicate did not evaluate to BOOLEAN");
+			}
+			spec.predicates.push_back(std::move(f));
+		}
+	}
+	for (auto &o : outs) {
+		if (!FusableType(o.expr->output_type)) {
+			throw std::runtime_error("gpu_executor: expression output type is not supported by generated kernels");
+		}
+		spec.projections.push_back(resolve(*o.expr));
+	}
+	if (spec.cols.size() > (size_t)vgpu::kFusedMaxCols || spec.projections.size() > (size_t)vgpu::kFusedMaxOuts) {
+		throw std::runtime_error("gpu_executor: a fused cross product reads or writes more than 16 columns");
+	}
+	const bool filtering = !spec.predicates.empty();
+	uint64_t row_bytes = 0;
+	for (auto &o : outs) {
+		row_bytes += TypeSize(o.expr->output_type);
+	}
+	auto *count_dev = filtering ? static_cast<uint64_t *>(ctx.Alloc(8)) : nullptr;
+	size_t state_bytes = filtering ? vgpu::FusedKernel::StateBytes(total) : 0;
+	void *state = filtering ? ctx.Alloc(state_bytes) : nullptr;
+
+	uint64_t capacity = total;
+	if (filtering && row_bytes > 0 && total * row_bytes > (8ull << 30)) {
+		// Outputs at the full pair count would be large: count the survivors first.
+		vgpu::FusedSpec cspec = spec;
+		cspec.projections.clear();
+		auto ck = vgpu::GetFusedKernel(cspec);
+		vgpu::FusedLaunch la;
+		la.cols = ptrs.data();
+		la.n = total;
+		la.right_rows = right_rows;
+		la.count_device = count_dev;
+		ck->Launch(la, state, state_bytes, ctx.stream);
+		VGPU_HIP(hipMemcpyAsync(&capacity, count_dev, 8, hipMemcpyDeviceToHost, ctx.stream));
+		VGPU_HIP(hipStreamSynchronize(ctx.stream));
+	}
+	Columns r;
+	std::vector<void *> out_ptrs;
+	for (auto &o : outs) {
+		DeviceColumn c;
+		c.name = o.name;
+		c.type = o.expr->output_type;
+		c.data = ctx.Alloc(capacity * TypeSize(c.type));
+		out_ptrs.push_back(c.data);
+		r.push_back(std::move(c));
+	}
+	auto kernel = vgpu::GetFusedKernel(spec);
+	vgpu::FusedLaunch la;
+	la.cols = ptrs.data();
+	la.outs = out_ptrs.data();
+	la.n = total;
+	la.right_rows = right_rows;
+	la.count_device = count_dev;
+	la.capacity = capacity;
+	kernel->Launch(la, state, state_bytes, ctx.stream);
+	uint64_t kept = total;
+	if (filtering) {
+		VGPU_HIP(hipMemcpyAsync(&kept, count_dev, 8, hipMemcpyDeviceToHost, ctx.stream));
+		VGPU_HIP(hipStreamSynchronize(ctx.stream));
+		if (kept > capacity) {
+			throw std::logic_error("gpu_executor: filtered cross product exceeded its counted capacity");
+		}
+	}
+	for (auto &c : r) {
+		c.rows = kept;
+	}
+	if (projection == nullptr) {
+		// FILTER output: every cross output column, the unread ones pruned.
+		Columns full;
+		size_t k = 0;
+		for (auto &c : cross_out) {
+			if (!c.pruned && Needed(need, c.name)) {
+				full.push_back(r[k++]);
+			} else {
+				full.push_back(PrunedLike(c, kept));
+			}
+		}
+		return full;
+	}
+	return r;
+}
+
+//! FILTER, optionally fused with the PROJECTION directly above it (`projection`).
+Columns ExecuteFilter(const GpuPlanNode &node, ExecContext &ctx, const NameSet *need, const GpuPlanNode *projection) {
+	if (node.children.size() != 1) {
+		throw std::runtime_error("gpu_executor: FILTER expects exactly one child");
+	}
+	const GpuPlanNode &child = *node.children[0];
+	if (child.op_type == GpuOpType::CROSS_PRODUCT && !node.expressions.empty()) {
+		return ExecuteCross(child, ctx, need, &node.expressions, projection);
+	}
+	NameSet child_need;
+	const NameSet *child_need_ptr = nullptr;
+	if (projection != nullptr) {
+		AddInputs(child_need, projection->expressions);
+		AddInputs(child_need, node.expressions);
+		child_need_ptr = &child_need;
+	} else if (need != nullptr) {
+		child_need = *need;
+		AddInputs(child_need, node.expressions);
+		child_need_ptr = &child_need;
+	}
+	Columns cols = ExecuteNode(child, ctx, child_need_ptr);
+	const uint64_t rows = cols.empty() ? 0 : cols.front().rows;
+	const bool filter_applies = !cols.empty() && !node.expressions.empty() && rows != 0;
+	static const std::vector<GpuExpr> kNoPredicates;
+
+	if (projection != nullptr) {
+		std::vector<NamedExpr> outs;
+		for (size_t i = 0; i < projection->expressions.size(); i++) {
+			outs.push_back({"col" + std::to_string(i), &projection->expressions[i]});
+		}
+		return RunFused(cols, rows, filter_applies ? node.expressions : kNoPredicates, outs, ctx);
+	}
+	if (!filter_applies) {
+		return cols;
+	}
+	std::vector<GpuExpr> identity;
+	identity.reserve(cols.size());
+	std::vector<NamedExpr> outs;
+	for (auto &c : cols) {
+		if (c.pruned || !Needed(need, c.name)) {
+			continue;
+		}
+		if (FusableType(c.type)) {
+			identity.push_back(IdentityExpr(c));
+			outs.push_back({c.name, &identity.back(), nullptr});
+		} else {
+			outs.push_back({c.name, nullptr, &c});
+		}
+	}
+	uint64_t kept = 0;
+	Columns carried = RunFused(cols, rows, node.expressions, outs, ctx, &kept);
+	Columns result;
+	size_t j = 0;
+	for (auto &c : cols) {
+		if (!c.pruned && Needed(need, c.name)) {
+			result.push_back(std::move(carried[j++]));
+		} else {
+			result.push_back(PrunedLike(c, kept));
+		}
+	}
+	return result;
+}
+
+Columns ExecuteProjection(const GpuPlanNode &node, ExecContext &ctx) {
+	if (node.children.size() != 1) {
+		throw std::runtime_error("gpu_executor: PROJECTION expects exactly one child");
+	}
+	const GpuPlanNode &child = *node.children[0];
+	if (child.op_type == GpuOpType::FILTER) {
+		return ExecuteFilter(child, ctx, nullptr, &node);
+	}
+	if (child.op_type == GpuOpType::CROSS_PRODUCT) {
+		return ExecuteCross(child, ctx, nullptr, nullptr, &node);
+	}
+	NameSet need;
+	AddInputs(need, node.expressions);
+	Columns cols = ExecuteNode(child, ctx, &need);
+	const uint64_t rows = cols.empty() ? 0 : cols.front().rows;
+	std::vector<NamedExpr> outs;
+	for (size_t i = 0; i < node.expressions.size(); i++) {
+		outs.push_back({"col" + std::to_string(i), &node.expressions[i]});
+	}
+	return RunFused(cols, rows, {}, outs, ctx);
+}
+
+// ------------------------------------------------------------------ GROUP BY
+
+void CheckGroupKey(const DeviceColumn &column, const std::string &key_name, uint64_t rows) {
+	if (column.rows != rows) {
+		throw std::runtime_error("gpu_groupby: group key '" + key_name + "' row count mismatch");
+	}
+	switch (column.type) {
+	case GpuValueType::INT16:
+	case GpuValueType::INT32:
+	case GpuValueType::INT64:
+	case GpuValueType::BOOLEAN:
+		return;
+	default:
+		throw std::runtime_error("gpu_groupby: group key '" + key_name +
+		                         "' is not an integer/boolean column, which this GPU path requires");
+	}
+}
+
+[[noreturn]] void ThrowGlobalEmpty() {
+	throw std::runtime_error("gpu_groupby: global aggregate over an empty input needs NULL results, "
+	                         "which this GPU path cannot represent");
+}
+
+void CheckGroupByShape(const GpuPlanNode &node) {
+	if (node.group_keys.empty() && node.aggregates.empty()) {
+		throw std::runtime_error("gpu_groupby: node has neither group keys nor aggregates");
+	}
+	if (node.aggregates.size() != node.aggregate_exprs.size() && !node.aggregate_exprs.empty()) {
+		throw std::runtime_error("gpu_groupby: aggregates and aggregate_exprs disagree in length");
+	}
+}
+
+//! The reference's aggregate rules (cuda_engine/src/operators/gpu_groupby.cu:224-274) with one
+//! extension: SUM over an integer column produces HUGEINT, DuckDB's own result type (the reference refuses
+//! it). MIN/MAX stay integer/boolean only, SUM over floating input produces DOUBLE.
+void ValidateEngineAggregate(const GpuAggregate &aggregate, GpuValueType input) {
+	const bool is_float = input == GpuValueType::FLOAT32 || input == GpuValueType::FLOAT64;
+	const bool is_int = input == GpuValueType::INT16 || input == GpuValueType::INT32 || input == GpuValueType::INT64;
+	switch (aggregate.kind) {
+	case GpuAggregateKind::SUM:
+		if (is_int) {
+			if (aggregate.output_type != GpuValueType::HUGEINT) {
+				throw std::runtime_error("gpu_groupby: SUM over an integer column must produce HUGEINT (DuckDB's "
+				                         "result type)");
+			}
+			return;
+		}
+		if (!is_float || aggregate.output_type != GpuValueType::FLOAT64) {
+			throw std::runtime_error("gpu_groupby: SUM on this GPU path supports only FLOAT/DOUBLE input producing "
+			                         "DOUBLE, or integer input producing HUGEINT");
+		}
+		return;
+	case GpuAggregateKind::MIN:
+	case GpuAggregateKind::MAX:
+		if (aggregate.output_type != input) {
+			throw std::runtime_error("gpu_groupby: MIN/MAX output type must match its input column type");
+		}
+		if (!is_int && input != GpuValueType::BOOLEAN) {
+			throw std::runtime_error("gpu_groupby: MIN/MAX on this GPU path supports only integer/boolean columns");
+		}
+		return;
+	default:
+		return;
+	}
+}
+
+//! Aggregate requests over `value_cols` (the distinct aggregate inputs, resolved in `cols`), validated
+//! with the GROUP BY rules.
+std::vector<vgpu::AggReq> BuildAggRequests(const GpuPlanNode &node, const Columns &cols, uint64_t rows,
+                                           std::vector<vgpu::DevCol> &values) {
+	std::vector<vgpu::AggReq> reqs;
+	std::map<std::string, int> value_of;
+	for (auto &aggregate : node.aggregates) {
+		vgpu::AggReq r;
+		r.kind = static_cast<int>(aggregate.kind);
+		r.out_type = static_cast<int>(aggregate.output_type);
+		if (aggregate.kind != GpuAggregateKind::COUNT_STAR) {
+			const DeviceColumn &c = FindColumn(cols, aggregate.input_column);
+			if (c.rows != rows) {
+				throw std::runtime_error("gpu_groupby: aggregate input column '" + aggregate.input_column +
+				                         "' row count mismatch");
+			}
+			auto it = value_of.find(aggregate.input_column);
+			if (it == value_of.end()) {
+				vgpu::DevCol d;
+				d.data = c.data;
+				d.type = static_cast<int>(c.type);
+				d.rows = c.rows;
+				it = value_of.emplace(aggregate.input_column, static_cast<int>(values.size())).first;
+				values.push_back(d);
+			}
+			r.input = it->second;
+			ValidateEngineAggregate(aggregate, c.type);
+		}
+		vgpu::ValidateAggregateReq(r, values);
+		reqs.push_back(r);
+	}
+	return reqs;
+}
+
+Columns ExecuteJoinAggregate(const GpuPlanNode &node, const GpuPlanNode &join, ExecContext &ctx);
+
+bool JoinAggregateFusable(const GpuPlanNode &node) {
+	if (!node.group_keys.empty() || node.aggregates.empty() || node.children.size() != 1 ||
+	    node.children[0]->op_type != GpuOpType::HASH_JOIN) {
+		return false;
+	}
+	for (auto &a : node.aggregates) {
+		// MIN/MAX over a join run on 8-byte states only; the other widths take the materializing path.
+		if (a.kind == GpuAggregateKind::MIN || a.kind == GpuAggregateKind::MAX) {
+			if (a.output_type != GpuValueType::INT64 && a.output_type != GpuValueType::FLOAT64) {
+				return false;
+			}
+		}
+	}
+	return node.aggregates.size() < 8;
+}
+
+Columns ExecuteGroupBy(const GpuPlanNode &node, ExecContext &ctx) {
+	if (node.children.size() != 1) {
+		throw std::runtime_error("gpu_executor: GROUP_BY_AGGREGATE expects exactly one child");
+	}
+	if (JoinAggregateFusable(node)) {
+		return ExecuteJoinAggregate(node, *node.children[0], ctx);
+	}
+	NameSet need(node.group_keys.begin(), node.group_keys.end());
+	for (auto &a : node.aggregates) {
+		if (a.kind != GpuAggregateKind::COUNT_STAR) {
+			need.insert(a.input_column);
+		}
+	}
+	Columns cols = ExecuteNode(*node.children[0], ctx, &need);
+	if (cols.empty()) {
+		throw std::runtime_error("gpu_groupby: input operator produced no columns to group");
+	}
+	CheckGroupByShape(node);
+	const uint64_t rows = cols.front().rows;
+	std::vector<vgpu::DevCol> keys;
+	std::vector<const DeviceColumn *> key_cols;
+	for (auto &key_name : node.group_keys) {
+		const DeviceColumn &c = FindColumn(cols, key_name);
+		CheckGroupKey(c, key_name, rows);
+		vgpu::DevCol d;
+		d.data = c.data;
+		d.type = static_cast<int>(c.type);
+		d.rows = c.rows;
+		keys.push_back(d);
+		key_cols.push_back(&c);
+	}
+	std::vector<vgpu::DevCol> values;
+	auto reqs = BuildAggRequests(node, cols, rows, values);
+
+	if (rows == 0) {
+		if (node.group_keys.empty()) {
+			ThrowGlobalEmpty();
+		}
+		Columns r;
+		for (size_t k = 0; k < key_cols.size(); k++) {
+			r.push_back(EmptyColumn(node.group_keys[k], key_cols[k]->type, ctx));
+		}
+		for (size_t i = 0; i < node.aggregates.size(); i++) {
+			r.push_back(EmptyColumn("agg_" + std::to_string(i), node.aggregates[i].output_type, ctx));
+		}
+		return r;
+	}
+
+	vgpu::KeyPack pack;
+	const uint32_t *hist8 = nullptr;
+	const uint64_t bound = vgpu::PlanGroupBy(keys, rows, pack, ctx.arena, ctx.stream, &hist8);
+	vgpu::SortTuning tuning = vgpu::TuneSort(pack.total_bits, rows);
+	tuning.hist8 = hist8;
+	std::vector<void *> key_out, agg_out;
+	vgpu::OutAlloc alloc = [&](size_t bytes) { return ctx.Alloc(bytes); };
+	const uint64_t groups = vgpu::GroupByDevice(keys, values, reqs, rows, pack, std::max<uint64_t>(bound, 1), alloc,
+	                                            key_out, agg_out, tuning, ctx.arena, ctx.stream, true);
+	if (groups == 0) {
+		throw std::runtime_error("gpu_groupby: computed zero groups for a non-empty input");
+	}
+	Columns r;
+	for (size_t k = 0; k < key_cols.size(); k++) {
+		DeviceColumn c;
+		c.name = node.group_keys[k];
+		c.type = key_cols[k]->type;
+		c.rows = groups;
+		c.data = key_out[k];
+		r.push_back(std::move(c));
+	}
+	for (size_t i = 0; i < node.aggregates.size(); i++) {
+		DeviceColumn c;
+		c.name = "agg_" + std::to_string(i);
+		c.type = node.aggregates[i].output_type;
+		c.rows = groups;
+		c.data = agg_out[i];
+		r.push_back(std::move(c));
+	}
+	return r;
+}
+
+// ------------------------------------------------------------------ HASH_JOIN
+
+void CheckJoinKeyType(const DeviceColumn &column, const char *side) {
+	switch (column.type) {
+	case GpuValueType::INT16:
+	case GpuValueType::INT32:
+	case GpuValueType::INT64:
+	case GpuValueType::BOOLEAN:
+		return;
+	default:
+		throw std::runtime_error(std::string("gpu_hash_join: ") + side +
+		                         " join key is not an integer/boolean column, which this GPU path requires");
+	}
+}
+
+struct JoinInputs {
+	Columns left, right;
+	const DeviceColumn *left_key = nullptr;
+	const DeviceColumn *right_key = nullptr;
+	std::vector<const DeviceColumn *> sources;
+	std::vector<bool> from_left;
+};
+
+//! Executes both join inputs and applies the operator's checks in the reference's order
+//! (cuda_engine/src/operators/gpu_hash_join.cu:138-176).
+JoinInputs PrepareJoin(const GpuPlanNode &node, ExecContext &ctx, const NameSet &reads) {
+	if (node.children.size() != 2) {
+		throw std::runtime_error("gpu_executor: HASH_JOIN expects exactly two children");
+	}
+	NameSet left_need = reads, right_need = reads;
+	left_need.insert(node.build_keys.begin(), node.build_keys.end());
+	right_need.insert(node.probe_keys.begin(), node.probe_keys.end());
+	JoinInputs j;
+	j.left = ExecuteNode(*node.children[0], ctx, &left_need);
+	j.right = ExecuteNode(*node.children[1], ctx, &right_need);
+	if (node.build_keys.size() != 1 || node.probe_keys.size() != 1) {
+		throw std::runtime_error("gpu_hash_join: this GPU path joins on exactly one equality key column");
+	}
+	if (node.output_columns.empty()) {
+		throw std::runtime_error("gpu_hash_join: node has no output columns");
+	}
+	j.left_key = &FindColumn(j.left, node.build_keys[0]);
+	j.right_key = &FindColumn(j.right, node.probe_keys[0]);
+	CheckJoinKeyType(*j.left_key, "left");
+	CheckJoinKeyType(*j.right_key, "right");
+	if (j.left_key->type != j.right_key->type) {
+		throw std::runtime_error("gpu_hash_join: join key columns must have the same type on both sides");
+	}
+	for (auto &name : node.output_columns) {
+		bool l = false;
+		j.sources.push_back(&ResolveSideColumn(name, j.left, j.right, l));
+		j.from_left.push_back(l);
+	}
+	return j;
+}
+
+vgpu::DevCol ToDev(const DeviceColumn &c) {
+	vgpu::DevCol d;
+	d.data = c.data;
+	d.type = static_cast<int>(c.type);
+	d.rows = c.rows;
+	return d;
+}
+
+Columns ExecuteJoin(const GpuPlanNode &node, ExecContext &ctx, const NameSet *need) {
+	NameSet reads;
+	for (auto &name : node.output_columns) {
+		if (Needed(need, name)) {
+			reads.insert(name);
+		}
+	}
+	JoinInputs j = PrepareJoin(node, ctx, reads);
+	const uint64_t left_rows = j.left_key->rows;
+	const uint64_t right_rows = j.right_key->rows;
+	auto shaped = [&](uint64_t rows) {
+		Columns r;
+		for (size_t i = 0; i < j.sources.size(); i++) {
+			if (rows == 0 && !j.sources[i]->pruned && Needed(need, node.output_columns[i])) {
+				r.push_back(EmptyColumn(node.output_columns[i], j.sources[i]->type, ctx));
+			} else {
+				DeviceColumn p = PrunedLike(*j.sources[i], rows);
+				p.name = node.output_columns[i];
+				r.push_back(p);
+			}
+		}
+		return r;
+	};
+	if (left_rows == 0 || right_rows == 0) {
+		return shaped(0);
+	}
+	uint32_t *lidx = nullptr, *ridx = nullptr;
+	const uint64_t total =
+	    vgpu::JoinMaterializeDevice(ToDev(*j.left_key), ToDev(*j.right_key), &lidx, &ridx, ctx.arena, ctx.stream);
+	if (total == 0) {
+		return shaped(0);
+	}
+	Columns r = shaped(total);
+	for (size_t i = 0; i < j.sources.size(); i++) {
+		const DeviceColumn &src = *j.sources[i];
+		if (src.pruned || !Needed(need, node.output_columns[i])) {
+			continue;
+		}
+		DeviceColumn o;
+		o.name = node.output_columns[i];
+		o.type = src.type;
+		o.rows = total;
+		o.pruned = false;
+		o.data = ctx.Alloc(total * TypeSize(src.type));
+		vgpu::Gather(src.data, static_cast<int>(TypeSize(src.type)), j.from_left[i] ? lidx : ridx, false, total, o.data,
+		             ctx.stream);
+		r[i] = std::move(o);
+	}
+	return r;
+}
+
+//! GROUP BY without keys directly over an INNER join: per-side sort + run reduction, merge of the
+//! distinct keys, exact combination of the per-key states. No pair is materialized. Semantics are those of
+//! materializing the join and aggregating it (including the refusal of an empty result).
+Columns ExecuteJoinAggregate(const GpuPlanNode &node, const GpuPlanNode &join, ExecContext &ctx) {
+	NameSet reads;
+	for (auto &a : node.aggregates) {
+		if (a.kind != GpuAggregateKind::COUNT_STAR) {
+			reads.insert(a.input_column);
+		}
+	}
+	JoinInputs j = PrepareJoin(join, ctx, reads);
+	CheckGroupByShape(node);
+	// The join's output columns, as the aggregate sees them.
+	Columns join_out;
+	for (size_t i = 0; i < j.sources.size(); i++) {
+		DeviceColumn c = *j.sources[i];
+		c.name = join.output_columns[i];
+		join_out.push_back(c);
+	}
+	std::vector<vgpu::DevCol> side_vals[2];
+	std::vector<vgpu::JoinAggReq> reqs;
+	std::vector<vgpu::DevCol> check_values;
+	for (auto &a : node.aggregates) {
+		vgpu::JoinAggReq r;
+		r.kind = static_cast<int>(a.kind);
+		r.out_type = static_cast<int>(a.output_type);
+		vgpu::AggReq check;
+		check.kind = r.kind;
+		check.out_type = r.out_type;
+		if (a.kind != GpuAggregateKind::COUNT_STAR) {
+			size_t idx = 0;
+			for (; idx < join_out.size() && join_out[idx].name != a.input_column; idx++) {
+			}
+			if (idx == join_out.size()) {
+				throw std::runtime_error("gpu_executor: expression references column '" + a.input_column +
+				                         "' which is not produced by its input operator");
+			}
+			const DeviceColumn &src = *j.sources[idx];
+			const int side = j.from_left[idx] ? 0 : 1;
+			r.side = side;
+			r.input = static_cast<int>(side_vals[side].size());
+			side_vals[side].push_back(ToDev(src));
+			check.input = static_cast<int>(check_values.size());
+			check_values.push_back(ToDev(src));
+			ValidateEngineAggregate(a, src.type);
+		}
+		vgpu::ValidateAggregateReq(check, check_values);
+		reqs.push_back(r);
+	}
+	if (j.left_key->rows == 0 || j.right_key->rows == 0) {
+		ThrowGlobalEmpty();
+	}
+	const size_t count_slot = reqs.size();
+	reqs.push_back(vgpu::JoinAggReq {vgpu::AGG_COUNT_STAR, -1, -1, vgpu::VT_INT64});
+	auto vals = vgpu::JoinAggregateDevice(ToDev(*j.left_key), side_vals[0], ToDev(*j.right_key), side_vals[1], reqs,
+	                                      ctx.arena, ctx.stream);
+	const uint64_t pairs = vals[count_slot].lo;
+	if (pairs == 0 && vals[count_slot].hi == 0) {
+		ThrowGlobalEmpty();
+	}
+	auto slot = PinnedBufferPool::Instance().Acquire(16 * node.aggregates.size());
+	auto *host = static_cast<uint8_t *>(slot.data());
+	std::memset(host, 0, 16 * node.aggregates.size());
+	Columns r;
+	for (size_t i = 0; i < node.aggregates.size(); i++) {
+		vgpu::FinishJoinAggregate(vals[i], reqs[i].kind, reqs[i].out_type, pairs, host + 16 * i);
+		DeviceColumn c;
+		c.name = "agg_" + std::to_string(i);
+		c.type = node.aggregates[i].output_type;
+		c.rows = 1;
+		c.data = ctx.Alloc(TypeSize(c.type));
+		VGPU_HIP(hipMemcpyAsync(c.data, host + 16 * i, TypeSize(c.type), hipMemcpyHostToDevice, ctx.stream));
+		r.push_back(std::move(c));
+	}
+	ctx.pinned.push_back(std::move(slot));
+	return r;
+}
+
+Columns ExecuteNode(const GpuPlanNode &node, ExecContext &ctx, const NameSet *need) {
+	switch (node.op_type) {
+	case GpuOpType::SCAN:
+		return ExecuteScan(node, ctx, need);
+	case GpuOpType::FILTER:
+		return ExecuteFilter(node, ctx, need, nullptr);
+	case GpuOpType::PROJECTION:
+		return ExecuteProjection(node, ctx);
+	case GpuOpType::GROUP_BY_AGGREGATE:
+		return ExecuteGroupBy(node, ctx);
+	case GpuOpType::HASH_JOIN:
+		return ExecuteJoin(node, ctx, need);
+	case GpuOpType::CROSS_PRODUCT:
+		return ExecuteCross(node, ctx, need, nullptr, nullptr);
+	default:
+		throw std::runtime_error("gpu_executor: unknown GpuOpType value " +
+		                         std::to_string(static_cast<int>(node.op_type)));
+	}
+}
+
+hipStream_t SharedStream() {
+	static hipStream_t stream = [] {
+		hipStream_t s = nullptr;
+		hipError_t status = hipStreamCreate(&s);
+		if (status != hipSuccess) {
+			(void)hipGetLastError();
+			throw std::runtime_error(std::string("gpu_executor: hipStreamCreate failed: ") + hipGetErrorString(status));
+		}
+		return s;
+	}();
+	return stream;
+}
+
+//! Waits for the stream before anything the plan staged on it (page-locked slots) is released.
+struct StreamDrain {
+	hipStream_t stream;
+	~StreamDrain() {
+		(void)hipStreamSynchronize(stream);
+	}
+};
+
+std::mutex &PlanMutex() {
+	static std::mutex m;
+	return m;
+}
+
+} // namespace
+
+bool ExecuteGpuPlan(const GpuPlanNode &plan, const std::vector<GpuColumn> &inputs, const GpuExecuteOptions &options,
+                    GpuExecutionResult &out) {
+	try {
+		std::lock_guard<std::mutex> plan_lock(PlanMutex());
+		hipStream_t stream = SharedStream();
+		vgpu::PoolArena arena(stream);
+		ExecContext ctx {inputs, 0, arena, stream};
+		StreamDrain drain {stream};
+		const bool serving_from_cache = options.cache_chunk_index >= 0 || options.serve_whole_scan_from_cache;
+		if (serving_from_cache && options.stage_scan_to_cache) {
+			throw std::runtime_error("gpu_executor: cache replay and cache staging cannot be requested together");
+		}
+		if (options.cache_chunk_index >= 0 && options.serve_whole_scan_from_cache) {
+			throw std::runtime_error("gpu_executor: cannot serve one cached chunk and the whole scan at once");
+		}
+		if (serving_from_cache && !inputs.empty()) {
+			throw std::runtime_error("gpu_executor: cache replay expects no host inputs");
+		}
+		ctx.cache_chunk_index = options.cache_chunk_index;
+		ctx.serve_whole_scan_from_cache = options.serve_whole_scan_from_cache;
+		ctx.stage_scan_to_cache = options.stage_scan_to_cache;
+		ctx.nulls_supported = IsRowIndependent(plan);
+		ctx.fast_math = IsGpuFastMathEnabled();
+
+		Columns columns = ExecuteNode(plan, ctx, nullptr);
+
+		// Egress: data and (device-packed) validity into page-locked slots, one sync, then into `out`.
+		struct Pending {
+			size_t data_slot = SIZE_MAX;
+			size_t valid_slot = SIZE_MAX;
+			size_t data_bytes = 0;
+			size_t packed_bytes = 0;
+		};
+		std::vector<Pending> pending(columns.size());
+		std::vector<PinnedBufferSlot> slots;
+		for (size_t i = 0; i < columns.size(); i++) {
+			const DeviceColumn &c = columns[i];
+			const size_t bytes = static_cast<size_t>(c.rows) * TypeSize(c.type);
+			pending[i].data_bytes = bytes;
+			auto start = std::chrono::steady_clock::now();
+			if (bytes > 0) {
+				slots.push_back(PinnedBufferPool::Instance().Acquire(bytes));
+				VGPU_HIP(hipMemcpyAsync(slots.back().data(), c.data, bytes, hipMemcpyDeviceToHost, stream));
+				pending[i].data_slot = slots.size() - 1;
+			}
+			if (c.valid != nullptr && c.rows > 0) {
+				const size_t packed_bytes = (static_cast<size_t>(c.rows) + 7) / 8;
+				auto *packed = static_cast<uint8_t *>(ctx.Alloc(packed_bytes));
+				vgpu::PackValidity(c.valid, packed, c.rows, stream);
+				slots.push_back(PinnedBufferPool::Instance().Acquire(packed_bytes));
+				VGPU_HIP(hipMemcpyAsync(slots.back().data(), packed, packed_bytes, hipMemcpyDeviceToHost, stream));
+				pending[i].valid_slot = slots.size() - 1;
+				pending[i].packed_bytes = packed_bytes;
+			}
+			VGPU_LOG(LogLevel::INFO, "d2h",
+			         "\"column\":" + GpuLogger::Quote(c.name) + ",\"bytes\":" + std::to_string(bytes) + ",\"us\":" +
+			             std::to_string(std::chrono::duration_cast<std::chrono::microseconds>(
+			                                std::chrono::steady_clock::now() - start)
+			                                .count()));
+		}
+		VGPU_HIP(hipStreamSynchronize(stream));
+
+		// Every owned buffer is created before any pointer into one is taken.
+		out.owned_buffers.reserve(columns.size() * 2);
+		std::vector<size_t> data_index(columns.size()), valid_index(columns.size(), SIZE_MAX);
+		for (size_t i = 0; i < columns.size(); i++) {
+			out.owned_buffers.emplace_back(pending[i].data_bytes);
+			data_index[i] = out.owned_buffers.size() - 1;
+			if (pending[i].valid_slot != SIZE_MAX) {
+				out.owned_buffers.emplace_back(pending[i].packed_bytes);
+				valid_index[i] = out.owned_buffers.size() - 1;
+			}
+		}
+		for (size_t i = 0; i < columns.size(); i++) {
+			auto &data = out.owned_buffers[data_index[i]];
+			if (pending[i].data_slot != SIZE_MAX) {
+				std::memcpy(data.data(), slots[pending[i].data_slot].data(), pending[i].data_bytes);
+			}
+			GpuColumn column;
+			column.name = columns[i].name;
+			column.type = columns[i].type;
+			column.row_count = columns[i].rows;
+			column.data = data.empty() ? nullptr : data.data();
+			if (valid_index[i] != SIZE_MAX) {
+				auto &packed = out.owned_buffers[valid_index[i]];
+				std::memcpy(packed.data(), slots[pending[i].valid_slot].data(), pending[i].packed_bytes);
+				column.validity = packed.data();
+			}
+			out.columns.push_back(std::move(column));
+		}
+		out.success = true;
+		return true;
+	} catch (const std::exception &e) {
+		out.success = false;
+		out.columns.clear();
+		out.owned_buffers.clear();
+		out.error_message = e.what();
+		return false;
+	}
+}
+
+namespace {
+int64_t FreeDeviceMiB() {
+	size_t free_bytes = 0, total_bytes = 0;
+	if (hipMemGetInfo(&free_bytes, &total_bytes) != hipSuccess) {
+		(void)hipGetLastError();
+		return -1;
+	}
+	return static_cast<int64_t>(free_bytes >> 20);
+}
+constexpr int64_t kLowFreeMiB = 128;
+} // namespace
+
+GpuEngine &GpuEngine::Instance() {
+	static GpuEngine instance;
+	return instance;
+}
+
+GpuExecutionResult GpuEngine::ExecutePlan(const GpuPlanNode &plan, const std::vector<GpuColumn> &inputs,
+                                          const GpuExecuteOptions &options) {
+	GpuExecutionResult result;
+	try {
+		RmmPool::Instance().EnsureInitialized();
+	} catch (const std::exception &e) {
+		result.success = false;
+		result.error_message = std::string("GpuEngine::ExecutePlan: GPU memory pool initialization failed "
+		                                   "(is a GPU device available?): ") +
+		                       e.what();
+		return result;
+	}
+	const bool watermark = GpuLogger::Instance().Enabled(LogLevel::INFO);
+	const int64_t before = watermark ? FreeDeviceMiB() : -1;
+	ExecuteGpuPlan(plan, inputs, options, result);
+	if (watermark) {
+		const int64_t after = FreeDeviceMiB();
+		if (before >= 0 && after >= 0) {
+			const bool low = before < kLowFreeMiB || after < kLowFreeMiB;
+			VGPU_LOG(LogLevel::INFO, "vram_watermark",
+			         "\"free_before_mib\":" + std::to_string(before) + ",\"free_after_mib\":" + std::to_string(after) +
+			             ",\"paging_risk\":" + (low ? "true" : "false") + ",\"threshold_mib\":" +
+			             std::to_string(kLowFreeMiB));
+		}
+	}
+	return result;
+}
+
+} // namespace vector_gpu

@@ -218,7 +218,11 @@ void Gather(const void *src, int width, const void *idx, bool idx64, uint64_t n,
 //! Dense byte validity from an Arrow LSB-first bitmap.
 void UnpackValidity(const uint8_t *packed, uint8_t *out, uint64_t n, hipStream_t stream);
 
+//! Arrow LSB-first bitmap ((n + 7) / 8 bytes) from dense byte validity.
+void PackValidity(const uint8_t *dense, uint8_t *packed, uint64_t n, hipStream_t stream);
+
 //! Zero-fill / value-fill without hipMemset (safe inside captured graphs).
+void FillU8(uint8_t *dst, uint8_t value, uint64_t n, hipStream_t stream);
 void FillU32(uint32_t *dst, uint32_t value, uint64_t n, hipStream_t stream);
 void FillU64(uint64_t *dst, uint64_t value, uint64_t n, hipStream_t stream);
 

@@ -225,6 +225,48 @@ void UnpackValidity(const uint8_t *packed, uint8_t *out, uint64_t n, hipStream_t
 	VGPU_HIP(hipGetLastError());
 }
 
+namespace {
+//! One thread per row: a wave's 64 validity bytes become one 64-bit ballot, i.e. 8 packed bytes, written
+//! by the lanes that own them (lane 8k writes byte k), so the packed output is written once, coalesced.
+__global__ void PackValidityKernel(const uint8_t *dense, uint8_t *packed, uint64_t n) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	const int lane = threadIdx.x & 63;
+	for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+		const uint64_t i = base + threadIdx.x;
+		const bool v = i < n && dense[i] != 0;
+		const u64 bits = __ballot(v);
+		if ((lane & 7) == 0 && i < n) {
+			packed[i >> 3] = (uint8_t)(bits >> lane);
+		}
+	}
+}
+
+__global__ void FillU8Kernel(uint8_t *dst, uint8_t value, uint64_t n) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		dst[i] = value;
+	}
+}
+} // namespace
+
+void PackValidity(const uint8_t *dense, uint8_t *packed, uint64_t n, hipStream_t stream) {
+	if (n == 0) {
+		return;
+	}
+	// Block size is a multiple of 64 and the grid stride is a multiple of the block, so every wave's
+	// 64 rows start on a byte boundary of the packed output (row index multiple of 64).
+	hipLaunchKernelGGL(PackValidityKernel, dim3(GridFor(n)), dim3(256), 0, stream, dense, packed, n);
+	VGPU_HIP(hipGetLastError());
+}
+
+void FillU8(uint8_t *dst, uint8_t value, uint64_t n, hipStream_t stream) {
+	if (n == 0) {
+		return;
+	}
+	hipLaunchKernelGGL(FillU8Kernel, dim3(GridFor(n)), dim3(256), 0, stream, dst, value, n);
+	VGPU_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ cross product expansion
 
 namespace {

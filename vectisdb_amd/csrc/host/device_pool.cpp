@@ -175,6 +175,12 @@ void *PoolArena::Alloc(size_t bytes) {
 	return p;
 }
 
+void PoolArena::Adopt(void *ptr) {
+	if (ptr != nullptr) {
+		blocks_.push_back(ptr);
+	}
+}
+
 void PoolArena::Release(void *ptr) {
 	auto it = std::find(blocks_.begin(), blocks_.end(), ptr);
 	if (it != blocks_.end()) {

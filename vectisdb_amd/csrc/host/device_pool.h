@@ -77,6 +77,9 @@ public:
 	void *Alloc(size_t bytes) override;
 	//! Hands ownership of `ptr` (allocated by this arena) to the caller; it is no longer freed here.
 	void Release(void *ptr);
+	//! Takes ownership of `ptr`, a block obtained from DevicePool::Alloc outside this arena; it is returned
+	//! to the pool with the arena's other blocks.
+	void Adopt(void *ptr);
 	hipStream_t stream() const {
 		return stream_;
 	}

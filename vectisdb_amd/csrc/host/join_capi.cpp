@@ -3,31 +3,11 @@
 #include "capi_util.h"
 #include "device_pool.h"
 
-#include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <string>
 
 using namespace vgpu;
-
-namespace {
-
-double DdToDouble(uint64_t hi_bits, uint64_t lo_bits) {
-	double hi, lo;
-	std::memcpy(&hi, &hi_bits, 8);
-	std::memcpy(&lo, &lo_bits, 8);
-	if (!std::isfinite(hi)) {
-		return hi;
-	}
-	return hi + lo;
-}
-
-double I128ToDoubleHost(uint64_t lo, uint64_t hi) {
-	__int128 v = (__int128)(((unsigned __int128)hi << 64) | lo);
-	return (double)v;
-}
-
-} // namespace
 
 extern "C" {
 
@@ -56,41 +36,9 @@ int vgpu_join_aggregate(const vgpu_column *akey, const vgpu_column *avals, int32
 		                                ToDevCols(bvals, nbvals), reqs, arena, s);
 		auto *out = static_cast<uint8_t *>(results);
 		std::memset(out, 0, (size_t)naggs * 16);
+		const uint64_t pairs = need_count ? vals[count_slot].lo : 0;
 		for (int32_t q = 0; q < naggs; q++) {
-			const auto &v = vals[q];
-			const int ot = aggs[q].output_type;
-			uint8_t *dst = out + 16 * q;
-			double d = 0;
-			switch (aggs[q].kind) {
-			case AGG_COUNT_STAR:
-			case AGG_COUNT:
-				std::memcpy(dst, &v.lo, 8);
-				break;
-			case AGG_SUM:
-				if (v.kind == JA_SUM_F64) {
-					d = DdToDouble(v.lo, v.hi);
-					std::memcpy(dst, &d, 8);
-				} else if (ot == VT_HUGEINT) {
-					std::memcpy(dst, &v.lo, 8);
-					std::memcpy(dst + 8, &v.hi, 8);
-				} else if (ot == VT_FLOAT64) {
-					d = I128ToDoubleHost(v.lo, v.hi);
-					std::memcpy(dst, &d, 8);
-				} else {
-					std::memcpy(dst, &v.lo, 8);
-				}
-				break;
-			case AGG_AVG: {
-				double sum = v.kind == JA_SUM_F64 ? DdToDouble(v.lo, v.hi) : I128ToDoubleHost(v.lo, v.hi);
-				double cnt = (double)vals[count_slot].lo;
-				d = sum / cnt;
-				std::memcpy(dst, &d, 8);
-				break;
-			}
-			default: // MIN / MAX
-				std::memcpy(dst, &v.lo, 8);
-				break;
-			}
+			FinishJoinAggregate(vals[q], aggs[q].kind, aggs[q].output_type, pairs, out + 16 * q);
 		}
 	});
 }

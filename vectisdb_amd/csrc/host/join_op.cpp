@@ -7,6 +7,8 @@
 #include "operators.h"
 
 #include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 
@@ -86,7 +88,52 @@ SideRuns ReduceSide(const DevCol &key, const std::vector<DevCol> &values, const 
 	return s;
 }
 
+//! The double-double state keeps the high double in `lo` and the low-order correction in `hi`.
+double DdStateToDouble(const JoinAggValue &v) {
+	double head, tail;
+	std::memcpy(&head, &v.lo, 8);
+	std::memcpy(&tail, &v.hi, 8);
+	return std::isfinite(head) ? head + tail : head;
+}
+
+double I128StateToDouble(const JoinAggValue &v) {
+	__int128 x = (__int128)(((unsigned __int128)v.hi << 64) | v.lo);
+	return (double)x;
+}
+
 } // namespace
+
+void FinishJoinAggregate(const JoinAggValue &v, int agg_kind, int out_type, uint64_t pairs, void *dst) {
+	auto *out = static_cast<uint8_t *>(dst);
+	double d = 0;
+	switch (agg_kind) {
+	case AGG_COUNT_STAR:
+	case AGG_COUNT:
+		std::memcpy(out, &v.lo, 8);
+		return;
+	case AGG_SUM:
+		if (v.kind == JA_SUM_F64) {
+			d = DdStateToDouble(v);
+			std::memcpy(out, &d, 8);
+		} else if (out_type == VT_HUGEINT) {
+			std::memcpy(out, &v.lo, 8);
+			std::memcpy(out + 8, &v.hi, 8);
+		} else if (out_type == VT_FLOAT64) {
+			d = I128StateToDouble(v);
+			std::memcpy(out, &d, 8);
+		} else {
+			std::memcpy(out, &v.lo, 8);
+		}
+		return;
+	case AGG_AVG:
+		d = (v.kind == JA_SUM_F64 ? DdStateToDouble(v) : I128StateToDouble(v)) / (double)pairs;
+		std::memcpy(out, &d, 8);
+		return;
+	default: // MIN / MAX keep the 8-byte input value
+		std::memcpy(out, &v.lo, 8);
+		return;
+	}
+}
 
 std::vector<JoinAggValue> JoinAggregateDevice(const DevCol &akey, const std::vector<DevCol> &avals, const DevCol &bkey,
                                               const std::vector<DevCol> &bvals, const std::vector<JoinAggReq> &aggs,

@@ -70,6 +70,11 @@ std::vector<JoinAggValue> JoinAggregateDevice(const DevCol &akey, const std::vec
                                               const std::vector<DevCol> &bvals, const std::vector<JoinAggReq> &aggs,
                                               Scratch &scratch, hipStream_t stream);
 
+//! Writes one aggregate-over-join result as its output type (TypeWidth(out_type) bytes at `dst`): COUNT as
+//! INT64, integer SUM as HUGEINT / INT64 (low word) / DOUBLE, floating SUM as DOUBLE (double-double rounded
+//! once), AVG as SUM / `pairs`, MIN/MAX as the input's own 8-byte value.
+void FinishJoinAggregate(const JoinAggValue &v, int agg_kind, int out_type, uint64_t pairs, void *dst);
+
 //! Materialized INNER equi-join: returns the number of output rows and, in *left_rows / *right_rows (from
 //! `scratch`), the source row of each side per output row, ordered by key, then left row, then right row.
 //! Synchronizes. Throws past 2^32 output rows.
