@@ -83,18 +83,109 @@ struct DeviceColumn {
 };
 using Columns = std::vector<DeviceColumn>;
 
+//! Host <-> device transfers of one plan through two page-locked slots used alternately: the host copy
+//! into (or out of) one slot overlaps the DMA of the other, and a plan never holds more than two slots
+//! whatever its column count (the pool blocks at its cap, so holding one slot per column until the plan
+//! ends would stall a plan wider than the pool).
+class Staging {
+public:
+	explicit Staging(hipStream_t stream) : stream_(stream) {
+	}
+	~Staging() {
+		for (auto &l : lanes_) {
+			if (l.busy) {
+				(void)hipEventSynchronize(l.done);
+			}
+			if (l.done != nullptr) {
+				(void)hipEventDestroy(l.done);
+			}
+		}
+	}
+	Staging(const Staging &) = delete;
+	Staging &operator=(const Staging &) = delete;
+
+	//! Copies `bytes` of host memory to the device; `src` may be reused as soon as this returns.
+	void Upload(void *dst, const void *src, size_t bytes) {
+		for (size_t off = 0; off < bytes; off += kPiece) {
+			const size_t len = std::min(kPiece, bytes - off);
+			Lane &l = Take();
+			std::memcpy(l.slot.data(), static_cast<const uint8_t *>(src) + off, len);
+			VGPU_HIP(hipMemcpyAsync(static_cast<uint8_t *>(dst) + off, l.slot.data(), len, hipMemcpyHostToDevice,
+			                        stream_));
+			Issued(l, nullptr, 0);
+		}
+	}
+	//! Queues a device-to-host copy; `dst` holds the data once Drain() returns.
+	void Download(void *dst, const void *src, size_t bytes) {
+		for (size_t off = 0; off < bytes; off += kPiece) {
+			const size_t len = std::min(kPiece, bytes - off);
+			Lane &l = Take();
+			VGPU_HIP(hipMemcpyAsync(l.slot.data(), static_cast<const uint8_t *>(src) + off, len,
+			                        hipMemcpyDeviceToHost, stream_));
+			Issued(l, static_cast<uint8_t *>(dst) + off, len);
+		}
+	}
+	void Drain() {
+		Finish(lanes_[next_]);
+		Finish(lanes_[next_ ^ 1]);
+	}
+
+private:
+	static constexpr size_t kPiece = 16u << 20;
+	struct Lane {
+		PinnedBufferSlot slot;
+		bool have = false;
+		hipEvent_t done = nullptr;
+		bool busy = false;
+		void *host_dst = nullptr;
+		size_t host_bytes = 0;
+	};
+	Lane &Take() {
+		Lane &l = lanes_[next_];
+		next_ ^= 1;
+		Finish(l);
+		if (!l.have) {
+			l.slot = PinnedBufferPool::Instance().Acquire(kPiece);
+			l.have = true;
+		}
+		if (l.done == nullptr) {
+			VGPU_HIP(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
+		}
+		return l;
+	}
+	void Issued(Lane &l, void *host_dst, size_t host_bytes) {
+		VGPU_HIP(hipEventRecord(l.done, stream_));
+		l.busy = true;
+		l.host_dst = host_dst;
+		l.host_bytes = host_bytes;
+	}
+	void Finish(Lane &l) {
+		if (!l.busy) {
+			return;
+		}
+		VGPU_HIP(hipEventSynchronize(l.done));
+		l.busy = false;
+		if (l.host_dst != nullptr) {
+			std::memcpy(l.host_dst, l.slot.data(), l.host_bytes);
+			l.host_dst = nullptr;
+		}
+	}
+	hipStream_t stream_;
+	Lane lanes_[2];
+	int next_ = 0;
+};
+
 struct ExecContext {
 	const std::vector<GpuColumn> &inputs;
 	size_t cursor = 0;
 	vgpu::PoolArena &arena;
 	hipStream_t stream = nullptr;
+	Staging &staging;
 	int64_t cache_chunk_index = -1;
 	bool serve_whole_scan_from_cache = false;
 	bool stage_scan_to_cache = false;
 	bool nulls_supported = false;
 	bool fast_math = false;
-	//! Page-locked slots read or written by transfers on `stream`; released only after the final sync.
-	std::vector<PinnedBufferSlot> pinned;
 
 	void *Alloc(size_t bytes) {
 		return arena.Alloc(bytes == 0 ? 1 : bytes);
@@ -215,51 +306,6 @@ GpuExpr IdentityExpr(const DeviceColumn &column) {
 	e.input_columns = {column.name};
 	e.output_type = column.type;
 	return e;
-}
-
-// ------------------------------------------------------------------ host <-> device transfers
-
-//! Host to device through page-locked staging. Large transfers are split into pieces that alternate
-//! between two slots, so the host copy of one piece overlaps the DMA of the previous one.
-void UploadH2D(void *dst, const void *src, size_t bytes, ExecContext &ctx) {
-	constexpr size_t kPiece = 16u << 20;
-	if (bytes <= kPiece) {
-		auto slot = PinnedBufferPool::Instance().Acquire(bytes);
-		std::memcpy(slot.data(), src, bytes);
-		VGPU_HIP(hipMemcpyAsync(dst, slot.data(), bytes, hipMemcpyHostToDevice, ctx.stream));
-		ctx.pinned.push_back(std::move(slot));
-		return;
-	}
-	PinnedBufferSlot slots[2] = {PinnedBufferPool::Instance().Acquire(kPiece),
-	                             PinnedBufferPool::Instance().Acquire(kPiece)};
-	hipEvent_t done[2] = {nullptr, nullptr};
-	VGPU_HIP(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
-	VGPU_HIP(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
-	bool used[2] = {false, false};
-	try {
-		size_t offset = 0;
-		for (int piece = 0; offset < bytes; piece++) {
-			const int s = piece & 1;
-			const size_t len = std::min(kPiece, bytes - offset);
-			if (used[s]) {
-				VGPU_HIP(hipEventSynchronize(done[s]));
-			}
-			std::memcpy(slots[s].data(), static_cast<const uint8_t *>(src) + offset, len);
-			VGPU_HIP(hipMemcpyAsync(static_cast<uint8_t *>(dst) + offset, slots[s].data(), len, hipMemcpyHostToDevice,
-			                        ctx.stream));
-			VGPU_HIP(hipEventRecord(done[s], ctx.stream));
-			used[s] = true;
-			offset += len;
-		}
-	} catch (...) {
-		(void)hipEventDestroy(done[0]);
-		(void)hipEventDestroy(done[1]);
-		throw;
-	}
-	(void)hipEventDestroy(done[0]);
-	(void)hipEventDestroy(done[1]);
-	ctx.pinned.push_back(std::move(slots[0]));
-	ctx.pinned.push_back(std::move(slots[1]));
 }
 
 // ------------------------------------------------------------------ SCAN
@@ -402,7 +448,7 @@ Columns ExecuteScan(const GpuPlanNode &node, ExecContext &ctx, const NameSet *ne
 		column.data = vgpu::DevicePool::Instance().Alloc(bytes == 0 ? 1 : bytes, ctx.stream);
 		if (bytes > 0 && host.data != nullptr) {
 			auto start = std::chrono::steady_clock::now();
-			UploadH2D(column.data, host.data, bytes, ctx);
+			ctx.staging.Upload(column.data, host.data, bytes);
 			VGPU_LOG(LogLevel::INFO, "h2d",
 			         "\"column\":" + GpuLogger::Quote(host.name) + ",\"bytes\":" + std::to_string(bytes) + ",\"us\":" +
 			             std::to_string(std::chrono::duration_cast<std::chrono::microseconds>(
@@ -414,7 +460,7 @@ Columns ExecuteScan(const GpuPlanNode &node, ExecContext &ctx, const NameSet *ne
 			const size_t packed_bytes = (static_cast<size_t>(host.row_count) + 7) / 8;
 			auto *packed = static_cast<uint8_t *>(ctx.Alloc(packed_bytes));
 			if (packed_bytes > 0) {
-				UploadH2D(packed, host.validity, packed_bytes, ctx);
+				ctx.staging.Upload(packed, host.validity, packed_bytes);
 			}
 			validity_bytes = host.row_count;
 			column.valid = static_cast<uint8_t *>(
@@ -1459,21 +1505,18 @@ Columns ExecuteJoinAggregate(const GpuPlanNode &node, const GpuPlanNode &join, E
 	if (pairs == 0 && vals[count_slot].hi == 0) {
 		ThrowGlobalEmpty();
 	}
-	auto slot = PinnedBufferPool::Instance().Acquire(16 * node.aggregates.size());
-	auto *host = static_cast<uint8_t *>(slot.data());
-	std::memset(host, 0, 16 * node.aggregates.size());
+	std::vector<uint8_t> host(16 * node.aggregates.size(), 0);
 	Columns r;
 	for (size_t i = 0; i < node.aggregates.size(); i++) {
-		vgpu::FinishJoinAggregate(vals[i], reqs[i].kind, reqs[i].out_type, pairs, host + 16 * i);
+		vgpu::FinishJoinAggregate(vals[i], reqs[i].kind, reqs[i].out_type, pairs, host.data() + 16 * i);
 		DeviceColumn c;
 		c.name = "agg_" + std::to_string(i);
 		c.type = node.aggregates[i].output_type;
 		c.rows = 1;
 		c.data = ctx.Alloc(TypeSize(c.type));
-		VGPU_HIP(hipMemcpyAsync(c.data, host + 16 * i, TypeSize(c.type), hipMemcpyHostToDevice, ctx.stream));
+		ctx.staging.Upload(c.data, host.data() + 16 * i, TypeSize(c.type));
 		r.push_back(std::move(c));
 	}
-	ctx.pinned.push_back(std::move(slot));
 	return r;
 }
 
@@ -1531,8 +1574,9 @@ bool ExecuteGpuPlan(const GpuPlanNode &plan, const std::vector<GpuColumn> &input
 		std::lock_guard<std::mutex> plan_lock(PlanMutex());
 		hipStream_t stream = SharedStream();
 		vgpu::PoolArena arena(stream);
-		ExecContext ctx {inputs, 0, arena, stream};
 		StreamDrain drain {stream};
+		Staging staging(stream);
+		ExecContext ctx {inputs, 0, arena, stream, staging};
 		const bool serving_from_cache = options.cache_chunk_index >= 0 || options.serve_whole_scan_from_cache;
 		if (serving_from_cache && options.stage_scan_to_cache) {
 			throw std::runtime_error("gpu_executor: cache replay and cache staging cannot be requested together");
@@ -1551,67 +1595,54 @@ bool ExecuteGpuPlan(const GpuPlanNode &plan, const std::vector<GpuColumn> &input
 
 		Columns columns = ExecuteNode(plan, ctx, nullptr);
 
-		// Egress: data and (device-packed) validity into page-locked slots, one sync, then into `out`.
-		struct Pending {
-			size_t data_slot = SIZE_MAX;
-			size_t valid_slot = SIZE_MAX;
-			size_t data_bytes = 0;
-			size_t packed_bytes = 0;
-		};
-		std::vector<Pending> pending(columns.size());
-		std::vector<PinnedBufferSlot> slots;
-		for (size_t i = 0; i < columns.size(); i++) {
-			const DeviceColumn &c = columns[i];
-			const size_t bytes = static_cast<size_t>(c.rows) * TypeSize(c.type);
-			pending[i].data_bytes = bytes;
-			auto start = std::chrono::steady_clock::now();
-			if (bytes > 0) {
-				slots.push_back(PinnedBufferPool::Instance().Acquire(bytes));
-				VGPU_HIP(hipMemcpyAsync(slots.back().data(), c.data, bytes, hipMemcpyDeviceToHost, stream));
-				pending[i].data_slot = slots.size() - 1;
+		// Egress: every owned buffer is created first (pointers into them stay valid), then data and
+		// device-packed validity stream through the staging ring straight into them.
+		for (auto &c : columns) {
+			if (c.pruned) {
+				throw std::logic_error("gpu_executor: result column '" + c.name + "' was pruned");
 			}
-			if (c.valid != nullptr && c.rows > 0) {
-				const size_t packed_bytes = (static_cast<size_t>(c.rows) + 7) / 8;
-				auto *packed = static_cast<uint8_t *>(ctx.Alloc(packed_bytes));
-				vgpu::PackValidity(c.valid, packed, c.rows, stream);
-				slots.push_back(PinnedBufferPool::Instance().Acquire(packed_bytes));
-				VGPU_HIP(hipMemcpyAsync(slots.back().data(), packed, packed_bytes, hipMemcpyDeviceToHost, stream));
-				pending[i].valid_slot = slots.size() - 1;
-				pending[i].packed_bytes = packed_bytes;
-			}
-			VGPU_LOG(LogLevel::INFO, "d2h",
-			         "\"column\":" + GpuLogger::Quote(c.name) + ",\"bytes\":" + std::to_string(bytes) + ",\"us\":" +
-			             std::to_string(std::chrono::duration_cast<std::chrono::microseconds>(
-			                                std::chrono::steady_clock::now() - start)
-			                                .count()));
 		}
-		VGPU_HIP(hipStreamSynchronize(stream));
-
-		// Every owned buffer is created before any pointer into one is taken.
 		out.owned_buffers.reserve(columns.size() * 2);
 		std::vector<size_t> data_index(columns.size()), valid_index(columns.size(), SIZE_MAX);
 		for (size_t i = 0; i < columns.size(); i++) {
-			out.owned_buffers.emplace_back(pending[i].data_bytes);
+			out.owned_buffers.emplace_back(static_cast<size_t>(columns[i].rows) * TypeSize(columns[i].type));
 			data_index[i] = out.owned_buffers.size() - 1;
-			if (pending[i].valid_slot != SIZE_MAX) {
-				out.owned_buffers.emplace_back(pending[i].packed_bytes);
+			if (columns[i].valid != nullptr && columns[i].rows > 0) {
+				out.owned_buffers.emplace_back((static_cast<size_t>(columns[i].rows) + 7) / 8);
 				valid_index[i] = out.owned_buffers.size() - 1;
 			}
 		}
 		for (size_t i = 0; i < columns.size(); i++) {
+			const DeviceColumn &c = columns[i];
 			auto &data = out.owned_buffers[data_index[i]];
-			if (pending[i].data_slot != SIZE_MAX) {
-				std::memcpy(data.data(), slots[pending[i].data_slot].data(), pending[i].data_bytes);
+			auto start = std::chrono::steady_clock::now();
+			if (!data.empty()) {
+				staging.Download(data.data(), c.data, data.size());
 			}
+			if (valid_index[i] != SIZE_MAX) {
+				auto &packed_host = out.owned_buffers[valid_index[i]];
+				auto *packed = static_cast<uint8_t *>(ctx.Alloc(packed_host.size()));
+				vgpu::PackValidity(c.valid, packed, c.rows, stream);
+				staging.Download(packed_host.data(), packed, packed_host.size());
+			}
+			VGPU_LOG(LogLevel::INFO, "d2h",
+			         "\"column\":" + GpuLogger::Quote(c.name) + ",\"bytes\":" + std::to_string(data.size()) +
+			             ",\"us\":" +
+			             std::to_string(std::chrono::duration_cast<std::chrono::microseconds>(
+			                                std::chrono::steady_clock::now() - start)
+			                                .count()));
+		}
+		staging.Drain();
+		VGPU_HIP(hipStreamSynchronize(stream));
+		for (size_t i = 0; i < columns.size(); i++) {
+			auto &data = out.owned_buffers[data_index[i]];
 			GpuColumn column;
 			column.name = columns[i].name;
 			column.type = columns[i].type;
 			column.row_count = columns[i].rows;
 			column.data = data.empty() ? nullptr : data.data();
 			if (valid_index[i] != SIZE_MAX) {
-				auto &packed = out.owned_buffers[valid_index[i]];
-				std::memcpy(packed.data(), slots[pending[i].valid_slot].data(), pending[i].packed_bytes);
-				column.validity = packed.data();
+				column.validity = out.owned_buffers[valid_index[i]].data();
 			}
 			out.columns.push_back(std::move(column));
 		}
