@@ -92,7 +92,7 @@ static void TestFilterAllNoneSomePass() {
 	CHECK(count == 0);
 	std::vector<int32_t> some = {5, -3, 10, 0, 7, -1, 100};
 	CHECK(RunFilter(some, 4, count) == CpuFilter(some, 4));
-	CHECK(count == 3);
+	CHECK(count == CpuFilter(some, 4).size());
 }
 
 static void TestFilterSingleElementsAndBoundary() {
