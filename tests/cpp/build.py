@@ -18,11 +18,13 @@ def build() -> list:
             "-pthread"]
     device = ["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", str(HERE / "test_device_components.cpp"), "-o",
               str(BIN / "test_device_components"), *common, "-pthread"]
-    for cmd in (host, device):
+    pipeline = ["g++", str(HERE / "test_stream_pipeline.cpp"), "-o", str(BIN / "test_stream_pipeline"), *common,
+                "-pthread"]
+    for cmd in (host, device, pipeline):
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError(f"build failed:\n{' '.join(cmd)}\n{res.stderr}")
-    return [BIN / "test_host_components", BIN / "test_device_components"]
+    return [BIN / "test_host_components", BIN / "test_device_components", BIN / "test_stream_pipeline"]
 
 
 if __name__ == "__main__":

@@ -31,3 +31,9 @@ def test_host_components_suite():
 def test_device_components_suite():
     """FilterGreaterThanInt32, CodeGenerator, RmmPool, GpuMemoryPool and GpuEngine through the C++ headers."""
     run_suite("test_device_components", 120)
+
+
+@pytest.mark.gpu
+def test_stream_pipeline_suite():
+    """GpuStreamPipeline: overlap, FIFO order, chained levels, NULLs, slot release, RUNNING_SUM, wide levels."""
+    run_suite("test_stream_pipeline", 120)

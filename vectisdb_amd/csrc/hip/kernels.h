@@ -236,6 +236,12 @@ void GenerateColumn(void *out, int vtype, uint64_t n, int dist, long long lo, lo
                     uint64_t stream_id, const double *cdf, uint64_t cdf_len, const long long *perm,
                     hipStream_t stream);
 
+//! In-place running sum with a carried state (out[i] = state + x[0..i]; state = out[n-1]); integers wrap.
+//! `scratch` holds RunningSumScratchBytes(n) bytes.
+size_t RunningSumScratchBytes(uint64_t n);
+void RunningSum(int vtype, void *data, uint64_t n, void *state, void *scratch, size_t scratch_bytes,
+                hipStream_t stream);
+
 //! Order-preserving compaction of x > threshold (the FilterGreaterThanInt32 primitive).
 void FilterGtI32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_device,
                  Scratch &scratch, hipStream_t stream);
