@@ -136,6 +136,35 @@ int vgpu_join_aggregate(const vgpu_column *akey, const vgpu_column *avals, int32
 int vgpu_join_pairs(const vgpu_column *akey, const vgpu_column *bkey, uint32_t *a_rows, uint32_t *b_rows,
                     uint64_t capacity, uint64_t *count, void *stream);
 
+/* ---------------------------------------------------------------- batched form
+ * Many `filter + GROUP BY` queries, each over its own row group of one device-resident table, in one
+ * pipeline: row group g is rows [segment_offsets[g], segment_offsets[g+1]) (device, nsegments + 1 entries).
+ * Every query applies the same predicates (inputs index `cols`) and groups by column `key_col` with the
+ * aggregates `aggs` (inputs index `value_cols`). Outputs (device, `capacity` rows each): segment_out (INT32
+ * segment of each group), key_out, agg_out[i]; groups are ordered by (segment, key), so query g's groups are
+ * rows [segment_group_offsets[g], segment_group_offsets[g+1]) (host array, nsegments + 1 entries). Each
+ * query's rows equal what the per-query path (vgpu_filter_project then vgpu_group_by) returns for it.
+ * No single reference counterpart: it batches many ExecuteGroupByNode calls
+ * (cuda_engine/src/operators/gpu_groupby.cu:303-592) over FILTER outputs (gpu_executor.cu:645-730).
+ * Synchronizes. */
+int vgpu_batched_filter_group_by(const vgpu_column *cols, int32_t ncols, uint64_t rows,
+                                 const uint64_t *segment_offsets, int32_t nsegments, const vgpu_expr *preds,
+                                 int32_t npreds, int32_t key_col, const int32_t *value_cols, int32_t nvalues,
+                                 const vgpu_agg *aggs, int32_t naggs, void *segment_out, void *key_out,
+                                 void *const *agg_out, uint64_t capacity, uint64_t *groups,
+                                 uint64_t *segment_group_offsets, void *stream);
+
+/* The same queries through the plain per-query path, one after another (per query: filter kernel, kept
+ * count read back, key range, GROUP BY, group count read back), with the same arguments and output layout
+ * as vgpu_batched_filter_group_by; segment_offsets_host is the host copy of the row offsets. The baseline
+ * the batched form is compared with. Synchronizes. */
+int vgpu_per_query_filter_group_by(const vgpu_column *cols, int32_t ncols, uint64_t rows,
+                                   const uint64_t *segment_offsets_host, int32_t nsegments, const vgpu_expr *preds,
+                                   int32_t npreds, int32_t key_col, const int32_t *value_cols, int32_t nvalues,
+                                   const vgpu_agg *aggs, int32_t naggs, void *segment_out, void *key_out,
+                                   void *const *agg_out, uint64_t capacity, uint64_t *groups,
+                                   uint64_t *segment_group_offsets, void *stream);
+
 /* ---------------------------------------------------------------- primitives */
 
 /* Order-preserving copy of in[i] > threshold into out; *count_host receives the kept count (synchronizes).

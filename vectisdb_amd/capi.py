@@ -84,6 +84,16 @@ _PROTOS = {
     "vgpu_profile_read": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vgpu_pool_stats": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "vgpu_pool_trim": (C.c_int, []),
+    "vgpu_batched_filter_group_by": (C.c_int, [C.POINTER(Column), C.c_int32, C.c_uint64, C.c_void_p, C.c_int32,
+                                               C.POINTER(Expr), C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.c_int32,
+                                               C.POINTER(Agg), C.c_int32, C.c_void_p, C.c_void_p,
+                                               C.POINTER(C.c_void_p), C.c_uint64, C.POINTER(C.c_uint64),
+                                               C.POINTER(C.c_uint64), C.c_void_p]),
+    "vgpu_per_query_filter_group_by": (C.c_int, [C.POINTER(Column), C.c_int32, C.c_uint64, C.POINTER(C.c_uint64),
+                                                 C.c_int32, C.POINTER(Expr), C.c_int32, C.c_int32,
+                                                 C.POINTER(C.c_int32), C.c_int32, C.POINTER(Agg), C.c_int32,
+                                                 C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.c_uint64,
+                                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_void_p]),
     "vgpu_execute_plan": (C.c_int, [C.c_char_p, C.POINTER(HostColumn), C.c_int32, C.POINTER(ExecuteOptions),
                                     C.POINTER(C.c_void_p)]),
     "vgpu_result_success": (C.c_int32, [C.c_void_p]),
@@ -240,6 +250,46 @@ def generate_column(out, vtype, dist, lo=0, hi=0, seed=0, stream_id=0, cdf=None,
                                    C.c_void_p(cdf.data_ptr() if cdf is not None else 0),
                                    cdf.numel() if cdf is not None else 0,
                                    C.c_void_p(perm.data_ptr() if perm is not None else 0), _stream_ptr(stream)))
+
+
+def batched_filter_group_by(cols, segment_offsets, preds, key_col, value_cols, aggs, capacity=None, stream=None,
+                            per_query=False):
+    """Batched form: cols are device tensors of one table, segment_offsets a device int64 tensor of
+    nseg + 1 row offsets; preds as (body, [col indices], BOOLEAN); aggs as (kind, value index, out type).
+    Returns (segment ids, keys, [aggregate columns], per-segment group offsets as a Python list).
+    per_query=True runs the same queries one by one through the plain path (same outputs)."""
+    import torch
+    lib = load()
+    rows = cols[0].numel()
+    nseg = segment_offsets.numel() - 1
+    cc = (Column * len(cols))(*[column(c) for c in cols])
+    pa, keep = _exprs(preds)
+    vc = (C.c_int32 * max(len(value_cols), 1))(*value_cols)
+    ac = (Agg * max(len(aggs), 1))(*[Agg(*a) for a in aggs])
+    if capacity is None:
+        capacity = max(rows, 1)
+    key_type = _tensor_type(cols[key_col])
+    seg_out = torch.empty(capacity, dtype=torch.int32, device=cols[0].device)
+    key_out = empty_column(key_type, capacity)
+    agg_out = [empty_column(a[2], capacity) for a in aggs]
+    ap = (C.c_void_p * max(len(agg_out), 1))(*[t.data_ptr() for t in agg_out])
+    groups = C.c_uint64(0)
+    offs = (C.c_uint64 * (nseg + 1))()
+    if per_query:
+        host = segment_offsets.cpu().numpy().astype("uint64")
+        hoffs = (C.c_uint64 * (nseg + 1))(*[int(x) for x in host])
+        check(lib.vgpu_per_query_filter_group_by(cc, len(cols), rows, hoffs, nseg, pa, len(preds), key_col, vc,
+                                                 len(value_cols), ac, len(aggs), C.c_void_p(seg_out.data_ptr()),
+                                                 C.c_void_p(key_out.data_ptr()), ap, capacity, C.byref(groups), offs,
+                                                 _stream_ptr(stream)))
+    else:
+        check(lib.vgpu_batched_filter_group_by(cc, len(cols), rows, C.c_void_p(segment_offsets.data_ptr()), nseg,
+                                               pa, len(preds), key_col, vc, len(value_cols), ac, len(aggs),
+                                               C.c_void_p(seg_out.data_ptr()), C.c_void_p(key_out.data_ptr()), ap,
+                                               capacity, C.byref(groups), offs, _stream_ptr(stream)))
+    g = groups.value
+    trimmed = [t[: 2 * g] if a[2] == HUGEINT else t[:g] for t, a in zip(agg_out, aggs)]
+    return seg_out[:g], key_out[:g], trimmed, list(offs)
 
 
 def _exprs(exprs):

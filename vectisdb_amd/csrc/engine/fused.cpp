@@ -59,6 +59,8 @@ struct VArgs {
   vg_u64* count;
   vg_u64 ntiles;
   vg_u64 cap;
+  const vg_u64* seg;
+  vg_u64 nseg;
 };
 __device__ __forceinline__ vg_u64 vg_load(const vg_u64* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -159,9 +161,19 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 		o << "  const vg_u64 tile = blockIdx.x;\n";
 	}
 	o << "  const vg_u64 tile_base = tile * " << kTile << "ull;\n";
+	bool has_segment = false;
+	for (auto &c : s.cols) {
+		has_segment = has_segment || c.segment_id;
+	}
+	if (has_segment && (s.cross || s.selection_in)) {
+		throw std::runtime_error("fused kernel: a segment-id column needs plain row order");
+	}
 	for (size_t c = 0; c < s.cols.size(); c++) {
 		const char *t = CType(s.cols[c].vtype);
-		o << "  const " << t << "* __c" << c << " = (const " << t << "*)a.cols[" << c << "];\n";
+		if (s.cols[c].segment_id && (s.cols[c].vtype != VT_INT32 || s.cols[c].has_valid)) {
+			throw std::runtime_error("fused kernel: a segment-id column is a non-null INT32");
+		}
+		o << "  const " << t << "* __c" << c << " = (const " << t << "*)a.cols[" << c << "]; (void)__c" << c << ";\n";
 		o << "  " << t << " __v" << c << "[" << kRounds << "][" << kVec << "];\n";
 		if (s.cols[c].has_valid) {
 			o << "  const unsigned char* __vc" << c << " = a.valid[" << c << "];\n";
@@ -215,10 +227,28 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 		  << "[" << first << " + __j] : (" << t << ")0;\n";
 		o << indent << "}\n";
 	};
+	if (has_segment) {
+		// Segment of the lane's first row by binary search over the offsets (segment g covers rows
+		// [seg[g], seg[g+1]); empty segments are skipped), then forward steps for the next rows.
+		o << "    unsigned int __sg[" << kVec << "];\n";
+		o << "    {\n";
+		o << "      vg_u64 __lo = 0, __hi = a.nseg;\n";
+		o << "      while (__lo < __hi) { vg_u64 __mid = (__lo + __hi) >> 1; if (a.seg[__mid + 1] <= i0) __lo = __mid + 1; else __hi = __mid; }\n";
+		o << "      vg_u64 __g = __lo;\n";
+		o << "#pragma unroll\n      for (int __j = 0; __j < " << kVec << "; __j++) {\n";
+		o << "        while (__g + 1 < a.nseg && a.seg[__g + 1] <= i0 + __j) __g++;\n";
+		o << "        __sg[__j] = (unsigned int)__g;\n";
+		o << "      }\n";
+		o << "    }\n";
+	}
 	for (size_t c = 0; c < s.cols.size(); c++) {
 		const char *t = CType(s.cols[c].vtype);
 		std::string cn = "__c" + std::to_string(c);
 		std::string vn = "__v" + std::to_string(c) + "[__r]";
+		if (s.cols[c].segment_id) {
+			o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) " << vn << "[__j] = (int)__sg[__j];\n";
+			continue;
+		}
 		if (s.cross) {
 			if (s.cols[c].side == 0) {
 				o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) " << vn
@@ -387,6 +417,8 @@ struct VArgsHost {
 	uint64_t *count;
 	uint64_t ntiles;
 	uint64_t cap;
+	const uint64_t *seg;
+	uint64_t nseg;
 };
 } // namespace
 
@@ -414,6 +446,13 @@ void FusedKernel::Launch(const FusedLaunch &args, void *scratch_state, size_t sc
 	h.ntiles = tiles;
 	h.cap = args.capacity == 0 ? args.n : args.capacity;
 	h.count = args.count_device;
+	h.seg = args.seg_offsets;
+	h.nseg = args.nseg;
+	for (auto &c : spec_.cols) {
+		if (c.segment_id && (args.seg_offsets == nullptr || args.nseg == 0)) {
+			throw std::runtime_error("fused kernel: a segment-id column needs segment offsets");
+		}
+	}
 	if (!spec_.predicates.empty()) {
 		if (scratch_state == nullptr || scratch_bytes < StateBytes(args.n)) {
 			throw std::runtime_error("fused kernel: look-back state buffer too small");

@@ -28,6 +28,8 @@ struct FusedColumn {
 	int vtype = 1;          // VType of the column
 	bool has_valid = false; // dense byte validity supplied at launch
 	int side = 0;           // cross-product mode: 0 = left (broadcast), 1 = right
+	//! Not read from memory: the row's segment index g (seg_offsets[g] <= row < seg_offsets[g+1]), as INT32.
+	bool segment_id = false;
 };
 
 struct FusedExpr {
@@ -57,6 +59,8 @@ struct FusedLaunch {
 	uint32_t *sel_out = nullptr;              // emit_selection
 	uint64_t *count_device = nullptr;         // kept rows (device); required when predicates exist
 	uint64_t capacity = 0;                    // output rows available (0 = n); writes past it are dropped
+	const uint64_t *seg_offsets = nullptr;    // device, nseg + 1 ascending row offsets (segment-id columns)
+	uint64_t nseg = 0;
 };
 
 class FusedKernel {

@@ -2,6 +2,7 @@
 // extern "C" boundary. Inputs and outputs are device columns; nothing here touches host data.
 #pragma once
 
+#include "../engine/fused.h"
 #include "../hip/kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -80,6 +81,38 @@ void FinishJoinAggregate(const JoinAggValue &v, int agg_kind, int out_type, uint
 //! Synchronizes. Throws past 2^32 output rows.
 uint64_t JoinMaterializeDevice(const DevCol &akey, const DevCol &bkey, uint32_t **left_rows, uint32_t **right_rows,
                                Scratch &scratch, hipStream_t stream);
+
+// ------------------------------------------------------------------ batched form
+
+//! Many filter + GROUP BY queries over the row groups of one table (segment g = rows
+//! [offsets[g], offsets[g+1])), run as one pipeline. Predicate slots index `cols`; aggregate inputs index
+//! `value_cols`. Outputs (caller-owned, `capacity` rows each): the segment of each group (INT32), the key,
+//! and one column per aggregate; groups ordered by (segment, key).
+struct BatchedGroupByArgs {
+	std::vector<DevCol> cols;
+	uint64_t rows = 0;
+	const uint64_t *seg_offsets_dev = nullptr;
+	uint32_t nseg = 0;
+	std::vector<FusedExpr> preds;
+	int key_col = -1;
+	std::vector<int> value_cols;
+	std::vector<AggReq> aggs;
+	void *seg_out = nullptr;
+	void *key_out = nullptr;
+	std::vector<void *> agg_out;
+	uint64_t capacity = 0;
+	uint64_t *seg_group_offsets_host = nullptr; // nseg + 1 entries, optional
+	bool fast_math = false;
+};
+
+//! Returns the total group count. Synchronizes.
+uint64_t BatchedFilterGroupByDevice(const BatchedGroupByArgs &args, Scratch &scratch, hipStream_t stream);
+
+//! The same queries run one at a time (filter kernel, GROUP BY, each with its own size read-backs), writing
+//! the same output layout: the plain per-query path the batched form is measured against.
+//! `seg_offsets_host` holds the nseg + 1 row offsets on the host.
+uint64_t PerQueryFilterGroupByDevice(const BatchedGroupByArgs &args, const uint64_t *seg_offsets_host,
+                                     Scratch &scratch, hipStream_t stream);
 
 //! Autotuner: radix digit width per key width / row count, from measured sweeps (see tuning.cpp).
 SortTuning TuneSort(int total_bits, uint64_t rows);
