@@ -209,6 +209,70 @@ def bench_cross(torch, capi, dev, steps):
             "algorithmic_bytes": alg, "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def make_batched_table(torch, capi, dev, nq=512, seed=7):
+    """512 row groups of one table, sizes log-uniform in [2K, 4M] rows; a INT32 uniform [0,1e6), c DOUBLE
+    uniform [0,1), key INT64 Zipf(1.1) over 1M values reduced modulo the query's group cardinality
+    (log-uniform in [16, 1M]), v1 INT64 uniform [-1e6,1e6], v2 DOUBLE normal(0,1)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(2048), np.log(4 << 20), nq)).astype(np.int64)
+    cards = np.exp(rng.uniform(np.log(16), np.log(1 << 20), nq)).astype(np.int64)
+    rows = int(sizes.sum())
+    a = torch.empty(rows, dtype=torch.int32, device=dev)
+    c = torch.empty(rows, dtype=torch.float64, device=dev)
+    capi.generate_column(a, capi.INT32, 0, lo=0, hi=999_999, seed=seed, stream_id=1)
+    capi.generate_column(c, capi.FLOAT64, 1, seed=seed, stream_id=2)
+    key, v1, v2 = make_groupby_table(torch, capi, dev, rows, seed=seed)
+    card_rows = torch.repeat_interleave(torch.from_numpy(cards).to(dev), torch.from_numpy(sizes).to(dev))
+    key.remainder_(card_rows)
+    del card_rows
+    offsets = torch.from_numpy(np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)).to(dev)
+    torch.cuda.synchronize()
+    return [a, c, key, v1, v2], offsets, sizes, cards
+
+
+def bench_batched(torch, capi, dev, steps, nq=512):
+    """Batched form against the per-query path: the same 512 filter + GROUP BY queries (WHERE a < 500000 AND
+    c > 0.25, GROUP BY key with COUNT/SUM/MIN/MAX/SUM/AVG) run as one batched pipeline, and one query after
+    another. Every query arrives at t=0, so each query's latency in the batch is the batch time, and in the
+    sequential loop its completion time."""
+    import numpy as np
+    cols, offsets, sizes, cards = make_batched_table(torch, capi, dev, nq)
+    rows = int(sizes.sum())
+    preds = [("((bool*)out)[idx] = (((const int*)inputs[0])[row]) < 500000;", [0], capi.BOOLEAN),
+             ("((bool*)out)[idx] = (((const double*)inputs[0])[row]) > 0.25;", [1], capi.BOOLEAN)]
+    aggs = [(capi.COUNT_STAR, -1, capi.INT64), (capi.SUM, 0, capi.HUGEINT), (capi.MIN, 0, capi.INT64),
+            (capi.MAX, 0, capi.INT64), (capi.SUM, 1, capi.FLOAT64), (capi.AVG, 1, capi.FLOAT64)]
+    run = lambda pq: capi.batched_filter_group_by(cols, offsets, preds, 2, [3, 4], aggs, per_query=pq)
+    b = run(False)
+    p = run(True)
+    same = b[3] == p[3] and bool(torch.equal(b[0], p[0])) and bool(torch.equal(b[1], p[1]))
+    for i, a in enumerate(aggs):
+        if a[2] == capi.FLOAT64:
+            same = same and bool(torch.allclose(b[2][i], p[2][i], rtol=1e-12, atol=0))
+        else:
+            same = same and bool(torch.equal(b[2][i], p[2][i]))
+    del b, p
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run(False)
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / steps
+    psteps = max(1, steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(psteps):
+        run(True)
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) / psteps
+    return {"operator": f"batched filter + GROUP BY, {nq} row groups (log-uniform 2K..4M rows, "
+                        f"16..1M groups), {rows} rows",
+            "rows": rows, "queries": nq, "batched_ms": round(tb * 1e3, 3), "per_query_ms": round(tp * 1e3, 3),
+            "batched_qps": round(nq / tb, 1), "per_query_qps": round(nq / tp, 1),
+            "qps_ratio": round(tp / tb, 2), "rows_per_s": round(rows / tb, 1),
+            "results_identical": same}
+
+
 def roofline_from_profile(prof, prefer=None):
     """Dominant kernel (most total time) -> roofline dict."""
     if not prof:
@@ -300,7 +364,7 @@ def main():
         os.environ.pop("VGPU_RADIX_BITS", None)
 
     operators = []
-    extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "cross", "join"]
+    extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "cross", "join", "batched"]
     for name in extra:
         try:
             if name == "filter":
@@ -309,6 +373,8 @@ def main():
                 operators.append(bench_cross(torch, capi, dev, 10))
             elif name == "join":
                 operators.append(bench_join(torch, capi, dev, 3))
+            elif name == "batched":
+                operators.append(bench_batched(torch, capi, dev, 4))
             else:
                 continue
         except Exception as exc:  # an operator line must not hide the headline number

@@ -268,7 +268,8 @@ def batched_filter_group_by(cols, segment_offsets, preds, key_col, value_cols, a
     ac = (Agg * max(len(aggs), 1))(*[Agg(*a) for a in aggs])
     if capacity is None:
         capacity = max(rows, 1)
-    key_type = _tensor_type(cols[key_col])
+    # an out-of-range key column is left for the library to reject with its own message
+    key_type = _tensor_type(cols[key_col]) if 0 <= key_col < len(cols) else INT64
     seg_out = torch.empty(capacity, dtype=torch.int32, device=cols[0].device)
     key_out = empty_column(key_type, capacity)
     agg_out = [empty_column(a[2], capacity) for a in aggs]
