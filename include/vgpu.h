@@ -270,6 +270,90 @@ int vgpu_vram_budget_init(double fraction_of_vram);
 int vgpu_vram_has_headroom(uint64_t bytes, int32_t *out);
 int vgpu_vram_budget_bytes(uint64_t *total, uint64_t *allocated);
 
+/* ---------------------------------------------------------------- served form
+ * A request loop over device-resident tables. Queries are instances of prepared templates applied to a
+ * row range of the template's table; `slots` executor threads (each with its own HIP stream) admit the
+ * oldest waiting query whose memory footprint fits the unreserved budget and, with batching on, every later
+ * waiting query of the same template that still fits (continuous batching: queries arriving while others
+ * run join the next batch an executor forms). Batches of different executors overlap on the device.
+ * No reference counterpart: the reference serves one plan at a time under a mutex
+ * (cuda_engine/src/gpu_engine.cpp:222-259); this is the serving loop around the same operators. */
+typedef struct vgpu_server vgpu_server;
+typedef struct vgpu_served_result vgpu_served_result;
+
+typedef struct vgpu_server_options {
+	int32_t slots;          /* executor threads / HIP streams, 1..16 (0 = 4) */
+	int32_t batching;       /* 1: batch same-template queries; 0: every query alone on the plain path */
+	int32_t max_batch;      /* queries per batch (0 = 512) */
+	uint64_t memory_budget; /* device bytes admitted work may reserve (0 = 3/4 of free memory) */
+	int32_t fast_math;
+	uint64_t max_batch_rows; /* table rows per batch (0 = no limit but the GROUP BY sort's 2^30) */
+} vgpu_server_options;
+
+enum vgpu_template_kind {
+	VGPU_TEMPLATE_FILTER_PROJECT = 0, /* preds + projs over cols: one output column per projection */
+	VGPU_TEMPLATE_GROUP_BY = 1,       /* preds, GROUP BY cols[key_col], aggs over value_cols: key, then aggs */
+	VGPU_TEMPLATE_JOIN_AGG = 2        /* SUM(build_payload), COUNT(*) of cols[0] = build_key: HUGEINT, INT64 */
+};
+
+typedef struct vgpu_template {
+	int32_t kind;               /* enum vgpu_template_kind */
+	const vgpu_column *cols;    /* the table (device-resident while the server lives) */
+	int32_t ncols;
+	const vgpu_expr *preds;     /* inputs index cols */
+	int32_t npreds;
+	const vgpu_expr *projs;     /* FILTER_PROJECT */
+	int32_t nprojs;
+	int32_t key_col;            /* GROUP_BY */
+	const int32_t *value_cols;  /* GROUP_BY; aggregate inputs index this list */
+	int32_t nvalues;
+	const vgpu_agg *aggs;       /* GROUP_BY */
+	int32_t naggs;
+	vgpu_column build_key;      /* JOIN_AGG: integer key without NULLs, same type as cols[0] */
+	vgpu_column build_payload;  /* JOIN_AGG: integer payload */
+} vgpu_template;
+
+typedef struct vgpu_served_info {
+	int32_t status;      /* 0 ok; else vgpu_served_result_error holds the message */
+	int32_t kind;
+	uint64_t rows;       /* kept rows, groups, or 1 */
+	int32_t ncols;
+	int32_t on_device;   /* column data are device pointers (0: host scalars) */
+	uint32_t batch_size; /* queries in the batch that computed this result */
+	uint64_t submit_ns, start_ns, done_ns; /* steady clock */
+} vgpu_served_info;
+
+typedef struct vgpu_server_stats {
+	uint64_t queries, batches, max_batch, peak_reserved, budget, rejected;
+} vgpu_server_stats;
+
+int vgpu_server_create(const vgpu_server_options *options, vgpu_server **server);
+/* Runs every queued query, then stops the executors. Results already collected stay valid. */
+int vgpu_server_destroy(vgpu_server *server);
+/* Prepares a template (a JOIN_AGG template with batching on reduces its build side once, here). */
+int vgpu_server_add_template(vgpu_server *server, const vgpu_template *tmpl, int32_t *template_id);
+int vgpu_server_submit(vgpu_server *server, int32_t template_id, uint64_t row_start, uint64_t row_count,
+                       uint64_t *ticket);
+/* Blocks until the query is done; the result belongs to the caller until vgpu_served_result_free. */
+int vgpu_server_wait(vgpu_server *server, uint64_t ticket, vgpu_served_result **result);
+int vgpu_server_stats_read(vgpu_server *server, vgpu_server_stats *stats);
+int vgpu_served_result_info(const vgpu_served_result *result, vgpu_served_info *info);
+/* Column i: data (device pointer, or host for JOIN_AGG scalars), type, and for scalars a NULL flag (SUM
+ * over no matched rows). The memory stays valid until the result is freed; finish device reads first. */
+int vgpu_served_result_column(const vgpu_served_result *result, int32_t i, const void **data, int32_t *type,
+                              int32_t *is_null);
+/* Copies column i (rows x type width bytes) to host memory `dst` (synchronous). */
+int vgpu_served_result_copy(const vgpu_served_result *result, int32_t i, void *dst);
+const char *vgpu_served_result_error(const vgpu_served_result *result);
+void vgpu_served_result_free(vgpu_served_result *result);
+/* Open-loop driver: query i (template tmpl[i], rows [starts[i], starts[i] + counts[i])) is submitted at
+ * arrival_s[i] seconds after the start whatever the server is doing; latency_s[i] = completion - arrival.
+ * *elapsed_s = start to last completion; batch_sizes (optional) = size of the batch that ran query i.
+ * Fails if any query fails. */
+int vgpu_server_run_open_loop(vgpu_server *server, const int32_t *tmpl, const uint64_t *starts,
+                              const uint64_t *counts, const double *arrival_s, uint64_t n, double *latency_s,
+                              uint32_t *batch_sizes, double *elapsed_s);
+
 /* ---------------------------------------------------------------- measurement */
 
 /* When on, every kernel launch is bracketed by HIP events on its own stream. */

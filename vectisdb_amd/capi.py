@@ -55,7 +55,47 @@ class PlanShape(C.Structure):
 
 
 # (name, restype, argtypes); every function returns int status unless noted
+class ServerOptions(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("batching", C.c_int32), ("max_batch", C.c_int32),
+                ("memory_budget", C.c_uint64), ("fast_math", C.c_int32), ("max_batch_rows", C.c_uint64)]
+
+
+class Template(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("cols", C.POINTER(Column)), ("ncols", C.c_int32),
+                ("preds", C.POINTER(Expr)), ("npreds", C.c_int32), ("projs", C.POINTER(Expr)), ("nprojs", C.c_int32),
+                ("key_col", C.c_int32), ("value_cols", C.POINTER(C.c_int32)), ("nvalues", C.c_int32),
+                ("aggs", C.POINTER(Agg)), ("naggs", C.c_int32), ("build_key", Column), ("build_payload", Column)]
+
+
+class ServedInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("kind", C.c_int32), ("rows", C.c_uint64), ("ncols", C.c_int32),
+                ("on_device", C.c_int32), ("batch_size", C.c_uint32), ("submit_ns", C.c_uint64),
+                ("start_ns", C.c_uint64), ("done_ns", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queries", C.c_uint64), ("batches", C.c_uint64), ("max_batch", C.c_uint64),
+                ("peak_reserved", C.c_uint64), ("budget", C.c_uint64), ("rejected", C.c_uint64)]
+
+
+TEMPLATE_FILTER_PROJECT, TEMPLATE_GROUP_BY, TEMPLATE_JOIN_AGG = 0, 1, 2
+
 _PROTOS = {
+    "vgpu_server_create": (C.c_int, [C.POINTER(ServerOptions), C.POINTER(C.c_void_p)]),
+    "vgpu_server_destroy": (C.c_int, [C.c_void_p]),
+    "vgpu_server_add_template": (C.c_int, [C.c_void_p, C.POINTER(Template), C.POINTER(C.c_int32)]),
+    "vgpu_server_submit": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "vgpu_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "vgpu_server_stats_read": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "vgpu_served_result_info": (C.c_int, [C.c_void_p, C.POINTER(ServedInfo)]),
+    "vgpu_served_result_column": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_int32)]),
+    "vgpu_served_result_copy": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "vgpu_served_result_error": (C.c_char_p, [C.c_void_p]),
+    "vgpu_served_result_free": (None, [C.c_void_p]),
+    "vgpu_server_run_open_loop": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_uint64),
+                                            C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.c_uint64,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "vgpu_last_error": (C.c_char_p, []),
     "vgpu_abi_version": (C.c_int, []),
     "vgpu_device_init": (C.c_int, [C.c_int]),
@@ -430,3 +470,129 @@ def pool_stats():
     a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
     check(load().vgpu_pool_stats(C.byref(a), C.byref(b), C.byref(c)))
     return {"in_use": a.value, "cached": b.value, "peak": c.value}
+
+
+_NP_DTYPES = {INT16: "int16", INT32: "int32", INT64: "int64", FLOAT32: "float32", FLOAT64: "float64",
+              BOOLEAN: "uint8", HUGEINT: "int64"}
+
+
+class ServedResult:
+    """A finished served query: info fields, and columns copied to host numpy arrays (HUGEINT as an (n, 2)
+    int64 array of low/high words); JOIN_AGG scalars carry a NULL flag."""
+
+    def __init__(self, info, columns, types, nulls, error):
+        self.status = info.status
+        self.kind = info.kind
+        self.rows = info.rows
+        self.batch_size = info.batch_size
+        self.latency_s = (info.done_ns - info.submit_ns) * 1e-9
+        self.columns = columns
+        self.types = types
+        self.nulls = nulls
+        self.error = error
+
+
+class Server:
+    """The served form over device-resident tables (tensors must stay alive while the server lives)."""
+
+    def __init__(self, slots=4, batching=True, max_batch=512, memory_budget=0, fast_math=False, max_batch_rows=0):
+        lib = load()
+        self._lib = lib
+        self._keep = []
+        opts = ServerOptions(slots, 1 if batching else 0, max_batch, memory_budget, 1 if fast_math else 0,
+                             max_batch_rows)
+        h = C.c_void_p()
+        check(lib.vgpu_server_create(C.byref(opts), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            check(self._lib.vgpu_server_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _add(self, t, keep):
+        self._keep.append(keep)
+        tid = C.c_int32(0)
+        check(self._lib.vgpu_server_add_template(self._h, C.byref(t), C.byref(tid)))
+        return tid.value
+
+    def add_filter_project(self, cols, preds, projs):
+        cc = (Column * len(cols))(*[column(c) for c in cols])
+        pa, k1 = _exprs(preds)
+        qa, k2 = _exprs(projs)
+        t = Template(kind=TEMPLATE_FILTER_PROJECT, cols=cc, ncols=len(cols), preds=pa, npreds=len(preds), projs=qa,
+                     nprojs=len(projs), key_col=-1)
+        return self._add(t, (cc, pa, k1, qa, k2))
+
+    def add_group_by(self, cols, preds, key_col, value_cols, aggs):
+        cc = (Column * len(cols))(*[column(c) for c in cols])
+        pa, k1 = _exprs(preds)
+        vc = (C.c_int32 * max(len(value_cols), 1))(*value_cols)
+        ac = (Agg * max(len(aggs), 1))(*[Agg(*a) for a in aggs])
+        t = Template(kind=TEMPLATE_GROUP_BY, cols=cc, ncols=len(cols), preds=pa, npreds=len(preds), key_col=key_col,
+                     value_cols=vc, nvalues=len(value_cols), aggs=ac, naggs=len(aggs))
+        return self._add(t, (cc, pa, k1, vc, ac))
+
+    def add_join_agg(self, probe_key, build_key, build_payload):
+        cc = (Column * 1)(column(probe_key))
+        t = Template(kind=TEMPLATE_JOIN_AGG, cols=cc, ncols=1, key_col=-1, build_key=column(build_key),
+                     build_payload=column(build_payload))
+        return self._add(t, (cc,))
+
+    def submit(self, template_id, row_start, row_count):
+        tk = C.c_uint64(0)
+        check(self._lib.vgpu_server_submit(self._h, template_id, row_start, row_count, C.byref(tk)))
+        return tk.value
+
+    def wait(self, ticket):
+        import numpy as np
+        lib = self._lib
+        r = C.c_void_p()
+        check(lib.vgpu_server_wait(self._h, ticket, C.byref(r)))
+        try:
+            info = ServedInfo()
+            check(lib.vgpu_served_result_info(r, C.byref(info)))
+            cols, types, nulls = [], [], []
+            if info.status == 0:
+                for i in range(info.ncols):
+                    ptr, ty, nl = C.c_void_p(), C.c_int32(0), C.c_int32(0)
+                    check(lib.vgpu_served_result_column(r, i, C.byref(ptr), C.byref(ty), C.byref(nl)))
+                    n = info.rows * (2 if ty.value == HUGEINT else 1)
+                    a = np.empty(n, dtype=_NP_DTYPES[ty.value])
+                    if n:
+                        check(lib.vgpu_served_result_copy(r, i, C.c_void_p(a.ctypes.data)))
+                    cols.append(a.reshape(-1, 2) if ty.value == HUGEINT else a)
+                    types.append(ty.value)
+                    nulls.append(bool(nl.value))
+            err = lib.vgpu_served_result_error(r).decode()
+            return ServedResult(info, cols, types, nulls, err)
+        finally:
+            lib.vgpu_served_result_free(r)
+
+    def stats(self):
+        s = ServerStats()
+        check(self._lib.vgpu_server_stats_read(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def run_open_loop(self, tmpl, starts, counts, arrivals):
+        """Submits query i at arrivals[i] seconds (open loop); returns (latencies, batch sizes, elapsed s)."""
+        import numpy as np
+        n = len(tmpl)
+        t = np.ascontiguousarray(tmpl, dtype=np.int32)
+        s = np.ascontiguousarray(starts, dtype=np.uint64)
+        c = np.ascontiguousarray(counts, dtype=np.uint64)
+        a = np.ascontiguousarray(arrivals, dtype=np.float64)
+        lat = np.zeros(n, dtype=np.float64)
+        bs = np.zeros(n, dtype=np.uint32)
+        el = C.c_double(0)
+        check(self._lib.vgpu_server_run_open_loop(
+            self._h, t.ctypes.data_as(C.POINTER(C.c_int32)), s.ctypes.data_as(C.POINTER(C.c_uint64)),
+            c.ctypes.data_as(C.POINTER(C.c_uint64)), a.ctypes.data_as(C.POINTER(C.c_double)), n,
+            lat.ctypes.data_as(C.POINTER(C.c_double)), bs.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(el)))
+        return lat, bs, el.value

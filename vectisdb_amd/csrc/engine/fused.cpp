@@ -61,6 +61,8 @@ struct VArgs {
   vg_u64 cap;
   const vg_u64* seg;
   vg_u64 nseg;
+  const vg_u64* seg_row;
+  const vg_u64* seg_len;
 };
 __device__ __forceinline__ vg_u64 vg_load(const vg_u64* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -165,9 +167,14 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 	for (auto &c : s.cols) {
 		has_segment = has_segment || c.segment_id;
 	}
-	if (has_segment && (s.cross || s.selection_in)) {
-		throw std::runtime_error("fused kernel: a segment-id column needs plain row order");
+	if ((has_segment || s.ranges) && (s.cross || s.selection_in)) {
+		throw std::runtime_error("fused kernel: a segment-id column or row ranges need plain row order");
 	}
+	if (s.ranges && !filter) {
+		throw std::runtime_error("fused kernel: row ranges need a predicate (the outputs are compacted)");
+	}
+	// A row of the launch is live when it is inside the input (and, with row ranges, not padding).
+	const std::string live = s.ranges ? "__in[__j]" : "(i0 + __j < a.n)";
 	for (size_t c = 0; c < s.cols.size(); c++) {
 		const char *t = CType(s.cols[c].vtype);
 		if (s.cols[c].segment_id && (s.cols[c].vtype != VT_INT32 || s.cols[c].has_valid)) {
@@ -194,7 +201,33 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 	o << "    const vg_u64 i0 = tile_base + (vg_u64)wave * " << (kVec * kRounds * 64) << "ull + (vg_u64)__r * "
 	  << (kVec * 64) << "ull + (vg_u64)lane * " << kVec << "ull;\n";
 	o << "    __i0[__r] = i0;\n";
-	o << "    const bool __full = i0 + " << (kVec - 1) << " < a.n;\n";
+	if (has_segment || s.ranges) {
+		// Segment of the lane's first row by binary search over the offsets (segment g covers rows
+		// [seg[g], seg[g+1]); empty segments are skipped), then forward steps for the next rows.
+		o << "    unsigned int __sg[" << kVec << "];\n";
+		o << "    {\n";
+		o << "      vg_u64 __lo = 0, __hi = a.nseg;\n";
+		o << "      while (__lo < __hi) { vg_u64 __mid = (__lo + __hi) >> 1; if (a.seg[__mid + 1] <= i0) __lo = __mid + 1; else __hi = __mid; }\n";
+		o << "      vg_u64 __g = __lo;\n";
+		o << "#pragma unroll\n      for (int __j = 0; __j < " << kVec << "; __j++) {\n";
+		o << "        while (__g + 1 < a.nseg && a.seg[__g + 1] <= i0 + __j) __g++;\n";
+		o << "        __sg[__j] = (unsigned int)__g;\n";
+		o << "      }\n";
+		o << "    }\n";
+	}
+	if (s.ranges) {
+		o << "    bool __in[" << kVec << "];\n";
+		o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) {\n";
+		o << "      const vg_u64 __g = __sg[__j];\n";
+		o << "      const vg_u64 __off = i0 + __j - a.seg[__g];\n";
+		o << "      __in[__j] = i0 + __j < a.n && __off < a.seg_len[__g];\n";
+		o << "      __rowid[__r][__j] = __in[__j] ? a.seg_row[__g] + __off : 0ull;\n";
+		o << "    }\n";
+		// one segment, first and last row live: the lane's rows are contiguous table rows
+		o << "    const bool __full = __in[0] && __in[" << (kVec - 1) << "] && __sg[0] == __sg[" << (kVec - 1) << "];\n";
+	} else {
+		o << "    const bool __full = i0 + " << (kVec - 1) << " < a.n;\n";
+	}
 	if (s.cross) {
 		o << "    vg_u64 __lrow[" << kVec << "], __rrow[" << kVec << "];\n";
 		o << "    if (__aligned_cross) {\n";
@@ -210,12 +243,12 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 		o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) {\n";
 		o << "      __rowid[__r][__j] = (i0 + __j < a.n) ? (vg_u64)a.sel[i0 + __j] : 0ull;\n";
 		o << "    }\n";
-	} else {
+	} else if (!s.ranges) {
 		o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) __rowid[__r][__j] = i0 + __j;\n";
 	}
 	// Loads: 128-bit (or 64/32-bit for narrow types) when the lane's 4 rows are contiguous and aligned.
 	auto emit_vector_load = [&](const std::string &cn, const std::string &vn, const char *t, const std::string &first,
-	                            const std::string &indent) {
+	                            const std::string &indent, const std::string &guard, const std::string &index) {
 		o << indent << "if (__full && ((vg_u64)(" << cn << " + " << first << ") & " << kVec << " * sizeof(" << t
 		  << ") - 1) == 0) {\n";
 		o << indent << "  const void* __ap = __builtin_assume_aligned((const void*)(" << cn << " + " << first << "), "
@@ -223,24 +256,10 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 		o << indent << "  __builtin_memcpy(&" << vn << "[0], __ap, " << kVec << " * sizeof(" << t << "));\n";
 		o << indent << "} else {\n";
 		o << indent << "#pragma unroll\n";
-		o << indent << "  for (int __j = 0; __j < " << kVec << "; __j++) " << vn << "[__j] = (i0 + __j < a.n) ? " << cn
-		  << "[" << first << " + __j] : (" << t << ")0;\n";
+		o << indent << "  for (int __j = 0; __j < " << kVec << "; __j++) " << vn << "[__j] = " << guard << " ? " << cn
+		  << "[" << index << "] : (" << t << ")0;\n";
 		o << indent << "}\n";
 	};
-	if (has_segment) {
-		// Segment of the lane's first row by binary search over the offsets (segment g covers rows
-		// [seg[g], seg[g+1]); empty segments are skipped), then forward steps for the next rows.
-		o << "    unsigned int __sg[" << kVec << "];\n";
-		o << "    {\n";
-		o << "      vg_u64 __lo = 0, __hi = a.nseg;\n";
-		o << "      while (__lo < __hi) { vg_u64 __mid = (__lo + __hi) >> 1; if (a.seg[__mid + 1] <= i0) __lo = __mid + 1; else __hi = __mid; }\n";
-		o << "      vg_u64 __g = __lo;\n";
-		o << "#pragma unroll\n      for (int __j = 0; __j < " << kVec << "; __j++) {\n";
-		o << "        while (__g + 1 < a.nseg && a.seg[__g + 1] <= i0 + __j) __g++;\n";
-		o << "        __sg[__j] = (unsigned int)__g;\n";
-		o << "      }\n";
-		o << "    }\n";
-	}
 	for (size_t c = 0; c < s.cols.size(); c++) {
 		const char *t = CType(s.cols[c].vtype);
 		std::string cn = "__c" + std::to_string(c);
@@ -255,7 +274,7 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 				  << "[__j] = (i0 + __j < a.n) ? " << cn << "[__lrow[__j]] : (" << t << ")0;\n";
 			} else {
 				o << "    if (__aligned_cross) {\n";
-				emit_vector_load(cn, vn, t, "__rrow[0]", "      ");
+				emit_vector_load(cn, vn, t, "__rrow[0]", "      ", "(i0 + __j < a.n)", "__rrow[0] + __j");
 				o << "    } else {\n";
 				o << "#pragma unroll\n      for (int __j = 0; __j < " << kVec << "; __j++) " << vn
 				  << "[__j] = (i0 + __j < a.n) ? " << cn << "[__rrow[__j]] : (" << t << ")0;\n";
@@ -264,20 +283,22 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 		} else if (s.selection_in) {
 			o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) " << vn << "[__j] = (i0 + __j < a.n) ? "
 			  << cn << "[__rowid[__r][__j]] : (" << t << ")0;\n";
+		} else if (s.ranges) {
+			emit_vector_load(cn, vn, t, "__rowid[__r][0]", "    ", live, "__rowid[__r][__j]");
 		} else {
-			emit_vector_load(cn, vn, t, "i0", "    ");
+			emit_vector_load(cn, vn, t, "i0", "    ", live, "i0 + __j");
 		}
 		if (s.cols[c].has_valid) {
 			std::string mn = "__m" + std::to_string(c) + "[__r]";
 			std::string idx = s.cross ? (s.cols[c].side == 0 ? "__lrow[__j]" : "__rrow[__j]") : "__rowid[__r][__j]";
-			o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) " << mn
-			  << "[__j] = (i0 + __j < a.n) ? (__vc" << c << " == 0 || __vc" << c << "[" << idx << "] != 0) : false;\n";
+			o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) " << mn << "[__j] = " << live
+			  << " ? (__vc" << c << " == 0 || __vc" << c << "[" << idx << "] != 0) : false;\n";
 		}
 	}
 	// Predicates.
 	o << "    unsigned int __k = 0;\n";
 	o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) {\n";
-	o << "      bool __ok = i0 + __j < a.n;\n";
+	o << "      bool __ok = " << live << ";\n";
 	for (size_t p = 0; p < s.predicates.size(); p++) {
 		auto &e = s.predicates[p];
 		o << "      bool __p" << p << " = false;\n";
@@ -419,6 +440,8 @@ struct VArgsHost {
 	uint64_t cap;
 	const uint64_t *seg;
 	uint64_t nseg;
+	const uint64_t *seg_row;
+	const uint64_t *seg_len;
 };
 } // namespace
 
@@ -448,10 +471,16 @@ void FusedKernel::Launch(const FusedLaunch &args, void *scratch_state, size_t sc
 	h.count = args.count_device;
 	h.seg = args.seg_offsets;
 	h.nseg = args.nseg;
+	h.seg_row = args.seg_rows;
+	h.seg_len = args.seg_lens;
 	for (auto &c : spec_.cols) {
 		if (c.segment_id && (args.seg_offsets == nullptr || args.nseg == 0)) {
 			throw std::runtime_error("fused kernel: a segment-id column needs segment offsets");
 		}
+	}
+	if (spec_.ranges && (args.seg_offsets == nullptr || args.nseg == 0 || args.seg_rows == nullptr ||
+	                     args.seg_lens == nullptr)) {
+		throw std::runtime_error("fused kernel: row ranges need segment offsets, rows and lengths");
 	}
 	if (!spec_.predicates.empty()) {
 		if (scratch_state == nullptr || scratch_bytes < StateBytes(args.n)) {

@@ -45,6 +45,10 @@ struct FusedSpec {
 	bool selection_in = false;   // rows are read through a uint32 selection vector
 	bool emit_selection = false; // write kept row ids (uint32, base-row numbering)
 	bool cross = false;          // rows are (left i, right j) pairs of a left-major cross product
+	//! Rows are row ranges of the table: segment g covers launch rows [seg[g], seg[g] + seg_len[g]), read
+	//! from table rows seg_row[g] + (row - seg[g]); launch rows from seg[g] + seg_len[g] up to seg[g+1] are
+	//! padding (never kept). Needs at least one predicate (the outputs are compacted).
+	bool ranges = false;
 	bool fast_math = false;
 };
 
@@ -61,6 +65,8 @@ struct FusedLaunch {
 	uint64_t capacity = 0;                    // output rows available (0 = n); writes past it are dropped
 	const uint64_t *seg_offsets = nullptr;    // device, nseg + 1 ascending row offsets (segment-id columns)
 	uint64_t nseg = 0;
+	const uint64_t *seg_rows = nullptr;       // ranges mode: device, nseg table rows of each segment's start
+	const uint64_t *seg_lens = nullptr;       // ranges mode: device, nseg segment lengths
 };
 
 class FusedKernel {

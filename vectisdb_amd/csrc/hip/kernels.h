@@ -242,6 +242,22 @@ size_t RunningSumScratchBytes(uint64_t n);
 void RunningSum(int vtype, void *data, uint64_t n, void *state, void *scratch, size_t scratch_bytes,
                 hipStream_t stream);
 
+//! SUM/COUNT of an INNER equi-join of many probe row ranges against a build-side index (the distinct build
+//! keys ascending, each with its row count and int128 payload sum as lo/hi words). Range g covers launch
+//! rows [seg[g], seg[g] + seg_len[g]) read from table rows seg_row[g] + (v - seg[g]); seg has nseg + 1
+//! entries, seg[nseg] = nvirt. acc (3 x nseg uint64: matched rows, sum lo, sum hi) is zeroed here. With
+//! `dense` (see BuildDenseIndex) a probe key is looked up directly instead of by binary search.
+void RangeProbeAgg(int key_type, const void *probe_keys, const uint64_t *seg, const uint64_t *seg_row,
+                   const uint64_t *seg_len, uint32_t nseg, uint64_t nvirt, const void *index_keys,
+                   const uint64_t *index_count, const uint64_t *index_sum, uint64_t nd, const uint64_t *dense,
+                   uint64_t dense_min, uint64_t dense_span, uint64_t *acc, hipStream_t stream);
+
+//! Direct-address form of a join index whose keys span [min, min + span): dense[key - min] packs the key's
+//! row count (top 23 bits) and payload sum (41-bit two's complement); 0 = no build row. Returns false (the
+//! table is then unusable) when a count or sum does not fit. Synchronizes.
+bool BuildDenseIndex(int key_type, const void *keys, const uint64_t *count, const uint64_t *sum, uint64_t nd,
+                     uint64_t min, uint64_t span, uint64_t *dense, unsigned int *overflow_dev, hipStream_t stream);
+
 //! Order-preserving compaction of x > threshold (the FilterGreaterThanInt32 primitive).
 void FilterGtI32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_device,
                  Scratch &scratch, hipStream_t stream);

@@ -10,11 +10,14 @@
 // contiguous run of output rows with its segment index (group order within a query = ascending key), the
 // same rows the query produces alone.
 #include "../engine/fused.h"
+#include "device_pool.h"
 #include "operators.h"
+#include "../hip/join_kernels.h"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -77,6 +80,10 @@ uint64_t BatchedFilterGroupByDevice(const BatchedGroupByArgs &args, Scratch &scr
 	if (args.nseg == 0 || args.nseg >= (1u << 30)) {
 		throw std::runtime_error("batched group by: 1 .. 2^30 - 1 row groups");
 	}
+	const bool ranged = args.range_rows_host != nullptr;
+	if (ranged != (args.range_lens_host != nullptr)) {
+		throw std::runtime_error("batched group by: row ranges need both starts and lengths");
+	}
 	if (args.key_col < 0 || args.key_col >= (int)cols.size() || !TypeIsInteger(cols[args.key_col].type)) {
 		throw std::runtime_error("batched group by: the group key must be an integer/boolean column");
 	}
@@ -112,6 +119,14 @@ uint64_t BatchedFilterGroupByDevice(const BatchedGroupByArgs &args, Scratch &scr
 		}
 		spec.predicates.push_back(p);
 	}
+	if (ranged && spec.predicates.empty()) {
+		// row ranges are compacted through the predicate path (padding rows are never kept)
+		FusedExpr all;
+		all.body = "((bool*)out)[idx] = true;";
+		all.out_type = VT_BOOLEAN;
+		spec.predicates.push_back(all);
+	}
+	spec.ranges = ranged;
 	const int seg_slot = (int)spec.cols.size();
 	{
 		FusedColumn fc;
@@ -133,36 +148,49 @@ uint64_t BatchedFilterGroupByDevice(const BatchedGroupByArgs &args, Scratch &scr
 	}
 	auto kernel = GetFusedKernel(spec);
 
+	DeviceRanges ranges;
+	if (ranged) {
+		ranges = UploadRanges(args.range_rows_host, args.range_lens_host, args.nseg, rows, scratch, stream);
+	}
+	const uint64_t n = ranged ? ranges.nvirt : rows; // launch rows
 	std::vector<void *> outs;
-	outs.push_back(scratch.Alloc(std::max<uint64_t>(rows, 1) * 4));
-	outs.push_back(scratch.Alloc(std::max<uint64_t>(rows, 1) * TypeWidth(cols[args.key_col].type)));
+	outs.push_back(scratch.Alloc(std::max<uint64_t>(n, 1) * 4));
+	outs.push_back(scratch.Alloc(std::max<uint64_t>(n, 1) * TypeWidth(cols[args.key_col].type)));
 	for (int v : args.value_cols) {
-		outs.push_back(scratch.Alloc(std::max<uint64_t>(rows, 1) * TypeWidth(cols[v].type)));
+		outs.push_back(scratch.Alloc(std::max<uint64_t>(n, 1) * TypeWidth(cols[v].type)));
 	}
 	auto *count = static_cast<uint64_t *>(scratch.Alloc(8));
-	uint64_t kept = rows;
+	uint64_t kept = n;
 	FusedLaunch la;
 	la.cols = ptrs.data();
 	la.outs = outs.data();
-	la.n = rows;
-	la.seg_offsets = args.seg_offsets_dev;
+	la.n = n;
+	la.seg_offsets = ranged ? ranges.seg : args.seg_offsets_dev;
 	la.nseg = args.nseg;
+	la.seg_rows = ranges.rows;
+	la.seg_lens = ranges.lens;
 	void *state = nullptr;
 	size_t state_bytes = 0;
 	if (!spec.predicates.empty()) {
 		la.count_device = count;
-		state_bytes = FusedKernel::StateBytes(rows);
+		state_bytes = FusedKernel::StateBytes(n);
 		state = scratch.Alloc(state_bytes);
 	}
 	kernel->Launch(la, state, state_bytes, stream);
 	if (!spec.predicates.empty()) {
 		VGPU_HIP(hipMemcpyAsync(&kept, count, 8, hipMemcpyDeviceToHost, stream));
 		VGPU_HIP(hipStreamSynchronize(stream));
-		if (rows == 0) {
+		if (n == 0) {
 			kept = 0;
 		}
 	}
 
+	if (kept == 0) { // nothing survived: every query has zero groups
+		if (args.seg_group_offsets_host != nullptr) {
+			std::fill(args.seg_group_offsets_host, args.seg_group_offsets_host + args.nseg + 1, 0ull);
+		}
+		return 0;
+	}
 	// GROUP BY (segment, key) over the surviving rows.
 	std::vector<DevCol> keys(2), values;
 	keys[0].data = outs[0];
@@ -315,6 +343,235 @@ uint64_t PerQueryFilterGroupByDevice(const BatchedGroupByArgs &args, const uint6
 	}
 	VGPU_HIP(hipStreamSynchronize(stream));
 	return total;
+}
+
+DeviceRanges UploadRanges(const uint64_t *rows, const uint64_t *lens, uint32_t nseg, uint64_t table_rows,
+                          Scratch &scratch, hipStream_t stream) {
+	if (nseg == 0) {
+		throw std::runtime_error("row ranges: at least one range");
+	}
+	std::vector<uint64_t> host(3ull * nseg + 1);
+	uint64_t *seg = host.data();
+	uint64_t v = 0;
+	for (uint32_t g = 0; g < nseg; g++) {
+		if (rows[g] > table_rows || lens[g] > table_rows - rows[g]) {
+			throw std::runtime_error("row ranges: range " + std::to_string(g) + " [" + std::to_string(rows[g]) +
+			                         ", +" + std::to_string(lens[g]) + ") is past the table's " +
+			                         std::to_string(table_rows) + " rows");
+		}
+		v += (rows[g] - v) & 3; // launch start = table start (mod 4)
+		seg[g] = v;
+		v += lens[g];
+	}
+	seg[nseg] = v;
+	std::copy(rows, rows + nseg, host.begin() + nseg + 1);
+	std::copy(lens, lens + nseg, host.begin() + 2 * nseg + 1);
+	auto *dev = static_cast<uint64_t *>(scratch.Alloc(host.size() * 8));
+	VGPU_HIP(hipMemcpyAsync(dev, host.data(), host.size() * 8, hipMemcpyHostToDevice, stream));
+	VGPU_HIP(hipStreamSynchronize(stream)); // `host` is pageable and local
+	DeviceRanges r;
+	r.seg = dev;
+	r.rows = dev + nseg + 1;
+	r.lens = dev + 2 * nseg + 1;
+	r.nvirt = v;
+	r.nseg = nseg;
+	return r;
+}
+
+uint64_t BatchedFilterProjectDevice(const std::vector<DevCol> &cols, const DeviceRanges &ranges,
+                                    const std::vector<FusedExpr> &preds, const std::vector<FusedExpr> &projs,
+                                    bool fast_math, std::vector<void *> &outs, std::vector<uint64_t> &kept_offsets,
+                                    Scratch &out_scratch, Scratch &scratch, hipStream_t stream) {
+	FusedSpec spec;
+	spec.fast_math = fast_math;
+	spec.ranges = true;
+	std::map<int, int> slot_of;
+	std::vector<const void *> ptrs;
+	auto slot = [&](int col) {
+		if (col < 0 || col >= (int)cols.size()) {
+			throw std::runtime_error("batched filter/project: expression input out of range");
+		}
+		auto it = slot_of.find(col);
+		if (it != slot_of.end()) {
+			return it->second;
+		}
+		const int s = (int)spec.cols.size();
+		FusedColumn fc;
+		fc.vtype = cols[col].type;
+		spec.cols.push_back(fc);
+		ptrs.push_back(cols[col].data);
+		slot_of.emplace(col, s);
+		return s;
+	};
+	for (auto p : preds) {
+		for (auto &s : p.slots) {
+			s = slot(s);
+		}
+		spec.predicates.push_back(p);
+	}
+	if (spec.predicates.empty()) {
+		FusedExpr all;
+		all.body = "((bool*)out)[idx] = true;";
+		all.out_type = VT_BOOLEAN;
+		spec.predicates.push_back(all);
+	}
+	for (auto p : projs) {
+		for (auto &s : p.slots) {
+			s = slot(s);
+		}
+		spec.projections.push_back(p);
+	}
+	const int seg_slot = (int)spec.cols.size();
+	{
+		FusedColumn fc;
+		fc.vtype = VT_INT32;
+		fc.segment_id = true;
+		spec.cols.push_back(fc);
+		ptrs.push_back(nullptr);
+	}
+	spec.projections.push_back(Identity(seg_slot, VT_INT32));
+	if (spec.cols.size() > (size_t)kFusedMaxCols || spec.projections.size() > (size_t)kFusedMaxOuts) {
+		throw std::runtime_error("batched filter/project: at most 15 columns and 15 projections");
+	}
+	auto kernel = GetFusedKernel(spec);
+	const uint64_t n = ranges.nvirt;
+	outs.clear();
+	for (auto &p : projs) {
+		outs.push_back(out_scratch.Alloc(std::max<uint64_t>(n, 1) * TypeWidth(p.out_type)));
+	}
+	auto *seg_out = static_cast<int32_t *>(scratch.Alloc(std::max<uint64_t>(n, 1) * 4));
+	std::vector<void *> all_outs = outs;
+	all_outs.push_back(seg_out);
+	auto *count = static_cast<uint64_t *>(scratch.Alloc(8));
+	const size_t state_bytes = FusedKernel::StateBytes(n);
+	void *state = scratch.Alloc(std::max<size_t>(state_bytes, 8));
+	FusedLaunch la;
+	la.cols = ptrs.data();
+	la.outs = all_outs.data();
+	la.n = n;
+	la.count_device = count;
+	la.seg_offsets = ranges.seg;
+	la.nseg = ranges.nseg;
+	la.seg_rows = ranges.rows;
+	la.seg_lens = ranges.lens;
+	kernel->Launch(la, state, state_bytes, stream);
+	auto *starts = static_cast<uint64_t *>(scratch.Alloc((ranges.nseg + 1) * 8));
+	uint64_t kept = 0;
+	VGPU_HIP(hipMemcpyAsync(&kept, count, 8, hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	if (n == 0) {
+		kept = 0;
+	}
+	hipLaunchKernelGGL(SegmentStartsKernel, dim3((ranges.nseg + 1 + 255) / 256), dim3(256), 0, stream, seg_out, kept,
+	                   ranges.nseg, starts);
+	VGPU_HIP(hipGetLastError());
+	kept_offsets.assign(ranges.nseg + 1, 0);
+	VGPU_HIP(hipMemcpyAsync(kept_offsets.data(), starts, (ranges.nseg + 1) * 8, hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	return kept;
+}
+
+JoinIndex::~JoinIndex() {
+	auto &pool = DevicePool::Instance();
+	for (void *p : {keys, static_cast<void *>(count), static_cast<void *>(sum), static_cast<void *>(dense)}) {
+		if (p != nullptr) {
+			pool.FreeAny(p);
+		}
+	}
+}
+
+std::unique_ptr<JoinIndex> BuildJoinIndex(const DevCol &key, const DevCol &payload, hipStream_t stream) {
+	if (!TypeIsInteger(key.type) || key.type == VT_BOOLEAN || !TypeIsInteger(payload.type)) {
+		throw std::runtime_error("join index: integer key (INT16/32/64) and integer payload");
+	}
+	if (key.rows != payload.rows) {
+		throw std::runtime_error("join index: key and payload of equal length");
+	}
+	auto index = std::unique_ptr<JoinIndex>(new JoinIndex());
+	index->key_type = key.type;
+	index->build_rows = key.rows;
+	if (key.rows == 0) {
+		return index;
+	}
+	PoolArena arena(stream);
+	std::vector<DevCol> keys = {key};
+	KeyPack pack;
+	const uint64_t bound = PlanGroupBy(keys, key.rows, pack, arena, stream);
+	auto &pool = DevicePool::Instance();
+	const uint64_t cap = std::max<uint64_t>(bound, 1);
+	index->keys = pool.Alloc(cap * TypeWidth(key.type), stream);
+	index->count = static_cast<uint64_t *>(pool.Alloc(cap * 8, stream));
+	index->sum = static_cast<uint64_t *>(pool.Alloc(cap * 16, stream));
+	std::vector<void *> fixed = {index->keys, index->count, index->sum};
+	size_t next = 0;
+	OutAlloc alloc = [&](size_t) { return fixed.at(next++); };
+	std::vector<AggReq> aggs = {{AGG_COUNT_STAR, -1, VT_INT64}, {AGG_SUM, 0, VT_HUGEINT}};
+	std::vector<void *> ko, ao;
+	SortTuning tuning = TuneSort(pack.total_bits, key.rows);
+	index->distinct =
+	    GroupByDevice(keys, {payload}, aggs, key.rows, pack, cap, alloc, ko, ao, tuning, arena, stream, true);
+	if (index->distinct > 0) {
+		// keys ascend: the span is last - first, as sign-extended 64-bit values
+		const int w = TypeWidth(key.type);
+		int64_t first = 0, last = 0;
+		uint8_t buf[2][8] = {};
+		VGPU_HIP(hipMemcpyAsync(buf[0], index->keys, w, hipMemcpyDeviceToHost, stream));
+		VGPU_HIP(hipMemcpyAsync(buf[1], static_cast<const uint8_t *>(index->keys) + (index->distinct - 1) * w, w,
+		                        hipMemcpyDeviceToHost, stream));
+		VGPU_HIP(hipStreamSynchronize(stream));
+		for (int i = 0; i < 2; i++) {
+			int64_t v = 0;
+			if (w == 2) {
+				int16_t t;
+				std::memcpy(&t, buf[i], 2);
+				v = t;
+			} else if (w == 4) {
+				int32_t t;
+				std::memcpy(&t, buf[i], 4);
+				v = t;
+			} else {
+				std::memcpy(&v, buf[i], 8);
+			}
+			(i == 0 ? first : last) = v;
+		}
+		const uint64_t span = (uint64_t)last - (uint64_t)first + 1; // exact: last >= first
+		if (span != 0 && span <= 2 * index->distinct + (1ull << 20)) {
+			auto *dense = static_cast<uint64_t *>(pool.Alloc(span * 8, stream));
+			auto *flag = static_cast<unsigned int *>(arena.Alloc(4));
+			if (BuildDenseIndex(key.type, index->keys, index->count, index->sum, index->distinct, (uint64_t)first,
+			                    span, dense, flag, stream)) {
+				index->dense = dense;
+				index->dense_min = (uint64_t)first;
+				index->dense_span = span;
+			} else {
+				pool.FreeAny(dense);
+			}
+		}
+	}
+	return index;
+}
+
+std::vector<JoinAggValue> BatchedJoinAggDevice(const DevCol &probe_key, const DeviceRanges &ranges,
+                                               const JoinIndex &index, Scratch &scratch, hipStream_t stream) {
+	if (probe_key.type != index.key_type) {
+		throw std::runtime_error("batched join: probe and build keys must have the same type");
+	}
+	auto *acc = static_cast<uint64_t *>(scratch.Alloc(3ull * ranges.nseg * 8));
+	RangeProbeAgg(probe_key.type, probe_key.data, ranges.seg, ranges.rows, ranges.lens, ranges.nseg, ranges.nvirt,
+	              index.keys, index.count, index.sum, index.distinct, index.dense, index.dense_min, index.dense_span,
+	              acc, stream);
+	std::vector<uint64_t> host(3ull * ranges.nseg);
+	VGPU_HIP(hipMemcpyAsync(host.data(), acc, host.size() * 8, hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	std::vector<JoinAggValue> out(2ull * ranges.nseg);
+	for (uint32_t g = 0; g < ranges.nseg; g++) {
+		out[2 * g].lo = host[3 * g + 1];
+		out[2 * g].hi = host[3 * g + 2];
+		out[2 * g].kind = JA_SUM_I128;
+		out[2 * g + 1].lo = host[3 * g];
+		out[2 * g + 1].kind = JA_COUNT;
+	}
+	return out;
 }
 
 } // namespace vgpu

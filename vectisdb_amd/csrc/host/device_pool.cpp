@@ -34,9 +34,12 @@ void *DevicePool::TakeCachedLocked(size_t size, hipStream_t stream) {
 	size_t limit = size < kLargeThreshold ? size * 2 : size + size / 4 + kLargeGranule;
 	for (auto it = free_.lower_bound(size); it != free_.end() && it->first <= limit; ++it) {
 		Block &b = it->second;
-		bool usable = b.stream == stream;
+		bool usable = b.idle || b.stream == stream;
 		if (!usable && b.ready != nullptr) {
 			usable = hipEventQuery(b.ready) == hipSuccess;
+			if (!usable) {
+				(void)hipGetLastError(); // "not ready" is an answer, not an error to report later
+			}
 		}
 		if (!usable) {
 			continue;
@@ -44,7 +47,8 @@ void *DevicePool::TakeCachedLocked(size_t size, hipStream_t stream) {
 		void *ptr = b.ptr;
 		size_t bsize = b.size;
 		if (b.ready != nullptr) {
-			spare_events_.push_back(b.ready);
+			// An event is never reused: its last record could be on a stream destroyed since.
+			(void)hipEventDestroy(b.ready);
 		}
 		free_.erase(it);
 		cached_ -= bsize;
@@ -96,13 +100,12 @@ void DevicePool::Free(void *ptr, hipStream_t stream) {
 	b.ptr = ptr;
 	b.size = it->second;
 	b.stream = stream;
-	if (!spare_events_.empty()) {
-		b.ready = spare_events_.back();
-		spare_events_.pop_back();
-	} else if (hipEventCreateWithFlags(&b.ready, hipEventDisableTiming) != hipSuccess) {
+	if (hipEventCreateWithFlags(&b.ready, hipEventDisableTiming) != hipSuccess) {
+		(void)hipGetLastError();
 		b.ready = nullptr;
 	}
 	if (b.ready != nullptr && hipEventRecord(b.ready, stream) != hipSuccess) {
+		(void)hipGetLastError();
 		(void)hipEventDestroy(b.ready);
 		b.ready = nullptr;
 	}
@@ -128,6 +131,23 @@ void DevicePool::FreeAny(void *ptr) {
 		return;
 	}
 	(void)hipFree(ptr);
+}
+
+void DevicePool::ForgetStream(hipStream_t stream) {
+	(void)hipStreamSynchronize(stream);
+	std::lock_guard<std::mutex> lock(mutex_);
+	for (auto &kv : free_) {
+		Block &b = kv.second;
+		if (b.stream != stream || b.idle) {
+			continue;
+		}
+		if (b.ready != nullptr) {
+			(void)hipEventDestroy(b.ready);
+			b.ready = nullptr;
+		}
+		b.stream = nullptr;
+		b.idle = true;
+	}
 }
 
 void DevicePool::Trim() {

@@ -35,6 +35,9 @@ public:
 
 	//! Releases every cached (free) block back to the driver. Synchronizes the device.
 	void Trim();
+	//! Call before destroying `stream`: synchronizes it and makes every cached block last used on it
+	//! reusable on any stream, so no event recorded on it is touched after it is gone.
+	void ForgetStream(hipStream_t stream);
 
 	size_t BytesInUse() const;
 	size_t BytesCached() const;
@@ -52,6 +55,7 @@ private:
 		size_t size = 0;
 		hipStream_t stream = nullptr;
 		hipEvent_t ready = nullptr; // recorded at free; block reusable elsewhere once complete
+		bool idle = false;          // its stream is gone and was drained: reusable anywhere
 	};
 	static size_t RoundSize(size_t bytes);
 	void *TakeCachedLocked(size_t size, hipStream_t stream);
@@ -59,7 +63,6 @@ private:
 	mutable std::mutex mutex_;
 	std::multimap<size_t, Block> free_;             // by size
 	std::unordered_map<void *, size_t> live_;        // ptr -> rounded size
-	std::vector<hipEvent_t> spare_events_;
 	size_t in_use_ = 0;
 	size_t cached_ = 0;
 	size_t peak_ = 0;

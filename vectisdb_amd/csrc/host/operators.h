@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <functional>
 #include <string>
+#include <memory>
 #include <vector>
 
 namespace vgpu {
@@ -93,6 +94,10 @@ struct BatchedGroupByArgs {
 	uint64_t rows = 0;
 	const uint64_t *seg_offsets_dev = nullptr;
 	uint32_t nseg = 0;
+	//! Row ranges instead of contiguous row groups (both host arrays of nseg entries, or both null): query g
+	//! reads table rows [range_rows[g], range_rows[g] + range_lens[g]); ranges may overlap or repeat.
+	const uint64_t *range_rows_host = nullptr;
+	const uint64_t *range_lens_host = nullptr;
 	std::vector<FusedExpr> preds;
 	int key_col = -1;
 	std::vector<int> value_cols;
@@ -113,6 +118,57 @@ uint64_t BatchedFilterGroupByDevice(const BatchedGroupByArgs &args, Scratch &scr
 //! `seg_offsets_host` holds the nseg + 1 row offsets on the host.
 uint64_t PerQueryFilterGroupByDevice(const BatchedGroupByArgs &args, const uint64_t *seg_offsets_host,
                                      Scratch &scratch, hipStream_t stream);
+
+//! Row ranges of one launch on the device: range g covers launch rows [seg[g], seg[g] + lens[g]) read from
+//! table rows rows[g] + (v - seg[g]). Launch starts are padded so seg[g] = rows[g] (mod 4): a lane's four
+//! rows stay 4-aligned in the table, which keeps the fused kernel's 128-bit loads.
+struct DeviceRanges {
+	const uint64_t *seg = nullptr;  // nseg + 1
+	const uint64_t *rows = nullptr; // nseg
+	const uint64_t *lens = nullptr; // nseg
+	uint64_t nvirt = 0;             // launch rows, seg[nseg]
+	uint32_t nseg = 0;
+};
+//! Checks every range against `table_rows` and uploads the launch layout (synchronizes the stream).
+DeviceRanges UploadRanges(const uint64_t *rows, const uint64_t *lens, uint32_t nseg, uint64_t table_rows,
+                          Scratch &scratch, hipStream_t stream);
+
+//! Many filter/projection queries over row ranges of one table in one generated kernel. Outputs (from
+//! `out_scratch`, one per projection) hold every query's kept rows, query after query in range order;
+//! query g's rows are [kept_offsets[g], kept_offsets[g+1]). Returns the total kept. Synchronizes.
+uint64_t BatchedFilterProjectDevice(const std::vector<DevCol> &cols, const DeviceRanges &ranges,
+                                    const std::vector<FusedExpr> &preds, const std::vector<FusedExpr> &projs,
+                                    bool fast_math, std::vector<void *> &outs, std::vector<uint64_t> &kept_offsets,
+                                    Scratch &out_scratch, Scratch &scratch, hipStream_t stream);
+
+//! Build-side index for aggregate-only joins: the distinct build keys ascending (key type), each with its
+//! row count (uint64) and the int128 sum of its payload (lo, hi words). Device memory from DevicePool,
+//! owned by the index.
+struct JoinIndex {
+	int key_type = VT_INT64;
+	void *keys = nullptr;
+	uint64_t *count = nullptr;
+	uint64_t *sum = nullptr;
+	uint64_t distinct = 0;
+	uint64_t build_rows = 0;
+	//! Direct-address table (BuildDenseIndex) when the keys are dense enough, else null.
+	uint64_t *dense = nullptr;
+	uint64_t dense_min = 0;  // ordered as uint64 of the sign-extended key
+	uint64_t dense_span = 0;
+	JoinIndex() = default;
+	JoinIndex(const JoinIndex &) = delete;
+	JoinIndex &operator=(const JoinIndex &) = delete;
+	~JoinIndex();
+};
+//! Reduces the build side (integer key, integer payload, no NULLs) with the GROUP BY operator; when the
+//! distinct keys span at most twice their number (plus 1M) and every count and sum packs into 64 bits, also
+//! builds the direct-address table.
+std::unique_ptr<JoinIndex> BuildJoinIndex(const DevCol &key, const DevCol &payload, hipStream_t stream);
+
+//! SUM(payload) and COUNT(*) of key = index key for every probe range, as JoinAggValue pairs
+//! (result[2g] = SUM as int128, result[2g+1] = COUNT). Synchronizes.
+std::vector<JoinAggValue> BatchedJoinAggDevice(const DevCol &probe_key, const DeviceRanges &ranges,
+                                               const JoinIndex &index, Scratch &scratch, hipStream_t stream);
 
 //! Autotuner: radix digit width per key width / row count, from measured sweeps (see tuning.cpp).
 SortTuning TuneSort(int total_bits, uint64_t rows);
