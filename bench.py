@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=20_000_000, help="rows in the CPU baseline sample")
     ap.add_argument("--only", default="", help="comma list of extra operator lines (none by default)")
     ap.add_argument("--sweep", action="store_true", help="also time each sort tuning candidate (stderr)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the static radix-pass variant choice")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (PMC calibration of FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -449,7 +450,10 @@ def main():
             (capi.MAX, 0, capi.INT64), (capi.SUM, 1, capi.FLOAT64), (capi.AVG, 1, capi.FLOAT64)]
     cap = capi.group_by_bound([capi.column(key)], rows)
     groups = 0
-    for i in range(args.warmup):
+    # The sort autotuner picks the radix-pass variant per shape on the first (warm-up) sort of that shape.
+    capi.sort_autotune(not args.no_autotune)
+    warmup = max(args.warmup, 0 if args.no_autotune else 1)
+    for i in range(warmup):
         _, _, groups = capi.group_by([key], [v1, v2], aggs, rows, capacity=cap)
     torch.cuda.synchronize()
     print(f"[bench] warmup done, groups={groups}", file=sys.stderr, flush=True)
@@ -511,6 +515,8 @@ def main():
             elif name == "batched":
                 operators.append(bench_batched(torch, capi, dev, 4))
             elif name == "served":
+                # served queries span many sort shapes: keep tuning runs out of their latencies
+                capi.sort_autotune(False)
                 operators.append(bench_served(torch, capi, dev))
             else:
                 continue
@@ -540,7 +546,7 @@ def main():
         "unit": "rows/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warmup,
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -551,6 +557,7 @@ def main():
                    "groups": groups, "step_algorithmic_bytes": step_bytes,
                    "step_gbs": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1)},
         "roofline": roof,
+        "sort_variants": capi.sort_autotune_report(),
         "cpu_baseline": cpu,
         "operators": operators,
     }

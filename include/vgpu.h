@@ -363,6 +363,12 @@ int vgpu_profile_reset(void);
  * needed size in *needed when buf is too small. */
 int vgpu_profile_read(char *buf, size_t cap, size_t *needed);
 
+/* Sort autotuner: when on, the first sort of each shape times every radix-pass variant (digit width, items
+ * per thread, payload prefetch) on its own input and caches the fastest for the shape. No reference
+ * counterpart (thrust picks its own sort). vgpu_sort_autotune_report writes a JSON array of the tuned shapes. */
+int vgpu_sort_autotune(int32_t on);
+int vgpu_sort_autotune_report(char *buf, size_t cap, size_t *needed);
+
 /* Device pool statistics (bytes). */
 int vgpu_pool_stats(uint64_t *in_use, uint64_t *cached, uint64_t *peak);
 int vgpu_pool_trim(void);

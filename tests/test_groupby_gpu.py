@@ -209,3 +209,27 @@ def test_filter_gt_i32(dev):
         x = rng.integers(-100, 100, n).astype(np.int32)
         got = capi.filter_gt_i32(torch.from_numpy(x).to(dev), 10)
         np.testing.assert_array_equal(got.cpu().numpy(), x[x > 10])
+
+
+def test_groupby_under_sort_autotuner(dev):
+    """With the autotuner on, the first sort of a shape runs every radix-pass variant and keeps the fastest;
+    the results stay bit-exact against the oracle on the tuning call and on the cached-choice call, and the
+    shape appears in the report with a valid choice."""
+    rng = np.random.default_rng(23)
+    n = 1_500_000
+    key = rng.integers(0, 300_000, n).astype(np.int64)
+    v1 = rng.integers(-10 ** 6, 10 ** 6 + 1, n).astype(np.int64)
+    v2 = rng.standard_normal(n)
+    capi.sort_autotune(True)
+    try:
+        g1 = _check(dev, key, v1, v2)   # tunes
+        g2 = _check(dev, key, v1, v2)   # uses the cached choice
+        assert g1 == g2
+        report = capi.sort_autotune_report()
+        mine = [r for r in report if r["npayload"] == 2 and r["log2rows"] == 20]
+        assert mine, report
+        for r in mine:
+            assert r["radix_bits"] in (8, 11) and r["items"] in (16, 32) and r["prefetch"] in (0, 1)
+            assert r["candidates"] >= 4 and r["ms"] > 0
+    finally:
+        capi.sort_autotune(False)

@@ -81,6 +81,8 @@ class ServerStats(C.Structure):
 TEMPLATE_FILTER_PROJECT, TEMPLATE_GROUP_BY, TEMPLATE_JOIN_AGG = 0, 1, 2
 
 _PROTOS = {
+    "vgpu_sort_autotune": (C.c_int, [C.c_int32]),
+    "vgpu_sort_autotune_report": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vgpu_server_create": (C.c_int, [C.POINTER(ServerOptions), C.POINTER(C.c_void_p)]),
     "vgpu_server_destroy": (C.c_int, [C.c_void_p]),
     "vgpu_server_add_template": (C.c_int, [C.c_void_p, C.POINTER(Template), C.POINTER(C.c_int32)]),
@@ -445,6 +447,19 @@ def join_pairs(akey, bkey, capacity=None, stream=None):
                               capacity, C.byref(count), _stream_ptr(stream)))
     n = count.value
     return a[:n], b[:n]
+
+
+def sort_autotune(on=True):
+    check(load().vgpu_sort_autotune(1 if on else 0))
+
+
+def sort_autotune_report():
+    import json
+    lib = load()
+    need = C.c_size_t(0)
+    buf = C.create_string_buffer(1 << 16)
+    check(lib.vgpu_sort_autotune_report(buf, len(buf), C.byref(need)))
+    return json.loads(buf.value.decode())
 
 
 def profile_enable(on=True):

@@ -119,6 +119,9 @@ void PlanKeyPackAligned(const void *col, int vtype, uint64_t omin, uint64_t omax
 struct SortTuning {
 	int radix_bits = 8;      // 8 or 11 (digit width upper bound)
 	int passes_override = 0; // 0 = derive from total_bits
+	int items = 16;          // items per thread of a pass with 8-bit digits: 16 or 32 (tile = 256 x items)
+	int prefetch = 1;        // load payload values before ranking (1) or just before their scatter (0)
+	bool fixed = false;      // variant forced (environment override): no defaults, no autotuning
 	//! Optional precomputed per-byte histograms (KeyRangeHist8) for a byte-aligned pack: skips the
 	//! histogram pass.
 	const uint32_t *hist8 = nullptr;
@@ -140,6 +143,17 @@ struct SortedRun {
 //! row ids" (a uint32 permutation). Requires n < 2^30.
 void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
                 uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream);
+//! SortPacked with exactly the given variant (no defaults, no autotuning).
+void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
+                    uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream);
+
+//! Sort autotuner: when on, the first sort of each shape (digit passes, rows rounded to a power of two,
+//! payload columns and bytes) times every variant on that sort's own data and keeps the fastest for the
+//! shape; later sorts of the shape use it. Off by default (the measured static choice is used).
+void SortAutotuneEnable(bool on);
+bool SortAutotuneEnabled();
+//! JSON array of the shapes tuned so far and their choice.
+std::string SortAutotuneReport();
 
 // ------------------------------------------------------------------ segmented multi-aggregate reduction
 
@@ -179,6 +193,7 @@ struct ReduceSpec {
 	// group key reconstruction
 	KeyPack pack;
 	void *key_out[kMaxKeyCols] = {};
+	uint64_t capacity = 0; // rows of every output (0: the input row count)
 };
 
 //! Reduces sorted keys/payload into one row per distinct key. `keys` may be nullptr: every row is then one

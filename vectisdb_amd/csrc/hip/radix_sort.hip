@@ -13,6 +13,7 @@
 //     across all 8 XCDs), so keys are read once and written once per pass.
 #include "common.h"
 #include "kernels.h"
+#include "onesweep.h"
 
 #include <algorithm>
 #include <cstring>
@@ -20,68 +21,7 @@
 namespace vgpu {
 namespace {
 
-constexpr int kSortBlock = 256;
-constexpr int kSortWaves = kSortBlock / kWave;
-constexpr uint32_t kStFlagAgg = 1u << 30;
-constexpr uint32_t kStFlagInc = 2u << 30;
-constexpr uint32_t kStValue = (1u << 30) - 1;
-
-struct PayloadArgs {
-	int n = 0;
-	const void *in[kMaxPayload] = {};
-	void *out[kMaxPayload] = {};
-	int width[kMaxPayload] = {};
-};
-
-struct PassList {
-	int npasses = 0;
-	int shift[8] = {};
-	int bits[8] = {};
-};
-
-__device__ __forceinline__ u64 PackRow(const KeyPack &p, uint64_t row) {
-	u64 packed = 0;
-#pragma unroll
-	for (int c = 0; c < kMaxKeyCols; c++) {
-		if (c < p.ncols && p.bits[c] > 0) {
-			u64 raw = LoadRawBits(p.col[c], row, p.width[c]);
-			u64 ord = OrderedKey(raw, p.width[c], p.is_signed[c] != 0);
-			packed |= (ord - p.min[c]) << p.shift[c];
-		}
-	}
-	return packed;
-}
-
-//! Loads the wave-striped items (row = wave_base + k * 64 + lane) of one column with the width switch
-//! hoisted out of the item loop, so all ITEMS loads issue back to back.
-template <int ITEMS, typename T>
-__device__ __forceinline__ void LoadWaveItemsT(const T *src, uint64_t wave_base, int lane, uint64_t n,
-                                               u64 (&out)[ITEMS]) {
-#pragma unroll
-	for (int k = 0; k < ITEMS; k++) {
-		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
-		out[k] = row < n ? (u64)src[row] : 0ull;
-	}
-}
-
-template <int ITEMS>
-__device__ __forceinline__ void LoadWaveItems(const void *src, int width, uint64_t wave_base, int lane, uint64_t n,
-                                              u64 (&out)[ITEMS]) {
-	switch (width) {
-	case 8:
-		LoadWaveItemsT<ITEMS>((const u64 *)src, wave_base, lane, n, out);
-		break;
-	case 4:
-		LoadWaveItemsT<ITEMS>((const uint32_t *)src, wave_base, lane, n, out);
-		break;
-	case 2:
-		LoadWaveItemsT<ITEMS>((const uint16_t *)src, wave_base, lane, n, out);
-		break;
-	default:
-		LoadWaveItemsT<ITEMS>((const uint8_t *)src, wave_base, lane, n, out);
-		break;
-	}
-}
+using namespace sortpass;
 
 struct RangeCols {
 	int ncols;
@@ -248,310 +188,21 @@ __global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, Pass
 	}
 }
 
-//! One digit pass. Template: R = digit table width (8 or 11 bits), KeyT = compact key (u32/u64), FIRST =
-//! the pass packs the raw key columns, NP = payload columns prefetched into registers (0..4).
-//!
-//! Tile = 256 threads x 16 items, wave-striped (lane-contiguous 64-item rows, so every global load is a
-//! coalesced 64 x width-byte line). Payload values are loaded at kernel entry so their latency overlaps the
-//! ranking and the look-back. One LDS buffer (TILE x 4 bytes per round) stages first the per-wave digit
-//! counters, then the locally sorted keys, then each payload column 32 bits at a time, so the scatter to
-//! global memory writes runs of consecutive addresses per digit bucket.
-template <int R, typename KeyT, bool FIRST, int NP>
-__global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const KeyT *keys_in, KeyT *keys_out,
-                                                             PayloadArgs pay, uint64_t n, int digit_shift,
-                                                             int digit_bits, const uint32_t *ghist,
-                                                             uint32_t *tile_state, uint32_t *tile_ticket) {
-	constexpr int ITEMS = 16;
-	constexpr int RADIX = 1 << R;
-	constexpr int TILE = kSortBlock * ITEMS;
-	constexpr int DPT = (RADIX + kSortBlock - 1) / kSortBlock;
-	constexpr int LDS_WORDS_KEYS = TILE * (int)(sizeof(KeyT) / 4);
-	constexpr int LDS_WORDS_COUNT = kSortWaves * RADIX;
-	constexpr int LDS_WORDS = LDS_WORDS_KEYS > LDS_WORDS_COUNT ? LDS_WORDS_KEYS : LDS_WORDS_COUNT;
-	__shared__ uint32_t s_buf[LDS_WORDS];
-	__shared__ uint32_t s_digit_base[RADIX];
-	__shared__ uint32_t s_scan[kSortWaves + 1];
-	__shared__ uint32_t s_tile;
-
-	const int tid = threadIdx.x;
-	const int lane = tid & (kWave - 1);
-	const int wave = tid / kWave;
-	if (tid == 0) {
-		s_tile = FetchAddAgent(tile_ticket, 1u);
-	}
-	for (int i = tid; i < kSortWaves * RADIX; i += kSortBlock) {
-		s_buf[i] = 0;
-	}
-	__syncthreads();
-	const uint32_t tile = s_tile;
-	const uint64_t tile_base = (uint64_t)tile * TILE;
-	const uint64_t wave_base = tile_base + (uint64_t)wave * ITEMS * kWave;
-	const uint32_t digit_mask = (1u << digit_bits) - 1;
-
-	KeyT key[ITEMS];
-	if constexpr (FIRST) {
-		if (pack.ncols == 1) {
-			// Common case: one key column. Width dispatch once, then 16 straight-line coalesced loads.
-			u64 raw[ITEMS];
-			LoadWaveItems<ITEMS>(pack.col[0], pack.width[0], wave_base, lane, n, raw);
-#pragma unroll
-			for (int k = 0; k < ITEMS; k++) {
-				u64 ord = OrderedKey(raw[k], pack.width[0], pack.is_signed[0] != 0);
-				key[k] = (KeyT)((ord - pack.min[0]) << pack.shift[0]);
-			}
-		} else {
-#pragma unroll
-			for (int k = 0; k < ITEMS; k++) {
-				uint64_t row = wave_base + (uint64_t)k * kWave + lane;
-				key[k] = row < n ? (KeyT)PackRow(pack, row) : (KeyT)0;
-			}
-		}
-	} else {
-#pragma unroll
-		for (int k = 0; k < ITEMS; k++) {
-			uint64_t row = wave_base + (uint64_t)k * kWave + lane;
-			key[k] = row < n ? keys_in[row] : (KeyT)0;
-		}
-	}
-	// Payload prefetch (raw bits), issued before the ranking so its latency is hidden.
-	u64 pv[NP > 0 ? NP : 1][ITEMS];
-#pragma unroll
-	for (int p = 0; p < NP; p++) {
-		if (p < pay.n) {
-			if (FIRST && pay.in[p] == nullptr) {
-#pragma unroll
-				for (int k = 0; k < ITEMS; k++) {
-					pv[p][k] = wave_base + (uint64_t)k * kWave + lane;
-				}
-			} else {
-				LoadWaveItems<ITEMS>(pay.in[p], pay.width[p], wave_base, lane, n, pv[p]);
-			}
-		}
-	}
-
-	// Wave-level multisplit: peers = lanes of this round holding the same digit. Rank = earlier rounds'
-	// count of that digit in this wave + earlier peers in this round, which keeps the sort stable.
-	uint32_t *my_counts = s_buf + wave * RADIX;
-	uint32_t pos[ITEMS];
-#pragma unroll
-	for (int k = 0; k < ITEMS; k++) {
-		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
-		bool valid = row < n;
-		uint32_t d = (uint32_t)(key[k] >> digit_shift) & digit_mask;
-		u64 peers = __ballot(valid);
-#pragma unroll
-		for (int b = 0; b < R; b++) {
-			if (b < digit_bits) {
-				bool bit = (d >> b) & 1u;
-				u64 bal = __ballot(bit);
-				peers &= bit ? bal : ~bal;
-			}
-		}
-		uint32_t r = (uint32_t)Popc64(peers & LaneMaskLt());
-		uint32_t base = valid ? my_counts[d] : 0u;
-		__builtin_amdgcn_wave_barrier();
-		if (valid && r == 0) {
-			my_counts[d] = base + (uint32_t)Popc64(peers);
-		}
-		__builtin_amdgcn_wave_barrier();
-		pos[k] = base + r;
-	}
-	__syncthreads();
-
-	// Per-digit block totals; the counters become each wave's exclusive offset within the digit.
-	uint32_t dtotal[DPT];
-#pragma unroll
-	for (int j = 0; j < DPT; j++) {
-		int d = tid * DPT + j;
-		dtotal[j] = 0;
-		if (d < RADIX) {
-			uint32_t run = 0;
-#pragma unroll
-			for (int w = 0; w < kSortWaves; w++) {
-				uint32_t c = s_buf[w * RADIX + d];
-				s_buf[w * RADIX + d] = run;
-				run += c;
-			}
-			dtotal[j] = run;
-			StoreAgent(tile_state + (uint64_t)tile * RADIX + d, (tile == 0 ? kStFlagInc : kStFlagAgg) | run);
-		}
-	}
-	uint32_t tsum = 0, gsum = 0;
-	uint32_t gh[DPT];
-#pragma unroll
-	for (int j = 0; j < DPT; j++) {
-		int d = tid * DPT + j;
-		tsum += dtotal[j];
-		gh[j] = d < RADIX ? ghist[d] : 0u;
-		gsum += gh[j];
-	}
-	uint32_t block_total;
-	uint32_t texcl = BlockExclusiveScan<uint32_t, kSortBlock>(tsum, s_scan, &block_total);
-	uint32_t gtotal;
-	uint32_t gexcl = BlockExclusiveScan<uint32_t, kSortBlock>(gsum, s_scan, &gtotal);
-
-	// Look-back: all of this thread's digits advance together, so their loads are issued in parallel.
-	uint32_t excl[DPT];
-	bool done[DPT];
-#pragma unroll
-	for (int j = 0; j < DPT; j++) {
-		excl[j] = 0;
-		done[j] = tile == 0 || tid * DPT + j >= RADIX;
-	}
-	int64_t t = (int64_t)tile - 1;
-	while (true) {
-		bool all = true;
-#pragma unroll
-		for (int j = 0; j < DPT; j++) {
-			all = all && done[j];
-		}
-		if (all) {
-			break;
-		}
-		uint32_t word[DPT];
-#pragma unroll
-		for (int j = 0; j < DPT; j++) {
-			word[j] = done[j] ? 0u : LoadAgent(tile_state + (uint64_t)t * RADIX + tid * DPT + j);
-		}
-		bool any_pending = false;
-#pragma unroll
-		for (int j = 0; j < DPT; j++) {
-			if (!done[j] && (word[j] & (3u << 30)) == 0) {
-				any_pending = true;
-			}
-		}
-		if (any_pending) {
-			__builtin_amdgcn_s_sleep(1);
-			continue; // re-read the same predecessor
-		}
-#pragma unroll
-		for (int j = 0; j < DPT; j++) {
-			if (!done[j]) {
-				excl[j] += word[j] & kStValue;
-				if ((word[j] & (3u << 30)) == kStFlagInc) {
-					done[j] = true;
-				}
-			}
-		}
-		t--;
-	}
-	uint32_t bstart = texcl;
-	uint32_t goff = gexcl;
-#pragma unroll
-	for (int j = 0; j < DPT; j++) {
-		int d = tid * DPT + j;
-		if (d < RADIX) {
-			if (tile > 0) {
-				StoreAgent(tile_state + (uint64_t)tile * RADIX + d, kStFlagInc | (excl[j] + dtotal[j]));
-			}
-			s_digit_base[d] = goff + excl[j] - bstart;
-#pragma unroll
-			for (int w = 0; w < kSortWaves; w++) {
-				s_buf[w * RADIX + d] += bstart;
-			}
-		}
-		bstart += dtotal[j];
-		goff += gh[j];
-	}
-	__syncthreads();
-
-#pragma unroll
-	for (int k = 0; k < ITEMS; k++) {
-		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
-		if (row < n) {
-			uint32_t d = (uint32_t)(key[k] >> digit_shift) & digit_mask;
-			pos[k] += my_counts[d];
-		}
-	}
-	__syncthreads();
-	KeyT *s_keys = reinterpret_cast<KeyT *>(s_buf);
-#pragma unroll
-	for (int k = 0; k < ITEMS; k++) {
-		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
-		if (row < n) {
-			s_keys[pos[k]] = key[k];
-		}
-	}
-	__syncthreads();
-
-	const uint32_t tile_count = (uint32_t)((n - tile_base) < (uint64_t)TILE ? (n - tile_base) : (uint64_t)TILE);
-	uint32_t dst[ITEMS];
-#pragma unroll
-	for (int j = 0; j < ITEMS; j++) {
-		uint32_t i = tid + j * kSortBlock;
-		dst[j] = 0;
-		if (i < tile_count) {
-			KeyT k2 = s_keys[i];
-			uint32_t d = (uint32_t)(k2 >> digit_shift) & digit_mask;
-			dst[j] = s_digit_base[d] + i;
-			keys_out[dst[j]] = k2;
-		}
-	}
-
-	// Payload: staged through LDS 32 bits at a time (two rounds for 8-byte columns).
-#pragma unroll
-	for (int p = 0; p < NP; p++) {
-		if (p < pay.n) {
-			const int width = pay.width[p];
-			const int rounds = width == 8 ? 2 : 1;
-			for (int half = 0; half < rounds; half++) {
-				__syncthreads();
-#pragma unroll
-				for (int k = 0; k < ITEMS; k++) {
-					uint64_t row = wave_base + (uint64_t)k * kWave + lane;
-					if (row < n) {
-						s_buf[pos[k]] = (uint32_t)(pv[p][k] >> (32 * half));
-					}
-				}
-				__syncthreads();
-				void *out = pay.out[p];
-#pragma unroll
-				for (int j = 0; j < ITEMS; j++) {
-					uint32_t i = tid + j * kSortBlock;
-					if (i < tile_count) {
-						uint32_t v = s_buf[i];
-						switch (width) {
-						case 8:
-							reinterpret_cast<uint32_t *>(out)[2 * (u64)dst[j] + half] = v;
-							break;
-						case 4:
-							reinterpret_cast<uint32_t *>(out)[dst[j]] = v;
-							break;
-						case 2:
-							reinterpret_cast<uint16_t *>(out)[dst[j]] = (uint16_t)v;
-							break;
-						default:
-							reinterpret_cast<uint8_t *>(out)[dst[j]] = (uint8_t)v;
-							break;
-						}
-					}
-				}
-			}
-		}
-	}
+//! Items per thread of the kernel variant that runs (11-bit digits always use 16).
+int EffectiveItems(int R, int items) {
+	return (R == 8 && items == 32) ? 32 : 16;
 }
 
-template <int R, typename KeyT, int NP>
-void LaunchPassNP(bool first, const KeyPack &pack, const void *keys_in, void *keys_out, const PayloadArgs &pay,
-                  uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state, uint32_t *ticket,
-                  hipStream_t stream, unsigned tiles) {
-	if (first) {
-		hipLaunchKernelGGL((OnesweepKernel<R, KeyT, true, NP>), dim3(tiles), dim3(kSortBlock), 0, stream, pack,
-		                   (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state, ticket);
-	} else {
-		hipLaunchKernelGGL((OnesweepKernel<R, KeyT, false, NP>), dim3(tiles), dim3(kSortBlock), 0, stream, pack,
-		                   (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state, ticket);
-	}
-}
-
-constexpr int kSortTile = kSortBlock * 16;
-
-template <int R, typename KeyT>
-void LaunchPass(bool first, const KeyPack &pack, const void *keys_in, void *keys_out, const PayloadArgs &pay,
-                uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state, uint32_t *ticket,
-                hipStream_t stream) {
-	const unsigned tiles = (unsigned)((n + kSortTile - 1) / kSortTile);
-	double key_in = first ? 0.0 : (double)sizeof(KeyT);
+//! One digit pass with the variant the tuning selects: 8-bit digits come with 16 or 32 items per thread and
+//! with or without the payload prefetch; 11-bit digits with 16 items and the prefetch.
+void LaunchPass(int R, bool keys64, bool first, const KeyPack &pack, const void *keys_in, void *keys_out,
+                const PayloadArgs &pay, uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state,
+                uint32_t *ticket, hipStream_t stream, int items, bool prefetch) {
+	const int it = EffectiveItems(R, items);
+	const uint64_t tile_rows = (uint64_t)kSortBlock * it;
+	const unsigned tiles = (unsigned)((n + tile_rows - 1) / tile_rows);
+	const size_t kw = keys64 ? 8 : 4;
+	double key_in = first ? 0.0 : (double)kw;
 	if (first) {
 		for (int c = 0; c < pack.ncols; c++) {
 			key_in += pack.width[c];
@@ -561,31 +212,25 @@ void LaunchPass(bool first, const KeyPack &pack, const void *keys_in, void *keys
 	for (int p = 0; p < pay.n; p++) {
 		pay_bytes += (first && pay.in[p] == nullptr ? 0.0 : pay.width[p]) + pay.width[p];
 	}
-	ProfileScope prof(first ? "radix_pass_first" : "radix_pass", stream,
-	                  (double)n * (key_in + sizeof(KeyT) + pay_bytes));
-	switch (pay.n) {
-	case 0:
-		LaunchPassNP<R, KeyT, 0>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket, stream, tiles);
-		break;
-	case 1:
-		LaunchPassNP<R, KeyT, 1>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket, stream, tiles);
-		break;
-	case 2:
-		LaunchPassNP<R, KeyT, 2>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket, stream, tiles);
-		break;
-	case 3:
-	case 4:
-		LaunchPassNP<R, KeyT, 4>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket, stream, tiles);
-		break;
-	default:
-		throw HipError("radix sort: at most 4 payload columns per pass");
+	ProfileScope prof(first ? "radix_pass_first" : "radix_pass", stream, (double)n * (key_in + kw + pay_bytes));
+	if (R == 11) {
+		LaunchVariant<11, 16, true>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
+		                            stream, tiles);
+	} else if (it == 32) {
+		if (prefetch) {
+			LaunchVariant<8, 32, true>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
+			                           stream, tiles);
+		} else {
+			LaunchVariant<8, 32, false>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
+			                            ticket, stream, tiles);
+		}
+	} else if (prefetch) {
+		LaunchVariant<8, 16, true>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
+		                           stream, tiles);
+	} else {
+		LaunchVariant<8, 16, false>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
+		                            stream, tiles);
 	}
-	VGPU_HIP(hipGetLastError());
-}
-
-template <int R>
-uint64_t TilesFor(uint64_t n) {
-	return (n + kSortTile - 1) / kSortTile;
 }
 
 } // namespace
@@ -715,8 +360,8 @@ bool PlanKeyPack(const void *const *cols, const int *vtypes, int ncols, const ui
 	return true;
 }
 
-void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
-                uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream) {
+void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
+                    uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream) {
 	if (n >= (1ull << 30)) {
 		throw HipError("radix sort: at most 2^30 - 1 rows per sort call");
 	}
@@ -765,7 +410,8 @@ void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *p
 	out.passes = P;
 	out.radix_bits = widest;
 	const int RADIX = 1 << R;
-	const uint64_t tiles = R == 8 ? TilesFor<8>(n) : TilesFor<11>(n);
+	const uint64_t tile_rows = (uint64_t)kSortBlock * EffectiveItems(R, tuning.items);
+	const uint64_t tiles = (n + tile_rows - 1) / tile_rows;
 
 	// Scratch: ping-pong keys and payloads; histogram; per-pass tile state + ticket.
 	void *kbuf[2] = {scratch.Alloc(n * kw), P > 1 ? scratch.Alloc(n * kw) : nullptr};
@@ -818,23 +464,8 @@ void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *p
 		uint32_t *st = state + (uint64_t)pass * tiles * RADIX;
 		const uint32_t *gh = hist + (uint64_t)pass * RADIX;
 		bool first = pass == 0;
-		if (R == 8) {
-			if (keys64) {
-				LaunchPass<8, u64>(first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass], passes.bits[pass],
-				                   gh, st, tickets + pass, stream);
-			} else {
-				LaunchPass<8, uint32_t>(first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass],
-				                        passes.bits[pass], gh, st, tickets + pass, stream);
-			}
-		} else {
-			if (keys64) {
-				LaunchPass<11, u64>(first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass],
-				                    passes.bits[pass], gh, st, tickets + pass, stream);
-			} else {
-				LaunchPass<11, uint32_t>(first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass],
-				                         passes.bits[pass], gh, st, tickets + pass, stream);
-			}
-		}
+		LaunchPass(R, keys64, first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass], passes.bits[pass], gh,
+		           st, tickets + pass, stream, tuning.items, tuning.prefetch != 0);
 		cur_keys = kbuf[slot];
 		for (int p = 0; p < npayload; p++) {
 			cur_pay[p] = pbuf[p][slot];

@@ -443,6 +443,108 @@ __global__ void __launch_bounds__(kRedBlock)
 	}
 }
 
+//! Count-only reduce (every aggregate is a COUNT): a run-length encoding of the sorted keys. Each tile flags
+//! the rows whose key differs from the row before, ranks the flags (block scan, then decoupled look-back
+//! over per-tile head counts) and writes each group's key and first row; RleCountKernel turns first rows
+//! into counts. No aggregate state is scanned or stitched across tiles.
+template <typename KeyT>
+__global__ void __launch_bounds__(kRedBlock)
+    RleHeadsKernel(const KeyT *keys, uint64_t n, KeyOutDesc ko, u64 *starts, u64 *tile_state, uint32_t *ticket,
+                   u64 *groups_out, uint64_t ntiles) {
+	__shared__ uint32_t s_tile;
+	__shared__ u64 s_scan[kRedWaves + 1];
+	__shared__ u64 s_hb;
+	__shared__ KeyT s_keys[kRedItems * kSlotStride];
+	__shared__ KeyT s_prev;
+	const int tid = threadIdx.x;
+	const int lane = tid & (kWave - 1);
+	const int wave = tid / kWave;
+	if (tid == 0) {
+		s_tile = FetchAddAgent(ticket, 1u);
+	}
+	__syncthreads();
+	const uint32_t tile = s_tile;
+	const uint64_t tile_base = (uint64_t)tile * kRedTile;
+	const uint32_t tile_count = (uint32_t)((n - tile_base) < (uint64_t)kRedTile ? (n - tile_base) : kRedTile);
+	StageTransposed<KeyT>(s_keys, keys + tile_base, tile_count);
+	if (tid == 0) {
+		s_prev = tile_base > 0 ? keys[tile_base - 1] : (KeyT)0;
+	}
+	__syncthreads();
+	const uint32_t local0 = (uint32_t)tid * kRedItems;
+	KeyT k[kRedItems];
+	bool head[kRedItems];
+	u64 hc = 0;
+	KeyT before = tid == 0 ? s_prev : s_keys[(kRedItems - 1) * kSlotStride + tid - 1];
+#pragma unroll
+	for (int i = 0; i < kRedItems; i++) {
+		const uint32_t lr = local0 + i;
+		k[i] = lr < tile_count ? s_keys[i * kSlotStride + tid] : (KeyT)0;
+		head[i] = lr < tile_count && (tile_base + lr == 0 || k[i] != before);
+		before = k[i];
+		hc += head[i] ? 1 : 0;
+	}
+	u64 H;
+	const u64 hb_excl = BlockExclusiveScan<u64, kRedBlock>(hc, s_scan, &H);
+	if (tid == 0) {
+		StoreAgent(tile_state + tile, LbPack(tile == 0 ? kLbInclusive : kLbAggregate, H));
+	}
+	if (wave == 0) {
+		const u64 excl = tile == 0 ? 0ull : LookbackExclusiveWave(tile_state, tile);
+		if (lane == 0) {
+			if (tile > 0) {
+				StoreAgent(tile_state + tile, LbPack(kLbInclusive, excl + H));
+			}
+			s_hb = excl;
+		}
+	}
+	__syncthreads();
+	u64 gid = s_hb + hb_excl;
+#pragma unroll
+	for (int i = 0; i < kRedItems; i++) {
+		if (head[i]) {
+			WriteKeys(ko, (u64)k[i], gid);
+			starts[gid] = tile_base + local0 + i;
+			gid++;
+		}
+	}
+	if (tid == 0 && tile == ntiles - 1) {
+		*groups_out = s_hb + H;
+	}
+}
+
+//! count[g] = first row of group g + 1 (or n) - first row of group g, into every COUNT output.
+__global__ void RleCountKernel(const u64 *starts, const u64 *groups, uint64_t n, AggOutDesc o) {
+	const u64 g_total = *groups;
+	const u64 stride = (u64)gridDim.x * blockDim.x;
+	for (u64 g = (u64)blockIdx.x * blockDim.x + threadIdx.x; g < g_total; g += stride) {
+		const u64 end = g + 1 < g_total ? starts[g + 1] : n;
+		const long long c = (long long)(end - starts[g]);
+		for (int i = 0; i < o.naggs; i++) {
+			((long long *)o.out[i])[g] = c;
+		}
+	}
+}
+
+template <typename KeyT>
+void LaunchRle(const SortedRun &run, uint64_t n, uint64_t capacity, const AggOutDesc &od, const KeyOutDesc &ko,
+               u64 *groups_device, Scratch &scratch, hipStream_t stream) {
+	const uint64_t ntiles = (n + kRedTile - 1) / kRedTile;
+	auto *lb = static_cast<u64 *>(scratch.Alloc((ntiles + 1) * sizeof(u64)));
+	auto *ticket = reinterpret_cast<uint32_t *>(lb + ntiles);
+	auto *starts = static_cast<u64 *>(scratch.Alloc(std::max<uint64_t>(capacity, 1) * sizeof(u64)));
+	FillU64(lb, 0ull, ntiles + 1, stream);
+	{
+		ProfileScope prof("seg_count", stream, (double)n * sizeof(KeyT));
+		hipLaunchKernelGGL((RleHeadsKernel<KeyT>), dim3((unsigned)ntiles), dim3(kRedBlock), 0, stream,
+		                   (const KeyT *)run.keys, n, ko, starts, lb, ticket, groups_device, ntiles);
+		VGPU_HIP(hipGetLastError());
+	}
+	const unsigned grid = (unsigned)std::min<uint64_t>((std::max<uint64_t>(capacity, 1) + 255) / 256, 8192);
+	hipLaunchKernelGGL(RleCountKernel, dim3(grid), dim3(256), 0, stream, starts, groups_device, n, od);
+	VGPU_HIP(hipGetLastError());
+}
+
 constexpr int kFixBlock = 256;
 constexpr int kFixWaves = kFixBlock / kWave;
 
@@ -631,6 +733,21 @@ void SegmentedReduce(const SortedRun &run, uint64_t n, const ReduceSpec &spec, u
 	ko.pack = spec.pack;
 	for (int c = 0; c < spec.pack.ncols; c++) {
 		ko.out[c] = spec.key_out[c];
+	}
+	bool count_only = run.keys != nullptr && spec.ncomp == 1 && spec.comp[0].kind == COMP_COUNT && spec.naggs > 0;
+	for (int i = 0; i < spec.naggs; i++) {
+		count_only = count_only && (spec.aggs[i].kind == AGG_COUNT_STAR || spec.aggs[i].kind == AGG_COUNT) &&
+		             spec.aggs[i].out_type == VT_INT64;
+	}
+	if (count_only) {
+		// Groups never outnumber rows, so n bounds the first-row buffer when no capacity is given.
+		const uint64_t cap = spec.capacity == 0 ? n : std::min<uint64_t>(spec.capacity, n);
+		if (run.keys64) {
+			LaunchRle<u64>(run, n, cap, od, ko, groups_device, scratch, stream);
+		} else {
+			LaunchRle<uint32_t>(run, n, cap, od, ko, groups_device, scratch, stream);
+		}
+		return;
 	}
 	if (run.keys64) {
 		DispatchNP<u64>(run, n, pay, cd, od, ko, groups_device, scratch, stream);

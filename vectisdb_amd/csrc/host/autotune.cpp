@@ -1,0 +1,196 @@
+// autotune.cpp -- the sort variant choice: a measured static default, and an online autotuner.
+//
+// A radix sort here has four free parameters per shape: the digit width (8 or 11 bits, which fixes the number
+// of passes), items per thread (16 or 32, which fixes the tile), and whether payload values are loaded before
+// the ranking or just before their scatter (registers versus latency hiding). With the autotuner on, the
+// first sort of a shape runs every variant twice on its own input, keeps the fastest, and caches it for the
+// shape; every variant produces the same stable order, so the choice never changes a result.
+#include "device_pool.h"
+#include "operators.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <tuple>
+
+namespace vgpu {
+
+namespace {
+
+std::atomic<bool> g_autotune {std::getenv("VGPU_AUTOTUNE") != nullptr && std::atoi(std::getenv("VGPU_AUTOTUNE")) != 0};
+
+struct ShapeKey {
+	int passes8;
+	int passes11;
+	int log2rows;
+	int npayload;
+	int payload_bytes;
+	bool hist8;
+	bool operator<(const ShapeKey &o) const {
+		return std::tie(passes8, passes11, log2rows, npayload, payload_bytes, hist8) <
+		       std::tie(o.passes8, o.passes11, o.log2rows, o.npayload, o.payload_bytes, o.hist8);
+	}
+};
+
+struct Choice {
+	SortTuning tuning;
+	double ms = 0;
+	int candidates = 0;
+};
+
+std::mutex g_mutex;
+std::map<ShapeKey, Choice> g_cache;
+
+//! Below this many rows a sort is launch-bound and not worth tuning.
+constexpr uint64_t kTuneMinRows = 1ull << 20;
+
+int Log2Rows(uint64_t n) {
+	int l = 0;
+	while ((2ull << l) <= n) {
+		l++;
+	}
+	return l;
+}
+
+//! Measured on MI355X (100M rows, 20-bit key + two 8-byte payloads; 256M rows, 25-bit key alone): sorts that
+//! carry payload are fastest with 32 items per thread and the payload loaded at its scatter (fewer live
+//! registers, higher occupancy); key-only sorts with 16 items.
+void StaticDefault(SortTuning &t, int npayload) {
+	t.items = npayload > 0 ? 32 : 16;
+	t.prefetch = 0;
+}
+
+double TimeSort(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
+                uint64_t n, const SortTuning &t, hipStream_t stream, hipEvent_t e0, hipEvent_t e1) {
+	PoolArena tmp(stream);
+	SortedRun run;
+	VGPU_HIP(hipEventRecord(e0, stream));
+	SortPackedWith(pack, payload_in, payload_width, npayload, n, t, run, tmp, stream);
+	VGPU_HIP(hipEventRecord(e1, stream));
+	VGPU_HIP(hipEventSynchronize(e1));
+	float ms = 0;
+	VGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
+	return ms;
+}
+
+} // namespace
+
+void SortAutotuneEnable(bool on) {
+	g_autotune.store(on);
+}
+
+bool SortAutotuneEnabled() {
+	return g_autotune.load();
+}
+
+std::string SortAutotuneReport() {
+	std::lock_guard<std::mutex> lock(g_mutex);
+	std::ostringstream o;
+	o << "[";
+	bool first = true;
+	for (auto &kv : g_cache) {
+		const auto &k = kv.first;
+		const auto &c = kv.second;
+		o << (first ? "" : ",") << "{\"passes8\":" << k.passes8 << ",\"log2rows\":" << k.log2rows
+		  << ",\"npayload\":" << k.npayload << ",\"payload_bytes\":" << k.payload_bytes
+		  << ",\"radix_bits\":" << c.tuning.radix_bits << ",\"items\":" << c.tuning.items
+		  << ",\"prefetch\":" << c.tuning.prefetch << ",\"ms\":" << c.ms << ",\"candidates\":" << c.candidates
+		  << "}";
+		first = false;
+	}
+	o << "]";
+	return o.str();
+}
+
+void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
+                uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream) {
+	SortTuning t = tuning;
+	if (t.fixed) {
+		SortPackedWith(pack, payload_in, payload_width, npayload, n, t, out, scratch, stream);
+		return;
+	}
+	StaticDefault(t, npayload);
+	if (!g_autotune.load() || n < kTuneMinRows) {
+		SortPackedWith(pack, payload_in, payload_width, npayload, n, t, out, scratch, stream);
+		return;
+	}
+	const int bits = std::max(pack.total_bits, 1);
+	int pbytes = 0;
+	for (int p = 0; p < npayload; p++) {
+		pbytes += payload_width[p];
+	}
+	const ShapeKey key {(bits + 7) / 8, (bits + 10) / 11, Log2Rows(n), npayload, pbytes, tuning.hist8 != nullptr};
+	{
+		std::lock_guard<std::mutex> lock(g_mutex);
+		auto it = g_cache.find(key);
+		if (it != g_cache.end()) {
+			SortTuning c = it->second.tuning;
+			c.hist8 = tuning.hist8;
+			SortPackedWith(pack, payload_in, payload_width, npayload, n, c, out, scratch, stream);
+			return;
+		}
+	}
+	std::vector<SortTuning> cands;
+	for (int items : {16, 32}) {
+		for (int pf : {1, 0}) {
+			if (npayload == 0 && pf == 0) {
+				continue; // the prefetch flag only moves payload loads
+			}
+			SortTuning c = tuning;
+			c.radix_bits = 8;
+			c.items = items;
+			c.prefetch = pf;
+			cands.push_back(c);
+		}
+	}
+	if (key.passes11 < key.passes8) {
+		SortTuning c = tuning;
+		c.radix_bits = 11;
+		c.items = 16;
+		c.prefetch = 1;
+		cands.push_back(c);
+	}
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	VGPU_HIP(hipEventCreate(&e0));
+	VGPU_HIP(hipEventCreate(&e1));
+	Choice best;
+	best.ms = 1e300;
+	try {
+		for (const auto &c : cands) {
+			double ms = 1e300;
+			for (int rep = 0; rep < 2; rep++) {
+				ms = std::min(ms, TimeSort(pack, payload_in, payload_width, npayload, n, c, stream, e0, e1));
+			}
+			if (ms < best.ms) {
+				best.ms = ms;
+				best.tuning = c;
+			}
+		}
+	} catch (...) {
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		throw;
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	best.candidates = (int)cands.size();
+	best.tuning.hist8 = nullptr;
+	if (std::getenv("VGPU_AUTOTUNE_VERBOSE") != nullptr) {
+		std::fprintf(stderr, "[autotune] passes8=%d rows=2^%d payload=%d/%dB -> radix %d items %d prefetch %d (%.3f ms)\n",
+		             key.passes8, key.log2rows, npayload, pbytes, best.tuning.radix_bits, best.tuning.items,
+		             best.tuning.prefetch, best.ms);
+	}
+	{
+		std::lock_guard<std::mutex> lock(g_mutex);
+		g_cache.emplace(key, best);
+	}
+	SortTuning c = best.tuning;
+	c.hist8 = tuning.hist8;
+	SortPackedWith(pack, payload_in, payload_width, npayload, n, c, out, scratch, stream);
+}
+
+} // namespace vgpu

@@ -263,6 +263,23 @@ int vgpu_profile_read(char *buf, size_t cap, size_t *needed) {
 	});
 }
 
+int vgpu_sort_autotune(int32_t on) {
+	return Guard([&] { SortAutotuneEnable(on != 0); });
+}
+
+int vgpu_sort_autotune_report(char *buf, size_t cap, size_t *needed) {
+	return Guard([&] {
+		const std::string r = SortAutotuneReport();
+		if (needed != nullptr) {
+			*needed = r.size() + 1;
+		}
+		if (buf == nullptr || cap < r.size() + 1) {
+			throw std::runtime_error("sort autotune report: buffer too small");
+		}
+		std::memcpy(buf, r.c_str(), r.size() + 1);
+	});
+}
+
 int vgpu_pool_stats(uint64_t *in_use, uint64_t *cached, uint64_t *peak) {
 	return Guard([&] {
 		auto &pool = DevicePool::Instance();

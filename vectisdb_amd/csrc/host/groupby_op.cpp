@@ -224,6 +224,7 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
 		spec.aggs[i].out = agg_out.back();
 	}
 	spec.pack = pack;
+	spec.capacity = cap;
 
 	auto *groups_dev = groups_device;
 	if (groups_dev == nullptr) {

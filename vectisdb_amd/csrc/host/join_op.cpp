@@ -73,6 +73,7 @@ SideRuns ReduceSide(const DevCol &key, const std::vector<DevCol> &values, const 
 	spec.aggs[0].out_type = VT_INT64;
 	spec.aggs[0].out = scratch.Alloc(cap * 8);
 	spec.pack = pack;
+	spec.capacity = cap;
 	spec.key_out[0] = scratch.Alloc(cap * TypeWidth(key.type));
 	// the reduce reads no payload: hide the permutation from it
 	SortedRun keys_only = run;

@@ -9,11 +9,27 @@
 
 namespace vgpu {
 
+namespace {
+//! Optional overrides for sweeps: VGPU_SORT_ITEMS (16/32), VGPU_SORT_PREFETCH (0/1).
+void ApplyVariantOverrides(SortTuning &t) {
+	if (const char *env = std::getenv("VGPU_SORT_ITEMS")) {
+		t.items = std::atoi(env) >= 32 ? 32 : 16;
+		t.fixed = true;
+	}
+	if (const char *env = std::getenv("VGPU_SORT_PREFETCH")) {
+		t.prefetch = std::atoi(env) != 0 ? 1 : 0;
+		t.fixed = true;
+	}
+}
+} // namespace
+
 SortTuning TuneSort(int total_bits, uint64_t rows) {
 	SortTuning t;
+	ApplyVariantOverrides(t);
 	if (const char *env = std::getenv("VGPU_RADIX_BITS")) {
 		int v = std::atoi(env);
 		t.radix_bits = v >= 11 ? 11 : 8;
+		t.fixed = true;
 		return t;
 	}
 	// Measured on MI355X, 100M rows, 20-bit keys + 16 B payload: three 8-bit passes take 4.8 ms, two 11-bit
