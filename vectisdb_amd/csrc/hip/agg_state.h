@@ -84,13 +84,80 @@ struct CompDesc {
 	int in_type[kMaxComponents];
 };
 
+//! Component signatures. DynSig reads each component's kind, input type and payload column from the
+//! CompDesc at run time (any mix of up to kMaxComponents). StaticSig<K...> fixes them at compile time, one
+//! code per component: kind | in_type << 4 | (payload + 1) << 8 (COUNT is just its kind), so once the
+//! component loops unroll every switch folds away and only the arithmetic of those components remains.
+struct DynSig {
+	static constexpr bool kStatic = false;
+	static constexpr int n = kMaxComponents;
+	__host__ __device__ static constexpr int code(int) {
+		return 0;
+	}
+};
+
+template <int... K>
+struct StaticSig {
+	static constexpr bool kStatic = true;
+	static constexpr int n = sizeof...(K);
+	__host__ __device__ static constexpr int code(int c) {
+		int i = 0, r = 0;
+		((r = (i++ == c) ? K : r), ...);
+		return r;
+	}
+};
+
+constexpr int SigCode(int kind, int in_type = 0, int payload = -1) {
+	return kind == COMP_COUNT ? COMP_COUNT : (kind | (in_type << 4) | ((payload + 1) << 8));
+}
+
+template <typename S>
+__device__ __forceinline__ bool CHas(const CompDesc &d, int c) {
+	if constexpr (S::kStatic) {
+		return true;
+	} else {
+		return c < d.ncomp;
+	}
+}
+template <typename S>
+__device__ __forceinline__ int CKind(const CompDesc &d, int c) {
+	if constexpr (S::kStatic) {
+		return S::code(c) & 15;
+	} else {
+		return d.kind[c];
+	}
+}
+template <typename S>
+__device__ __forceinline__ int CType(const CompDesc &d, int c) {
+	if constexpr (S::kStatic) {
+		return (S::code(c) >> 4) & 15;
+	} else {
+		return d.in_type[c];
+	}
+}
+template <typename S>
+__device__ __forceinline__ int CPayload(const CompDesc &d, int c) {
+	if constexpr (S::kStatic) {
+		return (S::code(c) >> 8) - 1;
+	} else {
+		return d.payload[c];
+	}
+}
+//! The component keeps a second word (SUM_I128 high, SUM_DD low).
+template <typename S>
+__device__ __forceinline__ bool CHasB(const CompDesc &d, int c) {
+	const int k = CKind<S>(d, c);
+	return k == COMP_SUM_I128 || k == COMP_SUM_DD;
+}
+
+template <typename S = DynSig>
 __device__ __forceinline__ void StateIdentity(const CompDesc &d, CompState &s) {
 #pragma unroll
-	for (int c = 0; c < kMaxComponents; c++) {
+	for (int c = 0; c < S::n; c++) {
 		s.a[c] = 0;
 		s.b[c] = 0;
-		if (c < d.ncomp) {
-			switch (d.kind[c]) {
+		if (CHas<S>(d, c)) {
+			switch (CKind<S>(d, c)) {
 			case COMP_MIN_I64:
 				s.a[c] = (u64)0x7fffffffffffffffll;
 				break;
@@ -135,12 +202,12 @@ __device__ __forceinline__ double RawAsF64(u64 raw, int vtype) {
 }
 
 //! Folds one row into the state. `vals[p]` holds the raw bits of payload column p at this row.
-template <int NP>
+template <int NP, typename S = DynSig>
 __device__ __forceinline__ void StateAddRow(const CompDesc &d, CompState &s, const u64 (&vals)[NP]) {
 #pragma unroll
-	for (int c = 0; c < kMaxComponents; c++) {
-		if (c < d.ncomp) {
-			const int p = d.payload[c];
+	for (int c = 0; c < S::n; c++) {
+		if (CHas<S>(d, c)) {
+			const int p = CPayload<S>(d, c);
 			u64 raw = 0;
 #pragma unroll
 			for (int q = 0; q < NP; q++) {
@@ -148,43 +215,44 @@ __device__ __forceinline__ void StateAddRow(const CompDesc &d, CompState &s, con
 					raw = vals[q];
 				}
 			}
-			switch (d.kind[c]) {
+			const int t = CType<S>(d, c);
+			switch (CKind<S>(d, c)) {
 			case COMP_COUNT:
 				s.a[c] += 1;
 				break;
 			case COMP_SUM_I128:
-				I128AddI64(s.a[c], s.b[c], RawAsI64(raw, d.in_type[c]));
+				I128AddI64(s.a[c], s.b[c], RawAsI64(raw, t));
 				break;
 			case COMP_SUM_DD: {
 				double hi = AsDouble(s.a[c]), lo = AsDouble(s.b[c]);
-				DdAdd(hi, lo, RawAsF64(raw, d.in_type[c]));
+				DdAdd(hi, lo, RawAsF64(raw, t));
 				s.a[c] = AsBits(hi);
 				s.b[c] = AsBits(lo);
 				break;
 			}
 			case COMP_MIN_I64: {
-				long long v = RawAsI64(raw, d.in_type[c]);
+				long long v = RawAsI64(raw, t);
 				if (v < (long long)s.a[c]) {
 					s.a[c] = (u64)v;
 				}
 				break;
 			}
 			case COMP_MAX_I64: {
-				long long v = RawAsI64(raw, d.in_type[c]);
+				long long v = RawAsI64(raw, t);
 				if (v > (long long)s.a[c]) {
 					s.a[c] = (u64)v;
 				}
 				break;
 			}
 			case COMP_MIN_F64: {
-				double v = RawAsF64(raw, d.in_type[c]);
+				double v = RawAsF64(raw, t);
 				if (DuckLess(v, AsDouble(s.a[c]))) {
 					s.a[c] = AsBits(v);
 				}
 				break;
 			}
 			case COMP_MAX_F64: {
-				double v = RawAsF64(raw, d.in_type[c]);
+				double v = RawAsF64(raw, t);
 				if (DuckLess(AsDouble(s.a[c]), v)) {
 					s.a[c] = AsBits(v);
 				}
@@ -198,11 +266,12 @@ __device__ __forceinline__ void StateAddRow(const CompDesc &d, CompState &s, con
 }
 
 //! s = s (earlier rows) combined with t (later rows).
+template <typename S = DynSig>
 __device__ __forceinline__ void StateCombine(const CompDesc &d, CompState &s, const CompState &t) {
 #pragma unroll
-	for (int c = 0; c < kMaxComponents; c++) {
-		if (c < d.ncomp) {
-			switch (d.kind[c]) {
+	for (int c = 0; c < S::n; c++) {
+		if (CHas<S>(d, c)) {
+			switch (CKind<S>(d, c)) {
 			case COMP_COUNT:
 				s.a[c] += t.a[c];
 				break;
@@ -243,40 +312,52 @@ __device__ __forceinline__ void StateCombine(const CompDesc &d, CompState &s, co
 	}
 }
 
+template <typename S = DynSig>
 __device__ __forceinline__ void StateShflUp(const CompDesc &d, CompState &dst, const CompState &src, int delta) {
 #pragma unroll
-	for (int c = 0; c < kMaxComponents; c++) {
-		if (c < d.ncomp) {
+	for (int c = 0; c < S::n; c++) {
+		if (CHas<S>(d, c)) {
 			dst.a[c] = __shfl_up(src.a[c], delta);
-			dst.b[c] = __shfl_up(src.b[c], delta);
+			if (!S::kStatic || CHasB<S>(d, c)) {
+				dst.b[c] = __shfl_up(src.b[c], delta);
+			} else {
+				dst.b[c] = 0;
+			}
 		}
 	}
 }
 
+template <typename S = DynSig>
 __device__ __forceinline__ void StateShflXor(const CompDesc &d, CompState &dst, const CompState &src, int mask) {
 #pragma unroll
-	for (int c = 0; c < kMaxComponents; c++) {
-		if (c < d.ncomp) {
+	for (int c = 0; c < S::n; c++) {
+		if (CHas<S>(d, c)) {
 			dst.a[c] = __shfl_xor(src.a[c], mask);
-			dst.b[c] = __shfl_xor(src.b[c], mask);
+			if (!S::kStatic || CHasB<S>(d, c)) {
+				dst.b[c] = __shfl_xor(src.b[c], mask);
+			} else {
+				dst.b[c] = 0;
+			}
 		}
 	}
 }
 
+template <typename S = DynSig>
 __device__ __forceinline__ void StateStore(const CompDesc &d, u64 *dst, const CompState &s) {
 #pragma unroll
-	for (int c = 0; c < kMaxComponents; c++) {
-		if (c < d.ncomp) {
+	for (int c = 0; c < S::n; c++) {
+		if (CHas<S>(d, c)) {
 			dst[2 * c] = s.a[c];
 			dst[2 * c + 1] = s.b[c];
 		}
 	}
 }
 
+template <typename S = DynSig>
 __device__ __forceinline__ void StateLoad(const CompDesc &d, const u64 *src, CompState &s) {
 #pragma unroll
-	for (int c = 0; c < kMaxComponents; c++) {
-		if (c < d.ncomp) {
+	for (int c = 0; c < S::n; c++) {
+		if (CHas<S>(d, c)) {
 			s.a[c] = src[2 * c];
 			s.b[c] = src[2 * c + 1];
 		} else {

@@ -84,10 +84,11 @@ __device__ __forceinline__ void StoreTyped(void *out, u64 gid, int vtype, long l
 }
 
 //! Register-array element select with a runtime index, unrolled so the state never leaves VGPRs.
+template <typename S = DynSig>
 __device__ __forceinline__ u64 Pick(const u64 (&arr)[kMaxComponents], int index) {
 	u64 v = 0;
 #pragma unroll
-	for (int c = 0; c < kMaxComponents; c++) {
+	for (int c = 0; c < S::n; c++) {
 		if (c == index) {
 			v = arr[c];
 		}
@@ -95,12 +96,13 @@ __device__ __forceinline__ u64 Pick(const u64 (&arr)[kMaxComponents], int index)
 	return v;
 }
 
+template <typename S = DynSig>
 __device__ __forceinline__ void WriteGroup(const AggOutDesc &o, const CompState &s, u64 gid) {
-	const u64 count = Pick(s.a, o.count_comp);
+	const u64 count = Pick<S>(s.a, o.count_comp);
 	for (int i = 0; i < o.naggs; i++) {
 		const int c = o.comp[i];
-		const u64 sa = Pick(s.a, c);
-		const u64 sb = Pick(s.b, c);
+		const u64 sa = Pick<S>(s.a, c);
+		const u64 sb = Pick<S>(s.b, c);
 		void *out = o.out[i];
 		switch (o.kind[i]) {
 		case AGG_COUNT_STAR:
@@ -235,7 +237,7 @@ __device__ __forceinline__ void StageTransposed(D *dst, const T *src, uint32_t c
 
 //! Reduce tile: 256 threads x 8 consecutive rows. Keys and payload columns are staged into LDS with one
 //! coalesced sweep each, then every thread folds its 8 rows serially from LDS.
-template <typename KeyT, int NP>
+template <typename KeyT, int NP, typename SG>
 __global__ void __launch_bounds__(kRedBlock)
     SegReduceKernel(const KeyT *keys, uint64_t n, PayloadPtrs pay, CompDesc d, AggOutDesc o, KeyOutDesc ko,
                     u64 *tile_state, uint32_t *ticket, uint32_t *tile_heads, u64 *lead_state, u64 *tail_state,
@@ -331,8 +333,8 @@ __global__ void __launch_bounds__(kRedBlock)
 
 	// Serial pass over this thread's rows.
 	CompState S, lead;
-	StateIdentity(d, S);
-	StateIdentity(d, lead);
+	StateIdentity<SG>(d, S);
+	StateIdentity<SG>(d, lead);
 	bool seen = false;
 	u64 gid = hb_global + hb_excl;
 	u64 cur_gid = 0;
@@ -347,16 +349,16 @@ __global__ void __launch_bounds__(kRedBlock)
 			}
 			if (head[i]) {
 				if (seen) {
-					WriteGroup(o, S, cur_gid);
+					WriteGroup<SG>(o, S, cur_gid);
 				} else {
 					lead = S;
 					seen = true;
 				}
 				cur_gid = gid++;
 				WriteKeys(ko, (u64)k[i], cur_gid);
-				StateIdentity(d, S);
+				StateIdentity<SG>(d, S);
 			}
-			StateAddRow<(NP > 0 ? NP : 1)>(d, S, vals);
+			StateAddRow<(NP > 0 ? NP : 1), SG>(d, S, vals);
 		}
 	}
 	if (!seen) {
@@ -369,50 +371,50 @@ __global__ void __launch_bounds__(kRedBlock)
 	bool F = seen;
 	for (int off = 1; off < kWave; off <<= 1) {
 		CompState up;
-		StateShflUp(d, up, X, off);
+		StateShflUp<SG>(d, up, X, off);
 		bool fup = __shfl_up((int)F, off) != 0;
 		if (lane >= off) {
 			if (!F) {
-				StateCombine(d, up, X);
+				StateCombine<SG>(d, up, X);
 				X = up;
 			}
 			F = F || fup;
 		}
 	}
 	if (lane == kWave - 1) {
-		StateStore(d, s_wave_state[wave], X);
+		StateStore<SG>(d, s_wave_state[wave], X);
 		s_wave_flag[wave] = F ? 1 : 0;
 	}
 	__syncthreads();
 	CompState P;
-	StateIdentity(d, P);
+	StateIdentity<SG>(d, P);
 	for (int w = 0; w < wave; w++) {
 		CompState t;
-		StateLoad(d, s_wave_state[w], t);
+		StateLoad<SG>(d, s_wave_state[w], t);
 		if (s_wave_flag[w]) {
 			P = t;
 		} else {
-			StateCombine(d, P, t);
+			StateCombine<SG>(d, P, t);
 		}
 	}
 	if (!F) {
 		CompState t = P;
-		StateCombine(d, t, X);
+		StateCombine<SG>(d, t, X);
 		X = t;
 	}
 	CompState E;
-	StateShflUp(d, E, X, 1);
+	StateShflUp<SG>(d, E, X, 1);
 	if (lane == 0) {
 		E = P;
 	}
 
 	if (seen) {
 		CompState V = E;
-		StateCombine(d, V, lead);
+		StateCombine<SG>(d, V, lead);
 		if (hb_excl > 0) {
-			WriteGroup(o, V, hb_global + hb_excl - 1);
+			WriteGroup<SG>(o, V, hb_global + hb_excl - 1);
 		} else {
-			StateStore(d, lead_state + (uint64_t)tile * 2 * d.ncomp, V);
+			StateStore<SG>(d, lead_state + (uint64_t)tile * 2 * d.ncomp, V);
 		}
 	}
 	if (tid == kRedBlock - 1) {
@@ -425,16 +427,16 @@ __global__ void __launch_bounds__(kRedBlock)
 				ends = keys[tile_end] != s_keys[(kRedItems - 1) * kSlotStride + kRedBlock - 1];
 			}
 			if (ends) {
-				WriteGroup(o, X, gid_last);
+				WriteGroup<SG>(o, X, gid_last);
 				tail_gid[tile] = ~0ull;
 			} else {
-				StateStore(d, tail_state + (uint64_t)tile * 2 * d.ncomp, X);
+				StateStore<SG>(d, tail_state + (uint64_t)tile * 2 * d.ncomp, X);
 				tail_gid[tile] = gid_last;
 				uint32_t slot = FetchAddAgent(tail_count, 1u);
 				tail_list[slot] = tile;
 			}
 		} else {
-			StateStore(d, lead_state + (uint64_t)tile * 2 * d.ncomp, X);
+			StateStore<SG>(d, lead_state + (uint64_t)tile * 2 * d.ncomp, X);
 			tail_gid[tile] = ~0ull;
 		}
 		if (tile == ntiles - 1) {
@@ -633,7 +635,7 @@ __global__ void __launch_bounds__(kFixBlock) TailFinishKernel(CompDesc d, AggOut
 	}
 }
 
-template <typename KeyT, int NP>
+template <typename KeyT, int NP, typename SG = DynSig>
 void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, const CompDesc &cd,
                   const AggOutDesc &od, const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch,
                   hipStream_t stream) {
@@ -654,7 +656,7 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 	}
 	{
 		ProfileScope prof("seg_reduce", stream, bytes);
-		hipLaunchKernelGGL((SegReduceKernel<KeyT, NP>), dim3((unsigned)ntiles), dim3(kRedBlock), 0, stream,
+		hipLaunchKernelGGL((SegReduceKernel<KeyT, NP, SG>), dim3((unsigned)ntiles), dim3(kRedBlock), 0, stream,
 		                   (const KeyT *)run.keys, n, pay, cd, od, ko, lb, ticket, tile_heads, lead_state,
 		                   tail_state, tail_gid, tail_list, tail_count, groups_device, ntiles);
 		VGPU_HIP(hipGetLastError());
@@ -673,9 +675,49 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 	}
 }
 
+//! Component signatures compiled ahead of time (COUNT plus the aggregates of the common queries); any other
+//! mix runs the generic kernel.
+using SigCountSumMinMaxDd = StaticSig<SigCode(COMP_COUNT), SigCode(COMP_SUM_I128, VT_INT64, 0),
+                                      SigCode(COMP_MIN_I64, VT_INT64, 0), SigCode(COMP_MAX_I64, VT_INT64, 0),
+                                      SigCode(COMP_SUM_DD, VT_FLOAT64, 1)>;
+using SigCountSumI32 = StaticSig<SigCode(COMP_COUNT), SigCode(COMP_SUM_I128, VT_INT32, 0)>;
+using SigCountSumI64 = StaticSig<SigCode(COMP_COUNT), SigCode(COMP_SUM_I128, VT_INT64, 0)>;
+using SigCountSumDd = StaticSig<SigCode(COMP_COUNT), SigCode(COMP_SUM_DD, VT_FLOAT64, 0)>;
+
+template <typename SG>
+bool SigMatches(const CompDesc &cd) {
+	if (cd.ncomp != SG::n) {
+		return false;
+	}
+	for (int c = 0; c < cd.ncomp; c++) {
+		const int code = cd.kind[c] == COMP_COUNT ? SigCode(COMP_COUNT)
+		                                          : SigCode(cd.kind[c], cd.in_type[c], cd.payload[c]);
+		if (code != SG::code(c)) {
+			return false;
+		}
+	}
+	return true;
+}
+
 template <typename KeyT>
 void DispatchNP(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, const CompDesc &cd, const AggOutDesc &od,
                 const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch, hipStream_t stream) {
+	if (pay.n == 2 && SigMatches<SigCountSumMinMaxDd>(cd)) {
+		LaunchReduce<KeyT, 2, SigCountSumMinMaxDd>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		return;
+	}
+	if (pay.n == 1 && SigMatches<SigCountSumI32>(cd)) {
+		LaunchReduce<KeyT, 1, SigCountSumI32>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		return;
+	}
+	if (pay.n == 1 && SigMatches<SigCountSumI64>(cd)) {
+		LaunchReduce<KeyT, 1, SigCountSumI64>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		return;
+	}
+	if (pay.n == 1 && SigMatches<SigCountSumDd>(cd)) {
+		LaunchReduce<KeyT, 1, SigCountSumDd>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		return;
+	}
 	if (pay.n == 0) {
 		LaunchReduce<KeyT, 0>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
 	} else if (pay.n == 1) {
