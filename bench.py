@@ -59,6 +59,27 @@ def calibrate_copy():
                       "launches": 5}), flush=True)
 
 
+def measure_copy_gbs(torch, dev, gib=4, reps=10):
+    """Achievable stream-copy bandwidth on this GPU: a device-to-device copy of `gib` GiB (read + write
+    bytes / time, HIP events), the practical ceiling next to the 8 TB/s peak."""
+    n = gib * (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.int64, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * gib * (1 << 30) / (ms * 1e-3) / 1e9
+
+
 def committed_traffic(kernel):
     """HBM traffic per launch of `kernel` measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by
     the copy-kernel calibration) and committed under profiles/; None when no measurement is committed."""
@@ -176,7 +197,10 @@ def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
     ms = (t1 - t0) * 1e3 / steps
     kernels = {e["name"]: round(e["ms"] / max(e["launches"], 1) * (e["launches"] / steps), 4) for e in prof}
     alg = sum(e["bytes"] for e in prof) / steps
+    compulsory = 12 * nb + 8 * nprobe  # read build key + payload and probe key once
     return {"operator": "sort-merge join + aggregate, 2^24 build x 2^28 probe", "probe_rows": nprobe, "build_rows": nb,
+            "compulsory_bytes": compulsory,
+            "compulsory_frac": round(compulsory / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "sum_payload": int(res[0]), "count": int(res[1]), "ms": round(ms, 3),
             "rows_per_s": round((nprobe + nb) / (ms * 1e-3), 1), "algorithmic_bytes": alg,
             "gbs": round(alg / (ms * 1e-3) / 1e9, 1), "kernel_ms_per_step": kernels}
@@ -212,24 +236,35 @@ def bench_cross(torch, capi, dev, steps):
 
 def make_batched_table(torch, capi, dev, nq=512, seed=7):
     """512 row groups of one table, sizes log-uniform in [2K, 4M] rows; a INT32 uniform [0,1e6), c DOUBLE
-    uniform [0,1), key INT64 Zipf(1.1) over 1M values reduced modulo the query's group cardinality
-    (log-uniform in [16, 1M]), v1 INT64 uniform [-1e6,1e6], v2 DOUBLE normal(0,1)."""
+    uniform [0,1), v1 INT64 uniform [-1e6,1e6], v2 DOUBLE normal(0,1) as in the plain configs; the group
+    count of query g is min(rows, log-uniform in [16, 1M]) and its keys are Zipf(1.1) ranks over that count
+    mapped through a seeded permutation of [0, count) (inverse CDF on the device)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(2048), np.log(4 << 20), nq)).astype(np.int64)
-    cards = np.exp(rng.uniform(np.log(16), np.log(1 << 20), nq)).astype(np.int64)
+    cards = np.minimum(sizes, np.exp(rng.uniform(np.log(16), np.log(1_000_000), nq)).astype(np.int64))
     rows = int(sizes.sum())
     a = torch.empty(rows, dtype=torch.int32, device=dev)
     c = torch.empty(rows, dtype=torch.float64, device=dev)
     capi.generate_column(a, capi.INT32, 0, lo=0, hi=999_999, seed=seed, stream_id=1)
     capi.generate_column(c, capi.FLOAT64, 1, seed=seed, stream_id=2)
-    key, v1, v2 = make_groupby_table(torch, capi, dev, rows, seed=seed)
-    card_rows = torch.repeat_interleave(torch.from_numpy(cards).to(dev), torch.from_numpy(sizes).to(dev))
-    key.remainder_(card_rows)
-    del card_rows
-    offsets = torch.from_numpy(np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)).to(dev)
+    _, v1, v2 = make_groupby_table(torch, capi, dev, rows, seed=seed)
+    key = torch.empty(rows, dtype=torch.int64, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    for q in range(nq):
+        g = int(cards[q])
+        w = torch.arange(1, g + 1, dtype=torch.float64, device=dev).pow_(-1.1)
+        cdf = torch.cumsum(w, 0)
+        cdf /= cdf[-1].clone()
+        u = torch.rand(int(sizes[q]), dtype=torch.float64, device=dev, generator=gen)
+        rank = torch.searchsorted(cdf, u).clamp_(max=g - 1)
+        perm = torch.randperm(g, device=dev, generator=gen)
+        key[offsets[q]:offsets[q + 1]] = perm[rank]
+    offsets_t = torch.from_numpy(offsets).to(dev)
     torch.cuda.synchronize()
-    return [a, c, key, v1, v2], offsets, sizes, cards
+    return [a, c, key, v1, v2], offsets_t, sizes, cards
 
 
 def bench_batched(torch, capi, dev, steps, nq=512):
@@ -267,7 +302,7 @@ def bench_batched(torch, capi, dev, steps, nq=512):
     torch.cuda.synchronize()
     tp = (time.perf_counter() - t0) / psteps
     return {"operator": f"batched filter + GROUP BY, {nq} row groups (log-uniform 2K..4M rows, "
-                        f"16..1M groups), {rows} rows",
+                        f"16..1M groups, Zipf(1.1) keys per group), {rows} rows",
             "rows": rows, "queries": nq, "batched_ms": round(tb * 1e3, 3), "per_query_ms": round(tp * 1e3, 3),
             "batched_qps": round(nq / tb, 1), "per_query_qps": round(nq / tp, 1),
             "qps_ratio": round(tp / tb, 2), "rows_per_s": round(rows / tb, 1),
@@ -311,19 +346,27 @@ def make_served_tables(torch, capi, dev, scale=1.0, seed=31):
         fk[lo:lo + m].copy_(outs[0][:m])
         del outs
     del hit, miss, sel
+    x = torch.empty(4096, dtype=torch.int32, device=dev)
+    y = torch.empty(65536, dtype=torch.int32, device=dev)
+    capi.generate_column(x, capi.INT32, 0, lo=0, hi=(1 << 30) - 1, seed=seed, stream_id=8)
+    capi.generate_column(y, capi.INT32, 0, lo=0, hi=(1 << 30) - 1, seed=seed, stream_id=9)
     torch.cuda.synchronize()
-    tables = {"filter": [a, b, c], "group": [key, v1, v2], "probe": fk, "build": build, "payload": payload}
-    total = sum(t.numel() * t.element_size() for t in [a, b, c, key, v1, v2, fk, build, payload])
+    tables = {"filter": [a, b, c], "group": [key, v1, v2], "probe": fk, "build": build, "payload": payload,
+              "left": [x], "right": [y]}
+    total = sum(t.numel() * t.element_size() for t in [a, b, c, key, v1, v2, fk, build, payload, x, y])
     return tables, total
 
 
-def served_queries(n, rows, seed, lo=10_000, hi=50_000_000):
-    """n queries drawn uniformly from the three templates, sizes log-uniform in [lo, hi] rows, each over a
-    uniformly placed row range of its template's table."""
+def served_queries(n, rows, seed, lo=10_000, hi=50_000_000, left_rows=4096):
+    """n queries drawn uniformly from the four templates (filter/project, GROUP BY, join + aggregate, cross
+    product + filter), input sizes log-uniform in [lo, hi] rows, each over a uniformly placed row range of its
+    template's table. A cross query's input size counts pairs: it takes size / 4,096 rows of the right side
+    against the whole 4,096-row left side."""
     import numpy as np
     rng = np.random.default_rng(seed)
-    kinds = rng.integers(0, 3, n)
+    kinds = rng.integers(0, 4, n)
     sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n)).astype(np.uint64)
+    sizes = np.where(kinds == 3, np.maximum(sizes // left_rows, 1), sizes).astype(np.uint64)
     starts = np.array([rng.integers(0, rows[k] - int(s) + 1) for k, s in zip(kinds, sizes)], dtype=np.uint64)
     return kinds, starts, sizes
 
@@ -342,7 +385,12 @@ def bench_served(torch, capi, dev, nq=400, scale=1.0):
     sys.path.insert(0, str(ROOT / "tests"))
     from exprs import cfg0_exprs
     tables, total_bytes = make_served_tables(torch, capi, dev, scale)
-    rows = [tables["filter"][0].numel(), tables["group"][0].numel(), tables["probe"].numel()]
+    rows = [tables["filter"][0].numel(), tables["group"][0].numel(), tables["probe"].numel(),
+            tables["right"][0].numel()]
+    cross_pred = ("((bool*)out)[idx] = (((const int*)inputs[0])[row] + ((const int*)inputs[1])[row]) < "
+                  "1073741824;", [0, 1], capi.BOOLEAN)
+    cross_projs = [("((int*)out)[idx] = ((const int*)inputs[0])[row];", [0], capi.INT32),
+                   ("((int*)out)[idx] = ((const int*)inputs[0])[row];", [1], capi.INT32)]
     kinds, starts, sizes = served_queries(nq, rows, seed=3)
     preds, projs = cfg0_exprs()
     aggs = [(capi.COUNT_STAR, -1, capi.INT64), (capi.SUM, 0, capi.HUGEINT), (capi.MIN, 0, capi.INT64),
@@ -353,7 +401,8 @@ def bench_served(torch, capi, dev, nq=400, scale=1.0):
                         max_batch_rows=int(os.environ.get("VGPU_SERVED_BATCH_ROWS", 256 << 20)))
         t = [s.add_filter_project(tables["filter"], preds, projs),
              s.add_group_by(tables["group"], [], 0, [1, 2], aggs),
-             s.add_join_agg(tables["probe"], tables["build"], tables["payload"])]
+             s.add_join_agg(tables["probe"], tables["build"], tables["payload"]),
+             s.add_cross(tables["left"], tables["right"], [cross_pred], cross_projs)]
         return s, np.array([t[k] for k in kinds], dtype=np.int32)
 
     def arrivals(rate, seed):
@@ -362,7 +411,7 @@ def bench_served(torch, capi, dev, nq=400, scale=1.0):
     def point(srv, tmpl, rate, seed):
         lat, bs, el = srv.run_open_loop(tmpl, starts, sizes, arrivals(rate, seed))
         per = {}
-        for k, name in enumerate(("filter", "group_by", "join")):
+        for k, name in enumerate(("filter", "group_by", "join", "cross")):
             m = kinds == k
             per[name] = [round(float(np.percentile(lat[m], 50)) * 1e3, 3),
                          round(float(np.percentile(lat[m], 99)) * 1e3, 3)]
@@ -371,8 +420,8 @@ def bench_served(torch, capi, dev, nq=400, scale=1.0):
                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                 "mean_batch": round(float(bs.mean()), 2), "p50_p99_ms_by_template": per}
 
-    out = {"operator": f"served form, {nq} queries (filter/project, GROUP BY, join+agg templates, "
-                       f"10K..50M rows each) over {total_bytes / 1e9:.1f} GB resident",
+    out = {"operator": f"served form, {nq} queries (filter/project, GROUP BY, join + aggregate, cross product "
+                       f"templates, 10K..50M rows each) over {total_bytes / 1e9:.1f} GB resident",
            "queries": nq, "resident_bytes": total_bytes}
     plain, ptm = server(False)
     zero = np.zeros(nq)
@@ -444,6 +493,8 @@ def main():
     capi.check(capi.load().vgpu_device_init(0))
     dev = torch.device("cuda:0")
     rows = args.rows
+    copy_gbs = measure_copy_gbs(torch, dev)
+    print(f"[bench] device copy ceiling {copy_gbs:.1f} GB/s", file=sys.stderr, flush=True)
     print(f"[bench] generating {rows} rows on device", file=sys.stderr, flush=True)
     key, v1, v2 = make_groupby_table(torch, capi, dev, rows)
     aggs = [(capi.COUNT_STAR, -1, capi.INT64), (capi.SUM, 0, capi.HUGEINT), (capi.MIN, 0, capi.INT64),
@@ -555,8 +606,13 @@ def main():
         "data": "synthetic (seeded device generator: Zipf(1.1) keys over 1M values, uniform int64, normal f64)",
         "config": {"workload": "sort-based GROUP BY, 100M rows, Zipf(1.1) keys over 1M values, 7 aggregates", "rows": rows,
                    "groups": groups, "step_algorithmic_bytes": step_bytes,
-                   "step_gbs": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1)},
-        "roofline": roof,
+                   "step_gbs": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                   # compulsory traffic: read key, v1, v2 once (24 B/row), write each group once (64 B)
+                   "compulsory_bytes": 24 * rows + 64 * groups,
+                   "compulsory_frac": round((24 * rows + 64 * groups) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                            4)},
+        "roofline": dict(roof, copy_gbs=round(copy_gbs, 1), frac_of_copy=round(roof["achieved"] / copy_gbs, 4))
+        if roof else roof,
         "sort_variants": capi.sort_autotune_report(),
         "cpu_baseline": cpu,
         "operators": operators,

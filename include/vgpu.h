@@ -293,7 +293,8 @@ typedef struct vgpu_server_options {
 enum vgpu_template_kind {
 	VGPU_TEMPLATE_FILTER_PROJECT = 0, /* preds + projs over cols: one output column per projection */
 	VGPU_TEMPLATE_GROUP_BY = 1,       /* preds, GROUP BY cols[key_col], aggs over value_cols: key, then aggs */
-	VGPU_TEMPLATE_JOIN_AGG = 2        /* SUM(build_payload), COUNT(*) of cols[0] = build_key: HUGEINT, INT64 */
+	VGPU_TEMPLATE_JOIN_AGG = 2,       /* SUM(build_payload), COUNT(*) of cols[0] = build_key: HUGEINT, INT64 */
+	VGPU_TEMPLATE_CROSS = 3           /* left_cols x a row range of cols, preds + projs over (left..., right...) */
 };
 
 typedef struct vgpu_template {
@@ -311,6 +312,8 @@ typedef struct vgpu_template {
 	int32_t naggs;
 	vgpu_column build_key;      /* JOIN_AGG: integer key without NULLs, same type as cols[0] */
 	vgpu_column build_payload;  /* JOIN_AGG: integer payload */
+	const vgpu_column *left_cols; /* CROSS: the whole left side; expression slots 0..nleft-1 are left columns */
+	int32_t nleft;
 } vgpu_template;
 
 typedef struct vgpu_served_info {

@@ -16,6 +16,12 @@ GB_AGGS = [(capi.COUNT_STAR, -1, capi.INT64), (capi.SUM, 0, capi.HUGEINT), (capi
            (capi.MAX, 0, capi.INT64), (capi.SUM, 1, capi.FLOAT64), (capi.AVG, 1, capi.FLOAT64)]
 
 
+CROSS_PRED = ("((bool*)out)[idx] = (((const int*)inputs[0])[row] + ((const int*)inputs[1])[row]) < 1073741824;",
+              [0, 1], capi.BOOLEAN)
+CROSS_PROJS = [("((int*)out)[idx] = ((const int*)inputs[0])[row];", [0], capi.INT32),
+               ("((int*)out)[idx] = ((const int*)inputs[0])[row];", [1], capi.INT32)]
+
+
 def hugeints(a):
     return [int(lo) % (1 << 64) + (int(hi) << 64) for lo, hi in a]
 
@@ -35,7 +41,9 @@ def tables(dev):
     bkey = np.concatenate([rng.permutation(nb), rng.integers(0, nb // 10, 20_000)]).astype(np.int64)  # duplicates
     payload = rng.integers(0, 1000, len(bkey)).astype(np.int32)
     fk = np.where(rng.random(n) < 0.9, rng.integers(0, nb, n), rng.integers(nb, 2 * nb, n)).astype(np.int64)
-    host = dict(a=a, b=b, c=c, key=key, v1=v1, v2=v2, bkey=bkey, payload=payload, fk=fk)
+    x = rng.integers(0, 1 << 30, 4096).astype(np.int32)
+    y = rng.integers(0, 1 << 30, 65536).astype(np.int32)
+    host = dict(a=a, b=b, c=c, key=key, v1=v1, v2=v2, bkey=bkey, payload=payload, fk=fk, x=x, y=y)
     devt = {k: torch.from_numpy(v).to(dev) for k, v in host.items()}
     torch.cuda.synchronize()
     return host, devt
@@ -46,20 +54,24 @@ def add_templates(server, d):
     t_fp = server.add_filter_project([d["a"], d["b"], d["c"]], preds, projs)
     t_gb = server.add_group_by([d["key"], d["v1"], d["v2"]], [], 0, [1, 2], GB_AGGS)
     t_j = server.add_join_agg(d["fk"], d["bkey"], d["payload"])
-    return t_fp, t_gb, t_j
+    t_x = server.add_cross([d["x"]], [d["y"]], [CROSS_PRED], CROSS_PROJS)
+    return t_fp, t_gb, t_j, t_x
 
 
-def make_queries(n_rows, nq, seed):
+def make_queries(n_rows, nq, seed, n_right=65536):
     rng = np.random.default_rng(seed)
     q = []
     for i in range(nq):
-        kind = i % 3
+        kind = i % 4
         size = int(np.exp(rng.uniform(np.log(1), np.log(400_000))))
-        if i in (3, 4, 5):
+        if kind == 3:
+            size = size % 2500  # right-side rows of a cross query (x 4,096 left rows)
+        if i in (4, 5, 6, 7):
             size = 0  # empty ranges of every template
-        if i in (6, 7, 8):
+        if i in (8, 9, 10, 11):
             size = 1
-        start = int(rng.integers(0, n_rows - size + 1))
+        rows = n_right if kind == 3 else n_rows
+        start = int(rng.integers(0, rows - size + 1))
         q.append((kind, start, size))
     q.append((1, 0, n_rows))  # a whole-table GROUP BY in the mix
     q.append((0, 100, 4093))  # unaligned start, ragged length
@@ -106,6 +118,11 @@ def check_oracle(r, kind, start, size, h):
         np.testing.assert_array_equal(r.columns[4], ref["max_v1"])
         np.testing.assert_allclose(r.columns[5], ref["sum_v2"], rtol=1e-12, atol=0)
         np.testing.assert_allclose(r.columns[6], ref["avg_v2"], rtol=1e-12, atol=0)
+    elif kind == 3:
+        ox, oy = oracle.cross_filter_pairs(h["x"], h["y"][sl], 1 << 30)
+        assert r.rows == len(ox)
+        np.testing.assert_array_equal(r.columns[0], ox)
+        np.testing.assert_array_equal(r.columns[1], oy)
     else:
         s, cnt = oracle.join_sum_count(h["bkey"], h["payload"], h["fk"][sl])
         assert int(r.columns[1][0]) == cnt
@@ -116,7 +133,7 @@ def check_oracle(r, kind, start, size, h):
 
 def test_served_results_same_alone_and_in_the_mix(tables):
     h, d = tables
-    queries = make_queries(len(h["a"]), 60, seed=1)
+    queries = make_queries(len(h["a"]), 80, seed=1)
     with capi.Server(slots=4, batching=True) as mixed:
         tids = add_templates(mixed, d)
         in_mix = run_burst(mixed, tids, queries)

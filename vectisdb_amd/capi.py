@@ -64,7 +64,8 @@ class Template(C.Structure):
     _fields_ = [("kind", C.c_int32), ("cols", C.POINTER(Column)), ("ncols", C.c_int32),
                 ("preds", C.POINTER(Expr)), ("npreds", C.c_int32), ("projs", C.POINTER(Expr)), ("nprojs", C.c_int32),
                 ("key_col", C.c_int32), ("value_cols", C.POINTER(C.c_int32)), ("nvalues", C.c_int32),
-                ("aggs", C.POINTER(Agg)), ("naggs", C.c_int32), ("build_key", Column), ("build_payload", Column)]
+                ("aggs", C.POINTER(Agg)), ("naggs", C.c_int32), ("build_key", Column), ("build_payload", Column),
+                ("left_cols", C.POINTER(Column)), ("nleft", C.c_int32)]
 
 
 class ServedInfo(C.Structure):
@@ -78,7 +79,7 @@ class ServerStats(C.Structure):
                 ("peak_reserved", C.c_uint64), ("budget", C.c_uint64), ("rejected", C.c_uint64)]
 
 
-TEMPLATE_FILTER_PROJECT, TEMPLATE_GROUP_BY, TEMPLATE_JOIN_AGG = 0, 1, 2
+TEMPLATE_FILTER_PROJECT, TEMPLATE_GROUP_BY, TEMPLATE_JOIN_AGG, TEMPLATE_CROSS = 0, 1, 2, 3
 
 _PROTOS = {
     "vgpu_sort_autotune": (C.c_int, [C.c_int32]),
@@ -559,6 +560,16 @@ class Server:
         t = Template(kind=TEMPLATE_JOIN_AGG, cols=cc, ncols=1, key_col=-1, build_key=column(build_key),
                      build_payload=column(build_payload))
         return self._add(t, (cc,))
+
+    def add_cross(self, left_cols, right_cols, preds, projs):
+        """Queries take a row range of the right side; expression slots number the left columns first."""
+        lc = (Column * len(left_cols))(*[column(c) for c in left_cols])
+        rc = (Column * len(right_cols))(*[column(c) for c in right_cols])
+        pa, k1 = _exprs(preds)
+        qa, k2 = _exprs(projs)
+        t = Template(kind=TEMPLATE_CROSS, cols=rc, ncols=len(right_cols), preds=pa, npreds=len(preds), projs=qa,
+                     nprojs=len(projs), key_col=-1, left_cols=lc, nleft=len(left_cols))
+        return self._add(t, (lc, rc, pa, k1, qa, k2))
 
     def submit(self, template_id, row_start, row_count):
         tk = C.c_uint64(0)

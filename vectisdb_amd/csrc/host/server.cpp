@@ -142,6 +142,30 @@ int Server::AddTemplate(std::unique_ptr<ServedTemplate> t) {
 			}
 		}
 		break;
+	case TMPL_CROSS: {
+		if (t->projs.empty() || t->left_cols.empty()) {
+			throw std::runtime_error("server template: a cross product needs left columns and a projection");
+		}
+		for (auto &c : t->left_cols) {
+			if (c.rows != t->left_cols[0].rows) {
+				throw std::runtime_error("server template: left columns differ in length");
+			}
+		}
+		const int slots = (int)(t->left_cols.size() + t->cols.size());
+		if (slots > kFusedMaxCols || (int)t->projs.size() > kFusedMaxOuts) {
+			throw std::runtime_error("server template: at most 16 cross-product columns and outputs");
+		}
+		for (auto *list : {&t->preds, &t->projs}) {
+			for (auto &e : *list) {
+				for (int sl : e.slots) {
+					if (sl < 0 || sl >= slots) {
+						throw std::runtime_error("server template: expression input out of range");
+					}
+				}
+			}
+		}
+		break;
+	}
 	case TMPL_JOIN_AGG:
 		if (t->cols[0].type != t->build_key.type) {
 			throw std::runtime_error("server template: probe and build keys must have the same type");
@@ -178,6 +202,15 @@ uint64_t Server::Footprint(const ServedTemplate &t, uint64_t rows, uint32_t nque
 		}
 		// compacted rows, sort buffers (packed keys twice, permuted payload twice), group outputs
 		return rows * (3 * (4 + kw + vw) + 32 + (4 + kw + aw)) + slack;
+	}
+	case TMPL_CROSS: {
+		// outputs sized for every pair of the query, plus the look-back state
+		const uint64_t pairs = rows * std::max<uint64_t>(t.left_cols[0].rows, 1);
+		uint64_t w = 0;
+		for (auto &p : t.projs) {
+			w += TypeWidth(p.out_type);
+		}
+		return pairs * w + pairs / 512 + slack;
 	}
 	default: // JOIN_AGG
 		if (options_.batching) {
@@ -264,7 +297,8 @@ void Server::Executor(int slot) {
 				max_rows = std::min(max_rows, options_.max_batch_rows);
 			}
 			uint64_t rows = 0;
-			for (auto it = queue_.begin(); it != queue_.end() && (int)batch.size() < options_.max_batch;) {
+			const int max_batch = t.kind == TMPL_CROSS ? 1 : options_.max_batch;
+			for (auto it = queue_.begin(); it != queue_.end() && (int)batch.size() < max_batch;) {
 				if (it->tmpl != tmpl) {
 					++it;
 					continue;
@@ -291,7 +325,9 @@ void Server::Executor(int slot) {
 			r.reset(new ServedResult());
 		}
 		try {
-			if (options_.batching) {
+			if (templates_[batch[0].tmpl]->kind == TMPL_CROSS) {
+				RunCross(batch[0], stream, *results[0]);
+			} else if (options_.batching) {
 				RunBatch(batch, stream, results);
 			} else {
 				RunAlone(batch[0], stream, *results[0]);
@@ -522,6 +558,70 @@ void Server::RunAlone(const Pending &q, hipStream_t stream, ServedResult &r) {
 	r.data = {r.scalars[0], r.scalars[1]};
 	r.types = {VT_HUGEINT, VT_INT64};
 	r.null = {(uint8_t)(pairs == 0), 0};
+}
+
+void Server::RunCross(const Pending &q, hipStream_t stream, ServedResult &r) {
+	// The fused cross kernel over the left side x rows [start, start + count) of the right side.
+	const auto &t = *templates_[q.tmpl];
+	PoolArena arena(stream);
+	auto hold = std::make_shared<BatchBuffers>();
+	auto &pool = DevicePool::Instance();
+	const uint64_t nl = t.left_cols[0].rows;
+	const uint64_t pairs = nl * q.count;
+	FusedSpec spec;
+	spec.cross = true;
+	spec.fast_math = t.fast_math;
+	std::vector<const void *> ptrs;
+	for (auto &c : t.left_cols) {
+		FusedColumn fc;
+		fc.vtype = c.type;
+		fc.side = 0;
+		spec.cols.push_back(fc);
+		ptrs.push_back(c.data);
+	}
+	for (auto &c : t.cols) {
+		FusedColumn fc;
+		fc.vtype = c.type;
+		fc.side = 1;
+		spec.cols.push_back(fc);
+		ptrs.push_back(static_cast<const uint8_t *>(c.data) + q.start * TypeWidth(c.type));
+	}
+	spec.predicates = t.preds;
+	spec.projections = t.projs;
+	auto kernel = GetFusedKernel(spec);
+	std::vector<void *> outs;
+	for (auto &p : t.projs) {
+		void *o = pool.Alloc(std::max<uint64_t>(pairs, 1) * TypeWidth(p.out_type), stream);
+		hold->blocks.push_back(o);
+		outs.push_back(o);
+	}
+	uint64_t kept = pairs;
+	FusedLaunch la;
+	la.cols = ptrs.data();
+	la.outs = outs.data();
+	la.n = pairs;
+	la.right_rows = q.count;
+	la.capacity = std::max<uint64_t>(pairs, 1);
+	void *state = nullptr;
+	size_t state_bytes = 0;
+	if (!t.preds.empty()) {
+		la.count_device = static_cast<uint64_t *>(arena.Alloc(8));
+		state_bytes = FusedKernel::StateBytes(pairs);
+		state = arena.Alloc(std::max<size_t>(state_bytes, 8));
+	}
+	kernel->Launch(la, state, state_bytes, stream);
+	if (!t.preds.empty() && pairs > 0) {
+		VGPU_HIP(hipMemcpyAsync(&kept, la.count_device, 8, hipMemcpyDeviceToHost, stream));
+	} else if (!t.preds.empty()) {
+		kept = 0;
+	}
+	VGPU_HIP(hipStreamSynchronize(stream));
+	r.rows = kept;
+	for (size_t i = 0; i < outs.size(); i++) {
+		r.data.push_back(outs[i]);
+		r.types.push_back(t.projs[i].out_type);
+	}
+	r.hold = hold;
 }
 
 double RunOpenLoop(Server &server, const int32_t *tmpl, const uint64_t *starts, const uint64_t *counts,

@@ -8,7 +8,8 @@
 // that still fits (up to max_batch): the batch runs as ONE pipeline (row-range fused kernel, composite-key
 // GROUP BY, or one probe pass against the template's resident join index). Queries that arrive while
 // others run are admitted as executors and memory free up, and batches of different executors overlap on
-// the device. With batching off each query runs alone on the plain per-query operators.
+// the device. With batching off each query runs alone on the plain per-query operators. A cross-product
+// query (the whole left side times a row range of the right side) always runs as its own launch.
 #pragma once
 
 #include "operators.h"
@@ -28,14 +29,15 @@
 
 namespace vgpu {
 
-enum TemplateKind : int { TMPL_FILTER_PROJECT = 0, TMPL_GROUP_BY = 1, TMPL_JOIN_AGG = 2 };
+enum TemplateKind : int { TMPL_FILTER_PROJECT = 0, TMPL_GROUP_BY = 1, TMPL_JOIN_AGG = 2, TMPL_CROSS = 3 };
 
 struct ServedTemplate {
 	int kind = TMPL_FILTER_PROJECT;
-	std::vector<DevCol> cols;        // the table; JOIN_AGG: cols[0] is the probe key
+	std::vector<DevCol> cols;        // the table; JOIN_AGG: cols[0] is the probe key; CROSS: the right side
+	std::vector<DevCol> left_cols;   // CROSS: the whole left side (expression slots 0..nleft-1)
 	uint64_t rows = 0;
 	std::vector<FusedExpr> preds;    // FILTER_PROJECT, GROUP_BY
-	std::vector<FusedExpr> projs;    // FILTER_PROJECT
+	std::vector<FusedExpr> projs;    // FILTER_PROJECT, CROSS
 	int key_col = -1;                // GROUP_BY
 	std::vector<int> value_cols;     // GROUP_BY (aggregate inputs index this list)
 	std::vector<AggReq> aggs;        // GROUP_BY
@@ -112,6 +114,7 @@ private:
 	void RunBatch(const std::vector<Pending> &batch, hipStream_t stream,
 	              std::vector<std::unique_ptr<ServedResult>> &results);
 	void RunAlone(const Pending &q, hipStream_t stream, ServedResult &r);
+	void RunCross(const Pending &q, hipStream_t stream, ServedResult &r);
 
 	ServerOptions options_;
 	int device_ = 0;

@@ -81,6 +81,12 @@ int vgpu_server_add_template(vgpu_server *server, const vgpu_template *tmpl, int
 		for (int32_t i = 0; i < tmpl->naggs; i++) {
 			t->aggs.push_back({tmpl->aggs[i].kind, tmpl->aggs[i].input, tmpl->aggs[i].output_type});
 		}
+		if (tmpl->kind == VGPU_TEMPLATE_CROSS) {
+			if (tmpl->nleft <= 0 || tmpl->left_cols == nullptr) {
+				throw std::runtime_error("server template: a cross product needs left columns");
+			}
+			t->left_cols = ToDevCols(tmpl->left_cols, tmpl->nleft);
+		}
 		if (tmpl->kind == VGPU_TEMPLATE_JOIN_AGG) {
 			t->build_key = ToDevCols(&tmpl->build_key, 1)[0];
 			t->build_payload = ToDevCols(&tmpl->build_payload, 1)[0];
