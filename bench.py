@@ -9,8 +9,8 @@ with every column resident in HBM before the timed region. One step = one full G
 (key-range pass, histogram, radix passes, segmented reduction, group-count readback).
 
 Prints ONE JSON line (rank 0). `roofline` is the dominant kernel's algorithmic bytes / its average launch
-time (HIP events on its own stream) against 8 TB/s; `cpu_baseline` is the single-threaded C restatement
-(oracle/) timed on a bounded sample of the same data on this host. `operators` carries the per-operator
+time (HIP events on its own stream) against 8 TB/s; `cpu_baseline` is the multithreaded C restatement
+(oracle/, all host threads) timed on the same data on this host. `operators` carries the per-operator
 lines (fused filter/project, join, cross product) measured the same way.
 """
 from __future__ import annotations
@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-rows", type=int, default=20_000_000, help="rows in the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=100_000_000, help="rows in the CPU baseline sample")
     ap.add_argument("--only", default="", help="comma list of extra operator lines (none by default)")
     ap.add_argument("--sweep", action="store_true", help="also time each sort tuning candidate (stderr)")
     ap.add_argument("--no-autotune", action="store_true", help="use the static radix-pass variant choice")
@@ -579,17 +579,20 @@ def main():
     cpu = None
     if not args.no_cpu:
         import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
         n = min(args.cpu_rows, rows)
         k = key[:n].cpu().numpy()
         a = v1[:n].cpu().numpy()
         b = v2[:n].cpu().numpy()
-        oracle.group_by_i64(k[:1000], a[:1000], b[:1000])
-        c0 = time.perf_counter()
-        oracle.group_by_i64(k, a, b)
-        c1 = time.perf_counter()
-        cpu = {"value": round(n / (c1 - c0), 1), "unit": "rows/s", "cores": 1, "kind": "port",
-               "sample": f"first {n} rows of the same device-generated table, oracle/oracle.c vo_group_by_i64 "
-                         f"(counting sort + exact per-group sums), single thread; DuckDB is not installed"}
+        oracle.time_group_by_i64(k[:100_000], a[:100_000], b[:100_000], threads)
+        sec, cpu_groups = oracle.time_group_by_i64(k, a, b, threads)
+        cpu = {"value": round(n / sec, 1), "unit": "rows/s", "cores": threads, "kind": "port",
+               "sample": f"{'all' if n == rows else 'first'} {n} rows of the same device-generated table; CPU "
+                         f"restatement (not DuckDB, which is not installed): oracle/oracle.c vo_group_by_i64_mt, "
+                         f"stable counting sort + exact per-group sums on {threads} threads, {cpu_groups} groups, "
+                         f"{sec:.2f} s"}
+        print(f"[bench] cpu baseline {cpu['value'] / 1e6:.1f} M rows/s on {threads} threads", file=sys.stderr,
+              flush=True)
 
     line = {
         "metric": METRIC,

@@ -31,7 +31,8 @@ LIB = HERE / "lib" / "liboracle.so"
 
 def build() -> Path:
     LIB.parent.mkdir(exist_ok=True)
-    cmd = ["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-shared", "-fPIC", "-o", str(LIB), str(SRC), "-lm"]
+    cmd = ["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-shared", "-fPIC", "-pthread", "-o", str(LIB),
+           str(SRC), "-lm"]
     subprocess.run(cmd, check=True)
     return LIB
 
@@ -58,6 +59,8 @@ def lib():
         L.vo_exact_sum.argtypes = [P, u64]
         L.vo_group_by_i64.restype = u64
         L.vo_group_by_i64.argtypes = [P, P, P, u64, P, P, P, P, P, P, P]
+        L.vo_group_by_i64_mt.restype = u64
+        L.vo_group_by_i64_mt.argtypes = [P, P, P, u64, C.c_int, P, P, P, P, P, P, P]
         L.vo_join_sum_count.restype = u64
         L.vo_join_sum_count.argtypes = [P, P, u64, P, u64, C.POINTER(u64), C.POINTER(C.c_int64)]
         L.vo_join_pairs.restype = u64
@@ -111,8 +114,9 @@ def exact_sum(x):
     return lib().vo_exact_sum(_p(x), len(x))
 
 
-def group_by_i64(key, v1=None, v2=None):
-    """Returns dict of per-group arrays in ascending key order (sum_v1 as Python ints)."""
+def group_by_i64(key, v1=None, v2=None, threads=1):
+    """Returns dict of per-group arrays in ascending key order (sum_v1 as Python ints). threads > 1 runs the
+    multithreaded restatement (same results)."""
     key = np.ascontiguousarray(key, np.int64)
     n = len(key)
     v1 = None if v1 is None else np.ascontiguousarray(v1, np.int64)
@@ -122,12 +126,37 @@ def group_by_i64(key, v1=None, v2=None):
     out["sum_v1"] = np.empty(2 * cap, np.int64)
     out["sum_v2"] = np.empty(cap, np.float64)
     out["avg_v2"] = np.empty(cap, np.float64)
-    g = lib().vo_group_by_i64(_p(key), _p(v1), _p(v2), n, _p(out["key"]), _p(out["count"]), _p(out["sum_v1"]),
-                              _p(out["min_v1"]), _p(out["max_v1"]), _p(out["sum_v2"]), _p(out["avg_v2"]))
+    if threads > 1:
+        g = lib().vo_group_by_i64_mt(_p(key), _p(v1), _p(v2), n, threads, _p(out["key"]), _p(out["count"]),
+                                     _p(out["sum_v1"]), _p(out["min_v1"]), _p(out["max_v1"]), _p(out["sum_v2"]),
+                                     _p(out["avg_v2"]))
+    else:
+        g = lib().vo_group_by_i64(_p(key), _p(v1), _p(v2), n, _p(out["key"]), _p(out["count"]), _p(out["sum_v1"]),
+                                  _p(out["min_v1"]), _p(out["max_v1"]), _p(out["sum_v2"]), _p(out["avg_v2"]))
     res = {k: v[:g] for k, v in out.items() if k != "sum_v1"}
     pairs = out["sum_v1"][: 2 * g].reshape(-1, 2)
     res["sum_v1"] = [int(lo) % (1 << 64) + (int(hi) << 64) for lo, hi in pairs]
     return res
+
+
+def time_group_by_i64(key, v1, v2, threads=1):
+    """Seconds taken by the C GROUP BY alone (outputs allocated beforehand, no Python conversion): the CPU
+    baseline timing. Returns (seconds, groups)."""
+    import time
+    key = np.ascontiguousarray(key, np.int64)
+    v1 = np.ascontiguousarray(v1, np.int64)
+    v2 = np.ascontiguousarray(v2, np.float64)
+    n = len(key)
+    cap = max(n, 1)
+    out = {k: np.empty(cap, np.int64) for k in ("key", "count", "min_v1", "max_v1")}
+    out["sum_v1"] = np.empty(2 * cap, np.int64)
+    out["sum_v2"] = np.empty(cap, np.float64)
+    out["avg_v2"] = np.empty(cap, np.float64)
+    t0 = time.perf_counter()
+    g = lib().vo_group_by_i64_mt(_p(key), _p(v1), _p(v2), n, threads, _p(out["key"]), _p(out["count"]),
+                                 _p(out["sum_v1"]), _p(out["min_v1"]), _p(out["max_v1"]), _p(out["sum_v2"]),
+                                 _p(out["avg_v2"]))
+    return time.perf_counter() - t0, g
 
 
 def join_sum_count(build_key, payload, fk):

@@ -1,12 +1,13 @@
 /* oracle.c -- CPU restatement of the engine's hot path. TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
  * Built with gcc -O2 -ffp-contract=off so every double operation is one IEEE rounding, as DuckDB's CPU
- * path computes it. Single-threaded on purpose: when bench.py times it as the CPU baseline it reports
- * cores = 1.
+ * path computes it. Every function is single-threaded except vo_group_by_i64_mt, the multithreaded GROUP
+ * BY the benchmark times as its CPU baseline (same per-group arithmetic, so the same results).
  */
 #include "oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -274,6 +275,337 @@ uint64_t vo_group_by_i64(const int64_t *key, const int64_t *v1, const double *v2
 	free(m.p);
 	free(order);
 	return g;
+}
+
+/* ------------------------------------------------------------------ multithreaded GROUP BY
+ * Stable counting sort by key with one histogram per thread (thread t owns rows [t*n/T, (t+1)*n/T), so
+ * bucket b's rows land thread after thread, in row order), then every group reduced exactly as in
+ * vo_group_by_i64 by the thread that owns its bucket block. Falls back to vo_group_by_i64 when the key
+ * range is wider than 2^24. */
+typedef struct {
+	const int64_t *key, *v1;
+	const double *v2;
+	uint64_t n, range;
+	int64_t mn;
+	int threads;
+	uint32_t *hist;  /* threads x range */
+	uint64_t *order;
+	uint64_t *gfirst; /* range + 1: group index of each bucket (prefix of non-empty buckets) */
+	uint64_t *bstart; /* range + 1: first position of each bucket in order */
+	int64_t *out_key, *out_count, *out_sum_v1, *out_min_v1, *out_max_v1;
+	double *out_sum_v2, *out_avg_v2;
+	int phase;
+	/* phase 3 work list: a task is a run of whole buckets [b0, b1), or rows [r0, r1) of one large bucket
+	 * whose partial state goes to slot `slot` (merged afterwards) */
+	uint64_t ntasks;
+	uint64_t *task_b0, *task_b1, *task_r0, *task_r1;
+	int64_t *task_slot;
+	uint64_t next_task; /* atomic */
+	i128 *part_s1;
+	int64_t *part_mn, *part_mx;
+	msum_t *part_m;
+} mt_ctx;
+
+typedef struct {
+	mt_ctx *c;
+	int t;
+	int64_t mn, mx;
+} mt_arg;
+
+static void *mt_worker(void *p) {
+	mt_arg *a = (mt_arg *)p;
+	mt_ctx *c = a->c;
+	const uint64_t lo = c->n * (uint64_t)a->t / (uint64_t)c->threads;
+	const uint64_t hi = c->n * (uint64_t)(a->t + 1) / (uint64_t)c->threads;
+	if (c->phase == 0) { /* min / max of the chunk */
+		int64_t mn = INT64_MAX, mx = INT64_MIN;
+		for (uint64_t i = lo; i < hi; i++) {
+			mn = c->key[i] < mn ? c->key[i] : mn;
+			mx = c->key[i] > mx ? c->key[i] : mx;
+		}
+		a->mn = mn;
+		a->mx = mx;
+	} else if (c->phase == 1) { /* histogram */
+		uint32_t *h = c->hist + (uint64_t)a->t * c->range;
+		for (uint64_t i = lo; i < hi; i++) {
+			h[(uint64_t)c->key[i] - (uint64_t)c->mn]++;
+		}
+	} else if (c->phase == 2) { /* scatter: hist now holds this thread's first position per bucket */
+		uint32_t *h = c->hist + (uint64_t)a->t * c->range;
+		for (uint64_t i = lo; i < hi; i++) {
+			c->order[h[(uint64_t)c->key[i] - (uint64_t)c->mn]++] = i;
+		}
+	} else { /* reduce: take tasks until none is left */
+		msum_t m = {0};
+		for (;;) {
+			const uint64_t k = __atomic_fetch_add(&c->next_task, 1, __ATOMIC_RELAXED);
+			if (k >= c->ntasks) {
+				break;
+			}
+			if (c->task_slot[k] >= 0) { /* part of a large group */
+				const int64_t sl = c->task_slot[k];
+				i128 s1 = 0;
+				int64_t mn = INT64_MAX, mx = INT64_MIN;
+				msum_t *pm = &c->part_m[sl];
+				msum_init(pm);
+				for (uint64_t j = c->task_r0[k]; j < c->task_r1[k]; j++) {
+					const uint64_t r = c->order[j];
+					if (c->v1) {
+						s1 += (i128)c->v1[r];
+						mn = c->v1[r] < mn ? c->v1[r] : mn;
+						mx = c->v1[r] > mx ? c->v1[r] : mx;
+					}
+					if (c->v2) {
+						msum_add(pm, c->v2[r]);
+					}
+				}
+				c->part_s1[sl] = s1;
+				c->part_mn[sl] = mn;
+				c->part_mx[sl] = mx;
+				continue;
+			}
+		for (uint64_t b = c->task_b0[k]; b < c->task_b1[k]; b++) {
+			const uint64_t i0 = c->bstart[b], i1 = c->bstart[b + 1];
+			if (i0 == i1) {
+				continue;
+			}
+			const uint64_t g = c->gfirst[b];
+			i128 s1 = 0;
+			int64_t mn = INT64_MAX, mx = INT64_MIN;
+			msum_init(&m);
+			for (uint64_t j = i0; j < i1; j++) {
+				const uint64_t r = c->order[j];
+				if (c->v1) {
+					s1 += (i128)c->v1[r];
+					mn = c->v1[r] < mn ? c->v1[r] : mn;
+					mx = c->v1[r] > mx ? c->v1[r] : mx;
+				}
+				if (c->v2) {
+					msum_add(&m, c->v2[r]);
+				}
+			}
+			const uint64_t cnt = i1 - i0;
+			if (c->out_key) {
+				c->out_key[g] = (int64_t)((uint64_t)c->mn + b);
+			}
+			if (c->out_count) {
+				c->out_count[g] = (int64_t)cnt;
+			}
+			if (c->out_sum_v1) {
+				c->out_sum_v1[2 * g] = (int64_t)(uint64_t)(u128)s1;
+				c->out_sum_v1[2 * g + 1] = (int64_t)(s1 >> 64);
+			}
+			if (c->out_min_v1) {
+				c->out_min_v1[g] = mn;
+			}
+			if (c->out_max_v1) {
+				c->out_max_v1[g] = mx;
+			}
+			const double s2 = c->v2 ? msum_value(&m) : 0.0;
+			if (c->out_sum_v2) {
+				c->out_sum_v2[g] = s2;
+			}
+			if (c->out_avg_v2) {
+				c->out_avg_v2[g] = s2 / (double)cnt;
+			}
+		}
+		}
+		free(m.p);
+	}
+	return NULL;
+}
+
+static uint64_t mn_key_base(int64_t mn) {
+	return (uint64_t)mn;
+}
+
+static void mt_run(mt_ctx *c, mt_arg *args, pthread_t *tid, int phase) {
+	c->phase = phase;
+	for (int t = 0; t < c->threads; t++) {
+		pthread_create(&tid[t], NULL, mt_worker, &args[t]);
+	}
+	for (int t = 0; t < c->threads; t++) {
+		pthread_join(tid[t], NULL);
+	}
+}
+
+uint64_t vo_group_by_i64_mt(const int64_t *key, const int64_t *v1, const double *v2, uint64_t n, int threads,
+                            int64_t *out_key, int64_t *out_count, int64_t *out_sum_v1, int64_t *out_min_v1,
+                            int64_t *out_max_v1, double *out_sum_v2, double *out_avg_v2) {
+	if (threads < 1) {
+		threads = 1;
+	}
+	if (threads > 256) {
+		threads = 256;
+	}
+	if (n == 0 || threads == 1 || n >= (1ull << 32)) {
+		return vo_group_by_i64(key, v1, v2, n, out_key, out_count, out_sum_v1, out_min_v1, out_max_v1, out_sum_v2,
+		                       out_avg_v2);
+	}
+	mt_ctx c;
+	memset(&c, 0, sizeof(c));
+	c.key = key;
+	c.v1 = v1;
+	c.v2 = v2;
+	c.n = n;
+	c.threads = threads;
+	mt_arg args[256];
+	pthread_t tid[256];
+	for (int t = 0; t < threads; t++) {
+		args[t].c = &c;
+		args[t].t = t;
+	}
+	mt_run(&c, args, tid, 0);
+	int64_t mn = INT64_MAX, mx = INT64_MIN;
+	for (int t = 0; t < threads; t++) {
+		mn = args[t].mn < mn ? args[t].mn : mn;
+		mx = args[t].mx > mx ? args[t].mx : mx;
+	}
+	const uint64_t range = (uint64_t)mx - (uint64_t)mn + 1;
+	if (range > (1ull << 24)) {
+		return vo_group_by_i64(key, v1, v2, n, out_key, out_count, out_sum_v1, out_min_v1, out_max_v1, out_sum_v2,
+		                       out_avg_v2);
+	}
+	c.mn = mn;
+	c.range = range;
+	c.hist = (uint32_t *)calloc((size_t)threads * range, sizeof(uint32_t));
+	c.order = (uint64_t *)malloc(n * sizeof(uint64_t));
+	c.gfirst = (uint64_t *)malloc((range + 1) * sizeof(uint64_t));
+	c.bstart = (uint64_t *)malloc((range + 1) * sizeof(uint64_t));
+	mt_run(&c, args, tid, 1);
+	/* bucket-major, thread-minor exclusive prefix: each thread's first slot per bucket */
+	uint64_t pos = 0, groups = 0;
+	for (uint64_t b = 0; b < range; b++) {
+		c.bstart[b] = pos;
+		c.gfirst[b] = groups;
+		const uint64_t before = pos;
+		for (int t = 0; t < threads; t++) {
+			uint32_t *h = c.hist + (uint64_t)t * range + b;
+			const uint32_t cnt = *h;
+			*h = (uint32_t)pos; /* positions stay below n < 2^32 */
+			pos += cnt;
+		}
+		groups += pos > before ? 1 : 0;
+	}
+	c.bstart[range] = pos;
+	c.gfirst[range] = groups;
+	mt_run(&c, args, tid, 2);
+	c.out_key = out_key;
+	c.out_count = out_count;
+	c.out_sum_v1 = out_sum_v1;
+	c.out_min_v1 = out_min_v1;
+	c.out_max_v1 = out_max_v1;
+	c.out_sum_v2 = out_sum_v2;
+	c.out_avg_v2 = out_avg_v2;
+	/* Tasks of about `chunk` rows: runs of small buckets, and large buckets cut into chunk-sized parts. */
+	const uint64_t chunk = n / ((uint64_t)threads * 16) + 1;
+	const uint64_t cap = range + n / chunk + 2;
+	c.task_b0 = (uint64_t *)malloc(cap * sizeof(uint64_t));
+	c.task_b1 = (uint64_t *)malloc(cap * sizeof(uint64_t));
+	c.task_r0 = (uint64_t *)malloc(cap * sizeof(uint64_t));
+	c.task_r1 = (uint64_t *)malloc(cap * sizeof(uint64_t));
+	c.task_slot = (int64_t *)malloc(cap * sizeof(int64_t));
+	uint64_t nt = 0, nslots = 0, run_b0 = 0, run_rows = 0;
+	for (uint64_t b = 0; b <= range; b++) {
+		const uint64_t cnt = b < range ? c.bstart[b + 1] - c.bstart[b] : 0;
+		if (b == range || cnt > chunk || run_rows >= chunk) {
+			if (b > run_b0) { /* close the run of small buckets */
+				c.task_b0[nt] = run_b0;
+				c.task_b1[nt] = b;
+				c.task_slot[nt++] = -1;
+			}
+			run_b0 = b;
+			run_rows = 0;
+		}
+		if (b == range) {
+			break;
+		}
+		if (cnt > chunk) {
+			for (uint64_t r = c.bstart[b]; r < c.bstart[b + 1]; r += chunk) {
+				c.task_b0[nt] = b;
+				c.task_b1[nt] = b + 1;
+				c.task_r0[nt] = r;
+				c.task_r1[nt] = r + chunk < c.bstart[b + 1] ? r + chunk : c.bstart[b + 1];
+				c.task_slot[nt++] = (int64_t)nslots++;
+			}
+			run_b0 = b + 1;
+		} else {
+			run_rows += cnt;
+		}
+	}
+	c.ntasks = nt;
+	c.next_task = 0;
+	c.part_s1 = (i128 *)malloc((nslots + 1) * sizeof(i128));
+	c.part_mn = (int64_t *)malloc((nslots + 1) * sizeof(int64_t));
+	c.part_mx = (int64_t *)malloc((nslots + 1) * sizeof(int64_t));
+	c.part_m = (msum_t *)calloc(nslots + 1, sizeof(msum_t));
+	mt_run(&c, args, tid, 3);
+	/* Merge the parts of each large group (exact: int128 adds, min/max, expansion components re-added). */
+	for (uint64_t k = 0; k < nt;) {
+		if (c.task_slot[k] < 0) {
+			k++;
+			continue;
+		}
+		const uint64_t b = c.task_b0[k];
+		i128 s1 = 0;
+		int64_t mn = INT64_MAX, mx = INT64_MIN;
+		msum_t m = {0};
+		msum_init(&m);
+		for (; k < nt && c.task_slot[k] >= 0 && c.task_b0[k] == b; k++) {
+			const int64_t sl = c.task_slot[k];
+			s1 += c.part_s1[sl];
+			mn = c.part_mn[sl] < mn ? c.part_mn[sl] : mn;
+			mx = c.part_mx[sl] > mx ? c.part_mx[sl] : mx;
+			const msum_t *pm = &c.part_m[sl];
+			for (size_t j = 0; j < pm->n; j++) {
+				msum_add(&m, pm->p[j]);
+			}
+			if (pm->has_special) {
+				msum_add(&m, pm->special);
+			}
+			free(pm->p);
+		}
+		const uint64_t g = c.gfirst[b];
+		const uint64_t cnt = c.bstart[b + 1] - c.bstart[b];
+		if (out_key) {
+			out_key[g] = (int64_t)((uint64_t)mn_key_base(c.mn) + b);
+		}
+		if (out_count) {
+			out_count[g] = (int64_t)cnt;
+		}
+		if (out_sum_v1) {
+			out_sum_v1[2 * g] = (int64_t)(uint64_t)(u128)s1;
+			out_sum_v1[2 * g + 1] = (int64_t)(s1 >> 64);
+		}
+		if (out_min_v1) {
+			out_min_v1[g] = v1 ? mn : INT64_MAX;
+		}
+		if (out_max_v1) {
+			out_max_v1[g] = v1 ? mx : INT64_MIN;
+		}
+		const double s2 = v2 ? msum_value(&m) : 0.0;
+		if (out_sum_v2) {
+			out_sum_v2[g] = s2;
+		}
+		if (out_avg_v2) {
+			out_avg_v2[g] = s2 / (double)cnt;
+		}
+		free(m.p);
+	}
+	free(c.task_b0);
+	free(c.task_b1);
+	free(c.task_r0);
+	free(c.task_r1);
+	free(c.task_slot);
+	free(c.part_s1);
+	free(c.part_mn);
+	free(c.part_mx);
+	free(c.part_m);
+	free(c.hist);
+	free(c.order);
+	free(c.gfirst);
+	free(c.bstart);
+	return groups;
 }
 
 /* ------------------------------------------------------------------ join (config 2a) */

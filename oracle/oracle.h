@@ -44,6 +44,11 @@ double vo_exact_sum(const double *x, uint64_t n);
 uint64_t vo_group_by_i64(const int64_t *key, const int64_t *v1, const double *v2, uint64_t n, int64_t *out_key,
                          int64_t *out_count, int64_t *out_sum_v1, int64_t *out_min_v1, int64_t *out_max_v1,
                          double *out_sum_v2, double *out_avg_v2);
+/* The same GROUP BY on `threads` threads (stable counting sort with per-thread histograms, then every group
+ * reduced by one thread with the same arithmetic): identical outputs. */
+uint64_t vo_group_by_i64_mt(const int64_t *key, const int64_t *v1, const double *v2, uint64_t n, int threads,
+                            int64_t *out_key, int64_t *out_count, int64_t *out_sum_v1, int64_t *out_min_v1,
+                            int64_t *out_max_v1, double *out_sum_v2, double *out_avg_v2);
 
 /* Returns COUNT(*); *sum_lo/*sum_hi receive SUM(payload) as int128. */
 uint64_t vo_join_sum_count(const int64_t *build_key, const int32_t *payload, uint64_t nb, const int64_t *fk,

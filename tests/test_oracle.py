@@ -173,3 +173,33 @@ def test_reference_known_answers_group_by():
         got = {int(k): (int(c), int(s)) for k, c, s in zip(res["key"], res["count"], res["sum_v1"])}
         exp = {int(k): tuple(x) for k, x in case["expected"].items()}
         assert got == exp, case["name"]
+
+
+@pytest.mark.parametrize("threads,n,distinct", [(2, 10, 3), (8, 100_000, 1000), (16, 300_000, 70_000),
+                                                (5, 50_001, 1), (8, 1000, 1000)])
+def test_multithreaded_group_by_matches_single_threaded(threads, n, distinct):
+    """The multithreaded restatement (the CPU baseline the bench times) returns exactly the single-threaded
+    oracle's groups: same keys and order, counts, int128 sums, min/max and the same double bits."""
+    rng = np.random.default_rng(n + threads)
+    key = (rng.zipf(1.3, n) % distinct - distinct // 2).astype(np.int64)
+    v1 = rng.integers(-10 ** 12, 10 ** 12, n).astype(np.int64)
+    v2 = rng.standard_normal(n) * 10.0 ** rng.integers(-5, 5, n)
+    a = oracle.group_by_i64(key, v1, v2)
+    b = oracle.group_by_i64(key, v1, v2, threads=threads)
+    for k in ("key", "count", "min_v1", "max_v1"):
+        np.testing.assert_array_equal(a[k], b[k])
+    assert a["sum_v1"] == b["sum_v1"]
+    np.testing.assert_array_equal(a["sum_v2"].view(np.int64), b["sum_v2"].view(np.int64))
+    np.testing.assert_array_equal(a["avg_v2"].view(np.int64), b["avg_v2"].view(np.int64))
+
+
+def test_multithreaded_group_by_wide_range_falls_back():
+    """Keys spanning more than 2^24 take the single-threaded path; the answer is the same."""
+    rng = np.random.default_rng(3)
+    key = rng.integers(-(1 << 40), 1 << 40, 5000).astype(np.int64)
+    v1 = rng.integers(-100, 100, 5000).astype(np.int64)
+    v2 = rng.random(5000)
+    a = oracle.group_by_i64(key, v1, v2)
+    b = oracle.group_by_i64(key, v1, v2, threads=4)
+    np.testing.assert_array_equal(a["key"], b["key"])
+    np.testing.assert_array_equal(a["sum_v2"], b["sum_v2"])
