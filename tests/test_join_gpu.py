@@ -115,3 +115,35 @@ def test_join_key_type_rules(dev):
         capi.join_pairs(_dev(a, dev), _dev(b, dev))
     with pytest.raises(capi.VgpuError, match="integer/boolean"):
         capi.join_pairs(_dev(f, dev), _dev(f, dev))
+
+
+@pytest.mark.parametrize("dtype,flip", [(np.int64, False), (np.int32, False), (np.int16, True), (np.int32, True)])
+def test_join_count_side_narrowed_to_partner_range(dev, dtype, flip):
+    """The larger side that only contributes counts is narrowed to the other side's key range before it is
+    sorted (keys below and above that range, negative keys): COUNT(*), SUM over the other side and MIN/MAX
+    stay exact against brute force, with the count-only side as A or as B."""
+    rng = np.random.default_rng(int(np.dtype(dtype).itemsize) + flip)
+    big = rng.integers(-3000, 9000, 200_000).astype(dtype)      # count-only side, wider range
+    small = rng.integers(-1000, 5000, 20_000).astype(dtype)     # side with values
+    vals = rng.integers(-10 ** 9, 10 ** 9, len(small)).astype(np.int64)
+    kb, cb = np.unique(big, return_counts=True)
+    ks, cs = np.unique(small, return_counts=True)
+    count = 0
+    total = 0
+    mn = None
+    for k, c in zip(kb.tolist(), cb.tolist()):
+        sel = small == k
+        if sel.any():
+            count += c * int(sel.sum())
+            total += c * int(vals[sel].sum())
+            m = int(vals[sel].min())
+            mn = m if mn is None else min(mn, m)
+    if not flip:  # A = big (count only), B = small with values
+        res = capi.join_aggregate(_dev(big, dev), [], _dev(small, dev), [_dev(vals, dev)],
+                                  [(capi.COUNT_STAR, -1, -1, capi.INT64), (capi.SUM, 1, 0, capi.HUGEINT),
+                                   (capi.MIN, 1, 0, capi.INT64)])
+    else:         # A = small with values, B = big (count only)
+        res = capi.join_aggregate(_dev(small, dev), [_dev(vals, dev)], _dev(big, dev), [],
+                                  [(capi.COUNT_STAR, -1, -1, capi.INT64), (capi.SUM, 0, 0, capi.HUGEINT),
+                                   (capi.MIN, 0, 0, capi.INT64)])
+    assert res[0] == count and res[1] == total and res[2] == mn

@@ -273,6 +273,11 @@ void RangeProbeAgg(int key_type, const void *probe_keys, const uint64_t *seg, co
 bool BuildDenseIndex(int key_type, const void *keys, const uint64_t *count, const uint64_t *sum, uint64_t nd,
                      uint64_t min, uint64_t span, uint64_t *dense, unsigned int *overflow_dev, hipStream_t stream);
 
+//! Order-preserving compaction of the signed keys with lo <= x <= hi into `out` (n slots); returns the kept
+//! count. Synchronizes.
+uint64_t KeepKeyRange(int vtype, const void *in, uint64_t n, long long lo, long long hi, void *out, Scratch &scratch,
+                      hipStream_t stream);
+
 //! Order-preserving compaction of x > threshold (the FilterGreaterThanInt32 primitive).
 void FilterGtI32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_device,
                  Scratch &scratch, hipStream_t stream);

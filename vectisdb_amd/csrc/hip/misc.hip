@@ -373,6 +373,134 @@ __global__ void __launch_bounds__(kFgBlock) FilterGtKernel(const int32_t *in, ui
 }
 } // namespace
 
+namespace {
+//! Order-preserving compaction of the keys with lo <= x <= hi (signed). Tile = 256 threads x kItems rows,
+//! wave-striped (item k of a thread is row base + k * 256 + tid, so every load is one coalesced line per
+//! wave); a row's slot in the tile is the kept count of the (item, wave) groups before it (ballots, one
+//! small scan in LDS) plus its rank among its wave's kept lanes, so each wave writes contiguous runs; the
+//! tile's global offset comes from a decoupled look-back.
+//! 128 bytes per thread: tiles of 4096 (8-byte keys) or 8192 (4-byte) rows.
+template <typename T>
+constexpr int KeepRangeItems() {
+	return (int)(128 / sizeof(T));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFgBlock) KeepRangeKernel(const T *in, uint64_t n, long long lo, long long hi,
+                                                           T *out, u64 *state, uint32_t *ticket, u64 *count_out,
+                                                           uint64_t ntiles) {
+	constexpr int kItems = KeepRangeItems<T>();
+	constexpr int kWaves = kFgBlock / kWave;
+	__shared__ uint32_t s_tile;
+	__shared__ uint32_t s_off[kItems * kWaves];
+	__shared__ u64 s_scan[kWaves + 1];
+	__shared__ u64 s_prefix;
+	const int tid = threadIdx.x;
+	const int lane = tid & (kWave - 1);
+	const int wave = tid / kWave;
+	if (tid == 0) {
+		s_tile = FetchAddAgent(ticket, 1u);
+	}
+	__syncthreads();
+	const uint32_t tile = s_tile;
+	const uint64_t base = (uint64_t)tile * kFgBlock * kItems;
+	T v[kItems];
+	u64 ball[kItems];
+#pragma unroll
+	for (int k = 0; k < kItems; k++) {
+		const uint64_t row = base + (uint64_t)k * kFgBlock + tid;
+		v[k] = row < n ? in[row] : (T)0;
+	}
+#pragma unroll
+	for (int k = 0; k < kItems; k++) {
+		const uint64_t row = base + (uint64_t)k * kFgBlock + tid;
+		const long long x = (long long)v[k];
+		ball[k] = __ballot(row < n && x >= lo && x <= hi);
+		if (lane == 0) {
+			s_off[k * kWaves + wave] = (uint32_t)Popc64(ball[k]);
+		}
+	}
+	__syncthreads();
+	static_assert(kItems * kWaves <= kFgBlock, "one (item, wave) count per thread");
+	{
+		// exclusive scan of the (item, wave) counts, one per thread
+		const u64 mine = tid < kItems * kWaves ? s_off[tid] : 0ull;
+		u64 run = 0;
+		const u64 ex = BlockExclusiveScan<u64, kFgBlock>(mine, s_scan, &run);
+		if (tid < kItems * kWaves) {
+			s_off[tid] = (uint32_t)ex;
+		}
+		if (tid == 0) {
+			StoreAgent(state + tile, LbPack(tile == 0 ? kLbInclusive : kLbAggregate, run));
+			s_prefix = run; // tile total until the look-back below replaces it
+		}
+	}
+	__syncthreads();
+	if (wave == 0) {
+		const u64 total = s_prefix;
+		const u64 prefix = tile == 0 ? 0ull : LookbackExclusiveWave(state, tile);
+		if (lane == 0) {
+			if (tile > 0) {
+				StoreAgent(state + tile, LbPack(kLbInclusive, prefix + total));
+			}
+			if (tile == ntiles - 1) {
+				*count_out = prefix + total;
+			}
+		}
+		__builtin_amdgcn_wave_barrier();
+		if (lane == 0) {
+			s_prefix = prefix;
+		}
+	}
+	__syncthreads();
+	const u64 tile_out = s_prefix;
+	const u64 lt = LaneMaskLt();
+#pragma unroll
+	for (int k = 0; k < kItems; k++) {
+		if ((ball[k] >> lane) & 1ull) {
+			out[tile_out + s_off[k * kWaves + wave] + (u64)Popc64(ball[k] & lt)] = v[k];
+		}
+	}
+}
+
+template <typename T>
+uint64_t KeepRangeT(const void *in, uint64_t n, long long lo, long long hi, void *out, Scratch &scratch,
+                    hipStream_t stream) {
+	constexpr uint64_t tile = (uint64_t)kFgBlock * KeepRangeItems<T>();
+	const uint64_t ntiles = (n + tile - 1) / tile;
+	auto *state = static_cast<u64 *>(scratch.Alloc((ntiles + 2) * sizeof(u64)));
+	u64 *count = state + ntiles + 1;
+	FillU64(state, 0ull, ntiles + 2, stream);
+	{
+		ProfileScope prof("keep_key_range", stream, (double)n * sizeof(T));
+		hipLaunchKernelGGL((KeepRangeKernel<T>), dim3((unsigned)ntiles), dim3(kFgBlock), 0, stream, (const T *)in, n,
+		                   lo, hi, (T *)out, state, reinterpret_cast<uint32_t *>(state + ntiles), count, ntiles);
+		VGPU_HIP(hipGetLastError());
+	}
+	uint64_t kept = 0;
+	VGPU_HIP(hipMemcpyAsync(&kept, count, 8, hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	return kept;
+}
+} // namespace
+
+uint64_t KeepKeyRange(int vtype, const void *in, uint64_t n, long long lo, long long hi, void *out, Scratch &scratch,
+                      hipStream_t stream) {
+	if (n == 0) {
+		return 0;
+	}
+	switch (vtype) {
+	case VT_INT16:
+		return KeepRangeT<int16_t>(in, n, lo, hi, out, scratch, stream);
+	case VT_INT32:
+		return KeepRangeT<int32_t>(in, n, lo, hi, out, scratch, stream);
+	case VT_INT64:
+		return KeepRangeT<int64_t>(in, n, lo, hi, out, scratch, stream);
+	default:
+		throw HipError("KeepKeyRange: INT16, INT32 or INT64 keys");
+	}
+}
+
 void FilterGtI32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_device,
                  Scratch &scratch, hipStream_t stream) {
 	if (n == 0) {

@@ -445,10 +445,15 @@ __global__ void __launch_bounds__(kRedBlock)
 	}
 }
 
-//! Count-only reduce (every aggregate is a COUNT): a run-length encoding of the sorted keys. Each tile flags
-//! the rows whose key differs from the row before, ranks the flags (block scan, then decoupled look-back
-//! over per-tile head counts) and writes each group's key and first row; RleCountKernel turns first rows
-//! into counts. No aggregate state is scanned or stitched across tiles.
+//! Count-only reduce (every aggregate is a COUNT): a run-length encoding of the sorted keys. Each thread
+//! reads 32 consecutive keys straight into registers (vector loads; the key before them is one extra load
+//! that hits the neighbour's cache line), flags the rows whose key differs from the row before, and the
+//! tile ranks its flags (block scan, then decoupled look-back over per-tile head counts). Each group's key
+//! and first row are written; RleCountKernel turns first rows into counts. No aggregate state is scanned
+//! or stitched across tiles.
+constexpr int kRleItems = 32;
+constexpr int kRleTile = kRedBlock * kRleItems;
+
 template <typename KeyT>
 __global__ void __launch_bounds__(kRedBlock)
     RleHeadsKernel(const KeyT *keys, uint64_t n, KeyOutDesc ko, u64 *starts, u64 *tile_state, uint32_t *ticket,
@@ -456,8 +461,6 @@ __global__ void __launch_bounds__(kRedBlock)
 	__shared__ uint32_t s_tile;
 	__shared__ u64 s_scan[kRedWaves + 1];
 	__shared__ u64 s_hb;
-	__shared__ KeyT s_keys[kRedItems * kSlotStride];
-	__shared__ KeyT s_prev;
 	const int tid = threadIdx.x;
 	const int lane = tid & (kWave - 1);
 	const int wave = tid / kWave;
@@ -466,25 +469,33 @@ __global__ void __launch_bounds__(kRedBlock)
 	}
 	__syncthreads();
 	const uint32_t tile = s_tile;
-	const uint64_t tile_base = (uint64_t)tile * kRedTile;
-	const uint32_t tile_count = (uint32_t)((n - tile_base) < (uint64_t)kRedTile ? (n - tile_base) : kRedTile);
-	StageTransposed<KeyT>(s_keys, keys + tile_base, tile_count);
-	if (tid == 0) {
-		s_prev = tile_base > 0 ? keys[tile_base - 1] : (KeyT)0;
-	}
-	__syncthreads();
-	const uint32_t local0 = (uint32_t)tid * kRedItems;
-	KeyT k[kRedItems];
-	bool head[kRedItems];
-	u64 hc = 0;
-	KeyT before = tid == 0 ? s_prev : s_keys[(kRedItems - 1) * kSlotStride + tid - 1];
+	const uint64_t row0 = (uint64_t)tile * kRleTile + (uint64_t)tid * kRleItems;
+	KeyT k[kRleItems];
+	if (row0 + kRleItems <= n && (reinterpret_cast<uintptr_t>(keys) & 15) == 0) {
+		// 32 consecutive keys: 128 B (u32) or 256 B (u64), 16-byte aligned (row0 is a multiple of 32)
+		constexpr int kVecs = (int)(kRleItems * sizeof(KeyT) / 16);
+		const uint4 *src = reinterpret_cast<const uint4 *>(keys + row0);
+		uint4 v[kVecs];
 #pragma unroll
-	for (int i = 0; i < kRedItems; i++) {
-		const uint32_t lr = local0 + i;
-		k[i] = lr < tile_count ? s_keys[i * kSlotStride + tid] : (KeyT)0;
-		head[i] = lr < tile_count && (tile_base + lr == 0 || k[i] != before);
+		for (int j = 0; j < kVecs; j++) {
+			v[j] = src[j];
+		}
+		__builtin_memcpy(k, v, sizeof(k));
+	} else {
+#pragma unroll
+		for (int i = 0; i < kRleItems; i++) {
+			k[i] = row0 + i < n ? keys[row0 + i] : (KeyT)0;
+		}
+	}
+	KeyT before = row0 > 0 && row0 <= n ? keys[row0 - 1] : (KeyT)0;
+	uint32_t heads = 0; // bit i = row0 + i starts a group
+	u64 hc = 0;
+#pragma unroll
+	for (int i = 0; i < kRleItems; i++) {
+		const bool h = row0 + i < n && (row0 + i == 0 || k[i] != before);
 		before = k[i];
-		hc += head[i] ? 1 : 0;
+		heads |= h ? (1u << i) : 0u;
+		hc += h ? 1 : 0;
 	}
 	u64 H;
 	const u64 hb_excl = BlockExclusiveScan<u64, kRedBlock>(hc, s_scan, &H);
@@ -503,10 +514,10 @@ __global__ void __launch_bounds__(kRedBlock)
 	__syncthreads();
 	u64 gid = s_hb + hb_excl;
 #pragma unroll
-	for (int i = 0; i < kRedItems; i++) {
-		if (head[i]) {
+	for (int i = 0; i < kRleItems; i++) {
+		if ((heads >> i) & 1u) {
 			WriteKeys(ko, (u64)k[i], gid);
-			starts[gid] = tile_base + local0 + i;
+			starts[gid] = row0 + i;
 			gid++;
 		}
 	}
@@ -531,7 +542,7 @@ __global__ void RleCountKernel(const u64 *starts, const u64 *groups, uint64_t n,
 template <typename KeyT>
 void LaunchRle(const SortedRun &run, uint64_t n, uint64_t capacity, const AggOutDesc &od, const KeyOutDesc &ko,
                u64 *groups_device, Scratch &scratch, hipStream_t stream) {
-	const uint64_t ntiles = (n + kRedTile - 1) / kRedTile;
+	const uint64_t ntiles = (n + kRleTile - 1) / kRleTile;
 	auto *lb = static_cast<u64 *>(scratch.Alloc((ntiles + 1) * sizeof(u64)));
 	auto *ticket = reinterpret_cast<uint32_t *>(lb + ntiles);
 	auto *starts = static_cast<u64 *>(scratch.Alloc(std::max<uint64_t>(capacity, 1) * sizeof(u64)));

@@ -89,6 +89,31 @@ SideRuns ReduceSide(const DevCol &key, const std::vector<DevCol> &values, const 
 	return s;
 }
 
+//! First and last key of an ascending distinct-key run, sign-extended.
+void KeyBounds(const void *keys, uint64_t groups, int vtype, long long *lo, long long *hi, hipStream_t stream) {
+	const int w = TypeWidth(vtype);
+	uint8_t buf[2][8] = {};
+	VGPU_HIP(hipMemcpyAsync(buf[0], keys, w, hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipMemcpyAsync(buf[1], static_cast<const uint8_t *>(keys) + (groups - 1) * w, w, hipMemcpyDeviceToHost,
+	                        stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	for (int i = 0; i < 2; i++) {
+		long long v = 0;
+		if (w == 2) {
+			int16_t t;
+			std::memcpy(&t, buf[i], 2);
+			v = t;
+		} else if (w == 4) {
+			int32_t t;
+			std::memcpy(&t, buf[i], 4);
+			v = t;
+		} else {
+			std::memcpy(&v, buf[i], 8);
+		}
+		(i == 0 ? *lo : *hi) = v;
+	}
+}
+
 //! The double-double state keeps the high double in `lo` and the low-order correction in `hi`.
 double DdStateToDouble(const JoinAggValue &v) {
 	double head, tail;
@@ -193,8 +218,40 @@ std::vector<JoinAggValue> JoinAggregateDevice(const DevCol &akey, const std::vec
 	if (akey.rows == 0 || bkey.rows == 0) {
 		return result; // zero rows: COUNT 0, SUM 0 (the caller applies SQL NULL rules for empty input)
 	}
-	SideRuns A = ReduceSide(akey, avals, side_aggs[0], false, scratch, stream);
-	SideRuns B = ReduceSide(bkey, bvals, side_aggs[1], false, scratch, stream);
+	// A side that only contributes its row counts and is the larger one is first narrowed to the other
+	// side's key range: its rows outside that range have no partner, and the narrower key span can save a
+	// radix pass (a probe side over twice the build keys' range loses a whole digit).
+	int narrow = -1;
+	if (akey.type != VT_BOOLEAN) {
+		if (side_aggs[0].empty() && akey.rows >= bkey.rows) {
+			narrow = 0;
+		} else if (side_aggs[1].empty() && bkey.rows > akey.rows) {
+			narrow = 1;
+		}
+	}
+	SideRuns A, B;
+	if (narrow < 0) {
+		A = ReduceSide(akey, avals, side_aggs[0], false, scratch, stream);
+		B = ReduceSide(bkey, bvals, side_aggs[1], false, scratch, stream);
+	} else {
+		const DevCol &wide = narrow == 0 ? akey : bkey;
+		SideRuns &other = narrow == 0 ? B : A;
+		other = narrow == 0 ? ReduceSide(bkey, bvals, side_aggs[1], false, scratch, stream)
+		                    : ReduceSide(akey, avals, side_aggs[0], false, scratch, stream);
+		if (other.groups == 0) {
+			return result;
+		}
+		long long lo = 0, hi = 0;
+		KeyBounds(other.keys, other.groups, wide.type, &lo, &hi, stream);
+		DevCol kept = wide;
+		void *buf = scratch.Alloc(wide.rows * TypeWidth(wide.type));
+		kept.data = buf;
+		kept.rows = KeepKeyRange(wide.type, wide.data, wide.rows, lo, hi, buf, scratch, stream);
+		if (kept.rows == 0) {
+			return result;
+		}
+		(narrow == 0 ? A : B) = ReduceSide(kept, {}, {}, false, scratch, stream);
+	}
 	auto *match = static_cast<long long *>(scratch.Alloc(std::max<uint64_t>(A.groups, 1) * 8));
 	MergeMatch(A.keys, A.groups, B.keys, B.groups, akey.type, match, stream);
 	args.na = A.groups;
