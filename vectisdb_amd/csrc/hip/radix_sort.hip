@@ -77,6 +77,10 @@ __global__ void __launch_bounds__(256) KeyRangeKernel(RangeCols cols, uint64_t n
 	}
 }
 
+//! Loads in flight per lane per loop step of the key-range pass (enough bytes in flight to cover HBM
+//! latency with one wave generation on the chip).
+constexpr int kHistUnroll = 8;
+
 //! Min/max of the ordered key and the histograms of all 8 byte digits in one read. A digit that is the
 //! same on every active lane (the constant high bytes of a narrow key range, or a Zipf head value) is
 //! counted with one LDS atomic per wave instead of 64 conflicting ones.
@@ -93,15 +97,15 @@ __global__ void __launch_bounds__(256) KeyRangeHist8Kernel(const T *col, int wid
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	const uint64_t start = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	const uint64_t wave_start = start - lane;
-	for (uint64_t row0 = start, w0 = wave_start; w0 < n; row0 += 4 * stride, w0 += 4 * stride) {
-		u64 raw[4];
+	for (uint64_t row0 = start, w0 = wave_start; w0 < n; row0 += kHistUnroll * stride, w0 += kHistUnroll * stride) {
+		u64 raw[kHistUnroll];
 #pragma unroll
-		for (int u = 0; u < 4; u++) {
+		for (int u = 0; u < kHistUnroll; u++) {
 			uint64_t row = row0 + u * stride;
 			raw[u] = row < n ? (u64)col[row] : 0ull;
 		}
 #pragma unroll
-		for (int u = 0; u < 4; u++) {
+		for (int u = 0; u < kHistUnroll; u++) {
 			uint64_t row = row0 + u * stride;
 			const bool valid = row < n;
 			const u64 vmask = __ballot(valid);

@@ -235,8 +235,35 @@ __device__ __forceinline__ void StageTransposed(D *dst, const T *src, uint32_t c
 	}
 }
 
-//! Reduce tile: 256 threads x 8 consecutive rows. Keys and payload columns are staged into LDS with one
-//! coalesced sweep each, then every thread folds its 8 rows serially from LDS.
+//! Loads `count` (<= 8) consecutive elements of width `width` at row `row0` as raw bits: vector loads when
+//! all 8 are there and the address is 16-byte aligned, element loads otherwise.
+template <typename T>
+__device__ __forceinline__ void LoadRun8(const T *src, uint64_t row0, uint64_t n, u64 (&out)[kRedItems]) {
+	constexpr int kVecs = (int)(kRedItems * sizeof(T) / 16);
+	if (kVecs > 0 && row0 + kRedItems <= n && (reinterpret_cast<uintptr_t>(src + row0) & 15) == 0) {
+		T t[kRedItems];
+		uint4 v[kVecs > 0 ? kVecs : 1];
+#pragma unroll
+		for (int j = 0; j < kVecs; j++) {
+			v[j] = reinterpret_cast<const uint4 *>(src + row0)[j];
+		}
+		__builtin_memcpy(t, v, sizeof(t));
+#pragma unroll
+		for (int i = 0; i < kRedItems; i++) {
+			out[i] = (u64)t[i];
+		}
+	} else {
+#pragma unroll
+		for (int i = 0; i < kRedItems; i++) {
+			out[i] = row0 + i < n ? (u64)src[row0 + i] : 0ull;
+		}
+	}
+}
+
+//! Reduce tile: 256 threads x 8 consecutive rows, each thread folding its rows serially. The generic kernel
+//! stages keys and payload columns into LDS with one coalesced sweep each (transposed, conflict-free); the
+//! kernels specialized for one component signature (DIRECT) load each thread's 8 rows straight into
+//! registers with 16-byte vector loads instead, which frees the 43 KB of LDS that capped occupancy.
 template <typename KeyT, int NP, typename SG>
 __global__ void __launch_bounds__(kRedBlock)
     SegReduceKernel(const KeyT *keys, uint64_t n, PayloadPtrs pay, CompDesc d, AggOutDesc o, KeyOutDesc ko,
@@ -247,9 +274,11 @@ __global__ void __launch_bounds__(kRedBlock)
 	__shared__ u64 s_hb;
 	__shared__ u64 s_wave_state[kRedWaves][2 * kMaxComponents];
 	__shared__ int s_wave_flag[kRedWaves];
-	__shared__ KeyT s_keys[kRedItems * kSlotStride];
+	constexpr bool DIRECT = SG::kStatic;
+	constexpr int kStage = DIRECT ? 1 : kRedItems * kSlotStride;
+	__shared__ KeyT s_keys[kStage];
 	__shared__ KeyT s_prev;
-	__shared__ u64 s_pay[NP > 0 ? NP : 1][kRedItems * kSlotStride]; // raw bits, zero-extended
+	__shared__ u64 s_pay[NP > 0 ? NP : 1][kStage]; // raw bits, zero-extended
 
 	const int tid = threadIdx.x;
 	const int lane = tid & (kWave - 1);
@@ -262,11 +291,41 @@ __global__ void __launch_bounds__(kRedBlock)
 	const uint64_t tile_base = (uint64_t)tile * kRedTile;
 	const uint32_t tile_count = (uint32_t)((n - tile_base) < (uint64_t)kRedTile ? (n - tile_base) : kRedTile);
 
+	// DIRECT: this thread's 8 rows in registers (raw bits) and the key of the row before them.
+	u64 kreg[kRedItems];
+	u64 preg[NP > 0 ? NP : 1][kRedItems];
+	KeyT kprev = (KeyT)0;
+	if constexpr (DIRECT) {
+		const uint64_t row0 = tile_base + (uint64_t)tid * kRedItems;
+		if (keys != nullptr) {
+			LoadRun8<KeyT>(keys, row0, n, kreg);
+			kprev = row0 > 0 && row0 <= n ? keys[row0 - 1] : (KeyT)0;
+		}
+#pragma unroll
+		for (int p = 0; p < NP; p++) {
+			if (p < pay.n) {
+				switch (pay.width[p]) {
+				case 8:
+					LoadRun8<u64>(reinterpret_cast<const u64 *>(pay.ptr[p]), row0, n, preg[p]);
+					break;
+				case 4:
+					LoadRun8<uint32_t>(reinterpret_cast<const uint32_t *>(pay.ptr[p]), row0, n, preg[p]);
+					break;
+				case 2:
+					LoadRun8<uint16_t>(reinterpret_cast<const uint16_t *>(pay.ptr[p]), row0, n, preg[p]);
+					break;
+				default:
+					LoadRun8<uint8_t>(reinterpret_cast<const uint8_t *>(pay.ptr[p]), row0, n, preg[p]);
+					break;
+				}
+			}
+		}
+	}
 	// Stage, transposed: row r of the tile goes to slot (r % 8) * kSlotStride + r / 8, so a thread's 8
 	// consecutive rows are 8 slots one stride apart and a wave reads 64 consecutive slots per step; the
 	// stride (264 = 8 mod 64) keeps the staging writes conflict-free too. Loads are coalesced per wave and
 	// all issued before the first LDS store. s_prev = key before the tile.
-	if (keys != nullptr) {
+	if (!DIRECT && keys != nullptr) {
 		StageTransposed<KeyT>(s_keys, keys + tile_base, tile_count);
 		if (tid == 0) {
 			s_prev = tile_base > 0 ? keys[tile_base - 1] : (KeyT)0;
@@ -274,7 +333,7 @@ __global__ void __launch_bounds__(kRedBlock)
 	}
 #pragma unroll
 	for (int p = 0; p < NP; p++) {
-		if (p < pay.n) {
+		if (!DIRECT && p < pay.n) {
 			const int w = pay.width[p];
 			switch (w) {
 			case 8:
@@ -302,11 +361,17 @@ __global__ void __launch_bounds__(kRedBlock)
 	KeyT k[kRedItems];
 	bool head[kRedItems];
 	u64 hc = 0;
-	KeyT before = keys == nullptr ? (KeyT)0 : (tid == 0 ? s_prev : s_keys[(kRedItems - 1) * kSlotStride + tid - 1]);
+	KeyT before = keys == nullptr ? (KeyT)0
+	                              : (DIRECT ? kprev
+	                                        : (tid == 0 ? s_prev : s_keys[(kRedItems - 1) * kSlotStride + tid - 1]));
 #pragma unroll
 	for (int i = 0; i < kRedItems; i++) {
 		uint32_t lr = local0 + i;
-		k[i] = (keys != nullptr && lr < tile_count) ? s_keys[i * kSlotStride + tid] : (KeyT)0;
+		if constexpr (DIRECT) {
+			k[i] = (keys != nullptr && lr < tile_count) ? (KeyT)kreg[i] : (KeyT)0;
+		} else {
+			k[i] = (keys != nullptr && lr < tile_count) ? s_keys[i * kSlotStride + tid] : (KeyT)0;
+		}
 		uint64_t row = base + i;
 		head[i] = lr < tile_count && (row == 0 || k[i] != before);
 		before = k[i];
@@ -345,7 +410,11 @@ __global__ void __launch_bounds__(kRedBlock)
 			u64 vals[NP > 0 ? NP : 1];
 #pragma unroll
 			for (int p = 0; p < (NP > 0 ? NP : 1); p++) {
-				vals[p] = (p < pay.n) ? s_pay[p][i * kSlotStride + tid] : 0ull;
+				if constexpr (DIRECT) {
+					vals[p] = (p < pay.n) ? preg[p][i] : 0ull;
+				} else {
+					vals[p] = (p < pay.n) ? s_pay[p][i * kSlotStride + tid] : 0ull;
+				}
 			}
 			if (head[i]) {
 				if (seen) {
@@ -424,7 +493,8 @@ __global__ void __launch_bounds__(kRedBlock)
 			bool ends = tile_end >= n;
 			if (!ends && keys != nullptr) {
 				// keys == nullptr: one global group, which only ends with the input
-				ends = keys[tile_end] != s_keys[(kRedItems - 1) * kSlotStride + kRedBlock - 1];
+				ends = keys[tile_end] != (DIRECT ? (KeyT)kreg[kRedItems - 1]
+				                                  : s_keys[(kRedItems - 1) * kSlotStride + kRedBlock - 1]);
 			}
 			if (ends) {
 				WriteGroup<SG>(o, X, gid_last);
