@@ -163,6 +163,35 @@ def bench_filter_project(torch, capi, dev, steps, copies=16, rows=1_048_576):
             "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def bench_filter_project_large(torch, capi, dev, steps, rows=1 << 28):
+    """The config-0 query (same columns, distributions and expressions) over 2^28 rows: the same generated
+    kernel when it is not launch-bound, i.e. its streaming efficiency."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from exprs import cfg0_exprs
+    preds, projs = cfg0_exprs()
+    a = torch.empty(rows, dtype=torch.int32, device=dev)
+    b = torch.empty(rows, dtype=torch.int64, device=dev)
+    x = torch.empty(rows, dtype=torch.float64, device=dev)
+    capi.generate_column(a, capi.INT32, 0, lo=0, hi=999_999, seed=5, stream_id=1)
+    capi.generate_column(b, capi.INT64, 0, lo=-(1 << 40), hi=(1 << 40) - 1, seed=5, stream_id=2)
+    capi.generate_column(x, capi.FLOAT64, 1, seed=5, stream_id=3)
+    torch.cuda.synchronize()
+    capi.filter_project([a, b, x], preds, projs)
+    capi.profile_reset()
+    capi.profile_enable(True)
+    kept = 0
+    for _ in range(steps):
+        _, _, _, kept = capi.filter_project([a, b, x], preds, projs)
+    torch.cuda.synchronize()
+    capi.profile_enable(False)
+    avg, _ = kernel_stats(capi.profile_read(), "fused_filter_project")
+    alg = rows * 20 + kept * 16
+    gbs = alg / (avg * 1e-3) / 1e9
+    return {"operator": "fused filter/project, config-0 query over 2^28 rows (streaming efficiency)",
+            "rows": rows, "kept": kept, "kernel_ms": round(avg, 4), "rows_per_s": round(rows / (avg * 1e-3), 1),
+            "algorithmic_bytes": alg, "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
     """Join + aggregate: build 2^24 unique keys (permutation) + INT32 payload, probe 2^28 fk (90% hits);
     SELECT SUM(payload), COUNT(*) FROM probe JOIN build ON fk = key."""
@@ -556,11 +585,13 @@ def main():
         os.environ.pop("VGPU_RADIX_BITS", None)
 
     operators = []
-    extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "cross", "join", "batched", "served"]
+    extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "filter_large", "cross", "join", "batched", "served"]
     for name in extra:
         try:
             if name == "filter":
                 operators.append(bench_filter_project(torch, capi, dev, 50))
+            elif name == "filter_large":
+                operators.append(bench_filter_project_large(torch, capi, dev, 10))
             elif name == "cross":
                 operators.append(bench_cross(torch, capi, dev, 10))
             elif name == "join":

@@ -17,7 +17,19 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kVec = 4;
 constexpr int kRounds = 2;
-constexpr int kTile = kBlock * kVec * kRounds; // 2048 rows (or pairs, or selection entries)
+constexpr int kTile = kBlock * kVec * kRounds; // smallest tile: 2048 rows (or pairs, or selection entries)
+
+//! Rounds of kVec rows per lane in a tile: narrow kernels (at most 4 input columns and 4 outputs) take 4
+//! (4096-row tiles: half the per-tile scan, look-back and staging overhead), cross products of at most 4
+//! columns 8 (the pairs come from cache, so per-tile overhead is all there is), wide kernels keep 2 so
+//! their register tiles stay small.
+int FusedRounds(const FusedSpec &s) {
+	const bool narrow = s.cols.size() <= 4 && s.projections.size() <= 4;
+	if (s.cross) {
+		return narrow ? 8 : 4;
+	}
+	return narrow ? 4 : kRounds;
+}
 
 const char *CType(int vtype) {
 	switch (vtype) {
@@ -147,12 +159,14 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 		}
 	}
 	const bool filter = !s.predicates.empty();
+	const int rounds = FusedRounds(s);
+	const int tile_rows = kBlock * kVec * rounds;
 	std::ostringstream o;
 	o << kPreamble;
 	o << "extern \"C\" __global__ void __launch_bounds__(" << kBlock << ") vectis_fused(VArgs a) {\n";
 	o << "  const int tid = threadIdx.x; const int lane = tid & 63; const int wave = tid >> 6; (void)lane;\n";
 	// Output staging buffer: each output (and the selection) leaves the block as one contiguous run.
-	o << "  __shared__ vg_u64 s_stage[" << kTile << "];\n";
+	o << "  __shared__ vg_u64 s_stage[" << tile_rows << "];\n";
 	if (filter) {
 		o << "  __shared__ unsigned int s_tile; __shared__ vg_u64 s_wtot[4]; __shared__ vg_u64 s_prefix;\n";
 		o << "  if (tid == 0) s_tile = __hip_atomic_fetch_add((unsigned int*)(a.state + a.ntiles), 1u, "
@@ -162,7 +176,7 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 	} else {
 		o << "  const vg_u64 tile = blockIdx.x;\n";
 	}
-	o << "  const vg_u64 tile_base = tile * " << kTile << "ull;\n";
+	o << "  const vg_u64 tile_base = tile * " << tile_rows << "ull;\n";
 	bool has_segment = false;
 	for (auto &c : s.cols) {
 		has_segment = has_segment || c.segment_id;
@@ -181,24 +195,24 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 			throw std::runtime_error("fused kernel: a segment-id column is a non-null INT32");
 		}
 		o << "  const " << t << "* __c" << c << " = (const " << t << "*)a.cols[" << c << "]; (void)__c" << c << ";\n";
-		o << "  " << t << " __v" << c << "[" << kRounds << "][" << kVec << "];\n";
+		o << "  " << t << " __v" << c << "[" << rounds << "][" << kVec << "];\n";
 		if (s.cols[c].has_valid) {
 			o << "  const unsigned char* __vc" << c << " = a.valid[" << c << "];\n";
-			o << "  bool __m" << c << "[" << kRounds << "][" << kVec << "];\n";
+			o << "  bool __m" << c << "[" << rounds << "][" << kVec << "];\n";
 		}
 	}
-	o << "  vg_u64 __i0[" << kRounds << "];\n";
-	o << "  vg_u64 __rowid[" << kRounds << "][" << kVec << "];\n";
-	o << "  unsigned int __keep[" << kRounds << "];\n";
+	o << "  vg_u64 __i0[" << rounds << "];\n";
+	o << "  vg_u64 __rowid[" << rounds << "][" << kVec << "];\n";
+	o << "  unsigned int __keep[" << rounds << "];\n";
 	if (s.cross) {
 		// Fast path: the right side is a multiple of the tile, so a tile is one left row x contiguous right
 		// rows -- one division per block, vector loads on the right, a broadcast on the left.
-		o << "  const bool __aligned_cross = (a.m % " << kTile << "ull) == 0;\n";
+		o << "  const bool __aligned_cross = (a.m % " << tile_rows << "ull) == 0;\n";
 		o << "  const vg_u64 __tli = __aligned_cross ? tile_base / a.m : 0;\n";
 		o << "  const vg_u64 __trj = __aligned_cross ? tile_base - __tli * a.m : 0;\n";
 	}
-	o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) {\n";
-	o << "    const vg_u64 i0 = tile_base + (vg_u64)wave * " << (kVec * kRounds * 64) << "ull + (vg_u64)__r * "
+	o << "#pragma unroll\n  for (int __r = 0; __r < " << rounds << "; __r++) {\n";
+	o << "    const vg_u64 i0 = tile_base + (vg_u64)wave * " << (kVec * rounds * 64) << "ull + (vg_u64)__r * "
 	  << (kVec * 64) << "ull + (vg_u64)lane * " << kVec << "ull;\n";
 	o << "    __i0[__r] = i0;\n";
 	if (has_segment || s.ranges) {
@@ -324,11 +338,11 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 	o << "  }\n";
 
 	// Tile-local output slot of every kept entry (__lpos), and the tile's global base (__gbase).
-	o << "  unsigned int __lpos0[" << kRounds << "];\n";
+	o << "  unsigned int __lpos0[" << rounds << "];\n";
 	if (filter) {
-		o << "  unsigned int __rtot[" << kRounds << "];\n";
+		o << "  unsigned int __rtot[" << rounds << "];\n";
 		o << "  unsigned int __wrun = 0;\n";
-		o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) {\n";
+		o << "#pragma unroll\n  for (int __r = 0; __r < " << rounds << "; __r++) {\n";
 		o << "    unsigned int __cnt = __popc(__keep[__r]);\n";
 		o << "    unsigned int __inc = vg_wave_incl(__cnt);\n";
 		o << "    __lpos0[__r] = __wrun + __inc - __cnt;\n";
@@ -339,7 +353,7 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 		o << "  __syncthreads();\n";
 		o << "  unsigned int __wexcl = 0, __ttot = 0;\n";
 		o << "  for (int w = 0; w < 4; w++) { if (w < wave) __wexcl += (unsigned int)s_wtot[w]; __ttot += (unsigned int)s_wtot[w]; }\n";
-		o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) __lpos0[__r] += __wexcl;\n";
+		o << "#pragma unroll\n  for (int __r = 0; __r < " << rounds << "; __r++) __lpos0[__r] += __wexcl;\n";
 		o << "  if (tid == 0) vg_store(a.state + tile, ((tile == 0 ? 2ull : 1ull) << 62) | (vg_u64)__ttot);\n";
 		o << "  if (wave == 0) {\n";
 		o << "    vg_u64 __pre = tile == 0 ? 0ull : vg_lookback(a.state, (long long)tile);\n";
@@ -352,9 +366,9 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 		o << "  __syncthreads();\n";
 		o << "  const vg_u64 __gbase = s_prefix;\n";
 	} else {
-		o << "#pragma unroll\n  for (int __r = 0; __r < " << kRounds << "; __r++) __lpos0[__r] = (unsigned int)(__i0[__r] - tile_base);\n";
-		o << "  const unsigned int __ttot = (unsigned int)((a.n - tile_base) < " << kTile << "ull ? (a.n - tile_base) : "
-		  << kTile << "ull);\n";
+		o << "#pragma unroll\n  for (int __r = 0; __r < " << rounds << "; __r++) __lpos0[__r] = (unsigned int)(__i0[__r] - tile_base);\n";
+		o << "  const unsigned int __ttot = (unsigned int)((a.n - tile_base) < " << tile_rows << "ull ? (a.n - tile_base) : "
+		  << tile_rows << "ull);\n";
 		o << "  const vg_u64 __gbase = tile_base;\n";
 	}
 	// Writes: every output goes through the LDS stage, then out as one coalesced run of __ttot entries.
@@ -362,7 +376,7 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 	                           const std::string &out_expr, const std::function<void()> &emit_value) {
 		o << "  {\n";
 		o << "    " << stage_type << "* __st = (" << stage_type << "*)s_stage;\n";
-		o << "#pragma unroll\n    for (int __r = 0; __r < " << kRounds << "; __r++) {\n";
+		o << "#pragma unroll\n    for (int __r = 0; __r < " << rounds << "; __r++) {\n";
 		o << "      unsigned int __lp = __lpos0[__r];\n";
 		o << "#pragma unroll\n      for (int __j = 0; __j < " << kVec << "; __j++) {\n";
 		o << "        if ((__keep[__r] >> __j) & 1u) {\n";
@@ -465,7 +479,8 @@ void FusedKernel::Launch(const FusedLaunch &args, void *scratch_state, size_t sc
 	h.m = args.right_rows == 0 ? 1 : args.right_rows;
 	h.sel = args.sel;
 	h.sel_out = args.sel_out;
-	const uint64_t tiles = (args.n + kTile - 1) / kTile;
+	const uint64_t tile_rows = (uint64_t)kBlock * kVec * FusedRounds(spec_);
+	const uint64_t tiles = (args.n + tile_rows - 1) / tile_rows;
 	h.ntiles = tiles;
 	h.cap = args.capacity == 0 ? args.n : args.capacity;
 	h.count = args.count_device;

@@ -235,13 +235,13 @@ __device__ __forceinline__ void StageTransposed(D *dst, const T *src, uint32_t c
 	}
 }
 
-//! Loads `count` (<= 8) consecutive elements of width `width` at row `row0` as raw bits: vector loads when
-//! all 8 are there and the address is 16-byte aligned, element loads otherwise.
-template <typename T>
-__device__ __forceinline__ void LoadRun8(const T *src, uint64_t row0, uint64_t n, u64 (&out)[kRedItems]) {
-	constexpr int kVecs = (int)(kRedItems * sizeof(T) / 16);
-	if (kVecs > 0 && row0 + kRedItems <= n && (reinterpret_cast<uintptr_t>(src + row0) & 15) == 0) {
-		T t[kRedItems];
+//! Loads ITEMS consecutive elements of type T at row `row0` as raw bits: 16-byte vector loads when all of
+//! them are there and the address allows, element loads otherwise.
+template <int ITEMS, typename T>
+__device__ __forceinline__ void LoadRun(const T *src, uint64_t row0, uint64_t n, u64 (&out)[ITEMS]) {
+	constexpr int kVecs = (int)(ITEMS * sizeof(T) / 16);
+	if (kVecs > 0 && row0 + ITEMS <= n && (reinterpret_cast<uintptr_t>(src + row0) & 15) == 0) {
+		T t[ITEMS];
 		uint4 v[kVecs > 0 ? kVecs : 1];
 #pragma unroll
 		for (int j = 0; j < kVecs; j++) {
@@ -249,15 +249,24 @@ __device__ __forceinline__ void LoadRun8(const T *src, uint64_t row0, uint64_t n
 		}
 		__builtin_memcpy(t, v, sizeof(t));
 #pragma unroll
-		for (int i = 0; i < kRedItems; i++) {
+		for (int i = 0; i < ITEMS; i++) {
 			out[i] = (u64)t[i];
 		}
 	} else {
 #pragma unroll
-		for (int i = 0; i < kRedItems; i++) {
+		for (int i = 0; i < ITEMS; i++) {
 			out[i] = row0 + i < n ? (u64)src[row0 + i] : 0ull;
 		}
 	}
+}
+
+//! Rows per thread of the reduce: 8 in the generic kernel (LDS staging), kRedItemsDirect in the
+//! specialized ones (register loads, so the cross-thread scan is paid once per more rows).
+constexpr int kRedItemsDirect = 8;
+
+template <typename SG>
+constexpr int RedItems() {
+	return SG::kStatic ? kRedItemsDirect : kRedItems;
 }
 
 //! Reduce tile: 256 threads x 8 consecutive rows, each thread folding its rows serially. The generic kernel
@@ -275,6 +284,8 @@ __global__ void __launch_bounds__(kRedBlock)
 	__shared__ u64 s_wave_state[kRedWaves][2 * kMaxComponents];
 	__shared__ int s_wave_flag[kRedWaves];
 	constexpr bool DIRECT = SG::kStatic;
+	constexpr int ITEMS = RedItems<SG>();
+	constexpr int TILE = kRedBlock * ITEMS;
 	constexpr int kStage = DIRECT ? 1 : kRedItems * kSlotStride;
 	__shared__ KeyT s_keys[kStage];
 	__shared__ KeyT s_prev;
@@ -288,17 +299,17 @@ __global__ void __launch_bounds__(kRedBlock)
 	}
 	__syncthreads();
 	const uint32_t tile = s_tile;
-	const uint64_t tile_base = (uint64_t)tile * kRedTile;
-	const uint32_t tile_count = (uint32_t)((n - tile_base) < (uint64_t)kRedTile ? (n - tile_base) : kRedTile);
+	const uint64_t tile_base = (uint64_t)tile * TILE;
+	const uint32_t tile_count = (uint32_t)((n - tile_base) < (uint64_t)TILE ? (n - tile_base) : TILE);
 
 	// DIRECT: this thread's 8 rows in registers (raw bits) and the key of the row before them.
-	u64 kreg[kRedItems];
-	u64 preg[NP > 0 ? NP : 1][kRedItems];
+	u64 kreg[ITEMS];
+	u64 preg[NP > 0 ? NP : 1][ITEMS];
 	KeyT kprev = (KeyT)0;
 	if constexpr (DIRECT) {
-		const uint64_t row0 = tile_base + (uint64_t)tid * kRedItems;
+		const uint64_t row0 = tile_base + (uint64_t)tid * ITEMS;
 		if (keys != nullptr) {
-			LoadRun8<KeyT>(keys, row0, n, kreg);
+			LoadRun<ITEMS, KeyT>(keys, row0, n, kreg);
 			kprev = row0 > 0 && row0 <= n ? keys[row0 - 1] : (KeyT)0;
 		}
 #pragma unroll
@@ -306,16 +317,16 @@ __global__ void __launch_bounds__(kRedBlock)
 			if (p < pay.n) {
 				switch (pay.width[p]) {
 				case 8:
-					LoadRun8<u64>(reinterpret_cast<const u64 *>(pay.ptr[p]), row0, n, preg[p]);
+					LoadRun<ITEMS, u64>(reinterpret_cast<const u64 *>(pay.ptr[p]), row0, n, preg[p]);
 					break;
 				case 4:
-					LoadRun8<uint32_t>(reinterpret_cast<const uint32_t *>(pay.ptr[p]), row0, n, preg[p]);
+					LoadRun<ITEMS, uint32_t>(reinterpret_cast<const uint32_t *>(pay.ptr[p]), row0, n, preg[p]);
 					break;
 				case 2:
-					LoadRun8<uint16_t>(reinterpret_cast<const uint16_t *>(pay.ptr[p]), row0, n, preg[p]);
+					LoadRun<ITEMS, uint16_t>(reinterpret_cast<const uint16_t *>(pay.ptr[p]), row0, n, preg[p]);
 					break;
 				default:
-					LoadRun8<uint8_t>(reinterpret_cast<const uint8_t *>(pay.ptr[p]), row0, n, preg[p]);
+					LoadRun<ITEMS, uint8_t>(reinterpret_cast<const uint8_t *>(pay.ptr[p]), row0, n, preg[p]);
 					break;
 				}
 			}
@@ -356,16 +367,16 @@ __global__ void __launch_bounds__(kRedBlock)
 	}
 	__syncthreads();
 
-	const uint32_t local0 = (uint32_t)tid * kRedItems;
+	const uint32_t local0 = (uint32_t)tid * ITEMS;
 	const uint64_t base = tile_base + local0;
-	KeyT k[kRedItems];
-	bool head[kRedItems];
+	KeyT k[ITEMS];
+	bool head[ITEMS];
 	u64 hc = 0;
 	KeyT before = keys == nullptr ? (KeyT)0
 	                              : (DIRECT ? kprev
 	                                        : (tid == 0 ? s_prev : s_keys[(kRedItems - 1) * kSlotStride + tid - 1]));
 #pragma unroll
-	for (int i = 0; i < kRedItems; i++) {
+	for (int i = 0; i < ITEMS; i++) {
 		uint32_t lr = local0 + i;
 		if constexpr (DIRECT) {
 			k[i] = (keys != nullptr && lr < tile_count) ? (KeyT)kreg[i] : (KeyT)0;
@@ -404,7 +415,7 @@ __global__ void __launch_bounds__(kRedBlock)
 	u64 gid = hb_global + hb_excl;
 	u64 cur_gid = 0;
 #pragma unroll
-	for (int i = 0; i < kRedItems; i++) {
+	for (int i = 0; i < ITEMS; i++) {
 		uint32_t lr = local0 + i;
 		if (lr < tile_count) {
 			u64 vals[NP > 0 ? NP : 1];
@@ -487,13 +498,13 @@ __global__ void __launch_bounds__(kRedBlock)
 		}
 	}
 	if (tid == kRedBlock - 1) {
-		uint64_t tile_end = tile_base + kRedTile;
+		uint64_t tile_end = tile_base + TILE;
 		if (H > 0) {
 			u64 gid_last = hb_global + H - 1;
 			bool ends = tile_end >= n;
 			if (!ends && keys != nullptr) {
 				// keys == nullptr: one global group, which only ends with the input
-				ends = keys[tile_end] != (DIRECT ? (KeyT)kreg[kRedItems - 1]
+				ends = keys[tile_end] != (DIRECT ? (KeyT)kreg[ITEMS - 1]
 				                                  : s_keys[(kRedItems - 1) * kSlotStride + kRedBlock - 1]);
 			}
 			if (ends) {
@@ -720,7 +731,8 @@ template <typename KeyT, int NP, typename SG = DynSig>
 void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, const CompDesc &cd,
                   const AggOutDesc &od, const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch,
                   hipStream_t stream) {
-	const uint64_t ntiles = (n + kRedTile - 1) / kRedTile;
+	constexpr uint64_t kTile = (uint64_t)kRedBlock * RedItems<SG>();
+	const uint64_t ntiles = (n + kTile - 1) / kTile;
 	const size_t state_words = 2 * (size_t)std::max(cd.ncomp, 1);
 	auto *lb = static_cast<u64 *>(scratch.Alloc((ntiles + 1) * sizeof(u64)));
 	auto *ticket = reinterpret_cast<uint32_t *>(lb + ntiles);
