@@ -171,8 +171,19 @@ int Server::AddTemplate(std::unique_ptr<ServedTemplate> t) {
 			throw std::runtime_error("server template: probe and build keys must have the same type");
 		}
 		if (options_.batching) {
-			// The resident intermediate: the build side reduced once, probed by every batch.
-			t->index = BuildJoinIndex(t->build_key, t->build_payload, streams_[0]);
+			// The resident intermediate: the build side reduced once, probed by every batch. Built on a stream
+			// of its own so it never interleaves with an executor's batch.
+			hipStream_t prep = nullptr;
+			VGPU_HIP(hipStreamCreateWithFlags(&prep, hipStreamNonBlocking));
+			try {
+				t->index = BuildJoinIndex(t->build_key, t->build_payload, prep);
+			} catch (...) {
+				DevicePool::Instance().ForgetStream(prep);
+				(void)hipStreamDestroy(prep);
+				throw;
+			}
+			DevicePool::Instance().ForgetStream(prep);
+			(void)hipStreamDestroy(prep);
 		}
 		break;
 	default:
@@ -274,6 +285,7 @@ void Server::Executor(int slot) {
 	for (;;) {
 		std::vector<Pending> batch;
 		uint64_t reserved = 0;
+		const ServedTemplate *tp = nullptr;
 		{
 			std::unique_lock<std::mutex> lock(mutex_);
 			auto head_fits = [&] {
@@ -290,6 +302,7 @@ void Server::Executor(int slot) {
 			// The oldest query, then every later query of its template that still fits.
 			const int tmpl = queue_.front().tmpl;
 			const auto &t = *templates_[tmpl];
+			tp = &t; // templates are never removed while the server lives; the vector may grow meanwhile
 			const uint64_t avail = options_.memory_budget - reserved_;
 			// a GROUP BY batch sorts fewer than 2^30 rows
 			uint64_t max_rows = t.kind == TMPL_GROUP_BY ? (1ull << 30) - (1ull << 20) : ~0ull;
@@ -325,12 +338,12 @@ void Server::Executor(int slot) {
 			r.reset(new ServedResult());
 		}
 		try {
-			if (templates_[batch[0].tmpl]->kind == TMPL_CROSS) {
-				RunCross(batch[0], stream, *results[0]);
+			if (tp->kind == TMPL_CROSS) {
+				RunCross(*tp, batch[0], stream, *results[0]);
 			} else if (options_.batching) {
-				RunBatch(batch, stream, results);
+				RunBatch(*tp, batch, stream, results);
 			} else {
-				RunAlone(batch[0], stream, *results[0]);
+				RunAlone(*tp, batch[0], stream, *results[0]);
 			}
 		} catch (const std::exception &e) {
 			for (auto &r : results) {
@@ -345,7 +358,7 @@ void Server::Executor(int slot) {
 			reserved_ -= reserved;
 			for (size_t i = 0; i < batch.size(); i++) {
 				auto &r = results[i];
-				r->kind = templates_[batch[i].tmpl]->kind;
+				r->kind = tp->kind;
 				r->batch_size = (uint32_t)batch.size();
 				r->submit_ns = batch[i].submit_ns;
 				r->start_ns = start;
@@ -358,9 +371,8 @@ void Server::Executor(int slot) {
 	}
 }
 
-void Server::RunBatch(const std::vector<Pending> &batch, hipStream_t stream,
+void Server::RunBatch(const ServedTemplate &t, const std::vector<Pending> &batch, hipStream_t stream,
                       std::vector<std::unique_ptr<ServedResult>> &results) {
-	const auto &t = *templates_[batch[0].tmpl];
 	const uint32_t nq = (uint32_t)batch.size();
 	std::vector<uint64_t> starts(nq), lens(nq);
 	for (uint32_t g = 0; g < nq; g++) {
@@ -454,8 +466,7 @@ void Server::RunBatch(const std::vector<Pending> &batch, hipStream_t stream,
 	}
 }
 
-void Server::RunAlone(const Pending &q, hipStream_t stream, ServedResult &r) {
-	const auto &t = *templates_[q.tmpl];
+void Server::RunAlone(const ServedTemplate &t, const Pending &q, hipStream_t stream, ServedResult &r) {
 	PoolArena arena(stream);
 	auto hold = std::make_shared<BatchBuffers>();
 	auto &pool = DevicePool::Instance();
@@ -560,9 +571,8 @@ void Server::RunAlone(const Pending &q, hipStream_t stream, ServedResult &r) {
 	r.null = {(uint8_t)(pairs == 0), 0};
 }
 
-void Server::RunCross(const Pending &q, hipStream_t stream, ServedResult &r) {
+void Server::RunCross(const ServedTemplate &t, const Pending &q, hipStream_t stream, ServedResult &r) {
 	// The fused cross kernel over the left side x rows [start, start + count) of the right side.
-	const auto &t = *templates_[q.tmpl];
 	PoolArena arena(stream);
 	auto hold = std::make_shared<BatchBuffers>();
 	auto &pool = DevicePool::Instance();

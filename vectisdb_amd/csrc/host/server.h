@@ -111,10 +111,10 @@ private:
 		uint64_t submit_ns;
 	};
 	void Executor(int slot);
-	void RunBatch(const std::vector<Pending> &batch, hipStream_t stream,
+	void RunBatch(const ServedTemplate &t, const std::vector<Pending> &batch, hipStream_t stream,
 	              std::vector<std::unique_ptr<ServedResult>> &results);
-	void RunAlone(const Pending &q, hipStream_t stream, ServedResult &r);
-	void RunCross(const Pending &q, hipStream_t stream, ServedResult &r);
+	void RunAlone(const ServedTemplate &t, const Pending &q, hipStream_t stream, ServedResult &r);
+	void RunCross(const ServedTemplate &t, const Pending &q, hipStream_t stream, ServedResult &r);
 
 	ServerOptions options_;
 	int device_ = 0;
