@@ -82,8 +82,8 @@ __global__ void __launch_bounds__(256) KeyRangeKernel(RangeCols cols, uint64_t n
 constexpr int kHistUnroll = 8;
 
 //! Min/max of the ordered key and the histograms of all 8 byte digits in one read. A digit that is the
-//! same on every active lane (the constant high bytes of a narrow key range, or a Zipf head value) is
-//! counted with one LDS atomic per wave instead of 64 conflicting ones.
+//! same for every key a wave reads in one step (the constant high bytes of a narrow key range) is counted
+//! with one LDS atomic instead of one per key.
 template <typename T>
 __global__ void __launch_bounds__(256) KeyRangeHist8Kernel(const T *col, int width, int is_signed, uint64_t n,
                                                            u64 *mins, u64 *maxs, uint32_t *hist8) {
@@ -104,34 +104,54 @@ __global__ void __launch_bounds__(256) KeyRangeHist8Kernel(const T *col, int wid
 			uint64_t row = row0 + u * stride;
 			raw[u] = row < n ? (u64)col[row] : 0ull;
 		}
+		// Digits that are the same for every key of this step (the constant high bytes of a narrow key
+		// range) are found once per step: each lane ORs its keys' differences from one reference key, and a
+		// wave OR tells which bytes vary anywhere. A constant byte is counted with one atomic for all the
+		// step's keys; only the varying bytes are counted key by key.
+		u64 ord[kHistUnroll];
+		uint32_t nvalid = 0;
 #pragma unroll
 		for (int u = 0; u < kHistUnroll; u++) {
-			uint64_t row = row0 + u * stride;
-			const bool valid = row < n;
-			const u64 vmask = __ballot(valid);
-			if (vmask == 0) {
-				continue;
-			}
-			const u64 ord = OrderedKey(raw[u], width, is_signed != 0);
-			if (valid) {
-				lo = ord < lo ? ord : lo;
-				hi = ord > hi ? ord : hi;
-			}
-			const int leader = __ffsll((long long)vmask) - 1;
+			ord[u] = OrderedKey(raw[u], width, is_signed != 0);
+			nvalid += (uint32_t)Popc64(__ballot(row0 + u * stride < n));
+		}
+		const u64 vmask0 = __ballot(row0 < n); // item 0 has the lowest rows: empty only if the step is empty
+		if (vmask0 == 0) {
+			continue;
+		}
+		const int leader = __ffsll((long long)vmask0) - 1;
+		const u64 ref = ((u64)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(ord[0] >> 32), leader) << 32) |
+		                (u64)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)ord[0], leader);
+		u64 diff = 0;
 #pragma unroll
-			for (int d = 0; d < 8; d++) {
-				uint32_t dg = (uint32_t)(ord >> (8 * d)) & 255u;
-				// every lane is active here, so readfirstlane reads lane 0; if lane 0 is past the end the
-				// uniform test below simply fails and the lanes count themselves
-				uint32_t ref = (uint32_t)__builtin_amdgcn_readfirstlane((int)dg);
-				(void)leader;
-				u64 same = __ballot(valid && dg == ref);
-				if (same == vmask) {
-					if (lane == leader) {
-						atomicAdd(&s_hist[d * 256 + ref], (uint32_t)Popc64(vmask));
+		for (int u = 0; u < kHistUnroll; u++) {
+			if (row0 + u * stride < n) {
+				diff |= ord[u] ^ ref;
+				lo = ord[u] < lo ? ord[u] : lo;
+				hi = ord[u] > hi ? ord[u] : hi;
+			}
+		}
+		uint32_t bytes = 0; // bit d: byte d differs from the reference in some key of this lane
+#pragma unroll
+		for (int d = 0; d < 8; d++) {
+			bytes |= ((diff >> (8 * d)) & 255u) != 0 ? (1u << d) : 0u;
+		}
+		for (int off = 32; off >= 1; off >>= 1) {
+			bytes |= (uint32_t)__shfl_xor((int)bytes, off);
+		}
+		const uint32_t vary = (uint32_t)__builtin_amdgcn_readfirstlane((int)bytes);
+#pragma unroll
+		for (int d = 0; d < 8; d++) {
+			if (((vary >> d) & 1u) == 0) {
+				if (lane == leader) {
+					atomicAdd(&s_hist[d * 256 + ((uint32_t)(ref >> (8 * d)) & 255u)], nvalid);
+				}
+			} else {
+#pragma unroll
+				for (int u = 0; u < kHistUnroll; u++) {
+					if (row0 + u * stride < n) {
+						atomicAdd(&s_hist[d * 256 + ((uint32_t)(ord[u] >> (8 * d)) & 255u)], 1u);
 					}
-				} else if (valid) {
-					atomicAdd(&s_hist[d * 256 + dg], 1u);
 				}
 			}
 		}
