@@ -246,16 +246,17 @@ __global__ void __launch_bounds__(kAggBlock) JoinAggKernel(JoinAggArgs args, u64
 	}
 }
 
-__global__ void JoinAggFinalKernel(JoinAggArgs args, const u64 *partials, uint32_t nparts, u64 *result) {
-	if (threadIdx.x != 0 || blockIdx.x != 0) {
-		return;
-	}
+//! One wave folds the per-block partials: lane l merges partials l, l + 64, ... in order, then the lanes
+//! merge in lane order (a fixed tree, so the double-double sums round the same way on every run).
+__global__ void __launch_bounds__(kWave) JoinAggFinalKernel(JoinAggArgs args, const u64 *partials, uint32_t nparts,
+                                                            u64 *result) {
+	const int lane = LaneId();
 	u64 tot[kJoinAggMax * 2];
 #pragma unroll
 	for (int q = 0; q < kJoinAggMax; q++) {
 		JoinAggIdentity(args, q, tot[2 * q], tot[2 * q + 1]);
 	}
-	for (uint32_t p = 0; p < nparts; p++) {
+	for (uint32_t p = lane; p < nparts; p += kWave) {
 		u64 t[kJoinAggMax * 2];
 #pragma unroll
 		for (int s = 0; s < kJoinAggMax * 2; s++) {
@@ -263,9 +264,27 @@ __global__ void JoinAggFinalKernel(JoinAggArgs args, const u64 *partials, uint32
 		}
 		JoinAggMerge(args, tot, t);
 	}
+	for (int off = 1; off < kWave; off <<= 1) {
+		u64 other[kJoinAggMax * 2];
 #pragma unroll
-	for (int s = 0; s < kJoinAggMax * 2; s++) {
-		result[s] = tot[s];
+		for (int s = 0; s < kJoinAggMax * 2; s++) {
+			other[s] = __shfl_xor(tot[s], off);
+		}
+		if ((lane & off) == 0) {
+			JoinAggMerge(args, tot, other);
+		} else {
+			JoinAggMerge(args, other, tot);
+#pragma unroll
+			for (int s = 0; s < kJoinAggMax * 2; s++) {
+				tot[s] = other[s];
+			}
+		}
+	}
+	if (lane == 0) {
+#pragma unroll
+		for (int s = 0; s < kJoinAggMax * 2; s++) {
+			result[s] = tot[s];
+		}
 	}
 }
 
@@ -334,7 +353,7 @@ void JoinAggregate(const JoinAggArgs &args, uint64_t *result_device, Scratch &sc
 		hipLaunchKernelGGL(JoinAggKernel, dim3(grid), dim3(kAggBlock), 0, stream, args, partials);
 		VGPU_HIP(hipGetLastError());
 	}
-	hipLaunchKernelGGL(JoinAggFinalKernel, dim3(1), dim3(64), 0, stream, args, partials, grid, (u64 *)result_device);
+	hipLaunchKernelGGL(JoinAggFinalKernel, dim3(1), dim3(kWave), 0, stream, args, partials, grid, (u64 *)result_device);
 	VGPU_HIP(hipGetLastError());
 }
 
