@@ -75,6 +75,7 @@ struct VArgs {
   vg_u64 nseg;
   const vg_u64* seg_row;
   const vg_u64* seg_len;
+  vg_u64* range;
 };
 __device__ __forceinline__ vg_u64 vg_load(const vg_u64* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -200,6 +201,19 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 			o << "  const unsigned char* __vc" << c << " = a.valid[" << c << "];\n";
 			o << "  bool __m" << c << "[" << rounds << "][" << kVec << "];\n";
 		}
+	}
+	const int nrange = (int)s.range_slots.size();
+	if (nrange > 4) {
+		throw std::runtime_error("fused kernel: at most 4 range columns");
+	}
+	for (int slot : s.range_slots) {
+		if (slot < 0 || slot >= (int)s.cols.size() || !TypeIsInteger(s.cols[slot].vtype) || s.cols[slot].has_valid) {
+			throw std::runtime_error("fused kernel: a range column is a non-null integer column slot");
+		}
+	}
+	if (nrange > 0) {
+		o << "  vg_u64 __rlo[" << nrange << "], __rnhi[" << nrange << "];\n";
+		o << "#pragma unroll\n  for (int __q = 0; __q < " << nrange << "; __q++) { __rlo[__q] = ~0ull; __rnhi[__q] = ~0ull; }\n";
 	}
 	o << "  vg_u64 __i0[" << rounds << "];\n";
 	o << "  vg_u64 __rowid[" << rounds << "][" << kVec << "];\n";
@@ -333,6 +347,15 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 		o << "      __ok = __ok && __p" << p << (v.empty() ? "" : " && (" + v + ")") << ";\n";
 	}
 	o << "      if (__ok) __k |= 1u << __j;\n";
+	for (int q = 0; q < nrange; q++) {
+		const int slot = s.range_slots[q];
+		const std::string v = "__v" + std::to_string(slot) + "[__r][__j]";
+		const std::string ord = TypeIsSigned(s.cols[slot].vtype) ? "((vg_u64)(long long)" + v + " ^ (1ull << 63))"
+		                                                           : "(vg_u64)" + v;
+		o << "      if (__ok) { const vg_u64 __o = " << ord << "; __rlo[" << q << "] = __o < __rlo[" << q
+		  << "] ? __o : __rlo[" << q << "]; __rnhi[" << q << "] = ~__o < __rnhi[" << q << "] ? ~__o : __rnhi[" << q
+		  << "]; }\n";
+	}
 	o << "    }\n";
 	o << "    __keep[__r] = __k;\n";
 	o << "  }\n";
@@ -425,6 +448,20 @@ std::string GenerateFusedSource(const FusedSpec &s) {
 			o << "  }\n";
 		}
 	}
+	// Range columns: wave minimum of {min, ~max}, then one atomic per wave, skipped when it cannot lower the
+	// stored word.
+	for (int q = 0; q < nrange; q++) {
+		o << "  {\n";
+		o << "    vg_u64 __a = __rlo[" << q << "], __b = __rnhi[" << q << "];\n";
+		o << "    for (int __s = 32; __s >= 1; __s >>= 1) { vg_u64 __x = __shfl_xor(__a, __s); vg_u64 __y = __shfl_xor(__b, __s); __a = __x < __a ? __x : __a; __b = __y < __b ? __y : __b; }\n";
+		o << "    if (lane == 0) {\n";
+		o << "      if (__a < vg_load(a.range + " << 2 * q << ")) __hip_atomic_fetch_min(a.range + " << 2 * q
+		  << ", __a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
+		o << "      if (__b < vg_load(a.range + " << 2 * q + 1 << ")) __hip_atomic_fetch_min(a.range + " << 2 * q + 1
+		  << ", __b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
+		o << "    }\n";
+		o << "  }\n";
+	}
 	o << "}\n";
 	return o.str();
 }
@@ -456,10 +493,17 @@ struct VArgsHost {
 	uint64_t nseg;
 	const uint64_t *seg_row;
 	const uint64_t *seg_len;
+	uint64_t *range;
 };
 } // namespace
 
 void FusedKernel::Launch(const FusedLaunch &args, void *scratch_state, size_t scratch_bytes, hipStream_t stream) const {
+	if (!spec_.range_slots.empty()) {
+		if (args.range_device == nullptr) {
+			throw std::runtime_error("fused kernel: range columns need a device range buffer");
+		}
+		FillU64(args.range_device, ~0ull, 2 * spec_.range_slots.size(), stream);
+	}
 	if (args.n == 0) {
 		if (!spec_.predicates.empty() && args.count_device != nullptr) {
 			FillU64(args.count_device, 0ull, 1, stream);
@@ -488,6 +532,7 @@ void FusedKernel::Launch(const FusedLaunch &args, void *scratch_state, size_t sc
 	h.nseg = args.nseg;
 	h.seg_row = args.seg_rows;
 	h.seg_len = args.seg_lens;
+	h.range = args.range_device;
 	for (auto &c : spec_.cols) {
 		if (c.segment_id && (args.seg_offsets == nullptr || args.nseg == 0)) {
 			throw std::runtime_error("fused kernel: a segment-id column needs segment offsets");

@@ -50,6 +50,10 @@ struct FusedSpec {
 	//! padding (never kept). Needs at least one predicate (the outputs are compacted).
 	bool ranges = false;
 	bool fast_math = false;
+	//! Non-null integer columns (a segment-id column included) whose minimum and maximum over the kept rows
+	//! the launch writes to `range_device`, as order-preserving unsigned keys (signed values sign-extended
+	//! to 64 bits with the top bit flipped).
+	std::vector<int> range_slots;
 };
 
 struct FusedLaunch {
@@ -67,6 +71,9 @@ struct FusedLaunch {
 	uint64_t nseg = 0;
 	const uint64_t *seg_rows = nullptr;       // ranges mode: device, nseg table rows of each segment's start
 	const uint64_t *seg_lens = nullptr;       // ranges mode: device, nseg segment lengths
+	//! range_slots: device, 2 words per slot, written as {min, ~max} (both kept by atomic minimum; the
+	//! launch sets them to ~0 first, so a launch that keeps no row leaves {~0, ~0}).
+	uint64_t *range_device = nullptr;
 };
 
 class FusedKernel {

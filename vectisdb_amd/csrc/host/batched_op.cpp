@@ -137,6 +137,8 @@ uint64_t BatchedFilterGroupByDevice(const BatchedGroupByArgs &args, Scratch &scr
 	}
 	spec.projections.push_back(Identity(seg_slot, VT_INT32));
 	spec.projections.push_back(Identity(slot(args.key_col), cols[args.key_col].type));
+	// The composite key's ranges come out of the same pass (no separate read of the compacted keys).
+	spec.range_slots = {seg_slot, slot(args.key_col)};
 	for (int v : args.value_cols) {
 		if (v < 0 || v >= (int)cols.size()) {
 			throw std::runtime_error("batched group by: value column out of range");
@@ -159,9 +161,11 @@ uint64_t BatchedFilterGroupByDevice(const BatchedGroupByArgs &args, Scratch &scr
 	for (int v : args.value_cols) {
 		outs.push_back(scratch.Alloc(std::max<uint64_t>(n, 1) * TypeWidth(cols[v].type)));
 	}
-	auto *count = static_cast<uint64_t *>(scratch.Alloc(8));
+	// count, then {min, ~max} of the segment id and of the key over the kept rows
+	auto *count = static_cast<uint64_t *>(scratch.Alloc(5 * 8));
 	uint64_t kept = n;
 	FusedLaunch la;
+	la.range_device = count + 1;
 	la.cols = ptrs.data();
 	la.outs = outs.data();
 	la.n = n;
@@ -177,12 +181,15 @@ uint64_t BatchedFilterGroupByDevice(const BatchedGroupByArgs &args, Scratch &scr
 		state = scratch.Alloc(state_bytes);
 	}
 	kernel->Launch(la, state, state_bytes, stream);
+	uint64_t words[5] = {};
+	if (spec.predicates.empty()) {
+		VGPU_HIP(hipMemcpyAsync(words + 1, count + 1, 4 * 8, hipMemcpyDeviceToHost, stream));
+	} else {
+		VGPU_HIP(hipMemcpyAsync(words, count, 5 * 8, hipMemcpyDeviceToHost, stream));
+	}
+	VGPU_HIP(hipStreamSynchronize(stream));
 	if (!spec.predicates.empty()) {
-		VGPU_HIP(hipMemcpyAsync(&kept, count, 8, hipMemcpyDeviceToHost, stream));
-		VGPU_HIP(hipStreamSynchronize(stream));
-		if (n == 0) {
-			kept = 0;
-		}
+		kept = n == 0 ? 0 : words[0];
 	}
 
 	if (kept == 0) { // nothing survived: every query has zero groups
@@ -207,7 +214,8 @@ uint64_t BatchedFilterGroupByDevice(const BatchedGroupByArgs &args, Scratch &scr
 		values.push_back(d);
 	}
 	KeyPack pack;
-	const uint64_t bound = PlanGroupBy(keys, kept, pack, scratch, stream);
+	const uint64_t mins[2] = {words[1], words[3]}, maxs[2] = {~words[2], ~words[4]};
+	const uint64_t bound = PlanGroupByRanges(keys, kept, mins, maxs, pack);
 	if (bound > args.capacity) {
 		throw std::runtime_error("batched group by: output capacity " + std::to_string(args.capacity) +
 		                         " is below the group bound " + std::to_string(bound));

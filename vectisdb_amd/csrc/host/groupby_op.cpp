@@ -116,6 +116,29 @@ uint64_t PlanGroupBy(const std::vector<DevCol> &keys, uint64_t rows, KeyPack &pa
 		return 0;
 	}
 	KeyRanges(cols.data(), types.data(), (int)keys.size(), rows, mins, maxs, scratch, stream);
+	return PlanGroupByRanges(keys, rows, mins, maxs, pack);
+}
+
+uint64_t PlanGroupByRanges(const std::vector<DevCol> &keys, uint64_t rows, const uint64_t *mins,
+                           const uint64_t *maxs, KeyPack &pack) {
+	pack = KeyPack {};
+	if (keys.empty() || keys.size() > (size_t)kMaxKeyCols) {
+		throw std::runtime_error("gpu_groupby: 1 to 4 group key columns are supported");
+	}
+	std::vector<const void *> cols;
+	std::vector<int> types;
+	for (auto &k : keys) {
+		if (!TypeIsInteger(k.type)) {
+			throw std::runtime_error("gpu_groupby: group key is not an integer/boolean column, which this GPU "
+			                         "path requires");
+		}
+		cols.push_back(k.data);
+		types.push_back(k.type);
+	}
+	if (rows == 0) {
+		PlanKeyPack(cols.data(), types.data(), (int)keys.size(), mins, mins, pack);
+		return 0;
+	}
 	if (!PlanKeyPack(cols.data(), types.data(), (int)keys.size(), mins, maxs, pack)) {
 		throw std::runtime_error("gpu_groupby: the group keys' value ranges need more than 64 bits once packed");
 	}

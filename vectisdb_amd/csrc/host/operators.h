@@ -40,6 +40,10 @@ void ValidateAggregateReq(const AggReq &agg, const std::vector<DevCol> &values);
 //! which the sort then reuses instead of reading the keys again.
 uint64_t PlanGroupBy(const std::vector<DevCol> &keys, uint64_t rows, KeyPack &pack, Scratch &scratch,
                      hipStream_t stream, const uint32_t **hist8 = nullptr);
+//! The same plan from key ranges already known (order-preserving min/max per key column, as the key-range
+//! pass returns them); reads nothing from the device.
+uint64_t PlanGroupByRanges(const std::vector<DevCol> &keys, uint64_t rows, const uint64_t *mins,
+                           const uint64_t *maxs, KeyPack &pack);
 
 //! Sort-based GROUP BY. Key outputs (one per key column, key type) and aggregate outputs (out_type) are
 //! allocated through `out_alloc` with `capacity` rows each. Returns the group count when `read_groups`
