@@ -184,6 +184,11 @@ int vgpu_generate_column(void *out, int32_t type, uint64_t n, int32_t dist, int6
                          uint64_t seed, uint64_t stream_id, const double *cdf, uint64_t cdf_len,
                          const int64_t *perm, void *stream);
 
+/* dst[0, bytes) = src[0, bytes) on the device (a block-chunked copy kernel; hipMemcpyAsync when a pointer is
+ * not 16-byte aligned). The engine's own device-to-device copies use it; the bench times it as the
+ * achievable stream-copy bandwidth. No reference counterpart. */
+int vgpu_device_copy(void *dst, const void *src, uint64_t bytes, void *stream);
+
 /* ---------------------------------------------------------------- plans
  * A plan is a GpuPlanNode tree (cuda_engine/include/gpu_engine.hpp:135-185) written as JSON:
  *   {"op": "SCAN"|"FILTER"|"PROJECTION"|"HASH_JOIN"|"GROUP_BY_AGGREGATE"|"CROSS_PRODUCT",

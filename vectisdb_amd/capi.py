@@ -111,6 +111,7 @@ _PROTOS = {
     "vgpu_gather": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p]),
     "vgpu_generate_column": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_int32, C.c_int64, C.c_int64, C.c_uint64,
                                        C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "vgpu_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "vgpu_filter_project": (C.c_int, [C.POINTER(Column), C.c_void_p, C.c_int32, C.POINTER(Expr), C.c_int32,
                                       C.POINTER(Expr), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.POINTER(C.c_uint64), C.c_void_p]),
@@ -293,6 +294,14 @@ def generate_column(out, vtype, dist, lo=0, hi=0, seed=0, stream_id=0, cdf=None,
                                    C.c_void_p(cdf.data_ptr() if cdf is not None else 0),
                                    cdf.numel() if cdf is not None else 0,
                                    C.c_void_p(perm.data_ptr() if perm is not None else 0), _stream_ptr(stream)))
+
+
+def device_copy(dst, src, nbytes=None, stream=None):
+    """dst[:nbytes] = src[:nbytes] (device tensors, bytes) through the library's copy kernel."""
+    n = nbytes if nbytes is not None else src.numel() * src.element_size()
+    if n > dst.numel() * dst.element_size() or n > src.numel() * src.element_size():
+        raise ValueError("device_copy: byte count exceeds a tensor")
+    check(load().vgpu_device_copy(C.c_void_p(dst.data_ptr()), C.c_void_p(src.data_ptr()), n, _stream_ptr(stream)))
 
 
 def batched_filter_group_by(cols, segment_offsets, preds, key_col, value_cols, aggs, capacity=None, stream=None,

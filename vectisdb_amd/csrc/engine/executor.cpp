@@ -405,11 +405,10 @@ Columns ExecuteScan(const GpuPlanNode &node, ExecContext &ctx, const NameSet *ne
 				}
 				uint64_t offset = 0;
 				for (auto &cached : chunks) {
-					VGPU_HIP(hipMemcpyAsync(static_cast<uint8_t *>(column.data) + offset * width, cached.device_data,
-					                        cached.row_count * width, hipMemcpyDeviceToDevice, ctx.stream));
+					vgpu::CopyDevice(static_cast<uint8_t *>(column.data) + offset * width, cached.device_data,
+					                        cached.row_count * width, ctx.stream);
 					if (any_validity && cached.device_validity != nullptr) {
-						VGPU_HIP(hipMemcpyAsync(column.valid + offset, cached.device_validity, cached.row_count,
-						                        hipMemcpyDeviceToDevice, ctx.stream));
+						vgpu::CopyDevice(column.valid + offset, cached.device_validity, cached.row_count, ctx.stream);
 					}
 					offset += cached.row_count;
 				}
@@ -721,14 +720,14 @@ Columns RunFused(const Columns &child, uint64_t rows, const std::vector<GpuExpr>
 		if (sel != nullptr) {
 			vgpu::Gather(src->data, width, sel, false, kept, result[e].data, ctx.stream);
 		} else {
-			VGPU_HIP(hipMemcpyAsync(result[e].data, src->data, kept * width, hipMemcpyDeviceToDevice, ctx.stream));
+			vgpu::CopyDevice(result[e].data, src->data, kept * width, ctx.stream);
 		}
 		if (src->valid != nullptr) {
 			result[e].valid = static_cast<uint8_t *>(ctx.Alloc(kept));
 			if (sel != nullptr) {
 				vgpu::Gather(src->valid, 1, sel, false, kept, result[e].valid, ctx.stream);
 			} else {
-				VGPU_HIP(hipMemcpyAsync(result[e].valid, src->valid, kept, hipMemcpyDeviceToDevice, ctx.stream));
+				vgpu::CopyDevice(result[e].valid, src->valid, kept, ctx.stream);
 			}
 		}
 	}

@@ -56,6 +56,19 @@ __device__ __forceinline__ void MinAgent(u64 *p, u64 v) {
 __device__ __forceinline__ void MaxAgent(u64 *p, u64 v) {
 	__hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+//! Atomic minimum/maximum issued only when `v` can still change the stored word (the stored word only moves
+//! toward the answer, so a value that does not beat a read of it can be dropped). Many waves reducing into
+//! one word mostly skip the atomic once the word has settled.
+__device__ __forceinline__ void MinAgentIfLower(u64 *p, u64 v) {
+	if (v < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+		MinAgent(p, v);
+	}
+}
+__device__ __forceinline__ void MaxAgentIfHigher(u64 *p, u64 v) {
+	if (v > __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+		MaxAgent(p, v);
+	}
+}
 
 // ---------------------------------------------------------------- decoupled look-back words
 //

@@ -240,6 +240,9 @@ void PackValidity(const uint8_t *dense, uint8_t *packed, uint64_t n, hipStream_t
 void FillU8(uint8_t *dst, uint8_t value, uint64_t n, hipStream_t stream);
 void FillU32(uint32_t *dst, uint32_t value, uint64_t n, hipStream_t stream);
 void FillU64(uint64_t *dst, uint64_t value, uint64_t n, hipStream_t stream);
+//! Device-to-device copy on `stream` (a block-chunked copy kernel; falls back to hipMemcpyAsync when a
+//! pointer is not 16-byte aligned).
+void CopyDevice(void *dst, const void *src, uint64_t bytes, hipStream_t stream);
 
 //! Left-major cross product expansion of one column: dst[i] = src[from_left ? i / m : i % m].
 void CrossExpand(const void *src, int width, uint64_t m, bool from_left, uint64_t total, void *dst,

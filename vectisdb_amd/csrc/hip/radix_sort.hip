@@ -70,8 +70,8 @@ __global__ void __launch_bounds__(256) KeyRangeKernel(RangeCols cols, uint64_t n
 				hi[c] = b > hi[c] ? b : hi[c];
 			}
 			if (LaneId() == 0) {
-				MinAgent(mins + c, lo[c]);
-				MaxAgent(maxs + c, hi[c]);
+				MinAgentIfLower(mins + c, lo[c]);
+				MaxAgentIfHigher(maxs + c, hi[c]);
 			}
 		}
 	}
@@ -170,8 +170,8 @@ __global__ void __launch_bounds__(256) KeyRangeHist8Kernel(const T *col, int wid
 		hi = b > hi ? b : hi;
 	}
 	if (lane == 0) {
-		MinAgent(mins, lo);
-		MaxAgent(maxs, hi);
+		MinAgentIfLower(mins, lo);
+		MaxAgentIfHigher(maxs, hi);
 	}
 }
 

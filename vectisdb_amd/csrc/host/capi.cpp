@@ -198,9 +198,9 @@ int vgpu_sort_column(const vgpu_column *key, void *keys_out, uint32_t *perm_out,
 		int pw[2] = {4, TypeWidth(key->type)};
 		SortedRun run;
 		SortPacked(pack, pin, pw, 2, n, TuneSort(pack.total_bits, n), run, arena, s);
-		VGPU_HIP(hipMemcpyAsync(keys_out, run.payload[1], n * pw[1], hipMemcpyDeviceToDevice, s));
+		CopyDevice(keys_out, run.payload[1], n * pw[1], s);
 		if (perm_out != nullptr) {
-			VGPU_HIP(hipMemcpyAsync(perm_out, run.payload[0], n * 4, hipMemcpyDeviceToDevice, s));
+			CopyDevice(perm_out, run.payload[0], n * 4, s);
 		}
 	});
 }
@@ -229,6 +229,10 @@ int vgpu_generate_column(void *out, int32_t type, uint64_t n, int32_t dist, int6
 		GenerateColumn(out, type, n, dist, (long long)lo, (long long)hi, seed, stream_id, cdf, cdf_len,
 		               reinterpret_cast<const long long *>(perm), static_cast<hipStream_t>(stream));
 	});
+}
+
+int vgpu_device_copy(void *dst, const void *src, uint64_t bytes, void *stream) {
+	return Guard([&] { CopyDevice(dst, src, bytes, static_cast<hipStream_t>(stream)); });
 }
 
 int vgpu_profile_enable(int32_t on) {

@@ -56,8 +56,8 @@ int vgpu_join_pairs(const vgpu_column *akey, const vgpu_column *bkey, uint32_t *
 			                         std::to_string(capacity));
 		}
 		if (total > 0) {
-			VGPU_HIP(hipMemcpyAsync(a_rows, l, total * 4, hipMemcpyDeviceToDevice, s));
-			VGPU_HIP(hipMemcpyAsync(b_rows, r, total * 4, hipMemcpyDeviceToDevice, s));
+			CopyDevice(a_rows, l, total * 4, s);
+			CopyDevice(b_rows, r, total * 4, s);
 		}
 		VGPU_HIP(hipStreamSynchronize(s));
 	});

@@ -45,7 +45,7 @@ void *KeepCopy(const void *src, size_t bytes, BatchBuffers &hold, hipStream_t st
 	void *dst = DevicePool::Instance().Alloc(std::max<size_t>(bytes, 8), stream);
 	hold.blocks.push_back(dst);
 	if (bytes > 0) {
-		VGPU_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream));
+		CopyDevice(dst, src, bytes, stream);
 	}
 	return dst;
 }
