@@ -147,3 +147,62 @@ def test_join_count_side_narrowed_to_partner_range(dev, dtype, flip):
                                   [(capi.COUNT_STAR, -1, -1, capi.INT64), (capi.SUM, 0, 0, capi.HUGEINT),
                                    (capi.MIN, 0, 0, capi.INT64)])
     assert res[0] == count and res[1] == total and res[2] == mn
+
+
+def _brute_join(probe, build, payload):
+    """COUNT(*) and SUM(payload) of probe JOIN build ON key, by per-key counts (exact Python ints)."""
+    kp, cp = np.unique(probe, return_counts=True)
+    count = 0
+    total = 0
+    for k, c in zip(kp.tolist(), cp.tolist()):
+        sel = build == k
+        if sel.any():
+            count += c * int(sel.sum())
+            if payload is not None:
+                total += c * int(payload[sel].astype(np.int64).sum())
+    return count, total
+
+
+@pytest.mark.parametrize("dtype,flip", [(np.int64, False), (np.int32, True), (np.int16, False), (np.int64, True)])
+def test_join_probe_form_duplicates_negative_keys_out_of_range(dev, dtype, flip):
+    """Probe form (count-only side read once against a direct-address table of the other side): build keys
+    with duplicates and negative values, probe keys below, inside and above the build range; COUNT(*),
+    SUM and AVG of the build payload exact against brute force, with the probe side as A or as B."""
+    rng = np.random.default_rng(int(np.dtype(dtype).itemsize) * 2 + flip)
+    build = rng.integers(-2000, 6000, 30_000).astype(dtype)
+    payload = rng.integers(-10 ** 6, 10 ** 6, len(build)).astype(np.int64)
+    probe = rng.integers(-5000, 9000, 300_000).astype(dtype)
+    count, total = _brute_join(probe, build, payload)
+    if not flip:
+        res = capi.join_aggregate(_dev(probe, dev), [], _dev(build, dev), [_dev(payload, dev)],
+                                  [(capi.COUNT_STAR, -1, -1, capi.INT64), (capi.SUM, 1, 0, capi.HUGEINT),
+                                   (capi.AVG, 1, 0, capi.FLOAT64)])
+    else:
+        res = capi.join_aggregate(_dev(build, dev), [_dev(payload, dev)], _dev(probe, dev), [],
+                                  [(capi.COUNT_STAR, -1, -1, capi.INT64), (capi.SUM, 0, 0, capi.HUGEINT),
+                                   (capi.AVG, 0, 0, capi.FLOAT64)])
+    assert res[0] == count and res[1] == total
+    assert abs(res[2] - total / count) <= 1e-12 * abs(total / count)
+
+
+@pytest.mark.parametrize("case", ["count_only", "sparse_keys", "sum_does_not_pack"])
+def test_join_probe_form_and_its_fallbacks(dev, case):
+    """COUNT-only join (count-only table); build keys too sparse for a direct-address table and per-key sums
+    past the table's 41-bit field both fall back to the sort-merge form. All exact against brute force."""
+    rng = np.random.default_rng(5)
+    build = rng.integers(0, 20_000, 10_000).astype(np.int64)
+    payload = rng.integers(0, 1000, len(build)).astype(np.int64)
+    if case == "sparse_keys":
+        build = build * 100_003 - 7   # span ~2e9, far above 2 x rows + 2^20
+    if case == "sum_does_not_pack":
+        payload = payload + (1 << 45)
+    probe = rng.choice(np.concatenate([build, build + 1]), 200_000).astype(np.int64)
+    count, total = _brute_join(probe, build, None if case == "count_only" else payload)
+    if case == "count_only":
+        res = capi.join_aggregate(_dev(probe, dev), [], _dev(build, dev), [],
+                                  [(capi.COUNT_STAR, -1, -1, capi.INT64)])
+        assert res == [count]
+    else:
+        res = capi.join_aggregate(_dev(probe, dev), [], _dev(build, dev), [_dev(payload, dev)],
+                                  [(capi.COUNT_STAR, -1, -1, capi.INT64), (capi.SUM, 1, 0, capi.HUGEINT)])
+        assert res == [count, total]

@@ -8,8 +8,9 @@
 //
 // The ranges are laid out like the fused kernel's row ranges: range g covers launch rows
 // [seg[g], seg[g] + seg_len[g]), read from table rows seg_row[g] + (v - seg[g]); other launch rows are
-// padding. A tile of kItems x 256 launch rows usually lies inside one range: it is reduced in registers and
-// LDS and added with three 64-bit atomics; a tile that straddles ranges adds row by row.
+// padding. A tile of kItems x 256 launch rows usually lies inside one range: its rows are summed in
+// registers, and a block carries those sums over its consecutive tiles of the same range, adding them with
+// three 64-bit atomics when the range changes; a tile that straddles ranges adds row by row.
 #include "common.h"
 #include "kernels.h"
 
@@ -21,6 +22,7 @@ namespace {
 constexpr int kProbeBlock = 256;
 constexpr int kProbeItems = 8;
 constexpr uint64_t kProbeTile = (uint64_t)kProbeBlock * kProbeItems;
+constexpr uint64_t kProbeBlocks = 2048;
 
 //! Index of the range that holds launch row v: the last g with seg[g] <= v (empty ranges skipped).
 __device__ __forceinline__ uint32_t RangeOf(const u64 *seg, uint32_t lo, uint32_t hi, u64 v) {
@@ -51,103 +53,15 @@ __device__ __forceinline__ void AtomicAdd128(u64 *p, u64 lo, u64 hi) {
 //! Dense entry: row count in the top 23 bits, the payload sum as a 41-bit two's complement below.
 constexpr int kDenseSumBits = 41;
 
-template <typename K>
-__global__ void __launch_bounds__(kProbeBlock)
-    RangeProbeAggKernel(const K *fk, const u64 *seg, const u64 *seg_row, const u64 *seg_len, uint32_t nseg,
-                        u64 nvirt, const K *dkeys, const u64 *dcount, const u64 *dsum, u64 nd, const u64 *dense,
-                        u64 dense_min, u64 dense_span, u64 *acc) {
-	__shared__ uint32_t s_g[2];
-	__shared__ u64 s_red[3][kProbeBlock / kWave];
-	const int tid = threadIdx.x;
-	const u64 tile_base = (u64)blockIdx.x * kProbeTile;
-	const u64 tile_last = (tile_base + kProbeTile - 1 < nvirt ? tile_base + kProbeTile : nvirt) - 1;
-	if (tid == 0) {
-		s_g[0] = RangeOf(seg, 0, nseg, tile_base);
-		s_g[1] = RangeOf(seg, 0, nseg, tile_last);
-	}
-	__syncthreads();
-	const uint32_t g0 = s_g[0], g1 = s_g[1];
-	K x[kProbeItems];
-	u64 pos[kProbeItems];
-	uint32_t gi[kProbeItems];
-	bool live[kProbeItems];
-#pragma unroll
-	for (int k = 0; k < kProbeItems; k++) {
-		const u64 v = tile_base + (u64)k * kProbeBlock + tid;
-		// v <= tile_last lies in ranges g0..g1, so the search never passes g1
-		const uint32_t g = (g0 == g1 || v > tile_last) ? g0 : RangeOf(seg, g0, g1 + 1, v);
-		const u64 off = v - seg[g];
-		live[k] = v <= tile_last && off < seg_len[g];
-		gi[k] = g;
-		x[k] = live[k] ? fk[seg_row[g] + off] : (K)0;
-		pos[k] = 0;
-	}
-	u64 c = 0, slo = 0, shi = 0;
-	if (dense != nullptr) {
-		// Direct index: one load per probe row (the table is small enough to stay in the Infinity Cache).
-		u64 e[kProbeItems];
-#pragma unroll
-		for (int k = 0; k < kProbeItems; k++) {
-			const u64 off = (u64)(long long)x[k] - dense_min; // wraps past dense_span when out of range
-			e[k] = (live[k] && off < dense_span) ? dense[off] : 0ull;
-		}
-#pragma unroll
-		for (int k = 0; k < kProbeItems; k++) {
-			const u64 n = e[k] >> kDenseSumBits;
-			if (n == 0) {
-				continue;
-			}
-			const long long sv = (long long)(e[k] << (64 - kDenseSumBits)) >> (64 - kDenseSumBits);
-			const u64 ylo = (u64)sv, yhi = sv < 0 ? ~0ull : 0ull;
-			if (g0 == g1) {
-				c += n;
-				Add128(slo, shi, ylo, yhi);
-			} else {
-				u64 *a = acc + 3 * (u64)gi[k];
-				atomicAdd((unsigned long long *)a, (unsigned long long)n);
-				AtomicAdd128(a + 1, ylo, yhi);
-			}
-		}
-	} else {
-		// Branch-free lower_bound of all items at once: every item halves the same span per step, so the
-		// loads of one step are independent and in flight together.
-		if (nd > 0) {
-			u64 len = nd;
-			while (len > 1) {
-				const u64 half = len >> 1;
-#pragma unroll
-				for (int k = 0; k < kProbeItems; k++) {
-					pos[k] = dkeys[pos[k] + half] < x[k] ? pos[k] + half : pos[k];
-				}
-				len -= half;
-			}
-#pragma unroll
-			for (int k = 0; k < kProbeItems; k++) {
-				pos[k] += dkeys[pos[k]] < x[k] ? 1ull : 0ull;
-			}
-		}
-#pragma unroll
-		for (int k = 0; k < kProbeItems; k++) {
-			const bool hit = live[k] && pos[k] < nd && dkeys[pos[k]] == x[k];
-			if (!hit) {
-				continue;
-			}
-			const u64 n = dcount[pos[k]];
-			const u64 ylo = dsum[2 * pos[k]], yhi = dsum[2 * pos[k] + 1];
-			if (g0 == g1) {
-				c += n;
-				Add128(slo, shi, ylo, yhi);
-			} else {
-				u64 *a = acc + 3 * (u64)gi[k];
-				atomicAdd((unsigned long long *)a, (unsigned long long)n);
-				AtomicAdd128(a + 1, ylo, yhi);
-			}
-		}
-	}
-	if (g0 != g1) {
+constexpr uint32_t kNoRange = 0xffffffffu;
+
+//! Adds the block's partial sums for range `g` to acc (block-wide: every thread calls it with the same g;
+//! g = kNoRange adds nothing).
+__device__ __forceinline__ void FlushRange(uint32_t g, u64 c, u64 slo, u64 shi, u64 (*s_red)[kProbeBlock / kWave],
+                                           u64 *acc) {
+	if (g == kNoRange) {
 		return;
 	}
-	// One range: reduce the block and add once.
 	for (int d = kWave / 2; d > 0; d >>= 1) {
 		const u64 oc = __shfl_xor(c, d);
 		const u64 olo = __shfl_xor(slo, d);
@@ -155,6 +69,7 @@ __global__ void __launch_bounds__(kProbeBlock)
 		c += oc;
 		Add128(slo, shi, olo, ohi);
 	}
+	const int tid = threadIdx.x;
 	const int wave = tid / kWave;
 	if ((tid & (kWave - 1)) == 0) {
 		s_red[0][wave] = c;
@@ -169,11 +84,126 @@ __global__ void __launch_bounds__(kProbeBlock)
 			Add128(tlo, thi, s_red[1][w], s_red[2][w]);
 		}
 		if (tc != 0) {
-			u64 *a = acc + 3 * (u64)g0;
+			u64 *a = acc + 3 * (u64)g;
 			atomicAdd((unsigned long long *)a, (unsigned long long)tc);
 			AtomicAdd128(a + 1, tlo, thi);
 		}
 	}
+	__syncthreads();
+}
+
+//! Each block walks `tiles_per_block` consecutive tiles. While consecutive tiles lie inside one range the
+//! block keeps summing in registers and adds to that range's accumulator once, when the range changes or
+//! the block ends (a long range costs one set of atomics per block, not per tile); a tile that straddles
+//! ranges adds row by row.
+template <typename K>
+__global__ void __launch_bounds__(kProbeBlock)
+    RangeProbeAggKernel(const K *fk, const u64 *seg, const u64 *seg_row, const u64 *seg_len, uint32_t nseg,
+                        u64 nvirt, u64 tiles_per_block, const K *dkeys, const u64 *dcount, const u64 *dsum, u64 nd,
+                        const u64 *dense, u64 dense_min, u64 dense_span, u64 *acc) {
+	__shared__ uint32_t s_g[2];
+	__shared__ u64 s_red[3][kProbeBlock / kWave];
+	const int tid = threadIdx.x;
+	const u64 ntiles = (nvirt + kProbeTile - 1) / kProbeTile;
+	const u64 t_begin = (u64)blockIdx.x * tiles_per_block;
+	const u64 t_end = t_begin + tiles_per_block < ntiles ? t_begin + tiles_per_block : ntiles;
+	u64 c = 0, slo = 0, shi = 0; // this thread's sums for range `cur`
+	uint32_t cur = kNoRange;
+	for (u64 t = t_begin; t < t_end; t++) {
+		const u64 tile_base = t * kProbeTile;
+		const u64 tile_last = (tile_base + kProbeTile - 1 < nvirt ? tile_base + kProbeTile : nvirt) - 1;
+		__syncthreads(); // every thread has read the previous tile's s_g
+		if (tid == 0) {
+			s_g[0] = RangeOf(seg, 0, nseg, tile_base);
+			s_g[1] = RangeOf(seg, 0, nseg, tile_last);
+		}
+		__syncthreads();
+		const uint32_t g0 = s_g[0], g1 = s_g[1];
+		if (g0 == g1 && g0 != cur) {
+			FlushRange(cur, c, slo, shi, s_red, acc);
+			c = 0;
+			slo = 0;
+			shi = 0;
+			cur = g0;
+		}
+		K x[kProbeItems];
+		u64 pos[kProbeItems];
+		uint32_t gi[kProbeItems];
+		bool live[kProbeItems];
+#pragma unroll
+		for (int k = 0; k < kProbeItems; k++) {
+			const u64 v = tile_base + (u64)k * kProbeBlock + tid;
+			// v <= tile_last lies in ranges g0..g1, so the search never passes g1
+			const uint32_t g = (g0 == g1 || v > tile_last) ? g0 : RangeOf(seg, g0, g1 + 1, v);
+			const u64 off = v - seg[g];
+			live[k] = v <= tile_last && off < seg_len[g];
+			gi[k] = g;
+			x[k] = live[k] ? fk[seg_row[g] + off] : (K)0;
+			pos[k] = 0;
+		}
+		if (dense != nullptr) {
+			// Direct index: one load per probe row (the table is small enough to stay in the Infinity Cache).
+			u64 e[kProbeItems];
+#pragma unroll
+			for (int k = 0; k < kProbeItems; k++) {
+				const u64 off = (u64)(long long)x[k] - dense_min; // wraps past dense_span when out of range
+				e[k] = (live[k] && off < dense_span) ? dense[off] : 0ull;
+			}
+#pragma unroll
+			for (int k = 0; k < kProbeItems; k++) {
+				const u64 n = e[k] >> kDenseSumBits;
+				if (n == 0) {
+					continue;
+				}
+				const long long sv = (long long)(e[k] << (64 - kDenseSumBits)) >> (64 - kDenseSumBits);
+				const u64 ylo = (u64)sv, yhi = sv < 0 ? ~0ull : 0ull;
+				if (g0 == g1) {
+					c += n;
+					Add128(slo, shi, ylo, yhi);
+				} else {
+					u64 *a = acc + 3 * (u64)gi[k];
+					atomicAdd((unsigned long long *)a, (unsigned long long)n);
+					AtomicAdd128(a + 1, ylo, yhi);
+				}
+			}
+		} else {
+			// Branch-free lower_bound of all items at once: every item halves the same span per step, so the
+			// loads of one step are independent and in flight together.
+			if (nd > 0) {
+				u64 len = nd;
+				while (len > 1) {
+					const u64 half = len >> 1;
+#pragma unroll
+					for (int k = 0; k < kProbeItems; k++) {
+						pos[k] = dkeys[pos[k] + half] < x[k] ? pos[k] + half : pos[k];
+					}
+					len -= half;
+				}
+#pragma unroll
+				for (int k = 0; k < kProbeItems; k++) {
+					pos[k] += dkeys[pos[k]] < x[k] ? 1ull : 0ull;
+				}
+			}
+#pragma unroll
+			for (int k = 0; k < kProbeItems; k++) {
+				const bool hit = live[k] && pos[k] < nd && dkeys[pos[k]] == x[k];
+				if (!hit) {
+					continue;
+				}
+				const u64 n = dcount[pos[k]];
+				const u64 ylo = dsum[2 * pos[k]], yhi = dsum[2 * pos[k] + 1];
+				if (g0 == g1) {
+					c += n;
+					Add128(slo, shi, ylo, yhi);
+				} else {
+					u64 *a = acc + 3 * (u64)gi[k];
+					atomicAdd((unsigned long long *)a, (unsigned long long)n);
+					AtomicAdd128(a + 1, ylo, yhi);
+				}
+			}
+		}
+	}
+	FlushRange(cur, c, slo, shi, s_red, acc);
 }
 
 struct DenseArgs {
@@ -187,9 +217,12 @@ void Launch(const void *fk, const uint64_t *seg, const uint64_t *seg_row, const 
             uint64_t nvirt, const void *dkeys, const uint64_t *dcount, const uint64_t *dsum, uint64_t nd,
             const DenseArgs &dense, uint64_t *acc, hipStream_t stream) {
 	const uint64_t tiles = (nvirt + kProbeTile - 1) / kProbeTile;
-	hipLaunchKernelGGL((RangeProbeAggKernel<K>), dim3((unsigned)tiles), dim3(kProbeBlock), 0, stream,
-	                   (const K *)fk, seg, seg_row, seg_len, nseg, nvirt, (const K *)dkeys, dcount, dsum, nd,
-	                   dense.table, dense.min, dense.span, acc);
+	// Up to kProbeBlocks blocks (8 waves per SIMD on every CU), each over a contiguous run of tiles.
+	const uint64_t blocks = std::min<uint64_t>(tiles, kProbeBlocks);
+	const uint64_t per_block = (tiles + blocks - 1) / blocks;
+	hipLaunchKernelGGL((RangeProbeAggKernel<K>), dim3((unsigned)blocks), dim3(kProbeBlock), 0, stream,
+	                   (const K *)fk, seg, seg_row, seg_len, nseg, nvirt, (u64)per_block, (const K *)dkeys, dcount,
+	                   dsum, nd, dense.table, dense.min, dense.span, acc);
 	VGPU_HIP(hipGetLastError());
 }
 

@@ -489,10 +489,11 @@ JoinIndex::~JoinIndex() {
 }
 
 std::unique_ptr<JoinIndex> BuildJoinIndex(const DevCol &key, const DevCol &payload, hipStream_t stream) {
-	if (!TypeIsInteger(key.type) || key.type == VT_BOOLEAN || !TypeIsInteger(payload.type)) {
+	const bool has_payload = payload.data != nullptr;
+	if (!TypeIsInteger(key.type) || key.type == VT_BOOLEAN || (has_payload && !TypeIsInteger(payload.type))) {
 		throw std::runtime_error("join index: integer key (INT16/32/64) and integer payload");
 	}
-	if (key.rows != payload.rows) {
+	if (has_payload && key.rows != payload.rows) {
 		throw std::runtime_error("join index: key and payload of equal length");
 	}
 	auto index = std::unique_ptr<JoinIndex>(new JoinIndex());
@@ -513,11 +514,18 @@ std::unique_ptr<JoinIndex> BuildJoinIndex(const DevCol &key, const DevCol &paylo
 	std::vector<void *> fixed = {index->keys, index->count, index->sum};
 	size_t next = 0;
 	OutAlloc alloc = [&](size_t) { return fixed.at(next++); };
-	std::vector<AggReq> aggs = {{AGG_COUNT_STAR, -1, VT_INT64}, {AGG_SUM, 0, VT_HUGEINT}};
+	std::vector<AggReq> aggs = {{AGG_COUNT_STAR, -1, VT_INT64}};
+	std::vector<DevCol> values;
+	if (has_payload) {
+		aggs.push_back({AGG_SUM, 0, VT_HUGEINT});
+		values.push_back(payload);
+	} else {
+		FillU64(index->sum, 0ull, cap * 2, stream); // no payload: every sum is 0
+	}
 	std::vector<void *> ko, ao;
 	SortTuning tuning = TuneSort(pack.total_bits, key.rows);
 	index->distinct =
-	    GroupByDevice(keys, {payload}, aggs, key.rows, pack, cap, alloc, ko, ao, tuning, arena, stream, true);
+	    GroupByDevice(keys, values, aggs, key.rows, pack, cap, alloc, ko, ao, tuning, arena, stream, true);
 	if (index->distinct > 0) {
 		// keys ascend: the span is last - first, as sign-extended 64-bit values
 		const int w = TypeWidth(key.type);

@@ -161,6 +161,54 @@ void FinishJoinAggregate(const JoinAggValue &v, int agg_kind, int out_type, uint
 	}
 }
 
+//! Aggregate join by probing: when one side only contributes its row counts and every aggregate is a COUNT
+//! or an integer SUM/AVG of one column of the other (build) side, the build side is reduced to a
+//! direct-address table (per key: row count and payload sum) and the probe side is read once, each row
+//! adding its key's count and sum; nothing of the probe side is sorted. Returns false, having written
+//! nothing, when the build keys are too sparse for the table or a count or sum does not pack.
+bool ProbeJoinAggregate(const DevCol &probe, const DevCol &build, const DevCol *payload, const JoinAggArgs &args,
+                        std::vector<JoinAggValue> &result, Scratch &scratch, hipStream_t stream) {
+	if (build.type == VT_BOOLEAN || build.rows == 0 || probe.rows == 0) {
+		return false;
+	}
+	// The table spans the build keys: skip when even all-distinct keys could not fill half of it.
+	uint64_t mn = 0, mx = 0;
+	const void *cols[1] = {build.data};
+	const int types[1] = {build.type};
+	KeyRanges(cols, types, 1, build.rows, &mn, &mx, scratch, stream);
+	const uint64_t span = mx - mn + 1;
+	if (span == 0 || span > 2 * build.rows + (1ull << 20)) {
+		return false;
+	}
+	DevCol pay {};
+	if (payload != nullptr) {
+		pay = *payload;
+	}
+	auto index = BuildJoinIndex(build, pay, stream);
+	if (index->dense == nullptr) {
+		return false;
+	}
+	const uint64_t row0 = 0, len = probe.rows;
+	const DeviceRanges r = UploadRanges(&row0, &len, 1, probe.rows, scratch, stream);
+	auto *acc = static_cast<uint64_t *>(scratch.Alloc(3 * 8));
+	RangeProbeAgg(probe.type, probe.data, r.seg, r.rows, r.lens, 1, r.nvirt, index->keys, index->count, index->sum,
+	              index->distinct, index->dense, index->dense_min, index->dense_span, acc, stream);
+	uint64_t host[3] = {};
+	VGPU_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	for (int q = 0; q < args.naggs; q++) {
+		result[q].kind = args.kind[q];
+		if (args.kind[q] == JA_COUNT) {
+			result[q].lo = host[0];
+			result[q].hi = 0;
+		} else {
+			result[q].lo = host[1];
+			result[q].hi = host[2];
+		}
+	}
+	return true;
+}
+
 std::vector<JoinAggValue> JoinAggregateDevice(const DevCol &akey, const std::vector<DevCol> &avals, const DevCol &bkey,
                                               const std::vector<DevCol> &bvals, const std::vector<JoinAggReq> &aggs,
                                               Scratch &scratch, hipStream_t stream) {
@@ -227,6 +275,28 @@ std::vector<JoinAggValue> JoinAggregateDevice(const DevCol &akey, const std::vec
 			narrow = 0;
 		} else if (side_aggs[1].empty() && bkey.rows > akey.rows) {
 			narrow = 1;
+		}
+	}
+	if (narrow >= 0) {
+		// Probe form: the count-only side is read once against a table of the other side.
+		const int build_side = 1 - narrow;
+		int payload_input = -1;
+		bool probe_ok = true;
+		for (size_t q = 0; q < aggs.size() && probe_ok; q++) {
+			if (args.kind[q] == JA_COUNT) {
+				continue;
+			}
+			probe_ok = args.kind[q] == JA_SUM_I128 && aggs[q].side == build_side &&
+			           (payload_input < 0 || payload_input == aggs[q].input);
+			payload_input = aggs[q].input;
+		}
+		if (probe_ok) {
+			const std::vector<DevCol> &bv = build_side == 0 ? avals : bvals;
+			const DevCol *payload = payload_input >= 0 ? &bv[payload_input] : nullptr;
+			if (ProbeJoinAggregate(narrow == 0 ? akey : bkey, build_side == 0 ? akey : bkey, payload, args, result,
+			                       scratch, stream)) {
+				return result;
+			}
 		}
 	}
 	SideRuns A, B;

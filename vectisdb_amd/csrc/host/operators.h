@@ -167,6 +167,7 @@ struct JoinIndex {
 //! Reduces the build side (integer key, integer payload, no NULLs) with the GROUP BY operator; when the
 //! distinct keys span at most twice their number (plus 1M) and every count and sum packs into 64 bits, also
 //! builds the direct-address table.
+//! A payload with data == nullptr builds a count-only index (every sum 0).
 std::unique_ptr<JoinIndex> BuildJoinIndex(const DevCol &key, const DevCol &payload, hipStream_t stream);
 
 //! SUM(payload) and COUNT(*) of key = index key for every probe range, as JoinAggValue pairs
