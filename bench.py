@@ -234,7 +234,7 @@ def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
     kernels = {e["name"]: round(e["ms"] / max(e["launches"], 1) * (e["launches"] / steps), 4) for e in prof}
     alg = sum(e["bytes"] for e in prof) / steps
     compulsory = 12 * nb + 8 * nprobe  # read build key + payload and probe key once
-    return {"operator": "sort-merge join + aggregate, 2^24 build x 2^28 probe", "probe_rows": nprobe, "build_rows": nb,
+    return {"operator": "join + aggregate (build-side table, key-range partitioned probe), 2^24 build x 2^28 probe", "probe_rows": nprobe, "build_rows": nb,
             "compulsory_bytes": compulsory,
             "compulsory_frac": round(compulsory / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "sum_payload": int(res[0]), "count": int(res[1]), "ms": round(ms, 3),

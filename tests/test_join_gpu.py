@@ -163,14 +163,18 @@ def _brute_join(probe, build, payload):
     return count, total
 
 
-@pytest.mark.parametrize("dtype,flip", [(np.int64, False), (np.int32, True), (np.int16, False), (np.int64, True)])
-def test_join_probe_form_duplicates_negative_keys_out_of_range(dev, dtype, flip):
+@pytest.mark.parametrize("dtype,flip,ptype", [(np.int64, False, np.int64), (np.int32, True, np.int64),
+                                               (np.int16, False, np.int64), (np.int64, True, np.int32),
+                                               (np.int32, False, np.int16), (np.int16, True, np.int32)])
+def test_join_probe_form_duplicates_negative_keys_out_of_range(dev, dtype, flip, ptype):
     """Probe form (count-only side read once against a direct-address table of the other side): build keys
-    with duplicates and negative values, probe keys below, inside and above the build range; COUNT(*),
-    SUM and AVG of the build payload exact against brute force, with the probe side as A or as B."""
-    rng = np.random.default_rng(int(np.dtype(dtype).itemsize) * 2 + flip)
+    with duplicates and negative values, 16/32/64-bit payloads, probe keys below, inside and above the build
+    range; COUNT(*), SUM and AVG of the build payload exact against brute force, with the probe side as A or
+    as B."""
+    rng = np.random.default_rng(int(np.dtype(dtype).itemsize) * 2 + flip + 7 * np.dtype(ptype).itemsize)
     build = rng.integers(-2000, 6000, 30_000).astype(dtype)
-    payload = rng.integers(-10 ** 6, 10 ** 6, len(build)).astype(np.int64)
+    lim = 30_000 if ptype == np.int16 else 10 ** 6
+    payload = rng.integers(-lim, lim, len(build)).astype(ptype)
     probe = rng.integers(-5000, 9000, 300_000).astype(dtype)
     count, total = _brute_join(probe, build, payload)
     if not flip:
@@ -185,7 +189,7 @@ def test_join_probe_form_duplicates_negative_keys_out_of_range(dev, dtype, flip)
     assert abs(res[2] - total / count) <= 1e-12 * abs(total / count)
 
 
-@pytest.mark.parametrize("case", ["count_only", "sparse_keys", "sum_does_not_pack"])
+@pytest.mark.parametrize("case", ["count_only", "sparse_keys", "sum_does_not_pack", "int32_sum_does_not_pack"])
 def test_join_probe_form_and_its_fallbacks(dev, case):
     """COUNT-only join (count-only table); build keys too sparse for a direct-address table and per-key sums
     past the table's 41-bit field both fall back to the sort-merge form. All exact against brute force."""
@@ -196,6 +200,9 @@ def test_join_probe_form_and_its_fallbacks(dev, case):
         build = build * 100_003 - 7   # span ~2e9, far above 2 x rows + 2^20
     if case == "sum_does_not_pack":
         payload = payload + (1 << 45)
+    if case == "int32_sum_does_not_pack":  # 10K INT32 rows near 2^31 over 8 keys: per-key sums past 2^40
+        build = build % 8
+        payload = (payload + (1 << 31) - 1000).astype(np.int32)
     probe = rng.choice(np.concatenate([build, build + 1]), 200_000).astype(np.int64)
     count, total = _brute_join(probe, build, None if case == "count_only" else payload)
     if case == "count_only":
@@ -206,3 +213,37 @@ def test_join_probe_form_and_its_fallbacks(dev, case):
         res = capi.join_aggregate(_dev(probe, dev), [], _dev(build, dev), [_dev(payload, dev)],
                                   [(capi.COUNT_STAR, -1, -1, capi.INT64), (capi.SUM, 1, 0, capi.HUGEINT)])
         assert res == [count, total]
+
+
+@pytest.mark.parametrize("dtype,skew,flip", [(np.int64, False, False), (np.int32, True, False),
+                                             (np.int64, True, True)])
+def test_join_partitioned_probe_form(dev, dtype, skew, flip):
+    """A probe side at least as large as a direct-address table above 4 MB goes through the key-range
+    partition (histogram, scatter into 256 buckets, bucket-ordered probe). Build keys: a shifted permutation
+    (negative keys included), so COUNT(*) and SUM(payload) have a closed form; probe keys below, inside and
+    above the table, uniform or with one key repeated 2^21 times (one bucket far larger than the rest)."""
+    nb = 1 << 20
+    shift = 300_000
+    rng = np.random.default_rng(41 + skew + 2 * flip)
+    build = (rng.permutation(nb) - shift).astype(dtype)
+    payload = rng.integers(0, 1000, nb).astype(np.int32)
+    hits = rng.integers(0, nb, 3 << 20) - shift
+    below = rng.integers(-2 * nb, -shift, 1 << 18)
+    above = rng.integers(nb - shift, 3 * nb, 1 << 18)
+    parts = [hits, below, above]
+    if skew:
+        parts.append(np.full(1 << 21, int(build[7]), dtype=np.int64))
+    probe = rng.permutation(np.concatenate(parts)).astype(dtype)
+    inv = np.empty(nb, np.int64)
+    inv[build.astype(np.int64) + shift] = np.arange(nb)
+    idx = probe.astype(np.int64) + shift
+    m = (idx >= 0) & (idx < nb)
+    count = int(m.sum())
+    total = int(payload[inv[idx[m]]].astype(np.int64).sum())
+    if not flip:
+        res = capi.join_aggregate(_dev(probe, dev), [], _dev(build, dev), [_dev(payload, dev)],
+                                  [(capi.SUM, 1, 0, capi.HUGEINT), (capi.COUNT_STAR, -1, -1, capi.INT64)])
+    else:
+        res = capi.join_aggregate(_dev(build, dev), [_dev(payload, dev)], _dev(probe, dev), [],
+                                  [(capi.SUM, 0, 0, capi.HUGEINT), (capi.COUNT_STAR, -1, -1, capi.INT64)])
+    assert res == [total, count]

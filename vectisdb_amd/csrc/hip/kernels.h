@@ -270,6 +270,18 @@ void RangeProbeAgg(int key_type, const void *probe_keys, const uint64_t *seg, co
                    const uint64_t *index_count, const uint64_t *index_sum, uint64_t nd, const uint64_t *dense,
                    uint64_t dense_min, uint64_t dense_span, uint64_t *acc, hipStream_t stream);
 
+//! SUM(payload) and COUNT(*) of one probe column against a direct-address table (see BuildDenseIndex), for a
+//! probe side much larger than the table: the probe keys are first partitioned by key range so each part's
+//! slice of the table is read from one XCD's L2. acc = {matched build rows, sum lo, sum hi}. n * 4 bytes
+//! and about 6 MB of scratch.
+void PartitionedProbeAgg(int key_type, const void *probe_keys, uint64_t n, const uint64_t *dense,
+                         uint64_t dense_min, uint64_t dense_span, uint64_t *acc, Scratch &scratch,
+                         hipStream_t stream);
+//! The same with the table's 16-bit form given (or nullptr: the probe reads the 64-bit table from L2).
+void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, const uint64_t *dense,
+                             const uint16_t *dense16, uint64_t dense_min, uint64_t dense_span, uint64_t *acc,
+                             Scratch &scratch, hipStream_t stream);
+
 //! Direct-address form of a join index whose keys span [min, min + span): dense[key - min] packs the key's
 //! row count (top 23 bits) and payload sum (41-bit two's complement); 0 = no build row. Returns false (the
 //! table is then unusable) when a count or sum does not fit. Synchronizes.

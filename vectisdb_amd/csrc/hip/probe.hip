@@ -244,7 +244,439 @@ __global__ void DenseIndexKernel(const K *keys, const u64 *count, const u64 *sum
 	}
 }
 
+// ---------------------------------------------------------------- partitioned probe (one large probe side)
+//
+// A probe side much larger than the build side reads the direct-address table at random: every lookup
+// fetches a whole cache line over the fabric into an L2 that holds 1/32 of a 128 MB table, so the probe is
+// bound by line traffic, not by HBM. Partitioned, the probe keys are first grouped by key range into 256
+// buckets (each bucket's slice of the table is 1/256 of it), and the probe then walks the buckets so that
+// one XCD works on one slice at a time, which stays in that XCD's L2. Steps:
+//   1. PartHistKernel: block k counts its fixed chunk of probe keys per bucket (keys outside the table's
+//      range are dropped here and in step 3 by the same test);
+//   2. PartScanKernel: per bucket, the exclusive prefix of the block counts (bucket-major layout), and
+//      PartStartsKernel the bucket starts;
+//   3. PartScatterKernel: block k walks the same chunk again and writes each key's table offset (uint32) to
+//      its bucket at base[bucket][k] + an LDS counter -- order inside a bucket does not matter for SUM and
+//      COUNT;
+//   4. PartProbeKernel: block i works on buckets i % 8, i % 8 + 8, ... (the blocks of one XCD on one bucket
+//      at a time, as the dispatcher deals blocks to the 8 XCDs in turn; a different placement only costs
+//      cache hits), summing count and payload sum per block; PartFinalKernel adds the block partials in
+//      block order.
+constexpr int kPartBuckets = 256;
+constexpr int kPartBlock = 1024;           // steps 1 and 3: 16 waves, two blocks per CU
+constexpr int kPartItems = 8;              // keys per thread per batch (a batch is 8192 keys)
+constexpr unsigned kPartBlocks = 512;      // chunks of the probe side (steps 1 and 3)
+constexpr int kProbeBlockP = 256;          // step 4
+constexpr unsigned kPartXcds = 8;
+constexpr unsigned kPartProbeBlocks = 2048; // step 4: 256 blocks per XCD
+
+template <typename K>
+__global__ void __launch_bounds__(kPartBlock) PartHistKernel(const K *keys, u64 n, u64 chunk, u64 dmin, u64 span,
+                                                            int shift, uint32_t *hist) {
+	__shared__ uint32_t s_h[kPartBuckets];
+	if (threadIdx.x < kPartBuckets) {
+		s_h[threadIdx.x] = 0;
+	}
+	__syncthreads();
+	const u64 begin = (u64)blockIdx.x * chunk;
+	const u64 end = begin + chunk < n ? begin + chunk : n;
+	for (u64 r0 = begin + threadIdx.x; r0 < end; r0 += (u64)kPartBlock * kPartItems) {
+		K x[kPartItems];
+#pragma unroll
+		for (int j = 0; j < kPartItems; j++) {
+			const u64 r = r0 + (u64)j * kPartBlock;
+			x[j] = r < end ? keys[r] : (K)0;
+		}
+#pragma unroll
+		for (int j = 0; j < kPartItems; j++) {
+			const u64 r = r0 + (u64)j * kPartBlock;
+			const u64 off = (u64)(long long)x[j] - dmin; // wraps past span when below the table
+			if (r < end && off < span) {
+				atomicAdd(&s_h[off >> shift], 1u);
+			}
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x < kPartBuckets) {
+		hist[(u64)threadIdx.x * gridDim.x + blockIdx.x] = s_h[threadIdx.x]; // bucket-major
+	}
+}
+
+//! Block b: exclusive prefix over the chunk blocks of bucket b's counts (rel[b][k]) and the bucket total.
+__global__ void __launch_bounds__(kPartBuckets) PartScanKernel(const uint32_t *hist, unsigned nblocks, u64 *rel,
+                                                              u64 *totals) {
+	__shared__ u64 s_scan[kPartBuckets / kWave + 1];
+	constexpr unsigned kPer = (kPartBlocks + kPartBuckets - 1) / kPartBuckets; // chunk blocks per thread
+	const unsigned b = blockIdx.x;
+	const unsigned k0 = threadIdx.x * kPer;
+	u64 mine = 0;
+#pragma unroll
+	for (unsigned i = 0; i < kPer; i++) {
+		mine += k0 + i < nblocks ? hist[(u64)b * nblocks + k0 + i] : 0u;
+	}
+	u64 total = 0;
+	u64 run = BlockExclusiveScan<u64, kPartBuckets>(mine, s_scan, &total);
+#pragma unroll
+	for (unsigned i = 0; i < kPer; i++) {
+		if (k0 + i < nblocks) {
+			rel[(u64)b * nblocks + k0 + i] = run;
+			run += hist[(u64)b * nblocks + k0 + i];
+		}
+	}
+	if (threadIdx.x == 0) {
+		totals[b] = total;
+	}
+}
+
+//! starts[b] = sum of the totals of buckets before b; starts[256] = all kept keys.
+__global__ void __launch_bounds__(kPartBuckets) PartStartsKernel(const u64 *totals, u64 *starts) {
+	__shared__ u64 s_scan[kPartBuckets / kWave + 1];
+	u64 total = 0;
+	const u64 excl = BlockExclusiveScan<u64, kPartBuckets>(totals[threadIdx.x], s_scan, &total);
+	starts[threadIdx.x] = excl;
+	if (threadIdx.x == kPartBuckets - 1) {
+		starts[kPartBuckets] = total;
+	}
+}
+
+//! Block k walks its chunk in batches of 8192 keys. A batch is grouped by bucket in LDS (LDS atomics give
+//! each key its rank inside its bucket, a scan of the 256 counts gives the buckets' starts), then written
+//! out as one contiguous run per bucket at the block's next position in that bucket.
+template <typename K>
+__global__ void __launch_bounds__(kPartBlock)
+    PartScatterKernel(const K *keys, u64 n, u64 chunk, u64 dmin, u64 span, int shift, const u64 *rel,
+                      const u64 *starts, unsigned nblocks, uint32_t *out) {
+	constexpr int kBatch = kPartBlock * kPartItems;
+	__shared__ u64 s_base[kPartBuckets];       // block's next output position per bucket
+	__shared__ uint32_t s_cnt[kPartBuckets];   // batch count per bucket
+	__shared__ uint32_t s_start[kPartBuckets]; // batch-local start per bucket
+	__shared__ uint32_t s_scan[kPartBlock / kWave + 1];
+	__shared__ uint32_t s_stage[kBatch];
+	const int tid = threadIdx.x;
+	if (tid < kPartBuckets) {
+		s_base[tid] = starts[tid] + rel[(u64)tid * nblocks + blockIdx.x];
+	}
+	const u64 begin = (u64)blockIdx.x * chunk;
+	const u64 end = begin + chunk < n ? begin + chunk : n;
+	// every thread runs the same batches (the loop bound is block-uniform: barriers inside)
+	for (u64 base = begin; base < end; base += kBatch) {
+		uint32_t off[kPartItems], rank[kPartItems];
+		bool valid[kPartItems];
+#pragma unroll
+		for (int j = 0; j < kPartItems; j++) {
+			const u64 r = base + (u64)j * kPartBlock + tid;
+			const K x = r < end ? keys[r] : (K)0;
+			const u64 o = (u64)(long long)x - dmin;
+			valid[j] = r < end && o < span;
+			off[j] = (uint32_t)o;
+		}
+		if (tid < kPartBuckets) {
+			s_cnt[tid] = 0;
+		}
+		__syncthreads();
+#pragma unroll
+		for (int j = 0; j < kPartItems; j++) {
+			rank[j] = valid[j] ? atomicAdd(&s_cnt[off[j] >> shift], 1u) : 0u;
+		}
+		__syncthreads();
+		uint32_t total = 0;
+		const uint32_t excl =
+		    BlockExclusiveScan<uint32_t, kPartBlock>(tid < kPartBuckets ? s_cnt[tid] : 0u, s_scan, &total);
+		if (tid < kPartBuckets) {
+			s_start[tid] = excl;
+		}
+		__syncthreads();
+#pragma unroll
+		for (int j = 0; j < kPartItems; j++) {
+			if (valid[j]) {
+				s_stage[s_start[off[j] >> shift] + rank[j]] = off[j];
+			}
+		}
+		__syncthreads();
+		for (uint32_t i = tid; i < total; i += kPartBlock) {
+			const uint32_t v = s_stage[i];
+			const uint32_t bucket = v >> shift;
+			out[s_base[bucket] + (i - s_start[bucket])] = v;
+		}
+		__syncthreads();
+		if (tid < kPartBuckets) {
+			s_base[tid] += s_cnt[tid];
+		}
+	}
+}
+
+//! Adds one dense entry (count in the top 23 bits, 41-bit two's complement sum) to (c, slo:shi).
+__device__ __forceinline__ void AddDense(u64 e, u64 &c, u64 &slo, u64 &shi) {
+	const long long sv = (long long)(e << (64 - kDenseSumBits)) >> (64 - kDenseSumBits);
+	c += e >> kDenseSumBits;
+	Add128(slo, shi, (u64)sv, sv < 0 ? ~0ull : 0ull);
+}
+
+__global__ void __launch_bounds__(kProbeBlockP) PartProbeKernel(const uint32_t *part, const u64 *starts,
+                                                             const u64 *dense, u64 *partials) {
+	__shared__ u64 s_red[3][kProbeBlockP / kWave];
+	const unsigned xcd = blockIdx.x % kPartXcds;
+	const u64 per_xcd = gridDim.x / kPartXcds;
+	const u64 j = blockIdx.x / kPartXcds;
+	u64 c = 0, slo = 0, shi = 0;
+	for (unsigned b = xcd; b < (unsigned)kPartBuckets; b += kPartXcds) {
+		const u64 s0 = starts[b], len = starts[b + 1] - s0;
+		const u64 per = (len + per_xcd - 1) / per_xcd;
+		const u64 c0 = s0 + (j * per < len ? j * per : len);
+		const u64 c1 = s0 + ((j + 1) * per < len ? (j + 1) * per : len);
+		for (u64 r0 = c0 + threadIdx.x; r0 < c1; r0 += (u64)kProbeBlockP * kPartItems) {
+			uint32_t v[kPartItems];
+#pragma unroll
+			for (int k = 0; k < kPartItems; k++) {
+				const u64 r = r0 + (u64)k * kProbeBlockP;
+				v[k] = r < c1 ? part[r] : 0u;
+			}
+			u64 e[kPartItems];
+#pragma unroll
+			for (int k = 0; k < kPartItems; k++) {
+				const u64 r = r0 + (u64)k * kProbeBlockP;
+				e[k] = r < c1 ? dense[v[k]] : 0ull;
+			}
+#pragma unroll
+			for (int k = 0; k < kPartItems; k++) {
+				AddDense(e[k], c, slo, shi);
+			}
+		}
+	}
+	for (int d = kWave / 2; d > 0; d >>= 1) {
+		const u64 oc = __shfl_xor(c, d);
+		const u64 olo = __shfl_xor(slo, d);
+		const u64 ohi = __shfl_xor(shi, d);
+		c += oc;
+		Add128(slo, shi, olo, ohi);
+	}
+	const int wave = threadIdx.x / kWave;
+	if ((threadIdx.x & (kWave - 1)) == 0) {
+		s_red[0][wave] = c;
+		s_red[1][wave] = slo;
+		s_red[2][wave] = shi;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		u64 tc = 0, tlo = 0, thi = 0;
+		for (int w = 0; w < kProbeBlockP / kWave; w++) {
+			tc += s_red[0][w];
+			Add128(tlo, thi, s_red[1][w], s_red[2][w]);
+		}
+		partials[3 * (u64)blockIdx.x] = tc;
+		partials[3 * (u64)blockIdx.x + 1] = tlo;
+		partials[3 * (u64)blockIdx.x + 2] = thi;
+	}
+}
+
+// 16-bit form of the direct-address table: count in the top 4 bits, the payload sum as a 12-bit two's
+// complement below (build keys that are nearly unique with small payloads). With 256 buckets over a span
+// of at most 2^24 a bucket's slice is at most 64K entries = 128 KB, which fits in LDS (160 KB per CU on
+// gfx950), so the probe reads the table from LDS instead of L2.
+constexpr int kNarrowSumBits = 12;
+constexpr u64 kSliceEntries = 65536;
+constexpr int kSliceBlock = 1024;
+constexpr unsigned kSliceSplits = 4; // blocks per bucket
+
+__global__ void __launch_bounds__(256) NarrowDenseKernel(const u64 *dense, u64 span, uint16_t *out,
+                                                        unsigned int *nofit) {
+	const u64 stride = (u64)gridDim.x * blockDim.x;
+	for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < span; i += stride) {
+		const u64 e = dense[i];
+		const u64 c = e >> kDenseSumBits;
+		const long long sv = (long long)(e << (64 - kDenseSumBits)) >> (64 - kDenseSumBits);
+		const bool fits = c < 16 && sv >= -(1ll << (kNarrowSumBits - 1)) && sv < (1ll << (kNarrowSumBits - 1));
+		out[i] = (uint16_t)((c << kNarrowSumBits) | ((u64)sv & ((1ull << kNarrowSumBits) - 1)));
+		if (__ballot(!fits) != 0 && LaneId() == 0 &&
+		    __hip_atomic_load(nofit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+			atomicOr(nofit, 1u);
+		}
+	}
+}
+
+//! Block i probes split i % kSliceSplits of bucket i / kSliceSplits: the bucket's table slice is copied to
+//! LDS once, then every key of the split is looked up there.
+__global__ void __launch_bounds__(kSliceBlock) PartProbeLdsKernel(const uint32_t *part, const u64 *starts,
+                                                                 const uint16_t *dense16, u64 span, int shift,
+                                                                 u64 *partials) {
+	__shared__ uint16_t s_slice[kSliceEntries];
+	__shared__ u64 s_red[2][kSliceBlock / kWave];
+	const unsigned b = blockIdx.x / kSliceSplits;
+	const unsigned split = blockIdx.x % kSliceSplits;
+	const u64 e0 = (u64)b << shift;
+	const u64 slice = 1ull << shift;
+	const u64 ne = e0 < span ? (span - e0 < slice ? span - e0 : slice) : 0;
+	for (u64 i = threadIdx.x; i < ne; i += kSliceBlock) {
+		s_slice[i] = dense16[e0 + i];
+	}
+	__syncthreads();
+	const u64 s0 = starts[b], len = starts[b + 1] - s0;
+	const u64 per = (len + kSliceSplits - 1) / kSliceSplits;
+	const u64 c0 = s0 + ((u64)split * per < len ? (u64)split * per : len);
+	const u64 c1 = s0 + ((u64)(split + 1) * per < len ? (u64)(split + 1) * per : len);
+	u64 c = 0;
+	long long sum = 0; // at most 2^32 keys x 2^11: fits
+	for (u64 r0 = c0 + threadIdx.x; r0 < c1; r0 += (u64)kSliceBlock * kPartItems) {
+		uint32_t v[kPartItems];
+#pragma unroll
+		for (int k = 0; k < kPartItems; k++) {
+			const u64 r = r0 + (u64)k * kSliceBlock;
+			v[k] = r < c1 ? part[r] : (uint32_t)e0; // a key of this bucket always lies in [e0, e0 + ne)
+		}
+#pragma unroll
+		for (int k = 0; k < kPartItems; k++) {
+			const u64 r = r0 + (u64)k * kSliceBlock;
+			const uint32_t e = r < c1 ? s_slice[v[k] - (uint32_t)e0] : 0u;
+			c += e >> kNarrowSumBits;
+			sum += (long long)((int)(e << (32 - kNarrowSumBits)) >> (32 - kNarrowSumBits));
+		}
+	}
+	for (int d = kWave / 2; d > 0; d >>= 1) {
+		c += __shfl_xor(c, d);
+		sum += __shfl_xor(sum, d);
+	}
+	const int wave = threadIdx.x / kWave;
+	if ((threadIdx.x & (kWave - 1)) == 0) {
+		s_red[0][wave] = c;
+		s_red[1][wave] = (u64)sum;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		u64 tc = 0;
+		long long ts = 0;
+		for (int w = 0; w < kSliceBlock / kWave; w++) {
+			tc += s_red[0][w];
+			ts += (long long)s_red[1][w];
+		}
+		partials[3 * (u64)blockIdx.x] = tc;
+		partials[3 * (u64)blockIdx.x + 1] = (u64)ts;
+		partials[3 * (u64)blockIdx.x + 2] = ts < 0 ? ~0ull : 0ull;
+	}
+}
+
+//! One wave adds the block partials (integers: any order gives the same result).
+__global__ void __launch_bounds__(kWave) PartFinalKernel(const u64 *partials, unsigned nparts, u64 *acc) {
+	u64 c = 0, slo = 0, shi = 0;
+	for (unsigned p = LaneId(); p < nparts; p += kWave) {
+		c += partials[3 * (u64)p];
+		Add128(slo, shi, partials[3 * (u64)p + 1], partials[3 * (u64)p + 2]);
+	}
+	for (int d = kWave / 2; d > 0; d >>= 1) {
+		const u64 oc = __shfl_xor(c, d);
+		const u64 olo = __shfl_xor(slo, d);
+		const u64 ohi = __shfl_xor(shi, d);
+		c += oc;
+		Add128(slo, shi, olo, ohi);
+	}
+	if (LaneId() == 0) {
+		acc[0] = c;
+		acc[1] = slo;
+		acc[2] = shi;
+	}
+}
+
+template <typename K>
+void LaunchPartitioned(const void *keys_v, uint64_t n, uint64_t dmin, uint64_t span, int shift, const uint64_t *dense,
+                       const uint16_t *dense16, uint32_t *hist, uint64_t *rel, uint64_t *totals, uint64_t *starts,
+                       uint32_t *part, uint64_t *partials, uint64_t *acc, hipStream_t stream) {
+	const K *keys = static_cast<const K *>(keys_v);
+	const uint64_t chunk = (n + kPartBlocks - 1) / kPartBlocks;
+	{
+		ProfileScope prof("partition_hist", stream, (double)n * sizeof(K));
+		hipLaunchKernelGGL((PartHistKernel<K>), dim3(kPartBlocks), dim3(kPartBlock), 0, stream, keys, (u64)n,
+		                   (u64)chunk, (u64)dmin, (u64)span, shift, hist);
+		VGPU_HIP(hipGetLastError());
+	}
+	hipLaunchKernelGGL(PartScanKernel, dim3(kPartBuckets), dim3(kPartBuckets), 0, stream, hist, kPartBlocks, rel,
+	                   totals);
+	VGPU_HIP(hipGetLastError());
+	hipLaunchKernelGGL(PartStartsKernel, dim3(1), dim3(kPartBuckets), 0, stream, totals, starts);
+	VGPU_HIP(hipGetLastError());
+	{
+		ProfileScope prof("partition_scatter", stream, (double)n * (sizeof(K) + 4));
+		hipLaunchKernelGGL((PartScatterKernel<K>), dim3(kPartBlocks), dim3(kPartBlock), 0, stream, keys, (u64)n,
+		                   (u64)chunk, (u64)dmin, (u64)span, shift, rel, starts, kPartBlocks, part);
+		VGPU_HIP(hipGetLastError());
+	}
+	unsigned nparts = kPartProbeBlocks;
+	{
+		ProfileScope prof("partition_probe", stream, (double)n * 4);
+		if (dense16 != nullptr && (1ull << shift) <= kSliceEntries) {
+			nparts = kPartBuckets * kSliceSplits;
+			hipLaunchKernelGGL(PartProbeLdsKernel, dim3(nparts), dim3(kSliceBlock), 0, stream, part, starts, dense16,
+			                   (u64)span, shift, partials);
+		} else {
+			hipLaunchKernelGGL(PartProbeKernel, dim3(kPartProbeBlocks), dim3(kProbeBlockP), 0, stream, part, starts,
+			                   dense, partials);
+		}
+		VGPU_HIP(hipGetLastError());
+	}
+	hipLaunchKernelGGL(PartFinalKernel, dim3(1), dim3(kWave), 0, stream, partials, nparts, acc);
+	VGPU_HIP(hipGetLastError());
+}
+
 } // namespace
+
+void PartitionedProbeAgg(int key_type, const void *probe_keys, uint64_t n, const uint64_t *dense,
+                         uint64_t dense_min, uint64_t dense_span, uint64_t *acc, Scratch &scratch,
+                         hipStream_t stream) {
+	if (dense != nullptr && dense_span > 0 && dense_span <= kPartBuckets * kSliceEntries && n > 0) {
+		// 16-bit table when every entry fits: the probe then reads bucket slices from LDS.
+		auto *dense16 = static_cast<uint16_t *>(scratch.Alloc(dense_span * 2));
+		auto *nofit = static_cast<unsigned int *>(scratch.Alloc(4));
+		FillU32(nofit, 0u, 1, stream);
+		const unsigned grid = (unsigned)std::min<uint64_t>((dense_span + 255) / 256, 4096);
+		hipLaunchKernelGGL(NarrowDenseKernel, dim3(grid), dim3(256), 0, stream, dense, (u64)dense_span, dense16,
+		                   nofit);
+		VGPU_HIP(hipGetLastError());
+		unsigned int host = 1;
+		VGPU_HIP(hipMemcpyAsync(&host, nofit, 4, hipMemcpyDeviceToHost, stream));
+		VGPU_HIP(hipStreamSynchronize(stream));
+		PartitionedProbeAggWith(key_type, probe_keys, n, dense, host == 0 ? dense16 : nullptr, dense_min, dense_span,
+		                        acc, scratch, stream);
+		return;
+	}
+	PartitionedProbeAggWith(key_type, probe_keys, n, dense, nullptr, dense_min, dense_span, acc, scratch, stream);
+}
+
+void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, const uint64_t *dense,
+                             const uint16_t *dense16, uint64_t dense_min, uint64_t dense_span, uint64_t *acc,
+                             Scratch &scratch, hipStream_t stream) {
+	if (dense == nullptr || dense_span == 0 || dense_span > (1ull << 32)) {
+		throw HipError("PartitionedProbeAgg: needs a direct-address table of at most 2^32 entries");
+	}
+	FillU64(acc, 0ull, 3, stream);
+	if (n == 0) {
+		return;
+	}
+	int bits = 0;
+	while (bits < 64 && (dense_span - 1) >> bits) {
+		bits++;
+	}
+	const int shift = bits > 8 ? bits - 8 : 0; // bucket = offset >> shift < 256
+	auto *hist = static_cast<uint32_t *>(scratch.Alloc((size_t)kPartBuckets * kPartBlocks * 4));
+	auto *rel = static_cast<uint64_t *>(scratch.Alloc((size_t)kPartBuckets * kPartBlocks * 8));
+	auto *totals = static_cast<uint64_t *>(scratch.Alloc(kPartBuckets * 8));
+	auto *starts = static_cast<uint64_t *>(scratch.Alloc((kPartBuckets + 1) * 8));
+	auto *part = static_cast<uint32_t *>(scratch.Alloc(n * 4));
+	auto *partials = static_cast<uint64_t *>(scratch.Alloc((size_t)kPartProbeBlocks * 3 * 8));
+	switch (key_type) {
+	case VT_INT16:
+		LaunchPartitioned<int16_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, hist, rel, totals,
+		                           starts, part, partials, acc, stream);
+		break;
+	case VT_INT32:
+		LaunchPartitioned<int32_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, hist, rel, totals,
+		                           starts, part, partials, acc, stream);
+		break;
+	case VT_INT64:
+		LaunchPartitioned<int64_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, hist, rel, totals,
+		                           starts, part, partials, acc, stream);
+		break;
+	default:
+		throw HipError("PartitionedProbeAgg: join keys must be INT16, INT32 or INT64");
+	}
+}
 
 bool BuildDenseIndex(int key_type, const void *keys, const uint64_t *count, const uint64_t *sum, uint64_t nd,
                      uint64_t min, uint64_t span, uint64_t *dense, unsigned int *overflow_dev, hipStream_t stream) {

@@ -161,6 +161,10 @@ void FinishJoinAggregate(const JoinAggValue &v, int agg_kind, int out_type, uint
 	}
 }
 
+//! Direct-address tables above this size are probed through a key-range partition of the probe side (one
+//! XCD's L2 is 4 MB).
+constexpr uint64_t kPartitionTableBytes = 4ull << 20;
+
 //! Aggregate join by probing: when one side only contributes its row counts and every aggregate is a COUNT
 //! or an integer SUM/AVG of one column of the other (build) side, the build side is reduced to a
 //! direct-address table (per key: row count and payload sum) and the probe side is read once, each row
@@ -180,6 +184,7 @@ bool ProbeJoinAggregate(const DevCol &probe, const DevCol &build, const DevCol *
 	if (span == 0 || span > 2 * build.rows + (1ull << 20)) {
 		return false;
 	}
+	// The table: the build side's GROUP BY (distinct keys with row count and payload sum), packed.
 	DevCol pay {};
 	if (payload != nullptr) {
 		pay = *payload;
@@ -188,11 +193,19 @@ bool ProbeJoinAggregate(const DevCol &probe, const DevCol &build, const DevCol *
 	if (index->dense == nullptr) {
 		return false;
 	}
-	const uint64_t row0 = 0, len = probe.rows;
-	const DeviceRanges r = UploadRanges(&row0, &len, 1, probe.rows, scratch, stream);
+	const uint64_t *dense = index->dense;
+	const uint64_t dense_min = index->dense_min, dense_span = index->dense_span;
 	auto *acc = static_cast<uint64_t *>(scratch.Alloc(3 * 8));
-	RangeProbeAgg(probe.type, probe.data, r.seg, r.rows, r.lens, 1, r.nvirt, index->keys, index->count, index->sum,
-	              index->distinct, index->dense, index->dense_min, index->dense_span, acc, stream);
+	if (dense_span * 8 > kPartitionTableBytes && probe.rows >= dense_span && dense_span <= (1ull << 32)) {
+		// The table is larger than one XCD's L2 and every line of it is read several times: group the probe
+		// keys by key range first.
+		PartitionedProbeAgg(probe.type, probe.data, probe.rows, dense, dense_min, dense_span, acc, scratch, stream);
+	} else {
+		const uint64_t row0 = 0, len = probe.rows;
+		const DeviceRanges r = UploadRanges(&row0, &len, 1, probe.rows, scratch, stream);
+		RangeProbeAgg(probe.type, probe.data, r.seg, r.rows, r.lens, 1, r.nvirt, nullptr, nullptr, nullptr, 0, dense,
+		              dense_min, dense_span, acc, stream);
+	}
 	uint64_t host[3] = {};
 	VGPU_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, stream));
 	VGPU_HIP(hipStreamSynchronize(stream));
