@@ -488,7 +488,8 @@ JoinIndex::~JoinIndex() {
 	}
 }
 
-std::unique_ptr<JoinIndex> BuildJoinIndex(const DevCol &key, const DevCol &payload, hipStream_t stream) {
+std::unique_ptr<JoinIndex> BuildJoinIndex(const DevCol &key, const DevCol &payload, hipStream_t stream,
+                                          bool dense_only) {
 	const bool has_payload = payload.data != nullptr;
 	if (!TypeIsInteger(key.type) || key.type == VT_BOOLEAN || (has_payload && !TypeIsInteger(payload.type))) {
 		throw std::runtime_error("join index: integer key (INT16/32/64) and integer payload");
@@ -505,7 +506,11 @@ std::unique_ptr<JoinIndex> BuildJoinIndex(const DevCol &key, const DevCol &paylo
 	PoolArena arena(stream);
 	std::vector<DevCol> keys = {key};
 	KeyPack pack;
-	const uint64_t bound = PlanGroupBy(keys, key.rows, pack, arena, stream);
+	uint64_t span = 0;
+	const uint64_t bound = PlanGroupBy(keys, key.rows, pack, arena, stream, nullptr, &span);
+	if (dense_only && (span == 0 || span > 2 * key.rows + (1ull << 20))) {
+		return index; // too sparse for the direct-address table: not built
+	}
 	auto &pool = DevicePool::Instance();
 	const uint64_t cap = std::max<uint64_t>(bound, 1);
 	index->keys = pool.Alloc(cap * TypeWidth(key.type), stream);

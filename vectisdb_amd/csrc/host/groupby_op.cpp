@@ -77,8 +77,11 @@ void ValidateAggregateReq(const AggReq &agg, const std::vector<DevCol> &values) 
 }
 
 uint64_t PlanGroupBy(const std::vector<DevCol> &keys, uint64_t rows, KeyPack &pack, Scratch &scratch,
-                     hipStream_t stream, const uint32_t **hist8) {
+                     hipStream_t stream, const uint32_t **hist8, uint64_t *span_out) {
 	pack = KeyPack {};
+	if (span_out != nullptr) {
+		*span_out = 0;
+	}
 	if (hist8 != nullptr) {
 		*hist8 = nullptr;
 	}
@@ -88,6 +91,9 @@ uint64_t PlanGroupBy(const std::vector<DevCol> &keys, uint64_t rows, KeyPack &pa
 		uint64_t omin = 0, omax = 0;
 		KeyRangeHist8(keys[0].data, keys[0].type, rows, &omin, &omax, h, stream);
 		PlanKeyPackAligned(keys[0].data, keys[0].type, omin, omax, pack);
+		if (span_out != nullptr) {
+			*span_out = omax - omin + 1;
+		}
 		if (hist8 != nullptr) {
 			*hist8 = h;
 		}
@@ -116,6 +122,9 @@ uint64_t PlanGroupBy(const std::vector<DevCol> &keys, uint64_t rows, KeyPack &pa
 		return 0;
 	}
 	KeyRanges(cols.data(), types.data(), (int)keys.size(), rows, mins, maxs, scratch, stream);
+	if (span_out != nullptr && keys.size() == 1) {
+		*span_out = maxs[0] - mins[0] + 1;
+	}
 	return PlanGroupByRanges(keys, rows, mins, maxs, pack);
 }
 

@@ -175,21 +175,13 @@ bool ProbeJoinAggregate(const DevCol &probe, const DevCol &build, const DevCol *
 	if (build.type == VT_BOOLEAN || build.rows == 0 || probe.rows == 0) {
 		return false;
 	}
-	// The table spans the build keys: skip when even all-distinct keys could not fill half of it.
-	uint64_t mn = 0, mx = 0;
-	const void *cols[1] = {build.data};
-	const int types[1] = {build.type};
-	KeyRanges(cols, types, 1, build.rows, &mn, &mx, scratch, stream);
-	const uint64_t span = mx - mn + 1;
-	if (span == 0 || span > 2 * build.rows + (1ull << 20)) {
-		return false;
-	}
-	// The table: the build side's GROUP BY (distinct keys with row count and payload sum), packed.
+	// The table: the build side's GROUP BY (distinct keys with row count and payload sum), packed; not built
+	// when the build keys span so many values that even all-distinct keys could not fill half of it.
 	DevCol pay {};
 	if (payload != nullptr) {
 		pay = *payload;
 	}
-	auto index = BuildJoinIndex(build, pay, stream);
+	auto index = BuildJoinIndex(build, pay, stream, true);
 	if (index->dense == nullptr) {
 		return false;
 	}

@@ -38,8 +38,9 @@ void ValidateAggregateReq(const AggReq &agg, const std::vector<DevCol> &values);
 //! packing. Synchronizes (reads key ranges back). Throws when the keys need more than 64 packed bits.
 //! For a single key column the range pass also builds the byte-digit histograms (*hist8, from `scratch`),
 //! which the sort then reuses instead of reading the keys again.
+//! For one key column, *span_out (when given) receives max - min + 1 of its values (0 when that is 2^64).
 uint64_t PlanGroupBy(const std::vector<DevCol> &keys, uint64_t rows, KeyPack &pack, Scratch &scratch,
-                     hipStream_t stream, const uint32_t **hist8 = nullptr);
+                     hipStream_t stream, const uint32_t **hist8 = nullptr, uint64_t *span_out = nullptr);
 //! The same plan from key ranges already known (order-preserving min/max per key column, as the key-range
 //! pass returns them); reads nothing from the device.
 uint64_t PlanGroupByRanges(const std::vector<DevCol> &keys, uint64_t rows, const uint64_t *mins,
@@ -167,8 +168,11 @@ struct JoinIndex {
 //! Reduces the build side (integer key, integer payload, no NULLs) with the GROUP BY operator; when the
 //! distinct keys span at most twice their number (plus 1M) and every count and sum packs into 64 bits, also
 //! builds the direct-address table.
-//! A payload with data == nullptr builds a count-only index (every sum 0).
-std::unique_ptr<JoinIndex> BuildJoinIndex(const DevCol &key, const DevCol &payload, hipStream_t stream);
+//! A payload with data == nullptr builds a count-only index (every sum 0). With dense_only, an index whose
+//! keys span more than 2 x rows + 2^20 values (so no direct-address table can be built) is returned
+//! without being built (distinct 0, dense null), skipping the sort.
+std::unique_ptr<JoinIndex> BuildJoinIndex(const DevCol &key, const DevCol &payload, hipStream_t stream,
+                                          bool dense_only = false);
 
 //! SUM(payload) and COUNT(*) of key = index key for every probe range, as JoinAggValue pairs
 //! (result[2g] = SUM as int128, result[2g+1] = COUNT). Synchronizes.
