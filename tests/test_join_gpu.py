@@ -215,18 +215,21 @@ def test_join_probe_form_and_its_fallbacks(dev, case):
         assert res == [count, total]
 
 
-@pytest.mark.parametrize("dtype,skew,flip", [(np.int64, False, False), (np.int32, True, False),
-                                             (np.int64, True, True)])
-def test_join_partitioned_probe_form(dev, dtype, skew, flip):
+@pytest.mark.parametrize("dtype,skew,flip,wide", [(np.int64, False, False, False), (np.int32, True, False, False),
+                                                  (np.int64, True, True, False), (np.int64, False, False, True),
+                                                  (np.int32, True, True, True)])
+def test_join_partitioned_probe_form(dev, dtype, skew, flip, wide):
     """A probe side at least as large as a direct-address table above 4 MB goes through the key-range
-    partition (histogram, scatter into 256 buckets, bucket-ordered probe). Build keys: a shifted permutation
-    (negative keys included), so COUNT(*) and SUM(payload) have a closed form; probe keys below, inside and
-    above the table, uniform or with one key repeated 2^21 times (one bucket far larger than the rest)."""
+    partition (histogram, scatter into 256 buckets, bucket-ordered probe): from 16-bit table slices in LDS
+    when every entry fits (payloads below 2^11), from the 64-bit table in L2 otherwise (wide payloads).
+    Build keys: a shifted permutation (negative keys included), so COUNT(*) and SUM(payload) have a closed
+    form; probe keys below, inside and above the table, uniform or with one key repeated 2^21 times (one
+    bucket far larger than the rest)."""
     nb = 1 << 20
     shift = 300_000
-    rng = np.random.default_rng(41 + skew + 2 * flip)
+    rng = np.random.default_rng(41 + skew + 2 * flip + 4 * wide)
     build = (rng.permutation(nb) - shift).astype(dtype)
-    payload = rng.integers(0, 1000, nb).astype(np.int32)
+    payload = rng.integers(-10 ** 6 if wide else 0, 10 ** 6 if wide else 1000, nb).astype(np.int32)
     hits = rng.integers(0, nb, 3 << 20) - shift
     below = rng.integers(-2 * nb, -shift, 1 << 18)
     above = rng.integers(nb - shift, 3 * nb, 1 << 18)
