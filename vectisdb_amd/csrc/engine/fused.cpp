@@ -89,13 +89,19 @@ __device__ __forceinline__ vg_u64 vg_lookback(const vg_u64* state, long long til
   long long end = tile;
   while (end > 0) {
     long long t = end - 1 - lane;
-    vg_u64 word = 0, flag = 2;
-    if (t >= 0) {
-      do { word = vg_load(state + t); flag = word >> 62; if (flag == 0) __builtin_amdgcn_s_sleep(1); } while (flag == 0);
-    }
-    vg_u64 value = t >= 0 ? (word & ((1ull << 62) - 1)) : 0ull;
+    vg_u64 word = t >= 0 ? vg_load(state + t) : (2ull << 62), flag = word >> 62;
+    // wait only for the predecessors nearer than the nearest inclusive one (the only ones summed)
     vg_u64 inc = __ballot(flag == 2);
     int stop = inc ? (__ffsll((long long)inc) - 1) : 64;
+    for (;;) {
+      vg_u64 before = stop >= 64 ? ~0ull : ((1ull << stop) - 1);
+      if ((__ballot(flag == 0) & before) == 0) break;
+      __builtin_amdgcn_s_sleep(1);
+      if (flag == 0 && lane < stop) { word = vg_load(state + t); flag = word >> 62; }
+      inc = __ballot(flag == 2);
+      stop = inc ? (__ffsll((long long)inc) - 1) : 64;
+    }
+    vg_u64 value = t >= 0 ? (word & ((1ull << 62) - 1)) : 0ull;
     vg_u64 c = lane <= stop ? value : 0ull;
     for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
     excl += c;

@@ -115,21 +115,27 @@ __device__ __forceinline__ u64 LookbackExclusiveWave(const u64 *state, int64_t t
 	int64_t window_end = tile; // tiles [window_end - 64, window_end) are examined this step
 	while (window_end > 0) {
 		int64_t t = window_end - 1 - lane;
-		u64 word = 0;
-		u64 flag = kLbInclusive; // lanes past tile 0 act as an inclusive zero
-		if (t >= 0) {
-			do {
-				word = LoadAgent(state + t);
-				flag = word >> kLbFlagShift;
-				if (flag == 0) {
-					__builtin_amdgcn_s_sleep(1);
-				}
-			} while (flag == 0);
-		}
-		u64 value = (t >= 0) ? (word & kLbValueMask) : 0ull;
-		// The nearest inclusive predecessor (smallest lane index with an inclusive flag) ends the walk.
+		// lanes past tile 0 act as an inclusive zero
+		u64 word = t >= 0 ? LoadAgent(state + t) : ((u64)kLbInclusive << kLbFlagShift);
+		u64 flag = word >> kLbFlagShift;
+		// Only the predecessors nearer than the nearest inclusive one are summed, so only they are waited
+		// for: one slow tile further back does not hold up the walk.
 		u64 inclusive_lanes = __ballot(flag == kLbInclusive);
 		int stop_lane = inclusive_lanes ? (__ffsll((long long)inclusive_lanes) - 1) : 64;
+		for (;;) {
+			const u64 before = stop_lane >= 64 ? ~0ull : ((1ull << stop_lane) - 1);
+			if ((__ballot(flag == 0) & before) == 0) {
+				break;
+			}
+			__builtin_amdgcn_s_sleep(1);
+			if (flag == 0 && lane < stop_lane) {
+				word = LoadAgent(state + t);
+				flag = word >> kLbFlagShift;
+			}
+			inclusive_lanes = __ballot(flag == kLbInclusive);
+			stop_lane = inclusive_lanes ? (__ffsll((long long)inclusive_lanes) - 1) : 64;
+		}
+		u64 value = (t >= 0) ? (word & kLbValueMask) : 0ull;
 		u64 contrib = (lane <= stop_lane) ? value : 0ull;
 		// wave sum
 		for (int offset = 32; offset >= 1; offset >>= 1) {
