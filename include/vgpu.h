@@ -111,7 +111,7 @@ int vgpu_cross_filter(const vgpu_column *cols, int32_t ncols, int32_t nleft, uin
                       const vgpu_expr *preds, int32_t npreds, const vgpu_expr *projs, int32_t nprojs,
                       void *const *outs, uint64_t capacity, uint64_t *kept, void *stream);
 
-/* ---------------------------------------------------------------- sort-merge equi-join
+/* ---------------------------------------------------------------- equi-join
  * Side A = children[0] (build_keys), side B = children[1] (probe_keys), one integer key of the same type on
  * both sides, INNER semantics (cuda_engine/src/operators/gpu_hash_join.cu:25-38). */
 typedef struct vgpu_join_agg {
@@ -122,8 +122,12 @@ typedef struct vgpu_join_agg {
 	                        MIN/MAX -> input type (BIGINT/DOUBLE) */
 } vgpu_join_agg;
 
-/* Global aggregates over A JOIN B ON akey = bkey without materializing the join (both sides radix-sorted
- * and reduced to distinct keys, then merge-path matched). Replaces a GROUP_BY_AGGREGATE without keys over
+/* Global aggregates over A JOIN B ON akey = bkey without materializing the join. When one side only
+ * contributes its row count, is the larger one, and every aggregate is a COUNT or an integer SUM/AVG of one
+ * column of the other side, that other side becomes a direct-address table (per key: row count, payload sum)
+ * and the larger side is read once against it (partitioned by key range when the table outgrows L2);
+ * otherwise both sides are radix-sorted, reduced to distinct keys and merge-path matched. Both forms give
+ * the same results. Replaces a GROUP_BY_AGGREGATE without keys over
  * a HASH_JOIN (cuda_engine/src/operators/gpu_hash_join.cu:138-267 followed by gpu_groupby.cu). results:
  * naggs x 16 host bytes, each the aggregate in its output_type (HUGEINT = {lower, upper}). Synchronizes. */
 int vgpu_join_aggregate(const vgpu_column *akey, const vgpu_column *avals, int32_t navals, const vgpu_column *bkey,
