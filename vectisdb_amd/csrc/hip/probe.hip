@@ -264,7 +264,8 @@ __global__ void DenseIndexKernel(const K *keys, const u64 *count, const u64 *sum
 //      block order.
 constexpr int kPartBuckets = 256;
 constexpr int kPartBlock = 1024;           // steps 1 and 3: 16 waves, two blocks per CU
-constexpr int kPartItems = 8;              // keys per thread per batch (a batch is 8192 keys)
+constexpr int kPartItems = 8;              // keys per thread per loop step (histogram, probe)
+constexpr int kScatterItems = 32;          // keys per thread per scatter batch (a batch is 32768 keys)
 constexpr unsigned kPartBlocks = 512;      // chunks of the probe side (steps 1 and 3)
 constexpr int kProbeBlockP = 256;          // step 4
 constexpr unsigned kPartXcds = 8;
@@ -339,14 +340,14 @@ __global__ void __launch_bounds__(kPartBuckets) PartStartsKernel(const u64 *tota
 	}
 }
 
-//! Block k walks its chunk in batches of 8192 keys. A batch is grouped by bucket in LDS (LDS atomics give
+//! Block k walks its chunk in batches of 16384 keys. A batch is grouped by bucket in LDS (LDS atomics give
 //! each key its rank inside its bucket, a scan of the 256 counts gives the buckets' starts), then written
 //! out as one contiguous run per bucket at the block's next position in that bucket.
 template <typename K>
 __global__ void __launch_bounds__(kPartBlock)
     PartScatterKernel(const K *keys, u64 n, u64 chunk, u64 dmin, u64 span, int shift, const u64 *rel,
                       const u64 *starts, unsigned nblocks, uint32_t *out) {
-	constexpr int kBatch = kPartBlock * kPartItems;
+	constexpr int kBatch = kPartBlock * kScatterItems;
 	__shared__ u64 s_base[kPartBuckets];       // block's next output position per bucket
 	__shared__ uint32_t s_cnt[kPartBuckets];   // batch count per bucket
 	__shared__ uint32_t s_start[kPartBuckets]; // batch-local start per bucket
@@ -360,10 +361,10 @@ __global__ void __launch_bounds__(kPartBlock)
 	const u64 end = begin + chunk < n ? begin + chunk : n;
 	// every thread runs the same batches (the loop bound is block-uniform: barriers inside)
 	for (u64 base = begin; base < end; base += kBatch) {
-		uint32_t off[kPartItems], rank[kPartItems];
-		bool valid[kPartItems];
+		uint32_t off[kScatterItems], rank[kScatterItems];
+		bool valid[kScatterItems];
 #pragma unroll
-		for (int j = 0; j < kPartItems; j++) {
+		for (int j = 0; j < kScatterItems; j++) {
 			const u64 r = base + (u64)j * kPartBlock + tid;
 			const K x = r < end ? keys[r] : (K)0;
 			const u64 o = (u64)(long long)x - dmin;
@@ -375,7 +376,7 @@ __global__ void __launch_bounds__(kPartBlock)
 		}
 		__syncthreads();
 #pragma unroll
-		for (int j = 0; j < kPartItems; j++) {
+		for (int j = 0; j < kScatterItems; j++) {
 			rank[j] = valid[j] ? atomicAdd(&s_cnt[off[j] >> shift], 1u) : 0u;
 		}
 		__syncthreads();
@@ -387,7 +388,7 @@ __global__ void __launch_bounds__(kPartBlock)
 		}
 		__syncthreads();
 #pragma unroll
-		for (int j = 0; j < kPartItems; j++) {
+		for (int j = 0; j < kScatterItems; j++) {
 			if (valid[j]) {
 				s_stage[s_start[off[j] >> shift] + rank[j]] = off[j];
 			}
