@@ -248,161 +248,73 @@ __global__ void DenseIndexKernel(const K *keys, const u64 *count, const u64 *sum
 //
 // A probe side much larger than the build side reads the direct-address table at random: every lookup
 // fetches a whole cache line over the fabric into an L2 that holds 1/32 of a 128 MB table, so the probe is
-// bound by line traffic, not by HBM. Partitioned, the probe keys are first grouped by key range into 256
-// buckets (each bucket's slice of the table is 1/256 of it), and the probe then walks the buckets so that
-// one XCD works on one slice at a time, which stays in that XCD's L2. Steps:
-//   1. PartHistKernel: block k counts its fixed chunk of probe keys per bucket (keys outside the table's
-//      range are dropped here and in step 3 by the same test);
-//   2. PartScanKernel: per bucket, the exclusive prefix of the block counts (bucket-major layout), and
-//      PartStartsKernel the bucket starts;
-//   3. PartScatterKernel: block k walks the same chunk again and writes each key's table offset (uint32) to
-//      its bucket at base[bucket][k] + an LDS counter -- order inside a bucket does not matter for SUM and
-//      COUNT;
-//   4. PartProbeKernel: block i works on buckets i % 8, i % 8 + 8, ... (the blocks of one XCD on one bucket
-//      at a time, as the dispatcher deals blocks to the 8 XCDs in turn; a different placement only costs
-//      cache hits), summing count and payload sum per block; PartFinalKernel adds the block partials in
-//      block order.
+// bound by line traffic, not by HBM. Partitioned, the probe keys are grouped by key range into 256 buckets
+// (each bucket's slice of the table is 1/256 of it), and the probe reads one slice at a time. Steps:
+//   1. PartBatchKernel: block g takes batch g (32768 consecutive probe rows), drops the keys outside the
+//      table, groups the rest by bucket in LDS (LDS atomics give each key its rank in its bucket, a scan of
+//      the 256 counts the buckets' starts) and writes them, as table offsets (uint32), over the batch's own
+//      positions of `part`: one contiguous write per batch, and tab[g] = the 257 bucket starts. Order inside
+//      a bucket does not matter for SUM and COUNT, and no global histogram pass is needed;
+//   2. PartProbeKernel<LDS>: block (b, split) walks bucket b's run in every batch of its split (a scan of the
+//      run lengths in LDS maps consecutive threads to consecutive keys of a run), looking each key up in
+//      the bucket's table slice -- copied to LDS when the table has a 16-bit form and the slice fits, read
+//      from L2 otherwise -- and sums count and payload sum per block; PartFinalKernel adds the partials.
 constexpr int kPartBuckets = 256;
-constexpr int kPartBlock = 1024;           // steps 1 and 3: 16 waves, two blocks per CU
-constexpr int kPartItems = 8;              // keys per thread per loop step (histogram, probe)
-constexpr int kScatterItems = 32;          // keys per thread per scatter batch (a batch is 32768 keys)
-constexpr unsigned kPartBlocks = 512;      // chunks of the probe side (steps 1 and 3)
-constexpr int kProbeBlockP = 256;          // step 4
-constexpr unsigned kPartXcds = 8;
-constexpr unsigned kPartProbeBlocks = 2048; // step 4: 256 blocks per XCD
+constexpr int kPartBlock = 1024;      // 16 waves
+constexpr int kPartItems = 8;         // probe: keys per thread per loop step
+constexpr int kScatterItems = 32;     // keys per thread per batch
+constexpr u64 kPartBatch = (u64)kPartBlock * kScatterItems; // 32768 rows
+constexpr unsigned kRunChunk = 2048;  // runs (batches) per LDS scan of run lengths
 
 template <typename K>
-__global__ void __launch_bounds__(kPartBlock) PartHistKernel(const K *keys, u64 n, u64 chunk, u64 dmin, u64 span,
-                                                            int shift, uint32_t *hist) {
-	__shared__ uint32_t s_h[kPartBuckets];
-	if (threadIdx.x < kPartBuckets) {
-		s_h[threadIdx.x] = 0;
-	}
-	__syncthreads();
-	const u64 begin = (u64)blockIdx.x * chunk;
-	const u64 end = begin + chunk < n ? begin + chunk : n;
-	for (u64 r0 = begin + threadIdx.x; r0 < end; r0 += (u64)kPartBlock * kPartItems) {
-		K x[kPartItems];
-#pragma unroll
-		for (int j = 0; j < kPartItems; j++) {
-			const u64 r = r0 + (u64)j * kPartBlock;
-			x[j] = r < end ? keys[r] : (K)0;
-		}
-#pragma unroll
-		for (int j = 0; j < kPartItems; j++) {
-			const u64 r = r0 + (u64)j * kPartBlock;
-			const u64 off = (u64)(long long)x[j] - dmin; // wraps past span when below the table
-			if (r < end && off < span) {
-				atomicAdd(&s_h[off >> shift], 1u);
-			}
-		}
-	}
-	__syncthreads();
-	if (threadIdx.x < kPartBuckets) {
-		hist[(u64)threadIdx.x * gridDim.x + blockIdx.x] = s_h[threadIdx.x]; // bucket-major
-	}
-}
-
-//! Block b: exclusive prefix over the chunk blocks of bucket b's counts (rel[b][k]) and the bucket total.
-__global__ void __launch_bounds__(kPartBuckets) PartScanKernel(const uint32_t *hist, unsigned nblocks, u64 *rel,
-                                                              u64 *totals) {
-	__shared__ u64 s_scan[kPartBuckets / kWave + 1];
-	constexpr unsigned kPer = (kPartBlocks + kPartBuckets - 1) / kPartBuckets; // chunk blocks per thread
-	const unsigned b = blockIdx.x;
-	const unsigned k0 = threadIdx.x * kPer;
-	u64 mine = 0;
-#pragma unroll
-	for (unsigned i = 0; i < kPer; i++) {
-		mine += k0 + i < nblocks ? hist[(u64)b * nblocks + k0 + i] : 0u;
-	}
-	u64 total = 0;
-	u64 run = BlockExclusiveScan<u64, kPartBuckets>(mine, s_scan, &total);
-#pragma unroll
-	for (unsigned i = 0; i < kPer; i++) {
-		if (k0 + i < nblocks) {
-			rel[(u64)b * nblocks + k0 + i] = run;
-			run += hist[(u64)b * nblocks + k0 + i];
-		}
-	}
-	if (threadIdx.x == 0) {
-		totals[b] = total;
-	}
-}
-
-//! starts[b] = sum of the totals of buckets before b; starts[256] = all kept keys.
-__global__ void __launch_bounds__(kPartBuckets) PartStartsKernel(const u64 *totals, u64 *starts) {
-	__shared__ u64 s_scan[kPartBuckets / kWave + 1];
-	u64 total = 0;
-	const u64 excl = BlockExclusiveScan<u64, kPartBuckets>(totals[threadIdx.x], s_scan, &total);
-	starts[threadIdx.x] = excl;
-	if (threadIdx.x == kPartBuckets - 1) {
-		starts[kPartBuckets] = total;
-	}
-}
-
-//! Block k walks its chunk in batches of 16384 keys. A batch is grouped by bucket in LDS (LDS atomics give
-//! each key its rank inside its bucket, a scan of the 256 counts gives the buckets' starts), then written
-//! out as one contiguous run per bucket at the block's next position in that bucket.
-template <typename K>
-__global__ void __launch_bounds__(kPartBlock)
-    PartScatterKernel(const K *keys, u64 n, u64 chunk, u64 dmin, u64 span, int shift, const u64 *rel,
-                      const u64 *starts, unsigned nblocks, uint32_t *out) {
-	constexpr int kBatch = kPartBlock * kScatterItems;
-	__shared__ u64 s_base[kPartBuckets];       // block's next output position per bucket
+__global__ void __launch_bounds__(kPartBlock) PartBatchKernel(const K *keys, u64 n, u64 dmin, u64 span, int shift,
+                                                             uint32_t *part, uint32_t *tab) {
 	__shared__ uint32_t s_cnt[kPartBuckets];   // batch count per bucket
 	__shared__ uint32_t s_start[kPartBuckets]; // batch-local start per bucket
 	__shared__ uint32_t s_scan[kPartBlock / kWave + 1];
-	__shared__ uint32_t s_stage[kBatch];
+	__shared__ uint32_t s_stage[kPartBatch];
 	const int tid = threadIdx.x;
-	if (tid < kPartBuckets) {
-		s_base[tid] = starts[tid] + rel[(u64)tid * nblocks + blockIdx.x];
+	const u64 base = (u64)blockIdx.x * kPartBatch;
+	uint32_t off[kScatterItems], rank[kScatterItems];
+	bool valid[kScatterItems];
+#pragma unroll
+	for (int j = 0; j < kScatterItems; j++) {
+		const u64 r = base + (u64)j * kPartBlock + tid;
+		const K x = r < n ? keys[r] : (K)0;
+		const u64 o = (u64)(long long)x - dmin; // wraps past span when below the table
+		valid[j] = r < n && o < span;
+		off[j] = (uint32_t)o;
 	}
-	const u64 begin = (u64)blockIdx.x * chunk;
-	const u64 end = begin + chunk < n ? begin + chunk : n;
-	// every thread runs the same batches (the loop bound is block-uniform: barriers inside)
-	for (u64 base = begin; base < end; base += kBatch) {
-		uint32_t off[kScatterItems], rank[kScatterItems];
-		bool valid[kScatterItems];
+	if (tid < kPartBuckets) {
+		s_cnt[tid] = 0;
+	}
+	__syncthreads();
 #pragma unroll
-		for (int j = 0; j < kScatterItems; j++) {
-			const u64 r = base + (u64)j * kPartBlock + tid;
-			const K x = r < end ? keys[r] : (K)0;
-			const u64 o = (u64)(long long)x - dmin;
-			valid[j] = r < end && o < span;
-			off[j] = (uint32_t)o;
-		}
-		if (tid < kPartBuckets) {
-			s_cnt[tid] = 0;
-		}
-		__syncthreads();
+	for (int j = 0; j < kScatterItems; j++) {
+		rank[j] = valid[j] ? atomicAdd(&s_cnt[off[j] >> shift], 1u) : 0u;
+	}
+	__syncthreads();
+	uint32_t total = 0;
+	const uint32_t excl =
+	    BlockExclusiveScan<uint32_t, kPartBlock>(tid < kPartBuckets ? s_cnt[tid] : 0u, s_scan, &total);
+	uint32_t *row = tab + (u64)blockIdx.x * (kPartBuckets + 1);
+	if (tid < kPartBuckets) {
+		s_start[tid] = excl;
+		row[tid] = excl;
+	}
+	if (tid == 0) {
+		row[kPartBuckets] = total;
+	}
+	__syncthreads();
 #pragma unroll
-		for (int j = 0; j < kScatterItems; j++) {
-			rank[j] = valid[j] ? atomicAdd(&s_cnt[off[j] >> shift], 1u) : 0u;
+	for (int j = 0; j < kScatterItems; j++) {
+		if (valid[j]) {
+			s_stage[s_start[off[j] >> shift] + rank[j]] = off[j];
 		}
-		__syncthreads();
-		uint32_t total = 0;
-		const uint32_t excl =
-		    BlockExclusiveScan<uint32_t, kPartBlock>(tid < kPartBuckets ? s_cnt[tid] : 0u, s_scan, &total);
-		if (tid < kPartBuckets) {
-			s_start[tid] = excl;
-		}
-		__syncthreads();
-#pragma unroll
-		for (int j = 0; j < kScatterItems; j++) {
-			if (valid[j]) {
-				s_stage[s_start[off[j] >> shift] + rank[j]] = off[j];
-			}
-		}
-		__syncthreads();
-		for (uint32_t i = tid; i < total; i += kPartBlock) {
-			const uint32_t v = s_stage[i];
-			const uint32_t bucket = v >> shift;
-			out[s_base[bucket] + (i - s_start[bucket])] = v;
-		}
-		__syncthreads();
-		if (tid < kPartBuckets) {
-			s_base[tid] += s_cnt[tid];
-		}
+	}
+	__syncthreads();
+	for (uint32_t i = tid; i < total; i += kPartBlock) {
+		part[base + i] = s_stage[i];
 	}
 }
 
@@ -413,70 +325,12 @@ __device__ __forceinline__ void AddDense(u64 e, u64 &c, u64 &slo, u64 &shi) {
 	Add128(slo, shi, (u64)sv, sv < 0 ? ~0ull : 0ull);
 }
 
-__global__ void __launch_bounds__(kProbeBlockP) PartProbeKernel(const uint32_t *part, const u64 *starts,
-                                                             const u64 *dense, u64 *partials) {
-	__shared__ u64 s_red[3][kProbeBlockP / kWave];
-	const unsigned xcd = blockIdx.x % kPartXcds;
-	const u64 per_xcd = gridDim.x / kPartXcds;
-	const u64 j = blockIdx.x / kPartXcds;
-	u64 c = 0, slo = 0, shi = 0;
-	for (unsigned b = xcd; b < (unsigned)kPartBuckets; b += kPartXcds) {
-		const u64 s0 = starts[b], len = starts[b + 1] - s0;
-		const u64 per = (len + per_xcd - 1) / per_xcd;
-		const u64 c0 = s0 + (j * per < len ? j * per : len);
-		const u64 c1 = s0 + ((j + 1) * per < len ? (j + 1) * per : len);
-		for (u64 r0 = c0 + threadIdx.x; r0 < c1; r0 += (u64)kProbeBlockP * kPartItems) {
-			uint32_t v[kPartItems];
-#pragma unroll
-			for (int k = 0; k < kPartItems; k++) {
-				const u64 r = r0 + (u64)k * kProbeBlockP;
-				v[k] = r < c1 ? part[r] : 0u;
-			}
-			u64 e[kPartItems];
-#pragma unroll
-			for (int k = 0; k < kPartItems; k++) {
-				const u64 r = r0 + (u64)k * kProbeBlockP;
-				e[k] = r < c1 ? dense[v[k]] : 0ull;
-			}
-#pragma unroll
-			for (int k = 0; k < kPartItems; k++) {
-				AddDense(e[k], c, slo, shi);
-			}
-		}
-	}
-	for (int d = kWave / 2; d > 0; d >>= 1) {
-		const u64 oc = __shfl_xor(c, d);
-		const u64 olo = __shfl_xor(slo, d);
-		const u64 ohi = __shfl_xor(shi, d);
-		c += oc;
-		Add128(slo, shi, olo, ohi);
-	}
-	const int wave = threadIdx.x / kWave;
-	if ((threadIdx.x & (kWave - 1)) == 0) {
-		s_red[0][wave] = c;
-		s_red[1][wave] = slo;
-		s_red[2][wave] = shi;
-	}
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		u64 tc = 0, tlo = 0, thi = 0;
-		for (int w = 0; w < kProbeBlockP / kWave; w++) {
-			tc += s_red[0][w];
-			Add128(tlo, thi, s_red[1][w], s_red[2][w]);
-		}
-		partials[3 * (u64)blockIdx.x] = tc;
-		partials[3 * (u64)blockIdx.x + 1] = tlo;
-		partials[3 * (u64)blockIdx.x + 2] = thi;
-	}
-}
-
 // 16-bit form of the direct-address table: count in the top 4 bits, the payload sum as a 12-bit two's
 // complement below (build keys that are nearly unique with small payloads). With 256 buckets over a span
 // of at most 2^24 a bucket's slice is at most 64K entries = 128 KB, which fits in LDS (160 KB per CU on
 // gfx950), so the probe reads the table from LDS instead of L2.
 constexpr int kNarrowSumBits = 12;
 constexpr u64 kSliceEntries = 65536;
-constexpr int kSliceBlock = 1024;
 constexpr unsigned kSliceSplits = 4; // blocks per bucket
 
 __global__ void __launch_bounds__(256) NarrowDenseKernel(const u64 *dense, u64 span, uint16_t *out,
@@ -495,63 +349,118 @@ __global__ void __launch_bounds__(256) NarrowDenseKernel(const u64 *dense, u64 s
 	}
 }
 
-//! Block i probes split i % kSliceSplits of bucket i / kSliceSplits: the bucket's table slice is copied to
-//! LDS once, then every key of the split is looked up there.
-__global__ void __launch_bounds__(kSliceBlock) PartProbeLdsKernel(const uint32_t *part, const u64 *starts,
-                                                                 const uint16_t *dense16, u64 span, int shift,
-                                                                 u64 *partials) {
-	__shared__ uint16_t s_slice[kSliceEntries];
-	__shared__ u64 s_red[2][kSliceBlock / kWave];
+//! Block i probes split i % kSliceSplits (a range of batches) of bucket i / kSliceSplits. LDS: the bucket's
+//! 16-bit table slice is copied to LDS once and every key is looked up there; otherwise the 64-bit table is
+//! read through L2.
+template <bool LDS>
+__global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *part, const uint32_t *tab,
+                                                             u64 nbatches, const uint16_t *dense16,
+                                                             const u64 *dense, u64 span, int shift,
+                                                             u64 *partials) {
+	__shared__ uint16_t s_slice[LDS ? kSliceEntries : 1];
+	__shared__ uint32_t s_pre[kRunChunk + 1]; // exclusive prefix of run lengths
+	__shared__ uint32_t s_first[kRunChunk];   // each run's start inside its batch
+	__shared__ uint32_t s_scan[kPartBlock / kWave + 1];
+	__shared__ u64 s_red[3][kPartBlock / kWave];
+	const int tid = threadIdx.x;
 	const unsigned b = blockIdx.x / kSliceSplits;
 	const unsigned split = blockIdx.x % kSliceSplits;
 	const u64 e0 = (u64)b << shift;
-	const u64 slice = 1ull << shift;
-	const u64 ne = e0 < span ? (span - e0 < slice ? span - e0 : slice) : 0;
-	for (u64 i = threadIdx.x; i < ne; i += kSliceBlock) {
-		s_slice[i] = dense16[e0 + i];
+	if constexpr (LDS) {
+		const u64 slice = 1ull << shift;
+		const u64 ne = e0 < span ? (span - e0 < slice ? span - e0 : slice) : 0;
+		for (u64 i = tid; i < ne; i += kPartBlock) {
+			s_slice[i] = dense16[e0 + i];
+		}
 	}
-	__syncthreads();
-	const u64 s0 = starts[b], len = starts[b + 1] - s0;
-	const u64 per = (len + kSliceSplits - 1) / kSliceSplits;
-	const u64 c0 = s0 + ((u64)split * per < len ? (u64)split * per : len);
-	const u64 c1 = s0 + ((u64)(split + 1) * per < len ? (u64)(split + 1) * per : len);
-	u64 c = 0;
-	long long sum = 0; // at most 2^32 keys x 2^11: fits
-	for (u64 r0 = c0 + threadIdx.x; r0 < c1; r0 += (u64)kSliceBlock * kPartItems) {
-		uint32_t v[kPartItems];
+	const u64 per = (nbatches + kSliceSplits - 1) / kSliceSplits;
+	const u64 g_begin = (u64)split * per < nbatches ? (u64)split * per : nbatches;
+	const u64 g_end = g_begin + per < nbatches ? g_begin + per : nbatches;
+	u64 c = 0, slo = 0, shi = 0;
+	for (u64 g0 = g_begin; g0 < g_end; g0 += kRunChunk) {
+		const unsigned nruns = (unsigned)(g_end - g0 < kRunChunk ? g_end - g0 : kRunChunk);
+		// run lengths of this chunk of batches: two runs per thread, scanned
+		uint32_t len[2], first[2];
 #pragma unroll
-		for (int k = 0; k < kPartItems; k++) {
-			const u64 r = r0 + (u64)k * kSliceBlock;
-			v[k] = r < c1 ? part[r] : (uint32_t)e0; // a key of this bucket always lies in [e0, e0 + ne)
+		for (int k = 0; k < 2; k++) {
+			const unsigned i = tid * 2 + k;
+			const uint32_t *row = tab + (g0 + i) * (kPartBuckets + 1);
+			first[k] = i < nruns ? row[b] : 0u;
+			len[k] = i < nruns ? row[b + 1] - first[k] : 0u;
 		}
+		uint32_t total = 0;
+		const uint32_t excl = BlockExclusiveScan<uint32_t, kPartBlock>(len[0] + len[1], s_scan, &total);
+		s_pre[tid * 2] = excl;
+		s_pre[tid * 2 + 1] = excl + len[0];
+		s_first[tid * 2] = first[0];
+		s_first[tid * 2 + 1] = first[1];
+		if (tid == 0) {
+			s_pre[kRunChunk] = total;
+		}
+		__syncthreads(); // also orders the slice copy before the first lookup
+		// Thread t takes kPartItems consecutive keys of the chunk's concatenated runs: one binary search for
+		// the first, then forward steps (the loads of one step hit the same cache lines across the wave).
+		for (uint32_t vb = (uint32_t)tid * kPartItems; vb < total; vb += kPartBlock * kPartItems) {
+			unsigned lo = 0, hi = nruns; // invariant: s_pre[lo] <= vb < s_pre[hi]
+			while (hi - lo > 1) {
+				const unsigned mid = (lo + hi) >> 1;
+				if (s_pre[mid] <= vb) {
+					lo = mid;
+				} else {
+					hi = mid;
+				}
+			}
+			uint32_t key[kPartItems];
 #pragma unroll
-		for (int k = 0; k < kPartItems; k++) {
-			const u64 r = r0 + (u64)k * kSliceBlock;
-			const uint32_t e = r < c1 ? s_slice[v[k] - (uint32_t)e0] : 0u;
-			c += e >> kNarrowSumBits;
-			sum += (long long)((int)(e << (32 - kNarrowSumBits)) >> (32 - kNarrowSumBits));
+			for (int k = 0; k < kPartItems; k++) {
+				const uint32_t v = vb + k;
+				key[k] = (uint32_t)e0; // any offset of this bucket
+				if (v < total) {
+					while (s_pre[lo + 1] <= v) { // v < total = s_pre[nruns], so lo stays below nruns
+						lo++;
+					}
+					key[k] = part[(g0 + lo) * kPartBatch + s_first[lo] + (v - s_pre[lo])];
+				}
+			}
+#pragma unroll
+			for (int k = 0; k < kPartItems; k++) {
+				if (vb + k < total) {
+					if constexpr (LDS) {
+						const uint32_t e = s_slice[key[k] - (uint32_t)e0];
+						c += e >> kNarrowSumBits;
+						const long long sv = (long long)((int)(e << (32 - kNarrowSumBits)) >> (32 - kNarrowSumBits));
+						Add128(slo, shi, (u64)sv, sv < 0 ? ~0ull : 0ull);
+					} else {
+						AddDense(dense[key[k]], c, slo, shi);
+					}
+				}
+			}
 		}
+		__syncthreads(); // s_pre / s_first are rewritten by the next chunk
 	}
 	for (int d = kWave / 2; d > 0; d >>= 1) {
-		c += __shfl_xor(c, d);
-		sum += __shfl_xor(sum, d);
+		const u64 oc = __shfl_xor(c, d);
+		const u64 olo = __shfl_xor(slo, d);
+		const u64 ohi = __shfl_xor(shi, d);
+		c += oc;
+		Add128(slo, shi, olo, ohi);
 	}
-	const int wave = threadIdx.x / kWave;
-	if ((threadIdx.x & (kWave - 1)) == 0) {
+	const int wave = tid / kWave;
+	if ((tid & (kWave - 1)) == 0) {
 		s_red[0][wave] = c;
-		s_red[1][wave] = (u64)sum;
+		s_red[1][wave] = slo;
+		s_red[2][wave] = shi;
 	}
 	__syncthreads();
-	if (threadIdx.x == 0) {
-		u64 tc = 0;
-		long long ts = 0;
-		for (int w = 0; w < kSliceBlock / kWave; w++) {
+	if (tid == 0) {
+		u64 tc = 0, tlo = 0, thi = 0;
+		for (int w = 0; w < kPartBlock / kWave; w++) {
 			tc += s_red[0][w];
-			ts += (long long)s_red[1][w];
+			Add128(tlo, thi, s_red[1][w], s_red[2][w]);
 		}
 		partials[3 * (u64)blockIdx.x] = tc;
-		partials[3 * (u64)blockIdx.x + 1] = (u64)ts;
-		partials[3 * (u64)blockIdx.x + 2] = ts < 0 ? ~0ull : 0ull;
+		partials[3 * (u64)blockIdx.x + 1] = tlo;
+		partials[3 * (u64)blockIdx.x + 2] = thi;
 	}
 }
 
@@ -578,37 +487,25 @@ __global__ void __launch_bounds__(kWave) PartFinalKernel(const u64 *partials, un
 
 template <typename K>
 void LaunchPartitioned(const void *keys_v, uint64_t n, uint64_t dmin, uint64_t span, int shift, const uint64_t *dense,
-                       const uint16_t *dense16, uint32_t *hist, uint64_t *rel, uint64_t *totals, uint64_t *starts,
-                       uint32_t *part, uint64_t *partials, uint64_t *acc, hipStream_t stream) {
+                       const uint16_t *dense16, uint32_t *part, uint32_t *tab, uint64_t *partials, uint64_t *acc,
+                       hipStream_t stream) {
 	const K *keys = static_cast<const K *>(keys_v);
-	const uint64_t chunk = (n + kPartBlocks - 1) / kPartBlocks;
-	{
-		ProfileScope prof("partition_hist", stream, (double)n * sizeof(K));
-		hipLaunchKernelGGL((PartHistKernel<K>), dim3(kPartBlocks), dim3(kPartBlock), 0, stream, keys, (u64)n,
-		                   (u64)chunk, (u64)dmin, (u64)span, shift, hist);
-		VGPU_HIP(hipGetLastError());
-	}
-	hipLaunchKernelGGL(PartScanKernel, dim3(kPartBuckets), dim3(kPartBuckets), 0, stream, hist, kPartBlocks, rel,
-	                   totals);
-	VGPU_HIP(hipGetLastError());
-	hipLaunchKernelGGL(PartStartsKernel, dim3(1), dim3(kPartBuckets), 0, stream, totals, starts);
-	VGPU_HIP(hipGetLastError());
+	const uint64_t nbatches = (n + kPartBatch - 1) / kPartBatch;
 	{
 		ProfileScope prof("partition_scatter", stream, (double)n * (sizeof(K) + 4));
-		hipLaunchKernelGGL((PartScatterKernel<K>), dim3(kPartBlocks), dim3(kPartBlock), 0, stream, keys, (u64)n,
-		                   (u64)chunk, (u64)dmin, (u64)span, shift, rel, starts, kPartBlocks, part);
+		hipLaunchKernelGGL((PartBatchKernel<K>), dim3((unsigned)nbatches), dim3(kPartBlock), 0, stream, keys, (u64)n,
+		                   (u64)dmin, (u64)span, shift, part, tab);
 		VGPU_HIP(hipGetLastError());
 	}
-	unsigned nparts = kPartProbeBlocks;
+	const unsigned nparts = kPartBuckets * kSliceSplits;
 	{
 		ProfileScope prof("partition_probe", stream, (double)n * 4);
 		if (dense16 != nullptr && (1ull << shift) <= kSliceEntries) {
-			nparts = kPartBuckets * kSliceSplits;
-			hipLaunchKernelGGL(PartProbeLdsKernel, dim3(nparts), dim3(kSliceBlock), 0, stream, part, starts, dense16,
-			                   (u64)span, shift, partials);
+			hipLaunchKernelGGL(PartProbeKernel<true>, dim3(nparts), dim3(kPartBlock), 0, stream, part, tab,
+			                   (u64)nbatches, dense16, dense, (u64)span, shift, partials);
 		} else {
-			hipLaunchKernelGGL(PartProbeKernel, dim3(kPartProbeBlocks), dim3(kProbeBlockP), 0, stream, part, starts,
-			                   dense, partials);
+			hipLaunchKernelGGL(PartProbeKernel<false>, dim3(nparts), dim3(kPartBlock), 0, stream, part, tab,
+			                   (u64)nbatches, dense16, dense, (u64)span, shift, partials);
 		}
 		VGPU_HIP(hipGetLastError());
 	}
@@ -655,24 +552,25 @@ void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, c
 		bits++;
 	}
 	const int shift = bits > 8 ? bits - 8 : 0; // bucket = offset >> shift < 256
-	auto *hist = static_cast<uint32_t *>(scratch.Alloc((size_t)kPartBuckets * kPartBlocks * 4));
-	auto *rel = static_cast<uint64_t *>(scratch.Alloc((size_t)kPartBuckets * kPartBlocks * 8));
-	auto *totals = static_cast<uint64_t *>(scratch.Alloc(kPartBuckets * 8));
-	auto *starts = static_cast<uint64_t *>(scratch.Alloc((kPartBuckets + 1) * 8));
-	auto *part = static_cast<uint32_t *>(scratch.Alloc(n * 4));
-	auto *partials = static_cast<uint64_t *>(scratch.Alloc((size_t)kPartProbeBlocks * 3 * 8));
+	const uint64_t nbatches = (n + kPartBatch - 1) / kPartBatch;
+	if (nbatches > 0x7fffffffull) {
+		throw HipError("PartitionedProbeAgg: too many probe rows for one launch");
+	}
+	auto *part = static_cast<uint32_t *>(scratch.Alloc(nbatches * kPartBatch * 4));
+	auto *tab = static_cast<uint32_t *>(scratch.Alloc(nbatches * (kPartBuckets + 1) * 4));
+	auto *partials = static_cast<uint64_t *>(scratch.Alloc((size_t)kPartBuckets * kSliceSplits * 3 * 8));
 	switch (key_type) {
 	case VT_INT16:
-		LaunchPartitioned<int16_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, hist, rel, totals,
-		                           starts, part, partials, acc, stream);
+		LaunchPartitioned<int16_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, part, tab, partials,
+		                           acc, stream);
 		break;
 	case VT_INT32:
-		LaunchPartitioned<int32_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, hist, rel, totals,
-		                           starts, part, partials, acc, stream);
+		LaunchPartitioned<int32_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, part, tab, partials,
+		                           acc, stream);
 		break;
 	case VT_INT64:
-		LaunchPartitioned<int64_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, hist, rel, totals,
-		                           starts, part, partials, acc, stream);
+		LaunchPartitioned<int64_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, part, tab, partials,
+		                           acc, stream);
 		break;
 	default:
 		throw HipError("PartitionedProbeAgg: join keys must be INT16, INT32 or INT64");
