@@ -271,9 +271,10 @@ void RangeProbeAgg(int key_type, const void *probe_keys, const uint64_t *seg, co
                    uint64_t dense_min, uint64_t dense_span, uint64_t *acc, hipStream_t stream);
 
 //! SUM(payload) and COUNT(*) of one probe column against a direct-address table (see BuildDenseIndex), for a
-//! probe side much larger than the table: the probe keys are first partitioned by key range so each part's
-//! slice of the table is read from one XCD's L2. acc = {matched build rows, sum lo, sum hi}. n * 4 bytes
-//! and about 6 MB of scratch.
+//! probe side much larger than the table: the probe keys are first grouped by key range (256 buckets,
+//! batch by batch) so the probe reads one bucket's slice of the table at a time, from LDS when the table
+//! has a 16-bit form. acc = {matched build rows, sum lo, sum hi}. About n * 4.03 bytes of scratch (plus
+//! span * 2 for the 16-bit form).
 void PartitionedProbeAgg(int key_type, const void *probe_keys, uint64_t n, const uint64_t *dense,
                          uint64_t dense_min, uint64_t dense_span, uint64_t *acc, Scratch &scratch,
                          hipStream_t stream);

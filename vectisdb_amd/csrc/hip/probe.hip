@@ -398,13 +398,17 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
 			s_pre[kRunChunk] = total;
 		}
 		__syncthreads(); // also orders the slice copy before the first lookup
-		// Thread t takes kPartItems consecutive keys of the chunk's concatenated runs: one binary search for
-		// the first, then forward steps (the loads of one step hit the same cache lines across the wave).
-		for (uint32_t vb = (uint32_t)tid * kPartItems; vb < total; vb += kPartBlock * kPartItems) {
-			unsigned lo = 0, hi = nruns; // invariant: s_pre[lo] <= vb < s_pre[hi]
+		// Wave w takes 64 x kPartItems consecutive keys of the chunk's concatenated runs per step, lane l the
+		// keys l, l + 64, ... of them: loads are coalesced within a run. Each lane finds its first key's run
+		// by binary search (the lanes of a wave follow nearly the same path) and steps forward from there.
+		const uint32_t wave = (uint32_t)tid / kWave, lane = (uint32_t)tid % kWave;
+		for (uint32_t vb = wave * kWave * kPartItems; vb < total; vb += kPartBlock * kPartItems) {
+			const uint32_t v0 = vb + lane;
+			unsigned lo = 0, hi = nruns; // invariant: s_pre[lo] <= min(v0, total - 1) < s_pre[hi]
+			const uint32_t vs = v0 < total ? v0 : total - 1;
 			while (hi - lo > 1) {
 				const unsigned mid = (lo + hi) >> 1;
-				if (s_pre[mid] <= vb) {
+				if (s_pre[mid] <= vs) {
 					lo = mid;
 				} else {
 					hi = mid;
@@ -413,7 +417,7 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
 			uint32_t key[kPartItems];
 #pragma unroll
 			for (int k = 0; k < kPartItems; k++) {
-				const uint32_t v = vb + k;
+				const uint32_t v = v0 + k * kWave;
 				key[k] = (uint32_t)e0; // any offset of this bucket
 				if (v < total) {
 					while (s_pre[lo + 1] <= v) { // v < total = s_pre[nruns], so lo stays below nruns
@@ -424,7 +428,7 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
 			}
 #pragma unroll
 			for (int k = 0; k < kPartItems; k++) {
-				if (vb + k < total) {
+				if (v0 + k * kWave < total) {
 					if constexpr (LDS) {
 						const uint32_t e = s_slice[key[k] - (uint32_t)e0];
 						c += e >> kNarrowSumBits;
