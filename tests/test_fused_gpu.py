@@ -28,8 +28,9 @@ def cfg0_data(n, seed=0):
     return a, b, c
 
 
-@pytest.mark.parametrize("n", [1, 5, 2047, 2048, 2049, 100_003, 1_048_576])
+@pytest.mark.parametrize("n", [1, 5, 2047, 2048, 2049, 100_003, 1_048_576, (1 << 22) - 1, (1 << 22) + 8_191])
 def test_config0_filter_project_bit_exact(dev, n):
+    """The last two sizes straddle the switch to the 8,192-row-tile variant of the generated kernel."""
     import oracle
     a, b, c = cfg0_data(n, seed=n)
     preds, projs = cfg0_exprs()

@@ -19,6 +19,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -93,13 +94,18 @@ private:
 	friend std::shared_ptr<FusedKernel> GetFusedKernel(const FusedSpec &spec);
 	FusedSpec spec_;
 	RtcKernel *rtc_ = nullptr;
+	//! Variant with larger tiles for large inputs, compiled on its first use (0 rounds: none).
+	int large_rounds_ = 0;
+	mutable std::mutex large_mutex_;
+	mutable RtcKernel *rtc_large_ = nullptr;
 };
 
 //! Builds (or fetches from the process cache, keyed by SHA-256 of the generated source) the kernel.
 std::shared_ptr<FusedKernel> GetFusedKernel(const FusedSpec &spec);
 
-//! The generated source (exposed for tests and the disk-cache key).
-std::string GenerateFusedSource(const FusedSpec &spec);
+//! The generated source (exposed for tests and the disk-cache key); rounds_override > 0 sets the tile's
+//! row rounds instead of the spec's default.
+std::string GenerateFusedSource(const FusedSpec &spec, int rounds_override = 0);
 
 constexpr int kFusedMaxCols = 16;
 constexpr int kFusedMaxOuts = 16;
