@@ -4,6 +4,7 @@
 #include "../../../include/kernel_cache.hpp"
 #include "../hip/kernels.h"
 
+#include <algorithm>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -183,7 +184,13 @@ std::string GenerateFusedSource(const FusedSpec &s, int rounds_override) {
 	o << "extern \"C\" __global__ void __launch_bounds__(" << kBlock << ") vectis_fused(VArgs a) {\n";
 	o << "  const int tid = threadIdx.x; const int lane = tid & 63; const int wave = tid >> 6; (void)lane;\n";
 	// Output staging buffer: each output (and the selection) leaves the block as one contiguous run.
-	o << "  __shared__ vg_u64 s_stage[" << tile_rows << "];\n";
+	// Sized for the widest staged value (outputs, selection ids, validity bytes): 4-byte outputs halve it,
+	// which lets more blocks share a CU.
+	int stage_width = s.emit_selection ? 4 : 1;
+	for (auto &e : s.projections) {
+		stage_width = std::max(stage_width, TypeWidth(e.out_type));
+	}
+	o << "  __shared__ vg_u64 s_stage[" << ((uint64_t)tile_rows * stage_width + 7) / 8 << "];\n";
 	if (filter) {
 		o << "  __shared__ unsigned int s_tile; __shared__ vg_u64 s_wtot[4]; __shared__ vg_u64 s_prefix;\n";
 		o << "  if (tid == 0) s_tile = __hip_atomic_fetch_add((unsigned int*)(a.state + a.ntiles), 1u, "
