@@ -189,13 +189,18 @@ def test_join_probe_form_duplicates_negative_keys_out_of_range(dev, dtype, flip,
     assert abs(res[2] - total / count) <= 1e-12 * abs(total / count)
 
 
-@pytest.mark.parametrize("case", ["count_only", "sparse_keys", "sum_does_not_pack", "int32_sum_does_not_pack"])
+@pytest.mark.parametrize("case", ["count_only", "sparse_keys", "sum_does_not_pack", "int32_sum_does_not_pack",
+                                  "long_runs"])
 def test_join_probe_form_and_its_fallbacks(dev, case):
     """COUNT-only join (count-only table); build keys too sparse for a direct-address table and per-key sums
-    past the table's 41-bit field both fall back to the sort-merge form. All exact against brute force."""
+    past the table's 41-bit field both fall back to the sort-merge form; keys with more than 64 build rows
+    build the table through the GROUP BY operator. All exact against brute force."""
     rng = np.random.default_rng(5)
     build = rng.integers(0, 20_000, 10_000).astype(np.int64)
     payload = rng.integers(0, 1000, len(build)).astype(np.int64)
+    if case == "long_runs":  # ~100 build rows per key
+        build = rng.integers(-50, 150, 20_000).astype(np.int64)
+        payload = rng.integers(-1000, 1000, len(build)).astype(np.int64)
     if case == "sparse_keys":
         build = build * 100_003 - 7   # span ~2e9, far above 2 x rows + 2^20
     if case == "sum_does_not_pack":

@@ -283,6 +283,15 @@ void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, c
                              const uint16_t *dense16, uint64_t dense_min, uint64_t dense_span, uint64_t *acc,
                              Scratch &scratch, hipStream_t stream);
 
+//! Direct-address table (BuildDenseIndex's format) straight from a sorted build side: keys are the packed
+//! sort keys of one column (key = ((packed + pack_min) ^ flip) as sign-extended value; `flip` is the top
+//! bit for signed types), payload the rows' payload in the same order (nullptr: count only; raw integer
+//! of payload_width bytes, sign-extended except for width 1). Returns flags: bit 0 an entry does not
+//! pack, bit 1 a key has more than 64 rows (the table is then incomplete). Synchronizes.
+unsigned BuildDenseFromRun(const void *keys, bool keys64, const void *payload, int payload_width, uint64_t n,
+                           uint64_t pack_min, uint64_t flip, uint64_t dense_min, uint64_t span, uint64_t *dense,
+                           unsigned int *flags_dev, hipStream_t stream);
+
 //! Direct-address form of a join index whose keys span [min, min + span): dense[key - min] packs the key's
 //! row count (top 23 bits) and payload sum (41-bit two's complement); 0 = no build row. Returns false (the
 //! table is then unusable) when a count or sum does not fit. Synchronizes.

@@ -517,7 +517,106 @@ void LaunchPartitioned(const void *keys_v, uint64_t n, uint64_t dmin, uint64_t s
 	VGPU_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------- table straight from the sorted build side
+
+constexpr int kRunItems = 8;        // sorted rows per thread
+constexpr uint64_t kMaxRunRows = 64; // a longer run of one key is left to the GROUP BY path
+
+//! Sorted build side (packed keys ascending, payload permuted with them) -> direct-address table entries.
+//! The thread holding a run's first row sums the run (reading past its own rows when the run continues)
+//! and stores the entry; every other entry stays 0. flags bit 0: an entry does not pack; bit 1: a run is
+//! longer than kMaxRunRows (the caller then takes the GROUP BY path).
+template <typename KeyT>
+__global__ void __launch_bounds__(256) DenseFromRunKernel(const KeyT *keys, const void *payload, int pwidth,
+                                                         u64 n, u64 pack_min, u64 flip, u64 dmin, u64 span,
+                                                         u64 *dense, unsigned int *flags) {
+	const u64 row0 = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * kRunItems;
+	unsigned int bad = 0;
+	for (int i = 0; i < kRunItems; i++) {
+		const u64 r = row0 + i;
+		if (r >= n) {
+			break;
+		}
+		const KeyT k = keys[r];
+		if (r > 0 && keys[r - 1] == k) {
+			continue; // not the first row of its run
+		}
+		u64 cnt = 0, slo = 0, shi = 0;
+		u64 j = r;
+		for (; j < n && cnt < kMaxRunRows && keys[j] == k; j++) {
+			if (payload != nullptr) {
+				long long v;
+				switch (pwidth) {
+				case 1:
+					v = ((const uint8_t *)payload)[j];
+					break;
+				case 2:
+					v = ((const int16_t *)payload)[j];
+					break;
+				case 4:
+					v = ((const int32_t *)payload)[j];
+					break;
+				default:
+					v = ((const long long *)payload)[j];
+					break;
+				}
+				Add128(slo, shi, (u64)v, v < 0 ? ~0ull : 0ull);
+			}
+			cnt++;
+		}
+		if (j < n && keys[j] == k) {
+			bad |= 2u;
+			continue;
+		}
+		const long long sv = (long long)slo;
+		const bool fits = cnt < (1ull << (64 - kDenseSumBits)) && shi == (sv < 0 ? ~0ull : 0ull) &&
+		                  sv >= -(1ll << (kDenseSumBits - 1)) && sv < (1ll << (kDenseSumBits - 1));
+		if (!fits) {
+			bad |= 1u;
+			continue;
+		}
+		const u64 off = (((u64)k + pack_min) ^ flip) - dmin; // sign-extended key minus the table minimum
+		if (off < span) {
+			dense[off] = (cnt << kDenseSumBits) | (slo & ((1ull << kDenseSumBits) - 1));
+		} else {
+			bad |= 1u; // cannot happen for keys inside the planned range; refuse rather than write past
+		}
+	}
+	if (bad != 0) {
+		atomicOr(flags, bad);
+	}
+}
+
 } // namespace
+
+unsigned BuildDenseFromRun(const void *keys, bool keys64, const void *payload, int payload_width, uint64_t n,
+                           uint64_t pack_min, uint64_t flip, uint64_t dense_min, uint64_t span, uint64_t *dense,
+                           unsigned int *flags_dev, hipStream_t stream) {
+	FillU64(dense, 0ull, span, stream);
+	FillU32(flags_dev, 0u, 1, stream);
+	if (n > 0) {
+		const uint64_t threads = (n + kRunItems - 1) / kRunItems;
+		const uint64_t grid = (threads + 255) / 256;
+		if (grid > 0x7fffffffull) {
+			throw HipError("BuildDenseFromRun: too many rows for one launch");
+		}
+		ProfileScope prof("dense_from_run", stream, (double)n * ((keys64 ? 8 : 4) + payload_width));
+		if (keys64) {
+			hipLaunchKernelGGL(DenseFromRunKernel<u64>, dim3((unsigned)grid), dim3(256), 0, stream,
+			                   (const u64 *)keys, payload, payload_width, (u64)n, (u64)pack_min, (u64)flip,
+			                   (u64)dense_min, (u64)span, dense, flags_dev);
+		} else {
+			hipLaunchKernelGGL(DenseFromRunKernel<uint32_t>, dim3((unsigned)grid), dim3(256), 0, stream,
+			                   (const uint32_t *)keys, payload, payload_width, (u64)n, (u64)pack_min, (u64)flip,
+			                   (u64)dense_min, (u64)span, dense, flags_dev);
+		}
+		VGPU_HIP(hipGetLastError());
+	}
+	unsigned int host = 0;
+	VGPU_HIP(hipMemcpyAsync(&host, flags_dev, 4, hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+	return host;
+}
 
 void PartitionedProbeAgg(int key_type, const void *probe_keys, uint64_t n, const uint64_t *dense,
                          uint64_t dense_min, uint64_t dense_span, uint64_t *acc, Scratch &scratch,

@@ -175,18 +175,58 @@ bool ProbeJoinAggregate(const DevCol &probe, const DevCol &build, const DevCol *
 	if (build.type == VT_BOOLEAN || build.rows == 0 || probe.rows == 0) {
 		return false;
 	}
-	// The table: the build side's GROUP BY (distinct keys with row count and payload sum), packed; not built
-	// when the build keys span so many values that even all-distinct keys could not fill half of it.
-	DevCol pay {};
-	if (payload != nullptr) {
-		pay = *payload;
-	}
-	auto index = BuildJoinIndex(build, pay, stream, true);
-	if (index->dense == nullptr) {
+	// The table: the build side sorted (one key column, payload alongside) and each key's run turned into its
+	// entry in one pass (no per-key arrays); not built when the build keys span so many values that even
+	// all-distinct keys could not fill half of it. A key with more than 64 build rows sends the build through
+	// the GROUP BY operator instead.
+	const uint64_t rows = build.rows;
+	auto *h8 = static_cast<uint32_t *>(scratch.Alloc((8 * 256 + 4) * sizeof(uint32_t)));
+	uint64_t omin = 0, omax = 0;
+	KeyRangeHist8(build.data, build.type, rows, &omin, &omax, h8, stream);
+	const uint64_t span = omax - omin + 1;
+	if (span == 0 || span > 2 * rows + (1ull << 20) || rows >= (1ull << 30)) {
 		return false;
 	}
-	const uint64_t *dense = index->dense;
-	const uint64_t dense_min = index->dense_min, dense_span = index->dense_span;
+	const uint64_t flip = TypeIsSigned(build.type) ? (1ull << 63) : 0ull;
+	std::unique_ptr<JoinIndex> index;
+	const uint64_t *dense = nullptr;
+	uint64_t dense_min = omin ^ flip, dense_span = span;
+	{
+		KeyPack pack;
+		PlanKeyPackAligned(build.data, build.type, omin, omax, pack);
+		SortTuning tuning = TuneSort(pack.total_bits, rows);
+		if (tuning.radix_bits == 8) {
+			tuning.hist8 = h8; // byte digits of the aligned packing are the key's bytes
+		}
+		const void *pin[1] = {payload != nullptr ? payload->data : nullptr};
+		int pw[1] = {payload != nullptr ? TypeWidth(payload->type) : 0};
+		if (payload != nullptr && payload->rows != rows) {
+			throw std::runtime_error("gpu_hash_join: aggregate input row count mismatch");
+		}
+		SortedRun run;
+		SortPacked(pack, pin, pw, payload != nullptr ? 1 : 0, rows, tuning, run, scratch, stream);
+		auto *table = static_cast<uint64_t *>(scratch.Alloc(span * 8));
+		auto *flags = static_cast<unsigned int *>(scratch.Alloc(4));
+		const unsigned f = BuildDenseFromRun(run.keys, run.keys64, payload != nullptr ? run.payload[0] : nullptr,
+		                                     pw[0], rows, pack.min[0], flip, dense_min, span, table, flags, stream);
+		if ((f & 1u) != 0) {
+			return false; // an entry does not pack
+		}
+		dense = table;
+		if ((f & 2u) != 0) {
+			DevCol pay {};
+			if (payload != nullptr) {
+				pay = *payload;
+			}
+			index = BuildJoinIndex(build, pay, stream, true);
+			if (index->dense == nullptr) {
+				return false;
+			}
+			dense = index->dense;
+			dense_min = index->dense_min;
+			dense_span = index->dense_span;
+		}
+	}
 	auto *acc = static_cast<uint64_t *>(scratch.Alloc(3 * 8));
 	if (dense_span * 8 > kPartitionTableBytes && probe.rows >= dense_span && dense_span <= (1ull << 32)) {
 		// The table is larger than one XCD's L2 and every line of it is read several times: group the probe
