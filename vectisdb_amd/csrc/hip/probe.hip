@@ -526,47 +526,86 @@ constexpr uint64_t kMaxRunRows = 64; // a longer run of one key is left to the G
 //! The thread holding a run's first row sums the run (reading past its own rows when the run continues)
 //! and stores the entry; every other entry stays 0. flags bit 0: an entry does not pack; bit 1: a run is
 //! longer than kMaxRunRows (the caller then takes the GROUP BY path).
+//! Raw payload value of row j, sign-extended (width 1 is BOOLEAN: zero-extended).
+__device__ __forceinline__ long long PayloadAt(const void *payload, int pwidth, u64 j) {
+	switch (pwidth) {
+	case 1:
+		return ((const uint8_t *)payload)[j];
+	case 2:
+		return ((const int16_t *)payload)[j];
+	case 4:
+		return ((const int32_t *)payload)[j];
+	default:
+		return ((const long long *)payload)[j];
+	}
+}
+
 template <typename KeyT>
 __global__ void __launch_bounds__(256) DenseFromRunKernel(const KeyT *keys, const void *payload, int pwidth,
                                                          u64 n, u64 pack_min, u64 flip, u64 dmin, u64 span,
                                                          u64 *dense, unsigned int *flags) {
 	const u64 row0 = ((u64)blockIdx.x * blockDim.x + threadIdx.x) * kRunItems;
+	if (row0 >= n) {
+		return;
+	}
+	// This thread's rows in registers (vector loads when all of them are there), and the key before them.
+	KeyT kr[kRunItems];
+	long long pr[kRunItems];
+	const bool full = row0 + kRunItems <= n;
+	if (full && (reinterpret_cast<uintptr_t>(keys + row0) & 15) == 0) {
+		uint4 v[kRunItems * sizeof(KeyT) / 16];
+#pragma unroll
+		for (int q = 0; q < (int)(kRunItems * sizeof(KeyT) / 16); q++) {
+			v[q] = reinterpret_cast<const uint4 *>(keys + row0)[q];
+		}
+		__builtin_memcpy(kr, v, sizeof(kr));
+	} else {
+#pragma unroll
+		for (int i = 0; i < kRunItems; i++) {
+			kr[i] = row0 + i < n ? keys[row0 + i] : (KeyT)0;
+		}
+	}
+#pragma unroll
+	for (int i = 0; i < kRunItems; i++) {
+		pr[i] = (payload != nullptr && row0 + i < n) ? PayloadAt(payload, pwidth, row0 + i) : 0ll;
+	}
+	const KeyT before = row0 > 0 ? keys[row0 - 1] : (KeyT)0;
 	unsigned int bad = 0;
+#pragma unroll
 	for (int i = 0; i < kRunItems; i++) {
 		const u64 r = row0 + i;
-		if (r >= n) {
-			break;
-		}
-		const KeyT k = keys[r];
-		if (r > 0 && keys[r - 1] == k) {
-			continue; // not the first row of its run
+		const KeyT k = kr[i];
+		const bool head = r < n && (r == 0 || (i == 0 ? before : kr[i > 0 ? i - 1 : 0]) != k);
+		if (!head) {
+			continue;
 		}
 		u64 cnt = 0, slo = 0, shi = 0;
-		u64 j = r;
-		for (; j < n && cnt < kMaxRunRows && keys[j] == k; j++) {
-			if (payload != nullptr) {
-				long long v;
-				switch (pwidth) {
-				case 1:
-					v = ((const uint8_t *)payload)[j];
-					break;
-				case 2:
-					v = ((const int16_t *)payload)[j];
-					break;
-				case 4:
-					v = ((const int32_t *)payload)[j];
-					break;
-				default:
-					v = ((const long long *)payload)[j];
-					break;
+		bool open = true; // the run reaches the end of this thread's rows
+#pragma unroll
+		for (int j = 0; j < kRunItems; j++) {
+			if (j >= i && open) {
+				if (row0 + j < n && kr[j] == k) {
+					cnt++;
+					Add128(slo, shi, (u64)pr[j], pr[j] < 0 ? ~0ull : 0ull);
+				} else {
+					open = false;
 				}
-				Add128(slo, shi, (u64)v, v < 0 ? ~0ull : 0ull);
 			}
-			cnt++;
 		}
-		if (j < n && keys[j] == k) {
-			bad |= 2u;
-			continue;
+		if (open) {
+			// the run continues into the next thread's rows
+			u64 j = row0 + kRunItems;
+			for (; j < n && cnt < kMaxRunRows && keys[j] == k; j++) {
+				if (payload != nullptr) {
+					const long long v = PayloadAt(payload, pwidth, j);
+					Add128(slo, shi, (u64)v, v < 0 ? ~0ull : 0ull);
+				}
+				cnt++;
+			}
+			if (j < n && keys[j] == k) {
+				bad |= 2u;
+				continue;
+			}
 		}
 		const long long sv = (long long)slo;
 		const bool fits = cnt < (1ull << (64 - kDenseSumBits)) && shi == (sv < 0 ? ~0ull : 0ull) &&
