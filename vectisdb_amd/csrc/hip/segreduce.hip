@@ -276,11 +276,8 @@ constexpr int RedItems() {
 template <typename KeyT, int NP, typename SG>
 __global__ void __launch_bounds__(kRedBlock)
     SegReduceKernel(const KeyT *keys, uint64_t n, PayloadPtrs pay, CompDesc d, AggOutDesc o, KeyOutDesc ko,
-                    u64 *tile_state, uint32_t *ticket, uint32_t *tile_heads, u64 *lead_state, u64 *tail_state,
-                    u64 *tail_gid, uint32_t *tail_list, uint32_t *tail_count, u64 *groups_out, uint64_t ntiles) {
-	__shared__ uint32_t s_tile;
+                    const u64 *tile_prefix, u64 *lead_state, u64 *tail_state, u64 *tail_gid, uint64_t ntiles) {
 	__shared__ u64 s_scan[kRedWaves + 1];
-	__shared__ u64 s_hb;
 	__shared__ u64 s_wave_state[kRedWaves][2 * kMaxComponents];
 	__shared__ int s_wave_flag[kRedWaves];
 	constexpr bool DIRECT = SG::kStatic;
@@ -294,11 +291,9 @@ __global__ void __launch_bounds__(kRedBlock)
 	const int tid = threadIdx.x;
 	const int lane = tid & (kWave - 1);
 	const int wave = tid / kWave;
-	if (tid == 0) {
-		s_tile = FetchAddAgent(ticket, 1u);
-	}
-	__syncthreads();
-	const uint32_t tile = s_tile;
+	// Static tiles: the group ids before this tile come from the head-count scan, so no tile waits on
+	// another and no tile takes a ticket (one same-address atomic per tile serializes at ~12 ns each).
+	const uint32_t tile = blockIdx.x;
 	const uint64_t tile_base = (uint64_t)tile * TILE;
 	const uint32_t tile_count = (uint32_t)((n - tile_base) < (uint64_t)TILE ? (n - tile_base) : TILE);
 
@@ -390,22 +385,7 @@ __global__ void __launch_bounds__(kRedBlock)
 	}
 	u64 H;
 	u64 hb_excl = BlockExclusiveScan<u64, kRedBlock>(hc, s_scan, &H);
-
-	if (tid == 0) {
-		StoreAgent(tile_state + tile, LbPack(tile == 0 ? kLbInclusive : kLbAggregate, H));
-		tile_heads[tile] = (uint32_t)H;
-	}
-	if (wave == 0) {
-		u64 excl = tile == 0 ? 0ull : LookbackExclusiveWave(tile_state, tile);
-		if (lane == 0) {
-			if (tile > 0) {
-				StoreAgent(tile_state + tile, LbPack(kLbInclusive, excl + H));
-			}
-			s_hb = excl;
-		}
-	}
-	__syncthreads();
-	const u64 hb_global = s_hb;
+	const u64 hb_global = tile_prefix[tile];
 
 	// Serial pass over this thread's rows.
 	CompState S, lead;
@@ -512,18 +492,16 @@ __global__ void __launch_bounds__(kRedBlock)
 				tail_gid[tile] = ~0ull;
 			} else {
 				StateStore<SG>(d, tail_state + (uint64_t)tile * 2 * d.ncomp, X);
-				tail_gid[tile] = gid_last;
-				uint32_t slot = FetchAddAgent(tail_count, 1u);
-				tail_list[slot] = tile;
+				tail_gid[tile] = gid_last; // an open tail: TailFinishKernel completes the group
 			}
 		} else {
 			StateStore<SG>(d, lead_state + (uint64_t)tile * 2 * d.ncomp, X);
 			tail_gid[tile] = ~0ull;
 		}
-		if (tile == ntiles - 1) {
-			*groups_out = hb_global + H;
-		}
 	}
+	(void)ntiles;
+	(void)wave;
+	(void)lane;
 }
 
 //! Count-only reduce (every aggregate is a COUNT): a run-length encoding of the sorted keys. Each thread
@@ -706,11 +684,12 @@ __global__ void __launch_bounds__(kFixBlock) ChunkRunKernel(CompDesc d, const ui
 //! following chunks (a few steps even for a group spanning thousands of tiles), then writes the group.
 __global__ void __launch_bounds__(kFixBlock) TailFinishKernel(CompDesc d, AggOutDesc o, const u64 *run_state,
                                                               const uint8_t *run_open, const u64 *tail_state,
-                                                              const u64 *tail_gid, const uint32_t *tail_list,
-                                                              const uint32_t *tail_count, uint64_t ntiles) {
-	const uint32_t ntails = *tail_count;
-	for (uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < ntails; item += gridDim.x * blockDim.x) {
-		const uint64_t T = tail_list[item];
+                                                              const u64 *tail_gid, uint64_t ntiles) {
+	for (uint64_t T = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; T < ntiles;
+	     T += (uint64_t)gridDim.x * blockDim.x) {
+		if (tail_gid[T] == ~0ull) {
+			continue; // this tile's last group ended inside it
+		}
 		CompState acc;
 		StateLoad(d, tail_state + T * 2 * d.ncomp, acc);
 		uint64_t u = T + 1;
@@ -727,22 +706,80 @@ __global__ void __launch_bounds__(kFixBlock) TailFinishKernel(CompDesc d, AggOut
 	}
 }
 
+//! Group heads per reduce tile (the same rows and the same head test as SegReduceKernel: row r starts a
+//! group when r == 0 or its key differs from row r - 1's; without keys only row 0 does).
+template <typename KeyT, int ITEMS>
+__global__ void __launch_bounds__(kRedBlock) HeadCountKernel(const KeyT *keys, uint64_t n, uint32_t *counts) {
+	__shared__ u64 s_scan[kRedWaves + 1];
+	const uint64_t row0 = (uint64_t)blockIdx.x * kRedBlock * ITEMS + (uint64_t)threadIdx.x * ITEMS;
+	u64 hc = 0;
+	if (keys == nullptr) {
+		hc = row0 == 0 && n > 0 ? 1 : 0;
+	} else if (row0 < n) {
+		u64 kreg[ITEMS];
+		LoadRun<ITEMS, KeyT>(keys, row0, n, kreg);
+		KeyT before = row0 > 0 ? keys[row0 - 1] : (KeyT)0;
+#pragma unroll
+		for (int i = 0; i < ITEMS; i++) {
+			const KeyT k = (KeyT)kreg[i];
+			hc += (row0 + i < n && (row0 + i == 0 || k != before)) ? 1 : 0;
+			before = k;
+		}
+	}
+	u64 total;
+	BlockExclusiveScan<u64, kRedBlock>(hc, s_scan, &total);
+	if (threadIdx.x == 0) {
+		counts[blockIdx.x] = (uint32_t)total;
+	}
+}
+
+constexpr int kScanBlock = 1024;
+
+//! Exclusive prefix of the per-tile head counts (one block; each thread a contiguous run of tiles), and the
+//! total group count.
+__global__ void __launch_bounds__(kScanBlock) TilePrefixKernel(const uint32_t *counts, uint64_t ntiles,
+                                                               u64 *prefix, u64 *groups_out) {
+	__shared__ u64 s_scan[kScanBlock / kWave + 1];
+	const uint64_t per = (ntiles + kScanBlock - 1) / kScanBlock;
+	const uint64_t t0 = (uint64_t)threadIdx.x * per;
+	const uint64_t t1 = t0 + per < ntiles ? t0 + per : ntiles;
+	u64 mine = 0;
+	for (uint64_t t = t0; t < t1; t++) {
+		mine += counts[t];
+	}
+	u64 total;
+	u64 run = BlockExclusiveScan<u64, kScanBlock>(mine, s_scan, &total);
+	for (uint64_t t = t0; t < t1; t++) {
+		prefix[t] = run;
+		run += counts[t];
+	}
+	if (threadIdx.x == 0) {
+		*groups_out = total;
+	}
+}
+
 template <typename KeyT, int NP, typename SG = DynSig>
 void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, const CompDesc &cd,
                   const AggOutDesc &od, const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch,
                   hipStream_t stream) {
-	constexpr uint64_t kTile = (uint64_t)kRedBlock * RedItems<SG>();
+	constexpr int kItems = RedItems<SG>();
+	constexpr uint64_t kTile = (uint64_t)kRedBlock * kItems;
 	const uint64_t ntiles = (n + kTile - 1) / kTile;
 	const size_t state_words = 2 * (size_t)std::max(cd.ncomp, 1);
-	auto *lb = static_cast<u64 *>(scratch.Alloc((ntiles + 1) * sizeof(u64)));
-	auto *ticket = reinterpret_cast<uint32_t *>(lb + ntiles);
 	auto *tile_heads = static_cast<uint32_t *>(scratch.Alloc(ntiles * sizeof(uint32_t)));
+	auto *tile_prefix = static_cast<u64 *>(scratch.Alloc(ntiles * sizeof(u64)));
 	auto *lead_state = static_cast<u64 *>(scratch.Alloc(ntiles * state_words * sizeof(u64)));
 	auto *tail_state = static_cast<u64 *>(scratch.Alloc(ntiles * state_words * sizeof(u64)));
 	auto *tail_gid = static_cast<u64 *>(scratch.Alloc(ntiles * sizeof(u64)));
-	auto *tail_list = static_cast<uint32_t *>(scratch.Alloc(ntiles * sizeof(uint32_t)));
-	auto *tail_count = ticket + 1; // second half of the zeroed ticket word
-	FillU64(lb, 0ull, ntiles + 1, stream);
+	{
+		ProfileScope prof("seg_heads", stream, (double)n * (run.keys ? (run.keys64 ? 8.0 : 4.0) : 0.0));
+		hipLaunchKernelGGL((HeadCountKernel<KeyT, kItems>), dim3((unsigned)ntiles), dim3(kRedBlock), 0, stream,
+		                   (const KeyT *)run.keys, n, tile_heads);
+		VGPU_HIP(hipGetLastError());
+		hipLaunchKernelGGL(TilePrefixKernel, dim3(1), dim3(kScanBlock), 0, stream, tile_heads, ntiles, tile_prefix,
+		                   groups_device);
+		VGPU_HIP(hipGetLastError());
+	}
 	double bytes = (double)n * (run.keys ? (run.keys64 ? 8.0 : 4.0) : 0.0);
 	for (int p = 0; p < pay.n; p++) {
 		bytes += (double)n * pay.width[p];
@@ -750,8 +787,8 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 	{
 		ProfileScope prof("seg_reduce", stream, bytes);
 		hipLaunchKernelGGL((SegReduceKernel<KeyT, NP, SG>), dim3((unsigned)ntiles), dim3(kRedBlock), 0, stream,
-		                   (const KeyT *)run.keys, n, pay, cd, od, ko, lb, ticket, tile_heads, lead_state,
-		                   tail_state, tail_gid, tail_list, tail_count, groups_device, ntiles);
+		                   (const KeyT *)run.keys, n, pay, cd, od, ko, tile_prefix, lead_state, tail_state, tail_gid,
+		                   ntiles);
 		VGPU_HIP(hipGetLastError());
 	}
 	{
@@ -763,7 +800,7 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 		                   lead_state, run_state, run_open, ntiles);
 		VGPU_HIP(hipGetLastError());
 		hipLaunchKernelGGL(TailFinishKernel, dim3((unsigned)nchunks), dim3(kFixBlock), 0, stream, cd, od, run_state,
-		                   run_open, tail_state, tail_gid, tail_list, tail_count, ntiles);
+		                   run_open, tail_state, tail_gid, ntiles);
 		VGPU_HIP(hipGetLastError());
 	}
 }
