@@ -28,6 +28,10 @@ constexpr int kRedBlock = 256;
 constexpr int kRedItems = 8;
 constexpr int kRedTile = kRedBlock * kRedItems;
 constexpr int kRedWaves = kRedBlock / kWave;
+//! Tile offsets come from a two-level scan of per-tile head counts: kScanTiles tiles per first-level block.
+constexpr int kScanBlock = 1024;
+constexpr int kScanPerThread = 4;
+constexpr uint64_t kScanTiles = (uint64_t)kScanBlock * kScanPerThread; // tiles per scan block
 
 struct PayloadPtrs {
 	int n = 0;
@@ -276,7 +280,8 @@ constexpr int RedItems() {
 template <typename KeyT, int NP, typename SG>
 __global__ void __launch_bounds__(kRedBlock)
     SegReduceKernel(const KeyT *keys, uint64_t n, PayloadPtrs pay, CompDesc d, AggOutDesc o, KeyOutDesc ko,
-                    const u64 *tile_prefix, u64 *lead_state, u64 *tail_state, u64 *tail_gid, uint64_t ntiles) {
+                    const u64 *tile_prefix, const u64 *tile_carry, u64 *lead_state, u64 *tail_state, u64 *tail_gid,
+                    uint64_t ntiles) {
 	__shared__ u64 s_scan[kRedWaves + 1];
 	__shared__ u64 s_wave_state[kRedWaves][2 * kMaxComponents];
 	__shared__ int s_wave_flag[kRedWaves];
@@ -385,7 +390,7 @@ __global__ void __launch_bounds__(kRedBlock)
 	}
 	u64 H;
 	u64 hb_excl = BlockExclusiveScan<u64, kRedBlock>(hc, s_scan, &H);
-	const u64 hb_global = tile_prefix[tile];
+	const u64 hb_global = tile_prefix[tile] + tile_carry[tile / kScanTiles];
 
 	// Serial pass over this thread's rows.
 	CompState S, lead;
@@ -707,54 +712,86 @@ __global__ void __launch_bounds__(kFixBlock) TailFinishKernel(CompDesc d, AggOut
 }
 
 //! Group heads per reduce tile (the same rows and the same head test as SegReduceKernel: row r starts a
-//! group when r == 0 or its key differs from row r - 1's; without keys only row 0 does).
-template <typename KeyT, int ITEMS>
-__global__ void __launch_bounds__(kRedBlock) HeadCountKernel(const KeyT *keys, uint64_t n, uint32_t *counts) {
-	__shared__ u64 s_scan[kRedWaves + 1];
-	const uint64_t row0 = (uint64_t)blockIdx.x * kRedBlock * ITEMS + (uint64_t)threadIdx.x * ITEMS;
+//! group when r == 0 or its key differs from row r - 1's; without keys only row 0 does). One wave per
+//! tile, each lane 32 consecutive keys read with vector loads; no barriers.
+constexpr int kHeadLaneRows = 32;
+template <typename KeyT>
+__global__ void __launch_bounds__(256) HeadCountKernel(const KeyT *keys, uint64_t n, uint64_t tile_rows,
+                                                      uint64_t ntiles, uint32_t *counts) {
+	const uint64_t tile = (uint64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+	if (tile >= ntiles) {
+		return;
+	}
+	const int lane = LaneId();
 	u64 hc = 0;
 	if (keys == nullptr) {
-		hc = row0 == 0 && n > 0 ? 1 : 0;
-	} else if (row0 < n) {
-		u64 kreg[ITEMS];
-		LoadRun<ITEMS, KeyT>(keys, row0, n, kreg);
-		KeyT before = row0 > 0 ? keys[row0 - 1] : (KeyT)0;
+		hc = (tile == 0 && lane == 0 && n > 0) ? 1 : 0;
+	} else {
+		// tile_rows = 64 lanes x kHeadLaneRows rows
+		const uint64_t row0 = tile * tile_rows + (uint64_t)lane * kHeadLaneRows;
+		if (row0 < n) {
+			u64 kreg[kHeadLaneRows];
+			LoadRun<kHeadLaneRows, KeyT>(keys, row0, n, kreg);
+			KeyT before = row0 > 0 ? keys[row0 - 1] : (KeyT)0;
 #pragma unroll
-		for (int i = 0; i < ITEMS; i++) {
-			const KeyT k = (KeyT)kreg[i];
-			hc += (row0 + i < n && (row0 + i == 0 || k != before)) ? 1 : 0;
-			before = k;
+			for (int i = 0; i < kHeadLaneRows; i++) {
+				const KeyT k = (KeyT)kreg[i];
+				hc += (row0 + i < n && (row0 + i == 0 || k != before)) ? 1 : 0;
+				before = k;
+			}
 		}
 	}
-	u64 total;
-	BlockExclusiveScan<u64, kRedBlock>(hc, s_scan, &total);
-	if (threadIdx.x == 0) {
-		counts[blockIdx.x] = (uint32_t)total;
+	for (int off = kWave / 2; off > 0; off >>= 1) {
+		hc += __shfl_xor(hc, off);
+	}
+	if (lane == 0) {
+		counts[tile] = (uint32_t)hc;
 	}
 }
 
-constexpr int kScanBlock = 1024;
 
-//! Exclusive prefix of the per-tile head counts (one block; each thread a contiguous run of tiles), and the
-//! total group count.
+//! Level 1: exclusive prefix of the head counts inside each run of kScanTiles tiles, and each run's total.
 __global__ void __launch_bounds__(kScanBlock) TilePrefixKernel(const uint32_t *counts, uint64_t ntiles,
-                                                               u64 *prefix, u64 *groups_out) {
+                                                               u64 *prefix, u64 *block_totals) {
 	__shared__ u64 s_scan[kScanBlock / kWave + 1];
-	const uint64_t per = (ntiles + kScanBlock - 1) / kScanBlock;
-	const uint64_t t0 = (uint64_t)threadIdx.x * per;
-	const uint64_t t1 = t0 + per < ntiles ? t0 + per : ntiles;
+	const uint64_t t0 = (uint64_t)blockIdx.x * kScanTiles + (uint64_t)threadIdx.x * kScanPerThread;
+	uint32_t c[kScanPerThread];
 	u64 mine = 0;
-	for (uint64_t t = t0; t < t1; t++) {
-		mine += counts[t];
+#pragma unroll
+	for (int i = 0; i < kScanPerThread; i++) {
+		c[i] = t0 + i < ntiles ? counts[t0 + i] : 0u;
+		mine += c[i];
 	}
 	u64 total;
 	u64 run = BlockExclusiveScan<u64, kScanBlock>(mine, s_scan, &total);
-	for (uint64_t t = t0; t < t1; t++) {
-		prefix[t] = run;
-		run += counts[t];
+#pragma unroll
+	for (int i = 0; i < kScanPerThread; i++) {
+		if (t0 + i < ntiles) {
+			prefix[t0 + i] = run;
+		}
+		run += c[i];
 	}
 	if (threadIdx.x == 0) {
-		*groups_out = total;
+		block_totals[blockIdx.x] = total;
+	}
+}
+
+//! Level 2 (one block): block_totals becomes each run's exclusive carry; *groups_out the total.
+__global__ void __launch_bounds__(kScanBlock) TileCarryKernel(u64 *block_totals, uint64_t nblocks, u64 *groups_out) {
+	__shared__ u64 s_scan[kScanBlock / kWave + 1];
+	u64 carry = 0;
+	for (uint64_t base = 0; base < nblocks; base += kScanBlock) {
+		const uint64_t i = base + threadIdx.x;
+		const u64 v = i < nblocks ? block_totals[i] : 0ull;
+		u64 total;
+		const u64 excl = BlockExclusiveScan<u64, kScanBlock>(v, s_scan, &total);
+		if (i < nblocks) {
+			block_totals[i] = carry + excl;
+		}
+		carry += total;
+	}
+	if (threadIdx.x == 0) {
+		*groups_out = carry;
 	}
 }
 
@@ -771,13 +808,18 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 	auto *lead_state = static_cast<u64 *>(scratch.Alloc(ntiles * state_words * sizeof(u64)));
 	auto *tail_state = static_cast<u64 *>(scratch.Alloc(ntiles * state_words * sizeof(u64)));
 	auto *tail_gid = static_cast<u64 *>(scratch.Alloc(ntiles * sizeof(u64)));
+	const uint64_t nscan = (ntiles + kScanTiles - 1) / kScanTiles;
+	auto *tile_carry = static_cast<u64 *>(scratch.Alloc(std::max<uint64_t>(nscan, 1) * sizeof(u64)));
 	{
+		static_assert(kTile == (uint64_t)kWave * kHeadLaneRows, "one wave per reduce tile");
 		ProfileScope prof("seg_heads", stream, (double)n * (run.keys ? (run.keys64 ? 8.0 : 4.0) : 0.0));
-		hipLaunchKernelGGL((HeadCountKernel<KeyT, kItems>), dim3((unsigned)ntiles), dim3(kRedBlock), 0, stream,
-		                   (const KeyT *)run.keys, n, tile_heads);
+		hipLaunchKernelGGL((HeadCountKernel<KeyT>), dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, stream,
+		                   (const KeyT *)run.keys, n, kTile, ntiles, tile_heads);
 		VGPU_HIP(hipGetLastError());
-		hipLaunchKernelGGL(TilePrefixKernel, dim3(1), dim3(kScanBlock), 0, stream, tile_heads, ntiles, tile_prefix,
-		                   groups_device);
+		hipLaunchKernelGGL(TilePrefixKernel, dim3((unsigned)nscan), dim3(kScanBlock), 0, stream, tile_heads, ntiles,
+		                   tile_prefix, tile_carry);
+		VGPU_HIP(hipGetLastError());
+		hipLaunchKernelGGL(TileCarryKernel, dim3(1), dim3(kScanBlock), 0, stream, tile_carry, nscan, groups_device);
 		VGPU_HIP(hipGetLastError());
 	}
 	double bytes = (double)n * (run.keys ? (run.keys64 ? 8.0 : 4.0) : 0.0);
@@ -787,8 +829,8 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 	{
 		ProfileScope prof("seg_reduce", stream, bytes);
 		hipLaunchKernelGGL((SegReduceKernel<KeyT, NP, SG>), dim3((unsigned)ntiles), dim3(kRedBlock), 0, stream,
-		                   (const KeyT *)run.keys, n, pay, cd, od, ko, tile_prefix, lead_state, tail_state, tail_gid,
-		                   ntiles);
+		                   (const KeyT *)run.keys, n, pay, cd, od, ko, tile_prefix, tile_carry, lead_state, tail_state,
+		                   tail_gid, ntiles);
 		VGPU_HIP(hipGetLastError());
 	}
 	{
