@@ -509,35 +509,109 @@ __global__ void __launch_bounds__(kRedBlock)
 	(void)lane;
 }
 
-//! Count-only reduce (every aggregate is a COUNT): a run-length encoding of the sorted keys. Each thread
-//! reads 32 consecutive keys straight into registers (vector loads; the key before them is one extra load
-//! that hits the neighbour's cache line), flags the rows whose key differs from the row before, and the
-//! tile ranks its flags (block scan, then decoupled look-back over per-tile head counts). Each group's key
-//! and first row are written; RleCountKernel turns first rows into counts. No aggregate state is scanned
-//! or stitched across tiles.
-constexpr int kRleItems = 32;
-constexpr int kRleTile = kRedBlock * kRleItems;
-
+//! Group heads per reduce tile (the same rows and the same head test as SegReduceKernel: row r starts a
+//! group when r == 0 or its key differs from row r - 1's; without keys only row 0 does). One wave per
+//! tile, each lane 32 consecutive keys read with vector loads; no barriers.
+constexpr int kHeadLaneRows = 32;
 template <typename KeyT>
-__global__ void __launch_bounds__(kRedBlock)
-    RleHeadsKernel(const KeyT *keys, uint64_t n, KeyOutDesc ko, u64 *starts, u64 *tile_state, uint32_t *ticket,
-                   u64 *groups_out, uint64_t ntiles) {
-	__shared__ uint32_t s_tile;
-	__shared__ u64 s_scan[kRedWaves + 1];
-	__shared__ u64 s_hb;
-	const int tid = threadIdx.x;
-	const int lane = tid & (kWave - 1);
-	const int wave = tid / kWave;
-	if (tid == 0) {
-		s_tile = FetchAddAgent(ticket, 1u);
+__global__ void __launch_bounds__(256) HeadCountKernel(const KeyT *keys, uint64_t n, uint64_t tile_rows,
+                                                      uint64_t ntiles, uint32_t *counts) {
+	const uint64_t tile = (uint64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+	if (tile >= ntiles) {
+		return;
 	}
-	__syncthreads();
-	const uint32_t tile = s_tile;
-	const uint64_t row0 = (uint64_t)tile * kRleTile + (uint64_t)tid * kRleItems;
-	KeyT k[kRleItems];
-	if (row0 + kRleItems <= n && (reinterpret_cast<uintptr_t>(keys) & 15) == 0) {
+	const int lane = LaneId();
+	u64 hc = 0;
+	if (keys == nullptr) {
+		hc = (tile == 0 && lane == 0 && n > 0) ? 1 : 0;
+	} else {
+		// tile_rows = 64 lanes x kHeadLaneRows rows
+		const uint64_t row0 = tile * tile_rows + (uint64_t)lane * kHeadLaneRows;
+		if (row0 < n) {
+			u64 kreg[kHeadLaneRows];
+			LoadRun<kHeadLaneRows, KeyT>(keys, row0, n, kreg);
+			KeyT before = row0 > 0 ? keys[row0 - 1] : (KeyT)0;
+#pragma unroll
+			for (int i = 0; i < kHeadLaneRows; i++) {
+				const KeyT k = (KeyT)kreg[i];
+				hc += (row0 + i < n && (row0 + i == 0 || k != before)) ? 1 : 0;
+				before = k;
+			}
+		}
+	}
+	for (int off = kWave / 2; off > 0; off >>= 1) {
+		hc += __shfl_xor(hc, off);
+	}
+	if (lane == 0) {
+		counts[tile] = (uint32_t)hc;
+	}
+}
+
+
+//! Level 1: exclusive prefix of the head counts inside each run of kScanTiles tiles, and each run's total.
+__global__ void __launch_bounds__(kScanBlock) TilePrefixKernel(const uint32_t *counts, uint64_t ntiles,
+                                                               u64 *prefix, u64 *block_totals) {
+	__shared__ u64 s_scan[kScanBlock / kWave + 1];
+	const uint64_t t0 = (uint64_t)blockIdx.x * kScanTiles + (uint64_t)threadIdx.x * kScanPerThread;
+	uint32_t c[kScanPerThread];
+	u64 mine = 0;
+#pragma unroll
+	for (int i = 0; i < kScanPerThread; i++) {
+		c[i] = t0 + i < ntiles ? counts[t0 + i] : 0u;
+		mine += c[i];
+	}
+	u64 total;
+	u64 run = BlockExclusiveScan<u64, kScanBlock>(mine, s_scan, &total);
+#pragma unroll
+	for (int i = 0; i < kScanPerThread; i++) {
+		if (t0 + i < ntiles) {
+			prefix[t0 + i] = run;
+		}
+		run += c[i];
+	}
+	if (threadIdx.x == 0) {
+		block_totals[blockIdx.x] = total;
+	}
+}
+
+//! Level 2 (one block): block_totals becomes each run's exclusive carry; *groups_out the total.
+__global__ void __launch_bounds__(kScanBlock) TileCarryKernel(u64 *block_totals, uint64_t nblocks, u64 *groups_out) {
+	__shared__ u64 s_scan[kScanBlock / kWave + 1];
+	u64 carry = 0;
+	for (uint64_t base = 0; base < nblocks; base += kScanBlock) {
+		const uint64_t i = base + threadIdx.x;
+		const u64 v = i < nblocks ? block_totals[i] : 0ull;
+		u64 total;
+		const u64 excl = BlockExclusiveScan<u64, kScanBlock>(v, s_scan, &total);
+		if (i < nblocks) {
+			block_totals[i] = carry + excl;
+		}
+		carry += total;
+	}
+	if (threadIdx.x == 0) {
+		*groups_out = carry;
+	}
+}
+
+//! Count-only reduce (every aggregate is a COUNT): a run-length encoding of the sorted keys. One wave per
+//! 2,048-row tile (the head-count tiles), each lane 32 consecutive keys in registers (vector loads; the
+//! key before them is one extra load that hits the neighbour's cache line); the tile's first group id comes
+//! from the head-count scan, so no tile takes a ticket or waits on another. Each group's key and first row
+//! are written; RleCountKernel turns first rows into counts.
+template <typename KeyT>
+__global__ void __launch_bounds__(256) RleHeadsKernel(const KeyT *keys, uint64_t n, KeyOutDesc ko, u64 *starts,
+                                                     const u64 *tile_prefix, const u64 *tile_carry,
+                                                     uint64_t ntiles) {
+	const uint64_t tile = (uint64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+	if (tile >= ntiles) {
+		return;
+	}
+	const int lane = LaneId();
+	const uint64_t row0 = tile * ((uint64_t)kWave * kHeadLaneRows) + (uint64_t)lane * kHeadLaneRows;
+	KeyT k[kHeadLaneRows];
+	if (row0 + kHeadLaneRows <= n && (reinterpret_cast<uintptr_t>(keys) & 15) == 0) {
 		// 32 consecutive keys: 128 B (u32) or 256 B (u64), 16-byte aligned (row0 is a multiple of 32)
-		constexpr int kVecs = (int)(kRleItems * sizeof(KeyT) / 16);
+		constexpr int kVecs = (int)(kHeadLaneRows * sizeof(KeyT) / 16);
 		const uint4 *src = reinterpret_cast<const uint4 *>(keys + row0);
 		uint4 v[kVecs];
 #pragma unroll
@@ -547,50 +621,37 @@ __global__ void __launch_bounds__(kRedBlock)
 		__builtin_memcpy(k, v, sizeof(k));
 	} else {
 #pragma unroll
-		for (int i = 0; i < kRleItems; i++) {
+		for (int i = 0; i < kHeadLaneRows; i++) {
 			k[i] = row0 + i < n ? keys[row0 + i] : (KeyT)0;
 		}
 	}
 	KeyT before = row0 > 0 && row0 <= n ? keys[row0 - 1] : (KeyT)0;
 	uint32_t heads = 0; // bit i = row0 + i starts a group
-	u64 hc = 0;
 #pragma unroll
-	for (int i = 0; i < kRleItems; i++) {
+	for (int i = 0; i < kHeadLaneRows; i++) {
 		const bool h = row0 + i < n && (row0 + i == 0 || k[i] != before);
 		before = k[i];
 		heads |= h ? (1u << i) : 0u;
-		hc += h ? 1 : 0;
 	}
-	u64 H;
-	const u64 hb_excl = BlockExclusiveScan<u64, kRedBlock>(hc, s_scan, &H);
-	if (tid == 0) {
-		StoreAgent(tile_state + tile, LbPack(tile == 0 ? kLbInclusive : kLbAggregate, H));
-	}
-	if (wave == 0) {
-		const u64 excl = tile == 0 ? 0ull : LookbackExclusiveWave(tile_state, tile);
-		if (lane == 0) {
-			if (tile > 0) {
-				StoreAgent(tile_state + tile, LbPack(kLbInclusive, excl + H));
-			}
-			s_hb = excl;
+	const u64 hc = (u64)__popc(heads);
+	u64 incl = hc;
+	for (int off = 1; off < kWave; off <<= 1) {
+		const u64 up = __shfl_up(incl, off);
+		if (lane >= off) {
+			incl += up;
 		}
 	}
-	__syncthreads();
-	u64 gid = s_hb + hb_excl;
+	u64 gid = tile_prefix[tile] + tile_carry[tile / kScanTiles] + incl - hc;
 #pragma unroll
-	for (int i = 0; i < kRleItems; i++) {
+	for (int i = 0; i < kHeadLaneRows; i++) {
 		if ((heads >> i) & 1u) {
 			WriteKeys(ko, (u64)k[i], gid);
 			starts[gid] = row0 + i;
 			gid++;
 		}
 	}
-	if (tid == 0 && tile == ntiles - 1) {
-		*groups_out = s_hb + H;
-	}
 }
 
-//! count[g] = first row of group g + 1 (or n) - first row of group g, into every COUNT output.
 __global__ void RleCountKernel(const u64 *starts, const u64 *groups, uint64_t n, AggOutDesc o) {
 	const u64 g_total = *groups;
 	const u64 stride = (u64)gridDim.x * blockDim.x;
@@ -606,15 +667,29 @@ __global__ void RleCountKernel(const u64 *starts, const u64 *groups, uint64_t n,
 template <typename KeyT>
 void LaunchRle(const SortedRun &run, uint64_t n, uint64_t capacity, const AggOutDesc &od, const KeyOutDesc &ko,
                u64 *groups_device, Scratch &scratch, hipStream_t stream) {
-	const uint64_t ntiles = (n + kRleTile - 1) / kRleTile;
-	auto *lb = static_cast<u64 *>(scratch.Alloc((ntiles + 1) * sizeof(u64)));
-	auto *ticket = reinterpret_cast<uint32_t *>(lb + ntiles);
+	constexpr uint64_t kTile = (uint64_t)kWave * kHeadLaneRows;
+	const uint64_t ntiles = (n + kTile - 1) / kTile;
+	const uint64_t nscan = (ntiles + kScanTiles - 1) / kScanTiles;
+	auto *tile_heads = static_cast<uint32_t *>(scratch.Alloc(std::max<uint64_t>(ntiles, 1) * sizeof(uint32_t)));
+	auto *tile_prefix = static_cast<u64 *>(scratch.Alloc(std::max<uint64_t>(ntiles, 1) * sizeof(u64)));
+	auto *tile_carry = static_cast<u64 *>(scratch.Alloc(std::max<uint64_t>(nscan, 1) * sizeof(u64)));
 	auto *starts = static_cast<u64 *>(scratch.Alloc(std::max<uint64_t>(capacity, 1) * sizeof(u64)));
-	FillU64(lb, 0ull, ntiles + 1, stream);
+	if (n == 0) {
+		FillU64(groups_device, 0ull, 1, stream);
+		return;
+	}
 	{
 		ProfileScope prof("seg_count", stream, (double)n * sizeof(KeyT));
-		hipLaunchKernelGGL((RleHeadsKernel<KeyT>), dim3((unsigned)ntiles), dim3(kRedBlock), 0, stream,
-		                   (const KeyT *)run.keys, n, ko, starts, lb, ticket, groups_device, ntiles);
+		hipLaunchKernelGGL((HeadCountKernel<KeyT>), dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, stream,
+		                   (const KeyT *)run.keys, n, kTile, ntiles, tile_heads);
+		VGPU_HIP(hipGetLastError());
+		hipLaunchKernelGGL(TilePrefixKernel, dim3((unsigned)nscan), dim3(kScanBlock), 0, stream, tile_heads, ntiles,
+		                   tile_prefix, tile_carry);
+		VGPU_HIP(hipGetLastError());
+		hipLaunchKernelGGL(TileCarryKernel, dim3(1), dim3(kScanBlock), 0, stream, tile_carry, nscan, groups_device);
+		VGPU_HIP(hipGetLastError());
+		hipLaunchKernelGGL((RleHeadsKernel<KeyT>), dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, stream,
+		                   (const KeyT *)run.keys, n, ko, starts, tile_prefix, tile_carry, ntiles);
 		VGPU_HIP(hipGetLastError());
 	}
 	const unsigned grid = (unsigned)std::min<uint64_t>((std::max<uint64_t>(capacity, 1) + 255) / 256, 8192);
@@ -708,90 +783,6 @@ __global__ void __launch_bounds__(kFixBlock) TailFinishKernel(CompDesc d, AggOut
 			u = (u / kFixBlock + 1) * kFixBlock; // first tile of the next chunk
 		}
 		WriteGroup(o, acc, tail_gid[T]);
-	}
-}
-
-//! Group heads per reduce tile (the same rows and the same head test as SegReduceKernel: row r starts a
-//! group when r == 0 or its key differs from row r - 1's; without keys only row 0 does). One wave per
-//! tile, each lane 32 consecutive keys read with vector loads; no barriers.
-constexpr int kHeadLaneRows = 32;
-template <typename KeyT>
-__global__ void __launch_bounds__(256) HeadCountKernel(const KeyT *keys, uint64_t n, uint64_t tile_rows,
-                                                      uint64_t ntiles, uint32_t *counts) {
-	const uint64_t tile = (uint64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
-	if (tile >= ntiles) {
-		return;
-	}
-	const int lane = LaneId();
-	u64 hc = 0;
-	if (keys == nullptr) {
-		hc = (tile == 0 && lane == 0 && n > 0) ? 1 : 0;
-	} else {
-		// tile_rows = 64 lanes x kHeadLaneRows rows
-		const uint64_t row0 = tile * tile_rows + (uint64_t)lane * kHeadLaneRows;
-		if (row0 < n) {
-			u64 kreg[kHeadLaneRows];
-			LoadRun<kHeadLaneRows, KeyT>(keys, row0, n, kreg);
-			KeyT before = row0 > 0 ? keys[row0 - 1] : (KeyT)0;
-#pragma unroll
-			for (int i = 0; i < kHeadLaneRows; i++) {
-				const KeyT k = (KeyT)kreg[i];
-				hc += (row0 + i < n && (row0 + i == 0 || k != before)) ? 1 : 0;
-				before = k;
-			}
-		}
-	}
-	for (int off = kWave / 2; off > 0; off >>= 1) {
-		hc += __shfl_xor(hc, off);
-	}
-	if (lane == 0) {
-		counts[tile] = (uint32_t)hc;
-	}
-}
-
-
-//! Level 1: exclusive prefix of the head counts inside each run of kScanTiles tiles, and each run's total.
-__global__ void __launch_bounds__(kScanBlock) TilePrefixKernel(const uint32_t *counts, uint64_t ntiles,
-                                                               u64 *prefix, u64 *block_totals) {
-	__shared__ u64 s_scan[kScanBlock / kWave + 1];
-	const uint64_t t0 = (uint64_t)blockIdx.x * kScanTiles + (uint64_t)threadIdx.x * kScanPerThread;
-	uint32_t c[kScanPerThread];
-	u64 mine = 0;
-#pragma unroll
-	for (int i = 0; i < kScanPerThread; i++) {
-		c[i] = t0 + i < ntiles ? counts[t0 + i] : 0u;
-		mine += c[i];
-	}
-	u64 total;
-	u64 run = BlockExclusiveScan<u64, kScanBlock>(mine, s_scan, &total);
-#pragma unroll
-	for (int i = 0; i < kScanPerThread; i++) {
-		if (t0 + i < ntiles) {
-			prefix[t0 + i] = run;
-		}
-		run += c[i];
-	}
-	if (threadIdx.x == 0) {
-		block_totals[blockIdx.x] = total;
-	}
-}
-
-//! Level 2 (one block): block_totals becomes each run's exclusive carry; *groups_out the total.
-__global__ void __launch_bounds__(kScanBlock) TileCarryKernel(u64 *block_totals, uint64_t nblocks, u64 *groups_out) {
-	__shared__ u64 s_scan[kScanBlock / kWave + 1];
-	u64 carry = 0;
-	for (uint64_t base = 0; base < nblocks; base += kScanBlock) {
-		const uint64_t i = base + threadIdx.x;
-		const u64 v = i < nblocks ? block_totals[i] : 0ull;
-		u64 total;
-		const u64 excl = BlockExclusiveScan<u64, kScanBlock>(v, s_scan, &total);
-		if (i < nblocks) {
-			block_totals[i] = carry + excl;
-		}
-		carry += total;
-	}
-	if (threadIdx.x == 0) {
-		*groups_out = carry;
 	}
 }
 
