@@ -84,30 +84,10 @@ __device__ __forceinline__ u64 LbPack(u64 flag, u64 value) {
 	return (flag << kLbFlagShift) | (value & kLbValueMask);
 }
 
-//! Exclusive prefix of `tile` over the 64-bit look-back array `state` (one word per tile), spinning on
-//! predecessors that have not published yet. Called by ONE lane. Tiles are handed out in launch order by
-//! an atomic ticket, so every predecessor is already resident and will publish: the spin terminates.
-__device__ __forceinline__ u64 LookbackExclusive(const u64 *state, int64_t tile) {
-	u64 exclusive = 0;
-	int64_t t = tile - 1;
-	while (t >= 0) {
-		u64 word = LoadAgent(state + t);
-		u64 flag = word >> kLbFlagShift;
-		if (flag == 0) {
-			__builtin_amdgcn_s_sleep(1);
-			continue;
-		}
-		exclusive += word & kLbValueMask;
-		if (flag == kLbInclusive) {
-			break;
-		}
-		t--;
-	}
-	return exclusive;
-}
-
-//! Same as LookbackExclusive, but a whole wave walks 64 predecessors per step (lane i reads tile-1-i), so a
-//! long chain of aggregate-only predecessors costs one round trip per 64 tiles instead of one per tile.
+//! Exclusive prefix of `tile` over the 64-bit look-back array `state` (one word per tile). A whole wave walks
+//! 64 predecessors per step (lane i reads tile-1-i), so a long chain of aggregate-only predecessors costs one
+//! round trip per 64 tiles instead of one per tile. Tiles are handed out in launch order by an atomic ticket,
+//! so every predecessor is already resident and will publish: the spin terminates.
 //! Every lane of the wave must call it; all lanes return the same value.
 __device__ __forceinline__ u64 LookbackExclusiveWave(const u64 *state, int64_t tile) {
 	const int lane = LaneId();
