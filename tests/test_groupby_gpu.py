@@ -233,3 +233,33 @@ def test_groupby_under_sort_autotuner(dev):
             assert r["candidates"] >= 4 and r["ms"] > 0
     finally:
         capi.sort_autotune(False)
+
+
+def test_groupby_many_tiles_carries(dev):
+    """20M rows = 9,766 reduce tiles, so the tiles' group offsets come from more than one first-level scan
+    block (4,096 tiles each) plus carries; Zipf keys put long groups across tile and scan-block edges."""
+    rng = np.random.default_rng(29)
+    n = 20_000_000
+    ranks = rng.zipf(1.2, n)
+    ranks = ranks[ranks <= 200_000]
+    key = (rng.permutation(200_000).astype(np.int64) * 3 - 100_000)[ranks - 1]
+    n = len(key)
+    v1 = rng.integers(-1_000_000, 1_000_001, n, dtype=np.int64)
+    v2 = rng.standard_normal(n)
+    _check(dev, key, v1, v2)
+
+
+def test_groupby_count_only_many_tiles(dev):
+    """COUNT-only GROUP BY (the run-length path) over 20M rows: keys and counts equal numpy's unique."""
+    import torch
+    rng = np.random.default_rng(31)
+    n = 20_000_000
+    key = rng.integers(-3_000_000, 3_000_000, n, dtype=np.int64)
+    key[: n // 4] = 17  # one group spanning thousands of tiles
+    k = torch.from_numpy(key).to(dev)
+    keys, aggs, g = capi.group_by([k], [], [(capi.COUNT_STAR, -1, capi.INT64)], n)
+    uk, uc = np.unique(key, return_counts=True)
+    assert g == len(uk)
+    gk = keys[0].cpu().numpy()[:g]
+    np.testing.assert_array_equal(gk, uk)
+    np.testing.assert_array_equal(aggs[0].cpu().numpy()[:g], uc)
