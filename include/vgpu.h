@@ -122,12 +122,12 @@ typedef struct vgpu_join_agg {
 	                        MIN/MAX -> input type (BIGINT/DOUBLE) */
 } vgpu_join_agg;
 
-/* Global aggregates over A JOIN B ON akey = bkey without materializing the join. When one side only
- * contributes its row count, is the larger one, and every aggregate is a COUNT or an integer SUM/AVG of one
- * column of the other side, that other side becomes a direct-address table (per key: row count, payload sum)
- * and the larger side is read once against it (partitioned by key range when the table outgrows L2);
- * otherwise both sides are radix-sorted, reduced to distinct keys and merge-path matched. Both forms give
- * the same results. Replaces a GROUP_BY_AGGREGATE without keys over
+/* Global aggregates over A JOIN B ON akey = bkey without materializing the join. When every aggregate is a
+ * COUNT or an integer SUM/AVG of one column of the smaller side and/or one column of the larger side, the
+ * smaller side becomes a direct-address table (per key: row count, payload sum) and the larger side is read
+ * once against it (partitioned by key range when the table outgrows L2 or the larger side's payload is
+ * summed); otherwise both sides are radix-sorted, reduced to distinct keys and merge-path matched. Both
+ * forms give the same results. Replaces a GROUP_BY_AGGREGATE without keys over
  * a HASH_JOIN (cuda_engine/src/operators/gpu_hash_join.cu:138-267 followed by gpu_groupby.cu). results:
  * naggs x 16 host bytes, each the aggregate in its output_type (HUGEINT = {lower, upper}). Synchronizes. */
 int vgpu_join_aggregate(const vgpu_column *akey, const vgpu_column *avals, int32_t navals, const vgpu_column *bkey,

@@ -255,3 +255,56 @@ def test_join_partitioned_probe_form(dev, dtype, skew, flip, wide):
         res = capi.join_aggregate(_dev(build, dev), [_dev(payload, dev)], _dev(probe, dev), [],
                                   [(capi.SUM, 0, 0, capi.HUGEINT), (capi.COUNT_STAR, -1, -1, capi.INT64)])
     assert res == [total, count]
+
+
+def _brute_probe_side_sums(probe, py, build, bpay):
+    """COUNT(*), SUM(build payload), SUM(probe payload) of probe JOIN build (exact Python ints)."""
+    ub, cb = np.unique(build, return_counts=True)
+    sums_b = {}
+    if bpay is not None:
+        order = np.argsort(build, kind="stable")
+        sb = np.add.reduceat(bpay[order].astype(object), np.searchsorted(build[order], ub)) if len(ub) else []
+        sums_b = dict(zip(ub.tolist(), [int(x) for x in sb]))
+    pos = np.searchsorted(ub, probe)
+    pos = np.minimum(pos, len(ub) - 1)
+    hit = ub[pos] == probe
+    c = np.where(hit, cb[pos], 0).astype(np.int64)
+    count = int(c.sum())
+    sum_y = int(np.dot(py.astype(object), c.astype(object))) if py is not None else 0
+    sum_b = sum(sums_b.get(k, 0) for k in probe[hit].tolist()) if bpay is not None else 0
+    return count, sum_b, sum_y
+
+
+@pytest.mark.parametrize("ktype,ptype,wide_build,flip", [(np.int64, np.int64, False, False),
+                                                         (np.int32, np.int32, False, True),
+                                                         (np.int64, np.int16, True, False),
+                                                         (np.int16, np.int64, False, True)])
+def test_join_probe_form_larger_side_sums(dev, ktype, ptype, wide_build, flip):
+    """SUM/AVG of a column of the larger (probe) side, with and without a SUM of the build side's payload:
+    the probe payload travels with its key through the key-range partition and each probe row adds its
+    payload times its key's build row count (64 x 64 -> 128-bit products). Duplicate and negative build
+    keys; both table forms (16-bit slices in LDS, 64-bit in L2 for wide build payloads); the probe side as
+    A or as B. Exact against brute force."""
+    rng = np.random.default_rng(int(np.dtype(ktype).itemsize) * 10 + int(np.dtype(ptype).itemsize) + 3 * flip)
+    lo, hi = (-3000, 9000) if ktype == np.int16 else (-200_000, 600_000)
+    build = rng.integers(lo // 2, hi // 2, 150_000).astype(ktype)
+    bpay = rng.integers(-10 ** 6 if wide_build else 0, 10 ** 6 if wide_build else 500, len(build)).astype(np.int32)
+    probe = rng.integers(lo, hi, 1_200_000).astype(ktype)
+    ylim = {np.int16: 30_000, np.int32: 2 ** 31 - 1, np.int64: 2 ** 62}[ptype]
+    py = rng.integers(-ylim, ylim, len(probe)).astype(ptype)
+    count, sum_b, sum_y = _brute_probe_side_sums(probe.astype(np.int64), py, build.astype(np.int64), bpay)
+    if not flip:  # A = probe (larger) with y, B = build with payload
+        res = capi.join_aggregate(_dev(probe, dev), [_dev(py, dev)], _dev(build, dev), [_dev(bpay, dev)],
+                                  [(capi.SUM, 0, 0, capi.HUGEINT), (capi.COUNT_STAR, -1, -1, capi.INT64),
+                                   (capi.SUM, 1, 0, capi.HUGEINT), (capi.AVG, 0, 0, capi.FLOAT64)])
+        only_y = capi.join_aggregate(_dev(probe, dev), [_dev(py, dev)], _dev(build, dev), [],
+                                     [(capi.SUM, 0, 0, capi.HUGEINT)])
+    else:         # A = build with payload, B = probe (larger) with y
+        res = capi.join_aggregate(_dev(build, dev), [_dev(bpay, dev)], _dev(probe, dev), [_dev(py, dev)],
+                                  [(capi.SUM, 1, 0, capi.HUGEINT), (capi.COUNT_STAR, -1, -1, capi.INT64),
+                                   (capi.SUM, 0, 0, capi.HUGEINT), (capi.AVG, 1, 0, capi.FLOAT64)])
+        only_y = capi.join_aggregate(_dev(build, dev), [], _dev(probe, dev), [_dev(py, dev)],
+                                     [(capi.SUM, 1, 0, capi.HUGEINT)])
+    assert res[0] == sum_y and res[1] == count and res[2] == sum_b
+    assert abs(res[3] - sum_y / count) <= 1e-12 * abs(sum_y / count)
+    assert only_y == [sum_y]

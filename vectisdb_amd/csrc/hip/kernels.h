@@ -270,18 +270,20 @@ void RangeProbeAgg(int key_type, const void *probe_keys, const uint64_t *seg, co
                    const uint64_t *index_count, const uint64_t *index_sum, uint64_t nd, const uint64_t *dense,
                    uint64_t dense_min, uint64_t dense_span, uint64_t *acc, hipStream_t stream);
 
-//! SUM(payload) and COUNT(*) of one probe column against a direct-address table (see BuildDenseIndex), for a
+//! SUM/COUNT of an equi-join of one probe column against a direct-address table (see BuildDenseIndex), for a
 //! probe side much larger than the table: the probe keys are first grouped by key range (256 buckets,
 //! batch by batch) so the probe reads one bucket's slice of the table at a time, from LDS when the table
-//! has a 16-bit form. acc = {matched build rows, sum lo, sum hi}. About n * 4.03 bytes of scratch (plus
-//! span * 2 for the 16-bit form).
-void PartitionedProbeAgg(int key_type, const void *probe_keys, uint64_t n, const uint64_t *dense,
-                         uint64_t dense_min, uint64_t dense_span, uint64_t *acc, Scratch &scratch,
-                         hipStream_t stream);
+//! has a 16-bit form. acc (5 words) = {matched build rows, build payload sum lo, hi, and -- with a probe
+//! payload (an integer column of payload_width bytes, or nullptr) -- the sum of each probe row's payload
+//! times its key's build row count, lo, hi}. About n * 4.03 bytes of scratch (n * 8 more with a probe
+//! payload, span * 2 for the 16-bit form).
+void PartitionedProbeAgg(int key_type, const void *probe_keys, uint64_t n, const void *probe_payload,
+                         int payload_width, const uint64_t *dense, uint64_t dense_min, uint64_t dense_span,
+                         uint64_t *acc, Scratch &scratch, hipStream_t stream);
 //! The same with the table's 16-bit form given (or nullptr: the probe reads the 64-bit table from L2).
-void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, const uint64_t *dense,
-                             const uint16_t *dense16, uint64_t dense_min, uint64_t dense_span, uint64_t *acc,
-                             Scratch &scratch, hipStream_t stream);
+void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, const void *probe_payload,
+                             int payload_width, const uint64_t *dense, const uint16_t *dense16, uint64_t dense_min,
+                             uint64_t dense_span, uint64_t *acc, Scratch &scratch, hipStream_t stream);
 
 //! Direct-address table (BuildDenseIndex's format) straight from a sorted build side: keys are the packed
 //! sort keys of one column (key = ((packed + pack_min) ^ flip) as sign-extended value; `flip` is the top

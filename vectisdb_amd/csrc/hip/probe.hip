@@ -262,23 +262,42 @@ __global__ void DenseIndexKernel(const K *keys, const u64 *count, const u64 *sum
 constexpr int kPartBuckets = 256;
 constexpr int kPartBlock = 1024;      // 16 waves
 constexpr int kPartItems = 8;         // probe: keys per thread per loop step
-constexpr int kScatterItems = 32;     // keys per thread per batch
-constexpr u64 kPartBatch = (u64)kPartBlock * kScatterItems; // 32768 rows
+constexpr int kScatterItems = 32;     // keys per thread per batch (keys only: 32768-row batches)
+constexpr int kScatterItemsY = 16;    // with a probe payload (16384-row batches: 8-byte values fill the stage)
 constexpr unsigned kRunChunk = 2048;  // runs (batches) per LDS scan of run lengths
 
-template <typename K>
+//! Raw integer of `width` bytes at row j, sign-extended (width 1 is BOOLEAN: zero-extended).
+__device__ __forceinline__ long long LoadIntAt(const void *p, int width, u64 j) {
+	switch (width) {
+	case 1:
+		return ((const uint8_t *)p)[j];
+	case 2:
+		return ((const int16_t *)p)[j];
+	case 4:
+		return ((const int32_t *)p)[j];
+	default:
+		return ((const long long *)p)[j];
+	}
+}
+
+//! Batch g = rows [g * ITEMS * 1024, ...): kept keys grouped by bucket, written over the batch's own positions
+//! of `part` (table offsets) and, with HASY, of `part_y` (the probe payload as int64, same order).
+template <typename K, int ITEMS, bool HASY>
 __global__ void __launch_bounds__(kPartBlock) PartBatchKernel(const K *keys, u64 n, u64 dmin, u64 span, int shift,
-                                                             uint32_t *part, uint32_t *tab) {
+                                                             const void *y, int ywidth, uint32_t *part,
+                                                             long long *part_y, uint32_t *tab) {
+	constexpr u64 kBatch = (u64)kPartBlock * ITEMS;
 	__shared__ uint32_t s_cnt[kPartBuckets];   // batch count per bucket
 	__shared__ uint32_t s_start[kPartBuckets]; // batch-local start per bucket
 	__shared__ uint32_t s_scan[kPartBlock / kWave + 1];
-	__shared__ uint32_t s_stage[kPartBatch];
+	__shared__ u64 s_stage[kBatch * (HASY ? 8 : 4) / 8]; // offsets, then (HASY) payloads
+	uint32_t *s_off = reinterpret_cast<uint32_t *>(s_stage);
 	const int tid = threadIdx.x;
-	const u64 base = (u64)blockIdx.x * kPartBatch;
-	uint32_t off[kScatterItems], rank[kScatterItems];
-	bool valid[kScatterItems];
+	const u64 base = (u64)blockIdx.x * kBatch;
+	uint32_t off[ITEMS], rank[ITEMS];
+	bool valid[ITEMS];
 #pragma unroll
-	for (int j = 0; j < kScatterItems; j++) {
+	for (int j = 0; j < ITEMS; j++) {
 		const u64 r = base + (u64)j * kPartBlock + tid;
 		const K x = r < n ? keys[r] : (K)0;
 		const u64 o = (u64)(long long)x - dmin; // wraps past span when below the table
@@ -290,7 +309,7 @@ __global__ void __launch_bounds__(kPartBlock) PartBatchKernel(const K *keys, u64
 	}
 	__syncthreads();
 #pragma unroll
-	for (int j = 0; j < kScatterItems; j++) {
+	for (int j = 0; j < ITEMS; j++) {
 		rank[j] = valid[j] ? atomicAdd(&s_cnt[off[j] >> shift], 1u) : 0u;
 	}
 	__syncthreads();
@@ -307,14 +326,33 @@ __global__ void __launch_bounds__(kPartBlock) PartBatchKernel(const K *keys, u64
 	}
 	__syncthreads();
 #pragma unroll
-	for (int j = 0; j < kScatterItems; j++) {
+	for (int j = 0; j < ITEMS; j++) {
 		if (valid[j]) {
-			s_stage[s_start[off[j] >> shift] + rank[j]] = off[j];
+			s_off[s_start[off[j] >> shift] + rank[j]] = off[j];
 		}
 	}
 	__syncthreads();
 	for (uint32_t i = tid; i < total; i += kPartBlock) {
-		part[base + i] = s_stage[i];
+		part[base + i] = s_off[i];
+	}
+	if constexpr (HASY) {
+		long long *s_y = reinterpret_cast<long long *>(s_stage);
+		long long yv[ITEMS];
+#pragma unroll
+		for (int j = 0; j < ITEMS; j++) {
+			yv[j] = valid[j] ? LoadIntAt(y, ywidth, base + (u64)j * kPartBlock + tid) : 0ll;
+		}
+		__syncthreads(); // every offset has left the stage
+#pragma unroll
+		for (int j = 0; j < ITEMS; j++) {
+			if (valid[j]) {
+				s_y[s_start[off[j] >> shift] + rank[j]] = yv[j];
+			}
+		}
+		__syncthreads();
+		for (uint32_t i = tid; i < total; i += kPartBlock) {
+			part_y[base + i] = s_y[i];
+		}
 	}
 }
 
@@ -349,19 +387,27 @@ __global__ void __launch_bounds__(256) NarrowDenseKernel(const u64 *dense, u64 s
 	}
 }
 
+//! y x c as int128 for a signed y and a count c < 2^63, added to (lo, hi).
+__device__ __forceinline__ void AddMul128(u64 &lo, u64 &hi, long long y, u64 c) {
+	const u64 plo = (u64)y * c;
+	const u64 phi = __umul64hi((u64)y, c) - (y < 0 ? c : 0ull); // (y + 2^64) * c overcounts 2^64 * c
+	Add128(lo, hi, plo, phi);
+}
+
 //! Block i probes split i % kSliceSplits (a range of batches) of bucket i / kSliceSplits. LDS: the bucket's
 //! 16-bit table slice is copied to LDS once and every key is looked up there; otherwise the 64-bit table is
-//! read through L2.
-template <bool LDS>
-__global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *part, const uint32_t *tab,
-                                                             u64 nbatches, const uint16_t *dense16,
-                                                             const u64 *dense, u64 span, int shift,
-                                                             u64 *partials) {
+//! read through L2. Sums per block: matched build rows, the build payload sum, and (HASY) the probe payload
+//! times its key's build row count.
+template <bool LDS, bool HASY>
+__global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *part, const long long *part_y,
+                                                             const uint32_t *tab, u64 nbatches, u64 batch,
+                                                             const uint16_t *dense16, const u64 *dense, u64 span,
+                                                             int shift, u64 *partials) {
 	__shared__ uint16_t s_slice[LDS ? kSliceEntries : 1];
 	__shared__ uint32_t s_pre[kRunChunk + 1]; // exclusive prefix of run lengths
 	__shared__ uint32_t s_first[kRunChunk];   // each run's start inside its batch
 	__shared__ uint32_t s_scan[kPartBlock / kWave + 1];
-	__shared__ u64 s_red[3][kPartBlock / kWave];
+	__shared__ u64 s_red[5][kPartBlock / kWave];
 	const int tid = threadIdx.x;
 	const unsigned b = blockIdx.x / kSliceSplits;
 	const unsigned split = blockIdx.x % kSliceSplits;
@@ -376,7 +422,7 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
 	const u64 per = (nbatches + kSliceSplits - 1) / kSliceSplits;
 	const u64 g_begin = (u64)split * per < nbatches ? (u64)split * per : nbatches;
 	const u64 g_end = g_begin + per < nbatches ? g_begin + per : nbatches;
-	u64 c = 0, slo = 0, shi = 0;
+	u64 c = 0, slo = 0, shi = 0, ylo = 0, yhi = 0;
 	for (u64 g0 = g_begin; g0 < g_end; g0 += kRunChunk) {
 		const unsigned nruns = (unsigned)(g_end - g0 < kRunChunk ? g_end - g0 : kRunChunk);
 		// run lengths of this chunk of batches: two runs per thread, scanned
@@ -415,27 +461,41 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
 				}
 			}
 			uint32_t key[kPartItems];
+			long long yv[kPartItems];
 #pragma unroll
 			for (int k = 0; k < kPartItems; k++) {
 				const uint32_t v = v0 + k * kWave;
 				key[k] = (uint32_t)e0; // any offset of this bucket
+				yv[k] = 0;
 				if (v < total) {
 					while (s_pre[lo + 1] <= v) { // v < total = s_pre[nruns], so lo stays below nruns
 						lo++;
 					}
-					key[k] = part[(g0 + lo) * kPartBatch + s_first[lo] + (v - s_pre[lo])];
+					const u64 at = (g0 + lo) * batch + s_first[lo] + (v - s_pre[lo]);
+					key[k] = part[at];
+					if constexpr (HASY) {
+						yv[k] = part_y[at];
+					}
 				}
 			}
 #pragma unroll
 			for (int k = 0; k < kPartItems; k++) {
 				if (v0 + k * kWave < total) {
+					u64 ec;
+					long long es;
 					if constexpr (LDS) {
 						const uint32_t e = s_slice[key[k] - (uint32_t)e0];
-						c += e >> kNarrowSumBits;
-						const long long sv = (long long)((int)(e << (32 - kNarrowSumBits)) >> (32 - kNarrowSumBits));
-						Add128(slo, shi, (u64)sv, sv < 0 ? ~0ull : 0ull);
+						ec = e >> kNarrowSumBits;
+						es = (long long)((int)(e << (32 - kNarrowSumBits)) >> (32 - kNarrowSumBits));
 					} else {
-						AddDense(dense[key[k]], c, slo, shi);
+						const u64 e = dense[key[k]];
+						ec = e >> kDenseSumBits;
+						es = (long long)(e << (64 - kDenseSumBits)) >> (64 - kDenseSumBits);
+					}
+					c += ec;
+					Add128(slo, shi, (u64)es, es < 0 ? ~0ull : 0ull);
+					if constexpr (HASY) {
+						AddMul128(ylo, yhi, yv[k], ec);
 					}
 				}
 			}
@@ -446,70 +506,98 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
 		const u64 oc = __shfl_xor(c, d);
 		const u64 olo = __shfl_xor(slo, d);
 		const u64 ohi = __shfl_xor(shi, d);
+		const u64 oylo = __shfl_xor(ylo, d);
+		const u64 oyhi = __shfl_xor(yhi, d);
 		c += oc;
 		Add128(slo, shi, olo, ohi);
+		Add128(ylo, yhi, oylo, oyhi);
 	}
 	const int wave = tid / kWave;
 	if ((tid & (kWave - 1)) == 0) {
 		s_red[0][wave] = c;
 		s_red[1][wave] = slo;
 		s_red[2][wave] = shi;
+		s_red[3][wave] = ylo;
+		s_red[4][wave] = yhi;
 	}
 	__syncthreads();
 	if (tid == 0) {
-		u64 tc = 0, tlo = 0, thi = 0;
+		u64 tc = 0, tlo = 0, thi = 0, tylo = 0, tyhi = 0;
 		for (int w = 0; w < kPartBlock / kWave; w++) {
 			tc += s_red[0][w];
 			Add128(tlo, thi, s_red[1][w], s_red[2][w]);
+			Add128(tylo, tyhi, s_red[3][w], s_red[4][w]);
 		}
-		partials[3 * (u64)blockIdx.x] = tc;
-		partials[3 * (u64)blockIdx.x + 1] = tlo;
-		partials[3 * (u64)blockIdx.x + 2] = thi;
+		u64 *out = partials + 5 * (u64)blockIdx.x;
+		out[0] = tc;
+		out[1] = tlo;
+		out[2] = thi;
+		out[3] = tylo;
+		out[4] = tyhi;
 	}
 }
 
 //! One wave adds the block partials (integers: any order gives the same result).
 __global__ void __launch_bounds__(kWave) PartFinalKernel(const u64 *partials, unsigned nparts, u64 *acc) {
-	u64 c = 0, slo = 0, shi = 0;
+	u64 c = 0, slo = 0, shi = 0, ylo = 0, yhi = 0;
 	for (unsigned p = LaneId(); p < nparts; p += kWave) {
-		c += partials[3 * (u64)p];
-		Add128(slo, shi, partials[3 * (u64)p + 1], partials[3 * (u64)p + 2]);
+		const u64 *in = partials + 5 * (u64)p;
+		c += in[0];
+		Add128(slo, shi, in[1], in[2]);
+		Add128(ylo, yhi, in[3], in[4]);
 	}
 	for (int d = kWave / 2; d > 0; d >>= 1) {
 		const u64 oc = __shfl_xor(c, d);
 		const u64 olo = __shfl_xor(slo, d);
 		const u64 ohi = __shfl_xor(shi, d);
+		const u64 oylo = __shfl_xor(ylo, d);
+		const u64 oyhi = __shfl_xor(yhi, d);
 		c += oc;
 		Add128(slo, shi, olo, ohi);
+		Add128(ylo, yhi, oylo, oyhi);
 	}
 	if (LaneId() == 0) {
 		acc[0] = c;
 		acc[1] = slo;
 		acc[2] = shi;
+		acc[3] = ylo;
+		acc[4] = yhi;
 	}
 }
 
 template <typename K>
-void LaunchPartitioned(const void *keys_v, uint64_t n, uint64_t dmin, uint64_t span, int shift, const uint64_t *dense,
-                       const uint16_t *dense16, uint32_t *part, uint32_t *tab, uint64_t *partials, uint64_t *acc,
-                       hipStream_t stream) {
+void LaunchPartitioned(const void *keys_v, uint64_t n, const void *y, int ywidth, uint64_t dmin, uint64_t span,
+                       int shift, const uint64_t *dense, const uint16_t *dense16, uint32_t *part, long long *part_y,
+                       uint32_t *tab, uint64_t batch, uint64_t *partials, uint64_t *acc, hipStream_t stream) {
 	const K *keys = static_cast<const K *>(keys_v);
-	const uint64_t nbatches = (n + kPartBatch - 1) / kPartBatch;
+	const uint64_t nbatches = (n + batch - 1) / batch;
 	{
-		ProfileScope prof("partition_scatter", stream, (double)n * (sizeof(K) + 4));
-		hipLaunchKernelGGL((PartBatchKernel<K>), dim3((unsigned)nbatches), dim3(kPartBlock), 0, stream, keys, (u64)n,
-		                   (u64)dmin, (u64)span, shift, part, tab);
+		ProfileScope prof("partition_scatter", stream, (double)n * (sizeof(K) + 4 + (y ? ywidth + 8 : 0)));
+		if (y != nullptr) {
+			hipLaunchKernelGGL((PartBatchKernel<K, kScatterItemsY, true>), dim3((unsigned)nbatches), dim3(kPartBlock),
+			                   0, stream, keys, (u64)n, (u64)dmin, (u64)span, shift, y, ywidth, part, part_y, tab);
+		} else {
+			hipLaunchKernelGGL((PartBatchKernel<K, kScatterItems, false>), dim3((unsigned)nbatches), dim3(kPartBlock),
+			                   0, stream, keys, (u64)n, (u64)dmin, (u64)span, shift, y, ywidth, part, part_y, tab);
+		}
 		VGPU_HIP(hipGetLastError());
 	}
 	const unsigned nparts = kPartBuckets * kSliceSplits;
 	{
-		ProfileScope prof("partition_probe", stream, (double)n * 4);
-		if (dense16 != nullptr && (1ull << shift) <= kSliceEntries) {
-			hipLaunchKernelGGL(PartProbeKernel<true>, dim3(nparts), dim3(kPartBlock), 0, stream, part, tab,
-			                   (u64)nbatches, dense16, dense, (u64)span, shift, partials);
+		ProfileScope prof("partition_probe", stream, (double)n * (4 + (y ? 8 : 0)));
+		const bool lds = dense16 != nullptr && (1ull << shift) <= kSliceEntries;
+		if (lds && y != nullptr) {
+			hipLaunchKernelGGL((PartProbeKernel<true, true>), dim3(nparts), dim3(kPartBlock), 0, stream, part, part_y,
+			                   tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
+		} else if (lds) {
+			hipLaunchKernelGGL((PartProbeKernel<true, false>), dim3(nparts), dim3(kPartBlock), 0, stream, part, part_y,
+			                   tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
+		} else if (y != nullptr) {
+			hipLaunchKernelGGL((PartProbeKernel<false, true>), dim3(nparts), dim3(kPartBlock), 0, stream, part, part_y,
+			                   tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
 		} else {
-			hipLaunchKernelGGL(PartProbeKernel<false>, dim3(nparts), dim3(kPartBlock), 0, stream, part, tab,
-			                   (u64)nbatches, dense16, dense, (u64)span, shift, partials);
+			hipLaunchKernelGGL((PartProbeKernel<false, false>), dim3(nparts), dim3(kPartBlock), 0, stream, part,
+			                   part_y, tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
 		}
 		VGPU_HIP(hipGetLastError());
 	}
@@ -526,20 +614,6 @@ constexpr uint64_t kMaxRunRows = 64; // a longer run of one key is left to the G
 //! The thread holding a run's first row sums the run (reading past its own rows when the run continues)
 //! and stores the entry; every other entry stays 0. flags bit 0: an entry does not pack; bit 1: a run is
 //! longer than kMaxRunRows (the caller then takes the GROUP BY path).
-//! Raw payload value of row j, sign-extended (width 1 is BOOLEAN: zero-extended).
-__device__ __forceinline__ long long PayloadAt(const void *payload, int pwidth, u64 j) {
-	switch (pwidth) {
-	case 1:
-		return ((const uint8_t *)payload)[j];
-	case 2:
-		return ((const int16_t *)payload)[j];
-	case 4:
-		return ((const int32_t *)payload)[j];
-	default:
-		return ((const long long *)payload)[j];
-	}
-}
-
 template <typename KeyT>
 __global__ void __launch_bounds__(256) DenseFromRunKernel(const KeyT *keys, const void *payload, int pwidth,
                                                          u64 n, u64 pack_min, u64 flip, u64 dmin, u64 span,
@@ -567,7 +641,7 @@ __global__ void __launch_bounds__(256) DenseFromRunKernel(const KeyT *keys, cons
 	}
 #pragma unroll
 	for (int i = 0; i < kRunItems; i++) {
-		pr[i] = (payload != nullptr && row0 + i < n) ? PayloadAt(payload, pwidth, row0 + i) : 0ll;
+		pr[i] = (payload != nullptr && row0 + i < n) ? LoadIntAt(payload, pwidth, row0 + i) : 0ll;
 	}
 	const KeyT before = row0 > 0 ? keys[row0 - 1] : (KeyT)0;
 	unsigned int bad = 0;
@@ -597,7 +671,7 @@ __global__ void __launch_bounds__(256) DenseFromRunKernel(const KeyT *keys, cons
 			u64 j = row0 + kRunItems;
 			for (; j < n && cnt < kMaxRunRows && keys[j] == k; j++) {
 				if (payload != nullptr) {
-					const long long v = PayloadAt(payload, pwidth, j);
+					const long long v = LoadIntAt(payload, pwidth, j);
 					Add128(slo, shi, (u64)v, v < 0 ? ~0ull : 0ull);
 				}
 				cnt++;
@@ -657,9 +731,9 @@ unsigned BuildDenseFromRun(const void *keys, bool keys64, const void *payload, i
 	return host;
 }
 
-void PartitionedProbeAgg(int key_type, const void *probe_keys, uint64_t n, const uint64_t *dense,
-                         uint64_t dense_min, uint64_t dense_span, uint64_t *acc, Scratch &scratch,
-                         hipStream_t stream) {
+void PartitionedProbeAgg(int key_type, const void *probe_keys, uint64_t n, const void *probe_payload,
+                         int payload_width, const uint64_t *dense, uint64_t dense_min, uint64_t dense_span,
+                         uint64_t *acc, Scratch &scratch, hipStream_t stream) {
 	if (dense != nullptr && dense_span > 0 && dense_span <= kPartBuckets * kSliceEntries && n > 0) {
 		// 16-bit table when every entry fits: the probe then reads bucket slices from LDS.
 		auto *dense16 = static_cast<uint16_t *>(scratch.Alloc(dense_span * 2));
@@ -672,20 +746,25 @@ void PartitionedProbeAgg(int key_type, const void *probe_keys, uint64_t n, const
 		unsigned int host = 1;
 		VGPU_HIP(hipMemcpyAsync(&host, nofit, 4, hipMemcpyDeviceToHost, stream));
 		VGPU_HIP(hipStreamSynchronize(stream));
-		PartitionedProbeAggWith(key_type, probe_keys, n, dense, host == 0 ? dense16 : nullptr, dense_min, dense_span,
-		                        acc, scratch, stream);
+		PartitionedProbeAggWith(key_type, probe_keys, n, probe_payload, payload_width, dense,
+		                        host == 0 ? dense16 : nullptr, dense_min, dense_span, acc, scratch, stream);
 		return;
 	}
-	PartitionedProbeAggWith(key_type, probe_keys, n, dense, nullptr, dense_min, dense_span, acc, scratch, stream);
+	PartitionedProbeAggWith(key_type, probe_keys, n, probe_payload, payload_width, dense, nullptr, dense_min,
+	                        dense_span, acc, scratch, stream);
 }
 
-void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, const uint64_t *dense,
-                             const uint16_t *dense16, uint64_t dense_min, uint64_t dense_span, uint64_t *acc,
-                             Scratch &scratch, hipStream_t stream) {
+void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, const void *probe_payload,
+                             int payload_width, const uint64_t *dense, const uint16_t *dense16, uint64_t dense_min,
+                             uint64_t dense_span, uint64_t *acc, Scratch &scratch, hipStream_t stream) {
 	if (dense == nullptr || dense_span == 0 || dense_span > (1ull << 32)) {
 		throw HipError("PartitionedProbeAgg: needs a direct-address table of at most 2^32 entries");
 	}
-	FillU64(acc, 0ull, 3, stream);
+	if (probe_payload != nullptr && !(payload_width == 1 || payload_width == 2 || payload_width == 4 ||
+	                                  payload_width == 8)) {
+		throw HipError("PartitionedProbeAgg: probe payload width must be 1, 2, 4 or 8 bytes");
+	}
+	FillU64(acc, 0ull, 5, stream);
 	if (n == 0) {
 		return;
 	}
@@ -694,25 +773,30 @@ void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, c
 		bits++;
 	}
 	const int shift = bits > 8 ? bits - 8 : 0; // bucket = offset >> shift < 256
-	const uint64_t nbatches = (n + kPartBatch - 1) / kPartBatch;
+	const uint64_t batch = (uint64_t)kPartBlock * (probe_payload != nullptr ? kScatterItemsY : kScatterItems);
+	const uint64_t nbatches = (n + batch - 1) / batch;
 	if (nbatches > 0x7fffffffull) {
 		throw HipError("PartitionedProbeAgg: too many probe rows for one launch");
 	}
-	auto *part = static_cast<uint32_t *>(scratch.Alloc(nbatches * kPartBatch * 4));
+	auto *part = static_cast<uint32_t *>(scratch.Alloc(nbatches * batch * 4));
+	long long *part_y = nullptr;
+	if (probe_payload != nullptr) {
+		part_y = static_cast<long long *>(scratch.Alloc(nbatches * batch * 8));
+	}
 	auto *tab = static_cast<uint32_t *>(scratch.Alloc(nbatches * (kPartBuckets + 1) * 4));
-	auto *partials = static_cast<uint64_t *>(scratch.Alloc((size_t)kPartBuckets * kSliceSplits * 3 * 8));
+	auto *partials = static_cast<uint64_t *>(scratch.Alloc((size_t)kPartBuckets * kSliceSplits * 5 * 8));
 	switch (key_type) {
 	case VT_INT16:
-		LaunchPartitioned<int16_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, part, tab, partials,
-		                           acc, stream);
+		LaunchPartitioned<int16_t>(probe_keys, n, probe_payload, payload_width, dense_min, dense_span, shift, dense,
+		                           dense16, part, part_y, tab, batch, partials, acc, stream);
 		break;
 	case VT_INT32:
-		LaunchPartitioned<int32_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, part, tab, partials,
-		                           acc, stream);
+		LaunchPartitioned<int32_t>(probe_keys, n, probe_payload, payload_width, dense_min, dense_span, shift, dense,
+		                           dense16, part, part_y, tab, batch, partials, acc, stream);
 		break;
 	case VT_INT64:
-		LaunchPartitioned<int64_t>(probe_keys, n, dense_min, dense_span, shift, dense, dense16, part, tab, partials,
-		                           acc, stream);
+		LaunchPartitioned<int64_t>(probe_keys, n, probe_payload, payload_width, dense_min, dense_span, shift, dense,
+		                           dense16, part, part_y, tab, batch, partials, acc, stream);
 		break;
 	default:
 		throw HipError("PartitionedProbeAgg: join keys must be INT16, INT32 or INT64");

@@ -165,15 +165,20 @@ void FinishJoinAggregate(const JoinAggValue &v, int agg_kind, int out_type, uint
 //! XCD's L2 is 4 MB).
 constexpr uint64_t kPartitionTableBytes = 4ull << 20;
 
-//! Aggregate join by probing: when one side only contributes its row counts and every aggregate is a COUNT
-//! or an integer SUM/AVG of one column of the other (build) side, the build side is reduced to a
+//! Aggregate join by probing: when every aggregate is a COUNT or an integer SUM/AVG of one column of the
+//! smaller (build) side or of one column of the larger (probe) side, the build side is reduced to a
 //! direct-address table (per key: row count and payload sum) and the probe side is read once, each row
-//! adding its key's count and sum; nothing of the probe side is sorted. Returns false, having written
-//! nothing, when the build keys are too sparse for the table or a count or sum does not pack.
-bool ProbeJoinAggregate(const DevCol &probe, const DevCol &build, const DevCol *payload, const JoinAggArgs &args,
+//! adding its key's count and sum, and its own payload times that count; nothing of the probe side is
+//! sorted. Returns false, having written nothing, when the build keys are too sparse for the table or a
+//! count or sum does not pack.
+bool ProbeJoinAggregate(const DevCol &probe, const DevCol *probe_payload, const DevCol &build, const DevCol *payload,
+                        const JoinAggArgs &args, const std::vector<JoinAggReq> &aggs, int build_side,
                         std::vector<JoinAggValue> &result, Scratch &scratch, hipStream_t stream) {
 	if (build.type == VT_BOOLEAN || build.rows == 0 || probe.rows == 0) {
 		return false;
+	}
+	if (probe_payload != nullptr && probe_payload->rows != probe.rows) {
+		throw std::runtime_error("gpu_hash_join: aggregate input row count mismatch");
 	}
 	// The table: the build side sorted (one key column, payload alongside) and each key's run turned into its
 	// entry in one pass (no per-key arrays); not built when the build keys span so many values that even
@@ -227,28 +232,37 @@ bool ProbeJoinAggregate(const DevCol &probe, const DevCol &build, const DevCol *
 			dense_span = index->dense_span;
 		}
 	}
-	auto *acc = static_cast<uint64_t *>(scratch.Alloc(3 * 8));
-	if (dense_span * 8 > kPartitionTableBytes && probe.rows >= dense_span && dense_span <= (1ull << 32)) {
-		// The table is larger than one XCD's L2 and every line of it is read several times: group the probe
-		// keys by key range first.
-		PartitionedProbeAgg(probe.type, probe.data, probe.rows, dense, dense_min, dense_span, acc, scratch, stream);
+	auto *acc = static_cast<uint64_t *>(scratch.Alloc(5 * 8));
+	const bool partition = dense_span * 8 > kPartitionTableBytes && probe.rows >= dense_span;
+	if (dense_span <= (1ull << 32) && (partition || probe_payload != nullptr)) {
+		// The table is larger than one XCD's L2 and every line of it is read several times (or the probe
+		// payload has to travel with its key): group the probe keys by key range first.
+		PartitionedProbeAgg(probe.type, probe.data, probe.rows, probe_payload ? probe_payload->data : nullptr,
+		                    probe_payload ? TypeWidth(probe_payload->type) : 0, dense, dense_min, dense_span, acc,
+		                    scratch, stream);
+	} else if (probe_payload != nullptr) {
+		return false;
 	} else {
 		const uint64_t row0 = 0, len = probe.rows;
 		const DeviceRanges r = UploadRanges(&row0, &len, 1, probe.rows, scratch, stream);
 		RangeProbeAgg(probe.type, probe.data, r.seg, r.rows, r.lens, 1, r.nvirt, nullptr, nullptr, nullptr, 0, dense,
 		              dense_min, dense_span, acc, stream);
 	}
-	uint64_t host[3] = {};
-	VGPU_HIP(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, stream));
+	uint64_t host[5] = {};
+	// RangeProbeAgg fills 3 words; the partitioned probe 5 (the probe-payload sum last)
+	VGPU_HIP(hipMemcpyAsync(host, acc, (probe_payload != nullptr ? 5 : 3) * 8, hipMemcpyDeviceToHost, stream));
 	VGPU_HIP(hipStreamSynchronize(stream));
 	for (int q = 0; q < args.naggs; q++) {
 		result[q].kind = args.kind[q];
 		if (args.kind[q] == JA_COUNT) {
 			result[q].lo = host[0];
 			result[q].hi = 0;
-		} else {
+		} else if (aggs[q].side == build_side) {
 			result[q].lo = host[1];
 			result[q].hi = host[2];
+		} else {
+			result[q].lo = host[3];
+			result[q].hi = host[4];
 		}
 	}
 	return true;
@@ -322,24 +336,30 @@ std::vector<JoinAggValue> JoinAggregateDevice(const DevCol &akey, const std::vec
 			narrow = 1;
 		}
 	}
-	if (narrow >= 0) {
-		// Probe form: the count-only side is read once against a table of the other side.
-		const int build_side = 1 - narrow;
-		int payload_input = -1;
+	if (akey.type != VT_BOOLEAN) {
+		// Probe form: the larger side is read once against a table of the smaller one. Every aggregate must
+		// be a COUNT or an integer SUM/AVG of one column of either side.
+		const int probe_side = akey.rows >= bkey.rows ? 0 : 1;
+		const int build_side = 1 - probe_side;
+		int input[2] = {-1, -1};
 		bool probe_ok = true;
 		for (size_t q = 0; q < aggs.size() && probe_ok; q++) {
 			if (args.kind[q] == JA_COUNT) {
 				continue;
 			}
-			probe_ok = args.kind[q] == JA_SUM_I128 && aggs[q].side == build_side &&
-			           (payload_input < 0 || payload_input == aggs[q].input);
-			payload_input = aggs[q].input;
+			const int sd = aggs[q].side;
+			probe_ok = args.kind[q] == JA_SUM_I128 && (sd == 0 || sd == 1) &&
+			           (input[sd] < 0 || input[sd] == aggs[q].input);
+			if (probe_ok) {
+				input[sd] = aggs[q].input;
+			}
 		}
 		if (probe_ok) {
-			const std::vector<DevCol> &bv = build_side == 0 ? avals : bvals;
-			const DevCol *payload = payload_input >= 0 ? &bv[payload_input] : nullptr;
-			if (ProbeJoinAggregate(narrow == 0 ? akey : bkey, build_side == 0 ? akey : bkey, payload, args, result,
-			                       scratch, stream)) {
+			const std::vector<DevCol> *vals[2] = {&avals, &bvals};
+			const DevCol *build_payload = input[build_side] >= 0 ? &(*vals[build_side])[input[build_side]] : nullptr;
+			const DevCol *probe_payload = input[probe_side] >= 0 ? &(*vals[probe_side])[input[probe_side]] : nullptr;
+			if (ProbeJoinAggregate(probe_side == 0 ? akey : bkey, probe_payload, build_side == 0 ? akey : bkey,
+			                       build_payload, args, aggs, build_side, result, scratch, stream)) {
 				return result;
 			}
 		}
