@@ -234,10 +234,26 @@ def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
     kernels = {e["name"]: round(e["ms"] / max(e["launches"], 1) * (e["launches"] / steps), 4) for e in prof}
     alg = sum(e["bytes"] for e in prof) / steps
     compulsory = 12 * nb + 8 * nprobe  # read build key + payload and probe key once
+    # The fact-table shape: SUM of an INT32 column of the probe side as well (it travels through the
+    # partition with its key).
+    fpay = torch.empty(nprobe, dtype=torch.int32, device=dev)
+    capi.generate_column(fpay, capi.INT32, 0, lo=-1000, hi=1000, seed=21, stream_id=5)
+    aggs2 = [(capi.SUM, 0, 0, capi.HUGEINT), (capi.SUM, 1, 0, capi.HUGEINT), (capi.COUNT_STAR, -1, -1, capi.INT64)]
+    res2 = capi.join_aggregate(fk, [fpay], build, [payload], aggs2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res2 = capi.join_aggregate(fk, [fpay], build, [payload], aggs2)
+    torch.cuda.synchronize()
+    ms2 = (time.perf_counter() - t0) * 1e3 / steps
+    if int(res2[1]) != int(res[0]) or int(res2[2]) != int(res[1]):
+        raise RuntimeError("join: the two query shapes disagree on SUM(build payload) / COUNT(*)")
+    del fpay
     return {"operator": "join + aggregate (build-side table, key-range partitioned probe), 2^24 build x 2^28 probe", "probe_rows": nprobe, "build_rows": nb,
             "compulsory_bytes": compulsory,
             "compulsory_frac": round(compulsory / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "sum_payload": int(res[0]), "count": int(res[1]), "ms": round(ms, 3),
+            "probe_side_sum_ms": round(ms2, 3), "probe_side_sum": int(res2[0]),
             "rows_per_s": round((nprobe + nb) / (ms * 1e-3), 1), "algorithmic_bytes": alg,
             "gbs": round(alg / (ms * 1e-3) / 1e9, 1), "kernel_ms_per_step": kernels}
 

@@ -15,6 +15,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace vgpu {
 namespace {
@@ -263,7 +264,7 @@ constexpr int kPartBuckets = 256;
 constexpr int kPartBlock = 1024;      // 16 waves
 constexpr int kPartItems = 8;         // probe: keys per thread per loop step
 constexpr int kScatterItems = 32;     // keys per thread per batch (keys only: 32768-row batches)
-constexpr int kScatterItemsY = 16;    // with a probe payload (16384-row batches: 8-byte values fill the stage)
+constexpr int kScatterItemsY = 16;    // with a probe payload (16384-row batches)
 constexpr unsigned kRunChunk = 2048;  // runs (batches) per LDS scan of run lengths
 
 //! Raw integer of `width` bytes at row j, sign-extended (width 1 is BOOLEAN: zero-extended).
@@ -281,16 +282,19 @@ __device__ __forceinline__ long long LoadIntAt(const void *p, int width, u64 j) 
 }
 
 //! Batch g = rows [g * ITEMS * 1024, ...): kept keys grouped by bucket, written over the batch's own positions
-//! of `part` (table offsets) and, with HASY, of `part_y` (the probe payload as int64, same order).
-template <typename K, int ITEMS, bool HASY>
+//! of `part` (table offsets) and, with a payload type Y (int32 for payloads of up to 4 bytes, int64 for 8;
+//! void: none), of `part_y` (the probe payload, same order).
+template <typename K, int ITEMS, typename Y>
 __global__ void __launch_bounds__(kPartBlock) PartBatchKernel(const K *keys, u64 n, u64 dmin, u64 span, int shift,
-                                                             const void *y, int ywidth, uint32_t *part,
-                                                             long long *part_y, uint32_t *tab) {
+                                                             const void *y, int ywidth, uint32_t *part, void *part_y,
+                                                             uint32_t *tab) {
+	constexpr bool HASY = !std::is_void<Y>::value;
+	constexpr int kYBytes = HASY ? (int)sizeof(typename std::conditional<HASY, Y, int>::type) : 4;
 	constexpr u64 kBatch = (u64)kPartBlock * ITEMS;
 	__shared__ uint32_t s_cnt[kPartBuckets];   // batch count per bucket
 	__shared__ uint32_t s_start[kPartBuckets]; // batch-local start per bucket
 	__shared__ uint32_t s_scan[kPartBlock / kWave + 1];
-	__shared__ u64 s_stage[kBatch * (HASY ? 8 : 4) / 8]; // offsets, then (HASY) payloads
+	__shared__ u64 s_stage[kBatch * (kYBytes > 4 ? kYBytes : 4) / 8]; // offsets, then payloads
 	uint32_t *s_off = reinterpret_cast<uint32_t *>(s_stage);
 	const int tid = threadIdx.x;
 	const u64 base = (u64)blockIdx.x * kBatch;
@@ -336,11 +340,12 @@ __global__ void __launch_bounds__(kPartBlock) PartBatchKernel(const K *keys, u64
 		part[base + i] = s_off[i];
 	}
 	if constexpr (HASY) {
-		long long *s_y = reinterpret_cast<long long *>(s_stage);
-		long long yv[ITEMS];
+		using YT = typename std::conditional<HASY, Y, int>::type;
+		YT *s_y = reinterpret_cast<YT *>(s_stage);
+		YT yv[ITEMS];
 #pragma unroll
 		for (int j = 0; j < ITEMS; j++) {
-			yv[j] = valid[j] ? LoadIntAt(y, ywidth, base + (u64)j * kPartBlock + tid) : 0ll;
+			yv[j] = valid[j] ? (YT)LoadIntAt(y, ywidth, base + (u64)j * kPartBlock + tid) : (YT)0;
 		}
 		__syncthreads(); // every offset has left the stage
 #pragma unroll
@@ -351,7 +356,7 @@ __global__ void __launch_bounds__(kPartBlock) PartBatchKernel(const K *keys, u64
 		}
 		__syncthreads();
 		for (uint32_t i = tid; i < total; i += kPartBlock) {
-			part_y[base + i] = s_y[i];
+			static_cast<YT *>(part_y)[base + i] = s_y[i];
 		}
 	}
 }
@@ -398,11 +403,13 @@ __device__ __forceinline__ void AddMul128(u64 &lo, u64 &hi, long long y, u64 c) 
 //! 16-bit table slice is copied to LDS once and every key is looked up there; otherwise the 64-bit table is
 //! read through L2. Sums per block: matched build rows, the build payload sum, and (HASY) the probe payload
 //! times its key's build row count.
-template <bool LDS, bool HASY>
-__global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *part, const long long *part_y,
+template <bool LDS, typename Y>
+__global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *part, const void *part_y,
                                                              const uint32_t *tab, u64 nbatches, u64 batch,
                                                              const uint16_t *dense16, const u64 *dense, u64 span,
                                                              int shift, u64 *partials) {
+	constexpr bool HASY = !std::is_void<Y>::value;
+	using YT = typename std::conditional<HASY, Y, int>::type;
 	__shared__ uint16_t s_slice[LDS ? kSliceEntries : 1];
 	__shared__ uint32_t s_pre[kRunChunk + 1]; // exclusive prefix of run lengths
 	__shared__ uint32_t s_first[kRunChunk];   // each run's start inside its batch
@@ -474,7 +481,7 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
 					const u64 at = (g0 + lo) * batch + s_first[lo] + (v - s_pre[lo]);
 					key[k] = part[at];
 					if constexpr (HASY) {
-						yv[k] = part_y[at];
+						yv[k] = (long long)static_cast<const YT *>(part_y)[at];
 					}
 				}
 			}
@@ -565,41 +572,50 @@ __global__ void __launch_bounds__(kWave) PartFinalKernel(const u64 *partials, un
 	}
 }
 
-template <typename K>
-void LaunchPartitioned(const void *keys_v, uint64_t n, const void *y, int ywidth, uint64_t dmin, uint64_t span,
-                       int shift, const uint64_t *dense, const uint16_t *dense16, uint32_t *part, long long *part_y,
-                       uint32_t *tab, uint64_t batch, uint64_t *partials, uint64_t *acc, hipStream_t stream) {
-	const K *keys = static_cast<const K *>(keys_v);
+template <typename K, typename Y>
+void LaunchPartitionedY(const K *keys, uint64_t n, const void *y, int ywidth, uint64_t dmin, uint64_t span, int shift,
+                        const uint64_t *dense, const uint16_t *dense16, uint32_t *part, void *part_y, uint32_t *tab,
+                        uint64_t batch, uint64_t *partials, unsigned nparts, hipStream_t stream) {
+	// with a payload (4 or 8 bytes) a thread holds 16 rows: 32 rows of offsets, ranks and payloads cost more in
+	// registers than the longer bucket runs save
+	constexpr int kItems = std::is_void<Y>::value ? kScatterItems : kScatterItemsY;
 	const uint64_t nbatches = (n + batch - 1) / batch;
+	const double ybytes = std::is_void<Y>::value ? 0.0 : (double)ywidth + 2.0 * sizeof(typename std::conditional<
+	                                                                                std::is_void<Y>::value, int, Y>::type);
 	{
-		ProfileScope prof("partition_scatter", stream, (double)n * (sizeof(K) + 4 + (y ? ywidth + 8 : 0)));
-		if (y != nullptr) {
-			hipLaunchKernelGGL((PartBatchKernel<K, kScatterItemsY, true>), dim3((unsigned)nbatches), dim3(kPartBlock),
-			                   0, stream, keys, (u64)n, (u64)dmin, (u64)span, shift, y, ywidth, part, part_y, tab);
+		ProfileScope prof("partition_scatter", stream, (double)n * (sizeof(K) + 4) + (double)n * ybytes / 2);
+		hipLaunchKernelGGL((PartBatchKernel<K, kItems, Y>), dim3((unsigned)nbatches), dim3(kPartBlock), 0, stream,
+		                   keys, (u64)n, (u64)dmin, (u64)span, shift, y, ywidth, part, part_y, tab);
+		VGPU_HIP(hipGetLastError());
+	}
+	{
+		ProfileScope prof("partition_probe", stream, (double)n * 4 + (double)n * ybytes / 2);
+		if (dense16 != nullptr && (1ull << shift) <= kSliceEntries) {
+			hipLaunchKernelGGL((PartProbeKernel<true, Y>), dim3(nparts), dim3(kPartBlock), 0, stream, part, part_y,
+			                   tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
 		} else {
-			hipLaunchKernelGGL((PartBatchKernel<K, kScatterItems, false>), dim3((unsigned)nbatches), dim3(kPartBlock),
-			                   0, stream, keys, (u64)n, (u64)dmin, (u64)span, shift, y, ywidth, part, part_y, tab);
+			hipLaunchKernelGGL((PartProbeKernel<false, Y>), dim3(nparts), dim3(kPartBlock), 0, stream, part, part_y,
+			                   tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
 		}
 		VGPU_HIP(hipGetLastError());
 	}
+}
+
+template <typename K>
+void LaunchPartitioned(const void *keys_v, uint64_t n, const void *y, int ywidth, uint64_t dmin, uint64_t span,
+                       int shift, const uint64_t *dense, const uint16_t *dense16, uint32_t *part, void *part_y,
+                       uint32_t *tab, uint64_t batch, uint64_t *partials, uint64_t *acc, hipStream_t stream) {
+	const K *keys = static_cast<const K *>(keys_v);
 	const unsigned nparts = kPartBuckets * kSliceSplits;
-	{
-		ProfileScope prof("partition_probe", stream, (double)n * (4 + (y ? 8 : 0)));
-		const bool lds = dense16 != nullptr && (1ull << shift) <= kSliceEntries;
-		if (lds && y != nullptr) {
-			hipLaunchKernelGGL((PartProbeKernel<true, true>), dim3(nparts), dim3(kPartBlock), 0, stream, part, part_y,
-			                   tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
-		} else if (lds) {
-			hipLaunchKernelGGL((PartProbeKernel<true, false>), dim3(nparts), dim3(kPartBlock), 0, stream, part, part_y,
-			                   tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
-		} else if (y != nullptr) {
-			hipLaunchKernelGGL((PartProbeKernel<false, true>), dim3(nparts), dim3(kPartBlock), 0, stream, part, part_y,
-			                   tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
-		} else {
-			hipLaunchKernelGGL((PartProbeKernel<false, false>), dim3(nparts), dim3(kPartBlock), 0, stream, part,
-			                   part_y, tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
-		}
-		VGPU_HIP(hipGetLastError());
+	if (y == nullptr) {
+		LaunchPartitionedY<K, void>(keys, n, y, ywidth, dmin, span, shift, dense, dense16, part, part_y, tab, batch,
+		                            partials, nparts, stream);
+	} else if (ywidth <= 4) {
+		LaunchPartitionedY<K, int32_t>(keys, n, y, ywidth, dmin, span, shift, dense, dense16, part, part_y, tab,
+		                               batch, partials, nparts, stream);
+	} else {
+		LaunchPartitionedY<K, long long>(keys, n, y, ywidth, dmin, span, shift, dense, dense16, part, part_y, tab,
+		                                 batch, partials, nparts, stream);
 	}
 	hipLaunchKernelGGL(PartFinalKernel, dim3(1), dim3(kWave), 0, stream, partials, nparts, acc);
 	VGPU_HIP(hipGetLastError());
@@ -779,9 +795,9 @@ void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, c
 		throw HipError("PartitionedProbeAgg: too many probe rows for one launch");
 	}
 	auto *part = static_cast<uint32_t *>(scratch.Alloc(nbatches * batch * 4));
-	long long *part_y = nullptr;
+	void *part_y = nullptr;
 	if (probe_payload != nullptr) {
-		part_y = static_cast<long long *>(scratch.Alloc(nbatches * batch * 8));
+		part_y = scratch.Alloc(nbatches * batch * (payload_width <= 4 ? 4 : 8));
 	}
 	auto *tab = static_cast<uint32_t *>(scratch.Alloc(nbatches * (kPartBuckets + 1) * 4));
 	auto *partials = static_cast<uint64_t *>(scratch.Alloc((size_t)kPartBuckets * kSliceSplits * 5 * 8));
