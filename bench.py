@@ -258,6 +258,62 @@ def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
             "gbs": round(alg / (ms * 1e-3) / 1e9, 1), "kernel_ms_per_step": kernels}
 
 
+def bench_execute_plan(torch, capi, dev, reps, n=1 << 26):
+    """GpuEngine::ExecutePlan of the config-0 query (SCAN a, b, c -> FILTER -> PROJECTION) from HOST columns:
+    the PCIe-inclusive rate of the boundary the extension calls (upload of 20 B/row through the pinned
+    staging pool, the fused kernel, download of the kept rows into the result's host buffers). The headline
+    and the other operator lines time device-resident data instead."""
+    import ctypes as C
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "tests"))
+    from exprs import CFG0_PRED_A, CFG0_PRED_C, CFG0_PROJ_1, CFG0_PROJ_2
+    from vectisdb_amd.engine import GpuExpr, GpuOpType as Op, GpuPlanNode, GpuTableRef, GpuValueType as T
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 1_000_000, n, dtype=np.int32)
+    b = rng.integers(-(1 << 40), 1 << 40, n, dtype=np.int64)
+    c = rng.random(n)
+    scan = GpuPlanNode(Op.SCAN, table=GpuTableRef("memory", "main", "t"), column_names=["a", "b", "c"])
+    filt = GpuPlanNode(Op.FILTER, children=[scan], expressions=[GpuExpr(CFG0_PRED_A, ["a"], T.BOOLEAN),
+                                                               GpuExpr(CFG0_PRED_C, ["c"], T.BOOLEAN)])
+    proj = GpuPlanNode(Op.PROJECTION, children=[filt], expressions=[GpuExpr(CFG0_PROJ_1, ["a", "b"], T.INT64),
+                                                                   GpuExpr(CFG0_PROJ_2, ["c", "a"], T.FLOAT64)])
+    plan = proj.to_json().encode()
+    hosts = (capi.HostColumn * 3)()
+    names = [b"a", b"b", b"c"]
+    for i, (arr, vt) in enumerate(((a, T.INT32), (b, T.INT64), (c, T.FLOAT64))):
+        hosts[i].name = names[i]
+        hosts[i].type = int(vt)
+        hosts[i].data = arr.ctypes.data
+        hosts[i].validity = None
+        hosts[i].rows = n
+    lib = capi.load()
+
+    def once():
+        h = C.c_void_p()
+        capi.check(lib.vgpu_execute_plan(plan, hosts, 3, None, C.byref(h)))
+        try:
+            if not lib.vgpu_result_success(h):
+                raise RuntimeError((lib.vgpu_result_error(h) or b"").decode())
+            hc = capi.HostColumn()
+            capi.check(lib.vgpu_result_column(h, 0, C.byref(hc)))
+            return int(hc.rows)
+        finally:
+            lib.vgpu_result_free(h)
+
+    kept = once()
+    expected = int(((a < 500000) & (c > 0.25)).sum())
+    if kept != expected:
+        raise RuntimeError(f"execute_plan: {kept} rows kept, expected {expected}")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        once()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    moved = n * 20 + kept * 16
+    return {"operator": f"ExecutePlan config-0 query from host columns, {n} rows (PCIe-inclusive)", "rows": n,
+            "kept": kept, "ms": round(ms, 3), "rows_per_s": round(n / (ms * 1e-3), 1),
+            "host_bytes_moved": moved, "host_gbs": round(moved / (ms * 1e-3) / 1e9, 1)}
+
+
 def bench_cross(torch, capi, dev, steps):
     """Cross product + filter: 4,096 x 65,536 INT32 pairs WHERE x + y < 2^30, materialized (x, y)."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -609,13 +665,16 @@ def main():
         os.environ.pop("VGPU_RADIX_BITS", None)
 
     operators = []
-    extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "filter_large", "cross", "join", "batched", "served"]
+    extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "filter_large", "execute_plan", "cross",
+                                                                       "join", "batched", "served"]
     for name in extra:
         try:
             if name == "filter":
                 operators.append(bench_filter_project(torch, capi, dev, 50))
             elif name == "filter_large":
                 operators.append(bench_filter_project_large(torch, capi, dev, 10))
+            elif name == "execute_plan":
+                operators.append(bench_execute_plan(torch, capi, dev, 3))
             elif name == "cross":
                 operators.append(bench_cross(torch, capi, dev, 10))
             elif name == "join":

@@ -19,6 +19,7 @@
 //     single pair.
 //   * Device memory comes from a stream-ordered caching pool, not one hipMalloc/hipFree per buffer.
 //   * NULL validity is packed into the Arrow bitmap on the device; only the bitmap crosses PCIe.
+#include "host_copy.h"
 #include "../../../include/code_generator.hpp"
 #include "../../../include/gpu_column_cache.hpp"
 #include "../../../include/gpu_engine.hpp"
@@ -40,6 +41,8 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <exception>
+#include <thread>
 #include <vector>
 
 namespace vector_gpu {
@@ -109,7 +112,7 @@ public:
 		for (size_t off = 0; off < bytes; off += kPiece) {
 			const size_t len = std::min(kPiece, bytes - off);
 			Lane &l = Take();
-			std::memcpy(l.slot.data(), static_cast<const uint8_t *>(src) + off, len);
+			vgpu::ParallelHostCopy(l.slot.data(), static_cast<const uint8_t *>(src) + off, len);
 			VGPU_HIP(hipMemcpyAsync(static_cast<uint8_t *>(dst) + off, l.slot.data(), len, hipMemcpyHostToDevice,
 			                        stream_));
 			Issued(l, nullptr, 0);
@@ -166,7 +169,7 @@ private:
 		VGPU_HIP(hipEventSynchronize(l.done));
 		l.busy = false;
 		if (l.host_dst != nullptr) {
-			std::memcpy(l.host_dst, l.slot.data(), l.host_bytes);
+			vgpu::ParallelHostCopy(l.host_dst, l.slot.data(), l.host_bytes);
 			l.host_dst = nullptr;
 		}
 	}
@@ -1601,14 +1604,43 @@ bool ExecuteGpuPlan(const GpuPlanNode &plan, const std::vector<GpuColumn> &input
 				throw std::logic_error("gpu_executor: result column '" + c.name + "' was pruned");
 			}
 		}
-		out.owned_buffers.reserve(columns.size() * 2);
 		std::vector<size_t> data_index(columns.size()), valid_index(columns.size(), SIZE_MAX);
+		std::vector<size_t> buffer_bytes;
 		for (size_t i = 0; i < columns.size(); i++) {
-			out.owned_buffers.emplace_back(static_cast<size_t>(columns[i].rows) * TypeSize(columns[i].type));
-			data_index[i] = out.owned_buffers.size() - 1;
+			buffer_bytes.push_back(static_cast<size_t>(columns[i].rows) * TypeSize(columns[i].type));
+			data_index[i] = buffer_bytes.size() - 1;
 			if (columns[i].valid != nullptr && columns[i].rows > 0) {
-				out.owned_buffers.emplace_back((static_cast<size_t>(columns[i].rows) + 7) / 8);
-				valid_index[i] = out.owned_buffers.size() - 1;
+				buffer_bytes.push_back((static_cast<size_t>(columns[i].rows) + 7) / 8);
+				valid_index[i] = buffer_bytes.size() - 1;
+			}
+		}
+		// The result's buffers are std::vector<uint8_t> (the reference's type), which zero-fills -- and so
+		// page-faults -- every byte on construction; large ones are built on threads of their own so those
+		// fills run side by side instead of one after another.
+		out.owned_buffers.resize(buffer_bytes.size());
+		{
+			std::vector<std::thread> makers;
+			std::vector<std::exception_ptr> failed(buffer_bytes.size());
+			for (size_t b = 0; b < buffer_bytes.size(); b++) {
+				if (buffer_bytes[b] >= (16u << 20)) {
+					makers.emplace_back([&out, &buffer_bytes, &failed, b] {
+						try {
+							out.owned_buffers[b].resize(buffer_bytes[b]);
+						} catch (...) {
+							failed[b] = std::current_exception();
+						}
+					});
+				} else {
+					out.owned_buffers[b].resize(buffer_bytes[b]);
+				}
+			}
+			for (auto &t : makers) {
+				t.join();
+			}
+			for (auto &e : failed) {
+				if (e) {
+					std::rethrow_exception(e);
+				}
 			}
 		}
 		for (size_t i = 0; i < columns.size(); i++) {
