@@ -6,7 +6,10 @@
 #include "gpu_engine.hpp"
 #include "gpu_logger.hpp"
 #include "kernel_cache.hpp"
+// the engine's host-copy pool (an internal header: the staging ring of host-column plans uses it)
+#include "../../vectisdb_amd/csrc/engine/host_copy.h"
 
+#include <algorithm>
 #include <filesystem>
 #include <unistd.h>
 #include <fstream>
@@ -317,6 +320,39 @@ static void TestRoutingThroughHeader() {
 	CHECK(EstimateWorkingSet(filter, 1 << 21).chunk_capped);
 }
 
+// ---------------------------------------------------------------- host-copy pool
+
+static void TestParallelHostCopySizesAndConcurrency() {
+	// sizes around the parallel threshold and the 4 KiB part rounding, odd tails
+	const size_t sizes[] = {0, 1, (2u << 20) - 1, 2u << 20, (16u << 20) + 7, (64u << 20) + 4095};
+	std::vector<uint8_t> src((64u << 20) + 4096), dst(src.size());
+	for (size_t i = 0; i < src.size(); i++) {
+		src[i] = (uint8_t)(i * 2654435761u >> 13);
+	}
+	for (size_t n : sizes) {
+		std::fill(dst.begin(), dst.end(), 0xA5);
+		vgpu::ParallelHostCopy(dst.data(), src.data(), n);
+		CHECK(std::equal(dst.begin(), dst.begin() + n, src.begin()));
+		CHECK(dst[n] == 0xA5); // nothing written past the end
+	}
+	// several callers at once (the pool serves them one after another)
+	std::vector<std::vector<uint8_t>> outs(4, std::vector<uint8_t>((8u << 20) + 123));
+	std::vector<std::thread> callers;
+	for (int t = 0; t < 4; t++) {
+		callers.emplace_back([&, t] {
+			for (int r = 0; r < 5; r++) {
+				vgpu::ParallelHostCopy(outs[t].data(), src.data() + t * 1000, outs[t].size());
+			}
+		});
+	}
+	for (auto &c : callers) {
+		c.join();
+	}
+	for (int t = 0; t < 4; t++) {
+		CHECK(std::equal(outs[t].begin(), outs[t].end(), src.begin() + t * 1000));
+	}
+}
+
 int main() {
 	RUN_TEST(TestMissThenHit);
 	RUN_TEST(TestOverwriteSameKey);
@@ -335,5 +371,6 @@ int main() {
 	RUN_TEST(TestLoggerConcurrentWritersProduceWholeLines);
 	RUN_TEST(TestLoggerCapEvictsOldestKeepsNewest);
 	RUN_TEST(TestRoutingThroughHeader);
+	RUN_TEST(TestParallelHostCopySizesAndConcurrency);
 	TEST_MAIN_EPILOGUE();
 }
