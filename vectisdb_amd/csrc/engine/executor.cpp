@@ -134,7 +134,7 @@ public:
 	}
 
 private:
-	static constexpr size_t kPiece = 16u << 20;
+	static constexpr size_t kPiece = 32u << 20;
 	struct Lane {
 		PinnedBufferSlot slot;
 		bool have = false;

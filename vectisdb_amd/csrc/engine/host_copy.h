@@ -14,6 +14,6 @@ namespace vgpu {
 //! concurrent callers are served one after another.
 void ParallelHostCopy(void *dst, const void *src, size_t bytes);
 
-constexpr int kHostCopyThreads = 8;
+constexpr int kHostCopyThreads = 12;
 
 } // namespace vgpu
