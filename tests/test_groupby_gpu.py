@@ -230,9 +230,39 @@ def test_groupby_under_sort_autotuner(dev):
         assert mine, report
         for r in mine:
             assert r["radix_bits"] in (8, 11) and r["items"] in (16, 32) and r["prefetch"] in (0, 1)
-            assert r["candidates"] >= 4 and r["ms"] > 0
+            assert r["block"] in (256, 512)
+            assert r["candidates"] >= 5 and r["ms"] > 0
     finally:
         capi.sort_autotune(False)
+
+
+@pytest.mark.parametrize("variant", [
+    {"VGPU_SORT_BLOCK": "512"},
+    {"VGPU_SORT_ITEMS": "32", "VGPU_SORT_PREFETCH": "0"},
+    {"VGPU_SORT_ITEMS": "32", "VGPU_SORT_PREFETCH": "1"},
+    {"VGPU_SORT_ITEMS": "16", "VGPU_SORT_PREFETCH": "0"},
+    {"VGPU_SORT_ITEMS": "16", "VGPU_SORT_PREFETCH": "1"},
+], ids=["b512_i16", "i32_load_late", "i32_prefetch", "i16_load_late", "i16_prefetch"])
+def test_sort_variants_forced(dev, monkeypatch, variant):
+    """Every 8-bit radix-pass variant (threads per tile x items per thread x payload prefetch), forced through
+    the tuning overrides: GROUP BY bit-exact against the oracle with row counts around the 4096- and
+    8192-row tile edges (all at least 64K rows, where 8-bit digits are used), one-pass and three-pass keys,
+    negative keys; and the key-only sort with its permutation against a stable argsort."""
+    import torch
+    for k, v in variant.items():
+        monkeypatch.setenv(k, v)
+    rng = np.random.default_rng(31)
+    for n, distinct in [(65_537, 600), (131_071, 131_071), (135_169, 5), (2_500_001, 1 << 20)]:
+        key = rng.integers(-distinct, distinct, n, dtype=np.int64)
+        v1 = rng.integers(-10 ** 9, 10 ** 9 + 1, n, dtype=np.int64)
+        v2 = rng.standard_normal(n)
+        _check(dev, key, v1, v2)
+    for n in (65_537, 1_000_003):
+        key = rng.integers(-(1 << 40), 1 << 40, n, dtype=np.int64)
+        out, perm = capi.sort_column(torch.from_numpy(key).to(dev))
+        order = np.argsort(key, kind="stable")
+        np.testing.assert_array_equal(out.cpu().numpy(), key[order])
+        np.testing.assert_array_equal(perm.cpu().numpy().astype(np.int64), order)
 
 
 def test_groupby_many_tiles_carries(dev):

@@ -119,7 +119,8 @@ void PlanKeyPackAligned(const void *col, int vtype, uint64_t omin, uint64_t omax
 struct SortTuning {
 	int radix_bits = 8;      // 8 or 11 (digit width upper bound)
 	int passes_override = 0; // 0 = derive from total_bits
-	int items = 16;          // items per thread of a pass with 8-bit digits: 16 or 32 (tile = 256 x items)
+	int items = 16;          // items per thread of a pass with 8-bit digits: 16 or 32 (tile = block x items)
+	int block = 256;         // threads per tile with 8-bit digits and 16 items: 256 or 512 (tile 4096 or 8192)
 	int prefetch = 1;        // load payload values before ranking (1) or just before their scatter (0)
 	bool fixed = false;      // variant forced (environment override): no defaults, no autotuning
 	//! Optional precomputed per-byte histograms (KeyRangeHist8) for a byte-aligned pack: skips the

@@ -1,0 +1,13 @@
+// onesweep_b512n.hip -- instantiation unit of the onesweep radix pass: 512 threads x 16 items (8192-row tile),
+// payload loaded at its scatter (8-bit digits).
+#include "onesweep_kernel.h"
+
+namespace vgpu {
+namespace sortpass {
+
+template void LaunchVariant<8, 16, false, 512>(bool, bool, const KeyPack &, const void *, void *,
+                                               const PayloadArgs &, uint64_t, int, int, const uint32_t *, uint32_t *,
+                                               uint32_t *, hipStream_t, unsigned);
+
+} // namespace sortpass
+} // namespace vgpu

@@ -4,7 +4,7 @@
 namespace vgpu {
 namespace sortpass {
 
-template void LaunchVariant<8, 16, false>(bool, bool, const KeyPack &, const void *, void *, const PayloadArgs &,
+template void LaunchVariant<8, 16, false, 256>(bool, bool, const KeyPack &, const void *, void *, const PayloadArgs &,
                                          uint64_t, int, int, const uint32_t *, uint32_t *, uint32_t *, hipStream_t,
                                          unsigned);
 

@@ -4,10 +4,10 @@
 namespace vgpu {
 namespace sortpass {
 
-template void LaunchVariant<8, 16, true>(bool, bool, const KeyPack &, const void *, void *, const PayloadArgs &,
+template void LaunchVariant<8, 16, true, 256>(bool, bool, const KeyPack &, const void *, void *, const PayloadArgs &,
                                          uint64_t, int, int, const uint32_t *, uint32_t *, uint32_t *, hipStream_t,
                                          unsigned);
-template void LaunchVariant<11, 16, true>(bool, bool, const KeyPack &, const void *, void *, const PayloadArgs &,
+template void LaunchVariant<11, 16, true, 256>(bool, bool, const KeyPack &, const void *, void *, const PayloadArgs &,
                                          uint64_t, int, int, const uint32_t *, uint32_t *, uint32_t *, hipStream_t,
                                          unsigned);
 

@@ -4,7 +4,7 @@
 namespace vgpu {
 namespace sortpass {
 
-template void LaunchVariant<8, 32, true>(bool, bool, const KeyPack &, const void *, void *, const PayloadArgs &,
+template void LaunchVariant<8, 32, true, 256>(bool, bool, const KeyPack &, const void *, void *, const PayloadArgs &,
                                          uint64_t, int, int, const uint32_t *, uint32_t *, uint32_t *, hipStream_t,
                                          unsigned);
 

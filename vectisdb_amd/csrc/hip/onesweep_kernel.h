@@ -8,29 +8,32 @@ namespace vgpu {
 namespace sortpass {
 
 //! One digit pass. Template: R = digit table width (8 or 11 bits), KeyT = compact key (u32/u64), FIRST =
-//! the pass packs the raw key columns, NP = payload columns prefetched into registers (0..4).
+//! the pass packs the raw key columns, NP = payload columns prefetched into registers (0..4), BLOCK = threads
+//! per tile (256, or 512 for an 8192-row tile at 16 items per thread: the long per-digit runs of 32 items
+//! per thread with half the registers per thread, so twice the waves stay resident).
 //!
-//! Tile = 256 threads x 16 items, wave-striped (lane-contiguous 64-item rows, so every global load is a
+//! Tile = BLOCK threads x ITEMS items, wave-striped (lane-contiguous 64-item rows, so every global load is a
 //! coalesced 64 x width-byte line). Payload values are loaded at kernel entry so their latency overlaps the
 //! ranking and the look-back. One LDS buffer (TILE x 4 bytes per round) stages first the per-wave digit
 //! counters, then the locally sorted keys, then each payload column 32 bits at a time, so the scatter to
 //! global memory writes runs of consecutive addresses per digit bucket.
-template <int R, typename KeyT, bool FIRST, int NP, int ITEMS, bool PF>
-__global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const KeyT *keys_in, KeyT *keys_out,
+template <int R, typename KeyT, bool FIRST, int NP, int ITEMS, bool PF, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) OnesweepKernel(KeyPack pack, const KeyT *keys_in, KeyT *keys_out,
                                                              PayloadArgs pay, uint64_t n, int digit_shift,
                                                              int digit_bits, const uint32_t *ghist,
                                                              uint32_t *tile_state, uint32_t *tile_ticket) {
 	constexpr int RADIX = 1 << R;
-	constexpr int TILE = kSortBlock * ITEMS;
-	constexpr int DPT = (RADIX + kSortBlock - 1) / kSortBlock;
+	constexpr int kWaves = BLOCK / kWave;
+	constexpr int TILE = BLOCK * ITEMS;
+	constexpr int DPT = (RADIX + BLOCK - 1) / BLOCK;
 	constexpr int LDS_WORDS_KEYS = TILE * (int)(sizeof(KeyT) / 4);
-	constexpr int LDS_WORDS_COUNT = kSortWaves * RADIX;
+	constexpr int LDS_WORDS_COUNT = kWaves * RADIX;
 	constexpr int LDS_WORDS_PAY = NP > 0 ? 2 * TILE : 0; // 8-byte payload values staged whole
 	constexpr int LDS_WORDS_A = LDS_WORDS_KEYS > LDS_WORDS_COUNT ? LDS_WORDS_KEYS : LDS_WORDS_COUNT;
 	constexpr int LDS_WORDS = LDS_WORDS_A > LDS_WORDS_PAY ? LDS_WORDS_A : LDS_WORDS_PAY;
 	__shared__ __align__(16) uint32_t s_buf[LDS_WORDS];
 	__shared__ uint32_t s_digit_base[RADIX];
-	__shared__ uint32_t s_scan[kSortWaves + 1];
+	__shared__ uint32_t s_scan[kWaves + 1];
 	__shared__ uint32_t s_tile;
 
 	const int tid = threadIdx.x;
@@ -39,7 +42,7 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 	if (tid == 0) {
 		s_tile = FetchAddAgent(tile_ticket, 1u);
 	}
-	for (int i = tid; i < kSortWaves * RADIX; i += kSortBlock) {
+	for (int i = tid; i < kWaves * RADIX; i += BLOCK) {
 		s_buf[i] = 0;
 	}
 	__syncthreads();
@@ -133,7 +136,7 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 		if (d < RADIX) {
 			uint32_t run = 0;
 #pragma unroll
-			for (int w = 0; w < kSortWaves; w++) {
+			for (int w = 0; w < kWaves; w++) {
 				uint32_t c = s_buf[w * RADIX + d];
 				s_buf[w * RADIX + d] = run;
 				run += c;
@@ -152,9 +155,9 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 		gsum += gh[j];
 	}
 	uint32_t block_total;
-	uint32_t texcl = BlockExclusiveScan<uint32_t, kSortBlock>(tsum, s_scan, &block_total);
+	uint32_t texcl = BlockExclusiveScan<uint32_t, BLOCK>(tsum, s_scan, &block_total);
 	uint32_t gtotal;
-	uint32_t gexcl = BlockExclusiveScan<uint32_t, kSortBlock>(gsum, s_scan, &gtotal);
+	uint32_t gexcl = BlockExclusiveScan<uint32_t, BLOCK>(gsum, s_scan, &gtotal);
 
 	// Look-back: all of this thread's digits advance together, so their loads are issued in parallel.
 	uint32_t excl[DPT];
@@ -212,7 +215,7 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 			}
 			s_digit_base[d] = goff + excl[j] - bstart;
 #pragma unroll
-			for (int w = 0; w < kSortWaves; w++) {
+			for (int w = 0; w < kWaves; w++) {
 				s_buf[w * RADIX + d] += bstart;
 			}
 		}
@@ -244,7 +247,7 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 	uint32_t dst[ITEMS];
 #pragma unroll
 	for (int j = 0; j < ITEMS; j++) {
-		uint32_t i = tid + j * kSortBlock;
+		uint32_t i = tid + j * BLOCK;
 		dst[j] = 0;
 		if (i < tile_count) {
 			KeyT k2 = s_keys[i];
@@ -284,7 +287,7 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 				u64 *out = reinterpret_cast<u64 *>(pay.out[p]);
 #pragma unroll
 				for (int j = 0; j < ITEMS; j++) {
-					uint32_t i = tid + j * kSortBlock;
+					uint32_t i = tid + j * BLOCK;
 					if (i < tile_count) {
 						out[dst[j]] = s_pay[i];
 					}
@@ -301,7 +304,7 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 				void *out = pay.out[p];
 #pragma unroll
 				for (int j = 0; j < ITEMS; j++) {
-					uint32_t i = tid + j * kSortBlock;
+					uint32_t i = tid + j * BLOCK;
 					if (i < tile_count) {
 						uint32_t v = s_buf[i];
 						switch (width) {
@@ -322,40 +325,40 @@ __global__ void __launch_bounds__(kSortBlock) OnesweepKernel(KeyPack pack, const
 	}
 }
 
-template <int R, int ITEMS, bool PF, typename KeyT, int NP>
+template <int R, int ITEMS, bool PF, int BLOCK, typename KeyT, int NP>
 void LaunchTyped(bool first, const KeyPack &pack, const void *keys_in, void *keys_out, const PayloadArgs &pay,
                  uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state, uint32_t *ticket,
                  hipStream_t stream, unsigned tiles) {
 	if (first) {
-		hipLaunchKernelGGL((OnesweepKernel<R, KeyT, true, NP, ITEMS, PF>), dim3(tiles), dim3(kSortBlock), 0, stream,
+		hipLaunchKernelGGL((OnesweepKernel<R, KeyT, true, NP, ITEMS, PF, BLOCK>), dim3(tiles), dim3(BLOCK), 0, stream,
 		                   pack, (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state, ticket);
 	} else {
-		hipLaunchKernelGGL((OnesweepKernel<R, KeyT, false, NP, ITEMS, PF>), dim3(tiles), dim3(kSortBlock), 0,
+		hipLaunchKernelGGL((OnesweepKernel<R, KeyT, false, NP, ITEMS, PF, BLOCK>), dim3(tiles), dim3(BLOCK), 0,
 		                   stream, pack, (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state,
 		                   ticket);
 	}
 }
 
-template <int R, int ITEMS, bool PF, typename KeyT>
+template <int R, int ITEMS, bool PF, int BLOCK, typename KeyT>
 void LaunchKeyed(bool first, const KeyPack &pack, const void *keys_in, void *keys_out, const PayloadArgs &pay,
                  uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state, uint32_t *ticket,
                  hipStream_t stream, unsigned tiles) {
 	switch (pay.n) {
 	case 0:
-		LaunchTyped<R, ITEMS, PF, KeyT, 0>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
+		LaunchTyped<R, ITEMS, PF, BLOCK, KeyT, 0>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
 		                                   stream, tiles);
 		break;
 	case 1:
-		LaunchTyped<R, ITEMS, PF, KeyT, 1>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
+		LaunchTyped<R, ITEMS, PF, BLOCK, KeyT, 1>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
 		                                   stream, tiles);
 		break;
 	case 2:
-		LaunchTyped<R, ITEMS, PF, KeyT, 2>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
+		LaunchTyped<R, ITEMS, PF, BLOCK, KeyT, 2>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
 		                                   stream, tiles);
 		break;
 	case 3:
 	case 4:
-		LaunchTyped<R, ITEMS, PF, KeyT, 4>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
+		LaunchTyped<R, ITEMS, PF, BLOCK, KeyT, 4>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
 		                                   stream, tiles);
 		break;
 	default:
@@ -363,15 +366,15 @@ void LaunchKeyed(bool first, const KeyPack &pack, const void *keys_in, void *key
 	}
 }
 
-template <int R, int ITEMS, bool PF>
+template <int R, int ITEMS, bool PF, int BLOCK>
 void LaunchVariant(bool first, bool keys64, const KeyPack &pack, const void *keys_in, void *keys_out,
                    const PayloadArgs &pay, uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state,
                    uint32_t *ticket, hipStream_t stream, unsigned tiles) {
 	if (keys64) {
-		LaunchKeyed<R, ITEMS, PF, u64>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket, stream,
+		LaunchKeyed<R, ITEMS, PF, BLOCK, u64>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket, stream,
 		                               tiles);
 	} else {
-		LaunchKeyed<R, ITEMS, PF, uint32_t>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
+		LaunchKeyed<R, ITEMS, PF, BLOCK, uint32_t>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
 		                                    stream, tiles);
 	}
 	VGPU_HIP(hipGetLastError());

@@ -217,13 +217,22 @@ int EffectiveItems(int R, int items) {
 	return (R == 8 && items == 32) ? 32 : 16;
 }
 
+//! Threads per tile of the kernel variant that runs: 512 only for 8-bit digits at 16 items without the
+//! payload prefetch (the one 512-thread variant built), else 256.
+int EffectiveBlock(int R, const SortTuning &t) {
+	return (R == 8 && t.block == 512 && EffectiveItems(R, t.items) == 16 && t.prefetch == 0) ? 512 : kSortBlock;
+}
+
 //! One digit pass with the variant the tuning selects: 8-bit digits come with 16 or 32 items per thread and
-//! with or without the payload prefetch; 11-bit digits with 16 items and the prefetch.
+//! with or without the payload prefetch, or as 512 threads x 16 items without it; 11-bit digits with 16
+//! items and the prefetch.
 void LaunchPass(int R, bool keys64, bool first, const KeyPack &pack, const void *keys_in, void *keys_out,
                 const PayloadArgs &pay, uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state,
-                uint32_t *ticket, hipStream_t stream, int items, bool prefetch) {
-	const int it = EffectiveItems(R, items);
-	const uint64_t tile_rows = (uint64_t)kSortBlock * it;
+                uint32_t *ticket, hipStream_t stream, const SortTuning &tuning) {
+	const int it = EffectiveItems(R, tuning.items);
+	const int block = EffectiveBlock(R, tuning);
+	const bool prefetch = tuning.prefetch != 0;
+	const uint64_t tile_rows = (uint64_t)block * it;
 	const unsigned tiles = (unsigned)((n + tile_rows - 1) / tile_rows);
 	const size_t kw = keys64 ? 8 : 4;
 	double key_in = first ? 0.0 : (double)kw;
@@ -238,22 +247,25 @@ void LaunchPass(int R, bool keys64, bool first, const KeyPack &pack, const void 
 	}
 	ProfileScope prof(first ? "radix_pass_first" : "radix_pass", stream, (double)n * (key_in + kw + pay_bytes));
 	if (R == 11) {
-		LaunchVariant<11, 16, true>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
-		                            stream, tiles);
+		LaunchVariant<11, 16, true, 256>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
+		                                 ticket, stream, tiles);
+	} else if (block == 512) {
+		LaunchVariant<8, 16, false, 512>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
+		                                 ticket, stream, tiles);
 	} else if (it == 32) {
 		if (prefetch) {
-			LaunchVariant<8, 32, true>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
-			                           stream, tiles);
+			LaunchVariant<8, 32, true, 256>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
+			                                ticket, stream, tiles);
 		} else {
-			LaunchVariant<8, 32, false>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
-			                            ticket, stream, tiles);
+			LaunchVariant<8, 32, false, 256>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
+			                                 ticket, stream, tiles);
 		}
 	} else if (prefetch) {
-		LaunchVariant<8, 16, true>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
-		                           stream, tiles);
+		LaunchVariant<8, 16, true, 256>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
+		                                ticket, stream, tiles);
 	} else {
-		LaunchVariant<8, 16, false>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
-		                            stream, tiles);
+		LaunchVariant<8, 16, false, 256>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
+		                                 ticket, stream, tiles);
 	}
 }
 
@@ -434,7 +446,7 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 	out.passes = P;
 	out.radix_bits = widest;
 	const int RADIX = 1 << R;
-	const uint64_t tile_rows = (uint64_t)kSortBlock * EffectiveItems(R, tuning.items);
+	const uint64_t tile_rows = (uint64_t)EffectiveBlock(R, tuning) * EffectiveItems(R, tuning.items);
 	const uint64_t tiles = (n + tile_rows - 1) / tile_rows;
 
 	// Scratch: ping-pong keys and payloads; histogram; per-pass tile state + ticket.
@@ -489,7 +501,7 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 		const uint32_t *gh = hist + (uint64_t)pass * RADIX;
 		bool first = pass == 0;
 		LaunchPass(R, keys64, first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass], passes.bits[pass], gh,
-		           st, tickets + pass, stream, tuning.items, tuning.prefetch != 0);
+		           st, tickets + pass, stream, tuning);
 		cur_keys = kbuf[slot];
 		for (int p = 0; p < npayload; p++) {
 			cur_pay[p] = pbuf[p][slot];

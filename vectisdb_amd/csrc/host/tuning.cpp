@@ -10,8 +10,17 @@
 namespace vgpu {
 
 namespace {
-//! Optional overrides for sweeps: VGPU_SORT_ITEMS (16/32), VGPU_SORT_PREFETCH (0/1).
+//! Optional overrides for sweeps: VGPU_SORT_ITEMS (16/32), VGPU_SORT_PREFETCH (0/1), VGPU_SORT_BLOCK
+//! (256/512; 512 runs with 16 items and no prefetch only).
 void ApplyVariantOverrides(SortTuning &t) {
+	if (const char *env = std::getenv("VGPU_SORT_BLOCK")) {
+		t.block = std::atoi(env) >= 512 ? 512 : 256;
+		if (t.block == 512) {
+			t.items = 16;
+			t.prefetch = 0;
+		}
+		t.fixed = true;
+	}
 	if (const char *env = std::getenv("VGPU_SORT_ITEMS")) {
 		t.items = std::atoi(env) >= 32 ? 32 : 16;
 		t.fixed = true;
