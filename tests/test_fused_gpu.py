@@ -161,6 +161,25 @@ def test_cross_filter_small_brute_force(dev):
         np.testing.assert_array_equal(outs[1].cpu().numpy(), ey)
 
 
+@pytest.mark.parametrize("nl,nr", [(300, 16384), (700, 9001), (257, 65536 + 3)])
+def test_cross_filter_large_exact(dev, nl, nr):
+    """Cross products of 4.9M to 16.8M pairs (hundreds of 8,192-pair tiles): every kept pair in left-major
+    order against the oracle, with the right side a multiple of the tile and not."""
+    import oracle
+    rng = np.random.default_rng(nl + nr)
+    x = rng.integers(0, 1 << 30, nl).astype(np.int32)
+    y = rng.integers(0, 1 << 30, nr).astype(np.int32)
+    assert nl * nr >= 1 << 22
+    pred = (out(f"(({col(0, capi.INT32)} + {col(1, capi.INT32)}) < 1073741824)", capi.BOOLEAN), [0, 1],
+            capi.BOOLEAN)
+    projs = [(copy_body(capi.INT32), [0], capi.INT32), (copy_body(capi.INT32), [1], capi.INT32)]
+    outs, k = capi.cross_filter([_dev(x, dev)], [_dev(y, dev)], [pred], projs)
+    ex, ey = oracle.cross_filter_pairs(x, y, 1 << 30)
+    assert k == len(ex)
+    np.testing.assert_array_equal(outs[0].cpu().numpy(), ex)
+    np.testing.assert_array_equal(outs[1].cpu().numpy(), ey)
+
+
 def test_cross_product_unfiltered_left_major(dev):
     x = np.array([10, 20, 30], np.int64)
     y = np.array([1.5, 2.5], np.float64)
