@@ -272,19 +272,41 @@ void RangeProbeAgg(int key_type, const void *probe_keys, const uint64_t *seg, co
                    uint64_t dense_min, uint64_t dense_span, uint64_t *acc, hipStream_t stream);
 
 //! SUM/COUNT of an equi-join of one probe column against a direct-address table (see BuildDenseIndex), for a
-//! probe side much larger than the table: the probe keys are first grouped by key range (256 buckets,
-//! batch by batch) so the probe reads one bucket's slice of the table at a time, from LDS when the table
-//! has a 16-bit form. acc (5 words) = {matched build rows, build payload sum lo, hi, and -- with a probe
-//! payload (an integer column of payload_width bytes, or nullptr) -- the sum of each probe row's payload
-//! times its key's build row count, lo, hi}. About n * 4.03 bytes of scratch (n * 8 more with a probe
-//! payload, span * 2 for the 16-bit form).
-void PartitionedProbeAgg(int key_type, const void *probe_keys, uint64_t n, const void *probe_payload,
-                         int payload_width, const uint64_t *dense, uint64_t dense_min, uint64_t dense_span,
-                         uint64_t *acc, Scratch &scratch, hipStream_t stream);
-//! The same with the table's 16-bit form given (or nullptr: the probe reads the 64-bit table from L2).
-void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, const void *probe_payload,
-                             int payload_width, const uint64_t *dense, const uint16_t *dense16, uint64_t dense_min,
-                             uint64_t dense_span, uint64_t *acc, Scratch &scratch, hipStream_t stream);
+//! probe side much larger than the table, in two launches that may run on different streams: the probe
+//! keys are grouped by key range (256 buckets, batch by batch; LaunchProbePartition), then each bucket
+//! reads its slice of the table once, from LDS when the table has a 16-bit form (LaunchPartitionedProbe).
+//! The partition needs only the table's key range, so it can run while the table is being built.
+struct ProbePartition {
+	int key_type = VT_INT64;
+	uint64_t n = 0;
+	bool has_payload = false;
+	int payload_width = 0;
+	uint64_t dense_min = 0; // ordered as uint64 of the sign-extended key
+	uint64_t span = 0;
+	int shift = 0;           // bucket = (key - dense_min) >> shift
+	uint64_t batch = 0;      // probe rows per scatter batch
+	uint64_t nbatches = 0;
+	uint32_t *part = nullptr; // table offsets of the probe rows, grouped by bucket within each batch
+	void *part_y = nullptr;   // their payload values (4 or 8 bytes), in the same order
+	uint32_t *tab = nullptr;  // per batch: 257 bucket starts
+	uint64_t *partials = nullptr;
+};
+//! Checks the shape and allocates the partition's buffers from `scratch`: about n * 4.03 bytes (n * 8 more
+//! with a probe payload, an integer column of payload_width bytes).
+ProbePartition PlanProbePartition(int key_type, uint64_t n, bool has_payload, int payload_width,
+                                  uint64_t dense_min, uint64_t dense_span, Scratch &scratch);
+//! Groups the probe keys (and payload) by key range into the plan's buffers.
+void LaunchProbePartition(const ProbePartition &p, const void *probe_keys, const void *probe_payload,
+                          hipStream_t stream);
+//! Probes the partition against the table (dense16: its 16-bit form, or nullptr to read the 64-bit table
+//! from L2). acc (5 words) = {matched build rows, build payload sum lo, hi, and -- with a probe payload --
+//! the sum of each probe row's payload times its key's build row count, lo, hi}.
+void LaunchPartitionedProbe(const ProbePartition &p, const uint64_t *dense, const uint16_t *dense16, uint64_t *acc,
+                            hipStream_t stream);
+//! The table's 16-bit form (4-bit count, 12-bit sum) when its span allows it (else nullptr); *nofit_dev
+//! becomes nonzero when an entry does not fit, in which case the form must not be used. No host sync.
+const uint16_t *LaunchNarrowDense(const uint64_t *dense, uint64_t dense_span, unsigned int *nofit_dev,
+                                  Scratch &scratch, hipStream_t stream);
 
 //! Direct-address table (BuildDenseIndex's format) straight from a sorted build side: keys are the packed
 //! sort keys of one column (key = ((packed + pack_min) ^ flip) as sign-extended value; `flip` is the top
@@ -294,6 +316,10 @@ void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, c
 unsigned BuildDenseFromRun(const void *keys, bool keys64, const void *payload, int payload_width, uint64_t n,
                            uint64_t pack_min, uint64_t flip, uint64_t dense_min, uint64_t span, uint64_t *dense,
                            unsigned int *flags_dev, hipStream_t stream);
+//! The same without the read-back: the flags stay in *flags_dev.
+void LaunchDenseFromRun(const void *keys, bool keys64, const void *payload, int payload_width, uint64_t n,
+                        uint64_t pack_min, uint64_t flip, uint64_t dense_min, uint64_t span, uint64_t *dense,
+                        unsigned int *flags_dev, hipStream_t stream);
 
 //! Direct-address form of a join index whose keys span [min, min + span): dense[key - min] packs the key's
 //! row count (top 23 bits) and payload sum (41-bit two's complement); 0 = no build row. Returns false (the

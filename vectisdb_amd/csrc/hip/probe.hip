@@ -572,52 +572,54 @@ __global__ void __launch_bounds__(kWave) PartFinalKernel(const u64 *partials, un
 	}
 }
 
+//! The batch scatter of one probe side (key type K, payload Y or void).
 template <typename K, typename Y>
-void LaunchPartitionedY(const K *keys, uint64_t n, const void *y, int ywidth, uint64_t dmin, uint64_t span, int shift,
-                        const uint64_t *dense, const uint16_t *dense16, uint32_t *part, void *part_y, uint32_t *tab,
-                        uint64_t batch, uint64_t *partials, unsigned nparts, hipStream_t stream) {
+void LaunchScatterY(const ProbePartition &p, const K *keys, const void *y, hipStream_t stream) {
 	// with a payload (4 or 8 bytes) a thread holds 16 rows: 32 rows of offsets, ranks and payloads cost more in
 	// registers than the longer bucket runs save
 	constexpr int kItems = std::is_void<Y>::value ? kScatterItems : kScatterItemsY;
-	const uint64_t nbatches = (n + batch - 1) / batch;
-	const double ybytes = std::is_void<Y>::value ? 0.0 : (double)ywidth + 2.0 * sizeof(typename std::conditional<
+	const double ybytes = std::is_void<Y>::value ? 0.0 : (double)p.payload_width + 2.0 * sizeof(typename std::conditional<
 	                                                                                std::is_void<Y>::value, int, Y>::type);
-	{
-		ProfileScope prof("partition_scatter", stream, (double)n * (sizeof(K) + 4) + (double)n * ybytes / 2);
-		hipLaunchKernelGGL((PartBatchKernel<K, kItems, Y>), dim3((unsigned)nbatches), dim3(kPartBlock), 0, stream,
-		                   keys, (u64)n, (u64)dmin, (u64)span, shift, y, ywidth, part, part_y, tab);
-		VGPU_HIP(hipGetLastError());
-	}
-	{
-		ProfileScope prof("partition_probe", stream, (double)n * 4 + (double)n * ybytes / 2);
-		if (dense16 != nullptr && (1ull << shift) <= kSliceEntries) {
-			hipLaunchKernelGGL((PartProbeKernel<true, Y>), dim3(nparts), dim3(kPartBlock), 0, stream, part, part_y,
-			                   tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
-		} else {
-			hipLaunchKernelGGL((PartProbeKernel<false, Y>), dim3(nparts), dim3(kPartBlock), 0, stream, part, part_y,
-			                   tab, (u64)nbatches, (u64)batch, dense16, dense, (u64)span, shift, partials);
-		}
-		VGPU_HIP(hipGetLastError());
-	}
+	ProfileScope prof("partition_scatter", stream, (double)p.n * (sizeof(K) + 4) + (double)p.n * ybytes / 2);
+	hipLaunchKernelGGL((PartBatchKernel<K, kItems, Y>), dim3((unsigned)p.nbatches), dim3(kPartBlock), 0, stream,
+	                   keys, (u64)p.n, (u64)p.dense_min, (u64)p.span, p.shift, y, p.payload_width, p.part, p.part_y,
+	                   p.tab);
+	VGPU_HIP(hipGetLastError());
 }
 
 template <typename K>
-void LaunchPartitioned(const void *keys_v, uint64_t n, const void *y, int ywidth, uint64_t dmin, uint64_t span,
-                       int shift, const uint64_t *dense, const uint16_t *dense16, uint32_t *part, void *part_y,
-                       uint32_t *tab, uint64_t batch, uint64_t *partials, uint64_t *acc, hipStream_t stream) {
+void LaunchScatter(const ProbePartition &p, const void *keys_v, const void *y, hipStream_t stream) {
 	const K *keys = static_cast<const K *>(keys_v);
-	const unsigned nparts = kPartBuckets * kSliceSplits;
 	if (y == nullptr) {
-		LaunchPartitionedY<K, void>(keys, n, y, ywidth, dmin, span, shift, dense, dense16, part, part_y, tab, batch,
-		                            partials, nparts, stream);
-	} else if (ywidth <= 4) {
-		LaunchPartitionedY<K, int32_t>(keys, n, y, ywidth, dmin, span, shift, dense, dense16, part, part_y, tab,
-		                               batch, partials, nparts, stream);
+		LaunchScatterY<K, void>(p, keys, y, stream);
+	} else if (p.payload_width <= 4) {
+		LaunchScatterY<K, int32_t>(p, keys, y, stream);
 	} else {
-		LaunchPartitionedY<K, long long>(keys, n, y, ywidth, dmin, span, shift, dense, dense16, part, part_y, tab,
-		                                 batch, partials, nparts, stream);
+		LaunchScatterY<K, long long>(p, keys, y, stream);
 	}
-	hipLaunchKernelGGL(PartFinalKernel, dim3(1), dim3(kWave), 0, stream, partials, nparts, acc);
+}
+
+//! The per-bucket probe of a partitioned probe side (payload Y or void) and the final fold into acc.
+template <typename Y>
+void LaunchBucketProbeY(const ProbePartition &p, const uint64_t *dense, const uint16_t *dense16, uint64_t *acc,
+                        hipStream_t stream) {
+	const unsigned nparts = kPartBuckets * kSliceSplits;
+	const double ybytes = std::is_void<Y>::value ? 0.0 : (double)p.payload_width + 2.0 * sizeof(typename std::conditional<
+	                                                                                std::is_void<Y>::value, int, Y>::type);
+	{
+		ProfileScope prof("partition_probe", stream, (double)p.n * 4 + (double)p.n * ybytes / 2);
+		if (dense16 != nullptr && (1ull << p.shift) <= kSliceEntries) {
+			hipLaunchKernelGGL((PartProbeKernel<true, Y>), dim3(nparts), dim3(kPartBlock), 0, stream, p.part,
+			                   p.part_y, p.tab, (u64)p.nbatches, (u64)p.batch, dense16, dense, (u64)p.span, p.shift,
+			                   p.partials);
+		} else {
+			hipLaunchKernelGGL((PartProbeKernel<false, Y>), dim3(nparts), dim3(kPartBlock), 0, stream, p.part,
+			                   p.part_y, p.tab, (u64)p.nbatches, (u64)p.batch, dense16, dense, (u64)p.span, p.shift,
+			                   p.partials);
+		}
+		VGPU_HIP(hipGetLastError());
+	}
+	hipLaunchKernelGGL(PartFinalKernel, dim3(1), dim3(kWave), 0, stream, p.partials, nparts, acc);
 	VGPU_HIP(hipGetLastError());
 }
 
@@ -718,9 +720,9 @@ __global__ void __launch_bounds__(256) DenseFromRunKernel(const KeyT *keys, cons
 
 } // namespace
 
-unsigned BuildDenseFromRun(const void *keys, bool keys64, const void *payload, int payload_width, uint64_t n,
-                           uint64_t pack_min, uint64_t flip, uint64_t dense_min, uint64_t span, uint64_t *dense,
-                           unsigned int *flags_dev, hipStream_t stream) {
+void LaunchDenseFromRun(const void *keys, bool keys64, const void *payload, int payload_width, uint64_t n,
+                        uint64_t pack_min, uint64_t flip, uint64_t dense_min, uint64_t span, uint64_t *dense,
+                        unsigned int *flags_dev, hipStream_t stream) {
 	FillU64(dense, 0ull, span, stream);
 	FillU32(flags_dev, 0u, 1, stream);
 	if (n > 0) {
@@ -741,81 +743,109 @@ unsigned BuildDenseFromRun(const void *keys, bool keys64, const void *payload, i
 		}
 		VGPU_HIP(hipGetLastError());
 	}
+}
+
+unsigned BuildDenseFromRun(const void *keys, bool keys64, const void *payload, int payload_width, uint64_t n,
+                           uint64_t pack_min, uint64_t flip, uint64_t dense_min, uint64_t span, uint64_t *dense,
+                           unsigned int *flags_dev, hipStream_t stream) {
+	LaunchDenseFromRun(keys, keys64, payload, payload_width, n, pack_min, flip, dense_min, span, dense, flags_dev,
+	                   stream);
 	unsigned int host = 0;
 	VGPU_HIP(hipMemcpyAsync(&host, flags_dev, 4, hipMemcpyDeviceToHost, stream));
 	VGPU_HIP(hipStreamSynchronize(stream));
 	return host;
 }
 
-void PartitionedProbeAgg(int key_type, const void *probe_keys, uint64_t n, const void *probe_payload,
-                         int payload_width, const uint64_t *dense, uint64_t dense_min, uint64_t dense_span,
-                         uint64_t *acc, Scratch &scratch, hipStream_t stream) {
-	if (dense != nullptr && dense_span > 0 && dense_span <= kPartBuckets * kSliceEntries && n > 0) {
-		// 16-bit table when every entry fits: the probe then reads bucket slices from LDS.
-		auto *dense16 = static_cast<uint16_t *>(scratch.Alloc(dense_span * 2));
-		auto *nofit = static_cast<unsigned int *>(scratch.Alloc(4));
-		FillU32(nofit, 0u, 1, stream);
-		const unsigned grid = (unsigned)std::min<uint64_t>((dense_span + 255) / 256, 4096);
-		hipLaunchKernelGGL(NarrowDenseKernel, dim3(grid), dim3(256), 0, stream, dense, (u64)dense_span, dense16,
-		                   nofit);
-		VGPU_HIP(hipGetLastError());
-		unsigned int host = 1;
-		VGPU_HIP(hipMemcpyAsync(&host, nofit, 4, hipMemcpyDeviceToHost, stream));
-		VGPU_HIP(hipStreamSynchronize(stream));
-		PartitionedProbeAggWith(key_type, probe_keys, n, probe_payload, payload_width, dense,
-		                        host == 0 ? dense16 : nullptr, dense_min, dense_span, acc, scratch, stream);
-		return;
+const uint16_t *LaunchNarrowDense(const uint64_t *dense, uint64_t dense_span, unsigned int *nofit_dev,
+                                  Scratch &scratch, hipStream_t stream) {
+	if (dense == nullptr || dense_span == 0 || dense_span > (uint64_t)kPartBuckets * kSliceEntries) {
+		FillU32(nofit_dev, 1u, 1, stream);
+		return nullptr;
 	}
-	PartitionedProbeAggWith(key_type, probe_keys, n, probe_payload, payload_width, dense, nullptr, dense_min,
-	                        dense_span, acc, scratch, stream);
+	auto *dense16 = static_cast<uint16_t *>(scratch.Alloc(dense_span * 2));
+	FillU32(nofit_dev, 0u, 1, stream);
+	const unsigned grid = (unsigned)std::min<uint64_t>((dense_span + 255) / 256, 4096);
+	hipLaunchKernelGGL(NarrowDenseKernel, dim3(grid), dim3(256), 0, stream, dense, (u64)dense_span, dense16,
+	                   nofit_dev);
+	VGPU_HIP(hipGetLastError());
+	return dense16;
 }
 
-void PartitionedProbeAggWith(int key_type, const void *probe_keys, uint64_t n, const void *probe_payload,
-                             int payload_width, const uint64_t *dense, const uint16_t *dense16, uint64_t dense_min,
-                             uint64_t dense_span, uint64_t *acc, Scratch &scratch, hipStream_t stream) {
-	if (dense == nullptr || dense_span == 0 || dense_span > (1ull << 32)) {
-		throw HipError("PartitionedProbeAgg: needs a direct-address table of at most 2^32 entries");
+ProbePartition PlanProbePartition(int key_type, uint64_t n, bool has_payload, int payload_width,
+                                  uint64_t dense_min, uint64_t dense_span, Scratch &scratch) {
+	if (dense_span == 0 || dense_span > (1ull << 32)) {
+		throw HipError("partitioned probe: needs a direct-address table of at most 2^32 entries");
 	}
-	if (probe_payload != nullptr && !(payload_width == 1 || payload_width == 2 || payload_width == 4 ||
-	                                  payload_width == 8)) {
-		throw HipError("PartitionedProbeAgg: probe payload width must be 1, 2, 4 or 8 bytes");
+	if (has_payload && !(payload_width == 1 || payload_width == 2 || payload_width == 4 || payload_width == 8)) {
+		throw HipError("partitioned probe: probe payload width must be 1, 2, 4 or 8 bytes");
 	}
-	FillU64(acc, 0ull, 5, stream);
-	if (n == 0) {
-		return;
+	if (key_type != VT_INT16 && key_type != VT_INT32 && key_type != VT_INT64) {
+		throw HipError("partitioned probe: join keys must be INT16, INT32 or INT64");
 	}
+	ProbePartition p;
+	p.key_type = key_type;
+	p.n = n;
+	p.has_payload = has_payload;
+	p.payload_width = has_payload ? payload_width : 0;
+	p.dense_min = dense_min;
+	p.span = dense_span;
 	int bits = 0;
 	while (bits < 64 && (dense_span - 1) >> bits) {
 		bits++;
 	}
-	const int shift = bits > 8 ? bits - 8 : 0; // bucket = offset >> shift < 256
-	const uint64_t batch = (uint64_t)kPartBlock * (probe_payload != nullptr ? kScatterItemsY : kScatterItems);
-	const uint64_t nbatches = (n + batch - 1) / batch;
-	if (nbatches > 0x7fffffffull) {
-		throw HipError("PartitionedProbeAgg: too many probe rows for one launch");
+	p.shift = bits > 8 ? bits - 8 : 0; // bucket = offset >> shift < 256
+	p.batch = (uint64_t)kPartBlock * (has_payload ? kScatterItemsY : kScatterItems);
+	p.nbatches = (n + p.batch - 1) / p.batch;
+	if (p.nbatches > 0x7fffffffull) {
+		throw HipError("partitioned probe: too many probe rows for one launch");
 	}
-	auto *part = static_cast<uint32_t *>(scratch.Alloc(nbatches * batch * 4));
-	void *part_y = nullptr;
-	if (probe_payload != nullptr) {
-		part_y = scratch.Alloc(nbatches * batch * (payload_width <= 4 ? 4 : 8));
+	if (n > 0) {
+		p.part = static_cast<uint32_t *>(scratch.Alloc(p.nbatches * p.batch * 4));
+		if (has_payload) {
+			p.part_y = scratch.Alloc(p.nbatches * p.batch * (payload_width <= 4 ? 4 : 8));
+		}
+		p.tab = static_cast<uint32_t *>(scratch.Alloc(p.nbatches * (kPartBuckets + 1) * 4));
 	}
-	auto *tab = static_cast<uint32_t *>(scratch.Alloc(nbatches * (kPartBuckets + 1) * 4));
-	auto *partials = static_cast<uint64_t *>(scratch.Alloc((size_t)kPartBuckets * kSliceSplits * 5 * 8));
-	switch (key_type) {
+	p.partials = static_cast<uint64_t *>(scratch.Alloc((size_t)kPartBuckets * kSliceSplits * 5 * 8));
+	return p;
+}
+
+void LaunchProbePartition(const ProbePartition &p, const void *probe_keys, const void *probe_payload,
+                          hipStream_t stream) {
+	if (p.n == 0) {
+		return;
+	}
+	if ((probe_payload != nullptr) != p.has_payload) {
+		throw HipError("partitioned probe: probe payload does not match the plan");
+	}
+	switch (p.key_type) {
 	case VT_INT16:
-		LaunchPartitioned<int16_t>(probe_keys, n, probe_payload, payload_width, dense_min, dense_span, shift, dense,
-		                           dense16, part, part_y, tab, batch, partials, acc, stream);
+		LaunchScatter<int16_t>(p, probe_keys, probe_payload, stream);
 		break;
 	case VT_INT32:
-		LaunchPartitioned<int32_t>(probe_keys, n, probe_payload, payload_width, dense_min, dense_span, shift, dense,
-		                           dense16, part, part_y, tab, batch, partials, acc, stream);
-		break;
-	case VT_INT64:
-		LaunchPartitioned<int64_t>(probe_keys, n, probe_payload, payload_width, dense_min, dense_span, shift, dense,
-		                           dense16, part, part_y, tab, batch, partials, acc, stream);
+		LaunchScatter<int32_t>(p, probe_keys, probe_payload, stream);
 		break;
 	default:
-		throw HipError("PartitionedProbeAgg: join keys must be INT16, INT32 or INT64");
+		LaunchScatter<int64_t>(p, probe_keys, probe_payload, stream);
+		break;
+	}
+}
+
+void LaunchPartitionedProbe(const ProbePartition &p, const uint64_t *dense, const uint16_t *dense16, uint64_t *acc,
+                            hipStream_t stream) {
+	if (dense == nullptr) {
+		throw HipError("partitioned probe: needs a direct-address table");
+	}
+	FillU64(acc, 0ull, 5, stream);
+	if (p.n == 0) {
+		return;
+	}
+	if (!p.has_payload) {
+		LaunchBucketProbeY<void>(p, dense, dense16, acc, stream);
+	} else if (p.payload_width <= 4) {
+		LaunchBucketProbeY<int32_t>(p, dense, dense16, acc, stream);
+	} else {
+		LaunchBucketProbeY<long long>(p, dense, dense16, acc, stream);
 	}
 }
 

@@ -9,8 +9,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace vgpu {
 
@@ -165,6 +168,89 @@ void FinishJoinAggregate(const JoinAggValue &v, int agg_kind, int out_type, uint
 //! XCD's L2 is 4 MB).
 constexpr uint64_t kPartitionTableBytes = 4ull << 20;
 
+namespace {
+
+//! A second stream (default priority) on which the probe side is partitioned while the build side is
+//! sorted and reduced on the caller's stream, with the two events that order it: `ready` (recorded on the
+//! caller's stream, waited by this one) and `done` (the reverse). Streams are pooled per device and reused;
+//! the destructor waits for everything issued on the stream, so no exit path can return scratch memory to
+//! the pool while the partition still writes it.
+class SideStream {
+public:
+	explicit SideStream(hipStream_t caller) : caller_(caller) {
+		VGPU_HIP(hipGetDevice(&device_));
+		{
+			std::lock_guard<std::mutex> lock(Mutex());
+			auto &free_list = Pool()[device_];
+			if (!free_list.empty()) {
+				stream_ = free_list.back();
+				free_list.pop_back();
+			}
+		}
+		if (stream_ == nullptr) {
+			VGPU_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+		}
+		if (hipEventCreateWithFlags(&ready_, hipEventDisableTiming) != hipSuccess ||
+		    hipEventCreateWithFlags(&done_, hipEventDisableTiming) != hipSuccess) {
+			Close();
+			throw HipError("join: cannot create stream events");
+		}
+	}
+	~SideStream() {
+		Close();
+	}
+	SideStream(const SideStream &) = delete;
+	SideStream &operator=(const SideStream &) = delete;
+
+	//! Work issued on stream() after this call starts after everything issued on the caller's stream so far.
+	hipStream_t Begin() {
+		VGPU_HIP(hipEventRecord(ready_, caller_));
+		VGPU_HIP(hipStreamWaitEvent(stream_, ready_, 0));
+		return stream_;
+	}
+	//! Work issued on the caller's stream after this call starts after everything issued on stream().
+	void Join() {
+		VGPU_HIP(hipEventRecord(done_, stream_));
+		VGPU_HIP(hipStreamWaitEvent(caller_, done_, 0));
+	}
+
+private:
+	static std::mutex &Mutex() {
+		static std::mutex m;
+		return m;
+	}
+	static std::map<int, std::vector<hipStream_t>> &Pool() {
+		static std::map<int, std::vector<hipStream_t>> pool;
+		return pool;
+	}
+	void Close() {
+		if (stream_ != nullptr) {
+			const bool ok = hipStreamSynchronize(stream_) == hipSuccess;
+			if (ok) {
+				std::lock_guard<std::mutex> lock(Mutex());
+				Pool()[device_].push_back(stream_);
+			}
+			stream_ = nullptr; // a stream left in error is not reused
+		}
+		if (ready_ != nullptr) {
+			(void)hipEventDestroy(ready_);
+			ready_ = nullptr;
+		}
+		if (done_ != nullptr) {
+			(void)hipEventDestroy(done_);
+			done_ = nullptr;
+		}
+	}
+
+	hipStream_t caller_;
+	int device_ = 0;
+	hipStream_t stream_ = nullptr;
+	hipEvent_t ready_ = nullptr;
+	hipEvent_t done_ = nullptr;
+};
+
+} // namespace
+
 //! Aggregate join by probing: when every aggregate is a COUNT or an integer SUM/AVG of one column of the
 //! smaller (build) side or of one column of the larger (probe) side, the build side is reduced to a
 //! direct-address table (per key: row count and payload sum) and the probe side is read once, each row
@@ -180,10 +266,7 @@ bool ProbeJoinAggregate(const DevCol &probe, const DevCol *probe_payload, const 
 	if (probe_payload != nullptr && probe_payload->rows != probe.rows) {
 		throw std::runtime_error("gpu_hash_join: aggregate input row count mismatch");
 	}
-	// The table: the build side sorted (one key column, payload alongside) and each key's run turned into its
-	// entry in one pass (no per-key arrays); not built when the build keys span so many values that even
-	// all-distinct keys could not fill half of it. A key with more than 64 build rows sends the build through
-	// the GROUP BY operator instead.
+	// The build keys' range decides whether a direct-address table is possible at all.
 	const uint64_t rows = build.rows;
 	auto *h8 = static_cast<uint32_t *>(scratch.Alloc((8 * 256 + 4) * sizeof(uint32_t)));
 	uint64_t omin = 0, omax = 0;
@@ -196,6 +279,31 @@ bool ProbeJoinAggregate(const DevCol &probe, const DevCol *probe_payload, const 
 	std::unique_ptr<JoinIndex> index;
 	const uint64_t *dense = nullptr;
 	uint64_t dense_min = omin ^ flip, dense_span = span;
+	if (payload != nullptr && payload->rows != rows) {
+		throw std::runtime_error("gpu_hash_join: aggregate input row count mismatch");
+	}
+	// The table is larger than one XCD's L2 and every line of it is read several times (or the probe payload
+	// has to travel with its key): the probe keys are grouped by key range first. The grouping needs only the
+	// key range, so it runs on a second stream while the build side is sorted and reduced on this one.
+	const bool partition =
+	    span <= (1ull << 32) && ((span * 8 > kPartitionTableBytes && probe.rows >= span) || probe_payload != nullptr);
+	if (!partition && probe_payload != nullptr) {
+		return false;
+	}
+	std::unique_ptr<SideStream> side;
+	ProbePartition part;
+	if (partition) {
+		part = PlanProbePartition(probe.type, probe.rows, probe_payload != nullptr,
+		                          probe_payload ? TypeWidth(probe_payload->type) : 0, dense_min, span, scratch);
+		side.reset(new SideStream(stream));
+		LaunchProbePartition(part, probe.data, probe_payload ? probe_payload->data : nullptr, side->Begin());
+	}
+	// The table: the build side sorted (one key column, payload alongside) and each key's run turned into its
+	// entry in one pass (no per-key arrays); not built when the build keys span so many values that even
+	// all-distinct keys could not fill half of it. A key with more than 64 build rows sends the build through
+	// the GROUP BY operator instead. With the partition, the table's 16-bit form is made in the same go, and
+	// one read-back returns both flags.
+	const uint16_t *dense16 = nullptr;
 	{
 		KeyPack pack;
 		PlanKeyPackAligned(build.data, build.type, omin, omax, pack);
@@ -205,20 +313,26 @@ bool ProbeJoinAggregate(const DevCol &probe, const DevCol *probe_payload, const 
 		}
 		const void *pin[1] = {payload != nullptr ? payload->data : nullptr};
 		int pw[1] = {payload != nullptr ? TypeWidth(payload->type) : 0};
-		if (payload != nullptr && payload->rows != rows) {
-			throw std::runtime_error("gpu_hash_join: aggregate input row count mismatch");
-		}
 		SortedRun run;
 		SortPacked(pack, pin, pw, payload != nullptr ? 1 : 0, rows, tuning, run, scratch, stream);
 		auto *table = static_cast<uint64_t *>(scratch.Alloc(span * 8));
-		auto *flags = static_cast<unsigned int *>(scratch.Alloc(4));
-		const unsigned f = BuildDenseFromRun(run.keys, run.keys64, payload != nullptr ? run.payload[0] : nullptr,
-		                                     pw[0], rows, pack.min[0], flip, dense_min, span, table, flags, stream);
-		if ((f & 1u) != 0) {
+		auto *flags = static_cast<unsigned int *>(scratch.Alloc(2 * 4));
+		LaunchDenseFromRun(run.keys, run.keys64, payload != nullptr ? run.payload[0] : nullptr, pw[0], rows,
+		                   pack.min[0], flip, dense_min, span, table, flags, stream);
+		if (partition) {
+			dense16 = LaunchNarrowDense(table, span, flags + 1, scratch, stream);
+		}
+		unsigned int f[2] = {0u, 1u};
+		VGPU_HIP(hipMemcpyAsync(f, flags, (partition ? 2 : 1) * 4, hipMemcpyDeviceToHost, stream));
+		VGPU_HIP(hipStreamSynchronize(stream));
+		if ((f[0] & 1u) != 0) {
 			return false; // an entry does not pack
 		}
 		dense = table;
-		if ((f & 2u) != 0) {
+		if (f[1] != 0) {
+			dense16 = nullptr; // an entry does not fit 16 bits
+		}
+		if ((f[0] & 2u) != 0) {
 			DevCol pay {};
 			if (payload != nullptr) {
 				pay = *payload;
@@ -228,20 +342,30 @@ bool ProbeJoinAggregate(const DevCol &probe, const DevCol *probe_payload, const 
 				return false;
 			}
 			dense = index->dense;
+			if (partition && (index->dense_min != dense_min || index->dense_span != dense_span)) {
+				// the index spans other keys than the key-range pass found: group the probe side again
+				part = PlanProbePartition(probe.type, probe.rows, probe_payload != nullptr,
+				                          probe_payload ? TypeWidth(probe_payload->type) : 0, index->dense_min,
+				                          index->dense_span, scratch);
+				LaunchProbePartition(part, probe.data, probe_payload ? probe_payload->data : nullptr, stream);
+			}
 			dense_min = index->dense_min;
 			dense_span = index->dense_span;
+			dense16 = nullptr;
+			if (partition) {
+				dense16 = LaunchNarrowDense(dense, dense_span, flags + 1, scratch, stream);
+				VGPU_HIP(hipMemcpyAsync(&f[1], flags + 1, 4, hipMemcpyDeviceToHost, stream));
+				VGPU_HIP(hipStreamSynchronize(stream));
+				if (f[1] != 0) {
+					dense16 = nullptr;
+				}
+			}
 		}
 	}
 	auto *acc = static_cast<uint64_t *>(scratch.Alloc(5 * 8));
-	const bool partition = dense_span * 8 > kPartitionTableBytes && probe.rows >= dense_span;
-	if (dense_span <= (1ull << 32) && (partition || probe_payload != nullptr)) {
-		// The table is larger than one XCD's L2 and every line of it is read several times (or the probe
-		// payload has to travel with its key): group the probe keys by key range first.
-		PartitionedProbeAgg(probe.type, probe.data, probe.rows, probe_payload ? probe_payload->data : nullptr,
-		                    probe_payload ? TypeWidth(probe_payload->type) : 0, dense, dense_min, dense_span, acc,
-		                    scratch, stream);
-	} else if (probe_payload != nullptr) {
-		return false;
+	if (partition) {
+		side->Join(); // the probe reads the partition
+		LaunchPartitionedProbe(part, dense, dense16, acc, stream);
 	} else {
 		const uint64_t row0 = 0, len = probe.rows;
 		const DeviceRanges r = UploadRanges(&row0, &len, 1, probe.rows, scratch, stream);
