@@ -262,7 +262,8 @@ __global__ void DenseIndexKernel(const K *keys, const u64 *count, const u64 *sum
 //      from L2 otherwise -- and sums count and payload sum per block; PartFinalKernel adds the partials.
 constexpr int kPartBuckets = 256;
 constexpr int kPartBlock = 1024;      // 16 waves
-constexpr int kPartItems = 8;         // probe: keys per thread per loop step
+constexpr int kPartItems = 32;        // probe: keys per thread per loop step (loads in flight per lane)
+constexpr int kPartItemsY = 16;       // the same with a payload (32 would spill registers)
 constexpr int kScatterItems = 32;     // keys per thread per batch (keys only: 32768-row batches)
 constexpr int kScatterItemsY = 16;    // with a probe payload (16384-row batches)
 constexpr unsigned kRunChunk = 2048;  // runs (batches) per LDS scan of run lengths
@@ -410,7 +411,8 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
                                                              int shift, u64 *partials) {
 	constexpr bool HASY = !std::is_void<Y>::value;
 	using YT = typename std::conditional<HASY, Y, int>::type;
-	__shared__ uint16_t s_slice[LDS ? kSliceEntries : 1];
+	constexpr int kItems = HASY ? kPartItemsY : kPartItems;
+	__shared__ __align__(16) uint16_t s_slice[LDS ? kSliceEntries : 1];
 	__shared__ uint32_t s_pre[kRunChunk + 1]; // exclusive prefix of run lengths
 	__shared__ uint32_t s_first[kRunChunk];   // each run's start inside its batch
 	__shared__ uint32_t s_scan[kPartBlock / kWave + 1];
@@ -422,7 +424,15 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
 	if constexpr (LDS) {
 		const u64 slice = 1ull << shift;
 		const u64 ne = e0 < span ? (span - e0 < slice ? span - e0 : slice) : 0;
-		for (u64 i = tid; i < ne; i += kPartBlock) {
+		// 16-byte pieces when the slice starts 16-byte aligned (every slice of an aligned table does), then
+		// the tail entry by entry
+		const u64 nv = (reinterpret_cast<uintptr_t>(dense16 + e0) & 15) == 0 ? ne / 8 : 0;
+		const uint4 *src = reinterpret_cast<const uint4 *>(dense16 + e0);
+		uint4 *dst = reinterpret_cast<uint4 *>(s_slice);
+		for (u64 i = tid; i < nv; i += kPartBlock) {
+			dst[i] = src[i];
+		}
+		for (u64 i = nv * 8 + tid; i < ne; i += kPartBlock) {
 			s_slice[i] = dense16[e0 + i];
 		}
 	}
@@ -451,11 +461,11 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
 			s_pre[kRunChunk] = total;
 		}
 		__syncthreads(); // also orders the slice copy before the first lookup
-		// Wave w takes 64 x kPartItems consecutive keys of the chunk's concatenated runs per step, lane l the
+		// Wave w takes 64 x kItems consecutive keys of the chunk's concatenated runs per step, lane l the
 		// keys l, l + 64, ... of them: loads are coalesced within a run. Each lane finds its first key's run
 		// by binary search (the lanes of a wave follow nearly the same path) and steps forward from there.
 		const uint32_t wave = (uint32_t)tid / kWave, lane = (uint32_t)tid % kWave;
-		for (uint32_t vb = wave * kWave * kPartItems; vb < total; vb += kPartBlock * kPartItems) {
+		for (uint32_t vb = wave * kWave * kItems; vb < total; vb += kPartBlock * kItems) {
 			const uint32_t v0 = vb + lane;
 			unsigned lo = 0, hi = nruns; // invariant: s_pre[lo] <= min(v0, total - 1) < s_pre[hi]
 			const uint32_t vs = v0 < total ? v0 : total - 1;
@@ -467,10 +477,10 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
 					hi = mid;
 				}
 			}
-			uint32_t key[kPartItems];
-			long long yv[kPartItems];
+			uint32_t key[kItems];
+			long long yv[kItems];
 #pragma unroll
-			for (int k = 0; k < kPartItems; k++) {
+			for (int k = 0; k < kItems; k++) {
 				const uint32_t v = v0 + k * kWave;
 				key[k] = (uint32_t)e0; // any offset of this bucket
 				yv[k] = 0;
@@ -486,7 +496,7 @@ __global__ void __launch_bounds__(kPartBlock) PartProbeKernel(const uint32_t *pa
 				}
 			}
 #pragma unroll
-			for (int k = 0; k < kPartItems; k++) {
+			for (int k = 0; k < kItems; k++) {
 				if (v0 + k * kWave < total) {
 					u64 ec;
 					long long es;
