@@ -1,7 +1,7 @@
 """CPU oracle for the vectis engine. TEST INFRASTRUCTURE ONLY.
 
-Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this package, and only as
-the checker (or the timed CPU baseline). Nothing under vectisdb_amd/ imports it: the product path fails
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this package, and only to
+check results against (or as the timed CPU baseline). Nothing under vectisdb_amd/ imports it: the product path fails
 loudly when its HIP library is missing instead of falling back to this code.
 
 Pinning (what keeps this restatement honest), per function -- see tests/test_oracle.py:

@@ -1,7 +1,7 @@
 /* oracle.h -- CPU restatement of the engine's hot-path semantics. TEST INFRASTRUCTURE ONLY.
  *
- * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library, and only as
- * the checker (or as the timed CPU baseline). The product path (libvgpu.so, vectisdb_amd/) never calls it.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library, and only to
+ * check results against (or as the timed CPU baseline). The product path (libvgpu.so, vectisdb_amd/) never calls it.
  *
  * What each function restates, with the reference site it follows:
  *   vo_mix_counter / vo_gen_*    the seeded column generator (same mixer as the device generator)

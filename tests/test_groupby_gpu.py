@@ -1,7 +1,7 @@
 """GROUP BY parity: the HIP radix-sort + segmented-reduce path (through the C ABI) against the oracle.
 
 Bar: bit-exact for keys, COUNT, SUM(int) (HUGEINT), MIN, MAX; DOUBLE SUM/AVG within 1e-12 relative of the
-correctly rounded sum (tolerance stated in the north-star contract; the engine's double-double accumulation
+correctly rounded sum (the floating-point tolerance of this path; the engine's double-double accumulation
 is normally exact).
 """
 import numpy as np

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker. CPU only.
+"""Pins the CPU oracle (oracle/oracle.c) before results are checked against it. CPU only.
 
 Each oracle function is checked against something independent of its own formula: Python's math.fsum
 (Shewchuk's exact summation, a library routine), exact rational arithmetic (fractions.Fraction), plain
