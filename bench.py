@@ -199,6 +199,28 @@ def bench_filter_project_large(torch, capi, dev, steps, rows=1 << 28):
             "algorithmic_bytes": alg, "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def bench_filter_gt(torch, capi, dev, steps, rows=1 << 28):
+    """FilterGreaterThanInt32 (vgpu_filter_gt_i32) over 2^28 INT32 rows uniform in [0, 2^30) with the threshold
+    at the middle: about half the rows kept, in order, by one look-back compaction kernel."""
+    x = torch.empty(rows, dtype=torch.int32, device=dev)
+    capi.generate_column(x, capi.INT32, 0, lo=0, hi=(1 << 30) - 1, seed=13, stream_id=1)
+    torch.cuda.synchronize()
+    threshold = 1 << 29
+    kept = capi.filter_gt_i32(x, threshold).numel()
+    capi.profile_reset()
+    capi.profile_enable(True)
+    for _ in range(steps):
+        capi.filter_gt_i32(x, threshold)
+    torch.cuda.synchronize()
+    capi.profile_enable(False)
+    avg, _ = kernel_stats(capi.profile_read(), "filter_gt_i32")
+    alg = rows * 4 + kept * 4
+    gbs = alg / (avg * 1e-3) / 1e9
+    return {"operator": "FilterGreaterThanInt32, 2^28 INT32 rows, half kept", "rows": rows, "kept": kept,
+            "kernel_ms": round(avg, 4), "rows_per_s": round(rows / (avg * 1e-3), 1), "algorithmic_bytes": alg,
+            "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
     """Join + aggregate: build 2^24 unique keys (permutation) + INT32 payload, probe 2^28 fk (90% hits);
     SELECT SUM(payload), COUNT(*) FROM probe JOIN build ON fk = key."""
@@ -665,14 +687,17 @@ def main():
         os.environ.pop("VGPU_RADIX_BITS", None)
 
     operators = []
-    extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "filter_large", "execute_plan", "cross",
-                                                                       "join", "batched", "served"]
+    extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "filter_large", "filter_gt",
+                                                                       "execute_plan", "cross", "join", "batched",
+                                                                       "served"]
     for name in extra:
         try:
             if name == "filter":
                 operators.append(bench_filter_project(torch, capi, dev, 50))
             elif name == "filter_large":
                 operators.append(bench_filter_project_large(torch, capi, dev, 10))
+            elif name == "filter_gt":
+                operators.append(bench_filter_gt(torch, capi, dev, 10))
             elif name == "execute_plan":
                 operators.append(bench_execute_plan(torch, capi, dev, 3))
             elif name == "cross":

@@ -203,12 +203,21 @@ def test_sort_column_matches_stable_argsort(dev):
 
 
 def test_filter_gt_i32(dev):
+    """FilterGreaterThanInt32: the kept values in input order, for sizes around the 8,192-row tile and
+    extreme thresholds (INT32_MAX keeps nothing, INT32_MIN everything but INT32_MIN)."""
     import torch
     rng = np.random.default_rng(23)
-    for n in [0, 1, 2047, 2048, 1_000_001]:
+    lo32, hi32 = np.iinfo(np.int32).min, np.iinfo(np.int32).max
+    for n in [0, 1, 2047, 2048, 8191, 8192, 8193, 1_000_001, (1 << 24) + 5]:
         x = rng.integers(-100, 100, n).astype(np.int32)
         got = capi.filter_gt_i32(torch.from_numpy(x).to(dev), 10)
         np.testing.assert_array_equal(got.cpu().numpy(), x[x > 10])
+    x = rng.integers(lo32, hi32, 100_003, dtype=np.int64, endpoint=True).astype(np.int32)
+    x[:3] = [lo32, hi32, 0]
+    xd = torch.from_numpy(x).to(dev)
+    for t in [hi32, lo32, -1, 0, hi32 - 1]:
+        got = capi.filter_gt_i32(xd, int(t))
+        np.testing.assert_array_equal(got.cpu().numpy(), x[x > t])
 
 
 def test_groupby_under_sort_autotuner(dev):

@@ -370,59 +370,6 @@ void CrossExpand(const void *src, int width, uint64_t m, bool from_left, uint64_
 
 namespace {
 constexpr int kFgBlock = 256;
-constexpr int kFgItems = 8;
-
-//! Order-preserving stream compaction of in[i] > threshold: per-tile wave ballots + block scan, global
-//! offset by decoupled look-back.
-__global__ void __launch_bounds__(kFgBlock) FilterGtKernel(const int32_t *in, uint64_t n, int32_t threshold,
-                                                          int32_t *out, u64 *state, uint32_t *ticket,
-                                                          u64 *count_out, uint64_t ntiles) {
-	__shared__ uint32_t s_tile;
-	__shared__ u64 s_scan[kFgBlock / kWave + 1];
-	__shared__ u64 s_prefix;
-	const int tid = threadIdx.x;
-	if (tid == 0) {
-		s_tile = FetchAddAgent(ticket, 1u);
-	}
-	__syncthreads();
-	const uint32_t tile = s_tile;
-	const uint64_t base = (uint64_t)tile * kFgBlock * kFgItems;
-	int32_t v[kFgItems];
-	bool keep[kFgItems];
-	u64 count = 0;
-#pragma unroll
-	for (int k = 0; k < kFgItems; k++) {
-		uint64_t row = base + (uint64_t)tid * kFgItems + k;
-		v[k] = row < n ? in[row] : 0;
-		keep[k] = row < n && v[k] > threshold;
-		count += keep[k];
-	}
-	u64 total;
-	u64 excl = BlockExclusiveScan<u64, kFgBlock>(count, s_scan, &total);
-	if (tid == 0) {
-		StoreAgent(state + tile, LbPack(tile == 0 ? kLbInclusive : kLbAggregate, total));
-	}
-	if (tid < kWave) {
-		u64 prefix = tile == 0 ? 0ull : LookbackExclusiveWave(state, tile);
-		if (tid == 0) {
-			if (tile > 0) {
-				StoreAgent(state + tile, LbPack(kLbInclusive, prefix + total));
-			}
-			s_prefix = prefix;
-			if (tile == ntiles - 1) {
-				*count_out = prefix + total;
-			}
-		}
-	}
-	__syncthreads();
-	u64 pos = s_prefix + excl;
-#pragma unroll
-	for (int k = 0; k < kFgItems; k++) {
-		if (keep[k]) {
-			out[pos++] = v[k];
-		}
-	}
-}
 } // namespace
 
 namespace {
@@ -437,12 +384,12 @@ constexpr int KeepRangeItems() {
 	return (int)(128 / sizeof(T));
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kFgBlock) KeepRangeKernel(const T *in, uint64_t n, long long lo, long long hi,
-                                                           T *out, u64 *state, uint32_t *ticket, u64 *count_out,
-                                                           uint64_t ntiles) {
+template <typename T, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) KeepRangeKernel(const T *in, uint64_t n, long long lo, long long hi,
+                                                        T *out, u64 *state, uint32_t *ticket, u64 *count_out,
+                                                        uint64_t ntiles) {
 	constexpr int kItems = KeepRangeItems<T>();
-	constexpr int kWaves = kFgBlock / kWave;
+	constexpr int kWaves = BLOCK / kWave;
 	__shared__ uint32_t s_tile;
 	__shared__ uint32_t s_off[kItems * kWaves];
 	__shared__ u64 s_scan[kWaves + 1];
@@ -455,17 +402,17 @@ __global__ void __launch_bounds__(kFgBlock) KeepRangeKernel(const T *in, uint64_
 	}
 	__syncthreads();
 	const uint32_t tile = s_tile;
-	const uint64_t base = (uint64_t)tile * kFgBlock * kItems;
+	const uint64_t base = (uint64_t)tile * BLOCK * kItems;
 	T v[kItems];
 	u64 ball[kItems];
 #pragma unroll
 	for (int k = 0; k < kItems; k++) {
-		const uint64_t row = base + (uint64_t)k * kFgBlock + tid;
+		const uint64_t row = base + (uint64_t)k * BLOCK + tid;
 		v[k] = row < n ? in[row] : (T)0;
 	}
 #pragma unroll
 	for (int k = 0; k < kItems; k++) {
-		const uint64_t row = base + (uint64_t)k * kFgBlock + tid;
+		const uint64_t row = base + (uint64_t)k * BLOCK + tid;
 		const long long x = (long long)v[k];
 		ball[k] = __ballot(row < n && x >= lo && x <= hi);
 		if (lane == 0) {
@@ -473,12 +420,12 @@ __global__ void __launch_bounds__(kFgBlock) KeepRangeKernel(const T *in, uint64_
 		}
 	}
 	__syncthreads();
-	static_assert(kItems * kWaves <= kFgBlock, "one (item, wave) count per thread");
+	static_assert(kItems * kWaves <= BLOCK, "one (item, wave) count per thread");
 	{
 		// exclusive scan of the (item, wave) counts, one per thread
 		const u64 mine = tid < kItems * kWaves ? s_off[tid] : 0ull;
 		u64 run = 0;
-		const u64 ex = BlockExclusiveScan<u64, kFgBlock>(mine, s_scan, &run);
+		const u64 ex = BlockExclusiveScan<u64, BLOCK>(mine, s_scan, &run);
 		if (tid < kItems * kWaves) {
 			s_off[tid] = (uint32_t)ex;
 		}
@@ -515,19 +462,43 @@ __global__ void __launch_bounds__(kFgBlock) KeepRangeKernel(const T *in, uint64_
 	}
 }
 
+//! Launches the range compaction over n rows (count of kept rows to *count, no host sync): 512-thread tiles
+//! (16,384 rows of 4 bytes) from 2^22 rows up, where the per-tile ticket (a same-address atomic, serialized
+//! across the device) would otherwise bound the kernel; 256-thread tiles below, so small inputs still fill
+//! the device. `state` needs KeepRangeTiles(n) + 1 zeroed words (the last one is the tile ticket).
+inline bool KeepRangeWide(uint64_t n) {
+	return n >= (1ull << 22);
+}
+template <typename T>
+uint64_t KeepRangeTiles(uint64_t n) {
+	const uint64_t tile = (uint64_t)(KeepRangeWide(n) ? 512 : kFgBlock) * KeepRangeItems<T>();
+	return (n + tile - 1) / tile;
+}
+template <typename T>
+void LaunchKeepRange(const T *in, uint64_t n, long long lo, long long hi, T *out, u64 *state, u64 *count,
+                     hipStream_t stream) {
+	const uint64_t ntiles = KeepRangeTiles<T>(n);
+	uint32_t *ticket = reinterpret_cast<uint32_t *>(state + ntiles);
+	if (KeepRangeWide(n)) {
+		hipLaunchKernelGGL((KeepRangeKernel<T, 512>), dim3((unsigned)ntiles), dim3(512), 0, stream, in, n, lo, hi,
+		                   out, state, ticket, count, ntiles);
+	} else {
+		hipLaunchKernelGGL((KeepRangeKernel<T, kFgBlock>), dim3((unsigned)ntiles), dim3(kFgBlock), 0, stream, in, n,
+		                   lo, hi, out, state, ticket, count, ntiles);
+	}
+	VGPU_HIP(hipGetLastError());
+}
+
 template <typename T>
 uint64_t KeepRangeT(const void *in, uint64_t n, long long lo, long long hi, void *out, Scratch &scratch,
                     hipStream_t stream) {
-	constexpr uint64_t tile = (uint64_t)kFgBlock * KeepRangeItems<T>();
-	const uint64_t ntiles = (n + tile - 1) / tile;
+	const uint64_t ntiles = KeepRangeTiles<T>(n);
 	auto *state = static_cast<u64 *>(scratch.Alloc((ntiles + 2) * sizeof(u64)));
 	u64 *count = state + ntiles + 1;
 	FillU64(state, 0ull, ntiles + 2, stream);
 	{
 		ProfileScope prof("keep_key_range", stream, (double)n * sizeof(T));
-		hipLaunchKernelGGL((KeepRangeKernel<T>), dim3((unsigned)ntiles), dim3(kFgBlock), 0, stream, (const T *)in, n,
-		                   lo, hi, (T *)out, state, reinterpret_cast<uint32_t *>(state + ntiles), count, ntiles);
-		VGPU_HIP(hipGetLastError());
+		LaunchKeepRange<T>((const T *)in, n, lo, hi, (T *)out, state, count, stream);
 	}
 	uint64_t kept = 0;
 	VGPU_HIP(hipMemcpyAsync(&kept, count, 8, hipMemcpyDeviceToHost, stream));
@@ -559,14 +530,15 @@ void FilterGtI32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out,
 		FillU64(count_device, 0ull, 1, stream);
 		return;
 	}
-	const uint64_t tile = (uint64_t)kFgBlock * kFgItems;
-	const uint64_t ntiles = (n + tile - 1) / tile;
+	// in[i] > threshold is the signed range [threshold + 1, INT32_MAX] (empty when threshold is INT32_MAX):
+	// the key-range compaction kernel: wave-striped loads, contiguous per-wave writes, 16,384-row tiles on
+	// large inputs.
+	const uint64_t ntiles = KeepRangeTiles<int32_t>(n);
 	auto *state = static_cast<u64 *>(scratch.Alloc((ntiles + 1) * sizeof(u64)));
 	FillU64(state, 0ull, ntiles + 1, stream);
 	ProfileScope prof("filter_gt_i32", stream, (double)n * 4.0);
-	hipLaunchKernelGGL(FilterGtKernel, dim3((unsigned)ntiles), dim3(kFgBlock), 0, stream, in, n, threshold, out,
-	                   state, reinterpret_cast<uint32_t *>(state + ntiles), (u64 *)count_device, ntiles);
-	VGPU_HIP(hipGetLastError());
+	LaunchKeepRange<int32_t>(in, n, (long long)threshold + 1, (long long)INT32_MAX, out, state, (u64 *)count_device,
+	                         stream);
 }
 
 } // namespace vgpu
