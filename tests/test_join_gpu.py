@@ -83,7 +83,7 @@ def test_join_aggregate_empty_and_disjoint(dev):
 
 
 @pytest.mark.parametrize("seed,na,nb,hi", [(0, 1, 1, 1), (1, 1000, 800, 50), (2, 50_000, 40_000, 10_000),
-                                           (3, 200_000, 3, 4)])
+                                           (3, 200_000, 3, 4), (4, 1_000_000, 800_000, 500_000)])
 def test_join_pairs_match_oracle(dev, seed, na, nb, hi):
     import oracle
     rng = np.random.default_rng(seed)

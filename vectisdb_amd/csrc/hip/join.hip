@@ -101,7 +101,7 @@ __global__ void PairCountKernel(const long long *match, const long long *ca, con
 // ---------------------------------------------------------------- exclusive scan (u64, look-back)
 
 constexpr int kScanBlock = 256;
-constexpr int kScanItems = 8;
+constexpr int kScanItems = 32; // 8,192 values per tile: one ticket (a serialized same-address atomic) per tile
 
 __global__ void __launch_bounds__(kScanBlock) ScanU64Kernel(const u64 *in, u64 *out, uint64_t n, u64 *state,
                                                             uint32_t *ticket, u64 *total_out, uint64_t ntiles) {
