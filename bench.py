@@ -221,6 +221,20 @@ def bench_filter_gt(torch, capi, dev, steps, rows=1 << 28):
             "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def bench_stream_pipeline():
+    """GpuStreamPipeline (H2D, generated projection kernel, D2H; three batches in flight) run by its own C++
+    program (tests/cpp/bench_stream_pipeline, built with the C++ suites) in a child process."""
+    import subprocess
+    exe = ROOT / "tests" / "cpp" / "_bin" / "bench_stream_pipeline"
+    if not exe.exists():
+        return {"operator": "stream_pipeline", "error": f"{exe} is not built"}
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=180)
+    lines = [l for l in res.stdout.splitlines() if l.startswith("{")]
+    if res.returncode != 0 or not lines:
+        return {"operator": "stream_pipeline", "error": (res.stderr or res.stdout)[-300:]}
+    return json.loads(lines[-1])
+
+
 def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
     """Join + aggregate: build 2^24 unique keys (permutation) + INT32 payload, probe 2^28 fk (90% hits);
     SELECT SUM(payload), COUNT(*) FROM probe JOIN build ON fk = key."""
@@ -688,8 +702,8 @@ def main():
 
     operators = []
     extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "filter_large", "filter_gt",
-                                                                       "execute_plan", "cross", "join", "batched",
-                                                                       "served"]
+                                                                       "execute_plan", "stream_pipeline", "cross",
+                                                                       "join", "batched", "served"]
     for name in extra:
         try:
             if name == "filter":
@@ -698,6 +712,8 @@ def main():
                 operators.append(bench_filter_project_large(torch, capi, dev, 10))
             elif name == "filter_gt":
                 operators.append(bench_filter_gt(torch, capi, dev, 10))
+            elif name == "stream_pipeline":
+                operators.append(bench_stream_pipeline())
             elif name == "execute_plan":
                 operators.append(bench_execute_plan(torch, capi, dev, 3))
             elif name == "cross":
