@@ -101,29 +101,6 @@ def committed_traffic(kernel):
     return None if entry is None else entry.get("traffic_bytes_per_launch")
 
 
-def zipf_tables(torch, dev, distinct, s, seed):
-    import numpy as np
-    ranks = np.arange(1, distinct + 1, dtype=np.float64)
-    w = ranks ** (-s)
-    cdf = np.cumsum(w)
-    cdf /= cdf[-1]
-    cdf[-1] = 1.0 + 1e-12  # every u in [0,1) lands inside the table
-    perm = np.random.RandomState(seed).permutation(distinct).astype(np.int64)
-    return torch.from_numpy(cdf).to(dev), torch.from_numpy(perm).to(dev)
-
-
-def make_groupby_table(torch, capi, dev, rows, seed=1):
-    cdf, perm = zipf_tables(torch, dev, 1_000_000, 1.1, seed)
-    key = torch.empty(rows, dtype=torch.int64, device=dev)
-    v1 = torch.empty(rows, dtype=torch.int64, device=dev)
-    v2 = torch.empty(rows, dtype=torch.float64, device=dev)
-    capi.generate_column(key, capi.INT64, 3, seed=seed, stream_id=1, cdf=cdf, perm=perm)
-    capi.generate_column(v1, capi.INT64, 0, lo=-1_000_000, hi=1_000_000, seed=seed, stream_id=2)
-    capi.generate_column(v2, capi.FLOAT64, 2, seed=seed, stream_id=3)
-    torch.cuda.synchronize()
-    return key, v1, v2
-
-
 def kernel_stats(prof, name):
     for e in prof:
         if e["name"] == name:
@@ -133,24 +110,43 @@ def kernel_stats(prof, name):
     return None, None
 
 
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+
+def cpu_line(rows, seconds, threads, sample, unit="rows/s"):
+    """CPU baseline of an operator line: the oracle's multithreaded C restatement timed on this host."""
+    return {"value": round(rows / seconds, 1), "unit": unit, "cores": threads, "kind": "port",
+            "sample": f"CPU restatement (not DuckDB, which is not installed): {sample}; {seconds:.3f} s"}
+
+
+class CheckFailed(RuntimeError):
+    pass
+
+
+def expect(cond, what):
+    if not cond:
+        raise CheckFailed(f"result check failed: {what}")
+
+
 def bench_filter_project(torch, capi, dev, steps, copies=16, rows=1_048_576):
-    """Fused scan-filter-project over 1,048,576 rows (SELECT a*2+b, c*1.5+a WHERE a < 500000 AND c > 0.25); `copies` table copies are rotated so the
-    27 MB working set is not served from the 256 MB Infinity Cache."""
-    sys.path.insert(0, str(ROOT / "tests"))
-    from exprs import cfg0_exprs
-    preds, projs = cfg0_exprs()
-    tables = []
-    for c in range(copies):
-        a = torch.empty(rows, dtype=torch.int32, device=dev)
-        b = torch.empty(rows, dtype=torch.int64, device=dev)
-        x = torch.empty(rows, dtype=torch.float64, device=dev)
-        capi.generate_column(a, capi.INT32, 0, lo=0, hi=999_999, seed=c, stream_id=1)
-        capi.generate_column(b, capi.INT64, 0, lo=-(1 << 40), hi=(1 << 40) - 1, seed=c, stream_id=2)
-        capi.generate_column(x, capi.FLOAT64, 1, seed=c, stream_id=3)
-        tables.append((a, b, x))
+    """Fused scan-filter-project over 1,048,576 rows (SELECT a*2+b, c*1.5+a WHERE a < 500000 AND c > 0.25);
+    `copies` table copies are rotated so the 27 MB working set is not served from the 256 MB Infinity Cache.
+    Every copy's output is checked bit for bit against the oracle's."""
+    import numpy as np
+    import oracle
+    from vectisdb_amd import workloads
+    preds, projs = workloads.cfg0_exprs()
+    tables = [workloads.make_filter_table(torch, dev, rows, seed=c) for c in range(copies)]
     torch.cuda.synchronize()
-    for t in tables[:2]:
-        capi.filter_project(list(t), preds, projs)
+    hosts = [tuple(t.cpu().numpy() for t in tab) for tab in tables]
+    threads = cpu_threads()
+    for tab, h in zip(tables, hosts):
+        outs, _, _, kept = capi.filter_project(list(tab), preds, projs)
+        e1, e2 = oracle.filter_project_cfg0(*h)
+        expect(kept == len(e1), "config-0 kept row count")
+        expect(np.array_equal(outs[0][:kept].cpu().numpy(), e1), "config-0 a*2+b")
+        expect(outs[1][:kept].cpu().numpy().tobytes() == e2.tobytes(), "config-0 c*1.5+a (bit-exact)")
     capi.profile_reset()
     capi.profile_enable(True)
     kept = 0
@@ -164,29 +160,34 @@ def bench_filter_project(torch, capi, dev, steps, copies=16, rows=1_048_576):
     # algorithmic bytes: read a, b, c (20 B/row) + write kept rows' two 8-byte outputs
     alg = rows * 20 + kept * 16
     gbs = alg / (avg * 1e-3) / 1e9
+    reps = 20
+    sec = sum(oracle.filter_project_cfg0_mt(*hosts[i % copies], threads)[2] for i in range(reps))
     return {"operator": "fused filter/project, 1,048,576 rows", "rows": rows, "kept": kept,
             "kernel_ms": round(avg, 5), "rows_per_s": round(rows / (avg * 1e-3), 1),
             "call_ms": round((t1 - t0) * 1e3 / steps, 4), "algorithmic_bytes": alg, "gbs": round(gbs, 1),
-            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "checked": True,
+            "cpu_baseline": cpu_line(rows * reps, sec, threads,
+                                     f"vo_filter_project_cfg0_mt, the same query over {reps} x {rows} rows")}
 
 
 def bench_filter_project_large(torch, capi, dev, steps, rows=1 << 28):
     """The config-0 query (same columns, distributions and expressions) over 2^28 rows: the same generated
-    kernel when it is not launch-bound, i.e. its streaming efficiency."""
-    sys.path.insert(0, str(ROOT / "tests"))
-    from exprs import cfg0_exprs
-    preds, projs = cfg0_exprs()
-    a = torch.empty(rows, dtype=torch.int32, device=dev)
-    b = torch.empty(rows, dtype=torch.int64, device=dev)
-    x = torch.empty(rows, dtype=torch.float64, device=dev)
-    capi.generate_column(a, capi.INT32, 0, lo=0, hi=999_999, seed=5, stream_id=1)
-    capi.generate_column(b, capi.INT64, 0, lo=-(1 << 40), hi=(1 << 40) - 1, seed=5, stream_id=2)
-    capi.generate_column(x, capi.FLOAT64, 1, seed=5, stream_id=3)
+    kernel when it is not launch-bound, i.e. its streaming efficiency. Output checked bit for bit."""
+    import numpy as np
+    import oracle
+    from vectisdb_amd import workloads
+    preds, projs = workloads.cfg0_exprs()
+    a, b, x = workloads.make_filter_table(torch, dev, rows, seed=5)
     torch.cuda.synchronize()
-    capi.filter_project([a, b, x], preds, projs)
+    outs, _, _, kept = capi.filter_project([a, b, x], preds, projs)
+    threads = cpu_threads()
+    e1, e2, sec = oracle.filter_project_cfg0_mt(a.cpu().numpy(), b.cpu().numpy(), x.cpu().numpy(), threads)
+    expect(kept == len(e1), "2^28-row config-0 kept count")
+    expect(np.array_equal(outs[0][:kept].cpu().numpy(), e1), "2^28-row config-0 a*2+b")
+    expect(outs[1][:kept].cpu().numpy().tobytes() == e2.tobytes(), "2^28-row config-0 c*1.5+a")
+    del outs, e1, e2
     capi.profile_reset()
     capi.profile_enable(True)
-    kept = 0
     for _ in range(steps):
         _, _, _, kept = capi.filter_project([a, b, x], preds, projs)
     torch.cuda.synchronize()
@@ -196,17 +197,28 @@ def bench_filter_project_large(torch, capi, dev, steps, rows=1 << 28):
     gbs = alg / (avg * 1e-3) / 1e9
     return {"operator": "fused filter/project, config-0 query over 2^28 rows (streaming efficiency)",
             "rows": rows, "kept": kept, "kernel_ms": round(avg, 4), "rows_per_s": round(rows / (avg * 1e-3), 1),
-            "algorithmic_bytes": alg, "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+            "algorithmic_bytes": alg, "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "checked": True,
+            "cpu_baseline": cpu_line(rows, sec, threads, f"vo_filter_project_cfg0_mt over the same {rows} rows")}
 
 
 def bench_filter_gt(torch, capi, dev, steps, rows=1 << 28):
     """FilterGreaterThanInt32 (vgpu_filter_gt_i32) over 2^28 INT32 rows uniform in [0, 2^30) with the threshold
-    at the middle: about half the rows kept, in order, by one look-back compaction kernel."""
+    at the middle: about half the rows kept, in order, by one look-back compaction kernel. The kept rows are
+    checked against numpy's boolean selection."""
+    import numpy as np
     x = torch.empty(rows, dtype=torch.int32, device=dev)
     capi.generate_column(x, capi.INT32, 0, lo=0, hi=(1 << 30) - 1, seed=13, stream_id=1)
     torch.cuda.synchronize()
     threshold = 1 << 29
-    kept = capi.filter_gt_i32(x, threshold).numel()
+    got = capi.filter_gt_i32(x, threshold)
+    hx = x.cpu().numpy()
+    t0 = time.perf_counter()
+    ref = hx[hx > threshold]
+    sec = time.perf_counter() - t0
+    expect(np.array_equal(got.cpu().numpy(), ref), "FilterGreaterThanInt32 kept rows")
+    kept = got.numel()
+    del got, ref
     capi.profile_reset()
     capi.profile_enable(True)
     for _ in range(steps):
@@ -218,44 +230,56 @@ def bench_filter_gt(torch, capi, dev, steps, rows=1 << 28):
     gbs = alg / (avg * 1e-3) / 1e9
     return {"operator": "FilterGreaterThanInt32, 2^28 INT32 rows, half kept", "rows": rows, "kept": kept,
             "kernel_ms": round(avg, 4), "rows_per_s": round(rows / (avg * 1e-3), 1), "algorithmic_bytes": alg,
-            "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+            "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "checked": True,
+            "cpu_baseline": {"value": round(rows / sec, 1), "unit": "rows/s", "cores": 1, "kind": "port",
+                             "sample": f"numpy boolean selection x[x > t] over the same {rows} rows, one thread "
+                                       f"(no reference config; not DuckDB); {sec:.3f} s"}}
 
 
 def bench_stream_pipeline():
     """GpuStreamPipeline (H2D, generated projection kernel, D2H; three batches in flight) run by its own C++
-    program (tests/cpp/bench_stream_pipeline, built with the C++ suites) in a child process."""
+    program (built with the library into vectisdb_amd/lib) in a child process; the program checks every
+    output row itself."""
     import subprocess
-    exe = ROOT / "tests" / "cpp" / "_bin" / "bench_stream_pipeline"
+    exe = ROOT / "vectisdb_amd" / "lib" / "bench_stream_pipeline"
     if not exe.exists():
         return {"operator": "stream_pipeline", "error": f"{exe} is not built"}
     res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=180)
     lines = [l for l in res.stdout.splitlines() if l.startswith("{")]
     if res.returncode != 0 or not lines:
         return {"operator": "stream_pipeline", "error": (res.stderr or res.stdout)[-300:]}
-    return json.loads(lines[-1])
+    line = json.loads(lines[-1])
+    line["checked"] = bool(line.get("results_checked"))
+    return line
+
+
+def join_closed_form(build, payload, fk, fpay):
+    """SUM(build payload), COUNT(*) and SUM(probe payload) of probe JOIN build ON fk = key for a build side whose
+    keys are a permutation of [0, nb): per-key probe counts by bincount, payloads looked up by key."""
+    import numpy as np
+    hb, hp, hf = build.cpu().numpy(), payload.cpu().numpy().astype(np.int64), fk.cpu().numpy()
+    nb = len(hb)
+    expect(np.array_equal(np.sort(hb), np.arange(nb)), "join build keys form a permutation")
+    hit = hf < nb
+    per_key = np.bincount(hf[hit], minlength=nb).astype(np.int64)
+    pay_by_key = np.empty(nb, np.int64)
+    pay_by_key[hb] = hp
+    return int(np.dot(per_key, pay_by_key)), int(hit.sum()), int(fpay.cpu().numpy()[hit].astype(np.int64).sum())
 
 
 def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
     """Join + aggregate: build 2^24 unique keys (permutation) + INT32 payload, probe 2^28 fk (90% hits);
-    SELECT SUM(payload), COUNT(*) FROM probe JOIN build ON fk = key."""
-    import numpy as np
-    build = torch.from_numpy(np.random.RandomState(21).permutation(nb).astype(np.int64)).to(dev)
-    payload = torch.empty(nb, dtype=torch.int32, device=dev)
-    capi.generate_column(payload, capi.INT32, 0, lo=0, hi=999, seed=21, stream_id=1)
-    hit = torch.empty(nprobe, dtype=torch.int64, device=dev)
-    miss = torch.empty(nprobe, dtype=torch.int64, device=dev)
-    sel = torch.empty(nprobe, dtype=torch.int64, device=dev)
-    capi.generate_column(hit, capi.INT64, 0, lo=0, hi=nb - 1, seed=21, stream_id=2)
-    capi.generate_column(miss, capi.INT64, 0, lo=nb, hi=2 * nb - 1, seed=21, stream_id=3)
-    capi.generate_column(sel, capi.INT64, 0, lo=0, hi=9, seed=21, stream_id=4)
-    body = ("((long long*)out)[idx] = (((const long long*)inputs[2])[row] == 0LL) ? "
-            "((const long long*)inputs[1])[row] : ((const long long*)inputs[0])[row];")
-    outs, _, _, _ = capi.filter_project([hit, miss, sel], [], [(body, [0, 1, 2], capi.INT64)])
-    fk = outs[0]
-    del hit, miss, sel
+    SELECT SUM(payload), COUNT(*) FROM probe JOIN build ON fk = key. Checked against the closed form."""
+    from vectisdb_amd import workloads
+    import oracle
+    build, payload, fk = workloads.make_join_tables(torch, dev, nb, nprobe)
+    fpay = torch.empty(nprobe, dtype=torch.int32, device=dev)
+    capi.generate_column(fpay, capi.INT32, 0, lo=-1000, hi=1000, seed=21, stream_id=5)
     torch.cuda.synchronize()
+    total, count, probe_sum = join_closed_form(build, payload, fk, fpay)
     aggs = [(capi.SUM, 1, 0, capi.HUGEINT), (capi.COUNT_STAR, -1, -1, capi.INT64)]
     res = capi.join_aggregate(fk, [], build, [payload], aggs)
+    expect(res == [total, count], f"join SUM/COUNT {res} vs closed form {[total, count]}")
     capi.profile_reset()
     capi.profile_enable(True)
     torch.cuda.synchronize()
@@ -265,6 +289,7 @@ def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     capi.profile_enable(False)
+    expect(res == [total, count], "join SUM/COUNT (timed calls)")
     prof = capi.profile_read()
     ms = (t1 - t0) * 1e3 / steps
     kernels = {e["name"]: round(e["ms"] / max(e["launches"], 1) * (e["launches"] / steps), 4) for e in prof}
@@ -272,8 +297,6 @@ def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
     compulsory = 12 * nb + 8 * nprobe  # read build key + payload and probe key once
     # The fact-table shape: SUM of an INT32 column of the probe side as well (it travels through the
     # partition with its key).
-    fpay = torch.empty(nprobe, dtype=torch.int32, device=dev)
-    capi.generate_column(fpay, capi.INT32, 0, lo=-1000, hi=1000, seed=21, stream_id=5)
     aggs2 = [(capi.SUM, 0, 0, capi.HUGEINT), (capi.SUM, 1, 0, capi.HUGEINT), (capi.COUNT_STAR, -1, -1, capi.INT64)]
     res2 = capi.join_aggregate(fk, [fpay], build, [payload], aggs2)
     torch.cuda.synchronize()
@@ -282,16 +305,21 @@ def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
         res2 = capi.join_aggregate(fk, [fpay], build, [payload], aggs2)
     torch.cuda.synchronize()
     ms2 = (time.perf_counter() - t0) * 1e3 / steps
-    if int(res2[1]) != int(res[0]) or int(res2[2]) != int(res[1]):
-        raise RuntimeError("join: the two query shapes disagree on SUM(build payload) / COUNT(*)")
+    expect(res2 == [probe_sum, total, count], "join with probe-side SUM vs closed form")
+    threads = cpu_threads()
+    hs, hc, sec = oracle.join_sum_count_mt(build.cpu().numpy(), payload.cpu().numpy(), fk.cpu().numpy(), threads)
+    expect((hs, hc) == (total, count), "CPU hash join agrees with the closed form")
     del fpay
-    return {"operator": "join + aggregate (build-side table, key-range partitioned probe), 2^24 build x 2^28 probe", "probe_rows": nprobe, "build_rows": nb,
-            "compulsory_bytes": compulsory,
+    return {"operator": "join + aggregate (build-side table, key-range partitioned probe), 2^24 build x 2^28 probe",
+            "probe_rows": nprobe, "build_rows": nb, "compulsory_bytes": compulsory,
             "compulsory_frac": round(compulsory / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "sum_payload": int(res[0]), "count": int(res[1]), "ms": round(ms, 3),
             "probe_side_sum_ms": round(ms2, 3), "probe_side_sum": int(res2[0]),
             "rows_per_s": round((nprobe + nb) / (ms * 1e-3), 1), "algorithmic_bytes": alg,
-            "gbs": round(alg / (ms * 1e-3) / 1e9, 1), "kernel_ms_per_step": kernels}
+            "gbs": round(alg / (ms * 1e-3) / 1e9, 1), "kernel_ms_per_step": kernels, "checked": True,
+            "cpu_baseline": cpu_line(nprobe + nb, sec, threads,
+                                     "vo_join_sum_count_mt, a hash join (shared open-addressing build table, "
+                                     f"probe split over threads) over the same {nb} x {nprobe} rows")}
 
 
 def bench_execute_plan(torch, capi, dev, reps, n=1 << 26):
@@ -301,8 +329,7 @@ def bench_execute_plan(torch, capi, dev, reps, n=1 << 26):
     and the other operator lines time device-resident data instead."""
     import ctypes as C
     import numpy as np
-    sys.path.insert(0, str(ROOT / "tests"))
-    from exprs import CFG0_PRED_A, CFG0_PRED_C, CFG0_PROJ_1, CFG0_PROJ_2
+    from vectisdb_amd.workloads import CFG0_PRED_A, CFG0_PRED_C, CFG0_PROJ_1, CFG0_PROJ_2
     from vectisdb_amd.engine import GpuExpr, GpuOpType as Op, GpuPlanNode, GpuTableRef, GpuValueType as T
     rng = np.random.default_rng(5)
     a = rng.integers(0, 1_000_000, n, dtype=np.int32)
@@ -351,22 +378,24 @@ def bench_execute_plan(torch, capi, dev, reps, n=1 << 26):
 
 
 def bench_cross(torch, capi, dev, steps):
-    """Cross product + filter: 4,096 x 65,536 INT32 pairs WHERE x + y < 2^30, materialized (x, y)."""
-    sys.path.insert(0, str(ROOT / "tests"))
-    from exprs import col, copy_body, out
-    x = torch.empty(4096, dtype=torch.int32, device=dev)
-    y = torch.empty(65536, dtype=torch.int32, device=dev)
-    capi.generate_column(x, capi.INT32, 0, lo=0, hi=(1 << 30) - 1, seed=11, stream_id=1)
-    capi.generate_column(y, capi.INT32, 0, lo=0, hi=(1 << 30) - 1, seed=11, stream_id=2)
-    pred = (out(f"(({col(0, capi.INT32)} + {col(1, capi.INT32)}) < 1073741824)", capi.BOOLEAN), [0, 1],
-            capi.BOOLEAN)
-    projs = [(copy_body(capi.INT32), [0], capi.INT32), (copy_body(capi.INT32), [1], capi.INT32)]
-    capi.cross_filter([x], [y], [pred], projs)
+    """Cross product + filter: 4,096 x 65,536 INT32 pairs WHERE x + y < 2^30, materialized (x, y) in left-major
+    order; every kept pair checked against the oracle's multithreaded materialization."""
+    import numpy as np
+    import oracle
+    from vectisdb_amd import workloads
+    x, y = workloads.make_cross_tables(torch, dev)
+    preds, projs = workloads.cross_exprs()
+    (ox, oy), kept = capi.cross_filter([x], [y], preds, projs)
+    threads = cpu_threads()
+    ex, ey, sec = oracle.cross_filter_pairs_mt(x.cpu().numpy(), y.cpu().numpy(), workloads.CROSS_LIMIT, threads)
+    expect(kept == len(ex), "cross kept pair count")
+    expect(np.array_equal(ox.cpu().numpy(), ex) and np.array_equal(oy.cpu().numpy(), ey),
+           "cross kept pairs in left-major order")
+    del ox, oy, ex, ey
     capi.profile_reset()
     capi.profile_enable(True)
-    kept = 0
     for i in range(steps):
-        _, kept = capi.cross_filter([x], [y], [pred], projs)
+        _, kept = capi.cross_filter([x], [y], preds, projs)
     torch.cuda.synchronize()
     capi.profile_enable(False)
     avg, _ = kernel_stats(capi.profile_read(), "fused_cross_filter")
@@ -375,63 +404,46 @@ def bench_cross(torch, capi, dev, steps):
     gbs = alg / (avg * 1e-3) / 1e9
     return {"operator": "cross product + filter, 4,096 x 65,536", "pairs": pairs, "kept": kept,
             "kernel_ms": round(avg, 4), "rows_per_s": round(pairs / (avg * 1e-3), 1),
-            "algorithmic_bytes": alg, "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
-
-
-def make_batched_table(torch, capi, dev, nq=512, seed=7):
-    """512 row groups of one table, sizes log-uniform in [2K, 4M] rows; a INT32 uniform [0,1e6), c DOUBLE
-    uniform [0,1), v1 INT64 uniform [-1e6,1e6], v2 DOUBLE normal(0,1) as in the plain configs; the group
-    count of query g is min(rows, log-uniform in [16, 1M]) and its keys are Zipf(1.1) ranks over that count
-    mapped through a seeded permutation of [0, count) (inverse CDF on the device)."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    sizes = np.exp(rng.uniform(np.log(2048), np.log(4 << 20), nq)).astype(np.int64)
-    cards = np.minimum(sizes, np.exp(rng.uniform(np.log(16), np.log(1_000_000), nq)).astype(np.int64))
-    rows = int(sizes.sum())
-    a = torch.empty(rows, dtype=torch.int32, device=dev)
-    c = torch.empty(rows, dtype=torch.float64, device=dev)
-    capi.generate_column(a, capi.INT32, 0, lo=0, hi=999_999, seed=seed, stream_id=1)
-    capi.generate_column(c, capi.FLOAT64, 1, seed=seed, stream_id=2)
-    _, v1, v2 = make_groupby_table(torch, capi, dev, rows, seed=seed)
-    key = torch.empty(rows, dtype=torch.int64, device=dev)
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(seed)
-    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-    for q in range(nq):
-        g = int(cards[q])
-        w = torch.arange(1, g + 1, dtype=torch.float64, device=dev).pow_(-1.1)
-        cdf = torch.cumsum(w, 0)
-        cdf /= cdf[-1].clone()
-        u = torch.rand(int(sizes[q]), dtype=torch.float64, device=dev, generator=gen)
-        rank = torch.searchsorted(cdf, u).clamp_(max=g - 1)
-        perm = torch.randperm(g, device=dev, generator=gen)
-        key[offsets[q]:offsets[q + 1]] = perm[rank]
-    offsets_t = torch.from_numpy(offsets).to(dev)
-    torch.cuda.synchronize()
-    return [a, c, key, v1, v2], offsets_t, sizes, cards
+            "algorithmic_bytes": alg, "gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "checked": True,
+            "cpu_baseline": cpu_line(pairs, sec, threads, "vo_cross_filter_pairs_mt, the same 4,096 x 65,536 pairs "
+                                                          "filtered and materialized in left-major order",
+                                     unit="pairs/s")}
 
 
 def bench_batched(torch, capi, dev, steps, nq=512):
     """Batched form against the per-query path: the same 512 filter + GROUP BY queries (WHERE a < 500000 AND
     c > 0.25, GROUP BY key with COUNT/SUM/MIN/MAX/SUM/AVG) run as one batched pipeline, and one query after
     another. Every query arrives at t=0, so each query's latency in the batch is the batch time, and in the
-    sequential loop its completion time."""
+    sequential loop its completion time. Every group of every query is checked against the oracle."""
     import numpy as np
-    cols, offsets, sizes, cards = make_batched_table(torch, capi, dev, nq)
+    import oracle
+    from vectisdb_amd import workloads
+    cols, offsets, sizes, cards = workloads.make_batched_table(torch, dev, nq)
     rows = int(sizes.sum())
-    preds = [("((bool*)out)[idx] = (((const int*)inputs[0])[row]) < 500000;", [0], capi.BOOLEAN),
-             ("((bool*)out)[idx] = (((const double*)inputs[0])[row]) > 0.25;", [1], capi.BOOLEAN)]
-    aggs = [(capi.COUNT_STAR, -1, capi.INT64), (capi.SUM, 0, capi.HUGEINT), (capi.MIN, 0, capi.INT64),
-            (capi.MAX, 0, capi.INT64), (capi.SUM, 1, capi.FLOAT64), (capi.AVG, 1, capi.FLOAT64)]
+    preds, aggs = workloads.BATCHED_PREDS, workloads.AGGS7
     run = lambda pq: capi.batched_filter_group_by(cols, offsets, preds, 2, [3, 4], aggs, per_query=pq)
     b = run(False)
     p = run(True)
-    same = b[3] == p[3] and bool(torch.equal(b[0], p[0])) and bool(torch.equal(b[1], p[1]))
+    expect(b[3] == p[3] and bool(torch.equal(b[0], p[0])) and bool(torch.equal(b[1], p[1])),
+           "batched and per-query group layouts")
     for i, a in enumerate(aggs):
         if a[2] == capi.FLOAT64:
-            same = same and bool(torch.allclose(b[2][i], p[2][i], rtol=1e-12, atol=0))
+            expect(bool(torch.allclose(b[2][i], p[2][i], rtol=1e-12, atol=0)), f"batched aggregate {i}")
         else:
-            same = same and bool(torch.equal(b[2][i], p[2][i]))
+            expect(bool(torch.equal(b[2][i], p[2][i])), f"batched aggregate {i}")
+    threads = cpu_threads()
+    host = [t.cpu().numpy() for t in cols]
+    ref, goffs, sec = oracle.batched_filter_group_by_mt(*host, offsets.cpu().numpy(), threads)
+    del host
+    g = len(ref["key"])
+    expect(list(goffs) == list(b[3]), "per-query group offsets vs the oracle")
+    expect(np.array_equal(b[1].cpu().numpy(), ref["key"]), "batched group keys vs the oracle")
+    for i, name in ((0, "count"), (2, "min_v1"), (3, "max_v1")):
+        expect(np.array_equal(b[2][i].cpu().numpy(), ref[name]), f"batched {name} vs the oracle")
+    expect(np.array_equal(b[2][1][: 2 * g].cpu().numpy().reshape(-1, 2), ref["sum_v1"]), "batched SUM(v1)")
+    for i, name in ((4, "sum_v2"), (5, "avg_v2")):
+        expect(np.allclose(b[2][i].cpu().numpy(), ref[name], rtol=1e-12, atol=0), f"batched {name}")
     del b, p
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -450,69 +462,11 @@ def bench_batched(torch, capi, dev, steps, nq=512):
             "rows": rows, "queries": nq, "batched_ms": round(tb * 1e3, 3), "per_query_ms": round(tp * 1e3, 3),
             "batched_qps": round(nq / tb, 1), "per_query_qps": round(nq / tp, 1),
             "qps_ratio": round(tp / tb, 2), "rows_per_s": round(rows / tb, 1),
-            "results_identical": same}
-
-
-def make_served_tables(torch, capi, dev, scale=1.0, seed=31):
-    """Device-resident tables of the served workload (~40 GB at scale 1): the filter table (a INT32 uniform
-    [0,1e6), b INT64 uniform [-2^40,2^40), c DOUBLE uniform [0,1)), the GROUP BY table (key INT64 Zipf(1.1)
-    over 1M values, v1 INT64 uniform [-1e6,1e6], v2 DOUBLE normal(0,1)), the probe table (fk INT64, 90%
-    uniform over the build keys, 10% uniform over [2^24, 2^25)) and the 2^24-row build side (unique keys, a
-    permutation, payload INT32 uniform [0,1000))."""
-    import numpy as np
-    nf = int(600_000_000 * scale)
-    ng = int(600_000_000 * scale)
-    nprobe = int(1_600_000_000 * scale)
-    nb = 1 << 24
-    a = torch.empty(nf, dtype=torch.int32, device=dev)
-    b = torch.empty(nf, dtype=torch.int64, device=dev)
-    c = torch.empty(nf, dtype=torch.float64, device=dev)
-    capi.generate_column(a, capi.INT32, 0, lo=0, hi=999_999, seed=seed, stream_id=1)
-    capi.generate_column(b, capi.INT64, 0, lo=-(1 << 40), hi=(1 << 40) - 1, seed=seed, stream_id=2)
-    capi.generate_column(c, capi.FLOAT64, 1, seed=seed, stream_id=3)
-    key, v1, v2 = make_groupby_table(torch, capi, dev, ng, seed=seed)
-    build = torch.from_numpy(np.random.RandomState(seed).permutation(nb).astype(np.int64)).to(dev)
-    payload = torch.empty(nb, dtype=torch.int32, device=dev)
-    capi.generate_column(payload, capi.INT32, 0, lo=0, hi=999, seed=seed, stream_id=4)
-    fk = torch.empty(nprobe, dtype=torch.int64, device=dev)
-    chunk = 1 << 27
-    body = ("((long long*)out)[idx] = (((const long long*)inputs[2])[row] == 0LL) ? "
-            "((const long long*)inputs[1])[row] : ((const long long*)inputs[0])[row];")
-    hit = torch.empty(chunk, dtype=torch.int64, device=dev)
-    miss = torch.empty(chunk, dtype=torch.int64, device=dev)
-    sel = torch.empty(chunk, dtype=torch.int64, device=dev)
-    for i, lo in enumerate(range(0, nprobe, chunk)):
-        m = min(chunk, nprobe - lo)
-        capi.generate_column(hit[:m], capi.INT64, 0, lo=0, hi=nb - 1, seed=seed + i, stream_id=5)
-        capi.generate_column(miss[:m], capi.INT64, 0, lo=nb, hi=2 * nb - 1, seed=seed + i, stream_id=6)
-        capi.generate_column(sel[:m], capi.INT64, 0, lo=0, hi=9, seed=seed + i, stream_id=7)
-        outs, _, _, _ = capi.filter_project([hit[:m], miss[:m], sel[:m]], [], [(body, [0, 1, 2], capi.INT64)])
-        fk[lo:lo + m].copy_(outs[0][:m])
-        del outs
-    del hit, miss, sel
-    x = torch.empty(4096, dtype=torch.int32, device=dev)
-    y = torch.empty(65536, dtype=torch.int32, device=dev)
-    capi.generate_column(x, capi.INT32, 0, lo=0, hi=(1 << 30) - 1, seed=seed, stream_id=8)
-    capi.generate_column(y, capi.INT32, 0, lo=0, hi=(1 << 30) - 1, seed=seed, stream_id=9)
-    torch.cuda.synchronize()
-    tables = {"filter": [a, b, c], "group": [key, v1, v2], "probe": fk, "build": build, "payload": payload,
-              "left": [x], "right": [y]}
-    total = sum(t.numel() * t.element_size() for t in [a, b, c, key, v1, v2, fk, build, payload, x, y])
-    return tables, total
-
-
-def served_queries(n, rows, seed, lo=10_000, hi=50_000_000, left_rows=4096):
-    """n queries drawn uniformly from the four templates (filter/project, GROUP BY, join + aggregate, cross
-    product + filter), input sizes log-uniform in [lo, hi] rows, each over a uniformly placed row range of its
-    template's table. A cross query's input size counts pairs: it takes size / 4,096 rows of the right side
-    against the whole 4,096-row left side."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    kinds = rng.integers(0, 4, n)
-    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n)).astype(np.uint64)
-    sizes = np.where(kinds == 3, np.maximum(sizes // left_rows, 1), sizes).astype(np.uint64)
-    starts = np.array([rng.integers(0, rows[k] - int(s) + 1) for k, s in zip(kinds, sizes)], dtype=np.uint64)
-    return kinds, starts, sizes
+            "results_identical": True, "checked": True,
+            "cpu_baseline": dict(cpu_line(nq, sec, threads, f"vo_batched_filter_group_by_mt: the {nq} queries one "
+                                                            f"after another (filter, then the multithreaded "
+                                                            f"GROUP BY) over the same {rows} rows", unit="queries/s"),
+                                 rows_per_s=round(rows / sec, 1))}
 
 
 def bench_served(torch, capi, dev, nq=400, scale=1.0):
@@ -526,9 +480,8 @@ def bench_served(torch, capi, dev, nq=400, scale=1.0):
     served form is run open-loop at rising rates, and the highest rate whose p99 stays at or below the
     plain path's p99 is reported beside it."""
     import numpy as np
-    sys.path.insert(0, str(ROOT / "tests"))
-    from exprs import cfg0_exprs
-    tables, total_bytes = make_served_tables(torch, capi, dev, scale)
+    from vectisdb_amd.workloads import cfg0_exprs, make_served_tables, served_queries
+    tables, total_bytes = make_served_tables(torch, dev, scale)
     rows = [tables["filter"][0].numel(), tables["group"][0].numel(), tables["probe"].numel(),
             tables["right"][0].numel()]
     cross_pred = ("((bool*)out)[idx] = (((const int*)inputs[0])[row] + ((const int*)inputs[1])[row]) < "
@@ -643,9 +596,9 @@ def main():
     print(f"[bench] device copy ceiling {copy_gbs:.1f} GB/s (torch copy_ {torch_copy_gbs:.1f} GB/s)",
           file=sys.stderr, flush=True)
     print(f"[bench] generating {rows} rows on device", file=sys.stderr, flush=True)
-    key, v1, v2 = make_groupby_table(torch, capi, dev, rows)
-    aggs = [(capi.COUNT_STAR, -1, capi.INT64), (capi.SUM, 0, capi.HUGEINT), (capi.MIN, 0, capi.INT64),
-            (capi.MAX, 0, capi.INT64), (capi.SUM, 1, capi.FLOAT64), (capi.AVG, 1, capi.FLOAT64)]
+    from vectisdb_amd import workloads
+    key, v1, v2 = workloads.make_groupby_table(torch, dev, rows)
+    aggs = workloads.AGGS7
     cap = capi.group_by_bound([capi.column(key)], rows)
     groups = 0
     # The sort autotuner picks the radix-pass variant per shape on the first (warm-up) sort of that shape.
@@ -661,7 +614,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        _, _, groups = capi.group_by([key], [v1, v2], aggs, rows, capacity=cap)
+        gkeys, gaggs, groups = capi.group_by([key], [v1, v2], aggs, rows, capacity=cap)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     capi.profile_enable(False)
@@ -700,6 +653,37 @@ def main():
                       file=sys.stderr, flush=True)
         os.environ.pop("VGPU_RADIX_BITS", None)
 
+    # Every group of the last timed step against the oracle (the multithreaded C restatement, pinned bit for bit
+    # against the single-threaded one): keys, COUNT, HUGEINT SUM, MIN, MAX exact; DOUBLE SUM/AVG within 1e-12.
+    import oracle
+    threads = cpu_threads()
+    hk, h1, h2 = key.cpu().numpy(), v1.cpu().numpy(), v2.cpu().numpy()
+    ref = oracle.group_by_i64(hk, h1, h2, threads=threads, raw=True)
+    g = groups
+    expect(g == len(ref["key"]), f"GROUP BY group count {g} vs {len(ref['key'])}")
+    expect(np.array_equal(gkeys[0].cpu().numpy(), ref["key"]), "GROUP BY keys")
+    expect(np.array_equal(gaggs[0].cpu().numpy(), ref["count"]), "COUNT(*)")
+    expect(np.array_equal(gaggs[1][: 2 * g].cpu().numpy().reshape(-1, 2), ref["sum_v1"]), "SUM(v1) (HUGEINT)")
+    expect(np.array_equal(gaggs[2].cpu().numpy(), ref["min_v1"]), "MIN(v1)")
+    expect(np.array_equal(gaggs[3].cpu().numpy(), ref["max_v1"]), "MAX(v1)")
+    expect(np.allclose(gaggs[4].cpu().numpy(), ref["sum_v2"], rtol=1e-12, atol=0), "SUM(v2) within 1e-12")
+    expect(np.allclose(gaggs[5].cpu().numpy(), ref["avg_v2"], rtol=1e-12, atol=0), "AVG(v2) within 1e-12")
+    del ref, gkeys, gaggs
+    print(f"[bench] GROUP BY checked: {g} groups equal the oracle's", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        n = min(args.cpu_rows, rows)
+        oracle.time_group_by_i64(hk[:100_000], h1[:100_000], h2[:100_000], threads)
+        sec, cpu_groups = oracle.time_group_by_i64(hk[:n], h1[:n], h2[:n], threads)
+        cpu = {"value": round(n / sec, 1), "unit": "rows/s", "cores": threads, "kind": "port",
+               "sample": f"{'all' if n == rows else 'first'} {n} rows of the same device-generated table; CPU "
+                         f"restatement (not DuckDB, which is not installed): oracle/oracle.c vo_group_by_i64_mt, "
+                         f"stable counting sort + exact per-group sums on {threads} threads, {cpu_groups} groups, "
+                         f"{sec:.2f} s"}
+        print(f"[bench] cpu baseline {cpu['value'] / 1e6:.1f} M rows/s on {threads} threads", file=sys.stderr,
+              flush=True)
+    del hk, h1, h2
+
     operators = []
     extra = [s for s in args.only.split(",") if s] if args.only else ["filter", "filter_large", "filter_gt",
                                                                        "execute_plan", "stream_pipeline", "cross",
@@ -733,24 +717,6 @@ def main():
         torch.cuda.empty_cache()
         print(f"[bench] operator {operators[-1]}", file=sys.stderr, flush=True)
 
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-        n = min(args.cpu_rows, rows)
-        k = key[:n].cpu().numpy()
-        a = v1[:n].cpu().numpy()
-        b = v2[:n].cpu().numpy()
-        oracle.time_group_by_i64(k[:100_000], a[:100_000], b[:100_000], threads)
-        sec, cpu_groups = oracle.time_group_by_i64(k, a, b, threads)
-        cpu = {"value": round(n / sec, 1), "unit": "rows/s", "cores": threads, "kind": "port",
-               "sample": f"{'all' if n == rows else 'first'} {n} rows of the same device-generated table; CPU "
-                         f"restatement (not DuckDB, which is not installed): oracle/oracle.c vo_group_by_i64_mt, "
-                         f"stable counting sort + exact per-group sums on {threads} threads, {cpu_groups} groups, "
-                         f"{sec:.2f} s"}
-        print(f"[bench] cpu baseline {cpu['value'] / 1e6:.1f} M rows/s on {threads} threads", file=sys.stderr,
-              flush=True)
-
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -775,6 +741,7 @@ def main():
                          frac_of_copy=round(roof["achieved"] / copy_gbs, 4))
         if roof else roof,
         "sort_variants": capi.sort_autotune_report(),
+        "checked": True,
         "cpu_baseline": cpu,
         "operators": operators,
     }

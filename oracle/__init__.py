@@ -69,6 +69,15 @@ def lib():
         L.vo_cross_filter_count.argtypes = [P, u64, P, u64, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.vo_cross_filter_pairs.restype = u64
         L.vo_cross_filter_pairs.argtypes = [P, u64, P, u64, C.c_int64, P, P, u64]
+        L.vo_filter_project_cfg0_mt.restype = u64
+        L.vo_filter_project_cfg0_mt.argtypes = [P, P, P, u64, C.c_int32, C.c_double, P, P, C.c_int]
+        L.vo_join_sum_count_mt.restype = u64
+        L.vo_join_sum_count_mt.argtypes = [P, P, u64, P, u64, C.c_int, C.POINTER(u64), C.POINTER(C.c_int64)]
+        L.vo_cross_filter_pairs_mt.restype = u64
+        L.vo_cross_filter_pairs_mt.argtypes = [P, u64, P, u64, C.c_int64, P, P, u64, C.c_int]
+        L.vo_batched_filter_group_by_mt.restype = u64
+        L.vo_batched_filter_group_by_mt.argtypes = [P, P, P, P, P, P, u64, C.c_int32, C.c_double, C.c_int,
+                                                    P, P, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -114,9 +123,10 @@ def exact_sum(x):
     return lib().vo_exact_sum(_p(x), len(x))
 
 
-def group_by_i64(key, v1=None, v2=None, threads=1):
-    """Returns dict of per-group arrays in ascending key order (sum_v1 as Python ints). threads > 1 runs the
-    multithreaded restatement (same results)."""
+def group_by_i64(key, v1=None, v2=None, threads=1, raw=False):
+    """Returns dict of per-group arrays in ascending key order (sum_v1 as Python ints, or with raw=True as a
+    (groups, 2) int64 array of HUGEINT {lower, upper} words). threads > 1 runs the multithreaded restatement
+    (same results)."""
     key = np.ascontiguousarray(key, np.int64)
     n = len(key)
     v1 = None if v1 is None else np.ascontiguousarray(v1, np.int64)
@@ -135,7 +145,7 @@ def group_by_i64(key, v1=None, v2=None, threads=1):
                                   _p(out["min_v1"]), _p(out["max_v1"]), _p(out["sum_v2"]), _p(out["avg_v2"]))
     res = {k: v[:g] for k, v in out.items() if k != "sum_v1"}
     pairs = out["sum_v1"][: 2 * g].reshape(-1, 2)
-    res["sum_v1"] = [int(lo) % (1 << 64) + (int(hi) << 64) for lo, hi in pairs]
+    res["sum_v1"] = pairs if raw else [int(lo) % (1 << 64) + (int(hi) << 64) for lo, hi in pairs]
     return res
 
 
@@ -196,3 +206,69 @@ def cross_filter_pairs(x, y, c):
     oy = np.empty(max(n, 1), np.int32)
     lib().vo_cross_filter_pairs(_p(x), len(x), _p(y), len(y), c, _p(ox), _p(oy), n)
     return ox[:n], oy[:n]
+
+
+def _timed(fn, *args):
+    import time
+    t0 = time.perf_counter()
+    r = fn(*args)
+    return r, time.perf_counter() - t0
+
+
+def filter_project_cfg0_mt(a, b, c, threads, a_lt=500000, c_gt=0.25):
+    """(out1, out2, seconds of the C call): vo_filter_project_cfg0 on `threads` threads."""
+    a = np.ascontiguousarray(a, np.int32)
+    b = np.ascontiguousarray(b, np.int64)
+    c = np.ascontiguousarray(c, np.float64)
+    o1 = np.empty(max(len(a), 1), np.int64)
+    o2 = np.empty(max(len(a), 1), np.float64)
+    k, sec = _timed(lib().vo_filter_project_cfg0_mt, _p(a), _p(b), _p(c), len(a), a_lt, c_gt, _p(o1), _p(o2),
+                    threads)
+    return o1[:k], o2[:k], sec
+
+
+def join_sum_count_mt(build_key, payload, fk, threads):
+    """(SUM(payload), COUNT(*), seconds of the C call): the hash-join restatement on `threads` threads."""
+    bk = np.ascontiguousarray(build_key, np.int64)
+    pl = np.ascontiguousarray(payload, np.int32)
+    fk = np.ascontiguousarray(fk, np.int64)
+    lo = C.c_uint64(0)
+    hi = C.c_int64(0)
+    cnt, sec = _timed(lib().vo_join_sum_count_mt, _p(bk), _p(pl), len(bk), _p(fk), len(fk), threads, C.byref(lo),
+                      C.byref(hi))
+    return (hi.value << 64) + lo.value, cnt, sec
+
+
+def cross_filter_pairs_mt(x, y, c, threads):
+    """(xs, ys, seconds of the C call): the materialized left-major pairs on `threads` threads."""
+    x = np.ascontiguousarray(x, np.int32)
+    y = np.ascontiguousarray(y, np.int32)
+    n, _, _ = cross_filter_count(x, y, c)
+    ox = np.empty(max(n, 1), np.int32)
+    oy = np.empty(max(n, 1), np.int32)
+    k, sec = _timed(lib().vo_cross_filter_pairs_mt, _p(x), len(x), _p(y), len(y), c, _p(ox), _p(oy), n, threads)
+    assert k == n
+    return ox[:n], oy[:n], sec
+
+
+def batched_filter_group_by_mt(a, c, key, v1, v2, offsets, threads, a_lt=500000, c_gt=0.25):
+    """(dict of per-group arrays as group_by_i64(raw=True), group offsets per query, seconds of the C call)."""
+    a = np.ascontiguousarray(a, np.int32)
+    c = np.ascontiguousarray(c, np.float64)
+    key = np.ascontiguousarray(key, np.int64)
+    v1 = np.ascontiguousarray(v1, np.int64)
+    v2 = np.ascontiguousarray(v2, np.float64)
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    nq = len(offsets) - 1
+    cap = max(len(key), 1)
+    out = {k: np.empty(cap, np.int64) for k in ("key", "count", "min_v1", "max_v1")}
+    out["sum_v1"] = np.empty(2 * cap, np.int64)
+    out["sum_v2"] = np.empty(cap, np.float64)
+    out["avg_v2"] = np.empty(cap, np.float64)
+    goffs = np.empty(nq + 1, np.uint64)
+    g, sec = _timed(lib().vo_batched_filter_group_by_mt, _p(a), _p(c), _p(key), _p(v1), _p(v2), _p(offsets), nq,
+                    a_lt, c_gt, threads, _p(out["key"]), _p(out["count"]), _p(out["sum_v1"]), _p(out["min_v1"]),
+                    _p(out["max_v1"]), _p(out["sum_v2"]), _p(out["avg_v2"]), _p(goffs))
+    res = {k: v[:g] for k, v in out.items() if k != "sum_v1"}
+    res["sum_v1"] = out["sum_v1"][: 2 * g].reshape(-1, 2)
+    return res, goffs.astype(np.int64), sec

@@ -752,3 +752,390 @@ uint64_t vo_cross_filter_pairs(const int32_t *x, uint64_t nl, const int32_t *y, 
 	}
 	return k;
 }
+
+/* ------------------------------------------------------------------ multithreaded CPU baselines
+ * The remaining configurations restated for all host threads, so each benchmark line can report a CPU
+ * figure with a thread count. Each returns exactly what its single-threaded counterpart above returns
+ * (tests/test_oracle.py checks them against each other); only the work is split. */
+
+typedef struct {
+	void (*fn)(void *ctx, int t, int threads);
+	void *ctx;
+	int t, threads;
+} par_arg;
+
+static void *par_trampoline(void *p) {
+	par_arg *a = (par_arg *)p;
+	a->fn(a->ctx, a->t, a->threads);
+	return NULL;
+}
+
+/* Runs fn(ctx, t, threads) for t = 0..threads-1 on threads of their own, and waits for all of them. */
+static void par_for(int threads, void (*fn)(void *, int, int), void *ctx) {
+	pthread_t tid[256];
+	par_arg args[256];
+	if (threads < 1) {
+		threads = 1;
+	}
+	if (threads > 256) {
+		threads = 256;
+	}
+	for (int t = 0; t < threads; t++) {
+		args[t].fn = fn;
+		args[t].ctx = ctx;
+		args[t].t = t;
+		args[t].threads = threads;
+		pthread_create(&tid[t], NULL, par_trampoline, &args[t]);
+	}
+	for (int t = 0; t < threads; t++) {
+		pthread_join(tid[t], NULL);
+	}
+}
+
+static int clamp_threads(int threads) {
+	return threads < 1 ? 1 : (threads > 256 ? 256 : threads);
+}
+
+/* --- config 0 on all threads: each thread counts its row range, a prefix over the counts gives every range its
+ * output offset, and each thread then writes its kept rows in row order. */
+typedef struct {
+	const int32_t *a;
+	const int64_t *b;
+	const double *c;
+	uint64_t n;
+	int32_t a_lt;
+	double c_gt;
+	int64_t *out1;
+	double *out2;
+	uint64_t count[256];
+	uint64_t offset[256];
+	int phase;
+} fp_ctx;
+
+static void fp_task(void *p, int t, int threads) {
+	fp_ctx *x = (fp_ctx *)p;
+	const uint64_t lo = x->n * (uint64_t)t / (uint64_t)threads, hi = x->n * (uint64_t)(t + 1) / (uint64_t)threads;
+	if (x->phase == 0) {
+		uint64_t k = 0;
+		for (uint64_t i = lo; i < hi; i++) {
+			k += (x->a[i] < x->a_lt) & (x->c[i] > x->c_gt);
+		}
+		x->count[t] = k;
+	} else {
+		uint64_t k = x->offset[t];
+		for (uint64_t i = lo; i < hi; i++) {
+			if (x->a[i] < x->a_lt && x->c[i] > x->c_gt) {
+				const int32_t twice = (int32_t)(x->a[i] * 2);
+				x->out1[k] = (int64_t)twice + x->b[i];
+				x->out2[k] = x->c[i] * 1.5 + (double)x->a[i];
+				k++;
+			}
+		}
+	}
+}
+
+uint64_t vo_filter_project_cfg0_mt(const int32_t *a, const int64_t *b, const double *c, uint64_t n, int32_t a_lt,
+                                   double c_gt, int64_t *out1, double *out2, int threads) {
+	fp_ctx *x = (fp_ctx *)calloc(1, sizeof(fp_ctx));
+	threads = clamp_threads(threads);
+	x->a = a;
+	x->b = b;
+	x->c = c;
+	x->n = n;
+	x->a_lt = a_lt;
+	x->c_gt = c_gt;
+	x->out1 = out1;
+	x->out2 = out2;
+	x->phase = 0;
+	par_for(threads, fp_task, x);
+	uint64_t total = 0;
+	for (int t = 0; t < threads; t++) {
+		x->offset[t] = total;
+		total += x->count[t];
+	}
+	x->phase = 1;
+	par_for(threads, fp_task, x);
+	free(x);
+	return total;
+}
+
+/* --- config 2a on all threads: a hash join. The build side goes into an open-addressing table (linear probing,
+ * multiplicative hash, twice the build rows rounded up to a power of two) that all threads fill at once: a
+ * slot is claimed by a compare-and-swap of its state word, and the key's row count and payload sum are
+ * added with atomics. Each thread then probes its range of foreign keys and sums its matches exactly. */
+typedef struct {
+	int64_t key;
+	uint64_t state; /* 0 empty, 1 being claimed, 2 holds key */
+	uint64_t count;
+	int64_t sum;
+} hj_slot;
+
+typedef struct {
+	const int64_t *bk;
+	const int32_t *pay;
+	uint64_t nb;
+	const int64_t *fk;
+	uint64_t np;
+	hj_slot *tab;
+	uint64_t mask;
+	int shift;
+	int phase;
+	uint64_t count[256];
+	i128 sum[256];
+} hj_ctx;
+
+static uint64_t hj_hash(int64_t k, int shift) {
+	return ((uint64_t)k * 0x9E3779B97F4A7C15ull) >> shift;
+}
+
+static void hj_task(void *p, int t, int threads) {
+	hj_ctx *x = (hj_ctx *)p;
+	if (x->phase == 0) {
+		const uint64_t lo = x->nb * (uint64_t)t / (uint64_t)threads, hi = x->nb * (uint64_t)(t + 1) / (uint64_t)threads;
+		for (uint64_t i = lo; i < hi; i++) {
+			const int64_t k = x->bk[i];
+			for (uint64_t h = hj_hash(k, x->shift);; h = (h + 1) & x->mask) {
+				hj_slot *s = &x->tab[h];
+				uint64_t st = __atomic_load_n(&s->state, __ATOMIC_ACQUIRE);
+				if (st == 0) {
+					uint64_t zero = 0;
+					if (__atomic_compare_exchange_n(&s->state, &zero, 1, 0, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE)) {
+						s->key = k;
+						__atomic_store_n(&s->state, 2, __ATOMIC_RELEASE);
+						st = 2;
+					} else {
+						st = zero;
+					}
+				}
+				while (st == 1) {
+					st = __atomic_load_n(&s->state, __ATOMIC_ACQUIRE);
+				}
+				if (s->key == k) {
+					__atomic_fetch_add(&s->count, 1, __ATOMIC_RELAXED);
+					__atomic_fetch_add(&s->sum, (int64_t)x->pay[i], __ATOMIC_RELAXED);
+					break;
+				}
+			}
+		}
+	} else {
+		const uint64_t lo = x->np * (uint64_t)t / (uint64_t)threads, hi = x->np * (uint64_t)(t + 1) / (uint64_t)threads;
+		uint64_t cnt = 0;
+		i128 sum = 0;
+		for (uint64_t i = lo; i < hi; i++) {
+			const int64_t k = x->fk[i];
+			for (uint64_t h = hj_hash(k, x->shift);; h = (h + 1) & x->mask) {
+				const hj_slot *s = &x->tab[h];
+				if (s->state == 0) {
+					break;
+				}
+				if (s->key == k) {
+					cnt += s->count;
+					sum += (i128)s->sum;
+					break;
+				}
+			}
+		}
+		x->count[t] = cnt;
+		x->sum[t] = sum;
+	}
+}
+
+uint64_t vo_join_sum_count_mt(const int64_t *build_key, const int32_t *payload, uint64_t nb, const int64_t *fk,
+                              uint64_t np, int threads, uint64_t *sum_lo, int64_t *sum_hi) {
+	hj_ctx *x = (hj_ctx *)calloc(1, sizeof(hj_ctx));
+	threads = clamp_threads(threads);
+	int bits = 1;
+	while ((1ull << bits) < 2 * (nb ? nb : 1)) {
+		bits++;
+	}
+	x->tab = (hj_slot *)calloc(1ull << bits, sizeof(hj_slot));
+	x->mask = (1ull << bits) - 1;
+	x->shift = 64 - bits;
+	x->bk = build_key;
+	x->pay = payload;
+	x->nb = nb;
+	x->fk = fk;
+	x->np = np;
+	x->phase = 0;
+	par_for(threads, hj_task, x);
+	x->phase = 1;
+	par_for(threads, hj_task, x);
+	uint64_t count = 0;
+	i128 sum = 0;
+	for (int t = 0; t < threads; t++) {
+		count += x->count[t];
+		sum += x->sum[t];
+	}
+	free(x->tab);
+	free(x);
+	*sum_lo = (uint64_t)(u128)sum;
+	*sum_hi = (int64_t)(sum >> 64);
+	return count;
+}
+
+/* --- config 2b on all threads: the materialized cross product + filter in left-major order. Each left row's
+ * output count comes from a binary search over the sorted right side; a prefix over those counts gives each
+ * left row its output offset, and each thread writes its left rows' pairs, scanning the right side in row
+ * order. */
+typedef struct {
+	const int32_t *x, *y;
+	uint64_t nl, nr;
+	int64_t c;
+	const int64_t *ys;
+	uint64_t *offset;
+	int32_t *ox, *oy;
+	int phase;
+} cf_ctx;
+
+static void cf_task(void *p, int t, int threads) {
+	cf_ctx *q = (cf_ctx *)p;
+	const uint64_t lo = q->nl * (uint64_t)t / (uint64_t)threads, hi = q->nl * (uint64_t)(t + 1) / (uint64_t)threads;
+	for (uint64_t i = lo; i < hi; i++) {
+		const int64_t bound = q->c - (int64_t)q->x[i];
+		if (q->phase == 0) {
+			uint64_t a = 0, b = q->nr;
+			while (a < b) {
+				const uint64_t mid = (a + b) >> 1;
+				if (q->ys[mid] < bound) {
+					a = mid + 1;
+				} else {
+					b = mid;
+				}
+			}
+			q->offset[i + 1] = a;
+		} else {
+			uint64_t k = q->offset[i];
+			const int32_t xi = q->x[i];
+			for (uint64_t j = 0; j < q->nr; j++) {
+				if ((int64_t)q->y[j] < bound) {
+					q->ox[k] = xi;
+					q->oy[k] = q->y[j];
+					k++;
+				}
+			}
+		}
+	}
+}
+
+uint64_t vo_cross_filter_pairs_mt(const int32_t *x, uint64_t nl, const int32_t *y, uint64_t nr, int64_t c,
+                                  int32_t *out_x, int32_t *out_y, uint64_t cap, int threads) {
+	cf_ctx q;
+	memset(&q, 0, sizeof(q));
+	threads = clamp_threads(threads);
+	int64_t *ys = (int64_t *)malloc((nr ? nr : 1) * sizeof(int64_t));
+	for (uint64_t j = 0; j < nr; j++) {
+		ys[j] = y[j];
+	}
+	qsort(ys, nr, sizeof(int64_t), cmp_i64);
+	q.x = x;
+	q.y = y;
+	q.nl = nl;
+	q.nr = nr;
+	q.c = c;
+	q.ys = ys;
+	q.offset = (uint64_t *)calloc(nl + 1, sizeof(uint64_t));
+	q.ox = out_x;
+	q.oy = out_y;
+	q.phase = 0;
+	par_for(threads, cf_task, &q);
+	for (uint64_t i = 0; i < nl; i++) {
+		q.offset[i + 1] += q.offset[i];
+	}
+	const uint64_t total = q.offset[nl];
+	if (total <= cap) {
+		q.phase = 1;
+		par_for(threads, cf_task, &q);
+	}
+	free(ys);
+	free(q.offset);
+	return total;
+}
+
+/* --- config 3 on all threads: the batched queries one after another, as a CPU engine runs them: for query q,
+ * rows [offsets[q], offsets[q+1]) are filtered (a < a_lt AND c > c_gt) into compact columns on all threads,
+ * then grouped by vo_group_by_i64_mt. Query q's groups land at out rows [group_offsets[q],
+ * group_offsets[q+1]) (nq + 1 entries written). Returns the total group count. */
+typedef struct {
+	const int32_t *a;
+	const double *c;
+	const int64_t *key, *v1;
+	const double *v2;
+	uint64_t r0, n;
+	int32_t a_lt;
+	double c_gt;
+	int64_t *ok, *o1;
+	double *o2;
+	uint64_t count[256];
+	uint64_t offset[256];
+	int phase;
+} bq_ctx;
+
+static void bq_task(void *p, int t, int threads) {
+	bq_ctx *x = (bq_ctx *)p;
+	const uint64_t lo = x->r0 + x->n * (uint64_t)t / (uint64_t)threads;
+	const uint64_t hi = x->r0 + x->n * (uint64_t)(t + 1) / (uint64_t)threads;
+	if (x->phase == 0) {
+		uint64_t k = 0;
+		for (uint64_t i = lo; i < hi; i++) {
+			k += (x->a[i] < x->a_lt) & (x->c[i] > x->c_gt);
+		}
+		x->count[t] = k;
+	} else {
+		uint64_t k = x->offset[t];
+		for (uint64_t i = lo; i < hi; i++) {
+			if (x->a[i] < x->a_lt && x->c[i] > x->c_gt) {
+				x->ok[k] = x->key[i];
+				x->o1[k] = x->v1[i];
+				x->o2[k] = x->v2[i];
+				k++;
+			}
+		}
+	}
+}
+
+uint64_t vo_batched_filter_group_by_mt(const int32_t *a, const double *c, const int64_t *key, const int64_t *v1,
+                                       const double *v2, const int64_t *offsets, uint64_t nq, int32_t a_lt,
+                                       double c_gt, int threads, int64_t *out_key, int64_t *out_count,
+                                       int64_t *out_sum_v1, int64_t *out_min_v1, int64_t *out_max_v1,
+                                       double *out_sum_v2, double *out_avg_v2, uint64_t *group_offsets) {
+	bq_ctx *x = (bq_ctx *)calloc(1, sizeof(bq_ctx));
+	threads = clamp_threads(threads);
+	uint64_t maxrows = 1;
+	for (uint64_t q = 0; q < nq; q++) {
+		const uint64_t r = (uint64_t)(offsets[q + 1] - offsets[q]);
+		maxrows = r > maxrows ? r : maxrows;
+	}
+	x->ok = (int64_t *)malloc(maxrows * sizeof(int64_t));
+	x->o1 = (int64_t *)malloc(maxrows * sizeof(int64_t));
+	x->o2 = (double *)malloc(maxrows * sizeof(double));
+	x->a = a;
+	x->c = c;
+	x->key = key;
+	x->v1 = v1;
+	x->v2 = v2;
+	x->a_lt = a_lt;
+	x->c_gt = c_gt;
+	uint64_t g = 0;
+	for (uint64_t q = 0; q < nq; q++) {
+		group_offsets[q] = g;
+		x->r0 = (uint64_t)offsets[q];
+		x->n = (uint64_t)(offsets[q + 1] - offsets[q]);
+		x->phase = 0;
+		par_for(threads, bq_task, x);
+		uint64_t kept = 0;
+		for (int t = 0; t < threads; t++) {
+			x->offset[t] = kept;
+			kept += x->count[t];
+		}
+		x->phase = 1;
+		par_for(threads, bq_task, x);
+		g += vo_group_by_i64_mt(x->ok, x->o1, x->o2, kept, threads, out_key + g, out_count + g, out_sum_v1 + 2 * g,
+		                        out_min_v1 + g, out_max_v1 + g, out_sum_v2 + g, out_avg_v2 + g);
+	}
+	group_offsets[nq] = g;
+	free(x->ok);
+	free(x->o1);
+	free(x->o2);
+	free(x);
+	return g;
+}

@@ -18,6 +18,7 @@
  *   vo_cross_filter_count        COUNT/SUM(x)/SUM(y) of L x R WHERE x + y < c, and the left-major pair list
  *                                 (cuda_engine/src/operators/gpu_cross_join.cu:66-71)
  *   vo_exact_sum                 correctly rounded double sum (the reference for DOUBLE SUM/AVG tolerance)
+ *   vo_*_mt                      the same results on all host threads: the timed CPU baselines
  */
 #ifndef VECTIS_ORACLE_H
 #define VECTIS_ORACLE_H
@@ -64,6 +65,24 @@ uint64_t vo_cross_filter_count(const int32_t *x, uint64_t nl, const int32_t *y, 
                                int64_t *sum_x, int64_t *sum_y);
 uint64_t vo_cross_filter_pairs(const int32_t *x, uint64_t nl, const int32_t *y, uint64_t nr, int64_t c,
                                int32_t *out_x, int32_t *out_y, uint64_t cap);
+
+/* Multithreaded CPU baselines (all host threads): the same results as the single-threaded functions above.
+ *   vo_filter_project_cfg0_mt       vo_filter_project_cfg0 split over row ranges
+ *   vo_join_sum_count_mt            vo_join_sum_count as a hash join (shared open-addressing build table)
+ *   vo_cross_filter_pairs_mt        vo_cross_filter_pairs split over left rows (left-major order kept);
+ *                                    writes nothing unless the count fits `cap`
+ *   vo_batched_filter_group_by_mt   per query: filter a < a_lt AND c > c_gt, then vo_group_by_i64_mt */
+uint64_t vo_filter_project_cfg0_mt(const int32_t *a, const int64_t *b, const double *c, uint64_t n, int32_t a_lt,
+                                   double c_gt, int64_t *out1, double *out2, int threads);
+uint64_t vo_join_sum_count_mt(const int64_t *build_key, const int32_t *payload, uint64_t nb, const int64_t *fk,
+                              uint64_t np, int threads, uint64_t *sum_lo, int64_t *sum_hi);
+uint64_t vo_cross_filter_pairs_mt(const int32_t *x, uint64_t nl, const int32_t *y, uint64_t nr, int64_t c,
+                                  int32_t *out_x, int32_t *out_y, uint64_t cap, int threads);
+uint64_t vo_batched_filter_group_by_mt(const int32_t *a, const double *c, const int64_t *key, const int64_t *v1,
+                                       const double *v2, const int64_t *offsets, uint64_t nq, int32_t a_lt,
+                                       double c_gt, int threads, int64_t *out_key, int64_t *out_count,
+                                       int64_t *out_sum_v1, int64_t *out_min_v1, int64_t *out_max_v1,
+                                       double *out_sum_v2, double *out_avg_v2, uint64_t *group_offsets);
 
 #ifdef __cplusplus
 }

@@ -20,9 +20,7 @@ def build() -> list:
               str(BIN / "test_device_components"), *common, "-pthread"]
     pipeline = ["g++", str(HERE / "test_stream_pipeline.cpp"), "-o", str(BIN / "test_stream_pipeline"), *common,
                 "-pthread"]
-    bench = ["g++", str(HERE / "bench_stream_pipeline.cpp"), "-o", str(BIN / "bench_stream_pipeline"), "-std=c++17",
-             "-O2", f"-I{ROOT / 'include'}", f"-L{ROOT / 'vectisdb_amd' / 'lib'}", "-lvgpu", RPATH, "-pthread"]
-    for cmd in (host, device, pipeline, bench):
+    for cmd in (host, device, pipeline):
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError(f"build failed:\n{' '.join(cmd)}\n{res.stderr}")

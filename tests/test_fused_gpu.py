@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from vectisdb_amd import capi
-from exprs import cast, cfg0_exprs, col, copy_body, dlit, out
+from vectisdb_amd.workloads import cast, cfg0_exprs, col, copy_body, dlit, out
 
 pytestmark = pytest.mark.gpu
 

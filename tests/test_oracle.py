@@ -203,3 +203,80 @@ def test_multithreaded_group_by_wide_range_falls_back():
     b = oracle.group_by_i64(key, v1, v2, threads=4)
     np.testing.assert_array_equal(a["key"], b["key"])
     np.testing.assert_array_equal(a["sum_v2"], b["sum_v2"])
+
+
+# ---------------------------------------------------------------- multithreaded CPU baselines
+# Each _mt function must return exactly what its single-threaded restatement returns (itself pinned above).
+
+@pytest.mark.parametrize("threads,n", [(1, 0), (3, 1), (8, 100_003), (16, 7)])
+def test_filter_project_mt_matches_single_threaded(threads, n):
+    rng = np.random.default_rng(n + threads)
+    a = rng.integers(0, 10**6, n).astype(np.int32)
+    b = rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+    c = rng.random(n)
+    o1, o2 = oracle.filter_project_cfg0(a, b, c)
+    m1, m2, _ = oracle.filter_project_cfg0_mt(a, b, c, threads)
+    np.testing.assert_array_equal(m1, o1)
+    assert m2.tobytes() == o2.tobytes()
+
+
+@pytest.mark.parametrize("threads", [1, 4, 8])
+def test_join_hash_mt_matches_sorted_restatement(threads):
+    rng = np.random.default_rng(threads)
+    nb = 20_000
+    build = rng.integers(-3000, 3000, nb).astype(np.int64)       # duplicate build keys, negative keys
+    payload = rng.integers(-1000, 1000, nb).astype(np.int32)
+    fk = rng.integers(-5000, 5000, 200_000).astype(np.int64)
+    s, n = oracle.join_sum_count(build, payload, fk)
+    s2, n2, _ = oracle.join_sum_count_mt(build, payload, fk, threads)
+    assert (s2, n2) == (s, n)
+    # a permutation build side: the closed form
+    perm = rng.permutation(1 << 14).astype(np.int64)
+    pay = rng.integers(0, 1000, 1 << 14).astype(np.int32)
+    fk = rng.integers(0, 1 << 15, 100_000).astype(np.int64)
+    inv = np.empty(1 << 14, np.int64)
+    inv[perm] = np.arange(1 << 14)
+    hit = fk < (1 << 14)
+    s3, n3, _ = oracle.join_sum_count_mt(perm, pay, fk, threads)
+    assert n3 == int(hit.sum()) and s3 == int(pay[inv[fk[hit]]].astype(np.int64).sum())
+    # empty sides
+    assert oracle.join_sum_count_mt(perm, pay, fk[:0], threads)[:2] == (0, 0)
+    assert oracle.join_sum_count_mt(perm[:0], pay[:0], fk, threads)[:2] == (0, 0)
+
+
+@pytest.mark.parametrize("threads,nl,nr", [(1, 5, 7), (4, 33, 1000), (16, 64, 4096), (8, 0, 10)])
+def test_cross_pairs_mt_matches_single_threaded(threads, nl, nr):
+    rng = np.random.default_rng(nl * 31 + nr)
+    x = rng.integers(0, 1 << 30, nl).astype(np.int32)
+    y = rng.integers(0, 1 << 30, nr).astype(np.int32)
+    ex, ey = oracle.cross_filter_pairs(x, y, 1 << 30)
+    gx, gy, _ = oracle.cross_filter_pairs_mt(x, y, 1 << 30, threads)
+    np.testing.assert_array_equal(gx, ex)
+    np.testing.assert_array_equal(gy, ey)
+
+
+@pytest.mark.parametrize("threads", [1, 6])
+def test_batched_group_by_mt_matches_per_query_restatement(threads):
+    rng = np.random.default_rng(threads + 100)
+    sizes = np.array([0, 1, 5000, 17, 40_000, 3])
+    n = int(sizes.sum())
+    a = rng.integers(0, 10**6, n).astype(np.int32)
+    c = rng.random(n)
+    key = rng.integers(0, 300, n).astype(np.int64)
+    v1 = rng.integers(-10**6, 10**6, n).astype(np.int64)
+    v2 = rng.standard_normal(n)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    res, goffs, _ = oracle.batched_filter_group_by_mt(a, c, key, v1, v2, offs, threads)
+    assert goffs[0] == 0 and goffs[-1] == len(res["key"])
+    for q in range(len(sizes)):
+        r0, r1 = offs[q], offs[q + 1]
+        keep = (a[r0:r1] < 500000) & (c[r0:r1] > 0.25)
+        g0, g1 = goffs[q], goffs[q + 1]
+        if not keep.any():
+            assert g0 == g1
+            continue
+        ref = oracle.group_by_i64(key[r0:r1][keep], v1[r0:r1][keep], v2[r0:r1][keep], raw=True)
+        for k in ("key", "count", "min_v1", "max_v1", "sum_v1"):
+            np.testing.assert_array_equal(res[k][g0:g1], ref[k])
+        assert res["sum_v2"][g0:g1].tobytes() == ref["sum_v2"].tobytes()
+        assert res["avg_v2"][g0:g1].tobytes() == ref["avg_v2"].tobytes()

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from exprs import cfg0_exprs
+from vectisdb_amd.workloads import cfg0_exprs
 from vectisdb_amd import capi
 
 pytestmark = pytest.mark.gpu

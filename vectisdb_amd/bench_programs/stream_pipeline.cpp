@@ -1,4 +1,4 @@
-// bench_stream_pipeline.cpp -- throughput of GpuStreamPipeline (cuda_engine/src/gpu_stream_pipeline.cu's
+// stream_pipeline.cpp -- the benchmark program for throughput of GpuStreamPipeline (cuda_engine/src/gpu_stream_pipeline.cu's
 // interface): host batches in page-locked pool slots go H2D, through the generated projection kernel, and
 // back D2H, three batches in flight. Prints one JSON line; bench.py reports it as an operator line.
 // The batch contents are written once per slot before the timed loop (the producer's own copy is not

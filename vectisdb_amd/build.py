@@ -76,6 +76,13 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources", flush=True)
+    # the benchmark's GpuStreamPipeline program: a C++ caller of the drop-in headers, linked like the extension
+    exe = LIBDIR / "bench_stream_pipeline"
+    cmd = ["g++", str(PKG / "bench_programs" / "stream_pipeline.cpp"), "-o", str(exe), "-std=c++17", "-O2",
+           f"-I{ROOT / 'include'}", f"-L{LIBDIR}", "-lvgpu", "-Wl,-rpath,$ORIGIN", "-pthread"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"build failed:\n{' '.join(cmd)}\n{res.stderr}")
     return LIB
 
 

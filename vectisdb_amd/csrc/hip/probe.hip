@@ -617,8 +617,12 @@ void LaunchBucketProbeY(const ProbePartition &p, const uint64_t *dense, const ui
 	const double ybytes = std::is_void<Y>::value ? 0.0 : (double)p.payload_width + 2.0 * sizeof(typename std::conditional<
 	                                                                                std::is_void<Y>::value, int, Y>::type);
 	{
-		ProfileScope prof("partition_probe", stream, (double)p.n * 4 + (double)p.n * ybytes / 2);
-		if (dense16 != nullptr && (1ull << p.shift) <= kSliceEntries) {
+		// the two forms are profiled under their own names: "_lds" looks keys up in the bucket's 16-bit table
+		// slice copied to LDS, "_l2" in the 64-bit table through the caches
+		const bool lds = dense16 != nullptr && (1ull << p.shift) <= kSliceEntries;
+		ProfileScope prof(lds ? "partition_probe_lds" : "partition_probe_l2", stream,
+		                  (double)p.n * 4 + (double)p.n * ybytes / 2);
+		if (lds) {
 			hipLaunchKernelGGL((PartProbeKernel<true, Y>), dim3(nparts), dim3(kPartBlock), 0, stream, p.part,
 			                   p.part_y, p.tab, (u64)p.nbatches, (u64)p.batch, dense16, dense, (u64)p.span, p.shift,
 			                   p.partials);
