@@ -177,6 +177,17 @@ int vgpu_per_query_filter_group_by(const vgpu_column *cols, int32_t ncols, uint6
 int vgpu_filter_gt_i32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_host,
                        void *stream);
 
+/* vgpu_filter_gt_i32 without any allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The kernel's look-back words, tile ticket and kept count live in a fixed array of 1 MiB per
+ * GPU, held until the process ends; n is limited to vgpu_filter_gt_i32_noalloc_max_rows() (2^31) and larger
+ * inputs fail with VGPU_ERROR before any GPU work. Same output and *count_host as vgpu_filter_gt_i32
+ * (synchronizes). Every call on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * threads, other streams) race and must be serialized by the caller. */
+int vgpu_filter_gt_i32_noalloc(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_host,
+                               void *stream);
+uint64_t vgpu_filter_gt_i32_noalloc_max_rows(void);
+
 /* out[i] = src[idx[i]] for elements of `width` bytes (1/2/4/8/16); idx is uint32 (idx64 = 0) or uint64.
  * Replaces GatherColumn / MaterializeColumn (cuda_engine/src/gpu_executor.cu:144-252). */
 int vgpu_gather(const void *src, int32_t width, const void *idx, int32_t idx64, uint64_t n, void *out,

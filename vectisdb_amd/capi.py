@@ -108,6 +108,9 @@ _PROTOS = {
                                 C.POINTER(C.c_uint64), C.c_void_p]),
     "vgpu_sort_column": (C.c_int, [C.POINTER(Column), C.c_void_p, C.c_void_p, C.c_void_p]),
     "vgpu_filter_gt_i32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
+    "vgpu_filter_gt_i32_noalloc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p, C.POINTER(C.c_uint64),
+                                             C.c_void_p]),
+    "vgpu_filter_gt_i32_noalloc_max_rows": (C.c_uint64, []),
     "vgpu_gather": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p]),
     "vgpu_generate_column": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_int32, C.c_int64, C.c_int64, C.c_uint64,
                                        C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
@@ -285,6 +288,17 @@ def filter_gt_i32(x, threshold, stream=None):
     check(lib.vgpu_filter_gt_i32(C.c_void_p(x.data_ptr()), x.numel(), threshold, C.c_void_p(out.data_ptr()),
                                  C.byref(count), _stream_ptr(stream)))
     return out[: count.value]
+
+
+def filter_gt_i32_noalloc(x, out, threshold, stream=None):
+    """vgpu_filter_gt_i32_noalloc into the caller's device buffer `out` (int32, at least x.numel() slots);
+    returns the kept count. The library allocates nothing for it."""
+    if out.numel() < x.numel():
+        raise ValueError("filter_gt_i32_noalloc: output buffer smaller than the input")
+    count = C.c_uint64(0)
+    check(load().vgpu_filter_gt_i32_noalloc(C.c_void_p(x.data_ptr()), x.numel(), threshold,
+                                            C.c_void_p(out.data_ptr()), C.byref(count), _stream_ptr(stream)))
+    return count.value
 
 
 def generate_column(out, vtype, dist, lo=0, hi=0, seed=0, stream_id=0, cdf=None, perm=None, stream=None):

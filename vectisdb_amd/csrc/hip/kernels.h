@@ -336,4 +336,13 @@ uint64_t KeepKeyRange(int vtype, const void *in, uint64_t n, long long lo, long 
 void FilterGtI32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_device,
                  Scratch &scratch, hipStream_t stream);
 
+//! The same compaction with its look-back words, tile ticket and count in a fixed per-device array instead of
+//! allocated scratch: no allocation of any kind. *count_host receives the kept count (synchronizes). Throws
+//! when n exceeds FilterGtI32NoAllocMaxRows(). Calls on one GPU share the array, so overlapping calls race.
+void FilterGtI32NoAlloc(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_host,
+                        hipStream_t stream);
+uint64_t FilterGtI32NoAllocMaxRows();
+//! Resolves the current device's copy of the fixed scratch array (once per device).
+void NoAllocScratchInit();
+
 } // namespace vgpu

@@ -541,4 +541,72 @@ void FilterGtI32(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out,
 	                         stream);
 }
 
+// ------------------------------------------------------------------ FilterGreaterThanInt32 without allocation
+//
+// The range-compaction kernel needs one look-back word per tile, the tile ticket and the kept count. For
+// callers that cannot allocate, they live in `scratch`, one array per GPU for the life of the process:
+// kNoAllocMaxTiles + 2 words (1 MiB + 16 B), enough for 2^31 rows at 16,384 rows per tile. In host code
+// `scratch` names a host-side stand-in, so the device address of each GPU's copy is resolved once per device
+// with hipGetSymbolAddress and only that address reaches kernels and fills.
+
+namespace {
+constexpr uint64_t kNoAllocMaxTiles = 1ull << 17;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) u64 scratch[kNoAllocMaxTiles + 2];
+
+std::mutex g_noalloc_mutex;
+u64 *g_noalloc_scratch[kNoAllocMaxDevices] = {};
+} // namespace
+
+void NoAllocScratchInit() {
+	int device = 0;
+	VGPU_HIP(hipGetDevice(&device));
+	if (device < 0 || device >= kNoAllocMaxDevices) {
+		throw HipError("no-allocation scratch: device ordinal out of range");
+	}
+	std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+	if (g_noalloc_scratch[device] != nullptr) {
+		return;
+	}
+	void *ptr = nullptr;
+	VGPU_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+	if (ptr == nullptr) {
+		throw HipError("no-allocation scratch: hipGetSymbolAddress returned no address");
+	}
+	g_noalloc_scratch[device] = static_cast<u64 *>(ptr);
+}
+
+uint64_t FilterGtI32NoAllocMaxRows() {
+	return kNoAllocMaxTiles * 512ull * KeepRangeItems<int32_t>();
+}
+
+void FilterGtI32NoAlloc(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_host,
+                        hipStream_t stream) {
+	if (n > FilterGtI32NoAllocMaxRows()) {
+		throw HipError("FilterGreaterThanInt32 (no allocation): " + std::to_string(n) +
+		               " rows exceed the fixed scratch limit of " + std::to_string(FilterGtI32NoAllocMaxRows()) +
+		               " rows");
+	}
+	if (n == 0) {
+		*count_host = 0;
+		return;
+	}
+	NoAllocScratchInit();
+	int device = 0;
+	VGPU_HIP(hipGetDevice(&device));
+	u64 *state = nullptr;
+	{
+		std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+		state = g_noalloc_scratch[device];
+	}
+	const uint64_t ntiles = KeepRangeTiles<int32_t>(n);
+	// look-back words and the ticket start at zero (a fill kernel, not a memset node, so the call can be
+	// captured); the count word is written by the last tile
+	FillU64(state, 0ull, ntiles + 1, stream);
+	LaunchKeepRange<int32_t>(in, n, (long long)threshold + 1, (long long)INT32_MAX, out, state, state + ntiles + 1,
+	                         stream);
+	VGPU_HIP(hipMemcpyAsync(count_host, state + ntiles + 1, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+	VGPU_HIP(hipStreamSynchronize(stream));
+}
+
 } // namespace vgpu

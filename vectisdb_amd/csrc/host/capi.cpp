@@ -50,6 +50,7 @@ int vgpu_device_init(int device) {
 	return Guard([&] {
 		VGPU_HIP(hipSetDevice(device));
 		VGPU_HIP(hipFree(nullptr));
+		NoAllocScratchInit();
 	});
 }
 
@@ -215,6 +216,15 @@ int vgpu_filter_gt_i32(const int32_t *in, uint64_t n, int32_t threshold, int32_t
 		VGPU_HIP(hipMemcpyAsync(count_host, count, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
 		VGPU_HIP(hipStreamSynchronize(s));
 	});
+}
+
+int vgpu_filter_gt_i32_noalloc(const int32_t *in, uint64_t n, int32_t threshold, int32_t *out, uint64_t *count_host,
+                               void *stream) {
+	return Guard([&] { FilterGtI32NoAlloc(in, n, threshold, out, count_host, static_cast<hipStream_t>(stream)); });
+}
+
+uint64_t vgpu_filter_gt_i32_noalloc_max_rows(void) {
+	return FilterGtI32NoAllocMaxRows();
 }
 
 int vgpu_gather(const void *src, int32_t width, const void *idx, int32_t idx64, uint64_t n, void *out,
