@@ -3,16 +3,17 @@
 //
 // Columns live in HBM as a sequence of chunks (one per scan batch). A scan stages chunks into an open slot
 // (BeginStaging / StageChunk) and publishes them only when CommitStaging confirms that every column holds
-// exactly the scanned row count; anything else is discarded. Committed columns are evicted LRU under a byte
-// budget (default half of free HBM: ~140 GB on one MI355X; VECTOR_GPU_CACHE_FRACTION / VECTOR_GPU_CACHE_MB
-// override). Device pointers handed out are borrows, valid until the next call that can evict.
+// exactly the scanned row count; anything else is discarded. Committed columns are evicted least recently used
+// first under a byte budget sized for a 288 GB MI355X: three quarters of the free HBM at first use, always
+// leaving 24 GiB free for operator intermediates (about 210 GB of tables; VECTOR_GPU_CACHE_FRACTION /
+// VECTOR_GPU_CACHE_MB override). Device pointers handed out are borrows, valid until the next call that can
+// evict.
 
 #include <cstdint>
-#include <functional>
-#include <list>
+#include <map>
 #include <mutex>
 #include <string>
-#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "gpu_engine.hpp"
@@ -39,8 +40,8 @@ public:
 	GpuColumnCache(const GpuColumnCache &) = delete;
 	GpuColumnCache &operator=(const GpuColumnCache &) = delete;
 
-	//! Sizes the budget once: fraction_of_vram x free device memory (env overrides).
-	void EnsureInitialized(double fraction_of_vram = 0.5);
+	//! Sizes the budget once: min(fraction_of_vram x free HBM, free HBM - 24 GiB) (env overrides).
+	void EnsureInitialized(double fraction_of_vram = 0.75);
 
 	//! Opens (or re-opens, discarding) the staging slot of `table`.
 	void BeginStaging(const GpuTableRef &table);
@@ -65,57 +66,46 @@ private:
 	GpuColumnCache();
 	~GpuColumnCache();
 
-	struct CachedColumn {
+	// A committed column: its chunks, and its place in the recency index.
+	struct Resident {
 		std::vector<GpuColumnChunk> chunks;
-		uint64_t row_count = 0;
-		uint64_t total_bytes = 0;
-		uint64_t generation = 0;
+		uint64_t rows = 0;
+		uint64_t bytes = 0;
+		uint64_t last_use = 0;
 	};
-	struct StagedColumn {
+	// A column being staged by an open scan.
+	struct Pending {
 		std::vector<GpuColumnChunk> chunks;
-		uint64_t row_count = 0;
-		uint64_t total_bytes = 0;
+		uint64_t rows = 0;
+		uint64_t bytes = 0;
 		GpuValueType type = GpuValueType::FLOAT64;
-		bool type_known = false;
-		bool failed = false;
+		bool typed = false;
+		bool dead = false; // a chunk was refused: the column cannot be committed by this scan
 	};
-	struct SlotKey {
-		GpuTableRef table;
-		std::string column;
-		bool operator==(const SlotKey &other) const {
-			return table == other.table && column == other.column;
-		}
+	struct Table {
+		uint64_t generation = 0;
+		bool staging = false;
+		std::map<std::string, Resident> resident;
+		std::map<std::string, Pending> pending;
 	};
-	struct TableRefHash {
-		size_t operator()(const GpuTableRef &ref) const {
-			std::hash<std::string> h;
-			size_t seed = h(ref.catalog);
-			seed = seed * 1000003u ^ h(ref.schema);
-			seed = seed * 1000003u ^ h(ref.table);
-			return seed;
-		}
-	};
-	struct SlotKeyHash {
-		size_t operator()(const SlotKey &key) const {
-			return TableRefHash {}(key.table) * 31u ^ std::hash<std::string> {}(key.column);
-		}
-	};
+	using Key = std::string; // catalog, schema and table joined by NUL bytes
 
-	void EvictLocked(const SlotKey &slot);
-	void EvictTableLocked(const GpuTableRef &table);
-	void AbortStagingLocked(const GpuTableRef &table);
-	void FailStagedColumnLocked(StagedColumn &staged);
-	void TouchLocked(const SlotKey &slot);
+	static Key KeyOf(const GpuTableRef &table);
+	void EnsureInitializedLocked(double fraction_of_vram);
+	bool MakeRoomLocked(uint64_t need);
+	void DropResidentLocked(Table &table, std::map<std::string, Resident>::iterator it);
+	void DropPendingLocked(Table &table);
+	void UseLocked(const Key &key, const std::string &column, Resident &r);
 
 	mutable std::mutex mutex_;
 	bool initialized_ = false;
 	uint64_t budget_bytes_ = 0;
-	uint64_t committed_bytes_ = 0;
-	uint64_t staged_bytes_ = 0;
-	std::list<SlotKey> lru_order_;
-	std::unordered_map<SlotKey, std::pair<CachedColumn, std::list<SlotKey>::iterator>, SlotKeyHash> entries_;
-	std::unordered_map<GpuTableRef, std::unordered_map<std::string, StagedColumn>, TableRefHash> staging_;
-	std::unordered_map<GpuTableRef, uint64_t, TableRefHash> generation_;
+	uint64_t resident_bytes_ = 0;
+	uint64_t pending_bytes_ = 0;
+	uint64_t use_clock_ = 0;
+	std::map<Key, Table> tables_;
+	// least recently used first: use tick -> (table key, column)
+	std::map<uint64_t, std::pair<Key, std::string>> recency_;
 };
 
 } // namespace vector_gpu

@@ -47,19 +47,12 @@ private:
 	GpuLogger(const GpuLogger &) = delete;
 	GpuLogger &operator=(const GpuLogger &) = delete;
 
-	void EnsureConfigured();
-	void RotateIfNeededLocked(size_t incoming_bytes);
-	void OpenActiveLocked(bool truncate);
-
+	// Read by the inline Enabled() on every VGPU_LOG: the only state callers touch.
 	std::atomic<bool> enabled_ {false};
 	std::atomic<int> level_ {static_cast<int>(LogLevel::INFO)};
-	mutable std::mutex mutex_;
-	bool configured_ = false;
-	std::string directory_;
-	size_t half_cap_bytes_ = 25u * 1024u * 1024u;
-	int active_file_ = 0;
-	size_t active_bytes_ = 0;
-	void *file_ = nullptr; // FILE*
+	// Everything else (directory, files, byte counts) lives in the sink, behind its own mutex.
+	struct Sink;
+	Sink *sink_ = nullptr;
 };
 
 } // namespace vector_gpu

@@ -2,7 +2,7 @@
 // gpu_memory_pool.hpp -- fixed ring of page-locked batch slots feeding the stream pipeline (drop-in for
 // cuda_engine/include/gpu_memory_pool.hpp).
 //
-// kSlotCount slots of kSlotBytes each are allocated with hipHostMalloc(hipHostMallocPortable) the first
+// kSlotCount slots of kSlotBytes each, carved from one page-locked region (hipHostMalloc, portable) the first
 // time Instance() runs, and handed out in cyclic order. One slot per concurrently draining batch of the
 // pipeline (kDepth = 3 in flight) plus one for the producer: GpuStreamPipeline static_asserts the
 // relationship. A slot returns to the ring when its GpuMemoryPoolSlot is destroyed.
@@ -51,33 +51,33 @@ public:
 	static constexpr size_t kSlotCount = 4;
 	static constexpr size_t kSlotBytes = 350ull * 1024 * 1024;
 
-	//! Allocates every slot on first use; throws std::runtime_error (freeing what was allocated) when one
-	//! cannot be page-locked, and the next call tries again.
+	//! Page-locks the whole ring on first use; throws std::runtime_error when it cannot be page-locked, and
+	//! the next call tries again.
 	static GpuMemoryPool &Instance();
 
 	GpuMemoryPool(const GpuMemoryPool &) = delete;
 	GpuMemoryPool &operator=(const GpuMemoryPool &) = delete;
 
-	//! Blocks until the next slot in cyclic order is free, checks it out and advances the cursor.
+	//! Takes the next slot in cyclic order, blocking until its previous holder has returned it. Callers are
+	//! served in arrival order (each takes a ticket; ticket t is slot t % kSlotCount).
 	GpuMemoryPoolSlot AcquireNext();
 
-	//! Test-only: rewinds the cursor to slot 0 (no slot may be checked out).
+	//! Test-only: the next AcquireNext takes slot 0 (no slot may be checked out).
 	void ResetCursorForTesting();
 
 private:
 	GpuMemoryPool();
 	~GpuMemoryPool();
 	friend class GpuMemoryPoolSlot;
-	void Release(int index);
+	void Return(int index);
 
-	struct Slot {
-		void *data = nullptr;
-		bool in_use = false;
-	};
+	// One page-locked region of kSlotCount * kSlotBytes; slot i is its i-th kSlotBytes.
+	unsigned char *region_ = nullptr;
+	// Turnstile: ticket t may take slot t % kSlotCount once that slot has been returned t / kSlotCount times.
 	std::mutex mutex_;
-	std::condition_variable cv_;
-	std::array<Slot, kSlotCount> slots_ {};
-	size_t next_index_ = 0;
+	std::condition_variable returned_;
+	unsigned long long next_ticket_ = 0;
+	std::array<unsigned long long, kSlotCount> returns_ {};
 };
 
 } // namespace vector_gpu

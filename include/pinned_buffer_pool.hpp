@@ -1,21 +1,23 @@
 #pragma once
-// pinned_buffer_pool.hpp -- process-wide pool of page-locked host staging slots (drop-in for
-// cuda_engine/include/pinned_buffer_pool.hpp).
+// pinned_buffer_pool.hpp -- process-wide cache of page-locked host staging buffers (engine-private, like
+// cuda_engine/include/pinned_buffer_pool.hpp; only the executor's upload/download staging uses it).
 //
-// Same classes, members and semantics as the reference: a slot is checked out per concurrently
-// transferring buffer, grows to the largest request it has served and never shrinks, and returns itself
-// to the pool when the PinnedBufferSlot is destroyed. The pool grows lazily up to kMaxSlots; beyond that
-// Acquire blocks until a slot is released. Memory comes from hipHostMalloc (page-locked from birth, so
-// DMA engines read it directly; no registration of existing memory is ever attempted).
+// Buffers come in power-of-two size classes (1 MiB and up) and are page-locked from birth with
+// hipHostMalloc, so DMA engines read them directly (no registration of existing memory is attempted).
+// Acquire hands out the smallest idle buffer whose class covers the request; a returned buffer goes back to
+// its class's idle stack (most recently used first). At most kMaxSlots buffers exist at once: at the cap an
+// idle buffer of a too-small class is released to make room, and with none idle Acquire blocks until a
+// buffer comes back.
 
 #include <condition_variable>
 #include <cstddef>
+#include <map>
 #include <mutex>
 #include <vector>
 
 namespace vector_gpu {
 
-//! One checked-out slot. Move-only; releases itself on destruction.
+//! One checked-out buffer. Move-only; goes back to the pool on destruction.
 class PinnedBufferSlot {
 public:
 	PinnedBufferSlot() = default;
@@ -29,44 +31,49 @@ public:
 	void *data() const {
 		return data_;
 	}
+	//! The buffer's size class (>= the request).
+	size_t bytes() const {
+		return bytes_;
+	}
 
 private:
 	friend class PinnedBufferPool;
-	PinnedBufferSlot(class PinnedBufferPool *pool, int index, void *data) : pool_(pool), index_(index), data_(data) {
+	PinnedBufferSlot(class PinnedBufferPool *pool, void *data, size_t bytes) : pool_(pool), data_(data), bytes_(bytes) {
 	}
 	class PinnedBufferPool *pool_ = nullptr;
-	int index_ = -1;
 	void *data_ = nullptr;
+	size_t bytes_ = 0;
 };
 
 class PinnedBufferPool {
 public:
 	static constexpr size_t kMaxSlots = 16;
+	static constexpr size_t kMinClassBytes = 1u << 20;
 
 	static PinnedBufferPool &Instance();
 
-	//! Returns a free slot of at least `bytes`, growing the pool (below kMaxSlots) or the slot's
-	//! allocation as needed; blocks while every slot is checked out at the cap. Throws
-	//! std::runtime_error when the page-locked allocation fails.
+	//! A buffer of at least `bytes` (rounded up to its size class). Blocks while kMaxSlots buffers are checked
+	//! out. Throws std::runtime_error when the page-locked allocation fails.
 	PinnedBufferSlot Acquire(size_t bytes);
 
+	//! Buffers that exist (checked out or idle) and the bytes they hold.
+	size_t LiveBuffers() const;
+	size_t LiveBytes() const;
+
 private:
-	PinnedBufferPool();
+	PinnedBufferPool() = default;
 	~PinnedBufferPool();
 	PinnedBufferPool(const PinnedBufferPool &) = delete;
 	PinnedBufferPool &operator=(const PinnedBufferPool &) = delete;
 
 	friend class PinnedBufferSlot;
-	void Release(int index);
+	void Return(void *data, size_t bytes);
 
-	struct Slot {
-		void *data = nullptr;
-		size_t capacity = 0;
-		bool in_use = false;
-	};
-	std::mutex mutex_;
-	std::condition_variable cv_;
-	std::vector<Slot> slots_;
+	mutable std::mutex mutex_;
+	std::condition_variable returned_;
+	std::map<size_t, std::vector<void *>> idle_; // size class -> idle buffers, most recently returned last
+	size_t live_buffers_ = 0;
+	size_t live_bytes_ = 0;
 };
 
 } // namespace vector_gpu

@@ -5,13 +5,15 @@
 // fraction of the device's currently free HBM (0.85 by default, VECTOR_GPU_VRAM_FRACTION overrides) and
 // HasHeadroomFor answers routing's "does this plan's working set fit" question. The arena itself is only
 // reserved by the first Allocate(), which the executor never calls (it draws from a stream-ordered
-// caching pool); Allocate/Free keep the reference's first-fit, coalescing, size-tracked semantics so
-// tests and any future caller see the same behaviour.
+// caching pool). Allocate/Free sub-allocate that arena best-fit, 256-byte aligned, merging neighbouring
+// holes on Free, with every live block's size tracked by the pool.
 
 #include <cstddef>
 #include <map>
 #include <mutex>
+#include <set>
 #include <unordered_map>
+#include <utility>
 
 namespace vector_gpu {
 
@@ -30,8 +32,9 @@ public:
 	//! the alignment arithmetic away from the top of size_t).
 	bool HasHeadroomFor(size_t estimated_bytes) const;
 
-	//! First-fit sub-allocation; nullptr for 0 bytes, when uninitialized, when the arena cannot be
-	//! reserved, or when no free block is large enough.
+	//! Best-fit sub-allocation (the smallest hole that fits; the lowest address among equal holes);
+	//! nullptr for 0 bytes, when uninitialized, when the arena cannot be reserved, or when no hole is large
+	//! enough.
 	void *Allocate(size_t bytes);
 
 	//! Returns a block from Allocate(). The tracked size is used (`bytes` is accepted for interface
@@ -58,16 +61,23 @@ private:
 	RmmPool(const RmmPool &) = delete;
 	RmmPool &operator=(const RmmPool &) = delete;
 
-	void AllocateArenaLocked(size_t requested_bytes);
-	bool EnsureArenaLocked();
-	void ResetBlocksLocked(size_t total);
+	void SetBudgetLocked(size_t bytes);
+	bool ReserveArenaLocked();
+	void AddHoleLocked(size_t offset, size_t size);
+	void RemoveHoleLocked(size_t offset, size_t size);
 
+	// Read by the inline TotalPoolBytes().
 	mutable std::mutex mutex_;
-	bool initialized_ = false;
-	void *arena_ = nullptr;
 	size_t total_pool_bytes_ = 0;
-	std::map<size_t, size_t> free_blocks_;           // offset -> size, coalesced
-	std::unordered_map<size_t, size_t> live_blocks_; // offset -> aligned size
+
+	bool initialized_ = false;
+	void *arena_ = nullptr; // reserved by the first Allocate()
+	// Free space as holes indexed twice: by offset (to merge neighbours on Free) and by (size, offset) (so
+	// Allocate takes the smallest hole that fits and the headroom test reads the largest hole directly).
+	std::map<size_t, size_t> holes_;
+	std::set<std::pair<size_t, size_t>> holes_by_size_;
+	std::unordered_map<size_t, size_t> taken_; // offset -> aligned size of each live allocation
+	size_t taken_bytes_ = 0;
 };
 
 } // namespace vector_gpu

@@ -18,8 +18,8 @@ def build() -> list:
             "-pthread"]
     device = ["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", str(HERE / "test_device_components.cpp"), "-o",
               str(BIN / "test_device_components"), *common, "-pthread"]
-    pipeline = ["g++", str(HERE / "test_stream_pipeline.cpp"), "-o", str(BIN / "test_stream_pipeline"), *common,
-                "-pthread"]
+    pipeline = ["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", str(HERE / "test_stream_pipeline.cpp"), "-o",
+                str(BIN / "test_stream_pipeline"), *common, "-pthread"]
     for cmd in (host, device, pipeline):
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:

@@ -1,376 +1,380 @@
-// test_host_components.cpp -- host-only drop-in components through the C++ headers, as the extension
-// links them: KernelCache + SHA-256 (cases of cuda_engine/tests/test_kernel_cache.cpp) and GpuLogger
-// (cases of cuda_engine/tests/test_gpu_logger.cpp), plus the routing functions of gpu_engine.hpp.
-// Needs no GPU.
-#include "check.hpp"
+// test_host_components.cpp -- the host-side drop-in components through include/*.hpp, linked to libvgpu.so the
+// way the DuckDB extension links the engine: KernelCache and its SHA-256 keys, GpuLogger, the routing
+// functions and thresholds of gpu_engine.hpp, and the engine's parallel host copy. No GPU needed.
+#include "harness.hpp"
+
+#include "code_generator.hpp"
 #include "gpu_engine.hpp"
 #include "gpu_logger.hpp"
 #include "kernel_cache.hpp"
-// the engine's host-copy pool (an internal header: the staging ring of host-column plans uses it)
+// the engine's host-copy pool (an internal header: the staging of host-column plans uses it)
 #include "../../vectisdb_amd/csrc/engine/host_copy.h"
 
-#include <algorithm>
-#include <filesystem>
 #include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <filesystem>
 #include <fstream>
+#include <map>
+#include <memory>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
 
 using namespace vector_gpu;
+namespace fs = std::filesystem;
+
+namespace {
 
 // ---------------------------------------------------------------- KernelCache
 
-static void TestMissThenHit() {
-	KernelCache cache(4);
-	CHECK(cache.Get("a") == nullptr);
-	int a = 1;
-	cache.Insert("a", &a);
-	CHECK(cache.Get("a") == &a);
+std::vector<CompiledKernelHandle> &Deleted() {
+	static std::vector<CompiledKernelHandle> deleted;
+	return deleted;
+}
+void CountingDeleter(CompiledKernelHandle h) {
+	Deleted().push_back(h);
+}
+void OtherDeleter(CompiledKernelHandle) {
+}
+CompiledKernelHandle H(uintptr_t v) {
+	return reinterpret_cast<CompiledKernelHandle>(v);
 }
 
-static void TestOverwriteSameKey() {
-	KernelCache cache(4);
-	int a1 = 1, a2 = 2;
-	cache.Insert("a", &a1);
-	cache.Insert("a", &a2);
-	CHECK(cache.Get("a") == &a2);
+} // namespace
+
+VT_CASE(CacheKeepsTheMostRecentlyUsedEntries) {
+	KernelCache cache(3);
+	cache.Insert("k1", H(1));
+	cache.Insert("k2", H(2));
+	cache.Insert("k3", H(3));
+	EXPECT(cache.Get("k1") == H(1)); // k1 becomes the most recent; k2 is now the oldest
+	cache.Insert("k4", H(4));
+	EXPECT(cache.Get("k2") == nullptr);
+	EXPECT(cache.Get("k1") == H(1) && cache.Get("k3") == H(3) && cache.Get("k4") == H(4));
+	cache.Insert("k5", H(5)); // oldest after those lookups: k1
+	EXPECT(cache.Get("k1") == nullptr && cache.Get("k5") == H(5));
 }
 
-static void TestLruEvictionOrder() {
-	KernelCache cache(2);
-	int a = 1, b = 2, c = 3;
-	cache.Insert("a", &a);
-	cache.Insert("b", &b);
-	CHECK(cache.Get("a") == &a); // a becomes most recently used
-	cache.Insert("c", &c);       // evicts b
-	CHECK(cache.Get("a") == &a);
-	CHECK(cache.Get("b") == nullptr);
-	CHECK(cache.Get("c") == &c);
-}
-
-static void TestEvictionAtExactCapacity() {
-	KernelCache cache(1);
-	int a = 1, b = 2;
-	cache.Insert("a", &a);
-	CHECK(cache.Get("a") == &a);
-	cache.Insert("b", &b);
-	CHECK(cache.Get("a") == nullptr);
-	CHECK(cache.Get("b") == &b);
-}
-
-static void TestHashSourceDeterministicAndKnownVectors() {
-	CHECK(KernelCache::HashSource("out[idx] = a + b;") == KernelCache::HashSource("out[idx] = a + b;"));
-	CHECK(KernelCache::HashSource("out[idx] = a + b;") != KernelCache::HashSource("out[idx] = a - b;"));
-	CHECK(KernelCache::HashSource("") == "e3b0c44298fc1c149afbf4c8996fb92427ae41e4649b934ca495991b7852b855");
-	CHECK(KernelCache::HashSource("abc") == "ba7816bf8f01cfea414140de5dae2223b00361a396177a9cb410ff61f20015ad");
-	// FIPS 180-4 two-block message.
-	CHECK(KernelCache::HashSource("abcdbcdecdefdefgefghfghighijhijkijkljklmklmnlmnomnopnopq") ==
-	      "248d6a61d20638b8e5c026930c3e6039a33ce45964ff2167f6ecedd419db06c1");
-	CHECK(KernelCache::HashSource("x").size() == 64);
-	CHECK(KernelCache::HashSource(std::string(10000, 'z')).size() == 64);
-}
-
-static void TestGetOnEmptyAndZeroCapacity() {
-	KernelCache empty(4);
-	CHECK(empty.Get("anything") == nullptr);
-	KernelCache zero(0);
-	int d = 1;
-	zero.Insert("a", &d);
-	zero.Insert("b", &d);
-	CHECK(zero.Get("a") == nullptr);
-	CHECK(zero.Get("b") == nullptr);
-}
-
-static std::vector<void *> g_deleted;
-static void RecordingDeleter(CompiledKernelHandle h) {
-	g_deleted.push_back(h);
-}
-
-static void TestDeleterOnEvictionReplacementDestruction() {
-	g_deleted.clear();
+VT_CASE(CacheReplacementReleasesTheOldHandleOnly) {
+	Deleted().clear();
 	{
-		KernelCache cache(1);
-		cache.SetHandleDeleter(&RecordingDeleter);
-		int a = 1, b = 2;
-		cache.Insert("a", &a);
-		cache.Insert("b", &b); // evicts a
-		CHECK(g_deleted.size() == 1 && g_deleted[0] == &a);
-		CHECK(cache.Get("b") == &b);
+		KernelCache cache(2);
+		cache.SetHandleDeleter(&CountingDeleter);
+		cache.SetHandleDeleter(&CountingDeleter); // the same deleter again is accepted
+		cache.Insert("a", H(10));
+		cache.Insert("a", H(10)); // same handle: nothing released
+		EXPECT(Deleted().empty());
+		cache.Insert("a", H(11));
+		EXPECT(Deleted() == std::vector<CompiledKernelHandle>{H(10)});
+		cache.Insert("b", H(20));
+		cache.Insert("c", H(30)); // evicts "a" (the oldest)
+		EXPECT(Deleted().size() == 2 && Deleted()[1] == H(11));
+		EXPECT_THROWS(cache.SetHandleDeleter(&OtherDeleter), "another deleter");
 	}
-	CHECK(g_deleted.size() == 2); // b freed by the destructor
-	g_deleted.clear();
-	KernelCache cache(4);
-	cache.SetHandleDeleter(&RecordingDeleter);
-	int a1 = 1, a2 = 2;
-	cache.Insert("a", &a1);
-	cache.Insert("a", &a2); // replacement frees a1
-	CHECK(g_deleted.size() == 1 && g_deleted[0] == &a1);
-	cache.Insert("a", &a2); // same handle again: nothing freed
-	CHECK(g_deleted.size() == 1);
-	CHECK(cache.Get("a") == &a2);
+	// destruction releases what is left
+	std::set<CompiledKernelHandle> released(Deleted().begin(), Deleted().end());
+	EXPECT(released.count(H(20)) == 1 && released.count(H(30)) == 1 && Deleted().size() == 4);
 }
 
-static void TestZeroCapacityDeletesDeclinedHandle() {
-	g_deleted.clear();
-	KernelCache cache(0);
-	cache.SetHandleDeleter(&RecordingDeleter);
-	int a = 1;
-	cache.Insert("a", &a);
-	CHECK(g_deleted.size() == 1 && g_deleted[0] == &a);
-	CHECK(cache.Get("a") == nullptr);
+VT_CASE(CacheWithoutCapacityDeclinesEveryHandle) {
+	Deleted().clear();
+	KernelCache owning(0);
+	owning.SetHandleDeleter(&CountingDeleter);
+	owning.Insert("x", H(7));
+	EXPECT(owning.Get("x") == nullptr);
+	EXPECT(Deleted() == std::vector<CompiledKernelHandle>{H(7)});
+	KernelCache borrowing(0); // no deleter: the caller keeps ownership, nothing is released
+	borrowing.Insert("y", H(8));
+	EXPECT(borrowing.Get("y") == nullptr && Deleted().size() == 1);
 }
 
-static void TestNoDeleterMeansNoOwnership() {
-	KernelCache cache(1);
-	int a = 1, b = 2;
-	cache.Insert("a", &a);
-	cache.Insert("b", &b);
-	CHECK(cache.Get("b") == &b);
-}
-
-static void TestSecondDifferentDeleterRefused() {
-	KernelCache cache(2);
-	cache.SetHandleDeleter(&RecordingDeleter);
-	cache.SetHandleDeleter(&RecordingDeleter); // same one again is fine
-	CHECK_THROWS(cache.SetHandleDeleter([](CompiledKernelHandle) {}));
+VT_CASE(HashSourceIsSha256Hex) {
+	// FIPS 180-4 example messages
+	EXPECT(KernelCache::HashSource("") == "e3b0c44298fc1c149afbf4c8996fb92427ae41e4649b934ca495991b7852b855");
+	EXPECT(KernelCache::HashSource("abc") == "ba7816bf8f01cfea414140de5dae2223b00361a396177a9cb410ff61f20015ad");
+	EXPECT(KernelCache::HashSource("abcdbcdecdefdefgefghfghighijhijkijkljklmklmnlmnomnopnopq") ==
+	       "248d6a61d20638b8e5c026930c3e6039a33ce45964ff2167f6ecedd419db06c1");
+	EXPECT(KernelCache::HashSource(std::string(1000000, 'a')) ==
+	       "cdc76e5c9914fb9281a1c7e284d73e67f1809a48a497200e046d39ccc7112cd0");
+	// lengths around the 55/56/64-byte padding boundaries give distinct, stable 64-digit keys
+	std::set<std::string> keys;
+	for (size_t n : {54, 55, 56, 57, 63, 64, 65, 119, 120}) {
+		const std::string k = KernelCache::HashSource(std::string(n, 'q'));
+		EXPECT(k.size() == 64 && k.find_first_not_of("0123456789abcdef") == std::string::npos);
+		EXPECT(k == KernelCache::HashSource(std::string(n, 'q')));
+		keys.insert(k);
+	}
+	EXPECT(keys.size() == 9);
 }
 
 // ---------------------------------------------------------------- GpuLogger
 
 namespace {
-std::filesystem::path LogDir() {
-	auto dir = std::filesystem::temp_directory_path() / ("vgpu_logger_test_" + std::to_string(::getpid()));
-	std::filesystem::create_directories(dir);
+
+fs::path FreshDir(const char *tag) {
+	fs::path dir = fs::temp_directory_path() / ("vgpu_log_" + std::string(tag) + "_" + std::to_string(::getpid()));
+	fs::remove_all(dir);
+	fs::create_directories(dir);
 	return dir;
 }
 
-std::vector<std::string> ReadLines(const std::filesystem::path &dir) {
-	std::vector<std::string> lines;
-	for (int i = 0; i < 2; i++) {
-		std::ifstream in(dir / ("gpu_log." + std::to_string(i) + ".jsonl"));
-		std::string line;
-		while (std::getline(in, line)) {
-			if (!line.empty()) {
-				lines.push_back(line);
-			}
+std::vector<std::string> Records(const fs::path &dir) {
+	std::vector<std::string> out;
+	for (const char *name : {"gpu_log.0.jsonl", "gpu_log.1.jsonl"}) {
+		std::ifstream in(dir / name);
+		for (std::string line; std::getline(in, line);) {
+			out.push_back(line);
 		}
 	}
-	return lines;
+	return out;
 }
 
-bool WholeJsonObject(const std::string &line) {
-	if (line.size() < 2 || line.front() != '{' || line.back() != '}') {
-		return false;
-	}
-	int depth = 0;
-	bool in_str = false, esc = false;
-	for (char c : line) {
-		if (in_str) {
-			if (esc) {
-				esc = false;
-			} else if (c == '\\') {
-				esc = true;
-			} else if (c == '"') {
-				in_str = false;
-			}
-			continue;
-		}
-		if (c == '"') {
-			in_str = true;
-		} else if (c == '{') {
-			depth++;
-		} else if (c == '}' && --depth < 0) {
-			return false;
-		}
-	}
-	return depth == 0 && !in_str;
+void LoggerOff(const fs::path &dir) {
+	GpuLogger::Instance().ConfigureForTesting(false, LogLevel::INFO, 1 << 20, dir.string());
+	fs::remove_all(dir);
 }
 
-void ClearLogs(const std::filesystem::path &dir) {
-	std::filesystem::remove(dir / "gpu_log.0.jsonl");
-	std::filesystem::remove(dir / "gpu_log.1.jsonl");
-}
 } // namespace
 
-static void TestLoggerDisabledWritesNothing() {
-	auto dir = LogDir();
-	ClearLogs(dir);
-	GpuLogger::Instance().ConfigureForTesting(false, LogLevel::DEBUG, 1 << 20, dir.string());
-	for (int i = 0; i < 100; i++) {
-		VGPU_LOG(LogLevel::ERROR, "should_not_appear", "");
+VT_CASE(LoggerWritesOneJsonObjectPerRecord) {
+	const fs::path dir = FreshDir("records");
+	auto &log = GpuLogger::Instance();
+	log.ConfigureForTesting(true, LogLevel::INFO, 1 << 20, dir.string());
+	EXPECT(log.Enabled(LogLevel::INFO) && !log.Enabled(LogLevel::DEBUG));
+	VGPU_LOG(LogLevel::INFO, "plan_done", "\"rows\":42,\"op\":" + GpuLogger::Quote("GROUP_BY"));
+	VGPU_LOG(LogLevel::ERROR, "plan_failed", std::string());
+	log.FlushForTesting();
+	const auto lines = Records(dir);
+	EXPECT(lines.size() == 2);
+	if (lines.size() == 2) {
+		EXPECT(lines[0].front() == '{' && lines[0].back() == '}');
+		EXPECT(lines[0].find("\"lvl\":\"info\"") != std::string::npos);
+		EXPECT(lines[0].find("\"evt\":\"plan_done\"") != std::string::npos);
+		EXPECT(lines[0].find("\"rows\":42,\"op\":\"GROUP_BY\"}") != std::string::npos);
+		EXPECT(lines[0].find("\"ts_us\":") != std::string::npos && lines[0].find("\"thread\":") != std::string::npos);
+		EXPECT(lines[1].find("\"lvl\":\"error\"") != std::string::npos && lines[1].find("plan_failed") != std::string::npos);
 	}
-	GpuLogger::Instance().FlushForTesting();
-	CHECK(ReadLines(dir).empty());
+	LoggerOff(dir);
+	EXPECT(!log.Enabled(LogLevel::ERROR));
 }
 
-static void TestLoggerRecordsRoundTripAsJsonLines() {
-	auto dir = LogDir();
-	ClearLogs(dir);
-	GpuLogger::Instance().ConfigureForTesting(true, LogLevel::DEBUG, 1 << 20, dir.string());
-	VGPU_LOG(LogLevel::INFO, "h2d", "\"column\":\"revenue\",\"bytes\":4096,\"us\":12");
-	VGPU_LOG(LogLevel::INFO, "decline", std::string("\"reason\":") + GpuLogger::Quote("ORDER BY is not offloadable"));
-	GpuLogger::Instance().FlushForTesting();
-	auto lines = ReadLines(dir);
-	CHECK(lines.size() == 2);
-	for (auto &l : lines) {
-		CHECK(WholeJsonObject(l));
-	}
-	CHECK(!lines.empty() && lines[0].find("\"evt\":\"h2d\"") != std::string::npos);
-	CHECK(!lines.empty() && lines[0].find("\"ts_us\":") != std::string::npos);
-	CHECK(!lines.empty() && lines[0].find("\"thread\":") != std::string::npos);
-	CHECK(lines.size() > 1 && lines[1].find("ORDER BY is not offloadable") != std::string::npos);
+VT_CASE(LoggerDropsRecordsAboveItsLevelAndWhenOff) {
+	const fs::path dir = FreshDir("levels");
+	auto &log = GpuLogger::Instance();
+	log.ConfigureForTesting(true, LogLevel::ERROR, 1 << 20, dir.string());
+	int evaluated = 0;
+	auto fields = [&] {
+		evaluated++;
+		return std::string("\"x\":1");
+	};
+	VGPU_LOG(LogLevel::DEBUG, "d", fields());
+	VGPU_LOG(LogLevel::INFO, "i", fields());
+	VGPU_LOG(LogLevel::ERROR, "e", fields());
+	EXPECT(evaluated == 1); // fields of a filtered record are never built
+	EXPECT(Records(dir).size() == 1);
+	log.ConfigureForTesting(false, LogLevel::DEBUG, 1 << 20, dir.string());
+	VGPU_LOG(LogLevel::ERROR, "off", fields());
+	EXPECT(evaluated == 1);
+	LoggerOff(dir);
 }
 
-static void TestLoggerLevelFiltering() {
-	auto dir = LogDir();
-	ClearLogs(dir);
-	GpuLogger::Instance().ConfigureForTesting(true, LogLevel::ERROR, 1 << 20, dir.string());
-	VGPU_LOG(LogLevel::ERROR, "kept", "");
-	VGPU_LOG(LogLevel::INFO, "dropped", "");
-	VGPU_LOG(LogLevel::DEBUG, "dropped", "");
-	GpuLogger::Instance().FlushForTesting();
-	auto lines = ReadLines(dir);
-	CHECK(lines.size() == 1);
-	CHECK(!lines.empty() && lines[0].find("\"evt\":\"kept\"") != std::string::npos);
+VT_CASE(QuoteEscapesJsonSpecialCharacters) {
+	EXPECT(GpuLogger::Quote("plain") == "\"plain\"");
+	EXPECT(GpuLogger::Quote("a\"b\\c") == "\"a\\\"b\\\\c\"");
+	EXPECT(GpuLogger::Quote("t\tn\nr\r") == "\"t\\tn\\nr\\r\"");
+	EXPECT(GpuLogger::Quote(std::string("\x01\x1f", 2)) == "\"\\u0001\\u001f\"");
+	EXPECT(GpuLogger::Quote(std::string("nul\0x", 5)) == "\"nul\\u0000x\"");
+	EXPECT(GpuLogger::Quote("caf\xc3\xa9") == "\"caf\xc3\xa9\""); // UTF-8 passes through
 }
 
-static void TestLoggerQuoteEscapes() {
-	CHECK(GpuLogger::Quote("a\"b\\c\nd") == "\"a\\\"b\\\\c\\nd\"");
-}
-
-static void TestLoggerConcurrentWritersProduceWholeLines() {
-	auto dir = LogDir();
-	ClearLogs(dir);
-	GpuLogger::Instance().ConfigureForTesting(true, LogLevel::DEBUG, 32u << 20, dir.string());
-	const int threads = 8, per_thread = 500;
-	std::vector<std::thread> workers;
-	for (int t = 0; t < threads; t++) {
-		workers.emplace_back([t] {
-			for (int i = 0; i < per_thread; i++) {
-				VGPU_LOG(LogLevel::INFO, "launch",
-				         "\"worker\":" + std::to_string(t) + ",\"i\":" + std::to_string(i) +
-				             ",\"pad\":\"xxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxx\"");
+VT_CASE(LoggerRecordsFromManyThreadsStayWhole) {
+	const fs::path dir = FreshDir("threads");
+	auto &log = GpuLogger::Instance();
+	log.ConfigureForTesting(true, LogLevel::DEBUG, 64 << 20, dir.string());
+	constexpr int kThreads = 6, kEach = 400;
+	std::vector<std::thread> writers;
+	for (int t = 0; t < kThreads; t++) {
+		writers.emplace_back([t] {
+			const std::string pad(100 + 37 * t, 'a' + t);
+			for (int i = 0; i < kEach; i++) {
+				VGPU_LOG(LogLevel::DEBUG, "w", "\"t\":" + std::to_string(t) + ",\"i\":" + std::to_string(i) +
+				                                   ",\"pad\":\"" + pad + "\"");
 			}
 		});
 	}
-	for (auto &w : workers) {
+	for (auto &w : writers) {
 		w.join();
 	}
-	GpuLogger::Instance().FlushForTesting();
-	auto lines = ReadLines(dir);
-	CHECK(lines.size() == static_cast<size_t>(threads * per_thread));
-	bool whole = !lines.empty();
-	for (auto &l : lines) {
-		whole = whole && WholeJsonObject(l);
+	const auto lines = Records(dir);
+	EXPECT(lines.size() == static_cast<size_t>(kThreads * kEach));
+	std::map<int, int> per_thread;
+	bool whole = true;
+	for (const auto &l : lines) {
+		const auto tpos = l.find("\"t\":");
+		if (tpos == std::string::npos || l.back() != '}') {
+			whole = false;
+			continue;
+		}
+		const int t = std::stoi(l.substr(tpos + 4));
+		per_thread[t]++;
+		whole = whole && l.find(std::string(100 + 37 * t, 'a' + t) + "\"}") != std::string::npos;
 	}
-	CHECK(whole);
+	EXPECT(whole);
+	EXPECT(per_thread.size() == static_cast<size_t>(kThreads));
+	for (const auto &kv : per_thread) {
+		EXPECT(kv.second == kEach);
+	}
+	LoggerOff(dir);
 }
 
-static void TestLoggerCapEvictsOldestKeepsNewest() {
-	auto dir = LogDir();
-	ClearLogs(dir);
-	const size_t cap = 256 * 1024;
-	GpuLogger::Instance().ConfigureForTesting(true, LogLevel::DEBUG, cap, dir.string());
-	const int records = 4000;
+VT_CASE(LoggerStaysUnderItsCapAndKeepsTheNewestRecords) {
+	const fs::path dir = FreshDir("cap");
+	auto &log = GpuLogger::Instance();
+	const size_t cap = 48 * 1024;
+	log.ConfigureForTesting(true, LogLevel::INFO, cap, dir.string());
+	const int records = 3000; // about ten times the cap
 	for (int i = 0; i < records; i++) {
-		VGPU_LOG(LogLevel::INFO, "seq", "\"n\":" + std::to_string(i) + ",\"pad\":\"yyyyyyyyyyyyyyyyyyyyyyyyyyyyyyyyyyyyyyyy\"");
+		VGPU_LOG(LogLevel::INFO, "rec", "\"seq\":" + std::to_string(i) + ",\"filler\":\"0123456789abcdef\"");
 	}
-	GpuLogger::Instance().FlushForTesting();
-	const size_t on_disk = GpuLogger::Instance().OnDiskBytesForTesting();
-	CHECK(on_disk <= cap && on_disk > 0);
-	auto lines = ReadLines(dir);
-	bool newest = false, oldest = false;
-	for (auto &l : lines) {
-		newest = newest || l.find("\"n\":" + std::to_string(records - 1) + ",") != std::string::npos;
-		oldest = oldest || l.find("\"n\":0,") != std::string::npos;
+	EXPECT(log.OnDiskBytesForTesting() <= cap);
+	EXPECT(log.OnDiskBytesForTesting() > cap / 4);
+	std::set<int> seen;
+	for (const auto &l : Records(dir)) {
+		seen.insert(std::stoi(l.substr(l.find("\"seq\":") + 6)));
 	}
-	CHECK(newest);
-	CHECK(!oldest);
-	CHECK(lines.size() > 100);
-	GpuLogger::Instance().ConfigureForTesting(false, LogLevel::INFO, 1 << 20, dir.string());
-	std::filesystem::remove_all(dir);
+	EXPECT(seen.count(records - 1) == 1); // the newest survived
+	EXPECT(seen.count(0) == 0);           // the oldest was rotated out
+	EXPECT(!seen.empty() && *seen.rbegin() - *seen.begin() + 1 == static_cast<int>(seen.size())); // no gaps
+	LoggerOff(dir);
 }
 
-// ---------------------------------------------------------------- routing through the C++ header
+// ---------------------------------------------------------------- routing (gpu_engine.hpp)
 
-static void TestRoutingThroughHeader() {
-	auto scan = std::make_shared<GpuPlanNode>();
-	scan->op_type = GpuOpType::SCAN;
-	scan->table = {"memory", "main", "bench"};
-	scan->column_names = {"a", "b"};
-	scan->estimated_rows = 20000000;
-	scan->estimated_row_width_bytes = 16;
-	GpuPlanNode filter;
-	filter.op_type = GpuOpType::FILTER;
+namespace {
+
+std::shared_ptr<GpuPlanNode> Scan(uint64_t rows, uint64_t width, std::vector<std::string> cols) {
+	auto s = std::make_shared<GpuPlanNode>();
+	s->op_type = GpuOpType::SCAN;
+	s->table = {"memory", "main", "t"};
+	s->column_names = std::move(cols);
+	s->estimated_rows = rows;
+	s->estimated_row_width_bytes = width;
+	return s;
+}
+
+GpuPlanNode Projection(std::shared_ptr<GpuPlanNode> child, double ops_per_row, uint64_t rows) {
+	GpuPlanNode p;
+	p.op_type = GpuOpType::PROJECTION;
 	GpuExpr e;
-	e.generated_cuda_source = "((bool*)out)[idx] = true;";
-	e.estimated_ops_per_row = 2.0;
-	filter.expressions.push_back(e);
-	filter.children.push_back(scan);
-	filter.estimated_rows = 200000;
-	filter.estimated_row_width_bytes = 16;
-	auto shape = AnalyzePlanShape(filter);
-	CHECK(shape.total_ops == 2.0 * 20000000.0);
-	CHECK(shape.ops_per_scanned_row == 2.0);
-	CHECK(shape.density == GpuWorkDensity::INSUFFICIENT);
-	CHECK(IsRowIndependent(filter));
-	CHECK(EstimateColumnsSize(filter) == 20000000ull * 16);
-	CHECK(DetermineExecutionMode(filter, 100, 0.75) == ExecutionMode::PIPELINE_STREAM_UNCACHED);
-	CHECK(EstimateWorkingSet(filter, 1 << 21).chunk_capped);
+	e.generated_cuda_source = "((long long*)out)[idx] = ((const long long*)inputs[0])[row] + 1LL;";
+	e.estimated_ops_per_row = ops_per_row;
+	p.expressions.push_back(e);
+	p.children.push_back(std::move(child));
+	p.estimated_rows = rows;
+	p.estimated_row_width_bytes = 8;
+	return p;
+}
+
+} // namespace
+
+VT_CASE(DensityClassesFollowTheExportedThresholds) {
+	EXPECT(kMinOpsPerScannedRow < kOffloadOpsPerRow);
+	const uint64_t rows = 1 << 20;
+	struct Probe {
+		double ops;
+		GpuWorkDensity want;
+	} probes[] = {{kMinOpsPerScannedRow * 0.5, GpuWorkDensity::INSUFFICIENT},
+	              {kMinOpsPerScannedRow, GpuWorkDensity::INCONCLUSIVE},
+	              {(kMinOpsPerScannedRow + kOffloadOpsPerRow) / 2, GpuWorkDensity::INCONCLUSIVE},
+	              {kOffloadOpsPerRow, GpuWorkDensity::SUFFICIENT},
+	              {kOffloadOpsPerRow * 4, GpuWorkDensity::SUFFICIENT}};
+	for (const auto &p : probes) {
+		const GpuPlanShape shape = AnalyzePlanShape(Projection(Scan(rows, 8, {"a"}), p.ops, rows));
+		EXPECT(shape.scanned_rows == rows && shape.scanned_bytes == rows * 8);
+		EXPECT(shape.ops_per_scanned_row == p.ops);
+		EXPECT(shape.density == p.want);
+	}
+	// no ops at all: nothing to judge
+	GpuPlanNode bare = *Scan(rows, 8, {"a"});
+	EXPECT(AnalyzePlanShape(bare).density == GpuWorkDensity::INCONCLUSIVE);
+}
+
+VT_CASE(RowIndependenceAndColumnSizes) {
+	auto scan = Scan(3'000'000, 12, {"k", "v"});
+	GpuPlanNode proj = Projection(scan, 2.0, 3'000'000);
+	EXPECT(IsRowIndependent(proj));
+	EXPECT(EstimateColumnsSize(proj) == 3'000'000ull * 12);
+	for (GpuOpType op : {GpuOpType::GROUP_BY_AGGREGATE, GpuOpType::HASH_JOIN, GpuOpType::CROSS_PRODUCT}) {
+		GpuPlanNode top;
+		top.op_type = op;
+		top.children.push_back(std::make_shared<GpuPlanNode>(proj));
+		EXPECT(!IsRowIndependent(top));
+	}
+	// a small budget sends a large row-independent plan to the streamed path; a roomy one keeps it cached
+	EXPECT(DetermineExecutionMode(proj, 1024, 0.75) == ExecutionMode::PIPELINE_STREAM_UNCACHED);
+	EXPECT(DetermineExecutionMode(proj, 1ull << 40, 0.75) != ExecutionMode::PIPELINE_STREAM_UNCACHED);
+	// row-independent row counts are capped at the chunk size
+	const GpuWorkingSetEstimate capped = EstimateWorkingSet(proj, 1 << 16);
+	const GpuWorkingSetEstimate whole = EstimateWorkingSet(proj, 0);
+	EXPECT(capped.chunk_capped && !whole.chunk_capped);
+	EXPECT(capped.total_bytes < whole.total_bytes && whole.total_bytes >= 3'000'000ull * 12);
+}
+
+VT_CASE(FastMathSwitchIsProcessWide) {
+	const bool before = IsGpuFastMathEnabled();
+	SetGpuFastMathEnabled(true);
+	EXPECT(IsGpuFastMathEnabled());
+	bool seen_on_other_thread = false;
+	std::thread([&] { seen_on_other_thread = IsGpuFastMathEnabled(); }).join();
+	EXPECT(seen_on_other_thread);
+	SetGpuFastMathEnabled(false);
+	EXPECT(!IsGpuFastMathEnabled());
+	SetGpuFastMathEnabled(before);
 }
 
 // ---------------------------------------------------------------- host-copy pool
 
-static void TestParallelHostCopySizesAndConcurrency() {
-	// sizes around the parallel threshold and the 4 KiB part rounding, odd tails
-	const size_t sizes[] = {0, 1, (2u << 20) - 1, 2u << 20, (16u << 20) + 7, (64u << 20) + 4095};
-	std::vector<uint8_t> src((64u << 20) + 4096), dst(src.size());
+VT_CASE(ParallelHostCopyAtOddSizesWritesExactlyTheRange) {
+	const size_t sizes[] = {0, 3, 4095, (1u << 20) + 1, (2u << 20) - 3, (24u << 20) + 11, (48u << 20) + 4097};
+	std::vector<unsigned char> src((48u << 20) + 8192), dst(src.size());
 	for (size_t i = 0; i < src.size(); i++) {
-		src[i] = (uint8_t)(i * 2654435761u >> 13);
+		src[i] = static_cast<unsigned char>((i * 0x9E3779B1u) >> 11);
 	}
 	for (size_t n : sizes) {
-		std::fill(dst.begin(), dst.end(), 0xA5);
-		vgpu::ParallelHostCopy(dst.data(), src.data(), n);
-		CHECK(std::equal(dst.begin(), dst.begin() + n, src.begin()));
-		CHECK(dst[n] == 0xA5); // nothing written past the end
-	}
-	// several callers at once (the pool serves them one after another)
-	std::vector<std::vector<uint8_t>> outs(4, std::vector<uint8_t>((8u << 20) + 123));
-	std::vector<std::thread> callers;
-	for (int t = 0; t < 4; t++) {
-		callers.emplace_back([&, t] {
-			for (int r = 0; r < 5; r++) {
-				vgpu::ParallelHostCopy(outs[t].data(), src.data() + t * 1000, outs[t].size());
-			}
-		});
-	}
-	for (auto &c : callers) {
-		c.join();
-	}
-	for (int t = 0; t < 4; t++) {
-		CHECK(std::equal(outs[t].begin(), outs[t].end(), src.begin() + t * 1000));
+		std::fill(dst.begin(), dst.end(), 0x5C);
+		vgpu::ParallelHostCopy(dst.data() + 1, src.data() + 3, n); // unaligned both sides
+		EXPECT(dst[0] == 0x5C && dst[n + 1] == 0x5C);
+		EXPECT(std::equal(dst.begin() + 1, dst.begin() + 1 + n, src.begin() + 3));
 	}
 }
 
-int main() {
-	RUN_TEST(TestMissThenHit);
-	RUN_TEST(TestOverwriteSameKey);
-	RUN_TEST(TestLruEvictionOrder);
-	RUN_TEST(TestEvictionAtExactCapacity);
-	RUN_TEST(TestHashSourceDeterministicAndKnownVectors);
-	RUN_TEST(TestGetOnEmptyAndZeroCapacity);
-	RUN_TEST(TestDeleterOnEvictionReplacementDestruction);
-	RUN_TEST(TestZeroCapacityDeletesDeclinedHandle);
-	RUN_TEST(TestNoDeleterMeansNoOwnership);
-	RUN_TEST(TestSecondDifferentDeleterRefused);
-	RUN_TEST(TestLoggerDisabledWritesNothing);
-	RUN_TEST(TestLoggerRecordsRoundTripAsJsonLines);
-	RUN_TEST(TestLoggerLevelFiltering);
-	RUN_TEST(TestLoggerQuoteEscapes);
-	RUN_TEST(TestLoggerConcurrentWritersProduceWholeLines);
-	RUN_TEST(TestLoggerCapEvictsOldestKeepsNewest);
-	RUN_TEST(TestRoutingThroughHeader);
-	RUN_TEST(TestParallelHostCopySizesAndConcurrency);
-	TEST_MAIN_EPILOGUE();
+VT_CASE(ParallelHostCopyServesConcurrentCallers) {
+	std::vector<unsigned char> src(32u << 20);
+	for (size_t i = 0; i < src.size(); i++) {
+		src[i] = static_cast<unsigned char>(i * 131 + 7);
+	}
+	constexpr int kCallers = 5;
+	std::vector<std::vector<unsigned char>> dsts(kCallers, std::vector<unsigned char>((6u << 20) + 333));
+	std::atomic<int> done {0};
+	std::vector<std::thread> callers;
+	for (int c = 0; c < kCallers; c++) {
+		callers.emplace_back([&, c] {
+			for (int r = 0; r < 4; r++) {
+				vgpu::ParallelHostCopy(dsts[c].data(), src.data() + 4097 * c, dsts[c].size());
+			}
+			done++;
+		});
+	}
+	for (auto &t : callers) {
+		t.join();
+	}
+	EXPECT(done == kCallers);
+	for (int c = 0; c < kCallers; c++) {
+		EXPECT(std::equal(dsts[c].begin(), dsts[c].end(), src.begin() + 4097 * c));
+	}
 }
+
+VT_MAIN()

@@ -1,40 +1,48 @@
-// test_stream_pipeline.cpp -- GpuStreamPipeline through its C++ header, with the cases of
-// cuda_engine/tests/test_gpu_stream_pipeline.cpp (bit-exact doubles, FIFO order, overlap, chained levels,
-// NULL propagation, early input-slot release, bad batches) plus RUNNING_SUM carried across batches and a
-// level wider than one generated kernel. Needs a GPU.
-#include "check.hpp"
+// test_stream_pipeline.cpp -- GpuStreamPipeline through its C++ header: which plans it takes, schema and batch
+// checks, a stream of ragged batches coming back bit-exact in submission order with several batches in flight,
+// chained projection levels, NULL bitmaps, the input slot going back to the ring before results are read,
+// RUNNING_SUM carried from batch to batch, and a level wider than one generated kernel. Needs a GPU.
+#include "harness.hpp"
+
 #include "gpu_memory_pool.hpp"
 #include "gpu_stream_pipeline.hpp"
 
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace vector_gpu;
 
 namespace {
 
-constexpr uint64_t kMaxRows = 8192;
+constexpr uint64_t kMaxRows = 1 << 14;
 
-std::shared_ptr<GpuPlanNode> Scan(const std::vector<std::string> &cols) {
+std::shared_ptr<GpuPlanNode> Scan(std::vector<std::string> cols) {
 	auto s = std::make_shared<GpuPlanNode>();
 	s->op_type = GpuOpType::SCAN;
-	s->table = {"memory", "main", "t"};
-	s->column_names = cols;
+	s->table = {"memory", "main", "stream"};
+	s->column_names = std::move(cols);
 	return s;
 }
 
-GpuExpr Expr(const std::string &body, const std::vector<std::string> &inputs, GpuValueType t) {
+GpuExpr E(std::string body, std::vector<std::string> inputs, GpuValueType type) {
 	GpuExpr e;
-	e.generated_cuda_source = body;
-	e.input_columns = inputs;
-	e.output_type = t;
+	e.generated_cuda_source = std::move(body);
+	e.input_columns = std::move(inputs);
+	e.output_type = type;
 	return e;
 }
 
-std::shared_ptr<GpuPlanNode> Projection(std::shared_ptr<GpuPlanNode> child, std::vector<GpuExpr> exprs) {
+std::shared_ptr<GpuPlanNode> Project(std::shared_ptr<GpuPlanNode> child, std::vector<GpuExpr> exprs) {
 	auto p = std::make_shared<GpuPlanNode>();
 	p->op_type = GpuOpType::PROJECTION;
 	p->children.push_back(std::move(child));
@@ -42,311 +50,299 @@ std::shared_ptr<GpuPlanNode> Projection(std::shared_ptr<GpuPlanNode> child, std:
 	return p;
 }
 
-std::shared_ptr<GpuPlanNode> SumDiffPlan() {
-	return Projection(
-	    Scan({"a", "b"}),
-	    {Expr("((double*)out)[idx] = ((const double*)inputs[0])[row] * 2.0 + ((const double*)inputs[1])[row];",
-	          {"a", "b"}, GpuValueType::FLOAT64),
-	     Expr("((double*)out)[idx] = ((const double*)inputs[0])[row] - ((const double*)inputs[1])[row];", {"a", "b"},
-	          GpuValueType::FLOAT64)});
+// inputs: x INT64, y DOUBLE
+const std::vector<std::string> kNames = {"x", "y"};
+const std::vector<GpuValueType> kTypes = {GpuValueType::INT64, GpuValueType::FLOAT64};
+
+// SELECT x * 3 - 7, y / 4.0 + x
+std::shared_ptr<GpuPlanNode> TwoOutputPlan() {
+	return Project(Scan(kNames),
+	               {E("((long long*)out)[idx] = ((const long long*)inputs[0])[row] * 3LL - 7LL;", {"x"},
+	                  GpuValueType::INT64),
+	                E("((double*)out)[idx] = ((const double*)inputs[0])[row] / 4.0 + "
+	                  "((double)(((const long long*)inputs[1])[row]));",
+	                  {"y", "x"}, GpuValueType::FLOAT64)});
 }
 
-std::shared_ptr<GpuPlanNode> ChainedPlan() {
-	auto inner = Projection(
-	    Scan({"a", "b"}),
-	    {Expr("((double*)out)[idx] = ((const double*)inputs[0])[row] + ((const double*)inputs[1])[row];", {"a", "b"},
-	          GpuValueType::FLOAT64),
-	     Expr("((double*)out)[idx] = ((const double*)inputs[0])[row] - ((const double*)inputs[1])[row];", {"a", "b"},
-	          GpuValueType::FLOAT64)});
-	return Projection(inner, {Expr("((double*)out)[idx] = ((const double*)inputs[0])[row] * 3.0;", {"col0"},
-	                               GpuValueType::FLOAT64),
-	                          Expr("((double*)out)[idx] = ((const double*)inputs[0])[row];", {"col1"},
-	                               GpuValueType::FLOAT64)});
-}
+//! Fills one batch in a ring slot: x, then y, then (optionally) the bitmap of y.
+struct Rows {
+	std::vector<int64_t> x;
+	std::vector<double> y;
+	std::vector<uint8_t> y_valid; // Arrow bitmap, empty = no NULLs
+};
 
-GpuStreamBatch Batch(GpuMemoryPoolSlot slot, const std::vector<double> &a, const std::vector<double> &b,
-                     const std::vector<uint8_t> *a_valid = nullptr) {
-	GpuStreamBatch batch;
-	batch.slot = std::move(slot);
-	batch.row_count = a.size();
-	auto *base = static_cast<uint8_t *>(batch.slot.data());
-	size_t off = 0;
-	const size_t bytes = a.size() * sizeof(double);
-	std::memcpy(base + off, a.data(), bytes);
-	GpuColumn ca;
-	ca.name = "a";
-	ca.type = GpuValueType::FLOAT64;
-	ca.data = base + off;
-	ca.row_count = a.size();
-	off += bytes;
-	std::memcpy(base + off, b.data(), bytes);
-	GpuColumn cb;
-	cb.name = "b";
-	cb.type = GpuValueType::FLOAT64;
-	cb.data = base + off;
-	cb.row_count = b.size();
-	off += bytes;
-	if (a_valid != nullptr) {
-		std::memcpy(base + off, a_valid->data(), a_valid->size());
-		ca.validity = base + off;
+GpuStreamBatch Pack(const Rows &r) {
+	GpuStreamBatch b;
+	b.slot = GpuMemoryPool::Instance().AcquireNext();
+	b.row_count = r.x.size();
+	auto *p = static_cast<uint8_t *>(b.slot.data());
+	GpuColumn cx, cy;
+	cx.name = "x";
+	cx.type = GpuValueType::INT64;
+	cx.row_count = r.x.size();
+	cx.data = p;
+	std::memcpy(p, r.x.data(), r.x.size() * 8);
+	p += r.x.size() * 8;
+	cy.name = "y";
+	cy.type = GpuValueType::FLOAT64;
+	cy.row_count = r.y.size();
+	cy.data = p;
+	std::memcpy(p, r.y.data(), r.y.size() * 8);
+	p += r.y.size() * 8;
+	if (!r.y_valid.empty()) {
+		std::memcpy(p, r.y_valid.data(), r.y_valid.size());
+		cy.validity = p;
 	}
-	batch.columns.push_back(ca);
-	batch.columns.push_back(cb);
-	return batch;
+	b.columns = {cx, cy};
+	return b;
 }
 
-std::vector<double> Doubles(const GpuColumn &c) {
-	std::vector<double> v(c.row_count);
-	std::memcpy(v.data(), c.data, c.row_count * sizeof(double));
-	return v;
+Rows Make(uint64_t n, uint64_t seed) {
+	Rows r;
+	r.x.resize(n);
+	r.y.resize(n);
+	for (uint64_t i = 0; i < n; i++) {
+		const uint64_t h = (i + 1) * 0x9E3779B97F4A7C15ull ^ seed;
+		r.x[i] = static_cast<int64_t>(h >> 20) - (1ll << 43);
+		r.y[i] = static_cast<double>(static_cast<int64_t>(h % 2000001) - 1000000) / 64.0;
+	}
+	return r;
 }
 
-bool ValidAt(const GpuColumn &c, uint64_t r) {
-	return c.validity == nullptr || (c.validity[r / 8] >> (r % 8)) & 1;
+template <typename T>
+const T *As(const GpuColumn &c) {
+	return static_cast<const T *>(c.data);
+}
+
+bool IsValid(const GpuColumn &c, uint64_t row) {
+	return c.validity == nullptr || ((c.validity[row >> 3] >> (row & 7)) & 1) != 0;
 }
 
 } // namespace
 
-static void TestSupportsOnlyProjectionChainsOverScan() {
-	CHECK(GpuStreamPipeline::Supports(*SumDiffPlan()));
-	CHECK(GpuStreamPipeline::Supports(*ChainedPlan()));
-	CHECK(!GpuStreamPipeline::Supports(*Scan({"a"})));
-	auto filter = std::make_shared<GpuPlanNode>();
-	filter->op_type = GpuOpType::FILTER;
-	filter->children.push_back(Scan({"a"}));
-	filter->expressions.push_back(Expr("((bool*)out)[idx] = ((const double*)inputs[0])[row] > 0.0;", {"a"},
-	                                   GpuValueType::BOOLEAN));
-	CHECK(!GpuStreamPipeline::Supports(*filter));
-	CHECK(!GpuStreamPipeline::Supports(*Projection(filter, {Expr("((double*)out)[idx] = ((const double*)inputs[0])[row];",
-	                                                             {"a"}, GpuValueType::FLOAT64)})));
-	CHECK(!GpuStreamPipeline::Supports(*Projection(Scan({"a"}), {Expr("((double*)out)[idx] = 0.0;", {"missing"},
-	                                                                   GpuValueType::FLOAT64)})));
+VT_CASE(SupportsOnlyProjectionChainsOverOneScan) {
+	EXPECT(GpuStreamPipeline::Supports(*TwoOutputPlan()));
+	EXPECT(GpuStreamPipeline::Supports(*Project(TwoOutputPlan(), {E("((long long*)out)[idx] = "
+	                                                                "((const long long*)inputs[0])[row];",
+	                                                                {"col0"}, GpuValueType::INT64)})));
+	EXPECT(!GpuStreamPipeline::Supports(*Scan(kNames))); // nothing to compute
+	auto where = std::make_shared<GpuPlanNode>();
+	where->op_type = GpuOpType::FILTER;
+	where->children.push_back(Scan(kNames));
+	where->expressions.push_back(E("((bool*)out)[idx] = ((const long long*)inputs[0])[row] > 0LL;", {"x"},
+	                               GpuValueType::BOOLEAN));
+	EXPECT(!GpuStreamPipeline::Supports(*where)); // a filter changes row counts
+	auto grouped = std::make_shared<GpuPlanNode>();
+	grouped->op_type = GpuOpType::GROUP_BY_AGGREGATE;
+	grouped->children.push_back(Scan(kNames));
+	EXPECT(!GpuStreamPipeline::Supports(*Project(grouped, {E("((long long*)out)[idx] = 1LL;", {}, GpuValueType::INT64)})));
+	EXPECT(!GpuStreamPipeline::Supports(
+	    *Project(Scan(kNames), {E("((double*)out)[idx] = ((const double*)inputs[0])[row];", {"nope"},
+	                              GpuValueType::FLOAT64)})));
 }
 
-static void TestConstructorRejectsMismatchedSchema() {
-	auto plan = SumDiffPlan();
-	CHECK_THROWS(GpuStreamPipeline(*plan, {"a"}, {GpuValueType::FLOAT64}, kMaxRows));
-	CHECK_THROWS(GpuStreamPipeline(*plan, {"a", "b"}, {GpuValueType::FLOAT64}, kMaxRows));
-	CHECK_THROWS(GpuStreamPipeline(*plan, {"a", "b"}, {GpuValueType::FLOAT64, GpuValueType::FLOAT64}, 0));
+VT_CASE(ConstructorChecksTheInputSchema) {
+	auto plan = TwoOutputPlan();
+	EXPECT_THROWS(GpuStreamPipeline(*plan, {"x"}, {GpuValueType::INT64}, kMaxRows), "");
+	EXPECT_THROWS(GpuStreamPipeline(*plan, kNames, {GpuValueType::INT64}, kMaxRows), "");
+	EXPECT_THROWS(GpuStreamPipeline(*plan, kNames, kTypes, 0), "");
+	GpuStreamPipeline ok(*plan, kNames, kTypes, kMaxRows);
+	EXPECT(ok.output_types() == (std::vector<GpuValueType> {GpuValueType::INT64, GpuValueType::FLOAT64}));
+	EXPECT(ok.in_flight() == 0);
 }
 
-static void TestStreamsManyBatchesInOrderAndOverlaps() {
+VT_CASE(RaggedBatchesComeBackExactAndInOrder) {
 	GpuMemoryPool::Instance().ResetCursorForTesting();
-	GpuStreamPipeline pipeline(*SumDiffPlan(), {"a", "b"}, {GpuValueType::FLOAT64, GpuValueType::FLOAT64}, kMaxRows);
-	CHECK(pipeline.output_types().size() == 2);
-	const int batches = 9;
-	const uint64_t rows = 5000;
-	std::vector<double> want_sum, want_diff, got_sum, got_diff;
-	int submitted = 0;
-	size_t max_in_flight = 0;
-	GpuStreamResult result;
-	auto producer = GpuMemoryPool::Instance().AcquireNext();
-	while (submitted < batches || pipeline.in_flight() > 0) {
-		if (submitted < batches && pipeline.CanSubmit()) {
-			std::vector<double> a(rows), b(rows);
-			for (uint64_t i = 0; i < rows; i++) {
-				a[i] = submitted * 1000.0 + i * 0.5;
-				b[i] = i * 0.25 - 3.0;
-				want_sum.push_back(a[i] * 2.0 + b[i]);
-				want_diff.push_back(a[i] - b[i]);
-			}
-			pipeline.Submit(Batch(std::move(producer), a, b));
-			producer = GpuMemoryPool::Instance().AcquireNext();
-			submitted++;
-			max_in_flight = std::max(max_in_flight, pipeline.in_flight());
+	GpuStreamPipeline pipe(*TwoOutputPlan(), kNames, kTypes, kMaxRows);
+	const uint64_t sizes[] = {kMaxRows, 1, 4097, 12345, 64, kMaxRows - 1, 999, 2048, 7, 16000, 3};
+	const int total = sizeof(sizes) / sizeof(sizes[0]);
+	int next = 0, taken = 0;
+	size_t deepest = 0;
+	bool exact = true, named = true;
+	while (taken < total) {
+		if (next < total && pipe.CanSubmit()) {
+			pipe.Submit(Pack(Make(sizes[next], next)));
+			next++;
+			deepest = std::max(deepest, pipe.in_flight());
 			continue;
 		}
-		if (!pipeline.TryTake(result) && !pipeline.TakeBlocking(result)) {
-			break;
+		GpuStreamResult r;
+		if (!pipe.TryTake(r)) {
+			EXPECT(pipe.TakeBlocking(r));
 		}
-		CHECK(result.columns().size() == 2 && result.columns()[0].name == "col0" && result.columns()[1].name == "col1");
-		auto s = Doubles(result.columns()[0]);
-		auto d = Doubles(result.columns()[1]);
-		got_sum.insert(got_sum.end(), s.begin(), s.end());
-		got_diff.insert(got_diff.end(), d.begin(), d.end());
-		result.Reset();
+		const Rows want = Make(sizes[taken], taken);
+		exact = exact && r.row_count() == sizes[taken] && r.columns().size() == 2;
+		named = named && r.columns()[0].name == "col0" && r.columns()[1].name == "col1";
+		for (uint64_t i = 0; exact && i < r.row_count(); i++) {
+			volatile double y = want.y[i];
+			const double q = y / 4.0;
+			exact = As<int64_t>(r.columns()[0])[i] == want.x[i] * 3 - 7 &&
+			        As<double>(r.columns()[1])[i] == q + static_cast<double>(want.x[i]);
+		}
+		taken++;
 	}
-	CHECK(submitted == batches && pipeline.in_flight() == 0);
-	CHECK(max_in_flight >= 2);
-	CHECK(got_sum == want_sum); // bit-exact, in submission order
-	CHECK(got_diff == want_diff);
+	EXPECT(exact && named);
+	EXPECT(deepest == GpuStreamPipeline::kDepth); // the pipeline filled up
+	EXPECT(pipe.in_flight() == 0);
+	GpuStreamResult none;
+	EXPECT(!pipe.TryTake(none));
 }
 
-static void TestChainedLevels() {
+VT_CASE(ChainedLevelsReadTheLevelBelow) {
 	GpuMemoryPool::Instance().ResetCursorForTesting();
-	GpuStreamPipeline pipeline(*ChainedPlan(), {"a", "b"}, {GpuValueType::FLOAT64, GpuValueType::FLOAT64}, kMaxRows);
-	const uint64_t rows = 4096;
-	std::vector<double> a(rows), b(rows);
-	for (uint64_t i = 0; i < rows; i++) {
-		a[i] = i * 0.125;
-		b[i] = double(i % 17) - 8.0;
-	}
-	CHECK(pipeline.CanSubmit());
-	pipeline.Submit(Batch(GpuMemoryPool::Instance().AcquireNext(), a, b));
+	// level 1: col0 = x * 3 - 7, col1 = y / 4 + x; level 2: col0 = col0 + 1, col1 = col1 * col1; level 3: col0 - col1
+	auto l2 = Project(TwoOutputPlan(),
+	                  {E("((long long*)out)[idx] = ((const long long*)inputs[0])[row] + 1LL;", {"col0"},
+	                     GpuValueType::INT64),
+	                   E("((double*)out)[idx] = ((const double*)inputs[0])[row] * ((const double*)inputs[0])[row];",
+	                     {"col1"}, GpuValueType::FLOAT64)});
+	auto l3 = Project(l2, {E("((double*)out)[idx] = ((double)((const long long*)inputs[0])[row]) - "
+	                         "((const double*)inputs[1])[row];",
+	                         {"col0", "col1"}, GpuValueType::FLOAT64)});
+	GpuStreamPipeline pipe(*l3, kNames, kTypes, kMaxRows);
+	const Rows in = Make(5000, 99);
+	pipe.Submit(Pack(in));
 	GpuStreamResult r;
-	CHECK(pipeline.TakeBlocking(r));
-	CHECK(r.row_count() == rows);
-	auto x = Doubles(r.columns()[0]);
-	auto y = Doubles(r.columns()[1]);
-	bool ok = true;
-	for (uint64_t i = 0; i < rows && ok; i++) {
-		ok = x[i] == (a[i] + b[i]) * 3.0 && y[i] == a[i] - b[i];
+	EXPECT(pipe.TakeBlocking(r) && r.columns().size() == 1);
+	bool same = true;
+	for (uint64_t i = 0; i < 5000; i++) {
+		volatile double y = in.y[i];
+		const double q = y / 4.0;
+		const double c1 = q + static_cast<double>(in.x[i]);
+		const double sq = c1 * c1;
+		same = same && As<double>(r.columns()[0])[i] == static_cast<double>(in.x[i] * 3 - 7 + 1) - sq;
 	}
-	CHECK(ok);
+	EXPECT(same);
 }
 
-static void TestNullPropagation() {
+VT_CASE(NullsReachOnlyTheOutputsThatReadThem) {
 	GpuMemoryPool::Instance().ResetCursorForTesting();
-	GpuStreamPipeline pipeline(*SumDiffPlan(), {"a", "b"}, {GpuValueType::FLOAT64, GpuValueType::FLOAT64}, kMaxRows);
-	const uint64_t rows = 1000;
-	std::vector<double> a(rows), b(rows);
-	for (uint64_t i = 0; i < rows; i++) {
-		a[i] = double(i);
-		b[i] = i * 0.5;
+	GpuStreamPipeline pipe(*TwoOutputPlan(), kNames, kTypes, kMaxRows);
+	Rows in = Make(777, 5);
+	in.y_valid.assign((777 + 7) / 8, 0xFF);
+	for (uint64_t i = 3; i < 777; i += 11) {
+		in.y_valid[i / 8] &= static_cast<uint8_t>(~(1u << (i % 8)));
 	}
-	std::vector<uint8_t> valid((rows + 7) / 8, 0xFF);
-	for (uint64_t i = 0; i < rows; i += 7) {
-		valid[i / 8] &= static_cast<uint8_t>(~(1u << (i % 8)));
-	}
-	CHECK(pipeline.CanSubmit());
-	pipeline.Submit(Batch(GpuMemoryPool::Instance().AcquireNext(), a, b, &valid));
+	pipe.Submit(Pack(in));
 	GpuStreamResult r;
-	CHECK(pipeline.TakeBlocking(r));
-	auto s = Doubles(r.columns()[0]);
-	bool nulls = true, values = true;
-	for (uint64_t i = 0; i < rows; i++) {
-		const bool want = i % 7 != 0;
-		nulls = nulls && ValidAt(r.columns()[0], i) == want && ValidAt(r.columns()[1], i) == want;
-		values = values && (!want || s[i] == a[i] * 2.0 + b[i]);
+	EXPECT(pipe.TakeBlocking(r));
+	const GpuColumn &c0 = r.columns()[0], &c1 = r.columns()[1];
+	bool c0_all_valid = true, c1_matches = true, values = true;
+	for (uint64_t i = 0; i < 777; i++) {
+		const bool y_ok = i < 3 || (i - 3) % 11 != 0;
+		c0_all_valid = c0_all_valid && IsValid(c0, i); // col0 reads only x
+		c1_matches = c1_matches && IsValid(c1, i) == y_ok;
+		values = values && As<int64_t>(c0)[i] == in.x[i] * 3 - 7;
 	}
-	CHECK(nulls);
-	CHECK(values);
-	CHECK(r.columns()[1].validity != nullptr);
+	EXPECT(c0_all_valid && c1_matches && values);
+	EXPECT(c1.validity != nullptr);
 	r.Reset();
-	// A batch without NULLs right after a nullable one runs the plain kernels and carries no validity.
-	pipeline.Submit(Batch(GpuMemoryPool::Instance().AcquireNext(), a, b));
-	CHECK(pipeline.TakeBlocking(r));
-	CHECK(r.columns()[0].validity == nullptr && Doubles(r.columns()[0])[7] == a[7] * 2.0 + b[7]);
+	pipe.Submit(Pack(Make(777, 5))); // the next batch has no NULLs: no bitmap comes back
+	EXPECT(pipe.TakeBlocking(r) && r.columns()[1].validity == nullptr);
 }
 
-static void TestInputSlotReturnedBeforeResultsAreRead() {
+VT_CASE(InputSlotGoesBackBeforeResultsAreTaken) {
 	GpuMemoryPool::Instance().ResetCursorForTesting();
-	GpuStreamPipeline pipeline(*SumDiffPlan(), {"a", "b"}, {GpuValueType::FLOAT64, GpuValueType::FLOAT64}, kMaxRows);
-	std::vector<double> a(512, 1.5), b(512, 2.5);
-	auto producer = GpuMemoryPool::Instance().AcquireNext();
-	CHECK(pipeline.CanSubmit());
-	pipeline.Submit(Batch(std::move(producer), a, b));
-	producer = GpuMemoryPool::Instance().AcquireNext();
-	CHECK(pipeline.CanSubmit());
-	pipeline.Submit(Batch(std::move(producer), a, b));
-	producer = GpuMemoryPool::Instance().AcquireNext();
-	CHECK(pipeline.in_flight() == 2);
-	bool released = false;
-	for (int i = 0; i < 1000000 && !released; i++) {
-		released = pipeline.CanSubmit();
+	GpuStreamPipeline pipe(*TwoOutputPlan(), kNames, kTypes, kMaxRows);
+	for (size_t i = 0; i < GpuStreamPipeline::kDepth; i++) {
+		EXPECT(pipe.CanSubmit());
+		pipe.Submit(Pack(Make(kMaxRows, 40 + i))); // ring tickets 0, 1, 2
 	}
-	CHECK(released);
-	CHECK(pipeline.in_flight() == 2);
-	pipeline.Submit(Batch(std::move(producer), a, b));
-	producer = GpuMemoryPool::Instance().AcquireNext();
-	CHECK(producer.valid() && pipeline.in_flight() == 3);
-	GpuStreamResult r;
-	while (pipeline.in_flight() > 0) {
-		CHECK(pipeline.TakeBlocking(r));
-		r.Reset();
+	EXPECT(pipe.in_flight() == GpuStreamPipeline::kDepth);
+	EXPECT(hipDeviceSynchronize() == hipSuccess); // every upload has finished
+	pipe.Poll();                                  // ... and the pipeline has seen it
+	// Tickets 3 and 4 take slot 3 and slot 0; slot 0 is only free if the first batch's input went back,
+	// with all three results still untaken. A waiter that never gets it must not hang the suite.
+	auto got = std::make_shared<std::atomic<int>>(0);
+	std::thread([got] {
+		GpuMemoryPoolSlot a = GpuMemoryPool::Instance().AcquireNext();
+		GpuMemoryPoolSlot b = GpuMemoryPool::Instance().AcquireNext();
+		*got = a.valid() && b.valid() ? 1 : -1;
+	}).detach();
+	for (int i = 0; i < 1000 && *got == 0; i++) {
+		std::this_thread::sleep_for(std::chrono::milliseconds(10));
 	}
+	EXPECT(*got == 1);
+	EXPECT(pipe.in_flight() == GpuStreamPipeline::kDepth);
+	while (pipe.in_flight() > 0) {
+		GpuStreamResult r;
+		EXPECT(pipe.TakeBlocking(r));
+	}
+	EXPECT(pipe.CanSubmit());
 }
 
-static void TestSubmitRejectsBadBatches() {
+VT_CASE(SubmitRefusesMalformedBatches) {
 	GpuMemoryPool::Instance().ResetCursorForTesting();
-	GpuStreamPipeline pipeline(*SumDiffPlan(), {"a", "b"}, {GpuValueType::FLOAT64, GpuValueType::FLOAT64}, kMaxRows);
-	std::vector<double> a(16, 1.0), b(16, 2.0);
-	auto oversized = Batch(GpuMemoryPool::Instance().AcquireNext(), a, b);
-	oversized.row_count = kMaxRows + 1;
-	CHECK_THROWS(pipeline.Submit(std::move(oversized)));
-	auto renamed = Batch(GpuMemoryPool::Instance().AcquireNext(), a, b);
-	renamed.columns[1].name = "z";
-	CHECK_THROWS(pipeline.Submit(std::move(renamed)));
-	auto empty = Batch(GpuMemoryPool::Instance().AcquireNext(), a, b);
+	GpuStreamPipeline pipe(*TwoOutputPlan(), kNames, kTypes, kMaxRows);
+	GpuStreamBatch too_big = Pack(Make(8, 1));
+	too_big.row_count = kMaxRows + 1;
+	EXPECT_THROWS(pipe.Submit(std::move(too_big)), "");
+	GpuStreamBatch empty = Pack(Make(8, 1));
 	empty.row_count = 0;
-	CHECK_THROWS(pipeline.Submit(std::move(empty)));
+	EXPECT_THROWS(pipe.Submit(std::move(empty)), "");
+	GpuStreamBatch wrong_name = Pack(Make(8, 1));
+	wrong_name.columns[0].name = "w";
+	EXPECT_THROWS(pipe.Submit(std::move(wrong_name)), "");
+	GpuStreamBatch missing = Pack(Make(8, 1));
+	missing.columns.pop_back();
+	EXPECT_THROWS(pipe.Submit(std::move(missing)), "");
+	EXPECT(pipe.in_flight() == 0);
 }
 
-static void TestRunningSumCarriesAcrossBatches() {
+VT_CASE(RunningSumCarriesFromBatchToBatch) {
 	GpuMemoryPool::Instance().ResetCursorForTesting();
-	auto plan = Projection(Scan({"a", "b"}),
-	                       {Expr("((long long*)out)[idx] = (long long)((const double*)inputs[0])[row];", {"a"},
-	                             GpuValueType::INT64),
-	                        Expr("((double*)out)[idx] = ((const double*)inputs[0])[row];", {"b"}, GpuValueType::FLOAT64)});
+	auto plan = Project(Scan(kNames),
+	                    {E("((long long*)out)[idx] = ((const long long*)inputs[0])[row] % 1000LL;", {"x"},
+	                       GpuValueType::INT64),
+	                     E("((double*)out)[idx] = ((const double*)inputs[0])[row];", {"y"}, GpuValueType::FLOAT64)});
 	plan->expressions[0].window_primitive = GpuWindowPrimitive::RUNNING_SUM;
 	plan->expressions[1].window_primitive = GpuWindowPrimitive::RUNNING_SUM;
-	CHECK(GpuStreamPipeline::Supports(*plan));
-	GpuStreamPipeline pipeline(*plan, {"a", "b"}, {GpuValueType::FLOAT64, GpuValueType::FLOAT64}, kMaxRows);
-	long long run_i = 0;
-	double run_d = 0.0;
-	bool ints = true, dbl = true;
-	for (int batch = 0; batch < 5; batch++) {
-		const uint64_t rows = 3000 + 1234 * batch;
-		std::vector<double> a(rows), b(rows);
-		for (uint64_t i = 0; i < rows; i++) {
-			a[i] = double((i * 7 + batch) % 101) - 50.0;
-			b[i] = 0.5 * double(i % 8); // exact in any order
+	EXPECT(GpuStreamPipeline::Supports(*plan));
+	GpuStreamPipeline pipe(*plan, kNames, kTypes, kMaxRows);
+	int64_t run_x = 0;
+	double run_y = 0.0;
+	bool ints = true, doubles = true;
+	for (int b = 0; b < 6; b++) {
+		Rows in = Make(1000 + 2500 * b, 300 + b);
+		for (auto &y : in.y) {
+			y = std::floor(y) / 2.0; // halves: every partial sum is exact in any order
 		}
-		pipeline.Submit(Batch(GpuMemoryPool::Instance().AcquireNext(), a, b));
+		pipe.Submit(Pack(in));
 		GpuStreamResult r;
-		CHECK(pipeline.TakeBlocking(r));
-		auto *gi = static_cast<const long long *>(r.columns()[0].data);
-		auto gd = Doubles(r.columns()[1]);
-		for (uint64_t i = 0; i < rows; i++) {
-			run_i += (long long)a[i];
-			run_d += b[i];
-			ints = ints && gi[i] == run_i;
-			dbl = dbl && gd[i] == run_d;
+		EXPECT(pipe.TakeBlocking(r));
+		for (uint64_t i = 0; i < in.x.size(); i++) {
+			run_x += in.x[i] % 1000;
+			run_y += in.y[i];
+			ints = ints && As<int64_t>(r.columns()[0])[i] == run_x;
+			doubles = doubles && As<double>(r.columns()[1])[i] == run_y;
 		}
 	}
-	CHECK(ints);
-	CHECK(dbl);
+	EXPECT(ints && doubles);
 }
 
-static void TestWideLevelSplitsAcrossKernels() {
+VT_CASE(WideLevelOfTwentyFourOutputs) {
 	GpuMemoryPool::Instance().ResetCursorForTesting();
-	std::vector<GpuExpr> exprs;
-	for (int i = 0; i < 20; i++) {
-		exprs.push_back(Expr("((double*)out)[idx] = ((const double*)inputs[0])[row] * " + std::to_string(i + 1) +
-		                         ".0 + ((const double*)inputs[1])[row];",
-		                     {"a", "b"}, GpuValueType::FLOAT64));
+	std::vector<GpuExpr> outs;
+	for (int k = 0; k < 24; k++) {
+		outs.push_back(E("((double*)out)[idx] = ((const double*)inputs[0])[row] - " + std::to_string(k) +
+		                     ".0 * (double)((const long long*)inputs[1])[row];",
+		                 {"y", "x"}, GpuValueType::FLOAT64));
 	}
-	auto plan = Projection(Scan({"a", "b"}), exprs);
-	GpuStreamPipeline pipeline(*plan, {"a", "b"}, {GpuValueType::FLOAT64, GpuValueType::FLOAT64}, kMaxRows);
-	const uint64_t rows = 3333;
-	std::vector<double> a(rows), b(rows);
-	for (uint64_t i = 0; i < rows; i++) {
-		a[i] = i * 0.25;
-		b[i] = double(i % 3);
+	GpuStreamPipeline pipe(*Project(Scan(kNames), outs), kNames, kTypes, kMaxRows);
+	Rows in = Make(2222, 8);
+	for (auto &x : in.x) {
+		x %= 4096; // keeps every product exact
 	}
-	pipeline.Submit(Batch(GpuMemoryPool::Instance().AcquireNext(), a, b));
+	pipe.Submit(Pack(in));
 	GpuStreamResult r;
-	CHECK(pipeline.TakeBlocking(r));
-	CHECK(r.columns().size() == 20);
-	bool ok = r.columns().size() == 20;
-	for (int e = 0; e < 20 && ok; e++) {
-		auto v = Doubles(r.columns()[static_cast<size_t>(e)]);
-		for (uint64_t i = 0; i < rows && ok; i++) {
-			ok = v[i] == a[i] * double(e + 1) + b[i];
+	EXPECT(pipe.TakeBlocking(r) && r.columns().size() == 24);
+	bool same = r.columns().size() == 24;
+	for (int k = 0; same && k < 24; k++) {
+		for (uint64_t i = 0; same && i < 2222; i++) {
+			same = As<double>(r.columns()[k])[i] == in.y[i] - static_cast<double>(k) * static_cast<double>(in.x[i]);
 		}
 	}
-	CHECK(ok);
+	EXPECT(same);
 }
 
-int main() {
-	RUN_TEST(TestSupportsOnlyProjectionChainsOverScan);
-	RUN_TEST(TestConstructorRejectsMismatchedSchema);
-	RUN_TEST(TestStreamsManyBatchesInOrderAndOverlaps);
-	RUN_TEST(TestChainedLevels);
-	RUN_TEST(TestNullPropagation);
-	RUN_TEST(TestInputSlotReturnedBeforeResultsAreRead);
-	RUN_TEST(TestSubmitRejectsBadBatches);
-	RUN_TEST(TestRunningSumCarriesAcrossBatches);
-	RUN_TEST(TestWideLevelSplitsAcrossKernels);
-	TEST_MAIN_EPILOGUE();
-}
+VT_MAIN()

@@ -1,106 +1,91 @@
-// gpu_memory_pool.cpp -- cyclic ring of page-locked batch slots (contract of
-// cuda_engine/src/gpu_memory_pool.cpp).
+// gpu_memory_pool.cpp -- the stream pipeline's ring of page-locked batch slots (the interface of
+// cuda_engine/include/gpu_memory_pool.hpp).
+//
+// The ring is one page-locked region: one hipHostMalloc call and one DMA mapping for all four slots. Slots go
+// out through a ticket turnstile: the n-th caller of AcquireNext holds ticket n and gets slot n % kSlotCount
+// as soon as that slot has come back from its previous holder, so concurrent producers are served strictly
+// in arrival order and never contend for the same slot.
 #include "../../../include/gpu_memory_pool.hpp"
 #include "../../../include/gpu_logger.hpp"
 
 #include <hip/hip_runtime.h>
 
+#include <numeric>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 namespace vector_gpu {
 
 GpuMemoryPoolSlot::GpuMemoryPoolSlot(GpuMemoryPoolSlot &&other) noexcept
-    : pool_(other.pool_), index_(other.index_), data_(other.data_), capacity_(other.capacity_) {
-	other.pool_ = nullptr;
-	other.index_ = -1;
-	other.data_ = nullptr;
-	other.capacity_ = 0;
+    : pool_(std::exchange(other.pool_, nullptr)), index_(std::exchange(other.index_, -1)),
+      data_(std::exchange(other.data_, nullptr)), capacity_(std::exchange(other.capacity_, 0)) {
 }
 
 GpuMemoryPoolSlot &GpuMemoryPoolSlot::operator=(GpuMemoryPoolSlot &&other) noexcept {
-	if (this != &other) {
-		if (pool_ != nullptr) {
-			pool_->Release(index_);
-		}
-		pool_ = other.pool_;
-		index_ = other.index_;
-		data_ = other.data_;
-		capacity_ = other.capacity_;
-		other.pool_ = nullptr;
-		other.index_ = -1;
-		other.data_ = nullptr;
-		other.capacity_ = 0;
-	}
-	return *this;
+	GpuMemoryPoolSlot taken(std::move(other)); // `other` is empty from here on, even when it is *this
+	std::swap(pool_, taken.pool_);
+	std::swap(index_, taken.index_);
+	std::swap(data_, taken.data_);
+	std::swap(capacity_, taken.capacity_);
+	return *this; // `taken` now holds what this slot held before and returns it to its ring
 }
 
 GpuMemoryPoolSlot::~GpuMemoryPoolSlot() {
-	if (pool_ != nullptr) {
-		pool_->Release(index_);
+	if (auto *pool = std::exchange(pool_, nullptr)) {
+		pool->Return(index_);
 	}
 }
 
 GpuMemoryPool &GpuMemoryPool::Instance() {
-	static GpuMemoryPool instance;
-	return instance;
+	static GpuMemoryPool ring;
+	return ring;
 }
 
 GpuMemoryPool::GpuMemoryPool() {
-	for (size_t i = 0; i < kSlotCount; i++) {
-		void *ptr = nullptr;
-		hipError_t status = hipHostMalloc(&ptr, kSlotBytes, hipHostMallocPortable);
-		if (status != hipSuccess) {
-			(void)hipGetLastError();
-			for (size_t j = 0; j < i; j++) {
-				(void)hipHostFree(slots_[j].data);
-				slots_[j].data = nullptr;
-			}
-			throw std::runtime_error("GpuMemoryPool: hipHostMalloc(" + std::to_string(kSlotBytes) +
-			                         ", hipHostMallocPortable) failed for slot " + std::to_string(i) + ": " +
-			                         hipGetErrorString(status));
-		}
-		slots_[i].data = ptr;
+	void *region = nullptr;
+	const size_t bytes = kSlotCount * kSlotBytes;
+	const hipError_t status = hipHostMalloc(&region, bytes, hipHostMallocPortable);
+	if (status != hipSuccess || region == nullptr) {
+		(void)hipGetLastError();
+		throw std::runtime_error("GpuMemoryPool: cannot page-lock the " + std::to_string(bytes >> 20) +
+		                         " MiB slot ring: " + hipGetErrorString(status));
 	}
+	region_ = static_cast<unsigned char *>(region);
 }
 
 GpuMemoryPool::~GpuMemoryPool() {
-	for (auto &slot : slots_) {
-		if (slot.data != nullptr) {
-			(void)hipHostFree(slot.data);
-		}
+	if (region_ != nullptr) {
+		(void)hipHostFree(region_);
 	}
 }
 
 GpuMemoryPoolSlot GpuMemoryPool::AcquireNext() {
 	std::unique_lock<std::mutex> lock(mutex_);
-	const size_t index = next_index_;
-	cv_.wait(lock, [&] { return !slots_[index].in_use; });
-	slots_[index].in_use = true;
-	next_index_ = (index + 1) % kSlotCount;
-	if (GpuLogger::Instance().Enabled(LogLevel::DEBUG)) {
-		size_t active = 0;
-		for (const auto &s : slots_) {
-			active += s.in_use ? 1 : 0;
-		}
-		VGPU_LOG(LogLevel::DEBUG, "gpu_memory_pool",
-		         "\"active_slots\":" + std::to_string(active) + ",\"total_slots\":" + std::to_string(kSlotCount));
-	}
-	return GpuMemoryPoolSlot(this, static_cast<int>(index), slots_[index].data, kSlotBytes);
+	const unsigned long long ticket = next_ticket_++;
+	const size_t slot = static_cast<size_t>(ticket % kSlotCount);
+	const unsigned long long round = ticket / kSlotCount;
+	returned_.wait(lock, [&] { return returns_[slot] >= round; });
+	VGPU_LOG(LogLevel::DEBUG, "gpu_memory_pool",
+	         "\"ticket\":" + std::to_string(ticket) + ",\"slot\":" + std::to_string(slot) + ",\"outstanding\":" +
+	             std::to_string(next_ticket_ - std::accumulate(returns_.begin(), returns_.end(), 0ull)));
+	return GpuMemoryPoolSlot(this, static_cast<int>(slot), region_ + slot * kSlotBytes, kSlotBytes);
 }
 
 void GpuMemoryPool::ResetCursorForTesting() {
-	std::lock_guard<std::mutex> guard(mutex_);
-	next_index_ = 0;
+	std::lock_guard<std::mutex> lock(mutex_);
+	// Start a fresh round: every slot counts as returned once per round issued so far.
+	const unsigned long long rounds = (next_ticket_ + kSlotCount - 1) / kSlotCount;
+	next_ticket_ = rounds * kSlotCount;
+	returns_.fill(rounds);
 }
 
-void GpuMemoryPool::Release(int index) {
+void GpuMemoryPool::Return(int index) {
 	{
-		std::lock_guard<std::mutex> guard(mutex_);
-		slots_[static_cast<size_t>(index)].in_use = false;
+		std::lock_guard<std::mutex> lock(mutex_);
+		returns_[static_cast<size_t>(index)]++;
 	}
-	// Waiters block on one specific index, so every waiter must re-check.
-	cv_.notify_all();
+	returned_.notify_all(); // each waiter waits for its own slot and round
 }
 
 } // namespace vector_gpu

@@ -1,113 +1,116 @@
-// pinned_buffer_pool.cpp -- page-locked staging slots (semantics of
-// cuda_engine/src/pinned_buffer_pool.cpp: lazy growth to kMaxSlots, grow-only slot capacity, blocking
-// Acquire at the cap).
+// pinned_buffer_pool.cpp -- size-classed cache of page-locked staging buffers (the engine-private role of
+// cuda_engine/include/pinned_buffer_pool.hpp: staging for host <-> device copies of plan columns).
 #include "../../../include/pinned_buffer_pool.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 namespace vector_gpu {
 
+namespace {
+size_t SizeClass(size_t bytes) {
+	size_t c = PinnedBufferPool::kMinClassBytes;
+	while (c < bytes) {
+		c <<= 1;
+	}
+	return c;
+}
+} // namespace
+
 PinnedBufferSlot::PinnedBufferSlot(PinnedBufferSlot &&other) noexcept
-    : pool_(other.pool_), index_(other.index_), data_(other.data_) {
-	other.pool_ = nullptr;
-	other.index_ = -1;
-	other.data_ = nullptr;
+    : pool_(std::exchange(other.pool_, nullptr)), data_(std::exchange(other.data_, nullptr)),
+      bytes_(std::exchange(other.bytes_, 0)) {
 }
 
 PinnedBufferSlot &PinnedBufferSlot::operator=(PinnedBufferSlot &&other) noexcept {
-	if (this != &other) {
-		if (pool_ != nullptr) {
-			pool_->Release(index_);
-		}
-		pool_ = other.pool_;
-		index_ = other.index_;
-		data_ = other.data_;
-		other.pool_ = nullptr;
-		other.index_ = -1;
-		other.data_ = nullptr;
-	}
-	return *this;
+	PinnedBufferSlot taken(std::move(other));
+	std::swap(pool_, taken.pool_);
+	std::swap(data_, taken.data_);
+	std::swap(bytes_, taken.bytes_);
+	return *this; // the buffer this slot held before goes back with `taken`
 }
 
 PinnedBufferSlot::~PinnedBufferSlot() {
-	if (pool_ != nullptr) {
-		pool_->Release(index_);
-	}
-}
-
-PinnedBufferPool::PinnedBufferPool() = default;
-
-PinnedBufferPool::~PinnedBufferPool() {
-	for (auto &slot : slots_) {
-		if (slot.data != nullptr) {
-			(void)hipHostFree(slot.data);
-		}
+	if (auto *pool = std::exchange(pool_, nullptr)) {
+		pool->Return(data_, bytes_);
 	}
 }
 
 PinnedBufferPool &PinnedBufferPool::Instance() {
-	// Leaked on purpose: freeing page-locked memory during static destruction races the runtime's teardown.
+	// Never destroyed: releasing page-locked memory during static destruction races the runtime's teardown.
 	static PinnedBufferPool *pool = new PinnedBufferPool();
 	return *pool;
 }
 
-PinnedBufferSlot PinnedBufferPool::Acquire(size_t bytes) {
-	std::unique_lock<std::mutex> lock(mutex_);
-	int chosen = -1;
-	cv_.wait(lock, [&] {
-		// Prefer a free slot that is already large enough, then any free slot, then a new slot.
-		int any = -1;
-		for (size_t i = 0; i < slots_.size(); i++) {
-			if (slots_[i].in_use) {
-				continue;
-			}
-			if (slots_[i].capacity >= bytes) {
-				chosen = static_cast<int>(i);
-				return true;
-			}
-			if (any < 0) {
-				any = static_cast<int>(i);
-			}
+PinnedBufferPool::~PinnedBufferPool() {
+	for (auto &cls : idle_) {
+		for (void *p : cls.second) {
+			(void)hipHostFree(p);
 		}
-		if (any >= 0) {
-			chosen = any;
-			return true;
-		}
-		if (slots_.size() < kMaxSlots) {
-			slots_.emplace_back();
-			chosen = static_cast<int>(slots_.size() - 1);
-			return true;
-		}
-		return false;
-	});
-	auto &slot = slots_[static_cast<size_t>(chosen)];
-	if (bytes > slot.capacity) {
-		void *grown = nullptr;
-		hipError_t status = hipHostMalloc(&grown, bytes, hipHostMallocDefault);
-		if (status != hipSuccess) {
-			(void)hipGetLastError();
-			throw std::runtime_error(std::string("PinnedBufferPool::Acquire: hipHostMalloc(") + std::to_string(bytes) +
-			                         ") failed: " + hipGetErrorString(status));
-		}
-		if (slot.data != nullptr) {
-			(void)hipHostFree(slot.data);
-		}
-		slot.data = grown;
-		slot.capacity = bytes;
 	}
-	slot.in_use = true;
-	return PinnedBufferSlot(this, chosen, slot.data);
 }
 
-void PinnedBufferPool::Release(int index) {
-	{
-		std::lock_guard<std::mutex> guard(mutex_);
-		slots_[static_cast<size_t>(index)].in_use = false;
+PinnedBufferSlot PinnedBufferPool::Acquire(size_t bytes) {
+	const size_t cls = SizeClass(bytes);
+	std::unique_lock<std::mutex> lock(mutex_);
+	for (;;) {
+		// the smallest idle buffer that covers the request
+		auto fit = idle_.lower_bound(cls);
+		while (fit != idle_.end() && fit->second.empty()) {
+			++fit;
+		}
+		if (fit != idle_.end()) {
+			void *p = fit->second.back();
+			fit->second.pop_back();
+			return PinnedBufferSlot(this, p, fit->first);
+		}
+		if (live_buffers_ >= kMaxSlots) {
+			// at the cap: give back the largest idle buffer (all idle ones are too small), else wait
+			auto largest = idle_.rbegin();
+			while (largest != idle_.rend() && largest->second.empty()) {
+				++largest;
+			}
+			if (largest == idle_.rend()) {
+				returned_.wait(lock);
+				continue;
+			}
+			(void)hipHostFree(largest->second.back());
+			largest->second.pop_back();
+			live_buffers_--;
+			live_bytes_ -= largest->first;
+		}
+		void *p = nullptr;
+		const hipError_t status = hipHostMalloc(&p, cls, hipHostMallocDefault);
+		if (status != hipSuccess || p == nullptr) {
+			(void)hipGetLastError();
+			throw std::runtime_error("PinnedBufferPool::Acquire: cannot page-lock " + std::to_string(cls) +
+			                         " bytes: " + hipGetErrorString(status));
+		}
+		live_buffers_++;
+		live_bytes_ += cls;
+		return PinnedBufferSlot(this, p, cls);
 	}
-	cv_.notify_one();
+}
+
+void PinnedBufferPool::Return(void *data, size_t bytes) {
+	{
+		std::lock_guard<std::mutex> lock(mutex_);
+		idle_[bytes].push_back(data);
+	}
+	returned_.notify_one();
+}
+
+size_t PinnedBufferPool::LiveBuffers() const {
+	std::lock_guard<std::mutex> lock(mutex_);
+	return live_buffers_;
+}
+
+size_t PinnedBufferPool::LiveBytes() const {
+	std::lock_guard<std::mutex> lock(mutex_);
+	return live_bytes_;
 }
 
 } // namespace vector_gpu

@@ -52,7 +52,7 @@ void *DevicePool::TakeCachedLocked(size_t size, hipStream_t stream) {
 		}
 		free_.erase(it);
 		cached_ -= bsize;
-		live_[ptr] = bsize;
+		live_[ptr] = Live {bsize, stream};
 		in_use_ += bsize;
 		peak_ = std::max(peak_, in_use_);
 		return ptr;
@@ -81,7 +81,7 @@ void *DevicePool::Alloc(size_t bytes, hipStream_t stream) {
 		}
 	}
 	std::lock_guard<std::mutex> lock(mutex_);
-	live_[ptr] = size;
+	live_[ptr] = Live {size, stream};
 	in_use_ += size;
 	peak_ = std::max(peak_, in_use_);
 	return ptr;
@@ -98,7 +98,7 @@ void DevicePool::Free(void *ptr, hipStream_t stream) {
 	}
 	Block b;
 	b.ptr = ptr;
-	b.size = it->second;
+	b.size = it->second.size;
 	b.stream = stream;
 	if (hipEventCreateWithFlags(&b.ready, hipEventDisableTiming) != hipSuccess) {
 		(void)hipGetLastError();
@@ -124,10 +124,19 @@ void DevicePool::FreeAny(void *ptr) {
 	if (ptr == nullptr) {
 		return;
 	}
-	if (Owns(ptr)) {
-		// The null stream synchronizes with every blocking stream, so the event recorded there is only
-		// complete once earlier work on them is.
-		Free(ptr, nullptr);
+	hipStream_t last = nullptr;
+	bool ours = false;
+	{
+		std::lock_guard<std::mutex> lock(mutex_);
+		auto it = live_.find(ptr);
+		if (it != live_.end()) {
+			ours = true;
+			last = it->second.stream;
+		}
+	}
+	if (ours) {
+		// The stream the block was allocated on is taken as its last user: the reuse event is recorded there.
+		Free(ptr, last);
 		return;
 	}
 	(void)hipFree(ptr);
@@ -136,6 +145,11 @@ void DevicePool::FreeAny(void *ptr) {
 void DevicePool::ForgetStream(hipStream_t stream) {
 	(void)hipStreamSynchronize(stream);
 	std::lock_guard<std::mutex> lock(mutex_);
+	for (auto &kv : live_) {
+		if (kv.second.stream == stream) {
+			kv.second.stream = nullptr; // a later FreeAny records on the null stream, never on a dead handle
+		}
+	}
 	for (auto &kv : free_) {
 		Block &b = kv.second;
 		if (b.stream != stream || b.idle) {

@@ -28,15 +28,18 @@ public:
 	void *Alloc(size_t bytes, hipStream_t stream);
 	//! Returns a block to the cache; `stream` is the last stream that used it.
 	void Free(void *ptr, hipStream_t stream);
-	//! Frees memory of unknown origin: a pool block goes back to the pool (after a device sync point is
-	//! not needed: it is tagged with the null stream), anything else is released with hipFree.
+	//! Frees memory of unknown origin: a pool block goes back to the pool, its reuse event recorded on the
+	//! stream it was allocated on (the null stream once that stream has been forgotten); anything else is
+	//! released with hipFree. A caller that used the block on any other stream must have synchronized that
+	//! stream first.
 	void FreeAny(void *ptr);
 	bool Owns(void *ptr) const;
 
 	//! Releases every cached (free) block back to the driver. Synchronizes the device.
 	void Trim();
 	//! Call before destroying `stream`: synchronizes it and makes every cached block last used on it
-	//! reusable on any stream, so no event recorded on it is touched after it is gone.
+	//! reusable on any stream, so no event recorded on it is touched after it is gone; live blocks allocated
+	//! on it are re-tagged with the null stream.
 	void ForgetStream(hipStream_t stream);
 
 	size_t BytesInUse() const;
@@ -62,7 +65,11 @@ private:
 
 	mutable std::mutex mutex_;
 	std::multimap<size_t, Block> free_;             // by size
-	std::unordered_map<void *, size_t> live_;        // ptr -> rounded size
+	struct Live {
+		size_t size;        // rounded size
+		hipStream_t stream; // stream the block was allocated on
+	};
+	std::unordered_map<void *, Live> live_;
 	size_t in_use_ = 0;
 	size_t cached_ = 0;
 	size_t peak_ = 0;
