@@ -392,9 +392,12 @@ int vgpu_profile_read(char *buf, size_t cap, size_t *needed);
 int vgpu_sort_autotune(int32_t on);
 int vgpu_sort_autotune_report(char *buf, size_t cap, size_t *needed);
 
-/* Device pool statistics (bytes). */
+/* Device pool statistics (bytes): in use, cached for reuse, and the peak in use since the last reset. The pool
+ * is the engine's allocator for every plan temporary (replaces the per-plan DeviceArena,
+ * cuda_engine/src/gpu_executor.cu:57-104); the peak is what EstimateWorkingSet predicts. */
 int vgpu_pool_stats(uint64_t *in_use, uint64_t *cached, uint64_t *peak);
 int vgpu_pool_trim(void);
+int vgpu_pool_reset_peak(void);
 
 #ifdef __cplusplus
 }

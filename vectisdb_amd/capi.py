@@ -131,6 +131,7 @@ _PROTOS = {
     "vgpu_profile_read": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vgpu_pool_stats": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "vgpu_pool_trim": (C.c_int, []),
+    "vgpu_pool_reset_peak": (C.c_int, []),
     "vgpu_batched_filter_group_by": (C.c_int, [C.POINTER(Column), C.c_int32, C.c_uint64, C.c_void_p, C.c_int32,
                                                C.POINTER(Expr), C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.c_int32,
                                                C.POINTER(Agg), C.c_int32, C.c_void_p, C.c_void_p,
@@ -503,6 +504,10 @@ def profile_read():
         buf = C.create_string_buffer(needed.value)
         check(lib.vgpu_profile_read(buf, len(buf), C.byref(needed)))
     return json.loads(buf.value.decode())
+
+
+def pool_reset_peak():
+    check(load().vgpu_pool_reset_peak())
 
 
 def pool_stats():

@@ -16,6 +16,9 @@
 //                                                  needs one output row and no sort
 //   HASH_JOIN     (left + right) x 48 +            per side: sorted key + row-id double buffers and its
 //                 out_rows x (8 + width)           distinct-key runs; u32 pair indices and gathered outputs
+//                 under a keyless GROUP BY:        aggregated without materializing: the smaller side sorted
+//                 larger x 8 + smaller x 32        with its payload into a per-key table, the larger side
+//                                                  regrouped by key range once
 //   CROSS_PRODUCT out_rows x width                 expanded outputs
 #include "../../../include/gpu_engine.hpp"
 
@@ -54,7 +57,7 @@ uint64_t ChildRows(const GpuPlanNode &node, size_t index, uint64_t cap) {
 	return index < node.children.size() && node.children[index] ? NodeRows(*node.children[index], cap) : 0;
 }
 
-uint64_t NodeBytes(const GpuPlanNode &node, uint64_t cap) {
+uint64_t NodeBytes(const GpuPlanNode &node, uint64_t cap, bool aggregated) {
 	const uint64_t width = RowWidth(node);
 	const uint64_t out_rows = NodeRows(node, cap);
 	switch (node.op_type) {
@@ -83,8 +86,15 @@ uint64_t NodeBytes(const GpuPlanNode &node, uint64_t cap) {
 		return SatAdd(SatMul(in_rows, per_row), SatMul(in_rows, SatAdd(width, 8)));
 	}
 	case GpuOpType::HASH_JOIN: {
-		const uint64_t sides = SatAdd(ChildRows(node, 0, cap), ChildRows(node, 1, cap));
-		return SatAdd(SatMul(sides, 48), SatMul(out_rows, SatAdd(8, width)));
+		const uint64_t a = ChildRows(node, 0, cap), b = ChildRows(node, 1, cap);
+		if (aggregated) {
+			// Under a keyless aggregate the join is never materialized: the smaller side is sorted with its
+			// payload and laid out as a per-key table, the larger side is read once (grouped by key range
+			// into 4-byte keys + payload).
+			const uint64_t small = a < b ? a : b, large = a < b ? b : a;
+			return SatAdd(SatMul(large, 8), SatMul(small, 32));
+		}
+		return SatAdd(SatMul(SatAdd(a, b), 48), SatMul(out_rows, SatAdd(8, width)));
 	}
 	case GpuOpType::SCAN:
 	case GpuOpType::PROJECTION:
@@ -112,13 +122,15 @@ const char *OperatorName(GpuOpType op) {
 	return "UNKNOWN";
 }
 
-void Accumulate(const GpuPlanNode &node, uint64_t cap, GpuWorkingSetEstimate &out) {
+//! `aggregated`: the node's parent is a GROUP BY without keys (a join there is aggregated, not materialized).
+void Accumulate(const GpuPlanNode &node, uint64_t cap, bool aggregated, GpuWorkingSetEstimate &out) {
+	const bool keyless_aggregate = node.op_type == GpuOpType::GROUP_BY_AGGREGATE && node.group_keys.empty();
 	for (auto &child : node.children) {
 		if (child) {
-			Accumulate(*child, cap, out);
+			Accumulate(*child, cap, keyless_aggregate, out);
 		}
 	}
-	const uint64_t bytes = NodeBytes(node, cap);
+	const uint64_t bytes = NodeBytes(node, cap, aggregated);
 	out.total_bytes = SatAdd(out.total_bytes, bytes);
 	if (bytes > out.largest_bytes) {
 		out.largest_bytes = bytes;
@@ -201,7 +213,7 @@ GpuWorkingSetEstimate EstimateWorkingSet(const GpuPlanNode &plan, uint64_t max_c
 	GpuWorkingSetEstimate estimate;
 	const bool chunked = max_chunk_rows != 0 && IsRowIndependent(plan);
 	estimate.chunk_capped = chunked;
-	Accumulate(plan, chunked ? max_chunk_rows : 0, estimate);
+	Accumulate(plan, chunked ? max_chunk_rows : 0, false, estimate);
 	return estimate;
 }
 

@@ -303,6 +303,10 @@ int vgpu_pool_stats(uint64_t *in_use, uint64_t *cached, uint64_t *peak) {
 	});
 }
 
+int vgpu_pool_reset_peak(void) {
+	return Guard([&] { DevicePool::Instance().ResetPeak(); });
+}
+
 int vgpu_pool_trim(void) {
 	return Guard([&] { DevicePool::Instance().Trim(); });
 }
