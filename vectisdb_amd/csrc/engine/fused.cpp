@@ -2,9 +2,11 @@
 #include "fused.h"
 
 #include "../../../include/kernel_cache.hpp"
+#include "../hip/join_kernels.h"
 #include "../hip/kernels.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -163,6 +165,29 @@ std::string ValidExpr(const FusedSpec &s, const FusedExpr &e, int r) {
 
 } // namespace
 
+const char *FusedPreamble() {
+	return kPreamble;
+}
+
+const char *FusedCType(int vtype) {
+	return CType(vtype);
+}
+
+std::string FusedKernelName(const FusedSpec &s, int rounds_override) {
+	const int rounds = rounds_override > 0 ? rounds_override : FusedRounds(s);
+	std::string name = "vectis_";
+	if (s.cross) {
+		name += s.predicates.empty() ? "cross_expand" : "cross_filter";
+	} else if (s.ranges) {
+		name += "batched_filter";
+	} else if (s.selection_in) {
+		name += s.predicates.empty() ? "gather_project" : "gather_filter";
+	} else {
+		name += s.predicates.empty() ? "project" : "filter_project";
+	}
+	return name + "_t" + std::to_string(kBlock * kVec * rounds);
+}
+
 std::string GenerateFusedSource(const FusedSpec &s, int rounds_override) {
 	if (s.cols.size() > (size_t)kFusedMaxCols || s.projections.size() > (size_t)kFusedMaxOuts) {
 		throw std::runtime_error("fused kernel: at most 16 input columns and 16 outputs");
@@ -181,7 +206,8 @@ std::string GenerateFusedSource(const FusedSpec &s, int rounds_override) {
 	const int tile_rows = kBlock * kVec * rounds;
 	std::ostringstream o;
 	o << kPreamble;
-	o << "extern \"C\" __global__ void __launch_bounds__(" << kBlock << ") vectis_fused(VArgs a) {\n";
+	o << "extern \"C\" __global__ void __launch_bounds__(" << kBlock << ") " << FusedKernelName(s, rounds)
+	  << "(VArgs a) {\n";
 	o << "  const int tid = threadIdx.x; const int lane = tid & 63; const int wave = tid >> 6; (void)lane;\n";
 	// Output staging buffer: each output (and the selection) leaves the block as one contiguous run.
 	// Sized for the widest staged value (outputs, selection ids, validity bytes): 4-byte outputs halve it,
@@ -489,8 +515,53 @@ std::string GenerateFusedSource(const FusedSpec &s, int rounds_override) {
 	return o.str();
 }
 
+namespace {
+//! Pairs per static tile of the aligned cross-product kernels (VGPU_CROSS_TILE: 4096, 8192 or 16384).
+int CrossTile() {
+	static const int tile = [] {
+		const char *v = std::getenv("VGPU_CROSS_TILE");
+		const int t = v != nullptr ? std::atoi(v) : 8192;
+		return (t == 4096 || t == 16384) ? t : 8192;
+	}();
+	return tile;
+}
+//! Left rows per block of the aligned cross-product count and write kernels (VGPU_CROSS_COUNT_ROWS,
+//! VGPU_CROSS_WRITE_ROWS: 1 to 64).
+int CrossRows(const char *env, int fallback) {
+	const char *v = std::getenv(env);
+	const int r = v != nullptr ? std::atoi(v) : fallback;
+	return r >= 1 && r <= 64 ? r : fallback;
+}
+//! Bump allocator over the unused tail of a launch's state buffer (for the scan of the tile counts).
+struct SpanScratch : Scratch {
+	SpanScratch(uint64_t *begin, uint64_t *end) : next(begin), end(end) {
+	}
+	void *Alloc(size_t bytes) override {
+		const size_t words = (bytes + 7) / 8;
+		if (next + words > end) {
+			throw std::runtime_error("fused kernel: state buffer too small for the tile-count scan");
+		}
+		void *p = next;
+		next += words;
+		return p;
+	}
+	uint64_t *next;
+	uint64_t *end;
+};
+bool CrossTwoPass(const FusedSpec &spec) {
+	return spec.cross && !spec.predicates.empty() && spec.range_slots.empty() && !spec.emit_selection &&
+	       !spec.selection_in && !spec.ranges;
+}
+} // namespace
+
 FusedKernel::~FusedKernel() {
 	DestroyRtcKernel(rtc_);
+	if (rtc_count_ != nullptr) {
+		DestroyRtcKernel(rtc_count_);
+	}
+	if (rtc_write_ != nullptr) {
+		DestroyRtcKernel(rtc_write_);
+	}
 	if (rtc_large_ != nullptr) {
 		DestroyRtcKernel(rtc_large_);
 	}
@@ -554,7 +625,8 @@ void FusedKernel::Launch(const FusedLaunch &args, void *scratch_state, size_t sc
 	if (large_rounds_ > 0 && args.n >= kLargeRows) {
 		std::lock_guard<std::mutex> lock(large_mutex_);
 		if (rtc_large_ == nullptr) {
-			rtc_large_ = CompileRtc(GenerateFusedSource(spec_, large_rounds_), "vectis_fused", spec_.fast_math);
+			rtc_large_ = CompileRtc(GenerateFusedSource(spec_, large_rounds_), FusedKernelName(spec_, large_rounds_),
+			                        spec_.fast_math);
 		}
 		rtc = rtc_large_;
 		rounds = large_rounds_;
@@ -586,14 +658,37 @@ void FusedKernel::Launch(const FusedLaunch &args, void *scratch_state, size_t sc
 			throw std::runtime_error("fused kernel: a filtering launch needs a device count slot");
 		}
 		h.state = static_cast<uint64_t *>(scratch_state);
-		FillU64(h.state, 0ull, tiles + 1, stream);
+	}
+	const bool aligned_cross = rtc_count_ != nullptr && args.right_rows >= (uint64_t)cross_tile_ &&
+	                           args.right_rows % (uint64_t)cross_tile_ == 0;
+	if (!spec_.predicates.empty() && !aligned_cross) {
+		FillU64(h.state, 0ull, tiles + 1, stream); // look-back words and the tile ticket
 	}
 	void *kargs[] = {&h};
 	double bytes = 0;
 	for (auto &c : spec_.cols) {
 		bytes += (double)args.n * TypeWidth(c.vtype);
 	}
-	ProfileScope prof(spec_.cross ? "fused_cross_filter" : "fused_filter_project", stream, bytes);
+	if (aligned_cross) {
+		// count pass (every static tile writes its kept count), scan to offsets in place (+ the total into
+		// *count), write pass; profiled as one operator
+		const uint64_t ctiles = args.n / (uint64_t)cross_tile_;
+		const uint64_t chunks = args.right_rows / (uint64_t)cross_tile_;
+		const uint64_t left = args.n / args.right_rows;
+		h.ntiles = ctiles;
+		const uint64_t count_grid = chunks * ((left + cross_count_rows_ - 1) / cross_count_rows_);
+		const uint64_t write_grid = chunks * ((left + cross_write_rows_ - 1) / cross_write_rows_);
+		ProfileScope prof("fused_cross_filter", stream, bytes);
+		VGPU_HIP(hipModuleLaunchKernel(rtc_count_->function, (unsigned)count_grid, 1, 1, kBlock, 1, 1, 0, stream, kargs,
+		                               nullptr));
+		SpanScratch span(h.state + ctiles + 1, h.state + scratch_bytes / sizeof(uint64_t));
+		ExclusiveScanU64(h.state, h.state, ctiles, args.count_device, span, stream);
+		VGPU_HIP(hipModuleLaunchKernel(rtc_write_->function, (unsigned)write_grid, 1, 1, kBlock, 1, 1, 0, stream, kargs,
+		                               nullptr));
+		return;
+	}
+	ProfileScope prof(spec_.cross ? "fused_cross_filter" : (spec_.ranges ? "fused_batched_filter" : "fused_filter_project"),
+	                  stream, bytes);
 	VGPU_HIP(hipModuleLaunchKernel(rtc->function, (unsigned)tiles, 1, 1, kBlock, 1, 1, 0, stream, kargs, nullptr));
 }
 
@@ -610,7 +705,17 @@ std::shared_ptr<FusedKernel> GetFusedKernel(const FusedSpec &spec) {
 	auto k = std::shared_ptr<FusedKernel>(new FusedKernel());
 	k->spec_ = spec;
 	k->source = source;
-	k->rtc_ = CompileRtc(source, "vectis_fused", spec.fast_math);
+	k->rtc_ = CompileRtc(source, FusedKernelName(spec), spec.fast_math);
+	if (CrossTwoPass(spec)) {
+		const int tile = CrossTile();
+		k->cross_tile_ = tile;
+		k->cross_count_rows_ = CrossRows("VGPU_CROSS_COUNT_ROWS", 16);
+		k->cross_write_rows_ = CrossRows("VGPU_CROSS_WRITE_ROWS", 2);
+		k->rtc_count_ = CompileRtc(GenerateCrossAlignedSource(spec, tile, 1, k->cross_count_rows_),
+		                           CrossAlignedKernelName(spec, tile, 1, k->cross_count_rows_), spec.fast_math);
+		k->rtc_write_ = CompileRtc(GenerateCrossAlignedSource(spec, tile, 2, k->cross_write_rows_),
+		                           CrossAlignedKernelName(spec, tile, 2, k->cross_write_rows_), spec.fast_math);
+	}
 	k->large_rounds_ = FusedRoundsLarge(spec);
 	cache.emplace(key, k);
 	return k;

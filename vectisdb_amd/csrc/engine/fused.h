@@ -94,6 +94,15 @@ private:
 	friend std::shared_ptr<FusedKernel> GetFusedKernel(const FusedSpec &spec);
 	FusedSpec spec_;
 	RtcKernel *rtc_ = nullptr;
+	//! Filtered cross product with the right side a multiple of cross_tile_ pairs (its inputs come from
+	//! cache, so evaluating the predicate twice costs less than a tile ticket and a look-back per tile):
+	//! rtc_count_ counts each static tile's kept pairs, a one-block scan turns the counts into offsets,
+	//! rtc_write_ writes. Other cross products take the single-pass rtc_.
+	RtcKernel *rtc_count_ = nullptr;
+	RtcKernel *rtc_write_ = nullptr;
+	int cross_tile_ = 0;
+	int cross_count_rows_ = 0; // left rows per block of the count kernel
+	int cross_write_rows_ = 0; // ... and of the write kernel
 	//! Variant with larger tiles for large inputs, compiled on its first use (0 rounds: none).
 	int large_rounds_ = 0;
 	mutable std::mutex large_mutex_;
@@ -106,6 +115,23 @@ std::shared_ptr<FusedKernel> GetFusedKernel(const FusedSpec &spec);
 //! The generated source (exposed for tests and the disk-cache key); rounds_override > 0 sets the tile's
 //! row rounds instead of the spec's default.
 std::string GenerateFusedSource(const FusedSpec &spec, int rounds_override = 0);
+//! The generated kernel's symbol, which names its form and tile so profiles attribute each variant:
+//! vectis_{filter_project|project|gather_filter|gather_project|cross_filter|cross_expand|batched_filter}_t<rows>.
+std::string FusedKernelName(const FusedSpec &spec, int rounds_override = 0);
+
+//! The helpers compiled into every generated kernel (VArgs, look-back, wave scan), and the C type of a VType.
+const char *FusedPreamble();
+const char *FusedCType(int vtype);
+
+//! Cross product with a filter whose right side is a whole number of tiles (one left row per tile): the
+//! count pass (phase 1) or the write pass (phase 2) over static tiles of `tile_pairs` pairs. Left columns are
+//! read once per tile; an output that reads only left columns is the same value for every kept pair of the
+//! tile and is written as a fill; the others are compacted through LDS, staged at the output's 16-byte
+//! alignment so the copy-out is 16-byte stores. Not for specs with a selection output or range columns.
+//! Grid: (m / tile_pairs) x ceil(left rows / rows_per_block) blocks; each block holds one tile-long chunk of the
+//! right side in registers and runs it against rows_per_block consecutive left rows.
+std::string GenerateCrossAlignedSource(const FusedSpec &spec, int tile_pairs, int phase, int rows_per_block);
+std::string CrossAlignedKernelName(const FusedSpec &spec, int tile_pairs, int phase, int rows_per_block);
 
 constexpr int kFusedMaxCols = 16;
 constexpr int kFusedMaxOuts = 16;
