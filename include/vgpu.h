@@ -169,6 +169,48 @@ int vgpu_per_query_filter_group_by(const vgpu_column *cols, int32_t ncols, uint6
                                    void *const *agg_out, uint64_t capacity, uint64_t *groups,
                                    uint64_t *segment_group_offsets, void *stream);
 
+/* vgpu_batched_filter_group_by over row ranges instead of contiguous row groups: query g reads table rows
+ * [row_starts[g], row_starts[g] + row_lens[g]) (host arrays of nranges entries; ranges may overlap or
+ * repeat). Same outputs, ordered by (range, key). Plans the batch on the host (reads the kept-row count
+ * and the kept keys' ranges back). No single reference counterpart, as vgpu_batched_filter_group_by.
+ * Synchronizes. */
+int vgpu_batched_ranges_filter_group_by(const vgpu_column *cols, int32_t ncols, uint64_t rows,
+                                        const uint64_t *row_starts, const uint64_t *row_lens, int32_t nranges,
+                                        const vgpu_expr *preds, int32_t npreds, int32_t key_col,
+                                        const int32_t *value_cols, int32_t nvalues, const vgpu_agg *aggs,
+                                        int32_t naggs, void *segment_out, void *key_out, void *const *agg_out,
+                                        uint64_t capacity, uint64_t *groups, uint64_t *segment_group_offsets,
+                                        void *stream);
+
+/* The batched form prepared once and replayed as ONE HIP graph per batch. create fixes the table, the
+ * predicates, the GROUP BY and the bounds: at most max_ranges row ranges holding at most max_rows rows per
+ * batch (max_rows + 3 x max_ranges < 2^30). The group key is packed from the range count and the key
+ * column's table-wide value range (read once here), every buffer is sized for the bounds (capacity must
+ * cover min(max_rows, max_ranges x key range), else VGPU_ERROR), and the pipeline -- descriptor upload,
+ * device-side work-queue layout, the persistent filter kernel that sweeps the queue's tiles, radix passes
+ * and segmented reduction reading the kept-row count on the device, per-range group offsets, result
+ * download -- is captured into a graph. The output columns (capacity rows, as vgpu_batched_filter_group_by)
+ * belong to the caller and must outlive the plan. No reference counterpart: the reference runs one
+ * GROUP BY per query (cuda_engine/src/operators/gpu_groupby.cu:303-592) with one chunk in flight
+ * (gpu_executor_internal.hpp:100-108).
+ * run: one batch of nranges ranges, checked against the table and the bounds before any GPU work; one
+ * hipGraphLaunch on `stream`, then waits. *groups and segment_group_offsets (nranges + 1 entries) as in
+ * vgpu_batched_filter_group_by; results equal vgpu_batched_ranges_filter_group_by's for the same ranges.
+ * Runs of one plan must not overlap.
+ * info: the captured graph's kernel / copy / other node counts, the device bytes the plan holds, and the
+ * group bound. */
+typedef struct vgpu_batched_plan vgpu_batched_plan;
+int vgpu_batched_plan_create(const vgpu_column *cols, int32_t ncols, uint64_t rows, int32_t max_ranges,
+                             uint64_t max_rows, const vgpu_expr *preds, int32_t npreds, int32_t key_col,
+                             const int32_t *value_cols, int32_t nvalues, const vgpu_agg *aggs, int32_t naggs,
+                             void *segment_out, void *key_out, void *const *agg_out, uint64_t capacity,
+                             vgpu_batched_plan **plan);
+int vgpu_batched_plan_run(vgpu_batched_plan *plan, const uint64_t *row_starts, const uint64_t *row_lens,
+                          int32_t nranges, uint64_t *groups, uint64_t *segment_group_offsets, void *stream);
+int vgpu_batched_plan_info(const vgpu_batched_plan *plan, uint64_t *kernel_nodes, uint64_t *copy_nodes,
+                           uint64_t *other_nodes, uint64_t *footprint_bytes, uint64_t *group_bound);
+void vgpu_batched_plan_destroy(vgpu_batched_plan *plan);
+
 /* ---------------------------------------------------------------- primitives */
 
 /* Order-preserving copy of in[i] > threshold into out; *count_host receives the kept count (synchronizes).

@@ -142,6 +142,21 @@ _PROTOS = {
                                                  C.POINTER(C.c_int32), C.c_int32, C.POINTER(Agg), C.c_int32,
                                                  C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.c_uint64,
                                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_void_p]),
+    "vgpu_batched_ranges_filter_group_by": (C.c_int, [C.POINTER(Column), C.c_int32, C.c_uint64,
+                                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_int32,
+                                                      C.POINTER(Expr), C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                                      C.c_int32, C.POINTER(Agg), C.c_int32, C.c_void_p, C.c_void_p,
+                                                      C.POINTER(C.c_void_p), C.c_uint64, C.POINTER(C.c_uint64),
+                                                      C.POINTER(C.c_uint64), C.c_void_p]),
+    "vgpu_batched_plan_create": (C.c_int, [C.POINTER(Column), C.c_int32, C.c_uint64, C.c_int32, C.c_uint64,
+                                           C.POINTER(Expr), C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.c_int32,
+                                           C.POINTER(Agg), C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p),
+                                           C.c_uint64, C.POINTER(C.c_void_p)]),
+    "vgpu_batched_plan_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_int32,
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_void_p]),
+    "vgpu_batched_plan_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "vgpu_batched_plan_destroy": (None, [C.c_void_p]),
     "vgpu_execute_plan": (C.c_int, [C.c_char_p, C.POINTER(HostColumn), C.c_int32, C.POINTER(ExecuteOptions),
                                     C.POINTER(C.c_void_p)]),
     "vgpu_result_success": (C.c_int32, [C.c_void_p]),
@@ -358,6 +373,111 @@ def batched_filter_group_by(cols, segment_offsets, preds, key_col, value_cols, a
     g = groups.value
     trimmed = [t[: 2 * g] if a[2] == HUGEINT else t[:g] for t, a in zip(agg_out, aggs)]
     return seg_out[:g], key_out[:g], trimmed, list(offs)
+
+
+def _u64s(values):
+    values = [int(v) for v in values]
+    return (C.c_uint64 * max(len(values), 1))(*values)
+
+
+def batched_ranges_filter_group_by(cols, row_starts, row_lens, preds, key_col, value_cols, aggs, capacity=None,
+                                   stream=None):
+    """batched_filter_group_by over row ranges (query g = table rows [row_starts[g], + row_lens[g])), planned
+    on the host per batch. Returns (segment ids, keys, [aggregate columns], per-range group offsets)."""
+    import torch
+    lib = load()
+    rows = cols[0].numel()
+    nq = len(row_starts)
+    cc = (Column * len(cols))(*[column(c) for c in cols])
+    pa, keep = _exprs(preds)
+    vc = (C.c_int32 * max(len(value_cols), 1))(*value_cols)
+    ac = (Agg * max(len(aggs), 1))(*[Agg(*a) for a in aggs])
+    if capacity is None:
+        capacity = max(int(sum(row_lens)), 1)
+    key_type = _tensor_type(cols[key_col]) if 0 <= key_col < len(cols) else INT64
+    seg_out = torch.empty(capacity, dtype=torch.int32, device=cols[0].device)
+    key_out = empty_column(key_type, capacity)
+    agg_out = [empty_column(a[2], capacity) for a in aggs]
+    ap = (C.c_void_p * max(len(agg_out), 1))(*[t.data_ptr() for t in agg_out])
+    groups = C.c_uint64(0)
+    offs = (C.c_uint64 * (nq + 1))()
+    check(lib.vgpu_batched_ranges_filter_group_by(cc, len(cols), rows, _u64s(row_starts), _u64s(row_lens), nq, pa,
+                                                  len(preds), key_col, vc, len(value_cols), ac, len(aggs),
+                                                  C.c_void_p(seg_out.data_ptr()), C.c_void_p(key_out.data_ptr()), ap,
+                                                  capacity, C.byref(groups), offs, _stream_ptr(stream)))
+    g = groups.value
+    trimmed = [t[: 2 * g] if a[2] == HUGEINT else t[:g] for t, a in zip(agg_out, aggs)]
+    return seg_out[:g], key_out[:g], trimmed, list(offs)
+
+
+class BatchedPlan:
+    """The batched form prepared once over a device-resident table and run as one HIP graph per batch.
+
+    cols: device tensors of the table; at most max_ranges row ranges holding at most max_rows rows per
+    batch; preds / key_col / value_cols / aggs as in batched_filter_group_by. The plan owns its output
+    columns (capacity rows; default: the group bound). run() returns views of them, valid until the next
+    run()."""
+
+    def __init__(self, cols, max_ranges, max_rows, preds, key_col, value_cols, aggs, capacity=None):
+        import torch
+        lib = load()
+        self._lib = lib
+        self._handle = None
+        self.cols = list(cols)  # the plan reads these buffers at every run
+        rows = cols[0].numel()
+        self.aggs = list(aggs)
+        self.max_ranges = max_ranges
+        if capacity is None:
+            capacity = max(int(max_rows), 1)
+        key_type = _tensor_type(cols[key_col]) if 0 <= key_col < len(cols) else INT64
+        self.seg_out = torch.empty(capacity, dtype=torch.int32, device=cols[0].device)
+        self.key_out = empty_column(key_type, capacity)
+        self.agg_out = [empty_column(a[2], capacity) for a in aggs]
+        cc = (Column * len(cols))(*[column(c) for c in cols])
+        pa, keep = _exprs(preds)
+        vc = (C.c_int32 * max(len(value_cols), 1))(*value_cols)
+        ac = (Agg * max(len(aggs), 1))(*[Agg(*a) for a in aggs])
+        ap = (C.c_void_p * max(len(self.agg_out), 1))(*[t.data_ptr() for t in self.agg_out])
+        h = C.c_void_p()
+        check(lib.vgpu_batched_plan_create(cc, len(cols), rows, max_ranges, max_rows, pa, len(preds), key_col, vc,
+                                           len(value_cols), ac, len(aggs), C.c_void_p(self.seg_out.data_ptr()),
+                                           C.c_void_p(self.key_out.data_ptr()), ap, capacity, C.byref(h)))
+        self._handle = h
+
+    def run(self, row_starts, row_lens, stream=None):
+        nq = len(row_starts)
+        if len(row_lens) != nq:
+            raise ValueError("row_starts and row_lens differ in length")
+        groups = C.c_uint64(0)
+        offs = (C.c_uint64 * (nq + 1))()
+        check(self._lib.vgpu_batched_plan_run(self._handle, _u64s(row_starts), _u64s(row_lens), nq, C.byref(groups),
+                                              offs, _stream_ptr(stream)))
+        g = groups.value
+        trimmed = [t[: 2 * g] if a[2] == HUGEINT else t[:g] for t, a in zip(self.agg_out, self.aggs)]
+        return self.seg_out[:g], self.key_out[:g], trimmed, list(offs)
+
+    def info(self):
+        vals = [C.c_uint64(0) for _ in range(5)]
+        check(self._lib.vgpu_batched_plan_info(self._handle, *[C.byref(v) for v in vals]))
+        return dict(zip(["kernel_nodes", "copy_nodes", "other_nodes", "footprint_bytes", "group_bound"],
+                        [v.value for v in vals]))
+
+    def close(self):
+        if self._handle is not None:
+            self._lib.vgpu_batched_plan_destroy(self._handle)
+            self._handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _exprs(exprs):

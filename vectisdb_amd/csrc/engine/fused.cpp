@@ -89,6 +89,7 @@ struct VArgs {
   const vg_u64* seg_row;
   const vg_u64* seg_len;
   vg_u64* range;
+  const vg_u64* dims;
 };
 __device__ __forceinline__ vg_u64 vg_load(const vg_u64* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -179,7 +180,7 @@ std::string FusedKernelName(const FusedSpec &s, int rounds_override) {
 	if (s.cross) {
 		name += s.predicates.empty() ? "cross_expand" : "cross_filter";
 	} else if (s.ranges) {
-		name += "batched_filter";
+		name += s.persistent ? "batched_filter_persistent" : "batched_filter";
 	} else if (s.selection_in) {
 		name += s.predicates.empty() ? "gather_project" : "gather_filter";
 	} else {
@@ -204,11 +205,21 @@ std::string GenerateFusedSource(const FusedSpec &s, int rounds_override) {
 	const bool filter = !s.predicates.empty();
 	const int rounds = rounds_override > 0 ? rounds_override : FusedRounds(s);
 	const int tile_rows = kBlock * kVec * rounds;
+	if (s.persistent && (!s.ranges || !s.range_slots.empty() || rounds_override > 0)) {
+		throw std::runtime_error("fused kernel: the persistent form is for row ranges without range columns");
+	}
 	std::ostringstream o;
 	o << kPreamble;
 	o << "extern \"C\" __global__ void __launch_bounds__(" << kBlock << ") " << FusedKernelName(s, rounds)
 	  << "(VArgs a) {\n";
 	o << "  const int tid = threadIdx.x; const int lane = tid & 63; const int wave = tid >> 6; (void)lane;\n";
+	// Launch rows and tiles: kernel arguments, or (persistent form) two device words an earlier kernel wrote.
+	if (s.persistent) {
+		o << "  const vg_u64 __n = a.dims[0]; const vg_u64 __ntiles = a.dims[1];\n";
+	} else {
+		o << "  const vg_u64 __n = a.n; const vg_u64 __ntiles = a.ntiles;\n";
+	}
+	o << "  (void)__n; (void)__ntiles;\n";
 	// Output staging buffer: each output (and the selection) leaves the block as one contiguous run.
 	// Sized for the widest staged value (outputs, selection ids, validity bytes): 4-byte outputs halve it,
 	// which lets more blocks share a CU.
@@ -217,9 +228,20 @@ std::string GenerateFusedSource(const FusedSpec &s, int rounds_override) {
 		stage_width = std::max(stage_width, TypeWidth(e.out_type));
 	}
 	o << "  __shared__ vg_u64 s_stage[" << ((uint64_t)tile_rows * stage_width + 7) / 8 << "];\n";
-	if (filter) {
+	if (filter && s.persistent) {
+		// A fixed grid of resident blocks. Each block takes one rank from the ticket (ranks follow the order in
+		// which blocks start), then runs tiles rank, rank + grid, rank + 2 grid, ...: a tile's look-back only
+		// waits on lower tiles, whose blocks have all started, so the sweep cannot deadlock even when fewer
+		// blocks are resident than launched, and the ticket is taken once per block instead of once per tile.
 		o << "  __shared__ unsigned int s_tile; __shared__ vg_u64 s_wtot[4]; __shared__ vg_u64 s_prefix;\n";
-		o << "  if (tid == 0) s_tile = __hip_atomic_fetch_add((unsigned int*)(a.state + a.ntiles), 1u, "
+		o << "  if (tid == 0) s_tile = __hip_atomic_fetch_add((unsigned int*)(a.state + __ntiles), 1u, "
+		     "__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
+		o << "  __syncthreads();\n";
+		o << "  const vg_u64 __rank = s_tile;\n";
+		o << "  for (vg_u64 tile = __rank; tile < __ntiles; tile += gridDim.x) {\n";
+	} else if (filter) {
+		o << "  __shared__ unsigned int s_tile; __shared__ vg_u64 s_wtot[4]; __shared__ vg_u64 s_prefix;\n";
+		o << "  if (tid == 0) s_tile = __hip_atomic_fetch_add((unsigned int*)(a.state + __ntiles), 1u, "
 		     "__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n";
 		o << "  __syncthreads();\n";
 		o << "  const vg_u64 tile = s_tile;\n";
@@ -297,7 +319,7 @@ std::string GenerateFusedSource(const FusedSpec &s, int rounds_override) {
 		o << "#pragma unroll\n    for (int __j = 0; __j < " << kVec << "; __j++) {\n";
 		o << "      const vg_u64 __g = __sg[__j];\n";
 		o << "      const vg_u64 __off = i0 + __j - a.seg[__g];\n";
-		o << "      __in[__j] = i0 + __j < a.n && __off < a.seg_len[__g];\n";
+		o << "      __in[__j] = i0 + __j < __n && __off < a.seg_len[__g];\n";
 		o << "      __rowid[__r][__j] = __in[__j] ? a.seg_row[__g] + __off : 0ull;\n";
 		o << "    }\n";
 		// one segment, first and last row live: the lane's rows are contiguous table rows
@@ -432,7 +454,7 @@ std::string GenerateFusedSource(const FusedSpec &s, int rounds_override) {
 		o << "    if (lane == 0) {\n";
 		o << "      if (tile > 0) vg_store(a.state + tile, (2ull << 62) | (__pre + __ttot));\n";
 		o << "      s_prefix = __pre;\n";
-		o << "      if (tile == a.ntiles - 1) *a.count = __pre + __ttot;\n";
+		o << "      if (tile == __ntiles - 1) *a.count = __pre + __ttot;\n";
 		o << "    }\n";
 		o << "  }\n";
 		o << "  __syncthreads();\n";
@@ -496,6 +518,9 @@ std::string GenerateFusedSource(const FusedSpec &s, int rounds_override) {
 			});
 			o << "  }\n";
 		}
+	}
+	if (filter && s.persistent) {
+		o << "  }\n"; // the tile loop (every stage pass ends with a barrier, so LDS is free for the next tile)
 	}
 	// Range columns: wave minimum of {min, ~max}, then one atomic per wave, skipped when it cannot lower the
 	// stored word.
@@ -591,8 +616,13 @@ struct VArgsHost {
 	const uint64_t *seg_row;
 	const uint64_t *seg_len;
 	uint64_t *range;
+	const uint64_t *dims;
 };
 } // namespace
+
+uint64_t FusedKernel::tile_rows() const {
+	return (uint64_t)kBlock * kVec * FusedRounds(spec_);
+}
 
 void FusedKernel::Launch(const FusedLaunch &args, void *scratch_state, size_t scratch_bytes, hipStream_t stream) const {
 	if (!spec_.range_slots.empty()) {
@@ -622,6 +652,33 @@ void FusedKernel::Launch(const FusedLaunch &args, void *scratch_state, size_t sc
 	h.sel_out = args.sel_out;
 	int rounds = FusedRounds(spec_);
 	const RtcKernel *rtc = rtc_;
+	if (spec_.persistent) {
+		if (args.dims_device == nullptr || args.count_device == nullptr || scratch_state == nullptr ||
+		    scratch_bytes < StateBytes(args.n)) {
+			throw std::runtime_error("fused kernel: the persistent form needs device dims, a count slot and state "
+			                         "for its row bound");
+		}
+		if (args.seg_offsets == nullptr || args.nseg == 0 || args.seg_rows == nullptr || args.seg_lens == nullptr) {
+			throw std::runtime_error("fused kernel: row ranges need segment offsets, rows and lengths");
+		}
+		h.ntiles = 0;
+		h.cap = args.capacity == 0 ? args.n : args.capacity;
+		h.count = args.count_device;
+		h.state = static_cast<uint64_t *>(scratch_state);
+		h.seg = args.seg_offsets;
+		h.nseg = args.nseg;
+		h.seg_row = args.seg_rows;
+		h.seg_len = args.seg_lens;
+		h.dims = args.dims_device;
+		void *pargs[] = {&h};
+		double bytes = 0;
+		for (auto &c : spec_.cols) {
+			bytes += (double)args.n * TypeWidth(c.vtype);
+		}
+		ProfileScope prof("fused_batched_filter", stream, bytes);
+		VGPU_HIP(hipModuleLaunchKernel(rtc->function, persistent_grid_, 1, 1, kBlock, 1, 1, 0, stream, pargs, nullptr));
+		return;
+	}
 	if (large_rounds_ > 0 && args.n >= kLargeRows) {
 		std::lock_guard<std::mutex> lock(large_mutex_);
 		if (rtc_large_ == nullptr) {
@@ -716,7 +773,14 @@ std::shared_ptr<FusedKernel> GetFusedKernel(const FusedSpec &spec) {
 		k->rtc_write_ = CompileRtc(GenerateCrossAlignedSource(spec, tile, 2, k->cross_write_rows_),
 		                           CrossAlignedKernelName(spec, tile, 2, k->cross_write_rows_), spec.fast_math);
 	}
-	k->large_rounds_ = FusedRoundsLarge(spec);
+	k->large_rounds_ = spec.persistent ? 0 : FusedRoundsLarge(spec);
+	if (spec.persistent) {
+		int per_cu = 0, dev = 0, cus = 0;
+		VGPU_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k->rtc_->function, kBlock, 0));
+		VGPU_HIP(hipGetDevice(&dev));
+		VGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+		k->persistent_grid_ = (unsigned)std::max(1, per_cu) * (unsigned)std::max(1, cus);
+	}
 	cache.emplace(key, k);
 	return k;
 }

@@ -50,6 +50,11 @@ struct FusedSpec {
 	//! from table rows seg_row[g] + (row - seg[g]); launch rows from seg[g] + seg_len[g] up to seg[g+1] are
 	//! padding (never kept). Needs at least one predicate (the outputs are compacted).
 	bool ranges = false;
+	//! With `ranges`: the launch's row and tile counts are read from device memory (FusedLaunch::dims_device,
+	//! written by an earlier kernel), and a fixed grid of resident blocks sweeps the tiles, so the launch
+	//! needs nothing from the host that depends on the data (it can be captured into a graph). The caller
+	//! zeroes the look-back words and the ticket (state[0 .. ntiles]) and the count before the launch.
+	bool persistent = false;
 	bool fast_math = false;
 	//! Non-null integer columns (a segment-id column included) whose minimum and maximum over the kept rows
 	//! the launch writes to `range_device`, as order-preserving unsigned keys (signed values sign-extended
@@ -75,6 +80,9 @@ struct FusedLaunch {
 	//! range_slots: device, 2 words per slot, written as {min, ~max} (both kept by atomic minimum; the
 	//! launch sets them to ~0 first, so a launch that keeps no row leaves {~0, ~0}).
 	uint64_t *range_device = nullptr;
+	//! Persistent form: device {launch rows, tiles of tile_rows() rows}; `n` is then the bound the state
+	//! buffer is sized for.
+	const uint64_t *dims_device = nullptr;
 };
 
 class FusedKernel {
@@ -87,6 +95,12 @@ public:
 	static size_t StateBytes(uint64_t n);
 	const FusedSpec &spec() const {
 		return spec_;
+	}
+	//! Rows per tile of the default variant (the persistent form always runs it).
+	uint64_t tile_rows() const;
+	//! Blocks of the persistent form's grid (resident blocks per CU x CUs).
+	unsigned persistent_grid() const {
+		return persistent_grid_;
 	}
 	std::string source;
 
@@ -105,6 +119,7 @@ private:
 	int cross_write_rows_ = 0; // ... and of the write kernel
 	//! Variant with larger tiles for large inputs, compiled on its first use (0 rounds: none).
 	int large_rounds_ = 0;
+	unsigned persistent_grid_ = 0;
 	mutable std::mutex large_mutex_;
 	mutable RtcKernel *rtc_large_ = nullptr;
 };

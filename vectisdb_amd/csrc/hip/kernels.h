@@ -143,10 +143,15 @@ struct SortedRun {
 //! LSD radix sort of the packed key with payload. payload_in[i] == nullptr with width 4 means "generate
 //! row ids" (a uint32 permutation). Requires n < 2^30.
 void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
-                uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream);
-//! SortPacked with exactly the given variant (no defaults, no autotuning).
+                uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream,
+                const uint64_t *n_dev = nullptr);
+//! SortPacked with exactly the given variant (no defaults, no autotuning). With n_dev the row count is read
+//! by the kernels from device memory (a count an earlier kernel on `stream` wrote, at most n): buffers and
+//! grids are sized for n, blocks past the count leave at once, and the host never reads the count, so the
+//! sort can be captured into a graph.
 void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
-                    uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream);
+                    uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream,
+                    const uint64_t *n_dev = nullptr);
 
 //! Sort autotuner: when on, the first sort of each shape (digit passes, rows rounded to a power of two,
 //! payload columns and bytes) times every variant on that sort's own data and keeps the fastest for the
@@ -198,9 +203,10 @@ struct ReduceSpec {
 };
 
 //! Reduces sorted keys/payload into one row per distinct key. `keys` may be nullptr: every row is then one
-//! group (global aggregate). Writes the group count to *groups_device (device uint64).
+//! group (global aggregate). Writes the group count to *groups_device (device uint64). With n_dev the row
+//! count is read on the device as in SortPackedWith (n is its bound).
 void SegmentedReduce(const SortedRun &run, uint64_t n, const ReduceSpec &spec, uint64_t *groups_device,
-                     Scratch &scratch, hipStream_t stream);
+                     Scratch &scratch, hipStream_t stream, const uint64_t *n_dev = nullptr);
 
 // ------------------------------------------------------------------ join
 

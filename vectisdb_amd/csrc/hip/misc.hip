@@ -76,6 +76,11 @@ ProfileScope::ProfileScope(const char *name, hipStream_t stream, double bytes)
 	if (!ProfileEnabled()) {
 		return;
 	}
+	// launches captured into a graph are not timed here (the events would never be recorded)
+	hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+	if (hipStreamIsCapturing(stream_, &capture) != hipSuccess || capture != hipStreamCaptureStatusNone) {
+		return;
+	}
 	VGPU_HIP(hipEventCreate(&start_));
 	VGPU_HIP(hipEventCreate(&stop_));
 	VGPU_HIP(hipEventRecord(start_, stream_));

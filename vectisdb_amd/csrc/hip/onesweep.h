@@ -75,11 +75,12 @@ __device__ __forceinline__ void LoadWaveItems(const void *src, int width, uint64
 
 
 //! Launches one pass of variant (R, ITEMS, PF, BLOCK) over `tiles` tiles of BLOCK x ITEMS rows; dispatches on
-//! the compact key width, FIRST and the payload count (0, 1, 2, or up to 4).
+//! the compact key width, FIRST and the payload count (0, 1, 2, or up to 4). With n_dev the kernel reads the
+//! row count from device memory (n and `tiles` are then its bound).
 template <int R, int ITEMS, bool PF, int BLOCK>
 void LaunchVariant(bool first, bool keys64, const KeyPack &pack, const void *keys_in, void *keys_out,
                    const PayloadArgs &pay, uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state,
-                   uint32_t *ticket, hipStream_t stream, unsigned tiles);
+                   uint32_t *ticket, hipStream_t stream, unsigned tiles, const uint64_t *n_dev);
 
 } // namespace sortpass
 } // namespace vgpu

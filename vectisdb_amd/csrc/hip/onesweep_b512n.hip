@@ -7,7 +7,7 @@ namespace sortpass {
 
 template void LaunchVariant<8, 16, false, 512>(bool, bool, const KeyPack &, const void *, void *,
                                                const PayloadArgs &, uint64_t, int, int, const uint32_t *, uint32_t *,
-                                               uint32_t *, hipStream_t, unsigned);
+                                               uint32_t *, hipStream_t, unsigned, const uint64_t *);
 
 } // namespace sortpass
 } // namespace vgpu

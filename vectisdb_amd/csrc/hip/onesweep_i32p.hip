@@ -6,7 +6,7 @@ namespace sortpass {
 
 template void LaunchVariant<8, 32, true, 256>(bool, bool, const KeyPack &, const void *, void *, const PayloadArgs &,
                                          uint64_t, int, int, const uint32_t *, uint32_t *, uint32_t *, hipStream_t,
-                                         unsigned);
+                                         unsigned, const uint64_t *);
 
 } // namespace sortpass
 } // namespace vgpu

@@ -21,7 +21,8 @@ template <int R, typename KeyT, bool FIRST, int NP, int ITEMS, bool PF, int BLOC
 __global__ void __launch_bounds__(BLOCK) OnesweepKernel(KeyPack pack, const KeyT *keys_in, KeyT *keys_out,
                                                              PayloadArgs pay, uint64_t n, int digit_shift,
                                                              int digit_bits, const uint32_t *ghist,
-                                                             uint32_t *tile_state, uint32_t *tile_ticket) {
+                                                             uint32_t *tile_state, uint32_t *tile_ticket,
+                                                             const uint64_t *n_dev) {
 	constexpr int RADIX = 1 << R;
 	constexpr int kWaves = BLOCK / kWave;
 	constexpr int TILE = BLOCK * ITEMS;
@@ -36,6 +37,14 @@ __global__ void __launch_bounds__(BLOCK) OnesweepKernel(KeyPack pack, const KeyT
 	__shared__ uint32_t s_scan[kWaves + 1];
 	__shared__ uint32_t s_tile;
 
+	if (n_dev != nullptr) {
+		// Row count written by an earlier kernel; the grid covers its bound. Blocks past the count leave
+		// before taking a ticket, so the tickets handed out are exactly the tiles that exist.
+		n = *n_dev;
+		if ((uint64_t)blockIdx.x * TILE >= n) {
+			return;
+		}
+	}
 	const int tid = threadIdx.x;
 	const int lane = tid & (kWave - 1);
 	const int wave = tid / kWave;
@@ -328,38 +337,39 @@ __global__ void __launch_bounds__(BLOCK) OnesweepKernel(KeyPack pack, const KeyT
 template <int R, int ITEMS, bool PF, int BLOCK, typename KeyT, int NP>
 void LaunchTyped(bool first, const KeyPack &pack, const void *keys_in, void *keys_out, const PayloadArgs &pay,
                  uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state, uint32_t *ticket,
-                 hipStream_t stream, unsigned tiles) {
+                 hipStream_t stream, unsigned tiles, const uint64_t *n_dev) {
 	if (first) {
 		hipLaunchKernelGGL((OnesweepKernel<R, KeyT, true, NP, ITEMS, PF, BLOCK>), dim3(tiles), dim3(BLOCK), 0, stream,
-		                   pack, (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state, ticket);
+		                   pack, (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state, ticket,
+		                   n_dev);
 	} else {
 		hipLaunchKernelGGL((OnesweepKernel<R, KeyT, false, NP, ITEMS, PF, BLOCK>), dim3(tiles), dim3(BLOCK), 0,
 		                   stream, pack, (const KeyT *)keys_in, (KeyT *)keys_out, pay, n, shift, bits, gh, state,
-		                   ticket);
+		                   ticket, n_dev);
 	}
 }
 
 template <int R, int ITEMS, bool PF, int BLOCK, typename KeyT>
 void LaunchKeyed(bool first, const KeyPack &pack, const void *keys_in, void *keys_out, const PayloadArgs &pay,
                  uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state, uint32_t *ticket,
-                 hipStream_t stream, unsigned tiles) {
+                 hipStream_t stream, unsigned tiles, const uint64_t *n_dev) {
 	switch (pay.n) {
 	case 0:
 		LaunchTyped<R, ITEMS, PF, BLOCK, KeyT, 0>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
-		                                   stream, tiles);
+		                                   stream, tiles, n_dev);
 		break;
 	case 1:
 		LaunchTyped<R, ITEMS, PF, BLOCK, KeyT, 1>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
-		                                   stream, tiles);
+		                                   stream, tiles, n_dev);
 		break;
 	case 2:
 		LaunchTyped<R, ITEMS, PF, BLOCK, KeyT, 2>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
-		                                   stream, tiles);
+		                                   stream, tiles, n_dev);
 		break;
 	case 3:
 	case 4:
 		LaunchTyped<R, ITEMS, PF, BLOCK, KeyT, 4>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
-		                                   stream, tiles);
+		                                   stream, tiles, n_dev);
 		break;
 	default:
 		throw HipError("radix sort: at most 4 payload columns per pass");
@@ -369,13 +379,13 @@ void LaunchKeyed(bool first, const KeyPack &pack, const void *keys_in, void *key
 template <int R, int ITEMS, bool PF, int BLOCK>
 void LaunchVariant(bool first, bool keys64, const KeyPack &pack, const void *keys_in, void *keys_out,
                    const PayloadArgs &pay, uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state,
-                   uint32_t *ticket, hipStream_t stream, unsigned tiles) {
+                   uint32_t *ticket, hipStream_t stream, unsigned tiles, const uint64_t *n_dev) {
 	if (keys64) {
 		LaunchKeyed<R, ITEMS, PF, BLOCK, u64>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket, stream,
-		                               tiles);
+		                               tiles, n_dev);
 	} else {
 		LaunchKeyed<R, ITEMS, PF, BLOCK, uint32_t>(first, pack, keys_in, keys_out, pay, n, shift, bits, gh, state, ticket,
-		                                    stream, tiles);
+		                                   stream, tiles, n_dev);
 	}
 	VGPU_HIP(hipGetLastError());
 }

@@ -177,8 +177,12 @@ __global__ void __launch_bounds__(256) KeyRangeHist8Kernel(const T *col, int wid
 
 //! One read of the keys builds the digit histogram of every pass.
 template <int R>
-__global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, PassList passes, uint32_t *hist) {
+__global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, PassList passes, uint32_t *hist,
+                                                  const uint64_t *n_dev) {
 	constexpr int RADIX = 1 << R;
+	if (n_dev != nullptr) {
+		n = *n_dev; // row count written by an earlier kernel (n was its bound)
+	}
 	constexpr int PMAX = (64 + R - 1) / R;
 	__shared__ uint32_t s_hist[PMAX * RADIX];
 	for (int i = threadIdx.x; i < PMAX * RADIX; i += blockDim.x) {
@@ -228,7 +232,7 @@ int EffectiveBlock(int R, const SortTuning &t) {
 //! items and the prefetch.
 void LaunchPass(int R, bool keys64, bool first, const KeyPack &pack, const void *keys_in, void *keys_out,
                 const PayloadArgs &pay, uint64_t n, int shift, int bits, const uint32_t *gh, uint32_t *state,
-                uint32_t *ticket, hipStream_t stream, const SortTuning &tuning) {
+                uint32_t *ticket, hipStream_t stream, const SortTuning &tuning, const uint64_t *n_dev) {
 	const int it = EffectiveItems(R, tuning.items);
 	const int block = EffectiveBlock(R, tuning);
 	const bool prefetch = tuning.prefetch != 0;
@@ -248,24 +252,24 @@ void LaunchPass(int R, bool keys64, bool first, const KeyPack &pack, const void 
 	ProfileScope prof(first ? "radix_pass_first" : "radix_pass", stream, (double)n * (key_in + kw + pay_bytes));
 	if (R == 11) {
 		LaunchVariant<11, 16, true, 256>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
-		                                 ticket, stream, tiles);
+		                                 ticket, stream, tiles, n_dev);
 	} else if (block == 512) {
 		LaunchVariant<8, 16, false, 512>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
-		                                 ticket, stream, tiles);
+		                                 ticket, stream, tiles, n_dev);
 	} else if (it == 32) {
 		if (prefetch) {
 			LaunchVariant<8, 32, true, 256>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
-			                                ticket, stream, tiles);
+			                                ticket, stream, tiles, n_dev);
 		} else {
 			LaunchVariant<8, 32, false, 256>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
-			                                 ticket, stream, tiles);
+			                                 ticket, stream, tiles, n_dev);
 		}
 	} else if (prefetch) {
 		LaunchVariant<8, 16, true, 256>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
-		                                ticket, stream, tiles);
+		                                ticket, stream, tiles, n_dev);
 	} else {
 		LaunchVariant<8, 16, false, 256>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
-		                                 ticket, stream, tiles);
+		                                 ticket, stream, tiles, n_dev);
 	}
 }
 
@@ -397,7 +401,8 @@ bool PlanKeyPack(const void *const *cols, const int *vtypes, int ncols, const ui
 }
 
 void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
-                    uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream) {
+                    uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream,
+                    const uint64_t *n_dev) {
 	if (n >= (1ull << 30)) {
 		throw HipError("radix sort: at most 2^30 - 1 rows per sort call");
 	}
@@ -456,7 +461,7 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 		pbuf[p][0] = scratch.Alloc(n * payload_width[p]);
 		pbuf[p][1] = P > 1 ? scratch.Alloc(n * payload_width[p]) : nullptr;
 	}
-	bool aligned8 = R == 8 && tuning.hist8 != nullptr;
+	bool aligned8 = R == 8 && tuning.hist8 != nullptr && n_dev == nullptr;
 	for (int p = 0; p < P; p++) {
 		aligned8 = aligned8 && passes.bits[p] == 8 && passes.shift[p] == 8 * p;
 	}
@@ -476,9 +481,11 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 		ProfileScope prof("radix_hist", stream, bytes);
 		uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 2048);
 		if (R == 8) {
-			hipLaunchKernelGGL(HistKernel<8>, dim3((unsigned)blocks), dim3(256), 0, stream, pack, n, passes, hist);
+			hipLaunchKernelGGL(HistKernel<8>, dim3((unsigned)blocks), dim3(256), 0, stream, pack, n, passes, hist,
+			                   n_dev);
 		} else {
-			hipLaunchKernelGGL(HistKernel<11>, dim3((unsigned)blocks), dim3(256), 0, stream, pack, n, passes, hist);
+			hipLaunchKernelGGL(HistKernel<11>, dim3((unsigned)blocks), dim3(256), 0, stream, pack, n, passes, hist,
+			                   n_dev);
 		}
 		VGPU_HIP(hipGetLastError());
 	}
@@ -501,7 +508,7 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 		const uint32_t *gh = hist + (uint64_t)pass * RADIX;
 		bool first = pass == 0;
 		LaunchPass(R, keys64, first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass], passes.bits[pass], gh,
-		           st, tickets + pass, stream, tuning);
+		           st, tickets + pass, stream, tuning, n_dev);
 		cur_keys = kbuf[slot];
 		for (int p = 0; p < npayload; p++) {
 			cur_pay[p] = pbuf[p][slot];

@@ -281,7 +281,7 @@ template <typename KeyT, int NP, typename SG>
 __global__ void __launch_bounds__(kRedBlock)
     SegReduceKernel(const KeyT *keys, uint64_t n, PayloadPtrs pay, CompDesc d, AggOutDesc o, KeyOutDesc ko,
                     const u64 *tile_prefix, const u64 *tile_carry, u64 *lead_state, u64 *tail_state, u64 *tail_gid,
-                    uint64_t ntiles) {
+                    uint64_t ntiles, const uint64_t *n_dev) {
 	__shared__ u64 s_scan[kRedWaves + 1];
 	__shared__ u64 s_wave_state[kRedWaves][2 * kMaxComponents];
 	__shared__ int s_wave_flag[kRedWaves];
@@ -292,6 +292,12 @@ __global__ void __launch_bounds__(kRedBlock)
 	__shared__ KeyT s_keys[kStage];
 	__shared__ KeyT s_prev;
 	__shared__ u64 s_pay[NP > 0 ? NP : 1][kStage]; // raw bits, zero-extended
+	if (n_dev != nullptr) {
+		n = *n_dev;
+		if ((uint64_t)blockIdx.x * TILE >= n) {
+			return;
+		}
+	}
 
 	const int tid = threadIdx.x;
 	const int lane = tid & (kWave - 1);
@@ -515,7 +521,11 @@ __global__ void __launch_bounds__(kRedBlock)
 constexpr int kHeadLaneRows = 32;
 template <typename KeyT>
 __global__ void __launch_bounds__(256) HeadCountKernel(const KeyT *keys, uint64_t n, uint64_t tile_rows,
-                                                      uint64_t ntiles, uint32_t *counts) {
+                                                      uint64_t ntiles, uint32_t *counts, const uint64_t *n_dev) {
+	if (n_dev != nullptr) { // row count written by an earlier kernel; n was its bound
+		n = *n_dev;
+		ntiles = (n + tile_rows - 1) / tile_rows;
+	}
 	const uint64_t tile = (uint64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
 	if (tile >= ntiles) {
 		return;
@@ -550,8 +560,15 @@ __global__ void __launch_bounds__(256) HeadCountKernel(const KeyT *keys, uint64_
 
 //! Level 1: exclusive prefix of the head counts inside each run of kScanTiles tiles, and each run's total.
 __global__ void __launch_bounds__(kScanBlock) TilePrefixKernel(const uint32_t *counts, uint64_t ntiles,
-                                                               u64 *prefix, u64 *block_totals) {
+                                                               u64 *prefix, u64 *block_totals,
+                                                               const uint64_t *n_dev, uint64_t tile_rows) {
 	__shared__ u64 s_scan[kScanBlock / kWave + 1];
+	if (n_dev != nullptr) {
+		ntiles = (*n_dev + tile_rows - 1) / tile_rows;
+		if ((uint64_t)blockIdx.x * kScanTiles >= ntiles) {
+			return;
+		}
+	}
 	const uint64_t t0 = (uint64_t)blockIdx.x * kScanTiles + (uint64_t)threadIdx.x * kScanPerThread;
 	uint32_t c[kScanPerThread];
 	u64 mine = 0;
@@ -575,8 +592,12 @@ __global__ void __launch_bounds__(kScanBlock) TilePrefixKernel(const uint32_t *c
 }
 
 //! Level 2 (one block): block_totals becomes each run's exclusive carry; *groups_out the total.
-__global__ void __launch_bounds__(kScanBlock) TileCarryKernel(u64 *block_totals, uint64_t nblocks, u64 *groups_out) {
+__global__ void __launch_bounds__(kScanBlock) TileCarryKernel(u64 *block_totals, uint64_t nblocks, u64 *groups_out,
+                                                              const uint64_t *n_dev, uint64_t tile_rows) {
 	__shared__ u64 s_scan[kScanBlock / kWave + 1];
+	if (n_dev != nullptr) {
+		nblocks = ((*n_dev + tile_rows - 1) / tile_rows + kScanTiles - 1) / kScanTiles;
+	}
 	u64 carry = 0;
 	for (uint64_t base = 0; base < nblocks; base += kScanBlock) {
 		const uint64_t i = base + threadIdx.x;
@@ -601,7 +622,11 @@ __global__ void __launch_bounds__(kScanBlock) TileCarryKernel(u64 *block_totals,
 template <typename KeyT>
 __global__ void __launch_bounds__(256) RleHeadsKernel(const KeyT *keys, uint64_t n, KeyOutDesc ko, u64 *starts,
                                                      const u64 *tile_prefix, const u64 *tile_carry,
-                                                     uint64_t ntiles) {
+                                                     uint64_t ntiles, const uint64_t *n_dev) {
+	if (n_dev != nullptr) {
+		n = *n_dev;
+		ntiles = (n + (uint64_t)kWave * kHeadLaneRows - 1) / ((uint64_t)kWave * kHeadLaneRows);
+	}
 	const uint64_t tile = (uint64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
 	if (tile >= ntiles) {
 		return;
@@ -652,7 +677,11 @@ __global__ void __launch_bounds__(256) RleHeadsKernel(const KeyT *keys, uint64_t
 	}
 }
 
-__global__ void RleCountKernel(const u64 *starts, const u64 *groups, uint64_t n, AggOutDesc o) {
+__global__ void RleCountKernel(const u64 *starts, const u64 *groups, uint64_t n, AggOutDesc o,
+                               const uint64_t *n_dev) {
+	if (n_dev != nullptr) {
+		n = *n_dev;
+	}
 	const u64 g_total = *groups;
 	const u64 stride = (u64)gridDim.x * blockDim.x;
 	for (u64 g = (u64)blockIdx.x * blockDim.x + threadIdx.x; g < g_total; g += stride) {
@@ -666,7 +695,7 @@ __global__ void RleCountKernel(const u64 *starts, const u64 *groups, uint64_t n,
 
 template <typename KeyT>
 void LaunchRle(const SortedRun &run, uint64_t n, uint64_t capacity, const AggOutDesc &od, const KeyOutDesc &ko,
-               u64 *groups_device, Scratch &scratch, hipStream_t stream) {
+               u64 *groups_device, Scratch &scratch, hipStream_t stream, const uint64_t *n_dev) {
 	constexpr uint64_t kTile = (uint64_t)kWave * kHeadLaneRows;
 	const uint64_t ntiles = (n + kTile - 1) / kTile;
 	const uint64_t nscan = (ntiles + kScanTiles - 1) / kScanTiles;
@@ -681,19 +710,20 @@ void LaunchRle(const SortedRun &run, uint64_t n, uint64_t capacity, const AggOut
 	{
 		ProfileScope prof("seg_count", stream, (double)n * sizeof(KeyT));
 		hipLaunchKernelGGL((HeadCountKernel<KeyT>), dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, stream,
-		                   (const KeyT *)run.keys, n, kTile, ntiles, tile_heads);
+		                   (const KeyT *)run.keys, n, kTile, ntiles, tile_heads, n_dev);
 		VGPU_HIP(hipGetLastError());
 		hipLaunchKernelGGL(TilePrefixKernel, dim3((unsigned)nscan), dim3(kScanBlock), 0, stream, tile_heads, ntiles,
-		                   tile_prefix, tile_carry);
+		                   tile_prefix, tile_carry, n_dev, kTile);
 		VGPU_HIP(hipGetLastError());
-		hipLaunchKernelGGL(TileCarryKernel, dim3(1), dim3(kScanBlock), 0, stream, tile_carry, nscan, groups_device);
+		hipLaunchKernelGGL(TileCarryKernel, dim3(1), dim3(kScanBlock), 0, stream, tile_carry, nscan, groups_device,
+		                   n_dev, kTile);
 		VGPU_HIP(hipGetLastError());
 		hipLaunchKernelGGL((RleHeadsKernel<KeyT>), dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, stream,
-		                   (const KeyT *)run.keys, n, ko, starts, tile_prefix, tile_carry, ntiles);
+		                   (const KeyT *)run.keys, n, ko, starts, tile_prefix, tile_carry, ntiles, n_dev);
 		VGPU_HIP(hipGetLastError());
 	}
 	const unsigned grid = (unsigned)std::min<uint64_t>((std::max<uint64_t>(capacity, 1) + 255) / 256, 8192);
-	hipLaunchKernelGGL(RleCountKernel, dim3(grid), dim3(256), 0, stream, starts, groups_device, n, od);
+	hipLaunchKernelGGL(RleCountKernel, dim3(grid), dim3(256), 0, stream, starts, groups_device, n, od, n_dev);
 	VGPU_HIP(hipGetLastError());
 }
 
@@ -706,12 +736,19 @@ constexpr int kFixWaves = kFixBlock / kWave;
 //! scan: wave shuffles, then waves right to left through LDS. Tiles past the input act as a stop.
 __global__ void __launch_bounds__(kFixBlock) ChunkRunKernel(CompDesc d, const uint32_t *tile_heads,
                                                             const u64 *lead_state, u64 *run_state,
-                                                            uint8_t *run_open, uint64_t ntiles) {
+                                                            uint8_t *run_open, uint64_t ntiles,
+                                                            const uint64_t *n_dev, uint64_t tile_rows) {
 	__shared__ u64 s_state[kFixWaves][2 * kMaxComponents];
 	__shared__ int s_flag[kFixWaves];
 	const int tid = threadIdx.x;
 	const int lane = tid & (kWave - 1);
 	const int wave = tid / kWave;
+	if (n_dev != nullptr) {
+		ntiles = (*n_dev + tile_rows - 1) / tile_rows;
+		if ((uint64_t)blockIdx.x * kFixBlock >= ntiles) {
+			return;
+		}
+	}
 	const uint64_t u = (uint64_t)blockIdx.x * kFixBlock + tid;
 	const bool in_range = u < ntiles;
 	bool F = !in_range || tile_heads[u] > 0; // stop flag
@@ -764,7 +801,11 @@ __global__ void __launch_bounds__(kFixBlock) ChunkRunKernel(CompDesc d, const ui
 //! following chunks (a few steps even for a group spanning thousands of tiles), then writes the group.
 __global__ void __launch_bounds__(kFixBlock) TailFinishKernel(CompDesc d, AggOutDesc o, const u64 *run_state,
                                                               const uint8_t *run_open, const u64 *tail_state,
-                                                              const u64 *tail_gid, uint64_t ntiles) {
+                                                              const u64 *tail_gid, uint64_t ntiles,
+                                                              const uint64_t *n_dev, uint64_t tile_rows) {
+	if (n_dev != nullptr) {
+		ntiles = (*n_dev + tile_rows - 1) / tile_rows;
+	}
 	for (uint64_t T = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; T < ntiles;
 	     T += (uint64_t)gridDim.x * blockDim.x) {
 		if (tail_gid[T] == ~0ull) {
@@ -789,7 +830,7 @@ __global__ void __launch_bounds__(kFixBlock) TailFinishKernel(CompDesc d, AggOut
 template <typename KeyT, int NP, typename SG = DynSig>
 void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, const CompDesc &cd,
                   const AggOutDesc &od, const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch,
-                  hipStream_t stream) {
+                  hipStream_t stream, const uint64_t *n_dev) {
 	constexpr int kItems = RedItems<SG>();
 	constexpr uint64_t kTile = (uint64_t)kRedBlock * kItems;
 	const uint64_t ntiles = (n + kTile - 1) / kTile;
@@ -805,12 +846,13 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 		static_assert(kTile == (uint64_t)kWave * kHeadLaneRows, "one wave per reduce tile");
 		ProfileScope prof("seg_heads", stream, (double)n * (run.keys ? (run.keys64 ? 8.0 : 4.0) : 0.0));
 		hipLaunchKernelGGL((HeadCountKernel<KeyT>), dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, stream,
-		                   (const KeyT *)run.keys, n, kTile, ntiles, tile_heads);
+		                   (const KeyT *)run.keys, n, kTile, ntiles, tile_heads, n_dev);
 		VGPU_HIP(hipGetLastError());
 		hipLaunchKernelGGL(TilePrefixKernel, dim3((unsigned)nscan), dim3(kScanBlock), 0, stream, tile_heads, ntiles,
-		                   tile_prefix, tile_carry);
+		                   tile_prefix, tile_carry, n_dev, kTile);
 		VGPU_HIP(hipGetLastError());
-		hipLaunchKernelGGL(TileCarryKernel, dim3(1), dim3(kScanBlock), 0, stream, tile_carry, nscan, groups_device);
+		hipLaunchKernelGGL(TileCarryKernel, dim3(1), dim3(kScanBlock), 0, stream, tile_carry, nscan, groups_device,
+		                   n_dev, kTile);
 		VGPU_HIP(hipGetLastError());
 	}
 	double bytes = (double)n * (run.keys ? (run.keys64 ? 8.0 : 4.0) : 0.0);
@@ -821,7 +863,7 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 		ProfileScope prof("seg_reduce", stream, bytes);
 		hipLaunchKernelGGL((SegReduceKernel<KeyT, NP, SG>), dim3((unsigned)ntiles), dim3(kRedBlock), 0, stream,
 		                   (const KeyT *)run.keys, n, pay, cd, od, ko, tile_prefix, tile_carry, lead_state, tail_state,
-		                   tail_gid, ntiles);
+		                   tail_gid, ntiles, n_dev);
 		VGPU_HIP(hipGetLastError());
 	}
 	{
@@ -830,10 +872,10 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 		auto *run_state = static_cast<u64 *>(scratch.Alloc(ntiles * state_words * sizeof(u64)));
 		auto *run_open = static_cast<uint8_t *>(scratch.Alloc(ntiles));
 		hipLaunchKernelGGL(ChunkRunKernel, dim3((unsigned)nchunks), dim3(kFixBlock), 0, stream, cd, tile_heads,
-		                   lead_state, run_state, run_open, ntiles);
+		                   lead_state, run_state, run_open, ntiles, n_dev, kTile);
 		VGPU_HIP(hipGetLastError());
 		hipLaunchKernelGGL(TailFinishKernel, dim3((unsigned)nchunks), dim3(kFixBlock), 0, stream, cd, od, run_state,
-		                   run_open, tail_state, tail_gid, ntiles);
+		                   run_open, tail_state, tail_gid, ntiles, n_dev, kTile);
 		VGPU_HIP(hipGetLastError());
 	}
 }
@@ -864,31 +906,32 @@ bool SigMatches(const CompDesc &cd) {
 
 template <typename KeyT>
 void DispatchNP(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, const CompDesc &cd, const AggOutDesc &od,
-                const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch, hipStream_t stream) {
+                const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch, hipStream_t stream,
+                const uint64_t *n_dev) {
 	if (pay.n == 2 && SigMatches<SigCountSumMinMaxDd>(cd)) {
-		LaunchReduce<KeyT, 2, SigCountSumMinMaxDd>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		LaunchReduce<KeyT, 2, SigCountSumMinMaxDd>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
 		return;
 	}
 	if (pay.n == 1 && SigMatches<SigCountSumI32>(cd)) {
-		LaunchReduce<KeyT, 1, SigCountSumI32>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		LaunchReduce<KeyT, 1, SigCountSumI32>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
 		return;
 	}
 	if (pay.n == 1 && SigMatches<SigCountSumI64>(cd)) {
-		LaunchReduce<KeyT, 1, SigCountSumI64>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		LaunchReduce<KeyT, 1, SigCountSumI64>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
 		return;
 	}
 	if (pay.n == 1 && SigMatches<SigCountSumDd>(cd)) {
-		LaunchReduce<KeyT, 1, SigCountSumDd>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		LaunchReduce<KeyT, 1, SigCountSumDd>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
 		return;
 	}
 	if (pay.n == 0) {
-		LaunchReduce<KeyT, 0>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		LaunchReduce<KeyT, 0>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
 	} else if (pay.n == 1) {
-		LaunchReduce<KeyT, 1>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		LaunchReduce<KeyT, 1>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
 	} else if (pay.n == 2) {
-		LaunchReduce<KeyT, 2>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		LaunchReduce<KeyT, 2>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
 	} else if (pay.n <= 4) {
-		LaunchReduce<KeyT, 4>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		LaunchReduce<KeyT, 4>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
 	} else {
 		throw HipError("segmented reduce: at most 4 payload columns");
 	}
@@ -897,7 +940,7 @@ void DispatchNP(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, const 
 } // namespace
 
 void SegmentedReduce(const SortedRun &run, uint64_t n, const ReduceSpec &spec, uint64_t *groups_device,
-                     Scratch &scratch, hipStream_t stream) {
+                     Scratch &scratch, hipStream_t stream, const uint64_t *n_dev) {
 	if (spec.ncomp > kMaxComponents || spec.naggs > 16) {
 		throw HipError("segmented reduce: too many aggregates");
 	}
@@ -948,16 +991,16 @@ void SegmentedReduce(const SortedRun &run, uint64_t n, const ReduceSpec &spec, u
 		// Groups never outnumber rows, so n bounds the first-row buffer when no capacity is given.
 		const uint64_t cap = spec.capacity == 0 ? n : std::min<uint64_t>(spec.capacity, n);
 		if (run.keys64) {
-			LaunchRle<u64>(run, n, cap, od, ko, groups_device, scratch, stream);
+			LaunchRle<u64>(run, n, cap, od, ko, groups_device, scratch, stream, n_dev);
 		} else {
-			LaunchRle<uint32_t>(run, n, cap, od, ko, groups_device, scratch, stream);
+			LaunchRle<uint32_t>(run, n, cap, od, ko, groups_device, scratch, stream, n_dev);
 		}
 		return;
 	}
 	if (run.keys64) {
-		DispatchNP<u64>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		DispatchNP<u64>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
 	} else {
-		DispatchNP<uint32_t>(run, n, pay, cd, od, ko, groups_device, scratch, stream);
+		DispatchNP<uint32_t>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
 	}
 }
 

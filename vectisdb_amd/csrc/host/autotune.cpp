@@ -110,15 +110,17 @@ std::string SortAutotuneReport() {
 }
 
 void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
-                uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream) {
+                uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream,
+                const uint64_t *n_dev) {
 	SortTuning t = tuning;
 	if (t.fixed) {
-		SortPackedWith(pack, payload_in, payload_width, npayload, n, t, out, scratch, stream);
+		SortPackedWith(pack, payload_in, payload_width, npayload, n, t, out, scratch, stream, n_dev);
 		return;
 	}
 	StaticDefault(t, npayload);
-	if (!g_autotune.load() || n < kTuneMinRows) {
-		SortPackedWith(pack, payload_in, payload_width, npayload, n, t, out, scratch, stream);
+	// a device-side row count is never timed (the host does not wait on it)
+	if (!g_autotune.load() || n < kTuneMinRows || n_dev != nullptr) {
+		SortPackedWith(pack, payload_in, payload_width, npayload, n, t, out, scratch, stream, n_dev);
 		return;
 	}
 	const int bits = std::max(pack.total_bits, 1);

@@ -32,6 +32,45 @@ std::vector<DevCol> ToDevCols(const vgpu_column *cols, int32_t n) {
 	return out;
 }
 
+BatchedGroupByArgs MakeBatchedArgs(const vgpu_column *cols, int32_t ncols, uint64_t rows, const uint64_t *offsets,
+                                   int32_t nsegments, const vgpu_expr *preds, int32_t npreds, int32_t key_col,
+                                   const int32_t *value_cols, int32_t nvalues, const vgpu_agg *aggs, int32_t naggs,
+                                   void *segment_out, void *key_out, void *const *agg_out, uint64_t capacity,
+                                   uint64_t *segment_group_offsets) {
+	if (ncols < 0 || nsegments <= 0 || nvalues < 0 || naggs < 0 || npreds < 0) {
+		throw std::runtime_error("batched group by: bad counts");
+	}
+	BatchedGroupByArgs a;
+	a.cols = ToDevCols(cols, ncols);
+	a.rows = rows;
+	a.seg_offsets_dev = offsets;
+	a.nseg = static_cast<uint32_t>(nsegments);
+	for (int32_t i = 0; i < npreds; i++) {
+		if (preds[i].body == nullptr) {
+			throw std::runtime_error("batched group by: predicate body is NULL");
+		}
+		FusedExpr f;
+		f.body = preds[i].body;
+		f.out_type = preds[i].output_type;
+		f.slots.assign(preds[i].inputs, preds[i].inputs + preds[i].ninputs);
+		a.preds.push_back(f);
+	}
+	a.key_col = key_col;
+	a.value_cols.assign(value_cols, value_cols + nvalues);
+	for (int32_t i = 0; i < naggs; i++) {
+		if (aggs[i].kind != AGG_COUNT_STAR && (aggs[i].input < 0 || aggs[i].input >= nvalues)) {
+			throw std::runtime_error("batched group by: aggregate input out of range");
+		}
+		a.aggs.push_back({aggs[i].kind, aggs[i].input, aggs[i].output_type});
+	}
+	a.seg_out = segment_out;
+	a.key_out = key_out;
+	a.agg_out.assign(agg_out, agg_out + naggs);
+	a.capacity = capacity;
+	a.seg_group_offsets_host = segment_group_offsets;
+	return a;
+}
+
 } // namespace vgpu
 
 using namespace vgpu;
@@ -99,46 +138,6 @@ int vgpu_group_by(const vgpu_column *keys, int32_t nkeys, const vgpu_column *val
 	});
 }
 
-namespace {
-BatchedGroupByArgs MakeBatchedArgs(const vgpu_column *cols, int32_t ncols, uint64_t rows, const uint64_t *offsets,
-                                   int32_t nsegments, const vgpu_expr *preds, int32_t npreds, int32_t key_col,
-                                   const int32_t *value_cols, int32_t nvalues, const vgpu_agg *aggs, int32_t naggs,
-                                   void *segment_out, void *key_out, void *const *agg_out, uint64_t capacity,
-                                   uint64_t *segment_group_offsets) {
-	if (ncols < 0 || nsegments <= 0 || nvalues < 0 || naggs < 0 || npreds < 0) {
-		throw std::runtime_error("batched group by: bad counts");
-	}
-	BatchedGroupByArgs a;
-	a.cols = ToDevCols(cols, ncols);
-	a.rows = rows;
-	a.seg_offsets_dev = offsets;
-	a.nseg = static_cast<uint32_t>(nsegments);
-	for (int32_t i = 0; i < npreds; i++) {
-		if (preds[i].body == nullptr) {
-			throw std::runtime_error("batched group by: predicate body is NULL");
-		}
-		FusedExpr f;
-		f.body = preds[i].body;
-		f.out_type = preds[i].output_type;
-		f.slots.assign(preds[i].inputs, preds[i].inputs + preds[i].ninputs);
-		a.preds.push_back(f);
-	}
-	a.key_col = key_col;
-	a.value_cols.assign(value_cols, value_cols + nvalues);
-	for (int32_t i = 0; i < naggs; i++) {
-		if (aggs[i].kind != AGG_COUNT_STAR && (aggs[i].input < 0 || aggs[i].input >= nvalues)) {
-			throw std::runtime_error("batched group by: aggregate input out of range");
-		}
-		a.aggs.push_back({aggs[i].kind, aggs[i].input, aggs[i].output_type});
-	}
-	a.seg_out = segment_out;
-	a.key_out = key_out;
-	a.agg_out.assign(agg_out, agg_out + naggs);
-	a.capacity = capacity;
-	a.seg_group_offsets_host = segment_group_offsets;
-	return a;
-}
-} // namespace
 
 int vgpu_batched_filter_group_by(const vgpu_column *cols, int32_t ncols, uint64_t rows,
                                  const uint64_t *segment_offsets, int32_t nsegments, const vgpu_expr *preds,

@@ -166,7 +166,7 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
                        const std::vector<AggReq> &aggs, uint64_t rows, const KeyPack &pack, uint64_t capacity,
                        const OutAlloc &out_alloc, std::vector<void *> &key_out, std::vector<void *> &agg_out,
                        const SortTuning &tuning, Scratch &scratch, hipStream_t stream, bool read_groups,
-                       uint64_t *groups_device) {
+                       uint64_t *groups_device, const uint64_t *rows_dev) {
 	if (aggs.size() > 16) {
 		throw std::runtime_error("gpu_groupby: at most 16 aggregates per GROUP BY");
 	}
@@ -272,12 +272,12 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
 			run.payload[p] = const_cast<void *>(payload_in[p]);
 			run.payload_width[p] = payload_width[p];
 		}
-		SegmentedReduce(run, rows, spec, groups_dev, scratch, stream);
+		SegmentedReduce(run, rows, spec, groups_dev, scratch, stream, rows_dev);
 	} else {
 		SortedRun run;
 		SortPacked(pack, payload_in.data(), payload_width.data(), (int)payload_in.size(), rows, tuning, run, scratch,
-		           stream);
-		SegmentedReduce(run, rows, spec, groups_dev, scratch, stream);
+		           stream, rows_dev);
+		SegmentedReduce(run, rows, spec, groups_dev, scratch, stream, rows_dev);
 	}
 	if (!read_groups) {
 		return 0;

@@ -48,12 +48,14 @@ uint64_t PlanGroupByRanges(const std::vector<DevCol> &keys, uint64_t rows, const
 
 //! Sort-based GROUP BY. Key outputs (one per key column, key type) and aggregate outputs (out_type) are
 //! allocated through `out_alloc` with `capacity` rows each. Returns the group count when `read_groups`
-//! (synchronizes), else writes it to *groups_device and returns 0.
+//! (synchronizes), else writes it to *groups_device and returns 0. With rows_dev the row count is read on
+//! the device (an earlier kernel on `stream` wrote it; `rows` and the columns' row counts are its bound),
+//! so a call with read_groups = false makes no host read and no synchronizing call.
 uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol> &values,
                        const std::vector<AggReq> &aggs, uint64_t rows, const KeyPack &pack, uint64_t capacity,
                        const OutAlloc &out_alloc, std::vector<void *> &key_out, std::vector<void *> &agg_out,
                        const SortTuning &tuning, Scratch &scratch, hipStream_t stream, bool read_groups,
-                       uint64_t *groups_device = nullptr);
+                       uint64_t *groups_device = nullptr, const uint64_t *rows_dev = nullptr);
 
 // ------------------------------------------------------------------ join
 
