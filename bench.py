@@ -457,16 +457,73 @@ def bench_batched(torch, capi, dev, steps, nq=512):
         run(True)
     torch.cuda.synchronize()
     tp = (time.perf_counter() - t0) / psteps
+    graph = bench_batched_graph(torch, capi, cols, offsets, sizes, run, steps)
     return {"operator": f"batched filter + GROUP BY, {nq} row groups (log-uniform 2K..4M rows, "
                         f"16..1M groups, Zipf(1.1) keys per group), {rows} rows",
             "rows": rows, "queries": nq, "batched_ms": round(tb * 1e3, 3), "per_query_ms": round(tp * 1e3, 3),
             "batched_qps": round(nq / tb, 1), "per_query_qps": round(nq / tp, 1),
             "qps_ratio": round(tp / tb, 2), "rows_per_s": round(rows / tb, 1),
+            "graph": graph,
             "results_identical": True, "checked": True,
             "cpu_baseline": dict(cpu_line(nq, sec, threads, f"vo_batched_filter_group_by_mt: the {nq} queries one "
                                                             f"after another (filter, then the multithreaded "
                                                             f"GROUP BY) over the same {rows} rows", unit="queries/s"),
                                  rows_per_s=round(rows / sec, 1))}
+
+
+def _same_batched(a, b):
+    """Bitwise equality of two batched results (segments, keys, every aggregate, group offsets)."""
+    return (a[3] == b[3] and bool(a[0].equal(b[0])) and bool(a[1].equal(b[1]))
+            and all(bool(x.equal(y)) for x, y in zip(a[2], b[2])))
+
+
+def bench_batched_graph(torch, capi, cols, offsets, sizes, eager_run, steps):
+    """The same row groups through a prepared plan (vgpu_batched_plan_*): one graph launch and one wait per
+    batch. Full size: the 512 row groups as row ranges, results checked bit for bit against the eager
+    batched pipeline. Small batches: the first 2K..64K rows (log-uniform) of each row group, where the eager
+    path's ~20 launches and 3 host round trips per batch weigh most, against the eager row-range path."""
+    import numpy as np
+    from vectisdb_amd import workloads
+    nq = len(sizes)
+    rows = int(sizes.sum())
+    preds, aggs = workloads.BATCHED_PREDS, workloads.AGGS7
+    starts = [int(x) for x in offsets.cpu().numpy()[:-1]]
+    lens = [int(x) for x in sizes]
+    out = {}
+    with capi.BatchedPlan(cols, nq, rows, preds, 2, [3, 4], aggs) as plan:
+        info = plan.info()
+        expect(_same_batched(plan.run(starts, lens), eager_run(False)), "graph and eager batched results")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.run(starts, lens)
+        tg = (time.perf_counter() - t0) / steps
+        out.update({"graph_ms": round(tg * 1e3, 3), "graph_qps": round(nq / tg, 1),
+                    "launches_per_batch": 1, "host_waits_per_batch": 1,
+                    "graph_nodes": {"kernels": info["kernel_nodes"], "copies": info["copy_nodes"],
+                                    "other": info["other_nodes"]},
+                    "plan_footprint_bytes": info["footprint_bytes"]})
+    rng = np.random.default_rng(5)
+    small = [int(min(s, x)) for s, x in zip(sizes, np.exp(rng.uniform(np.log(2048), np.log(65536), nq)))]
+    srows = int(sum(small))
+    eager_small = lambda: capi.batched_ranges_filter_group_by(cols, starts, small, preds, 2, [3, 4], aggs)
+    with capi.BatchedPlan(cols, nq, nq * 65536, preds, 2, [3, 4], aggs) as plan:
+        expect(_same_batched(plan.run(starts, small), eager_small()), "graph and eager small-batch results")
+        reps = max(10, 4 * steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            eager_small()
+        torch.cuda.synchronize()
+        te = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.run(starts, small)
+        tg = (time.perf_counter() - t0) / reps
+    out["small_batches"] = {"rows": srows, "queries": nq, "eager_ms": round(te * 1e3, 3),
+                            "graph_ms": round(tg * 1e3, 3), "graph_qps": round(nq / tg, 1),
+                            "graph_vs_eager": round(te / tg, 2)}
+    return out
 
 
 def bench_served(torch, capi, dev, nq=400, scale=1.0):
