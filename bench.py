@@ -527,15 +527,18 @@ def bench_batched_graph(torch, capi, cols, offsets, sizes, eager_run, steps):
 
 
 def bench_served(torch, capi, dev, nq=400, scale=1.0):
-    """Served form against the plain per-query path on the same GPU, same query stream.
+    """Served form against the plain per-query path on the same GPU, same query stream, and the steps in
+    between, so each one's share of the gain is visible.
 
-    Plain path: one query at a time (batching off, one executor), each query on the per-query operators
-    (fused filter kernel; filter + sort-based GROUP BY; sort-merge join of its probe range with the build
-    side). Served form: 4 executors on 4 HIP streams, continuous batching, footprint admission, the build
-    side reduced once and resident. Each form first runs the stream as a burst (capacity = queries / time);
-    the plain path is then run open-loop (Poisson arrivals) at 70% of its capacity, which gives its p99; the
-    served form is run open-loop at rising rates, and the highest rate whose p99 stays at or below the
-    plain path's p99 is reported beside it."""
+    plain: one query at a time (batching off, one executor), each query on the per-query operators (fused
+    filter kernel; filter + sort-based GROUP BY; sort-merge join of its probe range with the build side).
+    plain_resident_join: the same, but a join query probes the build side reduced once (the resident index).
+    executors4: four executors on four HIP streams, batching off, resident index.
+    served: four executors, continuous batching, footprint admission, resident index.
+    Each configuration first runs the stream as a burst (capacity = queries / time). The plain path is then
+    run open loop (Poisson arrivals) at 70% of its capacity, which gives the reference p99; every other
+    configuration is run at that same offered rate (per-template p50/p99 beside plain's) and then at rising
+    rates, and the highest rate whose p99 stays at or below plain's is its QPS at equal p99."""
     import numpy as np
     from vectisdb_amd.workloads import cfg0_exprs, make_served_tables, served_queries
     tables, total_bytes = make_served_tables(torch, dev, scale)
@@ -549,10 +552,11 @@ def bench_served(torch, capi, dev, nq=400, scale=1.0):
     preds, projs = cfg0_exprs()
     aggs = [(capi.COUNT_STAR, -1, capi.INT64), (capi.SUM, 0, capi.HUGEINT), (capi.MIN, 0, capi.INT64),
             (capi.MAX, 0, capi.INT64), (capi.SUM, 1, capi.FLOAT64), (capi.AVG, 1, capi.FLOAT64)]
+    names = ("filter", "group_by", "join", "cross")
 
-    def server(batching):
-        s = capi.Server(slots=4 if batching else 1, batching=batching, max_batch=512,
-                        max_batch_rows=int(os.environ.get("VGPU_SERVED_BATCH_ROWS", 256 << 20)))
+    def server(**opts):
+        s = capi.Server(max_batch=512, max_batch_rows=int(os.environ.get("VGPU_SERVED_BATCH_ROWS", 256 << 20)),
+                        **opts)
         t = [s.add_filter_project(tables["filter"], preds, projs),
              s.add_group_by(tables["group"], [], 0, [1, 2], aggs),
              s.add_join_agg(tables["probe"], tables["build"], tables["payload"]),
@@ -565,7 +569,7 @@ def bench_served(torch, capi, dev, nq=400, scale=1.0):
     def point(srv, tmpl, rate, seed):
         lat, bs, el = srv.run_open_loop(tmpl, starts, sizes, arrivals(rate, seed))
         per = {}
-        for k, name in enumerate(("filter", "group_by", "join", "cross")):
+        for k, name in enumerate(names):
             m = kinds == k
             per[name] = [round(float(np.percentile(lat[m], 50)) * 1e3, 3),
                          round(float(np.percentile(lat[m], 99)) * 1e3, 3)]
@@ -574,43 +578,62 @@ def bench_served(torch, capi, dev, nq=400, scale=1.0):
                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                 "mean_batch": round(float(bs.mean()), 2), "p50_p99_ms_by_template": per}
 
+    def capacity(srv, tmpl):
+        zero = np.zeros(nq)
+        # warm-up: the whole stream once (kernels compiled, the device pool grown to the working set)
+        srv.run_open_loop(tmpl, starts, sizes, zero)
+        _, _, el = srv.run_open_loop(tmpl, starts, sizes, zero)
+        return nq / el
+
     out = {"operator": f"served form, {nq} queries (filter/project, GROUP BY, join + aggregate, cross product "
                        f"templates, 10K..50M rows each) over {total_bytes / 1e9:.1f} GB resident",
            "queries": nq, "resident_bytes": total_bytes}
-    plain, ptm = server(False)
-    zero = np.zeros(nq)
-    # Warm-up: the whole stream once (kernels compiled, the device pool grown to the working set), then
-    # the burst that is timed.
-    plain.run_open_loop(ptm, starts, sizes, zero)
-    _, _, el = plain.run_open_loop(ptm, starts, sizes, zero)
-    cap_plain = nq / el
+    plain, ptm = server(slots=1, batching=False)
+    cap_plain = capacity(plain, ptm)
     p_pt = point(plain, ptm, 0.7 * cap_plain, 11)
     plain.close()
     print(f"[bench] served: plain capacity {cap_plain:.1f} QPS, at 70%: {p_pt}", file=sys.stderr, flush=True)
-    srv, stm = server(True)
-    srv.run_open_loop(stm, starts, sizes, zero)
-    _, _, el = srv.run_open_loop(stm, starts, sizes, zero)
-    cap_served = nq / el
-    print(f"[bench] served: batched capacity {cap_served:.1f} QPS", file=sys.stderr, flush=True)
-    ladder = []
-    best = None
-    for m in (1, 1.5, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20):
-        rate = m * p_pt["offered_qps"]
-        if rate > 1.2 * cap_served:
-            break
-        pt = point(srv, stm, rate, 11)
-        ladder.append(pt)
-        print(f"[bench] served: {pt}", file=sys.stderr, flush=True)
-        if pt["p99_ms"] <= p_pt["p99_ms"]:
-            best = pt
-        else:
-            break
-    out["stats"] = srv.stats()
-    srv.close()
-    out.update({"plain_capacity_qps": round(cap_plain, 1), "served_capacity_qps": round(cap_served, 1),
-                "capacity_ratio": round(cap_served / cap_plain, 2), "plain_at_70pct": p_pt, "served_ladder": ladder,
-                "served_qps_at_plain_p99": best["offered_qps"] if best else None,
-                "qps_ratio_at_equal_p99": round(best["offered_qps"] / p_pt["offered_qps"], 2) if best else None})
+    steps = {}
+    for label, opts in (("plain_resident_join", dict(slots=1, batching=False, resident_join=True)),
+                        ("executors4", dict(slots=4, batching=False, resident_join=True)),
+                        ("served", dict(slots=4, batching=True))):
+        srv, stm = server(**opts)
+        cap = capacity(srv, stm)
+        same = point(srv, stm, p_pt["offered_qps"], 11)
+        ratios = {n: [round(p_pt["p50_p99_ms_by_template"][n][0] / max(same["p50_p99_ms_by_template"][n][0], 1e-9), 2),
+                      round(p_pt["p50_p99_ms_by_template"][n][1] / max(same["p50_p99_ms_by_template"][n][1], 1e-9), 2)]
+                  for n in names}
+        ladder, best = [], None
+        for m in (1, 1.5, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20):
+            rate = m * p_pt["offered_qps"]
+            if rate > 1.2 * cap:
+                break
+            pt = same if m == 1 else point(srv, stm, rate, 11)
+            ladder.append(pt)
+            if pt["p99_ms"] <= p_pt["p99_ms"]:
+                best = pt
+            else:
+                break
+        step = {"capacity_qps": round(cap, 1), "capacity_ratio": round(cap / cap_plain, 2),
+                "at_plain_rate": same, "plain_over_this_p50_p99_by_template": ratios,
+                "qps_at_plain_p99": best["offered_qps"] if best else None,
+                "qps_ratio_at_equal_p99": round(best["offered_qps"] / p_pt["offered_qps"], 2) if best else None}
+        if label == "served":
+            step["ladder"] = ladder
+            step["stats"] = srv.stats()
+        srv.close()
+        steps[label] = step
+        print(f"[bench] served: {label}: {step}", file=sys.stderr, flush=True)
+    served = steps["served"]
+    out.update({"plain_capacity_qps": round(cap_plain, 1), "plain_at_70pct": p_pt, "steps": steps,
+                "served_capacity_qps": served["capacity_qps"], "capacity_ratio": served["capacity_ratio"],
+                "served_qps_at_plain_p99": served["qps_at_plain_p99"],
+                "qps_ratio_at_equal_p99": served["qps_ratio_at_equal_p99"],
+                # share of the served form's gain at equal p99 left once the resident index and the four
+                # executors are in place: what continuous batching adds on top of them
+                "batching_gain_over_executors4": (round(served["qps_at_plain_p99"] / steps["executors4"]["qps_at_plain_p99"], 2)
+                                                  if served["qps_at_plain_p99"] and steps["executors4"]["qps_at_plain_p99"]
+                                                  else None)})
     return out
 
 

@@ -350,6 +350,8 @@ typedef struct vgpu_server_options {
 	uint64_t memory_budget; /* device bytes admitted work may reserve (0 = 3/4 of free memory) */
 	int32_t fast_math;
 	uint64_t max_batch_rows; /* table rows per batch (0 = no limit but the GROUP BY sort's 2^30) */
+	int32_t resident_join;   /* 1: a JOIN_AGG template reduces its build side once and every query probes
+	                            it, with batching off too (batching on always does) */
 } vgpu_server_options;
 
 enum vgpu_template_kind {

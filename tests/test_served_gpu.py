@@ -238,13 +238,17 @@ def test_served_join_index_fallbacks(dev, case):
     queries = [(int(s), int(c)) for s, c in zip(rng.integers(0, n // 2, 30), rng.integers(0, n // 2, 30))]
     queries += [(0, n), (5, 0), (n - 1, 1)]
     out = {}
-    for batching in (True, False):
-        with capi.Server(slots=3, batching=batching) as srv:
+    # batched; plain (both sides sorted per query); plain with the resident index, one query per probe
+    for label, opts in (("batched", dict(batching=True)), ("plain", dict(batching=False)),
+                        ("resident", dict(batching=False, resident_join=True))):
+        with capi.Server(slots=3, **opts) as srv:
             t = srv.add_join_agg(d["fk"], d["bkey"], d["payload"])
             tickets = [srv.submit(t, s, c) for s, c in queries]
-            out[batching] = [srv.wait(k) for k in tickets]
-    for (s, c), x, y in zip(queries, out[True], out[False]):
+            out[label] = [srv.wait(k) for k in tickets]
+    assert all(r.batch_size == 1 for r in out["resident"])
+    for (s, c), x, y, z in zip(queries, out["batched"], out["plain"], out["resident"]):
         check_same(x, y, 2)
+        check_same(x, z, 2)
         ref_sum, ref_cnt = _join_reference(bkey, payload, fk[s:s + c])
         assert int(x.columns[1][0]) == ref_cnt
         assert x.nulls[0] == (ref_cnt == 0)

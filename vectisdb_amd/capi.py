@@ -57,7 +57,8 @@ class PlanShape(C.Structure):
 # (name, restype, argtypes); every function returns int status unless noted
 class ServerOptions(C.Structure):
     _fields_ = [("slots", C.c_int32), ("batching", C.c_int32), ("max_batch", C.c_int32),
-                ("memory_budget", C.c_uint64), ("fast_math", C.c_int32), ("max_batch_rows", C.c_uint64)]
+                ("memory_budget", C.c_uint64), ("fast_math", C.c_int32), ("max_batch_rows", C.c_uint64),
+                ("resident_join", C.c_int32)]
 
 
 class Template(C.Structure):
@@ -659,12 +660,13 @@ class ServedResult:
 class Server:
     """The served form over device-resident tables (tensors must stay alive while the server lives)."""
 
-    def __init__(self, slots=4, batching=True, max_batch=512, memory_budget=0, fast_math=False, max_batch_rows=0):
+    def __init__(self, slots=4, batching=True, max_batch=512, memory_budget=0, fast_math=False, max_batch_rows=0,
+                 resident_join=False):
         lib = load()
         self._lib = lib
         self._keep = []
         opts = ServerOptions(slots, 1 if batching else 0, max_batch, memory_budget, 1 if fast_math else 0,
-                             max_batch_rows)
+                             max_batch_rows, 1 if resident_join else 0)
         h = C.c_void_p()
         check(lib.vgpu_server_create(C.byref(opts), C.byref(h)))
         self._h = h

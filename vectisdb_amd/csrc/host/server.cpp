@@ -74,6 +74,8 @@ Server::Server(const ServerOptions &options) : options_(options) {
 	}
 	if (!options_.batching) {
 		options_.max_batch = 1;
+	} else {
+		options_.resident_join = true;
 	}
 	VGPU_HIP(hipGetDevice(&device_));
 	if (options_.memory_budget == 0) {
@@ -170,7 +172,7 @@ int Server::AddTemplate(std::unique_ptr<ServedTemplate> t) {
 		if (t->cols[0].type != t->build_key.type) {
 			throw std::runtime_error("server template: probe and build keys must have the same type");
 		}
-		if (options_.batching) {
+		if (options_.resident_join) {
 			// The resident intermediate: the build side reduced once, probed by every batch. Built on a stream
 			// of its own so it never interleaves with an executor's batch.
 			hipStream_t prep = nullptr;
@@ -224,7 +226,7 @@ uint64_t Server::Footprint(const ServedTemplate &t, uint64_t rows, uint32_t nque
 		return pairs * w + pairs / 512 + slack;
 	}
 	default: // JOIN_AGG
-		if (options_.batching) {
+		if (t.index != nullptr) {
 			return 24ull * nqueries + slack;
 		}
 		// plain path: both sides sorted and reduced per query
@@ -340,7 +342,10 @@ void Server::Executor(int slot) {
 		try {
 			if (tp->kind == TMPL_CROSS) {
 				RunCross(*tp, batch[0], stream, *results[0]);
-			} else if (options_.batching) {
+			} else if (batch.size() > 1 || tp->index != nullptr) {
+				// a resident join index is probed through the batch path, also for a batch of one; any other
+				// query that found no company runs alone on the plain operators (no segment column, no
+				// composite key: the shortest path for one query)
 				RunBatch(*tp, batch, stream, results);
 			} else {
 				RunAlone(*tp, batch[0], stream, *results[0]);

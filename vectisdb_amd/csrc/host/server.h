@@ -74,6 +74,9 @@ struct ServerOptions {
 	uint64_t memory_budget = 0; // 0: three quarters of the free device memory at creation
 	bool fast_math = false;
 	uint64_t max_batch_rows = 0; // 0: no limit (a GROUP BY batch still sorts fewer than 2^30 rows)
+	//! A JOIN_AGG template reduces its build side once and every query probes that resident index, with
+	//! batching off too (batching on always does).
+	bool resident_join = false;
 };
 
 struct ServerStats {

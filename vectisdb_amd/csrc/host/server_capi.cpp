@@ -50,6 +50,7 @@ int vgpu_server_create(const vgpu_server_options *options, vgpu_server **server)
 			o.memory_budget = options->memory_budget;
 			o.fast_math = options->fast_math != 0;
 			o.max_batch_rows = options->max_batch_rows;
+			o.resident_join = options->resident_join != 0;
 		}
 		auto s = std::unique_ptr<vgpu_server>(new vgpu_server());
 		s->impl.reset(new Server(o));
