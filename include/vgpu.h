@@ -311,6 +311,11 @@ int vgpu_hash_source(const char *text, size_t len, char *hex);
 /* IsGpuFastMathEnabled / SetGpuFastMathEnabled (cuda_engine/include/code_generator.hpp:14-16). */
 int32_t vgpu_fast_math_enabled(void);
 void vgpu_set_fast_math(int32_t on);
+/* IsGpuExtendedSemanticsEnabled / SetGpuExtendedSemanticsEnabled (include/gpu_semantics.hpp): the opt-in
+ * DuckDB semantics the reference operators refuse (NULL-aware GROUP BY and join, FLOAT/DOUBLE keys,
+ * composite and mixed-width join keys). No reference counterpart. Off by default. */
+int32_t vgpu_extended_semantics_enabled(void);
+void vgpu_set_extended_semantics(int32_t on);
 
 /* ---------------------------------------------------------------- column cache and device budget
  * GpuColumnCache (cuda_engine/include/gpu_column_cache.hpp:130-175) and RmmPool

@@ -21,3 +21,11 @@ def dev():
     assert torch.cuda.is_available(), "GPU test without a GPU"
     capi.check(capi.load().vgpu_device_init(0))
     return torch.device("cuda:0")
+
+
+@pytest.fixture(autouse=True)
+def _device_before_library(request):
+    """GPU tests bring the device up through torch first: the HIP runtime torch ships must initialize before
+    the library's own HIP calls, or torch no longer sees the device in that process."""
+    if request.node.get_closest_marker("gpu") is not None:
+        request.getfixturevalue("dev")

@@ -174,6 +174,8 @@ _PROTOS = {
     "vgpu_hash_source": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p]),
     "vgpu_fast_math_enabled": (C.c_int32, []),
     "vgpu_set_fast_math": (None, [C.c_int32]),
+    "vgpu_extended_semantics_enabled": (C.c_int32, []),
+    "vgpu_set_extended_semantics": (None, [C.c_int32]),
     "vgpu_cache_begin_staging": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p]),
     "vgpu_cache_commit_staging": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64]),
     "vgpu_cache_abort_staging": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p]),

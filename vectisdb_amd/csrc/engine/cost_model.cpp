@@ -18,7 +18,8 @@
 //                 out_rows x (8 + width)           distinct-key runs; u32 pair indices and gathered outputs
 //                 under a keyless GROUP BY:        aggregated without materializing: the smaller side sorted
 //                 larger x 8 + smaller x 32        with its payload into a per-key table, the larger side
-//                                                  regrouped by key range once
+//                                                  regrouped by key range once (COUNT / integer SUM)
+//                 (left + right) x 48              both sides sorted and merged (other aggregates)
 //   CROSS_PRODUCT out_rows x width                 expanded outputs
 #include "../../../include/gpu_engine.hpp"
 
@@ -57,7 +58,25 @@ uint64_t ChildRows(const GpuPlanNode &node, size_t index, uint64_t cap) {
 	return index < node.children.size() && node.children[index] ? NodeRows(*node.children[index], cap) : 0;
 }
 
-uint64_t NodeBytes(const GpuPlanNode &node, uint64_t cap, bool aggregated) {
+//! How a join under a keyless GROUP BY runs: not aggregated (materialized), as the per-key table probed by
+//! the larger side (every aggregate a COUNT or an integer SUM, whose output is HUGEINT), or sort-merge.
+enum JoinAggregation { kMaterialized = 0, kProbeAggregate = 1, kSortAggregate = 2 };
+
+JoinAggregation AggregationOf(const GpuPlanNode &node) {
+	if (node.op_type != GpuOpType::GROUP_BY_AGGREGATE || !node.group_keys.empty()) {
+		return kMaterialized;
+	}
+	for (auto &a : node.aggregates) {
+		const bool count = a.kind == GpuAggregateKind::COUNT_STAR || a.kind == GpuAggregateKind::COUNT;
+		const bool int_sum = a.kind == GpuAggregateKind::SUM && a.output_type == GpuValueType::HUGEINT;
+		if (!count && !int_sum) {
+			return kSortAggregate;
+		}
+	}
+	return kProbeAggregate;
+}
+
+uint64_t NodeBytes(const GpuPlanNode &node, uint64_t cap, JoinAggregation aggregated) {
 	const uint64_t width = RowWidth(node);
 	const uint64_t out_rows = NodeRows(node, cap);
 	switch (node.op_type) {
@@ -87,12 +106,17 @@ uint64_t NodeBytes(const GpuPlanNode &node, uint64_t cap, bool aggregated) {
 	}
 	case GpuOpType::HASH_JOIN: {
 		const uint64_t a = ChildRows(node, 0, cap), b = ChildRows(node, 1, cap);
-		if (aggregated) {
+		if (aggregated == kProbeAggregate) {
 			// Under a keyless aggregate the join is never materialized: the smaller side is sorted with its
 			// payload and laid out as a per-key table, the larger side is read once (grouped by key range
 			// into 4-byte keys + payload).
 			const uint64_t small = a < b ? a : b, large = a < b ? b : a;
 			return SatAdd(SatMul(large, 8), SatMul(small, 32));
+		}
+		if (aggregated == kSortAggregate) {
+			// The other aggregates (MIN/MAX, floating SUM, AVG) sort both sides with their payload and merge
+			// the runs; nothing is materialized.
+			return SatMul(SatAdd(a, b), 48);
 		}
 		return SatAdd(SatMul(SatAdd(a, b), 48), SatMul(out_rows, SatAdd(8, width)));
 	}
@@ -122,12 +146,12 @@ const char *OperatorName(GpuOpType op) {
 	return "UNKNOWN";
 }
 
-//! `aggregated`: the node's parent is a GROUP BY without keys (a join there is aggregated, not materialized).
-void Accumulate(const GpuPlanNode &node, uint64_t cap, bool aggregated, GpuWorkingSetEstimate &out) {
-	const bool keyless_aggregate = node.op_type == GpuOpType::GROUP_BY_AGGREGATE && node.group_keys.empty();
+//! `aggregated`: how the node's parent, when it is a GROUP BY without keys, aggregates a join below it.
+void Accumulate(const GpuPlanNode &node, uint64_t cap, JoinAggregation aggregated, GpuWorkingSetEstimate &out) {
+	const JoinAggregation below = AggregationOf(node);
 	for (auto &child : node.children) {
 		if (child) {
-			Accumulate(*child, cap, keyless_aggregate, out);
+			Accumulate(*child, cap, below, out);
 		}
 	}
 	const uint64_t bytes = NodeBytes(node, cap, aggregated);
@@ -213,7 +237,7 @@ GpuWorkingSetEstimate EstimateWorkingSet(const GpuPlanNode &plan, uint64_t max_c
 	GpuWorkingSetEstimate estimate;
 	const bool chunked = max_chunk_rows != 0 && IsRowIndependent(plan);
 	estimate.chunk_capped = chunked;
-	Accumulate(plan, chunked ? max_chunk_rows : 0, false, estimate);
+	Accumulate(plan, chunked ? max_chunk_rows : 0, kMaterialized, estimate);
 	return estimate;
 }
 

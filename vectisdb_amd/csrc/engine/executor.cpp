@@ -24,16 +24,20 @@
 #include "../../../include/gpu_column_cache.hpp"
 #include "../../../include/gpu_engine.hpp"
 #include "../../../include/gpu_logger.hpp"
+#include "../../../include/gpu_semantics.hpp"
 #include "../../../include/pinned_buffer_pool.hpp"
 #include "../../../include/rmm_pool.hpp"
 #include "../host/device_pool.h"
 #include "../host/operators.h"
+#include "extended.h"
 #include "fused.h"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -46,6 +50,23 @@
 #include <vector>
 
 namespace vector_gpu {
+
+namespace {
+std::atomic<int> g_extended_override {-1};
+} // namespace
+
+bool IsGpuExtendedSemanticsEnabled() {
+	const int v = g_extended_override.load();
+	if (v != -1) {
+		return v != 0;
+	}
+	const char *env = std::getenv("VECTOR_GPU_EXTENDED_SEMANTICS");
+	return env != nullptr && (std::strcmp(env, "1") == 0 || std::strcmp(env, "true") == 0);
+}
+
+void SetGpuExtendedSemanticsEnabled(bool enabled) {
+	g_extended_override.store(enabled ? 1 : 0);
+}
 
 namespace {
 
@@ -189,6 +210,7 @@ struct ExecContext {
 	bool stage_scan_to_cache = false;
 	bool nulls_supported = false;
 	bool fast_math = false;
+	bool extended = false; // IsGpuExtendedSemanticsEnabled() for this plan
 
 	void *Alloc(size_t bytes) {
 		return arena.Alloc(bytes == 0 ? 1 : bytes);
@@ -821,6 +843,10 @@ Columns ExecuteCross(const GpuPlanNode &node, ExecContext &ctx, const NameSet *n
 		bool l = false;
 		sources.push_back(&ResolveSideColumn(name, left, right, l));
 		from_left.push_back(l);
+		if (sources.back()->valid != nullptr) {
+			throw std::runtime_error("gpu_cross_join: column '" + name + "' contains NULLs, which this operator "
+			                         "does not model");
+		}
 	}
 	const uint64_t left_rows = left.front().rows;
 	const uint64_t right_rows = right.front().rows;
@@ -1252,6 +1278,343 @@ bool JoinAggregateFusable(const GpuPlanNode &node) {
 	return node.aggregates.size() < 8;
 }
 
+bool IsFloatType(GpuValueType t) {
+	return t == GpuValueType::FLOAT32 || t == GpuValueType::FLOAT64;
+}
+
+//! Bits of the value MIN (max = false) or MAX (max = true) must never prefer over a real one: the type's
+//! largest / smallest value (NaN is DuckDB's largest floating value, -inf its smallest).
+uint64_t ExtremeBits(GpuValueType t, bool largest) {
+	switch (t) {
+	case GpuValueType::INT16:
+		return largest ? 0x7FFFull : 0x8000ull;
+	case GpuValueType::INT32:
+		return largest ? 0x7FFFFFFFull : 0x80000000ull;
+	case GpuValueType::INT64:
+		return largest ? 0x7FFFFFFFFFFFFFFFull : 0x8000000000000000ull;
+	case GpuValueType::BOOLEAN:
+		return largest ? 1 : 0;
+	case GpuValueType::FLOAT32:
+		return largest ? 0x7FC00000ull : 0xFF800000ull;
+	case GpuValueType::FLOAT64:
+		return largest ? 0x7FF8000000000000ull : 0xFFF0000000000000ull;
+	default:
+		throw std::runtime_error("gpu_groupby: no MIN/MAX over this column type");
+	}
+}
+
+//! GROUP BY with the extended semantics (gpu_semantics.hpp): nullable keys become an "is NULL" key column
+//! plus the key with NULLs zeroed; FLOAT/DOUBLE keys become their ordered integer image; an aggregate over
+//! a nullable column reads a copy with NULLs replaced by the aggregate's neutral value, and a hidden
+//! SUM of the column's validity (as INT32) counts its non-NULL values, from which its validity (and AVG)
+//! is completed after the reduction. Columns without NULLs are read as they are.
+Columns GroupByExtended(const GpuPlanNode &node, const Columns &cols, uint64_t rows, ExecContext &ctx) {
+	std::vector<vgpu::DevCol> keys;
+	struct KeyOut {
+		GpuValueType type;
+		int value_slot = -1;
+		int null_slot = -1;          // "is NULL" key column of a nullable integer key
+		uint64_t null_sentinel = 0;  // a nullable floating key: NULL is this ordered value (above NaN's)
+		bool has_sentinel = false;
+	};
+	std::vector<KeyOut> key_info;
+	for (auto &key_name : node.group_keys) {
+		const DeviceColumn &c = FindColumn(cols, key_name);
+		if (c.rows != rows) {
+			throw std::runtime_error("gpu_groupby: group key '" + key_name + "' row count mismatch");
+		}
+		const bool is_float = IsFloatType(c.type);
+		if (!is_float && c.type != GpuValueType::INT16 && c.type != GpuValueType::INT32 &&
+		    c.type != GpuValueType::INT64 && c.type != GpuValueType::BOOLEAN) {
+			throw std::runtime_error("gpu_groupby: group key '" + key_name +
+			                         "' is not an integer, boolean or floating column");
+		}
+		KeyOut k;
+		k.type = c.type;
+		if (c.valid != nullptr && !is_float) {
+			auto *flags = static_cast<uint8_t *>(ctx.Alloc(rows));
+			vgpu::NullFlags(c.valid, rows, flags, ctx.stream);
+			k.null_slot = static_cast<int>(keys.size());
+			keys.push_back({flags, vgpu::VT_BOOLEAN, rows});
+		}
+		const int width = static_cast<int>(TypeSize(c.type));
+		int key_type = static_cast<int>(c.type);
+		const void *data = c.data;
+		if (is_float) {
+			void *ordered = ctx.Alloc(rows * width);
+			vgpu::FloatKeyToOrdered(c.data, static_cast<int>(c.type), rows, ordered, ctx.stream);
+			data = ordered;
+			key_type = c.type == GpuValueType::FLOAT64 ? vgpu::VT_INT64 : vgpu::VT_INT32;
+		}
+		if (c.valid != nullptr) {
+			// NULL -> 0 beside the "is NULL" column; a floating key's NULL takes the largest ordered value, which
+			// no canonical value has (every NaN orders just above +inf), so it needs no extra column
+			if (is_float) {
+				k.has_sentinel = true;
+				k.null_sentinel = width == 8 ? 0x7FFFFFFFFFFFFFFFull : 0x7FFFFFFFull;
+			}
+			void *masked = ctx.Alloc(rows * width);
+			vgpu::MaskNulls(data, c.valid, width, k.null_sentinel, rows, masked, ctx.stream);
+			data = masked;
+		}
+		k.value_slot = static_cast<int>(keys.size());
+		keys.push_back({data, key_type, rows});
+		key_info.push_back(k);
+	}
+	if (keys.size() > 4) {
+		throw std::runtime_error("gpu_groupby: at most 4 packed key columns (a nullable key takes two)");
+	}
+
+	// Aggregate requests over value columns; nullable inputs get masked copies and a validity count.
+	std::vector<vgpu::DevCol> values;
+	std::map<std::pair<std::string, uint64_t>, int> value_of; // (column, fill or ~0 for as-is)
+	std::map<std::string, int> count_req_of;                  // column -> hidden SUM(valid) request
+	std::vector<vgpu::AggReq> reqs;
+	auto value_slot = [&](const DeviceColumn &c, bool masked, uint64_t fill, GpuValueType type) {
+		const auto key = std::make_pair(c.name, masked ? fill : ~0ull);
+		auto it = value_of.find(key);
+		if (it != value_of.end()) {
+			return it->second;
+		}
+		const void *data = c.data;
+		if (masked) {
+			void *m = ctx.Alloc(rows * TypeSize(c.type));
+			vgpu::MaskNulls(c.data, c.valid, static_cast<int>(TypeSize(c.type)), fill, rows, m, ctx.stream);
+			data = m;
+		}
+		values.push_back({data, static_cast<int>(type), rows});
+		const int slot = static_cast<int>(values.size()) - 1;
+		value_of.emplace(key, slot);
+		return slot;
+	};
+	struct Fix {
+		size_t agg = 0;     // index into node.aggregates
+		int count_req = -1; // hidden request holding the non-NULL count
+		int sum_req = -1;   // AVG: hidden request holding the SUM
+	};
+	std::vector<Fix> fixes;
+	std::vector<int> out_req(node.aggregates.size(), -1);
+	for (size_t i = 0; i < node.aggregates.size(); i++) {
+		const auto &a = node.aggregates[i];
+		vgpu::AggReq r;
+		r.kind = static_cast<int>(a.kind);
+		r.out_type = static_cast<int>(a.output_type);
+		if (a.kind == GpuAggregateKind::COUNT_STAR) {
+			out_req[i] = static_cast<int>(reqs.size());
+			reqs.push_back(r);
+			continue;
+		}
+		const DeviceColumn &c = FindColumn(cols, a.input_column);
+		if (c.rows != rows) {
+			throw std::runtime_error("gpu_groupby: aggregate input column '" + a.input_column + "' row count mismatch");
+		}
+		if (c.valid == nullptr) {
+			r.input = value_slot(c, false, 0, c.type);
+			vgpu::ValidateAggregateReq(r, values);
+			out_req[i] = static_cast<int>(reqs.size());
+			reqs.push_back(r);
+			continue;
+		}
+		auto count_req = [&]() {
+			auto it = count_req_of.find(c.name);
+			if (it != count_req_of.end()) {
+				return it->second;
+			}
+			auto *flags = static_cast<int32_t *>(ctx.Alloc(rows * 4));
+			vgpu::ValidAsInt32(c.valid, rows, flags, ctx.stream);
+			values.push_back({flags, vgpu::VT_INT32, rows});
+			vgpu::AggReq cr;
+			cr.kind = vgpu::AGG_SUM;
+			cr.input = static_cast<int>(values.size()) - 1;
+			cr.out_type = vgpu::VT_INT64;
+			const int idx = static_cast<int>(reqs.size());
+			reqs.push_back(cr);
+			count_req_of.emplace(c.name, idx);
+			return idx;
+		};
+		switch (a.kind) {
+		case GpuAggregateKind::COUNT:
+			out_req[i] = count_req(); // the non-NULL count is the result (never NULL)
+			if (a.output_type != GpuValueType::INT64) {
+				throw std::runtime_error("gpu_groupby: COUNT must produce INT64");
+			}
+			continue;
+		case GpuAggregateKind::SUM:
+		case GpuAggregateKind::AVG:
+			r.input = value_slot(c, true, 0, c.type);
+			break;
+		case GpuAggregateKind::MIN:
+			r.input = value_slot(c, true, ExtremeBits(c.type, true), c.type);
+			break;
+		case GpuAggregateKind::MAX:
+			r.input = value_slot(c, true, ExtremeBits(c.type, false), c.type);
+			break;
+		default:
+			throw std::runtime_error("gpu_groupby: unknown aggregate kind");
+		}
+		Fix f;
+		f.agg = i;
+		if (a.kind == GpuAggregateKind::AVG) {
+			// SUM then divide by the non-NULL count
+			vgpu::AggReq sr;
+			sr.kind = vgpu::AGG_SUM;
+			sr.input = r.input;
+			sr.out_type = IsFloatType(c.type) ? vgpu::VT_FLOAT64 : vgpu::VT_HUGEINT;
+			vgpu::ValidateAggregateReq(r, values);
+			out_req[i] = static_cast<int>(reqs.size());
+			reqs.push_back(r); // the AVG itself only reserves the DOUBLE output column
+			f.sum_req = static_cast<int>(reqs.size());
+			reqs.push_back(sr);
+		} else {
+			vgpu::ValidateAggregateReq(r, values);
+			out_req[i] = static_cast<int>(reqs.size());
+			reqs.push_back(r);
+		}
+		f.count_req = count_req();
+		fixes.push_back(f);
+	}
+
+	const bool global = node.group_keys.empty();
+	if (rows == 0 && !global) {
+		Columns r;
+		for (size_t k = 0; k < key_info.size(); k++) {
+			r.push_back(EmptyColumn(node.group_keys[k], key_info[k].type, ctx));
+		}
+		for (size_t i = 0; i < node.aggregates.size(); i++) {
+			r.push_back(EmptyColumn("agg_" + std::to_string(i), node.aggregates[i].output_type, ctx));
+		}
+		return r;
+	}
+	if (rows == 0) {
+		// keyless aggregate over no rows: one row, COUNT 0 and every other aggregate NULL
+		Columns r;
+		for (size_t i = 0; i < node.aggregates.size(); i++) {
+			const auto &a = node.aggregates[i];
+			DeviceColumn c;
+			c.name = "agg_" + std::to_string(i);
+			c.type = a.output_type;
+			c.rows = 1;
+			c.data = ctx.Alloc(16);
+			vgpu::FillU64(static_cast<uint64_t *>(c.data), 0, 2, ctx.stream);
+			if (a.kind != GpuAggregateKind::COUNT_STAR && a.kind != GpuAggregateKind::COUNT) {
+				c.valid = static_cast<uint8_t *>(ctx.Alloc(8));
+				vgpu::FillU64(reinterpret_cast<uint64_t *>(c.valid), 0, 1, ctx.stream);
+			}
+			r.push_back(std::move(c));
+		}
+		return r;
+	}
+	vgpu::KeyPack pack;
+	const uint32_t *hist8 = nullptr;
+	const uint64_t bound = vgpu::PlanGroupBy(keys, rows, pack, ctx.arena, ctx.stream, &hist8);
+	vgpu::SortTuning tuning = vgpu::TuneSort(pack.total_bits, rows);
+	tuning.hist8 = hist8;
+	vgpu::OutAlloc alloc = [&](size_t bytes) { return ctx.Alloc(bytes); };
+	// The reduction takes at most 4 distinct payload columns and 8 distinct component states per call; the
+	// requests are split into runs that fit, each run groups the same keys (so its groups line up with the
+	// other runs'), and the masked copies and validity counts make several runs common here.
+	std::vector<std::vector<int>> runs(1);
+	{
+		std::set<int> payloads;
+		std::set<std::pair<int, int>> comps;
+		for (int q = 0; q < static_cast<int>(reqs.size()); q++) {
+			const auto &rq = reqs[q];
+			const bool counting = rq.kind == vgpu::AGG_COUNT_STAR || rq.kind == vgpu::AGG_COUNT;
+			const int cls = rq.kind == vgpu::AGG_MIN ? 1 : (rq.kind == vgpu::AGG_MAX ? 2 : 0);
+			const int newp = !counting && !payloads.count(rq.input) ? 1 : 0;
+			const int newc = !counting && !comps.count({cls, rq.input}) ? 1 : 0;
+			if (!runs.back().empty() && (payloads.size() + newp > 4 || 1 + comps.size() + newc > 8 ||
+			                             runs.back().size() >= 16)) {
+				runs.emplace_back();
+				payloads.clear();
+				comps.clear();
+			}
+			if (!counting) {
+				payloads.insert(rq.input);
+				comps.insert({cls, rq.input});
+			}
+			runs.back().push_back(q);
+		}
+	}
+	std::vector<void *> key_out, agg_out(reqs.size(), nullptr);
+	uint64_t groups = 0;
+	for (size_t ri = 0; ri < runs.size(); ri++) {
+		std::vector<vgpu::DevCol> run_values;
+		std::map<int, int> local;
+		std::vector<vgpu::AggReq> run_reqs;
+		for (int q : runs[ri]) {
+			vgpu::AggReq rq = reqs[q];
+			if (rq.input >= 0) {
+				auto it = local.find(rq.input);
+				if (it == local.end()) {
+					it = local.emplace(rq.input, static_cast<int>(run_values.size())).first;
+					run_values.push_back(values[rq.input]);
+				}
+				rq.input = it->second;
+			}
+			run_reqs.push_back(rq);
+		}
+		std::vector<void *> run_keys, run_aggs;
+		const uint64_t g = vgpu::GroupByDevice(keys, run_values, run_reqs, rows, pack, std::max<uint64_t>(bound, 1),
+		                                       alloc, run_keys, run_aggs, tuning, ctx.arena, ctx.stream, true);
+		if (ri == 0) {
+			groups = g;
+			key_out = run_keys;
+		} else if (g != groups) {
+			throw std::logic_error("gpu_groupby: aggregate runs over the same keys disagree on the group count");
+		}
+		for (size_t i = 0; i < runs[ri].size(); i++) {
+			agg_out[runs[ri][i]] = run_aggs[i];
+		}
+	}
+	if (groups == 0) {
+		throw std::runtime_error("gpu_groupby: computed zero groups for a non-empty input");
+	}
+	Columns r;
+	for (size_t k = 0; k < key_info.size(); k++) {
+		const KeyOut &ki = key_info[k];
+		DeviceColumn c;
+		c.name = node.group_keys[k];
+		c.type = ki.type;
+		c.rows = groups;
+		c.data = key_out[ki.value_slot];
+		if (IsFloatType(ki.type)) {
+			void *f = ctx.Alloc(groups * TypeSize(ki.type));
+			vgpu::OrderedToFloatKey(c.data, static_cast<int>(ki.type), groups, f, ctx.stream);
+			c.data = f;
+		}
+		if (ki.null_slot >= 0) {
+			c.valid = static_cast<uint8_t *>(ctx.Alloc(groups));
+			vgpu::ValidFromNullFlags(static_cast<const uint8_t *>(key_out[ki.null_slot]), groups, c.valid, ctx.stream);
+		}
+		if (ki.has_sentinel) {
+			c.valid = static_cast<uint8_t *>(ctx.Alloc(groups));
+			vgpu::ValidUnlessEqual(key_out[ki.value_slot], static_cast<int>(TypeSize(ki.type)), ki.null_sentinel, groups,
+			                       c.valid, ctx.stream);
+		}
+		r.push_back(std::move(c));
+	}
+	std::vector<uint8_t *> agg_valid(node.aggregates.size(), nullptr);
+	for (auto &f : fixes) {
+		const auto &a = node.aggregates[f.agg];
+		agg_valid[f.agg] = static_cast<uint8_t *>(ctx.Alloc(groups));
+		vgpu::FinishNullableAggregate(static_cast<int>(a.kind), static_cast<int>(a.output_type), agg_out[out_req[f.agg]],
+		                              agg_valid[f.agg], f.sum_req >= 0 ? agg_out[f.sum_req] : nullptr,
+		                              f.sum_req >= 0 ? reqs[f.sum_req].out_type : 0,
+		                              static_cast<const int64_t *>(agg_out[f.count_req]), groups, ctx.stream);
+	}
+	for (size_t i = 0; i < node.aggregates.size(); i++) {
+		DeviceColumn c;
+		c.name = "agg_" + std::to_string(i);
+		c.type = node.aggregates[i].output_type;
+		c.rows = groups;
+		c.data = agg_out[out_req[i]];
+		c.valid = agg_valid[i];
+		r.push_back(std::move(c));
+	}
+	return r;
+}
+
 Columns ExecuteGroupBy(const GpuPlanNode &node, ExecContext &ctx) {
 	if (node.children.size() != 1) {
 		throw std::runtime_error("gpu_executor: GROUP_BY_AGGREGATE expects exactly one child");
@@ -1271,6 +1634,9 @@ Columns ExecuteGroupBy(const GpuPlanNode &node, ExecContext &ctx) {
 	}
 	CheckGroupByShape(node);
 	const uint64_t rows = cols.front().rows;
+	if (ctx.extended) {
+		return GroupByExtended(node, cols, rows, ctx);
+	}
 	std::vector<vgpu::DevCol> keys;
 	std::vector<const DeviceColumn *> key_cols;
 	for (auto &key_name : node.group_keys) {
@@ -1353,7 +1719,151 @@ struct JoinInputs {
 	const DeviceColumn *right_key = nullptr;
 	std::vector<const DeviceColumn *> sources;
 	std::vector<bool> from_left;
+	//! Key columns built for the join (packed composite or ordered floating keys); heap-held so the pointers
+	//! above stay valid when the struct moves.
+	std::unique_ptr<DeviceColumn> packed_left, packed_right;
 };
+
+bool IsIntegerKeyType(GpuValueType t) {
+	return t == GpuValueType::INT16 || t == GpuValueType::INT32 || t == GpuValueType::INT64 ||
+	       t == GpuValueType::BOOLEAN;
+}
+
+//! Drops the rows of `side` whose value in any of `keys` is NULL (an INNER equi-join never matches them):
+//! one generated predicate kernel over the nullable key columns emits the kept rows, every other column
+//! is gathered through them with its validity. Pruned columns only take the new row count.
+void DropNullKeyRows(Columns &side, const std::vector<std::string> &keys, ExecContext &ctx) {
+	GpuExpr pred;
+	pred.generated_cuda_source = "((bool*)out)[idx] = true;";
+	pred.output_type = GpuValueType::BOOLEAN;
+	for (auto &k : keys) {
+		if (FindColumn(side, k).valid != nullptr) {
+			pred.input_columns.push_back(k);
+		}
+	}
+	if (pred.input_columns.empty() || side.empty()) {
+		return;
+	}
+	const uint64_t rows = side.front().rows;
+	std::vector<NamedExpr> outs;
+	std::vector<size_t> where;
+	for (size_t i = 0; i < side.size(); i++) {
+		if (!side[i].pruned) {
+			outs.push_back(NamedExpr {side[i].name, nullptr, &side[i]});
+			where.push_back(i);
+		}
+	}
+	uint64_t kept = 0;
+	Columns compact = RunFused(side, rows, {pred}, outs, ctx, &kept);
+	Columns out;
+	size_t next = 0;
+	for (size_t i = 0; i < side.size(); i++) {
+		if (next < where.size() && where[next] == i) {
+			DeviceColumn c = compact[next++];
+			if (std::find(keys.begin(), keys.end(), c.name) != keys.end()) {
+				c.valid = nullptr; // every kept key is non-NULL
+			}
+			out.push_back(std::move(c));
+		} else {
+			out.push_back(PrunedLike(side[i], kept));
+		}
+	}
+	side = std::move(out);
+}
+
+//! Extended join keys (gpu_semantics.hpp): NULL-key rows dropped; FLOAT/DOUBLE keys replaced by their ordered
+//! images; several key columns, or integer keys of different widths, packed into one 64-bit key per row with
+//! the same packing on both sides (ranges over both sides; signed values compare after sign extension).
+void PrepareJoinKeysExtended(const GpuPlanNode &node, ExecContext &ctx, JoinInputs &j) {
+	const size_t k = node.build_keys.size();
+	if (k == 0 || k != node.probe_keys.size()) {
+		throw std::runtime_error("gpu_hash_join: build and probe key lists must have the same nonzero length");
+	}
+	DropNullKeyRows(j.left, node.build_keys, ctx);
+	DropNullKeyRows(j.right, node.probe_keys, ctx);
+	std::vector<DeviceColumn> lk, rk;
+	bool same_single = k == 1;
+	for (size_t i = 0; i < k; i++) {
+		DeviceColumn l = FindColumn(j.left, node.build_keys[i]);
+		DeviceColumn r = FindColumn(j.right, node.probe_keys[i]);
+		const bool ints = IsIntegerKeyType(l.type) && IsIntegerKeyType(r.type);
+		const bool floats = IsFloatType(l.type) && l.type == r.type;
+		if (!ints && !floats) {
+			throw std::runtime_error("gpu_hash_join: join keys '" + node.build_keys[i] + "' and '" + node.probe_keys[i] +
+			                         "' are not both integer/boolean or both the same floating type");
+		}
+		same_single = same_single && l.type == r.type && ints;
+		for (auto *c : {&l, &r}) {
+			if (floats && c->rows > 0) {
+				void *o = ctx.Alloc(c->rows * TypeSize(c->type));
+				vgpu::FloatKeyToOrdered(c->data, static_cast<int>(c->type), c->rows, o, ctx.stream);
+				c->data = o;
+			}
+			if (floats) {
+				c->type = c->type == GpuValueType::FLOAT64 ? GpuValueType::INT64 : GpuValueType::INT32;
+			}
+		}
+		lk.push_back(l);
+		rk.push_back(r);
+	}
+	if (same_single) {
+		j.left_key = &FindColumn(j.left, node.build_keys[0]);
+		j.right_key = &FindColumn(j.right, node.probe_keys[0]);
+		return;
+	}
+	if (k == 1 && lk[0].type == rk[0].type) {
+		j.packed_left.reset(new DeviceColumn(lk[0]));
+		j.packed_right.reset(new DeviceColumn(rk[0]));
+	} else {
+		// One packing for both sides: per key column, the range over both sides' values.
+		uint64_t mins[vgpu::kMaxKeyCols], maxs[vgpu::kMaxKeyCols];
+		if (k > (size_t)vgpu::kMaxKeyCols) {
+			throw std::runtime_error("gpu_hash_join: at most 4 join key columns");
+		}
+		for (size_t i = 0; i < k; i++) {
+			mins[i] = ~0ull;
+			maxs[i] = 0;
+			for (auto *c : {&lk[i], &rk[i]}) {
+				if (c->rows == 0) {
+					continue;
+				}
+				const void *cp[1] = {c->data};
+				int ct[1] = {static_cast<int>(c->type)};
+				uint64_t mn = 0, mx = 0;
+				vgpu::KeyRanges(cp, ct, 1, c->rows, &mn, &mx, ctx.arena, ctx.stream);
+				mins[i] = std::min(mins[i], mn);
+				maxs[i] = std::max(maxs[i], mx);
+			}
+			if (mins[i] > maxs[i]) {
+				mins[i] = maxs[i] = 0;
+			}
+		}
+		auto pack_side = [&](std::vector<DeviceColumn> &cols, std::unique_ptr<DeviceColumn> &dst, const char *name) {
+			std::vector<const void *> ptrs;
+			std::vector<int> types;
+			for (auto &c : cols) {
+				ptrs.push_back(c.data);
+				types.push_back(static_cast<int>(c.type));
+			}
+			vgpu::KeyPack pack;
+			if (!vgpu::PlanKeyPack(ptrs.data(), types.data(), static_cast<int>(k), mins, maxs, pack)) {
+				throw std::runtime_error("gpu_hash_join: the join key columns' value ranges need more than 64 bits "
+				                         "once packed");
+			}
+			DeviceColumn d;
+			d.name = name;
+			d.type = GpuValueType::INT64;
+			d.rows = cols[0].rows;
+			d.data = ctx.Alloc(d.rows * 8);
+			vgpu::PackKeys(pack, d.rows, static_cast<uint64_t *>(d.data), ctx.stream);
+			dst.reset(new DeviceColumn(d));
+		};
+		pack_side(lk, j.packed_left, "$left_key");
+		pack_side(rk, j.packed_right, "$right_key");
+	}
+	j.left_key = j.packed_left.get();
+	j.right_key = j.packed_right.get();
+}
 
 //! Executes both join inputs and applies the operator's checks in the reference's order
 //! (cuda_engine/src/operators/gpu_hash_join.cu:138-176).
@@ -1367,6 +1877,18 @@ JoinInputs PrepareJoin(const GpuPlanNode &node, ExecContext &ctx, const NameSet 
 	JoinInputs j;
 	j.left = ExecuteNode(*node.children[0], ctx, &left_need);
 	j.right = ExecuteNode(*node.children[1], ctx, &right_need);
+	if (ctx.extended) {
+		if (node.output_columns.empty()) {
+			throw std::runtime_error("gpu_hash_join: node has no output columns");
+		}
+		PrepareJoinKeysExtended(node, ctx, j);
+		for (auto &name : node.output_columns) {
+			bool l = false;
+			j.sources.push_back(&ResolveSideColumn(name, j.left, j.right, l));
+			j.from_left.push_back(l);
+		}
+		return j;
+	}
 	if (node.build_keys.size() != 1 || node.probe_keys.size() != 1) {
 		throw std::runtime_error("gpu_hash_join: this GPU path joins on exactly one equality key column");
 	}
@@ -1396,14 +1918,9 @@ vgpu::DevCol ToDev(const DeviceColumn &c) {
 	return d;
 }
 
-Columns ExecuteJoin(const GpuPlanNode &node, ExecContext &ctx, const NameSet *need) {
-	NameSet reads;
-	for (auto &name : node.output_columns) {
-		if (Needed(need, name)) {
-			reads.insert(name);
-		}
-	}
-	JoinInputs j = PrepareJoin(node, ctx, reads);
+//! The join's output columns from prepared inputs: row-id pairs, then one gather per needed column (and
+//! per validity array).
+Columns MaterializeJoin(const GpuPlanNode &node, const JoinInputs &j, ExecContext &ctx, const NameSet *need) {
 	const uint64_t left_rows = j.left_key->rows;
 	const uint64_t right_rows = j.right_key->rows;
 	auto shaped = [&](uint64_t rows) {
@@ -1442,9 +1959,24 @@ Columns ExecuteJoin(const GpuPlanNode &node, ExecContext &ctx, const NameSet *ne
 		o.data = ctx.Alloc(total * TypeSize(src.type));
 		vgpu::Gather(src.data, static_cast<int>(TypeSize(src.type)), j.from_left[i] ? lidx : ridx, false, total, o.data,
 		             ctx.stream);
+		if (src.valid != nullptr) {
+			o.valid = static_cast<uint8_t *>(ctx.Alloc(total));
+			vgpu::Gather(src.valid, 1, j.from_left[i] ? lidx : ridx, false, total, o.valid, ctx.stream);
+		}
 		r[i] = std::move(o);
 	}
 	return r;
+}
+
+Columns ExecuteJoin(const GpuPlanNode &node, ExecContext &ctx, const NameSet *need) {
+	NameSet reads;
+	for (auto &name : node.output_columns) {
+		if (Needed(need, name)) {
+			reads.insert(name);
+		}
+	}
+	JoinInputs j = PrepareJoin(node, ctx, reads);
+	return MaterializeJoin(node, j, ctx, need);
 }
 
 //! GROUP BY without keys directly over an INNER join: per-side sort + run reduction, merge of the
@@ -1459,6 +1991,18 @@ Columns ExecuteJoinAggregate(const GpuPlanNode &node, const GpuPlanNode &join, E
 	}
 	JoinInputs j = PrepareJoin(join, ctx, reads);
 	CheckGroupByShape(node);
+	if (ctx.extended) {
+		// NULLs in an aggregate input, or an empty join (one row of NULLs), take the materializing path.
+		bool materialize = j.left_key->rows == 0 || j.right_key->rows == 0;
+		for (auto *src : j.sources) {
+			materialize = materialize || src->valid != nullptr;
+		}
+		if (materialize) {
+			Columns joined = MaterializeJoin(join, j, ctx, &reads);
+			const uint64_t rows = joined.empty() ? 0 : joined.front().rows;
+			return GroupByExtended(node, joined, rows, ctx);
+		}
+	}
 	// The join's output columns, as the aggregate sees them.
 	Columns join_out;
 	for (size_t i = 0; i < j.sources.size(); i++) {
@@ -1491,7 +2035,9 @@ Columns ExecuteJoinAggregate(const GpuPlanNode &node, const GpuPlanNode &join, E
 			side_vals[side].push_back(ToDev(src));
 			check.input = static_cast<int>(check_values.size());
 			check_values.push_back(ToDev(src));
-			ValidateEngineAggregate(a, src.type);
+			if (!ctx.extended) {
+				ValidateEngineAggregate(a, src.type);
+			}
 		}
 		vgpu::ValidateAggregateReq(check, check_values);
 		reqs.push_back(r);
@@ -1505,6 +2051,9 @@ Columns ExecuteJoinAggregate(const GpuPlanNode &node, const GpuPlanNode &join, E
 	                                      ctx.arena, ctx.stream);
 	const uint64_t pairs = vals[count_slot].lo;
 	if (pairs == 0 && vals[count_slot].hi == 0) {
+		if (ctx.extended) {
+			return GroupByExtended(node, MaterializeJoin(join, j, ctx, &reads), 0, ctx);
+		}
 		ThrowGlobalEmpty();
 	}
 	std::vector<uint8_t> host(16 * node.aggregates.size(), 0);
@@ -1592,7 +2141,8 @@ bool ExecuteGpuPlan(const GpuPlanNode &plan, const std::vector<GpuColumn> &input
 		ctx.cache_chunk_index = options.cache_chunk_index;
 		ctx.serve_whole_scan_from_cache = options.serve_whole_scan_from_cache;
 		ctx.stage_scan_to_cache = options.stage_scan_to_cache;
-		ctx.nulls_supported = IsRowIndependent(plan);
+		ctx.extended = IsGpuExtendedSemanticsEnabled();
+		ctx.nulls_supported = IsRowIndependent(plan) || ctx.extended;
 		ctx.fast_math = IsGpuFastMathEnabled();
 
 		Columns columns = ExecuteNode(plan, ctx, nullptr);

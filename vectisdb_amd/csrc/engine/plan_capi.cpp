@@ -3,6 +3,7 @@
 #include "../../../include/code_generator.hpp"
 #include "../../../include/gpu_column_cache.hpp"
 #include "../../../include/gpu_engine.hpp"
+#include "../../../include/gpu_semantics.hpp"
 #include "../../../include/kernel_cache.hpp"
 #include "../../../include/rmm_pool.hpp"
 #include "../host/capi_util.h"
@@ -155,6 +156,14 @@ int32_t vgpu_fast_math_enabled(void) {
 
 void vgpu_set_fast_math(int32_t on) {
 	vector_gpu::SetGpuFastMathEnabled(on != 0);
+}
+
+int32_t vgpu_extended_semantics_enabled(void) {
+	return vector_gpu::IsGpuExtendedSemanticsEnabled() ? 1 : 0;
+}
+
+void vgpu_set_extended_semantics(int32_t on) {
+	vector_gpu::SetGpuExtendedSemanticsEnabled(on != 0);
 }
 
 int vgpu_cache_begin_staging(const char *catalog, const char *schema, const char *table) {

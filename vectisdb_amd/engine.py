@@ -335,6 +335,16 @@ def is_gpu_fast_math_enabled() -> bool:
     return bool(capi.load().vgpu_fast_math_enabled())
 
 
+def set_gpu_extended_semantics_enabled(on: bool) -> None:
+    """SetGpuExtendedSemanticsEnabled (include/gpu_semantics.hpp): DuckDB's NULL, floating-key and composite
+    join-key semantics instead of the reference operators' refusals."""
+    capi.load().vgpu_set_extended_semantics(1 if on else 0)
+
+
+def is_gpu_extended_semantics_enabled() -> bool:
+    return bool(capi.load().vgpu_extended_semantics_enabled())
+
+
 # ---------------------------------------------------------------- column cache / device budget
 
 def _t(ref: GpuTableRef):
