@@ -101,6 +101,21 @@ def committed_traffic(kernel):
     return None if entry is None else entry.get("traffic_bytes_per_launch")
 
 
+def committed_operator_traffic(operator):
+    """Per-kernel HBM traffic of an operator line (profiles/traffic.json "operators", measured by the same
+    rocprofv3 --pmc passes), or None."""
+    path = ROOT / "profiles" / "traffic.json"
+    try:
+        ops = json.loads(path.read_text()).get("operators", {})
+    except (OSError, ValueError):
+        return None
+    entry = ops.get(operator)
+    if entry is None:
+        return None
+    return {k: {"kernel": v["rocprof_kernel"], "traffic_bytes_per_launch": v["traffic_bytes_per_launch"],
+                "traffic_over_algorithmic": v.get("traffic_over_algorithmic")} for k, v in entry.items()}
+
+
 def kernel_stats(prof, name):
     for e in prof:
         if e["name"] == name:
@@ -306,6 +321,31 @@ def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
     torch.cuda.synchronize()
     ms2 = (time.perf_counter() - t0) * 1e3 / steps
     expect(res2 == [probe_sum, total, count], "join with probe-side SUM vs closed form")
+    # The same query on the sort-merge form (both sides radix-sorted and reduced to their distinct keys, the
+    # runs matched by merge path): the form every other aggregate mix takes, forced here for measurement.
+    os.environ["VGPU_JOIN_FORM"] = "sort"
+    try:
+        res_s = capi.join_aggregate(fk, [], build, [payload], aggs)
+        expect(res_s == [total, count], f"sort-merge join SUM/COUNT {res_s} vs closed form {[total, count]}")
+        capi.profile_reset()
+        capi.profile_enable(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            res_s = capi.join_aggregate(fk, [], build, [payload], aggs)
+        torch.cuda.synchronize()
+        ms_s = (time.perf_counter() - t0) * 1e3 / steps
+        capi.profile_enable(False)
+        prof_s = capi.profile_read()
+    finally:
+        del os.environ["VGPU_JOIN_FORM"]
+    expect(res_s == [total, count], "sort-merge join SUM/COUNT (timed calls)")
+    alg_s = sum(e["bytes"] for e in prof_s) / steps
+    sort_merge = {"ms": round(ms_s, 3), "rows_per_s": round((nprobe + nb) / (ms_s * 1e-3), 1),
+                  "compulsory_frac": round(compulsory / (ms_s * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "algorithmic_bytes": alg_s, "gbs": round(alg_s / (ms_s * 1e-3) / 1e9, 1),
+                  "hbm_frac": round(alg_s / (ms_s * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "kernel_ms_per_step": {e["name"]: round(e["ms"] / steps, 4) for e in prof_s}, "checked": True}
     threads = cpu_threads()
     hs, hc, sec = oracle.join_sum_count_mt(build.cpu().numpy(), payload.cpu().numpy(), fk.cpu().numpy(), threads)
     expect((hs, hc) == (total, count), "CPU hash join agrees with the closed form")
@@ -317,6 +357,7 @@ def bench_join(torch, capi, dev, steps, nb=1 << 24, nprobe=1 << 28):
             "probe_side_sum_ms": round(ms2, 3), "probe_side_sum": int(res2[0]),
             "rows_per_s": round((nprobe + nb) / (ms * 1e-3), 1), "algorithmic_bytes": alg,
             "gbs": round(alg / (ms * 1e-3) / 1e9, 1), "kernel_ms_per_step": kernels, "checked": True,
+            "sort_merge": sort_merge,
             "cpu_baseline": cpu_line(nprobe + nb, sec, threads,
                                      "vo_join_sum_count_mt, a hash join (shared open-addressing build table, "
                                      f"probe split over threads) over the same {nb} x {nprobe} rows")}
@@ -794,6 +835,9 @@ def main():
                 continue
         except Exception as exc:  # an operator line must not hide the headline number
             operators.append({"operator": name, "error": str(exc)[:300]})
+        traffic = committed_operator_traffic(operators[-1].get("operator", ""))
+        if traffic is not None:
+            operators[-1]["traffic"] = traffic
         torch.cuda.empty_cache()
         print(f"[bench] operator {operators[-1]}", file=sys.stderr, flush=True)
 

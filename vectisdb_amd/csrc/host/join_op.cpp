@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -460,7 +461,10 @@ std::vector<JoinAggValue> JoinAggregateDevice(const DevCol &akey, const std::vec
 			narrow = 1;
 		}
 	}
-	if (akey.type != VT_BOOLEAN) {
+	// VGPU_JOIN_FORM=sort keeps every aggregate on the sort-merge form (to measure it beside the probe form).
+	const char *form = std::getenv("VGPU_JOIN_FORM");
+	const bool sort_only = form != nullptr && std::strcmp(form, "sort") == 0;
+	if (akey.type != VT_BOOLEAN && !sort_only) {
 		// Probe form: the larger side is read once against a table of the smaller one. Every aggregate must
 		// be a COUNT or an integer SUM/AVG of one column of either side.
 		const int probe_side = akey.rows >= bkey.rows ? 0 : 1;
