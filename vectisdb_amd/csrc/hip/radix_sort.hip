@@ -216,6 +216,73 @@ __global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, Pass
 	}
 }
 
+//! Keys packed from several columns: one read of the key columns writes the packed keys (so the first radix
+//! pass reads one compact key instead of every key column, one column at a time) and builds the 8-bit digit
+//! histogram of every pass. A digit that a whole wave shares (a composite key's leading column, constant over
+//! long runs of rows) is counted by one lane with one LDS add instead of 64 colliding ones; each wave counts
+//! into its own copy of the histograms, summed at the end.
+template <typename KeyT>
+__global__ void __launch_bounds__(256) PackHistKernel(KeyPack pack, uint64_t n, PassList passes, KeyT *keys_out,
+                                                      uint32_t *hist, const uint64_t *n_dev) {
+	constexpr int RADIX = 256;
+	constexpr int PMAX = 8;
+	constexpr int kWaves = 256 / kWave;
+	__shared__ uint32_t s_hist[kWaves][PMAX * RADIX];
+	if (n_dev != nullptr) {
+		n = *n_dev; // row count written by an earlier kernel (n was its bound)
+	}
+	for (int i = threadIdx.x; i < kWaves * PMAX * RADIX; i += blockDim.x) {
+		(&s_hist[0][0])[i] = 0;
+	}
+	__syncthreads();
+	const int lane = LaneId();
+	uint32_t *mine = s_hist[threadIdx.x / kWave];
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	// every lane of a wave runs the same trip count (the ballots need the whole wave); four rows per lane are
+	// loaded before any is counted, to keep enough reads in flight
+	const uint64_t wave_row0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1));
+	for (uint64_t base = wave_row0; base < n; base += 4 * stride) {
+		u64 key[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const uint64_t row = base + u * stride + lane;
+			key[u] = row < n ? PackRow(pack, row) : 0ull;
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const uint64_t row = base + u * stride + lane;
+			const bool valid = row < n;
+			if (valid) {
+				keys_out[row] = (KeyT)key[u];
+			}
+			const u64 live = __ballot(valid);
+			for (int p = 0; p < passes.npasses; p++) {
+				const uint32_t d = (uint32_t)(key[u] >> passes.shift[p]) & ((1u << passes.bits[p]) - 1);
+				const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+				if (__ballot(valid && d == d0) == live) {
+					// the whole wave holds one digit: one add instead of 64 to one address
+					if (lane == 0) {
+						atomicAdd(&mine[p * RADIX + d0], (uint32_t)Popc64(live));
+					}
+				} else if (valid) {
+					atomicAdd(&mine[p * RADIX + d], 1u);
+				}
+			}
+		}
+	}
+	__syncthreads();
+	for (int i = threadIdx.x; i < passes.npasses * RADIX; i += blockDim.x) {
+		uint32_t v = 0;
+#pragma unroll
+		for (int w = 0; w < kWaves; w++) {
+			v += s_hist[w][i];
+		}
+		if (v) {
+			FetchAddAgent(hist + i, v);
+		}
+	}
+}
+
 //! Items per thread of the kernel variant that runs (11-bit digits always use 16).
 int EffectiveItems(int R, int items) {
 	return (R == 8 && items == 32) ? 32 : 16;
@@ -472,15 +539,33 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 	FillU32(state, 0u, state_words, stream);
 	uint32_t *tickets = state + (uint64_t)P * tiles * RADIX;
 
+	// Several key columns with real payload columns: pack once, beside the histograms, and start the passes
+	// from the packed keys (pass 0 then reads one compact key; row-id payloads need the first-pass kernel).
+	bool prepack = !aligned8 && R == 8 && pack.ncols > 1;
+	for (int p = 0; p < npayload; p++) {
+		prepack = prepack && payload_in[p] != nullptr;
+	}
+	void *packed = nullptr;
+	if (prepack) {
+		packed = P > 1 ? kbuf[1] : scratch.Alloc(n * kw); // pass 0 writes kbuf[0]; pass 1 overwrites kbuf[1]
+	}
 	if (!aligned8) {
 		FillU32(hist, 0u, (uint64_t)P * RADIX, stream);
-		double bytes = 0;
+		double bytes = prepack ? (double)n * kw : 0.0;
 		for (int c = 0; c < pack.ncols; c++) {
 			bytes += (double)n * pack.width[c];
 		}
 		ProfileScope prof("radix_hist", stream, bytes);
 		uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 2048);
-		if (R == 8) {
+		if (prepack) {
+			if (keys64) {
+				hipLaunchKernelGGL(PackHistKernel<u64>, dim3((unsigned)blocks), dim3(256), 0, stream, pack, n, passes,
+				                   (u64 *)packed, hist, n_dev);
+			} else {
+				hipLaunchKernelGGL(PackHistKernel<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, stream, pack, n,
+				                   passes, (uint32_t *)packed, hist, n_dev);
+			}
+		} else if (R == 8) {
 			hipLaunchKernelGGL(HistKernel<8>, dim3((unsigned)blocks), dim3(256), 0, stream, pack, n, passes, hist,
 			                   n_dev);
 		} else {
@@ -490,7 +575,7 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 		VGPU_HIP(hipGetLastError());
 	}
 
-	const void *cur_keys = nullptr;
+	const void *cur_keys = packed;
 	const void *cur_pay[kMaxPayload];
 	for (int p = 0; p < npayload; p++) {
 		cur_pay[p] = payload_in[p];
@@ -506,7 +591,7 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 		}
 		uint32_t *st = state + (uint64_t)pass * tiles * RADIX;
 		const uint32_t *gh = hist + (uint64_t)pass * RADIX;
-		bool first = pass == 0;
+		bool first = pass == 0 && !prepack;
 		LaunchPass(R, keys64, first, pack, cur_keys, kbuf[slot], pa, n, passes.shift[pass], passes.bits[pass], gh,
 		           st, tickets + pass, stream, tuning, n_dev);
 		cur_keys = kbuf[slot];
