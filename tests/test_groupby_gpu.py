@@ -302,3 +302,46 @@ def test_groupby_count_only_many_tiles(dev):
     gk = keys[0].cpu().numpy()[:g]
     np.testing.assert_array_equal(gk, uk)
     np.testing.assert_array_equal(aggs[0].cpu().numpy()[:g], uc)
+
+
+def test_groupby_int64_values_at_int32_bounds(dev):
+    """INT64 aggregate inputs are carried through the sort as INT32 when they fit: values exactly at
+    INT32_MIN / INT32_MAX fit, so SUM (HUGEINT), MIN and MAX come from the narrowed carry and must still be
+    exact."""
+    rng = np.random.default_rng(41)
+    n = 3_000_000
+    key = rng.integers(0, 20_000, n, dtype=np.int64)
+    v1 = rng.integers(-(1 << 31), 1 << 31, n, dtype=np.int64)
+    v1[::7] = -(1 << 31)
+    v1[3::11] = (1 << 31) - 1
+    v2 = rng.standard_normal(n)
+    _check(dev, key, v1, v2)
+
+
+@pytest.mark.parametrize("where", ["first", "middle", "last"])
+def test_groupby_int64_values_beyond_int32(dev, where):
+    """One value just outside INT32 anywhere in the column makes the narrowed sort report an overflow; the
+    GROUP BY then runs again carrying the column whole. The result must equal the oracle's, and so must a
+    second call over the same column (remembered as wide) and a call over a column that fits."""
+    import torch
+    import oracle
+    rng = np.random.default_rng({"first": 43, "middle": 47, "last": 53}[where])
+    n = 2_500_000
+    key = rng.integers(-5_000, 5_000, n, dtype=np.int64)
+    v1 = rng.integers(-1_000_000, 1_000_001, n, dtype=np.int64)
+    row = {"first": 0, "middle": n // 2 + 123, "last": n - 1}[where]
+    v1[row] = (1 << 31) if where != "middle" else -(1 << 31) - 1
+    v1[rng.integers(0, n, 5)] = rng.integers(-(1 << 62), 1 << 62, 5)
+    v2 = rng.standard_normal(n)
+    _check(dev, key, v1, v2)
+    _check(dev, key, v1, v2)
+    fits = np.clip(v1, -(1 << 31), (1 << 31) - 1)
+    _check(dev, key, fits, v2)
+    # an executor-style call with COUNT(col)-free MIN/MAX only over the wide column
+    k = torch.from_numpy(key).to(dev)
+    a = torch.from_numpy(v1).to(dev)
+    keys, aggs, g = capi.group_by([k], [a], [(capi.MIN, 0, capi.INT64), (capi.MAX, 0, capi.INT64)], n)
+    ref = oracle.group_by_i64(key, v1, v2)
+    assert g == len(ref["key"])
+    np.testing.assert_array_equal(aggs[0].cpu().numpy(), ref["min_v1"])
+    np.testing.assert_array_equal(aggs[1].cpu().numpy(), ref["max_v1"])

@@ -1674,7 +1674,8 @@ Columns ExecuteGroupBy(const GpuPlanNode &node, ExecContext &ctx) {
 	std::vector<void *> key_out, agg_out;
 	vgpu::OutAlloc alloc = [&](size_t bytes) { return ctx.Alloc(bytes); };
 	const uint64_t groups = vgpu::GroupByDevice(keys, values, reqs, rows, pack, std::max<uint64_t>(bound, 1), alloc,
-	                                            key_out, agg_out, tuning, ctx.arena, ctx.stream, true);
+	                                            key_out, agg_out, tuning, ctx.arena, ctx.stream, true, nullptr,
+	                                            nullptr, true);
 	if (groups == 0) {
 		throw std::runtime_error("gpu_groupby: computed zero groups for a non-empty input");
 	}

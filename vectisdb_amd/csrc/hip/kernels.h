@@ -126,6 +126,12 @@ struct SortTuning {
 	//! Optional precomputed per-byte histograms (KeyRangeHist8) for a byte-aligned pack: skips the
 	//! histogram pass.
 	const uint32_t *hist8 = nullptr;
+	//! Payload narrowing (one key column only): bit p = payload p is an INT64 column whose values the
+	//! caller expects in the INT32 range, given with payload_width 4. The first pass reads it whole and
+	//! carries the low 32 bits; any value outside that range sets *overflow (device word, cleared by the
+	//! caller), and the sort's payload is then wrong: the caller sorts again without narrowing.
+	uint32_t narrow = 0;
+	uint32_t *overflow = nullptr;
 };
 
 //! Output of SortPacked: the compact keys (uint32 when total_bits <= 32, else uint64) in ascending order and

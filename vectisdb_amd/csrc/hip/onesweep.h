@@ -21,6 +21,10 @@ struct PayloadArgs {
 	const void *in[kMaxPayload] = {};
 	void *out[kMaxPayload] = {};
 	int width[kMaxPayload] = {};
+	//! First pass only: bit p = column p is INT64 read whole and carried as its low 32 bits (width 4); a
+	//! value outside the INT32 range sets *overflow, and the caller discards the sort.
+	uint32_t narrow = 0;
+	uint32_t *overflow = nullptr;
 };
 
 struct PassList {

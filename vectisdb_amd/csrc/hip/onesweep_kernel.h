@@ -94,6 +94,16 @@ __global__ void __launch_bounds__(BLOCK) OnesweepKernel(KeyPack pack, const KeyT
 			for (int k = 0; k < ITEMS; k++) {
 				dst[k] = wave_base + (uint64_t)k * kWave + lane;
 			}
+		} else if (FIRST && ((pay.narrow >> p) & 1u) != 0) {
+			LoadWaveItemsT<ITEMS>((const u64 *)pay.in[p], wave_base, lane, n, dst);
+			bool wide = false;
+#pragma unroll
+			for (int k = 0; k < ITEMS; k++) {
+				wide = wide || (long long)dst[k] != (long long)(int)(uint32_t)dst[k];
+			}
+			if (wide) {
+				atomicOr(pay.overflow, 1u);
+			}
 		} else {
 			LoadWaveItems<ITEMS>(pay.in[p], pay.width[p], wave_base, lane, n, dst);
 		}

@@ -291,7 +291,10 @@ int EffectiveItems(int R, int items) {
 //! Threads per tile of the kernel variant that runs: 512 only for 8-bit digits at 16 items without the
 //! payload prefetch (the one 512-thread variant built), else 256.
 int EffectiveBlock(int R, const SortTuning &t) {
-	return (R == 8 && t.block == 512 && EffectiveItems(R, t.items) == 16 && t.prefetch == 0) ? 512 : kSortBlock;
+	if (R == 8 && t.block >= 512 && EffectiveItems(R, t.items) == 16 && t.prefetch == 0) {
+		return t.block >= 1024 ? 1024 : 512;
+	}
+	return kSortBlock;
 }
 
 //! One digit pass with the variant the tuning selects: 8-bit digits come with 16 or 32 items per thread and
@@ -314,12 +317,16 @@ void LaunchPass(int R, bool keys64, bool first, const KeyPack &pack, const void 
 	}
 	double pay_bytes = 0;
 	for (int p = 0; p < pay.n; p++) {
-		pay_bytes += (first && pay.in[p] == nullptr ? 0.0 : pay.width[p]) + pay.width[p];
+		const int in_width = (first && ((pay.narrow >> p) & 1u) != 0) ? 8 : pay.width[p];
+		pay_bytes += (first && pay.in[p] == nullptr ? 0.0 : in_width) + pay.width[p];
 	}
 	ProfileScope prof(first ? "radix_pass_first" : "radix_pass", stream, (double)n * (key_in + kw + pay_bytes));
 	if (R == 11) {
 		LaunchVariant<11, 16, true, 256>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
 		                                 ticket, stream, tiles, n_dev);
+	} else if (block == 1024) {
+		LaunchVariant<8, 16, false, 1024>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
+		                                  ticket, stream, tiles, n_dev);
 	} else if (block == 512) {
 		LaunchVariant<8, 16, false, 512>(first, keys64, pack, keys_in, keys_out, pay, n, shift, bits, gh, state,
 		                                 ticket, stream, tiles, n_dev);
@@ -545,6 +552,18 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 	for (int p = 0; p < npayload; p++) {
 		prepack = prepack && payload_in[p] != nullptr;
 	}
+	if (tuning.narrow != 0) {
+		// the narrowing load lives in the first pass that packs the key columns
+		prepack = false;
+		if (tuning.overflow == nullptr) {
+			throw HipError("radix sort: payload narrowing needs an overflow word");
+		}
+		for (int p = 0; p < npayload; p++) {
+			if (((tuning.narrow >> p) & 1u) != 0 && (payload_width[p] != 4 || payload_in[p] == nullptr)) {
+				throw HipError("radix sort: a narrowed payload is a 4-byte carry of an INT64 column");
+			}
+		}
+	}
 	void *packed = nullptr;
 	if (prepack) {
 		packed = P > 1 ? kbuf[1] : scratch.Alloc(n * kw); // pass 0 writes kbuf[0]; pass 1 overwrites kbuf[1]
@@ -588,6 +607,10 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 			pa.in[p] = cur_pay[p];
 			pa.out[p] = pbuf[p][slot];
 			pa.width[p] = payload_width[p];
+		}
+		if (pass == 0) {
+			pa.narrow = tuning.narrow;
+			pa.overflow = tuning.overflow;
 		}
 		uint32_t *st = state + (uint64_t)pass * tiles * RADIX;
 		const uint32_t *gh = hist + (uint64_t)pass * RADIX;

@@ -885,6 +885,10 @@ void LaunchReduce(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, cons
 using SigCountSumMinMaxDd = StaticSig<SigCode(COMP_COUNT), SigCode(COMP_SUM_I128, VT_INT64, 0),
                                       SigCode(COMP_MIN_I64, VT_INT64, 0), SigCode(COMP_MAX_I64, VT_INT64, 0),
                                       SigCode(COMP_SUM_DD, VT_FLOAT64, 1)>;
+//! The same with the INT64 column carried as INT32 (the sort's payload narrowing).
+using SigCountSumMinMaxDdN = StaticSig<SigCode(COMP_COUNT), SigCode(COMP_SUM_I128, VT_INT32, 0),
+                                       SigCode(COMP_MIN_I64, VT_INT32, 0), SigCode(COMP_MAX_I64, VT_INT32, 0),
+                                       SigCode(COMP_SUM_DD, VT_FLOAT64, 1)>;
 using SigCountSumI32 = StaticSig<SigCode(COMP_COUNT), SigCode(COMP_SUM_I128, VT_INT32, 0)>;
 using SigCountSumI64 = StaticSig<SigCode(COMP_COUNT), SigCode(COMP_SUM_I128, VT_INT64, 0)>;
 using SigCountSumDd = StaticSig<SigCode(COMP_COUNT), SigCode(COMP_SUM_DD, VT_FLOAT64, 0)>;
@@ -910,6 +914,10 @@ void DispatchNP(const SortedRun &run, uint64_t n, const PayloadPtrs &pay, const 
                 const uint64_t *n_dev) {
 	if (pay.n == 2 && SigMatches<SigCountSumMinMaxDd>(cd)) {
 		LaunchReduce<KeyT, 2, SigCountSumMinMaxDd>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
+		return;
+	}
+	if (pay.n == 2 && SigMatches<SigCountSumMinMaxDdN>(cd)) {
+		LaunchReduce<KeyT, 2, SigCountSumMinMaxDdN>(run, n, pay, cd, od, ko, groups_device, scratch, stream, n_dev);
 		return;
 	}
 	if (pay.n == 1 && SigMatches<SigCountSumI32>(cd)) {

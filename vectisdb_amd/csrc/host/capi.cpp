@@ -134,7 +134,8 @@ int vgpu_group_by(const vgpu_column *keys, int32_t nkeys, const vgpu_column *val
 		std::vector<void *> ko, ao;
 		SortTuning tuning = TuneSort(pack.total_bits, rows);
 		tuning.hist8 = hist8;
-		*groups = GroupByDevice(kc, vc, reqs, rows, pack, capacity, alloc, ko, ao, tuning, arena, s, true);
+		*groups = GroupByDevice(kc, vc, reqs, rows, pack, capacity, alloc, ko, ao, tuning, arena, s, true, nullptr,
+		                        nullptr, true);
 	});
 }
 

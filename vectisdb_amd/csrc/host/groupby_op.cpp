@@ -9,13 +9,35 @@
 #include "operators.h"
 
 #include <algorithm>
+#include <deque>
+#include <mutex>
 #include <stdexcept>
+#include <utility>
 
 namespace vgpu {
 
 namespace {
 bool IsFloat(int t) {
 	return t == VT_FLOAT32 || t == VT_FLOAT64;
+}
+
+//! INT64 columns (address, rows) whose values left the INT32 range when narrowed: later GROUP BYs over them
+//! carry them whole. A short FIFO: a recycled address costs at most one narrowed attempt more.
+std::mutex g_wide_mu;
+std::deque<std::pair<const void *, uint64_t>> g_wide;
+constexpr size_t kWideMemory = 64;
+
+bool KnownWide(const void *col, uint64_t rows) {
+	std::lock_guard<std::mutex> lock(g_wide_mu);
+	return std::find(g_wide.begin(), g_wide.end(), std::make_pair(col, rows)) != g_wide.end();
+}
+
+void NoteWide(const void *col, uint64_t rows) {
+	std::lock_guard<std::mutex> lock(g_wide_mu);
+	g_wide.emplace_back(col, rows);
+	if (g_wide.size() > kWideMemory) {
+		g_wide.pop_front();
+	}
 }
 bool IsNumeric(int t) {
 	return t == VT_INT16 || t == VT_INT32 || t == VT_INT64 || IsFloat(t);
@@ -166,7 +188,7 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
                        const std::vector<AggReq> &aggs, uint64_t rows, const KeyPack &pack, uint64_t capacity,
                        const OutAlloc &out_alloc, std::vector<void *> &key_out, std::vector<void *> &agg_out,
                        const SortTuning &tuning, Scratch &scratch, hipStream_t stream, bool read_groups,
-                       uint64_t *groups_device, const uint64_t *rows_dev) {
+                       uint64_t *groups_device, const uint64_t *rows_dev, bool narrow_payload) {
 	if (aggs.size() > 16) {
 		throw std::runtime_error("gpu_groupby: at most 16 aggregates per GROUP BY");
 	}
@@ -189,6 +211,7 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
 	std::vector<int> payload_of_value(values.size(), -1);
 	std::vector<const void *> payload_in;
 	std::vector<int> payload_width;
+	std::vector<int> payload_type;
 	auto payload_for = [&](int value_index) {
 		if (payload_of_value[value_index] < 0) {
 			if ((int)payload_in.size() >= kMaxPayload) {
@@ -197,6 +220,7 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
 			payload_of_value[value_index] = (int)payload_in.size();
 			payload_in.push_back(values[value_index].data);
 			payload_width.push_back(TypeWidth(values[value_index].type));
+			payload_type.push_back(values[value_index].type);
 		}
 		return payload_of_value[value_index];
 	};
@@ -258,33 +282,92 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
 	spec.pack = pack;
 	spec.capacity = cap;
 
-	auto *groups_dev = groups_device;
-	if (groups_dev == nullptr) {
-		groups_dev = static_cast<uint64_t *>(scratch.Alloc(sizeof(uint64_t)));
+	// Word 0: the group count (unless the caller keeps it), word 1: the narrowing overflow flag, so one
+	// read-back brings both.
+	auto *words = static_cast<uint64_t *>(scratch.Alloc(2 * sizeof(uint64_t)));
+	auto *groups_dev = groups_device != nullptr ? groups_device : words;
+	auto *overflow = reinterpret_cast<uint32_t *>(words + 1);
+	uint32_t narrow = 0;
+	if (narrow_payload && read_groups && rows_dev == nullptr && keys.size() == 1 && rows > 0) {
+		for (size_t p = 0; p < payload_in.size(); p++) {
+			if (payload_type[p] == VT_INT64 && !KnownWide(payload_in[p], rows)) {
+				narrow |= 1u << p;
+			}
+		}
 	}
+	auto sort_reduce = [&](uint32_t narrow_mask) {
+		if (keys.empty()) {
+			SortedRun run;
+			run.keys = nullptr;
+			run.npayload = (int)payload_in.size();
+			for (int p = 0; p < run.npayload; p++) {
+				run.payload[p] = const_cast<void *>(payload_in[p]);
+				run.payload_width[p] = payload_width[p];
+			}
+			SegmentedReduce(run, rows, spec, groups_dev, scratch, stream, rows_dev);
+			return;
+		}
+		ReduceSpec rs = spec;
+		std::vector<int> widths = payload_width;
+		SortTuning t = tuning;
+		if (narrow_mask != 0) {
+			for (int p = 0; p < (int)widths.size(); p++) {
+				if (((narrow_mask >> p) & 1u) != 0) {
+					widths[p] = 4;
+					for (int c = 0; c < rs.ncomp; c++) {
+						if (rs.comp[c].kind != COMP_COUNT && rs.comp[c].payload == p) {
+							rs.comp[c].in_type = VT_INT32;
+						}
+					}
+				}
+			}
+			t.narrow = narrow_mask;
+			t.overflow = overflow;
+			FillU32(overflow, 0u, 1, stream);
+		}
+		SortedRun run;
+		SortPacked(pack, payload_in.data(), widths.data(), (int)payload_in.size(), rows, t, run, scratch, stream,
+		           rows_dev);
+		SegmentedReduce(run, rows, rs, groups_dev, scratch, stream, rows_dev);
+	};
 	if (rows == 0) {
 		FillU64(groups_dev, 0ull, 1, stream);
-	} else if (keys.empty()) {
-		SortedRun run;
-		run.keys = nullptr;
-		run.npayload = (int)payload_in.size();
-		for (int p = 0; p < run.npayload; p++) {
-			run.payload[p] = const_cast<void *>(payload_in[p]);
-			run.payload_width[p] = payload_width[p];
-		}
-		SegmentedReduce(run, rows, spec, groups_dev, scratch, stream, rows_dev);
 	} else {
-		SortedRun run;
-		SortPacked(pack, payload_in.data(), payload_width.data(), (int)payload_in.size(), rows, tuning, run, scratch,
-		           stream, rows_dev);
-		SegmentedReduce(run, rows, spec, groups_dev, scratch, stream, rows_dev);
+		sort_reduce(narrow);
 	}
 	if (!read_groups) {
 		return 0;
 	}
-	uint64_t groups = 0;
-	VGPU_HIP(hipMemcpyAsync(&groups, groups_dev, sizeof(groups), hipMemcpyDeviceToHost, stream));
-	VGPU_HIP(hipStreamSynchronize(stream));
+	auto read_back = [&](uint64_t &groups, uint64_t &flag) {
+		flag = 0;
+		if (groups_dev == words) {
+			uint64_t host[2] = {};
+			VGPU_HIP(hipMemcpyAsync(host, words, narrow != 0 ? sizeof(host) : sizeof(uint64_t),
+			                        hipMemcpyDeviceToHost, stream));
+			VGPU_HIP(hipStreamSynchronize(stream));
+			groups = host[0];
+			flag = narrow != 0 ? host[1] : 0;
+			return;
+		}
+		VGPU_HIP(hipMemcpyAsync(&groups, groups_dev, sizeof(groups), hipMemcpyDeviceToHost, stream));
+		if (narrow != 0) {
+			VGPU_HIP(hipMemcpyAsync(&flag, words + 1, sizeof(flag), hipMemcpyDeviceToHost, stream));
+		}
+		VGPU_HIP(hipStreamSynchronize(stream));
+	};
+	uint64_t groups = 0, flag = 0;
+	read_back(groups, flag);
+	if ((uint32_t)flag != 0) {
+		// A narrowed column held a value outside INT32: run again carrying every column whole.
+		for (size_t p = 0; p < payload_in.size(); p++) {
+			if (((narrow >> p) & 1u) != 0) {
+				NoteWide(payload_in[p], rows);
+			}
+		}
+		narrow = 0;
+		sort_reduce(0);
+		read_back(groups, flag);
+	}
 	return groups;
 }
 

@@ -14,8 +14,9 @@ namespace {
 //! (256/512; 512 runs with 16 items and no prefetch only).
 void ApplyVariantOverrides(SortTuning &t) {
 	if (const char *env = std::getenv("VGPU_SORT_BLOCK")) {
-		t.block = std::atoi(env) >= 512 ? 512 : 256;
-		if (t.block == 512) {
+		const int b = std::atoi(env);
+		t.block = b >= 1024 ? 1024 : b >= 512 ? 512 : 256;
+		if (t.block >= 512) {
 			t.items = 16;
 			t.prefetch = 0;
 		}
