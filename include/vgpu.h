@@ -441,6 +441,12 @@ int vgpu_profile_read(char *buf, size_t cap, size_t *needed);
 int vgpu_sort_autotune(int32_t on);
 int vgpu_sort_autotune_report(char *buf, size_t cap, size_t *needed);
 
+/* GROUP BY with the sort's last digit pass fused with the reduction (each tile of that pass reduces its runs
+ * of equal keys to partial states, then the partials are combined): 0 never, 1 when it moves fewer bytes
+ * (the default), 2 whenever the aggregate mix is supported. Results are the same in every mode. No reference
+ * counterpart (the reference sorts fully, then reduces). */
+int vgpu_group_by_fused_last_pass(int32_t mode);
+
 /* Device pool statistics (bytes): in use, cached for reuse, and the peak in use since the last reset. The pool
  * is the engine's allocator for every plan temporary (replaces the per-plan DeviceArena,
  * cuda_engine/src/gpu_executor.cu:57-104); the peak is what EstimateWorkingSet predicts. */

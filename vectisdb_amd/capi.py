@@ -85,6 +85,7 @@ TEMPLATE_FILTER_PROJECT, TEMPLATE_GROUP_BY, TEMPLATE_JOIN_AGG, TEMPLATE_CROSS = 
 _PROTOS = {
     "vgpu_sort_autotune": (C.c_int, [C.c_int32]),
     "vgpu_sort_autotune_report": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vgpu_group_by_fused_last_pass": (C.c_int, [C.c_int32]),
     "vgpu_server_create": (C.c_int, [C.POINTER(ServerOptions), C.POINTER(C.c_void_p)]),
     "vgpu_server_destroy": (C.c_int, [C.c_void_p]),
     "vgpu_server_add_template": (C.c_int, [C.c_void_p, C.POINTER(Template), C.POINTER(C.c_int32)]),
@@ -599,6 +600,11 @@ def join_pairs(akey, bkey, capacity=None, stream=None):
 
 def sort_autotune(on=True):
     check(load().vgpu_sort_autotune(1 if on else 0))
+
+
+def group_by_fused_last_pass(mode):
+    """0 never, 1 when it moves fewer bytes (default), 2 whenever the aggregate mix is supported."""
+    check(load().vgpu_group_by_fused_last_pass(int(mode)))
 
 
 def sort_autotune_report():

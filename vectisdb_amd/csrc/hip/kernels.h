@@ -93,6 +93,7 @@ struct KeyPack {
 	int shift[kMaxKeyCols] = {};
 	int bits[kMaxKeyCols] = {};
 	int total_bits = 0; // sum of bits; <= 64
+	uint64_t max_packed = 0; // largest packed key the ranges allow (0: not known, 2^total_bits - 1 is assumed)
 };
 
 //! Ordered min/max of each key column (synchronizes: the host needs them to plan the passes).
@@ -132,6 +133,9 @@ struct SortTuning {
 	//! caller), and the sort's payload is then wrong: the caller sorts again without narrowing.
 	uint32_t narrow = 0;
 	uint32_t *overflow = nullptr;
+	//! Leave the last (most significant) digit pass to the caller when it is an 8-bit pass that is not the
+	//! first: SortedRun then holds the keys ordered by every bit below that digit (see RunReduceLastPass).
+	bool leave_last = false;
 };
 
 //! Output of SortPacked: the compact keys (uint32 when total_bits <= 32, else uint64) in ascending order and
@@ -144,6 +148,13 @@ struct SortedRun {
 	int npayload = 0;
 	int passes = 0;
 	int radix_bits = 0;
+	//! Set when the sort left its last digit pass (SortTuning::leave_last): that digit's bits, its row
+	//! histogram on the device, and the largest packed key.
+	bool last_pending = false;
+	int last_shift = 0;
+	int last_bits = 0;
+	const uint32_t *last_hist = nullptr;
+	uint64_t max_packed = 0;
 };
 
 //! LSD radix sort of the packed key with payload. payload_in[i] == nullptr with width 4 means "generate
@@ -213,6 +224,19 @@ struct ReduceSpec {
 //! count is read on the device as in SortPackedWith (n is its bound).
 void SegmentedReduce(const SortedRun &run, uint64_t n, const ReduceSpec &spec, uint64_t *groups_device,
                      Scratch &scratch, hipStream_t stream, const uint64_t *n_dev = nullptr);
+
+//! The GROUP BY's last digit pass fused with the reduction, for a SortedRun whose sort left that pass
+//! (last_pending): each 8192-row tile is ranked by the digit in LDS, which leaves its rows in full-key order,
+//! and reduces its runs of equal keys to one partial state each; a small reduction then combines each key's
+//! partials into its group. Same outputs as SegmentedReduce over the fully sorted run.
+void RunReduceLastPass(const SortedRun &run, uint64_t n, const ReduceSpec &spec, uint64_t *groups_device,
+                       Scratch &scratch, hipStream_t stream);
+//! The aggregate mixes the fused pass is compiled for (COUNT alone, or with one SUM, or the
+//! COUNT/SUM/MIN/MAX + DOUBLE SUM mix, with the integer column carried at 4 or 8 bytes).
+bool RunReduceSupported(const ReduceSpec &spec, int npayload, bool keys64);
+//! Partial states the fused pass may write: per value of the last digit, at most one per distinct value of the
+//! bits below it plus one per tile boundary.
+uint64_t RunReduceRecordBound(uint64_t n, int shift, int bits, uint64_t max_packed);
 
 // ------------------------------------------------------------------ join
 

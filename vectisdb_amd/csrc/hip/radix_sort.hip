@@ -442,6 +442,7 @@ void PlanKeyPackAligned(const void *col, int vtype, uint64_t omin, uint64_t omax
 	pack.shift[0] = 0;
 	pack.total_bits = bits;
 	pack.min[0] = bits >= 64 ? 0ull : (omin & ~((1ull << bits) - 1));
+	pack.max_packed = omax - pack.min[0];
 }
 
 bool PlanKeyPack(const void *const *cols, const int *vtypes, int ncols, const uint64_t *mins, const uint64_t *maxs,
@@ -470,6 +471,11 @@ bool PlanKeyPack(const void *const *cols, const int *vtypes, int ncols, const ui
 	for (int c = ncols - 1; c >= 0; c--) {
 		pack.shift[c] = pack.bits[c] > 0 ? shift : 0;
 		shift += pack.bits[c];
+	}
+	for (int c = 0; c < ncols; c++) {
+		if (pack.bits[c] > 0) {
+			pack.max_packed |= (maxs[c] - mins[c]) << pack.shift[c];
+		}
 	}
 	return true;
 }
@@ -599,7 +605,10 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 	for (int p = 0; p < npayload; p++) {
 		cur_pay[p] = payload_in[p];
 	}
-	for (int pass = 0; pass < P; pass++) {
+	// The caller may take over the last digit pass (an 8-bit pass that is not the first).
+	const bool leave = tuning.leave_last && P >= 2 && R == 8 && n_dev == nullptr;
+	const int run_passes = leave ? P - 1 : P;
+	for (int pass = 0; pass < run_passes; pass++) {
 		int slot = pass & 1;
 		PayloadArgs pa;
 		pa.n = npayload;
@@ -626,6 +635,14 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 	for (int p = 0; p < npayload; p++) {
 		out.payload[p] = const_cast<void *>(cur_pay[p]);
 		out.payload_width[p] = payload_width[p];
+	}
+	out.max_packed = pack.max_packed != 0 ? pack.max_packed
+	                                      : (pack.total_bits >= 64 ? ~0ull : (1ull << pack.total_bits) - 1);
+	if (leave) {
+		out.last_pending = true;
+		out.last_shift = passes.shift[P - 1];
+		out.last_bits = passes.bits[P - 1];
+		out.last_hist = hist + (uint64_t)(P - 1) * RADIX;
 	}
 }
 

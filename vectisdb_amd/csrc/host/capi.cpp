@@ -281,6 +281,15 @@ int vgpu_sort_autotune(int32_t on) {
 	return Guard([&] { SortAutotuneEnable(on != 0); });
 }
 
+int vgpu_group_by_fused_last_pass(int32_t mode) {
+	return Guard([&] {
+		if (mode < 0 || mode > 2) {
+			throw std::runtime_error("vgpu_group_by_fused_last_pass: mode is 0, 1 or 2");
+		}
+		SetRunReduceMode(mode);
+	});
+}
+
 int vgpu_sort_autotune_report(char *buf, size_t cap, size_t *needed) {
 	return Guard([&] {
 		const std::string r = SortAutotuneReport();

@@ -9,6 +9,8 @@
 #include "operators.h"
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <deque>
 #include <mutex>
 #include <stdexcept>
@@ -42,7 +44,46 @@ void NoteWide(const void *col, uint64_t rows) {
 bool IsNumeric(int t) {
 	return t == VT_INT16 || t == VT_INT32 || t == VT_INT64 || IsFloat(t);
 }
+
+std::atomic<int> g_run_mode {[] {
+	const char *e = std::getenv("VGPU_RUN_REDUCE");
+	return e == nullptr ? 1 : std::max(0, std::min(2, std::atoi(e)));
+}()};
+
+//! The last radix pass fused with the reduction pays when the partial states it may write (one per distinct
+//! low-bits value and tile boundary, per value of the top digit), written once and read back by the record
+//! combine, are fewer bytes than what the plain path moves after the last ranking: the rows the last pass
+//! scatters, the keys the head count reads and the rows the segmented reduction reads.
+bool RunReduceWorthIt(const KeyPack &pack, const ReduceSpec &rs, const std::vector<int> &widths, uint64_t rows) {
+	const int mode = g_run_mode.load();
+	const int B = std::max(pack.total_bits, 1);
+	const int P = (B + 7) / 8;
+	if (mode == 0 || P < 2 || rows == 0 || !RunReduceSupported(rs, (int)widths.size(), B > 32)) {
+		return false;
+	}
+	if (mode == 2) {
+		return true;
+	}
+	const int last_bits = B / P;
+	const int shift = B - last_bits;
+	const uint64_t max_packed = pack.max_packed != 0 ? pack.max_packed : (B >= 64 ? ~0ull : (1ull << B) - 1);
+	const uint64_t records = RunReduceRecordBound(rows, shift, last_bits, max_packed);
+	const double kw = B > 32 ? 8.0 : 4.0;
+	double row_bytes = kw;
+	for (int w : widths) {
+		row_bytes += w;
+	}
+	return (double)records * (3.0 * kw + 32.0 * rs.ncomp) <= (double)rows * (2.0 * row_bytes + kw);
+}
 } // namespace
+
+void SetRunReduceMode(int mode) {
+	g_run_mode.store(std::max(0, std::min(2, mode)));
+}
+
+int RunReduceMode() {
+	return g_run_mode.load();
+}
 
 void ValidateAggregateReq(const AggReq &agg, const std::vector<DevCol> &values) {
 	if (agg.kind == AGG_COUNT_STAR) {
@@ -325,10 +366,18 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
 			t.overflow = overflow;
 			FillU32(overflow, 0u, 1, stream);
 		}
+		t.leave_last = rows_dev == nullptr && RunReduceWorthIt(pack, rs, widths, rows);
+		if (t.leave_last) {
+			t.radix_bits = 8; // the fused pass is an 8-bit digit pass
+		}
 		SortedRun run;
 		SortPacked(pack, payload_in.data(), widths.data(), (int)payload_in.size(), rows, t, run, scratch, stream,
 		           rows_dev);
-		SegmentedReduce(run, rows, rs, groups_dev, scratch, stream, rows_dev);
+		if (run.last_pending) {
+			RunReduceLastPass(run, rows, rs, groups_dev, scratch, stream);
+		} else {
+			SegmentedReduce(run, rows, rs, groups_dev, scratch, stream, rows_dev);
+		}
 	};
 	if (rows == 0) {
 		FillU64(groups_dev, 0ull, 1, stream);

@@ -64,6 +64,12 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
                        uint64_t *groups_device = nullptr, const uint64_t *rows_dev = nullptr,
                        bool narrow_payload = false);
 
+//! The GROUP BY's last radix pass fused with the reduction (RunReduceLastPass): 0 never, 1 when the partial
+//! states it may write are fewer bytes than the rows a plain last pass scatters (the default, or the value of
+//! VGPU_RUN_REDUCE at first use), 2 whenever the aggregate mix is supported.
+void SetRunReduceMode(int mode);
+int RunReduceMode();
+
 // ------------------------------------------------------------------ join
 
 struct JoinAggReq {
