@@ -94,6 +94,9 @@ struct DynSig {
 	__host__ __device__ static constexpr int code(int) {
 		return 0;
 	}
+	__host__ __device__ static constexpr bool has(int) {
+		return true;
+	}
 };
 
 template <int... K>
@@ -105,6 +108,24 @@ struct StaticSig {
 		((r = (i++ == c) ? K : r), ...);
 		return r;
 	}
+	__host__ __device__ static constexpr bool has(int) {
+		return true;
+	}
+};
+
+//! The components of a static signature selected by MASK (bit c = component c): every state operation skips
+//! the others, so a kernel can fold and scan one payload column's components at a time with only their
+//! registers live. Component positions (and so the stored state layout) are those of SG.
+template <typename SG, unsigned MASK>
+struct MaskSig {
+	static constexpr bool kStatic = true;
+	static constexpr int n = SG::n;
+	__host__ __device__ static constexpr int code(int c) {
+		return SG::code(c);
+	}
+	__host__ __device__ static constexpr bool has(int c) {
+		return ((MASK >> c) & 1u) != 0;
+	}
 };
 
 constexpr int SigCode(int kind, int in_type = 0, int payload = -1) {
@@ -114,7 +135,7 @@ constexpr int SigCode(int kind, int in_type = 0, int payload = -1) {
 template <typename S>
 __device__ __forceinline__ bool CHas(const CompDesc &d, int c) {
 	if constexpr (S::kStatic) {
-		return true;
+		return S::has(c);
 	} else {
 		return c < d.ncomp;
 	}

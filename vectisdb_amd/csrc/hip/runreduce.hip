@@ -21,6 +21,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <string>
+#include <type_traits>
 
 namespace vgpu {
 namespace {
@@ -57,15 +59,291 @@ __device__ __forceinline__ uint32_t RunSlot(uint32_t p, int items, int stride) {
 	return (p % (uint32_t)items) * (uint32_t)stride + p / (uint32_t)items;
 }
 
+//! Components folded in phase PH: those reading payload column PH, and COUNT in phase 0.
+template <typename SG>
+constexpr unsigned PhaseMask(int ph) {
+	unsigned m = 0;
+	for (int c = 0; c < SG::n; c++) {
+		const int code = SG::code(c);
+		if ((code & 15) == COMP_COUNT) {
+			m |= ph == 0 ? (1u << c) : 0u;
+		} else if ((code >> 8) - 1 == ph) {
+			m |= 1u << c;
+		}
+	}
+	return m;
+}
+
+constexpr int VTypeWidth(int vtype) {
+	return vtype == VT_INT16 ? 2 : (vtype == VT_INT32 || vtype == VT_FLOAT32) ? 4 : vtype == VT_BOOLEAN ? 1 : 8;
+}
+
+//! Bytes per value of payload column p as the signature reads it (the input type of its components).
+template <typename SG>
+constexpr int PayloadWidth(int p) {
+	for (int c = 0; c < SG::n; c++) {
+		const int code = SG::code(c);
+		if ((code & 15) != COMP_COUNT && (code >> 8) - 1 == p) {
+			return VTypeWidth((code >> 4) & 15);
+		}
+	}
+	return 8;
+}
+
+template <int W>
+using ColT = typename std::conditional<W == 8, u64,
+                                       typename std::conditional<W == 4, uint32_t,
+                                                                 typename std::conditional<W == 2, uint16_t,
+                                                                                           uint8_t>::type>::type>::type;
+
+// ---------------------------------------------------------------- tile-local run states
+//
+// Inside one tile a run has at most 2^16 rows, so its state fits narrower types than a group's: COUNT in 32
+// bits, the sum of a column of at most 32 bits in 64, MIN/MAX of such a column in 32. The fold and the
+// cross-thread scans run on these (half the instructions and shuffles of the 128-bit group state); a run's
+// record is written in the group state's layout (two 64-bit words per component), so the record combine
+// is the ordinary state combine.
+
+template <int CODE, bool ACTIVE>
+struct TComp {
+	// An inactive component (not folded in this phase): no storage, no work.
+	__device__ __forceinline__ void identity() {}
+	__device__ __forceinline__ void add(u64) {}
+	__device__ __forceinline__ void combine(const TComp &) {}
+	__device__ __forceinline__ void shfl_up(const TComp &, int) {}
+	__device__ __forceinline__ void store(u64 *) const {}
+	__device__ __forceinline__ void load(const u64 *) {}
+};
+
+constexpr int CodeKind(int code) {
+	return code & 15;
+}
+constexpr int CodeType(int code) {
+	return (code >> 4) & 15;
+}
+constexpr bool CodeNarrowInt(int code) {
+	return CodeType(code) == VT_INT32 || CodeType(code) == VT_INT16 || CodeType(code) == VT_BOOLEAN;
+}
+
+template <int CODE>
+struct TComp<CODE, true> {
+	static constexpr int K = CodeKind(CODE);
+	static constexpr int T = CodeType(CODE);
+	static constexpr bool kWideSum = K == COMP_SUM_I128 && !CodeNarrowInt(CODE);
+	static constexpr bool kNarrowMinMax = (K == COMP_MIN_I64 || K == COMP_MAX_I64) && CodeNarrowInt(CODE);
+	// COUNT: u32 | SUM of <= 32-bit ints: i64 | SUM of INT64: (lo, hi) | SUM_DD: (hi, lo) doubles |
+	// MIN/MAX of <= 32-bit ints: i32 | MIN/MAX of INT64: i64 | MIN/MAX of doubles: double bits
+	using A = typename std::conditional<
+	    K == COMP_COUNT || kNarrowMinMax, uint32_t, u64>::type;
+	static constexpr bool kHasB = K == COMP_SUM_DD || kWideSum;
+	A a;
+	u64 b;
+
+	__device__ __forceinline__ void identity() {
+		b = 0;
+		switch (K) {
+		case COMP_MIN_I64:
+			a = kNarrowMinMax ? (A)0x7fffffffu : (A)0x7fffffffffffffffull;
+			break;
+		case COMP_MAX_I64:
+			a = kNarrowMinMax ? (A)0x80000000u : (A)0x8000000000000000ull;
+			break;
+		case COMP_MIN_F64:
+			a = (A)AsBits(__builtin_nan(""));
+			break;
+		case COMP_MAX_F64:
+			a = (A)AsBits(-__builtin_inf());
+			break;
+		default:
+			a = 0;
+			break;
+		}
+	}
+	__device__ __forceinline__ void add(u64 raw) {
+		if constexpr (K == COMP_COUNT) {
+			a += 1;
+		} else if constexpr (K == COMP_SUM_I128) {
+			if constexpr (kWideSum) {
+				I128AddI64(a, b, (long long)raw);
+			} else {
+				a = (A)((long long)a + RawAsI64(raw, T));
+			}
+		} else if constexpr (K == COMP_SUM_DD) {
+			double hi = AsDouble(a), lo = AsDouble(b);
+			DdAdd(hi, lo, RawAsF64(raw, T));
+			a = AsBits(hi);
+			b = AsBits(lo);
+		} else if constexpr (K == COMP_MIN_I64 || K == COMP_MAX_I64) {
+			if constexpr (kNarrowMinMax) {
+				const int v = (int)RawAsI64(raw, T);
+				a = (A)(K == COMP_MIN_I64 ? min((int)a, v) : max((int)a, v));
+			} else {
+				const long long v = (long long)raw;
+				a = (A)(K == COMP_MIN_I64 ? (v < (long long)a ? v : (long long)a)
+				                          : (v > (long long)a ? v : (long long)a));
+			}
+		} else if constexpr (K == COMP_MIN_F64) {
+			const double v = RawAsF64(raw, T);
+			if (DuckLess(v, AsDouble(a))) {
+				a = AsBits(v);
+			}
+		} else if constexpr (K == COMP_MAX_F64) {
+			const double v = RawAsF64(raw, T);
+			if (DuckLess(AsDouble(a), v)) {
+				a = AsBits(v);
+			}
+		}
+	}
+	//! this = this (earlier rows) combined with o (later rows)
+	__device__ __forceinline__ void combine(const TComp &o) {
+		if constexpr (K == COMP_COUNT) {
+			a += o.a;
+		} else if constexpr (K == COMP_SUM_I128) {
+			if constexpr (kWideSum) {
+				I128AddI128(a, b, o.a, o.b);
+			} else {
+				a = (A)((long long)a + (long long)o.a);
+			}
+		} else if constexpr (K == COMP_SUM_DD) {
+			double hi = AsDouble(a), lo = AsDouble(b);
+			DdAddDd(hi, lo, AsDouble(o.a), AsDouble(o.b));
+			a = AsBits(hi);
+			b = AsBits(lo);
+		} else if constexpr (K == COMP_MIN_I64 || K == COMP_MAX_I64) {
+			if constexpr (kNarrowMinMax) {
+				a = (A)(K == COMP_MIN_I64 ? min((int)a, (int)o.a) : max((int)a, (int)o.a));
+			} else {
+				const long long x = (long long)a, y = (long long)o.a;
+				a = (A)(K == COMP_MIN_I64 ? (y < x ? y : x) : (y > x ? y : x));
+			}
+		} else if constexpr (K == COMP_MIN_F64) {
+			if (DuckLess(AsDouble(o.a), AsDouble(a))) {
+				a = o.a;
+			}
+		} else if constexpr (K == COMP_MAX_F64) {
+			if (DuckLess(AsDouble(a), AsDouble(o.a))) {
+				a = o.a;
+			}
+		}
+	}
+	__device__ __forceinline__ void shfl_up(const TComp &src, int delta) {
+		a = __shfl_up(src.a, delta);
+		if constexpr (kHasB) {
+			b = __shfl_up(src.b, delta);
+		} else {
+			b = 0;
+		}
+	}
+	//! The group state's two words (agg_state.h layout).
+	__device__ __forceinline__ void store(u64 *w) const {
+		if constexpr (K == COMP_SUM_I128 && !kWideSum) {
+			w[0] = (u64)a;
+			w[1] = (long long)a < 0 ? ~0ull : 0ull;
+		} else if constexpr (kNarrowMinMax) {
+			w[0] = (u64)(long long)(int)a;
+			w[1] = 0;
+		} else {
+			w[0] = (u64)a;
+			w[1] = b;
+		}
+	}
+	//! Raw words for the cross-wave exchange through LDS (not the group layout).
+	__device__ __forceinline__ void put(u64 *w) const {
+		w[0] = (u64)a;
+		w[1] = b;
+	}
+	__device__ __forceinline__ void get(const u64 *w) {
+		a = (A)w[0];
+		b = w[1];
+	}
+};
+
+//! A phase's run state: components I.. of the signature, those selected by MASK active.
+template <unsigned MASK, int I, int... C>
+struct TList {
+	__device__ __forceinline__ void identity() {}
+	template <int NV>
+	__device__ __forceinline__ void add(const u64 (&)[NV]) {}
+	__device__ __forceinline__ void combine(const TList &) {}
+	__device__ __forceinline__ void shfl_up(const TList &, int) {}
+	__device__ __forceinline__ void store(u64 *) const {}
+	__device__ __forceinline__ void put(u64 *) const {}
+	__device__ __forceinline__ void get(const u64 *) {}
+};
+
+template <unsigned MASK, int I, int C0, int... Cs>
+struct TList<MASK, I, C0, Cs...> {
+	static constexpr bool kOn = ((MASK >> I) & 1u) != 0;
+	static constexpr int kPayload = (C0 >> 8) - 1;
+	TComp<C0, kOn> h;
+	TList<MASK, I + 1, Cs...> t;
+	__device__ __forceinline__ void identity() {
+		h.identity();
+		t.identity();
+	}
+	template <int NV>
+	__device__ __forceinline__ void add(const u64 (&v)[NV]) {
+		h.add(kPayload >= 0 && kPayload < NV ? v[kPayload >= 0 && kPayload < NV ? kPayload : 0] : 0ull);
+		t.add(v);
+	}
+	__device__ __forceinline__ void combine(const TList &o) {
+		h.combine(o.h);
+		t.combine(o.t);
+	}
+	__device__ __forceinline__ void shfl_up(const TList &src, int delta) {
+		h.shfl_up(src.h, delta);
+		t.shfl_up(src.t, delta);
+	}
+	__device__ __forceinline__ void store(u64 *rec) const {
+		if constexpr (kOn) {
+			h.store(rec + 2 * I);
+		}
+		t.store(rec);
+	}
+	__device__ __forceinline__ void put(u64 *w) const {
+		if constexpr (kOn) {
+			h.put(w + 2 * I);
+		}
+		t.put(w);
+	}
+	__device__ __forceinline__ void get(const u64 *w) {
+		if constexpr (kOn) {
+			h.get(w + 2 * I);
+		}
+		t.get(w);
+	}
+};
+
+template <typename SG>
+struct TileStateOf;
+template <int... K>
+struct TileStateOf<StaticSig<K...>> {
+	template <unsigned MASK>
+	using State = TList<MASK, 0, K...>;
+};
+
+//! Wave-striped column load (row = wave_base + k * 64 + lane) in the column's own type.
+template <int ITEMS, typename T>
+__device__ __forceinline__ void LoadWaveCol(const T *src, uint64_t wave_base, int lane, uint64_t n, T (&out)[ITEMS]) {
+#pragma unroll
+	for (int k = 0; k < ITEMS; k++) {
+		const uint64_t row = wave_base + (uint64_t)k * kWave + lane;
+		out[k] = row < n ? src[row] : (T)0;
+	}
+}
+
 //! One tile: rank by the last digit (wave multisplit, stable), stage keys in LDS in that order (transposed:
 //! sorted row p at slot (p % ITEMS) * STRIDE + p / ITEMS, so thread t reads its ITEMS consecutive sorted rows
-//! with conflict-free loads), find the runs of equal keys, stage each payload column the same way, fold each
-//! thread's rows serially and stitch runs across threads with a segmented scan, then write each run's key and
-//! state at its record position.
-template <typename KeyT, int NP, typename SG, int BLOCK, int ITEMS>
-__global__ void __launch_bounds__(BLOCK) RunReduceKernel(RunPassArgs a, CompDesc d) {
+//! with conflict-free loads), find the runs of equal keys, then one phase per payload column: stage the column
+//! the same way, fold each thread's rows serially into that column's components, stitch runs across threads
+//! with a segmented scan (wave shuffles, then one wave scanning the wave totals), and write those components
+//! of each run's record. Column p + 1 is loaded while phase p runs.
+template <typename KeyT, int NP, typename SG, int BLOCK, int ITEMS, bool PF1>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 256 ? 4 : 1)))
+RunReduceKernel(RunPassArgs a, CompDesc d) {
 	static_assert(BLOCK >= kRunRadix, "one thread per digit");
 	static_assert(ITEMS <= 32, "head flags are a 32-bit mask");
+	static_assert(NP <= 2, "the fused pass is compiled for at most two payload columns");
 	constexpr int RADIX = kRunRadix;
 	constexpr int kWaves = BLOCK / kWave;
 	constexpr int TILE = BLOCK * ITEMS;
@@ -74,14 +352,20 @@ __global__ void __launch_bounds__(BLOCK) RunReduceKernel(RunPassArgs a, CompDesc
 	constexpr int SC = SG::n;
 	constexpr int W_COUNT = kWaves * RADIX;
 	constexpr int W_KEYS = SLOTS * (int)(sizeof(KeyT) / 4);
-	constexpr int W_PAY = NP > 0 ? 2 * SLOTS : 0;
+	constexpr int W0 = NP > 0 ? PayloadWidth<SG>(0) : 4;
+	constexpr int W1 = NP > 1 ? PayloadWidth<SG>(1) : 4;
+	constexpr int W_PAY = (W0 == 8 || W1 == 8) ? 2 * SLOTS : SLOTS;
 	constexpr int W_A = W_COUNT > W_KEYS ? W_COUNT : W_KEYS;
 	constexpr int W = W_A > W_PAY ? W_A : W_PAY;
+	using T0 = ColT<W0>;
+	using T1 = ColT<W1>;
 	__shared__ __align__(16) uint32_t s_buf[W];
 	__shared__ uint32_t s_runs[RADIX];
 	__shared__ uint32_t s_rbase[RADIX];
 	__shared__ uint32_t s_scan[kWaves + 1];
+	__shared__ u64 s_scan64[kWaves + 1];
 	__shared__ u64 s_wave_state[kWaves][2 * SC];
+	__shared__ u64 s_wave_pref[kWaves][2 * SC];
 	__shared__ int s_wave_flag[kWaves];
 	__shared__ uint32_t s_tile;
 	__shared__ KeyT s_last_key;
@@ -107,14 +391,23 @@ __global__ void __launch_bounds__(BLOCK) RunReduceKernel(RunPassArgs a, CompDesc
 	const int dshift = a.shift;
 	const KeyT *keys_in = reinterpret_cast<const KeyT *>(a.keys);
 
-	uint32_t pos[ITEMS];
-	{
-		KeyT key[ITEMS];
+	// Each row's LDS slot in sorted order, two 16-bit slots per register (a slot is below ITEMS * STRIDE).
+	static_assert(ITEMS % 2 == 0 && SLOTS < 65536, "16-bit slots");
+	uint32_t slots[ITEMS / 2];
+	auto slot_of = [&](int k) -> uint32_t { return (slots[k / 2] >> (16 * (k & 1))) & 0xffffu; };
+	KeyT key[ITEMS];
 #pragma unroll
-		for (int k = 0; k < ITEMS; k++) {
-			const uint64_t row = wave_base + (uint64_t)k * kWave + lane;
-			key[k] = row < n ? keys_in[row] : (KeyT)0;
-		}
+	for (int k = 0; k < ITEMS; k++) {
+		const uint64_t row = wave_base + (uint64_t)k * kWave + lane;
+		key[k] = row < n ? keys_in[row] : (KeyT)0;
+	}
+	// The first payload column is loaded now: its latency hides behind the ranking.
+	T0 col0[ITEMS]; // unused (and dropped) without payload
+	if constexpr (NP > 0) {
+		LoadWaveCol<ITEMS>(reinterpret_cast<const T0 *>(a.pay[0]), wave_base, lane, n, col0);
+	}
+	{
+		uint32_t pos[ITEMS];
 		// Wave multisplit, as in the sort pass: rank = this wave's earlier count of the digit + earlier peers.
 		uint32_t *my_counts = s_buf + wave * RADIX;
 #pragma unroll
@@ -167,13 +460,17 @@ __global__ void __launch_bounds__(BLOCK) RunReduceKernel(RunPassArgs a, CompDesc
 				pos[k] += my_counts[(uint32_t)(key[k] >> dshift) & dmask];
 			}
 		}
+#pragma unroll
+		for (int k = 0; k < ITEMS; k += 2) {
+			slots[k / 2] = RunSlot(pos[k], ITEMS, STRIDE) | (RunSlot(pos[k + 1], ITEMS, STRIDE) << 16);
+		}
 		__syncthreads();
 		KeyT *s_keys = reinterpret_cast<KeyT *>(s_buf);
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint64_t row = wave_base + (uint64_t)k * kWave + lane;
 			if (row < n) {
-				s_keys[RunSlot(pos[k], ITEMS, STRIDE)] = key[k];
+				s_keys[slot_of(k)] = key[k];
 			}
 		}
 		__syncthreads();
@@ -193,8 +490,11 @@ __global__ void __launch_bounds__(BLOCK) RunReduceKernel(RunPassArgs a, CompDesc
 			sk[i] = r0 + i < tile_count ? s_keys[i * STRIDE + tid] : (KeyT)0;
 		}
 		kprev = tid > 0 ? s_keys[(ITEMS - 1) * STRIDE + tid - 1] : (KeyT)0;
-		if (r0 < tile_count && tile_count <= r0 + ITEMS) {
-			s_last_key = sk[tile_count - 1 - r0];
+#pragma unroll
+		for (int i = 0; i < ITEMS; i++) {
+			if (r0 + i + 1 == tile_count) {
+				s_last_key = sk[i];
+			}
 		}
 		KeyT before = kprev;
 #pragma unroll
@@ -210,189 +510,232 @@ __global__ void __launch_bounds__(BLOCK) RunReduceKernel(RunPassArgs a, CompDesc
 	}
 	uint32_t H;
 	const uint32_t hb_excl = BlockExclusiveScan<uint32_t, BLOCK>(hc, s_scan, &H);
+	// The key of the run this thread's first head closes (the row just before that head).
+	KeyT key_before_first = kprev;
+	{
+		const int fh = heads != 0 ? __builtin_ctz(heads) : 0;
+#pragma unroll
+		for (int i = 0; i + 1 < ITEMS; i++) {
+			if (i + 1 == fh) {
+				key_before_first = sk[i];
+			}
+		}
+	}
 
 	// Runs per digit: published at once (aggregate flag) so later tiles can go on; the look-back comes after
-	// the payload staging, by when the predecessors have most likely published too.
+	// the first column is staged, by when the predecessors have most likely published too. One scan gives
+	// each digit's first run in the tile (high word) and the start of its record region (low word: regions
+	// are min(rows of the digit, region_cap) long, and their total is below 2^32).
 	const uint32_t rc = tid < RADIX ? s_runs[tid] : 0u;
 	if (tid < RADIX) {
-		StoreAgent(a.tile_state + (uint64_t)tile * RADIX + tid, (tile == 0 ? sortpass::kStFlagInc : sortpass::kStFlagAgg) | rc);
+		StoreAgent(a.tile_state + (uint64_t)tile * RADIX + tid,
+		           (tile == 0 ? sortpass::kStFlagInc : sortpass::kStFlagAgg) | rc);
 	}
-	uint32_t runs_total;
-	const uint32_t rfirst = BlockExclusiveScan<uint32_t, BLOCK>(rc, s_scan, &runs_total);
 	const uint32_t rsize = tid < RADIX ? min(a.ghist[tid], a.region_cap) : 0u;
-	uint32_t regions_total;
-	const uint32_t rstart = BlockExclusiveScan<uint32_t, BLOCK>(rsize, s_scan, &regions_total);
-
-	// Payload columns: staged in sorted order, each thread keeps its ITEMS values (raw bits).
-	u64 pv[NP > 0 ? NP : 1][ITEMS];
-#pragma unroll
-	for (int p = 0; p < NP; p++) {
-		if (p < a.npay) {
-			u64 v[ITEMS];
-			sortpass::LoadWaveItems<ITEMS>(a.pay[p], a.width[p], wave_base, lane, n, v);
-			__syncthreads();
-			if (a.width[p] == 8) {
-				u64 *s_pay = reinterpret_cast<u64 *>(s_buf);
-#pragma unroll
-				for (int k = 0; k < ITEMS; k++) {
-					if (wave_base + (uint64_t)k * kWave + lane < n) {
-						s_pay[RunSlot(pos[k], ITEMS, STRIDE)] = v[k];
-					}
-				}
-				__syncthreads();
-#pragma unroll
-				for (int i = 0; i < ITEMS; i++) {
-					pv[p][i] = s_pay[i * STRIDE + tid];
-				}
-			} else {
-#pragma unroll
-				for (int k = 0; k < ITEMS; k++) {
-					if (wave_base + (uint64_t)k * kWave + lane < n) {
-						s_buf[RunSlot(pos[k], ITEMS, STRIDE)] = (uint32_t)v[k];
-					}
-				}
-				__syncthreads();
-#pragma unroll
-				for (int i = 0; i < ITEMS; i++) {
-					pv[p][i] = s_buf[i * STRIDE + tid];
-				}
-			}
-		}
-	}
-
-	// Look-back over the predecessors' per-digit run counts (thread d owns digit d).
-	if (tid < RADIX) {
-		uint32_t excl = 0;
-		if (tile > 0) {
-			int64_t t = (int64_t)tile - 1;
-			while (true) {
-				const uint32_t word = LoadAgent(a.tile_state + (uint64_t)t * RADIX + tid);
-				const uint32_t flag = word & (3u << 30);
-				if (flag == 0) {
-					__builtin_amdgcn_s_sleep(1);
-					continue;
-				}
-				excl += word & sortpass::kStValue;
-				if (flag == sortpass::kStFlagInc) {
-					break;
-				}
-				t--;
-			}
-			StoreAgent(a.tile_state + (uint64_t)tile * RADIX + tid, sortpass::kStFlagInc | (excl + rc));
-		}
-		s_rbase[tid] = rstart + excl - rfirst;
-		if (tile == a.ntiles - 1) {
-			a.layout[RADIX + 1 + tid] = rstart;
-			s_runs[tid] = excl + rc; // this digit's record count (every tile's runs)
-		}
-	}
-	if (tile == a.ntiles - 1) {
-		// Dense start of each digit's records, and the record count.
-		__syncthreads();
-		const uint32_t tot = tid < RADIX ? s_runs[tid] : 0u;
-		uint32_t all;
-		const uint32_t dstart = BlockExclusiveScan<uint32_t, BLOCK>(tot, s_scan, &all);
-		if (tid < RADIX) {
-			a.layout[tid] = dstart;
-		}
-		if (tid == 0) {
-			a.layout[RADIX] = all;
-			*a.nrec = all;
-		}
-	}
-	__syncthreads();
+	u64 both_total;
+	const u64 both = BlockExclusiveScan<u64, BLOCK>(((u64)rc << 32) | rsize, s_scan64, &both_total);
+	const uint32_t rfirst = (uint32_t)(both >> 32);
+	const uint32_t rstart = (uint32_t)both;
 
 	KeyT *rec_keys = reinterpret_cast<KeyT *>(a.rec_keys);
-	auto write_run = [&](KeyT key, uint32_t run, const CompState &st) {
-		const uint64_t at = (uint64_t)s_rbase[(uint32_t)(key >> dshift) & dmask] + run;
-		rec_keys[at] = key;
-		StateStore<SG>(d, a.rec_state + at * (2 * SC), st);
-	};
+	T1 col1[ITEMS]; // unused (and dropped) below two payload columns
 
-	// Serial fold of this thread's rows; runs that start and end here are written at once.
-	CompState S, lead;
-	StateIdentity<SG>(d, S);
-	StateIdentity<SG>(d, lead);
-	bool seen = false;
-	uint32_t cur_run = 0;
-	uint32_t nheads = 0;
-	KeyT cur_key = (KeyT)0;
-	KeyT key_before_first = kprev;
-#pragma unroll
-	for (int i = 0; i < ITEMS; i++) {
-		if (r0 + i < tile_count) {
-			if ((heads >> i) & 1u) {
-				if (seen) {
-					write_run(cur_key, cur_run, S);
-				} else {
-					lead = S;
-					seen = true;
-					key_before_first = i > 0 ? sk[i - 1] : kprev;
+	auto phase = [&](auto ph) {
+		constexpr int PH = decltype(ph)::value;
+		using TP = typename std::conditional<PH == 0, T0, T1>::type;
+		constexpr int WP = PH == 0 ? W0 : W1;
+		TP vals[ITEMS];
+		if constexpr (NP > 0) {
+			const TP *src = nullptr;
+			if constexpr (PH == 0) {
+				src = col0;
+			} else {
+				if constexpr (!PF1) {
+					LoadWaveCol<ITEMS>(reinterpret_cast<const T1 *>(a.pay[1]), wave_base, lane, n, col1);
 				}
-				cur_run = hb_excl + nheads;
-				nheads++;
-				cur_key = sk[i];
-				StateIdentity<SG>(d, S);
+				src = col1;
 			}
-			u64 vals[NP > 0 ? NP : 1];
+			__syncthreads(); // the LDS buffer is free again (keys or the previous column have been read)
+			TP *s_col = reinterpret_cast<TP *>(s_buf);
 #pragma unroll
-			for (int p = 0; p < (NP > 0 ? NP : 1); p++) {
-				vals[p] = (NP > 0 && p < a.npay) ? pv[p][i] : 0ull;
+			for (int k = 0; k < ITEMS; k++) {
+				if (wave_base + (uint64_t)k * kWave + lane < n) {
+					s_col[slot_of(k)] = src[k];
+				}
 			}
-			StateAddRow<(NP > 0 ? NP : 1), SG>(d, S, vals);
+			__syncthreads();
+#pragma unroll
+			for (int i = 0; i < ITEMS; i++) {
+				vals[i] = s_col[i * STRIDE + tid];
+			}
+			(void)WP;
 		}
-	}
+		if constexpr (PH == 0 && NP > 1 && PF1) {
+			// The next column's loads fly while this phase folds.
+			LoadWaveCol<ITEMS>(reinterpret_cast<const T1 *>(a.pay[1]), wave_base, lane, n, col1);
+		}
+		if constexpr (PH == 0) {
+			// Look-back over the predecessors' per-digit run counts (thread d owns digit d).
+			if (tid < RADIX) {
+				uint32_t excl = 0;
+				if (tile > 0) {
+					int64_t t = (int64_t)tile - 1;
+					while (true) {
+						const uint32_t word = LoadAgent(a.tile_state + (uint64_t)t * RADIX + tid);
+						const uint32_t flag = word & (3u << 30);
+						if (flag == 0) {
+							__builtin_amdgcn_s_sleep(1);
+							continue;
+						}
+						excl += word & sortpass::kStValue;
+						if (flag == sortpass::kStFlagInc) {
+							break;
+						}
+						t--;
+					}
+					StoreAgent(a.tile_state + (uint64_t)tile * RADIX + tid, sortpass::kStFlagInc | (excl + rc));
+				}
+				s_rbase[tid] = rstart + excl - rfirst;
+				if (tile == a.ntiles - 1) {
+					a.layout[RADIX + 1 + tid] = rstart;
+					s_runs[tid] = excl + rc; // this digit's record count (every tile's runs)
+				}
+			}
+			if (tile == a.ntiles - 1) {
+				// Dense start of each digit's records, and the record count.
+				__syncthreads();
+				const uint32_t tot = tid < RADIX ? s_runs[tid] : 0u;
+				uint32_t all;
+				const uint32_t dstart = BlockExclusiveScan<uint32_t, BLOCK>(tot, s_scan, &all);
+				if (tid < RADIX) {
+					a.layout[tid] = dstart;
+				}
+				if (tid == 0) {
+					a.layout[RADIX] = all;
+					*a.nrec = all;
+				}
+			}
+			__syncthreads();
+		}
+		using PS = typename TileStateOf<SG>::template State<PhaseMask<SG>(PH)>;
+		auto write_run = [&](KeyT k, uint32_t run, const PS &st) {
+			const uint64_t at = (uint64_t)s_rbase[(uint32_t)(k >> dshift) & dmask] + run;
+			if constexpr (PH == 0) {
+				rec_keys[at] = k;
+			}
+			st.store(a.rec_state + at * (2 * SC));
+		};
 
-	// Segmented inclusive scan of (seen, S) over the block: X covers from the latest head at or before this
-	// thread's last row.
-	CompState X = S;
-	bool F = seen;
-	for (int off = 1; off < kWave; off <<= 1) {
-		CompState up;
-		StateShflUp<SG>(d, up, X, off);
-		const bool fup = __shfl_up((int)F, off) != 0;
-		if (lane >= off) {
-			if (!F) {
-				StateCombine<SG>(d, up, X);
-				X = up;
+		// Serial fold of this thread's rows; runs that start and end here are written at once.
+		PS S, lead;
+		S.identity();
+		lead.identity();
+		bool seen = false;
+		uint32_t cur_run = 0;
+		uint32_t nheads = 0;
+		KeyT cur_key = (KeyT)0;
+#pragma unroll
+		for (int i = 0; i < ITEMS; i++) {
+			if (r0 + i < tile_count) {
+				if ((heads >> i) & 1u) {
+					if (seen) {
+						write_run(cur_key, cur_run, S);
+					} else {
+						lead = S;
+						seen = true;
+					}
+					cur_run = hb_excl + nheads;
+					nheads++;
+					cur_key = sk[i];
+					S.identity();
+				}
+				u64 v[NP > 0 ? NP : 1];
+#pragma unroll
+				for (int q = 0; q < (NP > 0 ? NP : 1); q++) {
+					v[q] = (NP > 0 && q == PH) ? (u64)vals[i] : 0ull;
+				}
+				S.add(v);
 			}
-			F = F || fup;
 		}
-	}
-	if (lane == kWave - 1) {
-		StateStore<SG>(d, s_wave_state[wave], X);
-		s_wave_flag[wave] = F ? 1 : 0;
-	}
-	__syncthreads();
-	CompState P;
-	StateIdentity<SG>(d, P);
-	for (int w = 0; w < wave; w++) {
-		CompState t;
-		StateLoad<SG>(d, s_wave_state[w], t);
-		if (s_wave_flag[w]) {
-			P = t;
-		} else {
-			StateCombine<SG>(d, P, t);
+
+		// Segmented inclusive scan of (seen, S) over the block: X covers from the latest head at or before this
+		// thread's last row.
+		PS X = S;
+		bool F = seen;
+		for (int off = 1; off < kWave; off <<= 1) {
+			PS up;
+			up.shfl_up(X, off);
+			const bool fup = __shfl_up((int)F, off) != 0;
+			if (lane >= off) {
+				if (!F) {
+					up.combine(X);
+					X = up;
+				}
+				F = F || fup;
+			}
 		}
-	}
-	if (!F) {
-		CompState t = P;
-		StateCombine<SG>(d, t, X);
-		X = t;
-	}
-	CompState E;
-	StateShflUp<SG>(d, E, X, 1);
-	if (lane == 0) {
-		E = P;
-	}
-	if (seen && hb_excl > 0) {
-		// The run open at this thread's start ends before its first head.
-		CompState V = E;
-		StateCombine<SG>(d, V, lead);
-		write_run(key_before_first, hb_excl - 1, V);
-	}
-	if (tid == BLOCK - 1) {
-		write_run(s_last_key, H - 1, X); // the tile's last run
+		if (lane == kWave - 1) {
+			X.put(s_wave_state[wave]);
+			s_wave_flag[wave] = F ? 1 : 0;
+		}
+		__syncthreads();
+		if (wave == 0) {
+			// Exclusive segmented scan of the wave totals, lane w for wave w.
+			PS Wst;
+			bool WF = false;
+			if (lane < kWaves) {
+				Wst.get(s_wave_state[lane]);
+				WF = s_wave_flag[lane] != 0;
+			} else {
+				Wst.identity();
+			}
+			for (int off = 1; off < kWaves; off <<= 1) {
+				PS up;
+				up.shfl_up(Wst, off);
+				const bool fup = __shfl_up((int)WF, off) != 0;
+				if (lane >= off) {
+					if (!WF) {
+						up.combine(Wst);
+						Wst = up;
+					}
+					WF = WF || fup;
+				}
+			}
+			PS Ex;
+			Ex.shfl_up(Wst, 1);
+			if (lane == 0) {
+				Ex.identity();
+			}
+			if (lane < kWaves) {
+				Ex.put(s_wave_pref[lane]);
+			}
+		}
+		__syncthreads();
+		PS P;
+		P.get(s_wave_pref[wave]);
+		if (!F) {
+			PS t = P;
+			t.combine(X);
+			X = t;
+		}
+		PS E;
+		E.shfl_up(X, 1);
+		if (lane == 0) {
+			E = P;
+		}
+		if (seen && hb_excl > 0) {
+			// The run open at this thread's start ends before its first head.
+			PS V = E;
+			V.combine(lead);
+			write_run(key_before_first, hb_excl - 1, V);
+		}
+		if (tid == BLOCK - 1) {
+			write_run(s_last_key, H - 1, X); // the tile's last run
+		}
+	};
+	phase(std::integral_constant<int, 0> {});
+	if constexpr (NP > 1) {
+		phase(std::integral_constant<int, 1> {});
 	}
 }
 
@@ -636,12 +979,46 @@ __global__ void __launch_bounds__(kRecBlock)
 //! Component signatures the fused pass is compiled for.
 using SigCount = StaticSig<SigCode(COMP_COUNT)>;
 
+//! Kernel shapes (threads x rows per thread, and whether the second payload column is loaded while the first
+//! is folded): 256 x 16 (the default), 256 x 16 loading the second column at its turn, 512 x 16, 1024 x 8.
+//! Measured on the 100M-row, 20-bit-key GROUP BY: 256 x 16 held to 128 VGPRs (4 waves per SIMD, 4 blocks per
+//! CU, 28 bytes of spill) 0.76 ms; the same at 151 VGPRs (3 blocks per CU) 0.84 ms; 512 x 16 1.08 ms and
+//! 1024 x 8 1.02 ms (one block per CU); the first form, with every column's full group state in registers,
+//! 1.23 ms. VGPU_RUN_TILE selects one for sweeps.
+struct RunShape {
+	int block;
+	int items;
+	bool prefetch;
+};
+
+RunShape RunShapeChoice() {
+	static const RunShape shape = [] {
+		const char *e = std::getenv("VGPU_RUN_TILE");
+		const std::string v = e != nullptr ? e : "";
+		if (v == "512x16") {
+			return RunShape {512, 16, true};
+		}
+		if (v == "1024x8") {
+			return RunShape {1024, 8, true};
+		}
+		if (v == "256x16n") {
+			return RunShape {256, 16, false};
+		}
+		return RunShape {256, 16, true};
+	}();
+	return shape;
+}
+
 template <typename KeyT, int NP, typename SG>
-void LaunchRunReduce(const RunPassArgs &args, const CompDesc &cd, unsigned tiles, int block, hipStream_t stream) {
-	if (block == 1024) {
-		hipLaunchKernelGGL((RunReduceKernel<KeyT, NP, SG, 1024, 8>), dim3(tiles), dim3(1024), 0, stream, args, cd);
+void LaunchRunReduce(const RunPassArgs &args, const CompDesc &cd, unsigned tiles, RunShape shape, hipStream_t stream) {
+	if (shape.block == 512) {
+		hipLaunchKernelGGL((RunReduceKernel<KeyT, NP, SG, 512, 16, true>), dim3(tiles), dim3(512), 0, stream, args, cd);
+	} else if (shape.block == 1024) {
+		hipLaunchKernelGGL((RunReduceKernel<KeyT, NP, SG, 1024, 8, true>), dim3(tiles), dim3(1024), 0, stream, args, cd);
+	} else if (!shape.prefetch) {
+		hipLaunchKernelGGL((RunReduceKernel<KeyT, NP, SG, 256, 16, false>), dim3(tiles), dim3(256), 0, stream, args, cd);
 	} else {
-		hipLaunchKernelGGL((RunReduceKernel<KeyT, NP, SG, 512, 16>), dim3(tiles), dim3(512), 0, stream, args, cd);
+		hipLaunchKernelGGL((RunReduceKernel<KeyT, NP, SG, 256, 16, true>), dim3(tiles), dim3(256), 0, stream, args, cd);
 	}
 	VGPU_HIP(hipGetLastError());
 }
@@ -709,19 +1086,24 @@ int RunSig(const CompDesc &cd, int npayload) {
 }
 
 template <typename KeyT, int NP, typename SG>
-void RunAll(const RunPassArgs &args, const CompDesc &cd, unsigned tiles, int block, uint64_t rec_bound,
+void RunAll(const RunPassArgs &args, const CompDesc &cd, unsigned tiles, RunShape shape, uint64_t rec_bound,
             const AggOutDesc &od, const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch, hipStream_t stream,
             double pass_bytes) {
+	for (int p = 0; p < NP; p++) {
+		if (args.width[p] != PayloadWidth<SG>(p)) {
+			throw HipError("run reduce: payload width does not match the aggregate input type");
+		}
+	}
 	{
 		ProfileScope prof("radix_pass_reduce", stream, pass_bytes);
-		LaunchRunReduce<KeyT, NP, SG>(args, cd, tiles, block, stream);
+		LaunchRunReduce<KeyT, NP, SG>(args, cd, tiles, shape, stream);
 	}
 	LaunchRecordReduce<KeyT, SG>(args.rec_keys, args.rec_state, args.layout, args.nrec, rec_bound, cd, od, ko,
 	                             groups_device, scratch, stream);
 }
 
 template <typename KeyT>
-void RunDispatch(int sig, const RunPassArgs &args, const CompDesc &cd, unsigned tiles, int block,
+void RunDispatch(int sig, const RunPassArgs &args, const CompDesc &cd, unsigned tiles, RunShape block,
                  uint64_t rec_bound, const AggOutDesc &od, const KeyOutDesc &ko, u64 *groups_device, Scratch &scratch,
                  hipStream_t stream, double pass_bytes) {
 	switch (sig) {
@@ -752,18 +1134,11 @@ void RunDispatch(int sig, const RunPassArgs &args, const CompDesc &cd, unsigned 
 	}
 }
 
-int RunBlock() {
-	static const int block = [] {
-		const char *e = std::getenv("VGPU_RUN_BLOCK");
-		return (e != nullptr && std::atoi(e) == 512) ? 512 : 1024;
-	}();
-	return block;
-}
-
 } // namespace
 
 uint64_t RunReduceRecordBound(uint64_t n, int shift, int bits, uint64_t max_packed) {
-	const uint64_t tile = 8192; // both kernel shapes use 8192-row tiles
+	const RunShape shape = RunShapeChoice();
+	const uint64_t tile = (uint64_t)shape.block * shape.items;
 	const uint64_t tiles = (n + tile - 1) / tile;
 	const uint64_t top = shift >= 64 ? 0 : (max_packed >> shift);
 	const uint64_t digits = std::min<uint64_t>(top + 1, 1ull << bits);
@@ -801,9 +1176,9 @@ void RunReduceLastPass(const SortedRun &run, uint64_t n, const ReduceSpec &spec,
 		FillU64(groups_device, 0ull, 1, stream);
 		return;
 	}
-	const int block = RunBlock();
-	constexpr uint64_t kTileRows = 8192;
-	const uint64_t tiles = (n + kTileRows - 1) / kTileRows;
+	const RunShape block = RunShapeChoice();
+	const uint64_t tile_rows = (uint64_t)block.block * block.items;
+	const uint64_t tiles = (n + tile_rows - 1) / tile_rows;
 	const uint64_t rec_bound = RunReduceRecordBound(n, run.last_shift, run.last_bits, run.max_packed);
 	const size_t kw = run.keys64 ? 8 : 4;
 	const int sc = cd.ncomp;
@@ -816,7 +1191,14 @@ void RunReduceLastPass(const SortedRun &run, uint64_t n, const ReduceSpec &spec,
 	}
 	a.n = n;
 	a.shift = run.last_shift;
-	a.bits = run.last_bits;
+	// Rank on the digit bits the keys can use: the top digit of the largest packed key bounds them (a 20-bit
+	// key range leaves 4 of the last pass's 8 bits, half the ballots of the multisplit).
+	const uint64_t top = run.last_shift >= 64 ? 0 : (run.max_packed >> run.last_shift);
+	int used = 1;
+	while (used < run.last_bits && (top >> used) != 0) {
+		used++;
+	}
+	a.bits = used;
 	a.ghist = run.last_hist;
 	a.region_cap = run.last_shift >= 31 ? 0xffffffffu
 	                                    : (uint32_t)std::min<uint64_t>((1ull << run.last_shift) + tiles - 1, 0xffffffffull);
