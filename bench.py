@@ -725,8 +725,10 @@ def main():
     # The sort autotuner picks the radix-pass variant per shape on the first (warm-up) sort of that shape.
     capi.sort_autotune(not args.no_autotune)
     warmup = max(args.warmup, 0 if args.no_autotune else 1)
+    # Output buffers are the caller's (the C ABI writes into them); one set serves every step.
+    outs = capi.group_by_outputs([key], aggs, cap)
     for i in range(warmup):
-        _, _, groups = capi.group_by([key], [v1, v2], aggs, rows, capacity=cap)
+        _, _, groups = capi.group_by([key], [v1, v2], aggs, rows, capacity=cap, out=outs)
     torch.cuda.synchronize()
     print(f"[bench] warmup done, groups={groups}", file=sys.stderr, flush=True)
 
@@ -735,7 +737,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        gkeys, gaggs, groups = capi.group_by([key], [v1, v2], aggs, rows, capacity=cap)
+        gkeys, gaggs, groups = capi.group_by([key], [v1, v2], aggs, rows, capacity=cap, out=outs)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     capi.profile_enable(False)

@@ -263,10 +263,17 @@ def group_by_bound(keys, rows, stream=None) -> int:
     return out.value
 
 
-def group_by(keys, values, aggs, rows, capacity=None, stream=None):
+def group_by_outputs(keys, aggs, capacity):
+    """Output buffers for group_by(..., out=...): one per key column and aggregate, `capacity` rows each."""
+    return ([empty_column(column(k).type, capacity) for k in keys], [empty_column(a[2], capacity) for a in aggs])
+
+
+def group_by(keys, values, aggs, rows, capacity=None, stream=None, out=None):
     """GROUP BY over device columns.
 
-    keys/values: lists of device tensors; aggs: list of (kind, input_index, output_type).
+    keys/values: lists of device tensors; aggs: list of (kind, input_index, output_type). out: optional
+    (key_buffers, agg_buffers) from group_by_outputs with at least `capacity` rows, reused by the caller
+    across calls (the C ABI writes into caller-owned outputs).
     Returns (key_tensors, agg_tensors, groups) with tensors trimmed to `groups` rows.
     """
     lib = load()
@@ -274,8 +281,19 @@ def group_by(keys, values, aggs, rows, capacity=None, stream=None):
     val_cols = [column(v) for v in values]
     if capacity is None:
         capacity = group_by_bound(key_cols, rows, stream)
-    key_out = [empty_column(kc.type, capacity) for kc in key_cols]
-    agg_out = [empty_column(a[2], capacity) for a in aggs]
+    if out is not None:
+        key_out, agg_out = out
+        if len(key_out) != len(key_cols) or len(agg_out) != len(aggs):
+            raise ValueError("group_by: out holds one buffer per key column and per aggregate")
+        for t, a in zip(agg_out, aggs):
+            if t.numel() < (2 * capacity if a[2] == HUGEINT else capacity):
+                raise ValueError("group_by: an output buffer is smaller than the capacity")
+        for t in key_out:
+            if t.numel() < capacity:
+                raise ValueError("group_by: an output buffer is smaller than the capacity")
+    else:
+        key_out = [empty_column(kc.type, capacity) for kc in key_cols]
+        agg_out = [empty_column(a[2], capacity) for a in aggs]
     kc = (Column * max(len(key_cols), 1))(*key_cols)
     vc = (Column * max(len(val_cols), 1))(*val_cols)
     ac = (Agg * max(len(aggs), 1))(*[Agg(*a) for a in aggs])

@@ -277,6 +277,12 @@ void PackValidity(const uint8_t *dense, uint8_t *packed, uint64_t n, hipStream_t
 void FillU8(uint8_t *dst, uint8_t value, uint64_t n, hipStream_t stream);
 void FillU32(uint32_t *dst, uint32_t value, uint64_t n, hipStream_t stream);
 void FillU64(uint64_t *dst, uint64_t value, uint64_t n, hipStream_t stream);
+//! Small device-to-host read-back on `stream` that the host waits for: through a page-locked staging buffer
+//! of the calling thread (up to 4 KiB; larger reads copy directly), with the host polling the stream instead
+//! of sleeping in hipStreamSynchronize (a planning read-back sits between two kernels of one operator, so
+//! its wake-up latency is on the critical path). Synchronizes `stream`.
+void ReadBack(void *host, const void *dev, size_t bytes, hipStream_t stream);
+
 //! Device-to-device copy on `stream` (a block-chunked copy kernel; falls back to hipMemcpyAsync when a
 //! pointer is not 16-byte aligned).
 void CopyDevice(void *dst, const void *src, uint64_t bytes, hipStream_t stream);

@@ -4,6 +4,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -59,6 +60,23 @@ std::mutex g_prof_mutex;
 bool g_prof_on = false;
 std::vector<ProfRecord> g_prof_pending;
 std::map<std::string, ProfileEntry> g_prof_totals;
+//! Events of read-out records, reused by later scopes (creating two events per launch cost microseconds of
+//! host time inside the timed region).
+std::vector<hipEvent_t> g_prof_free;
+
+hipEvent_t TakeProfEvent() {
+	{
+		std::lock_guard<std::mutex> lock(g_prof_mutex);
+		if (!g_prof_free.empty()) {
+			hipEvent_t e = g_prof_free.back();
+			g_prof_free.pop_back();
+			return e;
+		}
+	}
+	hipEvent_t e = nullptr;
+	VGPU_HIP(hipEventCreate(&e));
+	return e;
+}
 } // namespace
 
 void ProfileEnable(bool on) {
@@ -81,8 +99,8 @@ ProfileScope::ProfileScope(const char *name, hipStream_t stream, double bytes)
 	if (hipStreamIsCapturing(stream_, &capture) != hipSuccess || capture != hipStreamCaptureStatusNone) {
 		return;
 	}
-	VGPU_HIP(hipEventCreate(&start_));
-	VGPU_HIP(hipEventCreate(&stop_));
+	start_ = TakeProfEvent();
+	stop_ = TakeProfEvent();
 	VGPU_HIP(hipEventRecord(start_, stream_));
 }
 
@@ -111,8 +129,8 @@ std::vector<ProfileEntry> ProfileSnapshot() {
 		entry.total_ms += ms;
 		entry.launches += 1;
 		entry.bytes += rec.bytes;
-		(void)hipEventDestroy(rec.start);
-		(void)hipEventDestroy(rec.stop);
+		g_prof_free.push_back(rec.start);
+		g_prof_free.push_back(rec.stop);
 	}
 	g_prof_pending.clear();
 	std::vector<ProfileEntry> out;
@@ -126,6 +144,37 @@ void ProfileReset() {
 	ProfileSnapshot();
 	std::lock_guard<std::mutex> lock(g_prof_mutex);
 	g_prof_totals.clear();
+}
+
+// ------------------------------------------------------------------ read-back
+
+void ReadBack(void *host, const void *dev, size_t bytes, hipStream_t stream) {
+	constexpr size_t kStage = 4096;
+	thread_local void *stage = nullptr;
+	if (bytes == 0) {
+		return;
+	}
+	if (stage == nullptr && bytes <= kStage) {
+		if (hipHostMalloc(&stage, kStage, hipHostMallocDefault) != hipSuccess) {
+			(void)hipGetLastError();
+			stage = nullptr;
+		}
+	}
+	void *dst = (bytes <= kStage && stage != nullptr) ? stage : host;
+	VGPU_HIP(hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, stream));
+	while (true) {
+		const hipError_t st = hipStreamQuery(stream);
+		if (st == hipSuccess) {
+			break;
+		}
+		if (st != hipErrorNotReady) {
+			VGPU_HIP(st);
+		}
+		(void)hipGetLastError();
+	}
+	if (dst != host) {
+		std::memcpy(host, dst, bytes);
+	}
 }
 
 // ------------------------------------------------------------------ fills

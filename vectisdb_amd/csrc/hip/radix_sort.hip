@@ -376,8 +376,7 @@ void KeyRanges(const void *const *cols, const int *vtypes, int ncols, uint64_t n
 		VGPU_HIP(hipGetLastError());
 	}
 	u64 host[2 * kMaxKeyCols];
-	VGPU_HIP(hipMemcpyAsync(host, dev, sizeof(host), hipMemcpyDeviceToHost, stream));
-	VGPU_HIP(hipStreamSynchronize(stream));
+	ReadBack(host, dev, sizeof(host), stream);
 	for (int c = 0; c < ncols; c++) {
 		min_out[c] = host[c];
 		max_out[c] = host[kMaxKeyCols + c];
@@ -417,8 +416,7 @@ void KeyRangeHist8(const void *col, int vtype, uint64_t n, uint64_t *min_out, ui
 		VGPU_HIP(hipGetLastError());
 	}
 	u64 host[2];
-	VGPU_HIP(hipMemcpyAsync(host, mm, sizeof(host), hipMemcpyDeviceToHost, stream));
-	VGPU_HIP(hipStreamSynchronize(stream));
+	ReadBack(host, mm, sizeof(host), stream);
 	*min_out = host[0];
 	*max_out = host[1];
 }

@@ -36,21 +36,20 @@ void *DevicePool::TakeCachedLocked(size_t size, hipStream_t stream) {
 		Block &b = it->second;
 		bool usable = b.idle || b.stream == stream;
 		if (!usable && b.ready != nullptr) {
-			usable = hipEventQuery(b.ready) == hipSuccess;
-			if (!usable) {
-				(void)hipGetLastError(); // "not ready" is an answer, not an error to report later
+			if (!b.ready->done) {
+				b.ready->done = hipEventQuery(b.ready->event) == hipSuccess;
+				if (!b.ready->done) {
+					(void)hipGetLastError(); // "not ready" is an answer, not an error to report later
+				}
 			}
+			usable = b.ready->done;
 		}
 		if (!usable) {
 			continue;
 		}
 		void *ptr = b.ptr;
 		size_t bsize = b.size;
-		if (b.ready != nullptr) {
-			// An event is never reused: its last record could be on a stream destroyed since.
-			(void)hipEventDestroy(b.ready);
-		}
-		free_.erase(it);
+		free_.erase(it); // drops this block's share of its event (an event is never recorded twice)
 		cached_ -= bsize;
 		live_[ptr] = Live {bsize, stream};
 		in_use_ += bsize;
@@ -87,11 +86,21 @@ void *DevicePool::Alloc(size_t bytes, hipStream_t stream) {
 	return ptr;
 }
 
-void DevicePool::Free(void *ptr, hipStream_t stream) {
-	if (ptr == nullptr) {
-		return;
+std::shared_ptr<DevicePool::ReadyEvent> DevicePool::RecordReady(hipStream_t stream) {
+	auto ready = std::make_shared<ReadyEvent>();
+	if (hipEventCreateWithFlags(&ready->event, hipEventDisableTiming) != hipSuccess) {
+		(void)hipGetLastError();
+		ready->event = nullptr;
+		return nullptr;
 	}
-	std::lock_guard<std::mutex> lock(mutex_);
+	if (hipEventRecord(ready->event, stream) != hipSuccess) {
+		(void)hipGetLastError();
+		return nullptr; // the holder destroys the event
+	}
+	return ready;
+}
+
+void DevicePool::CacheLocked(void *ptr, hipStream_t stream, const std::shared_ptr<ReadyEvent> &ready) {
 	auto it = live_.find(ptr);
 	if (it == live_.end()) {
 		return; // not ours; never free foreign memory
@@ -100,19 +109,39 @@ void DevicePool::Free(void *ptr, hipStream_t stream) {
 	b.ptr = ptr;
 	b.size = it->second.size;
 	b.stream = stream;
-	if (hipEventCreateWithFlags(&b.ready, hipEventDisableTiming) != hipSuccess) {
-		(void)hipGetLastError();
-		b.ready = nullptr;
-	}
-	if (b.ready != nullptr && hipEventRecord(b.ready, stream) != hipSuccess) {
-		(void)hipGetLastError();
-		(void)hipEventDestroy(b.ready);
-		b.ready = nullptr;
-	}
+	b.ready = ready;
 	live_.erase(it);
 	in_use_ -= b.size;
 	cached_ += b.size;
 	free_.emplace(b.size, b);
+}
+
+void DevicePool::Free(void *ptr, hipStream_t stream) {
+	if (ptr == nullptr) {
+		return;
+	}
+	{
+		std::lock_guard<std::mutex> lock(mutex_);
+		if (live_.count(ptr) == 0) {
+			return;
+		}
+	}
+	const auto ready = RecordReady(stream);
+	std::lock_guard<std::mutex> lock(mutex_);
+	CacheLocked(ptr, stream, ready);
+}
+
+void DevicePool::FreeBatch(const std::vector<void *> &ptrs, hipStream_t stream) {
+	if (ptrs.empty()) {
+		return;
+	}
+	const auto ready = RecordReady(stream);
+	std::lock_guard<std::mutex> lock(mutex_);
+	for (void *p : ptrs) {
+		if (p != nullptr) {
+			CacheLocked(p, stream, ready);
+		}
+	}
 }
 
 bool DevicePool::Owns(void *ptr) const {
@@ -155,10 +184,7 @@ void DevicePool::ForgetStream(hipStream_t stream) {
 		if (b.stream != stream || b.idle) {
 			continue;
 		}
-		if (b.ready != nullptr) {
-			(void)hipEventDestroy(b.ready);
-			b.ready = nullptr;
-		}
+		b.ready.reset();
 		b.stream = nullptr;
 		b.idle = true;
 	}
@@ -169,11 +195,8 @@ void DevicePool::Trim() {
 	std::lock_guard<std::mutex> lock(mutex_);
 	for (auto &kv : free_) {
 		(void)hipFree(kv.second.ptr);
-		if (kv.second.ready != nullptr) {
-			(void)hipEventDestroy(kv.second.ready);
-		}
 	}
-	free_.clear();
+	free_.clear(); // and with the blocks, their events
 	cached_ = 0;
 }
 
@@ -198,9 +221,7 @@ void DevicePool::ResetPeak() {
 }
 
 PoolArena::~PoolArena() {
-	for (void *p : blocks_) {
-		DevicePool::Instance().Free(p, stream_);
-	}
+	DevicePool::Instance().FreeBatch(blocks_, stream_);
 }
 
 void *PoolArena::Alloc(size_t bytes) {

@@ -14,6 +14,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -28,6 +29,10 @@ public:
 	void *Alloc(size_t bytes, hipStream_t stream);
 	//! Returns a block to the cache; `stream` is the last stream that used it.
 	void Free(void *ptr, hipStream_t stream);
+	//! Returns several blocks last used on `stream` with one reuse event recorded for all of them (a plan's
+	//! scratch arena frees dozens of blocks at once; one event per block cost tens of microseconds of host
+	//! time per plan).
+	void FreeBatch(const std::vector<void *> &ptrs, hipStream_t stream);
 	//! Frees memory of unknown origin: a pool block goes back to the pool, its reuse event recorded on the
 	//! stream it was allocated on (the null stream once that stream has been forgotten); anything else is
 	//! released with hipFree. A caller that used the block on any other stream must have synchronized that
@@ -53,13 +58,26 @@ private:
 	DevicePool(const DevicePool &) = delete;
 	DevicePool &operator=(const DevicePool &) = delete;
 
+	//! One event recorded when a batch of blocks was freed, shared by the blocks of the batch; destroyed with
+	//! the last of them. Once seen complete it is not queried again.
+	struct ReadyEvent {
+		hipEvent_t event = nullptr;
+		bool done = false;
+		~ReadyEvent() {
+			if (event != nullptr) {
+				(void)hipEventDestroy(event);
+			}
+		}
+	};
 	struct Block {
 		void *ptr = nullptr;
 		size_t size = 0;
 		hipStream_t stream = nullptr;
-		hipEvent_t ready = nullptr; // recorded at free; block reusable elsewhere once complete
-		bool idle = false;          // its stream is gone and was drained: reusable anywhere
+		std::shared_ptr<ReadyEvent> ready; // recorded at free; block reusable elsewhere once complete
+		bool idle = false;                 // its stream is gone and was drained: reusable anywhere
 	};
+	static std::shared_ptr<ReadyEvent> RecordReady(hipStream_t stream);
+	void CacheLocked(void *ptr, hipStream_t stream, const std::shared_ptr<ReadyEvent> &ready);
 	static size_t RoundSize(size_t bytes);
 	void *TakeCachedLocked(size_t size, hipStream_t stream);
 

@@ -391,9 +391,7 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
 		flag = 0;
 		if (groups_dev == words) {
 			uint64_t host[2] = {};
-			VGPU_HIP(hipMemcpyAsync(host, words, narrow != 0 ? sizeof(host) : sizeof(uint64_t),
-			                        hipMemcpyDeviceToHost, stream));
-			VGPU_HIP(hipStreamSynchronize(stream));
+			ReadBack(host, words, narrow != 0 ? sizeof(host) : sizeof(uint64_t), stream);
 			groups = host[0];
 			flag = narrow != 0 ? host[1] : 0;
 			return;
