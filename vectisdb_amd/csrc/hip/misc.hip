@@ -516,6 +516,116 @@ __global__ void __launch_bounds__(BLOCK) KeepRangeKernel(const T *in, uint64_t n
 	}
 }
 
+//! The same compaction for a 16-byte aligned input: each lane loads 16-byte vectors of kVec consecutive rows (a
+//! wave reads one contiguous 1 KiB line per load), a kept row's rank in its wave comes from one ballot per
+//! vector component (mbcnt of the ballots below the lane plus the lane's own earlier components), kept rows are
+//! staged in LDS in tile order and written out as one contiguous run per tile (every wave store a full
+//! 256-byte line). Same tile, look-back words and ticket as KeepRangeKernel. Measured on 2^28 INT32 rows, half
+//! kept: 0.43 -> 0.41 ms; a persistent form that loads its next tile during the current one's look-back and
+//! write-out ran 0.60 ms (the ticket and the next tile's loads then sit on each block's serial chain).
+template <typename T, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) KeepRangeVecKernel(const T *in, uint64_t n, long long lo, long long hi,
+                                                           T *out, u64 *state, uint32_t *ticket, u64 *count_out,
+                                                           uint64_t ntiles) {
+	constexpr int kItems = KeepRangeItems<T>();
+	constexpr int kVec = (int)(16 / sizeof(T));
+	constexpr int kVecs = kItems / kVec;
+	constexpr int kWaves = BLOCK / kWave;
+	constexpr int kTile = BLOCK * kItems;
+	constexpr int kWaveRows = kItems * kWave;
+	static_assert(kVecs * kVec == kItems, "whole vectors per thread");
+	__shared__ __align__(16) T s_stage[kTile];
+	__shared__ uint32_t s_wtot[kWaves];
+	__shared__ uint32_t s_tile;
+	__shared__ u64 s_prefix;
+	const int tid = threadIdx.x;
+	const int lane = tid & (kWave - 1);
+	const int wave = tid / kWave;
+	if (tid == 0) {
+		s_tile = FetchAddAgent(ticket, 1u);
+	}
+	__syncthreads();
+	const uint32_t tile = s_tile;
+	const uint64_t wbase = (uint64_t)tile * kTile + (uint64_t)wave * kWaveRows;
+	T v[kVecs][kVec];
+#pragma unroll
+	for (int j = 0; j < kVecs; j++) {
+		const uint64_t row0 = wbase + ((uint64_t)j * kWave + lane) * kVec;
+		if (row0 + kVec <= n) {
+			const uint4 w = *reinterpret_cast<const uint4 *>(in + row0);
+			__builtin_memcpy(v[j], &w, sizeof(w));
+		} else {
+#pragma unroll
+			for (int c = 0; c < kVec; c++) {
+				v[j][c] = row0 + c < n ? in[row0 + c] : (T)0;
+			}
+		}
+	}
+	const u64 lt = LaneMaskLt();
+	uint32_t lpos[kVecs];
+	uint32_t keep[kVecs];
+	uint32_t wrun = 0; // this wave's kept rows so far (the same in every lane)
+#pragma unroll
+	for (int j = 0; j < kVecs; j++) {
+		const uint64_t row0 = wbase + ((uint64_t)j * kWave + lane) * kVec;
+		uint32_t k = 0, below = 0, tot = 0;
+#pragma unroll
+		for (int c = 0; c < kVec; c++) {
+			const long long x = (long long)v[j][c];
+			const bool ok = row0 + c < n && x >= lo && x <= hi;
+			const u64 bal = __ballot(ok);
+			k |= ok ? (1u << c) : 0u;
+			below += (uint32_t)Popc64(bal & lt);
+			tot += (uint32_t)Popc64(bal);
+		}
+		lpos[j] = wrun + below;
+		keep[j] = k;
+		wrun += tot;
+	}
+	if (lane == 0) {
+		s_wtot[wave] = wrun;
+	}
+	__syncthreads();
+	uint32_t wexcl = 0, ttot = 0;
+#pragma unroll
+	for (int w = 0; w < kWaves; w++) {
+		const uint32_t c = s_wtot[w];
+		wexcl += w < wave ? c : 0u;
+		ttot += c;
+	}
+	if (tid == 0) {
+		StoreAgent(state + tile, LbPack(tile == 0 ? kLbInclusive : kLbAggregate, ttot));
+	}
+	if (wave == 0) {
+		const u64 prefix = tile == 0 ? 0ull : LookbackExclusiveWave(state, tile);
+		if (lane == 0) {
+			if (tile > 0) {
+				StoreAgent(state + tile, LbPack(kLbInclusive, prefix + ttot));
+			}
+			if (tile == ntiles - 1) {
+				*count_out = prefix + ttot;
+			}
+			s_prefix = prefix;
+		}
+	}
+	// Kept rows into LDS in tile order (each lane's earlier components first, then the next lanes').
+#pragma unroll
+	for (int j = 0; j < kVecs; j++) {
+		uint32_t lp = lpos[j] + wexcl;
+#pragma unroll
+		for (int c = 0; c < kVec; c++) {
+			if ((keep[j] >> c) & 1u) {
+				s_stage[lp++] = v[j][c];
+			}
+		}
+	}
+	__syncthreads();
+	const u64 gbase = s_prefix;
+	for (uint32_t i = tid; i < ttot; i += BLOCK) {
+		out[gbase + i] = s_stage[i];
+	}
+}
+
 //! Launches the range compaction over n rows (count of kept rows to *count, no host sync): 512-thread tiles
 //! (16,384 rows of 4 bytes) from 2^22 rows up, where the per-tile ticket (a same-address atomic, serialized
 //! across the device) would otherwise bound the kernel; 256-thread tiles below, so small inputs still fill
@@ -533,7 +643,14 @@ void LaunchKeepRange(const T *in, uint64_t n, long long lo, long long hi, T *out
                      hipStream_t stream) {
 	const uint64_t ntiles = KeepRangeTiles<T>(n);
 	uint32_t *ticket = reinterpret_cast<uint32_t *>(state + ntiles);
-	if (KeepRangeWide(n)) {
+	const bool vec = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+	if (vec && KeepRangeWide(n)) {
+		hipLaunchKernelGGL((KeepRangeVecKernel<T, 512>), dim3((unsigned)ntiles), dim3(512), 0, stream, in, n, lo,
+		                   hi, out, state, ticket, count, ntiles);
+	} else if (vec) {
+		hipLaunchKernelGGL((KeepRangeVecKernel<T, kFgBlock>), dim3((unsigned)ntiles), dim3(kFgBlock), 0, stream, in,
+		                   n, lo, hi, out, state, ticket, count, ntiles);
+	} else if (KeepRangeWide(n)) {
 		hipLaunchKernelGGL((KeepRangeKernel<T, 512>), dim3((unsigned)ntiles), dim3(512), 0, stream, in, n, lo, hi,
 		                   out, state, ticket, count, ntiles);
 	} else {
