@@ -124,10 +124,17 @@ __device__ __forceinline__ vg_u64 vg_lookback(const vg_u64* state, long long til
   }
   return excl;
 }
-__device__ __forceinline__ unsigned int vg_wave_incl(unsigned int v) {
-  const int lane = __lane_id();
-  for (int o = 1; o < 64; o <<= 1) { unsigned int t = __shfl_up(v, o); if (lane >= o) v += t; }
-  return v;
+// In-order rank of a lane's kept rows in one round (4 consecutive rows per lane, bit j of k = row j kept):
+// below = kept rows of the lower lanes, total = the wave's kept rows. One ballot per row position and mbcnt
+// instead of a 6-step shuffle scan of the per-lane counts.
+__device__ __forceinline__ void vg_rank4(unsigned int k, unsigned int& below, unsigned int& total) {
+  below = 0; total = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const unsigned long long b = __ballot((k >> j) & 1u);
+    below += __builtin_amdgcn_mbcnt_hi((unsigned int)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)b, 0u));
+    total += (unsigned int)__popcll(b);
+  }
 }
 )VGPU";
 
@@ -437,11 +444,11 @@ std::string GenerateFusedSource(const FusedSpec &s, int rounds_override) {
 		o << "  unsigned int __rtot[" << rounds << "];\n";
 		o << "  unsigned int __wrun = 0;\n";
 		o << "#pragma unroll\n  for (int __r = 0; __r < " << rounds << "; __r++) {\n";
-		o << "    unsigned int __cnt = __popc(__keep[__r]);\n";
-		o << "    unsigned int __inc = vg_wave_incl(__cnt);\n";
-		o << "    __lpos0[__r] = __wrun + __inc - __cnt;\n";
-		o << "    __rtot[__r] = __shfl(__inc, 63);\n";
-		o << "    __wrun += __rtot[__r];\n";
+		o << "    unsigned int __below, __tot;\n";
+		o << "    vg_rank4(__keep[__r], __below, __tot);\n";
+		o << "    __lpos0[__r] = __wrun + __below;\n";
+		o << "    __rtot[__r] = __tot;\n";
+		o << "    __wrun += __tot;\n";
 		o << "  }\n";
 		o << "  if (lane == 0) s_wtot[wave] = __wrun;\n";
 		o << "  __syncthreads();\n";

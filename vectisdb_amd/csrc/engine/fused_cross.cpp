@@ -283,10 +283,10 @@ std::string GenerateCrossAlignedSource(const FusedSpec &s, int tile_pairs, int p
 	o << "  unsigned int __lpos0[" << rounds << "];\n";
 	o << "  unsigned int __wrun = 0;\n";
 	o << "#pragma unroll\n  for (int __r = 0; __r < " << rounds << "; __r++) {\n";
-	o << "    const unsigned int __cnt = __popc(__keep[__r]);\n";
-	o << "    const unsigned int __inc = vg_wave_incl(__cnt);\n";
-	o << "    __lpos0[__r] = __wrun + __inc - __cnt;\n";
-	o << "    __wrun += __shfl(__inc, 63);\n";
+	o << "    unsigned int __below, __tot;\n";
+	o << "    vg_rank4(__keep[__r], __below, __tot);\n";
+	o << "    __lpos0[__r] = __wrun + __below;\n";
+	o << "    __wrun += __tot;\n";
 	o << "  }\n";
 	o << "  if (lane == 0) s_wtot[wave] = __wrun;\n";
 	o << "  __syncthreads();\n";
