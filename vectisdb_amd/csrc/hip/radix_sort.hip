@@ -84,9 +84,9 @@ constexpr int kHistUnroll = 8;
 //! Min/max of the ordered key and the histograms of all 8 byte digits in one read. A digit that is the
 //! same for every key a wave reads in one step (the constant high bytes of a narrow key range) is counted
 //! with one LDS atomic instead of one per key.
-template <typename T>
-__global__ void __launch_bounds__(256) KeyRangeHist8Kernel(const T *col, int width, int is_signed, uint64_t n,
-                                                           u64 *mins, u64 *maxs, uint32_t *hist8) {
+template <typename T, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
+KeyRangeHist8Kernel(const T *col, int width, int is_signed, uint64_t n, u64 *mins, u64 *maxs, uint32_t *hist8) {
 	__shared__ uint32_t s_hist[8 * 256];
 	for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) {
 		s_hist[i] = 0;
@@ -392,24 +392,27 @@ void KeyRangeHist8(const void *col, int vtype, uint64_t n, uint64_t *min_out, ui
 	FillU64(mm + 1, 0ull, 1, stream);
 	if (n > 0) {
 		const int width = TypeWidth(vtype);
-		uint64_t blocks = std::min<uint64_t>((n + 1023) / 1024, 2048);
+		// Every block merges its LDS histograms into the global ones with device-scope atomics at its end, so
+		// the block count is kept low: 1,024-thread blocks, at most 512 (two per CU, 32 waves per CU).
+		constexpr int kHB = 1024;
+		uint64_t blocks = std::min<uint64_t>((n + 4 * kHB - 1) / (4 * kHB), 512);
 		ProfileScope prof("key_range_hist", stream, (double)n * width);
 		const int sg = TypeIsSigned(vtype) ? 1 : 0;
 		switch (width) {
 		case 8:
-			hipLaunchKernelGGL(KeyRangeHist8Kernel<u64>, dim3((unsigned)blocks), dim3(256), 0, stream,
+			hipLaunchKernelGGL((KeyRangeHist8Kernel<u64, kHB>), dim3((unsigned)blocks), dim3(kHB), 0, stream,
 			                   (const u64 *)col, width, sg, n, mm, mm + 1, hist8_dev);
 			break;
 		case 4:
-			hipLaunchKernelGGL(KeyRangeHist8Kernel<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, stream,
+			hipLaunchKernelGGL((KeyRangeHist8Kernel<uint32_t, kHB>), dim3((unsigned)blocks), dim3(kHB), 0, stream,
 			                   (const uint32_t *)col, width, sg, n, mm, mm + 1, hist8_dev);
 			break;
 		case 2:
-			hipLaunchKernelGGL(KeyRangeHist8Kernel<uint16_t>, dim3((unsigned)blocks), dim3(256), 0, stream,
+			hipLaunchKernelGGL((KeyRangeHist8Kernel<uint16_t, kHB>), dim3((unsigned)blocks), dim3(kHB), 0, stream,
 			                   (const uint16_t *)col, width, sg, n, mm, mm + 1, hist8_dev);
 			break;
 		default:
-			hipLaunchKernelGGL(KeyRangeHist8Kernel<uint8_t>, dim3((unsigned)blocks), dim3(256), 0, stream,
+			hipLaunchKernelGGL((KeyRangeHist8Kernel<uint8_t, kHB>), dim3((unsigned)blocks), dim3(kHB), 0, stream,
 			                   (const uint8_t *)col, width, sg, n, mm, mm + 1, hist8_dev);
 			break;
 		}
