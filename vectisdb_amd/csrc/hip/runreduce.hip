@@ -340,7 +340,7 @@ __device__ __forceinline__ void LoadWaveCol(const T *src, uint64_t wave_base, in
 //! with a segmented scan (wave shuffles, then one wave scanning the wave totals), and write those components
 //! of each run's record. Column p + 1 is loaded while phase p runs.
 template <typename KeyT, int NP, typename SG, int BLOCK, int ITEMS, bool PF1>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 256 ? 4 : 1)))
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK <= 512 ? 4 : 1)))
 RunReduceKernel(RunPassArgs a, CompDesc d) {
 	static_assert(BLOCK >= kRunRadix, "one thread per digit");
 	static_assert(ITEMS <= 32, "head flags are a 32-bit mask");
