@@ -30,6 +30,29 @@ __device__ __forceinline__ int Popc64(u64 v) {
 	return __popcll(v);
 }
 
+//! Set bits of `mask` in lanes strictly below this lane (popcount of mask & LaneMaskLt(), two mbcnt steps).
+__device__ __forceinline__ uint32_t CountBelow(u64 mask) {
+	return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+//! Lanes of `active` whose R-bit digit equals this lane's (wave multisplit: one ballot per digit bit). Digits
+//! are masked to their width, so above a narrower digit every lane's bit is clear, the ballot is empty and
+//! the step keeps the mask: all R steps run without a per-bit width test. Each step is one compare into a
+//! lane mask and two 3-input bit operations (x = 0 where the bit is set, all ones where it is clear; the
+//! lanes agreeing with this one are bal ^ x).
+template <int R>
+__device__ __forceinline__ u64 DigitPeers(uint32_t d, u64 active) {
+	uint32_t lo = (uint32_t)active, hi = (uint32_t)(active >> 32);
+#pragma unroll
+	for (int b = 0; b < R; b++) {
+		const uint32_t x = ((d >> b) & 1u) - 1u;
+		const u64 bal = __ballot(x == 0u);
+		lo &= (uint32_t)bal ^ x;
+		hi &= (uint32_t)(bal >> 32) ^ x;
+	}
+	return ((u64)hi << 32) | lo;
+}
+
 // ---------------------------------------------------------------- agent-scope atomics (cross-XCD visible)
 
 __device__ __forceinline__ uint32_t LoadAgent(const uint32_t *p) {

@@ -126,16 +126,8 @@ __global__ void __launch_bounds__(BLOCK) OnesweepKernel(KeyPack pack, const KeyT
 		uint64_t row = wave_base + (uint64_t)k * kWave + lane;
 		bool valid = row < n;
 		uint32_t d = (uint32_t)(key[k] >> digit_shift) & digit_mask;
-		u64 peers = __ballot(valid);
-#pragma unroll
-		for (int b = 0; b < R; b++) {
-			if (b < digit_bits) {
-				bool bit = (d >> b) & 1u;
-				u64 bal = __ballot(bit);
-				peers &= bit ? bal : ~bal;
-			}
-		}
-		uint32_t r = (uint32_t)Popc64(peers & LaneMaskLt());
+		const u64 peers = DigitPeers<R>(d, __ballot(valid));
+		uint32_t r = CountBelow(peers);
 		uint32_t base = valid ? my_counts[d] : 0u;
 		__builtin_amdgcn_wave_barrier();
 		if (valid && r == 0) {

@@ -416,16 +416,8 @@ RunReduceKernel(RunPassArgs a, CompDesc d) {
 			const uint64_t row = wave_base + (uint64_t)k * kWave + lane;
 			const bool valid = row < n;
 			const uint32_t dg = (uint32_t)(key[k] >> dshift) & dmask;
-			u64 peers = __ballot(valid);
-#pragma unroll
-			for (int b = 0; b < 8; b++) {
-				if (b < a.bits) {
-					const bool bit = (dg >> b) & 1u;
-					const u64 bal = __ballot(bit);
-					peers &= bit ? bal : ~bal;
-				}
-			}
-			const uint32_t r = (uint32_t)Popc64(peers & LaneMaskLt());
+			const u64 peers = DigitPeers<8>(dg, __ballot(valid));
+			const uint32_t r = CountBelow(peers);
 			const uint32_t base = valid ? my_counts[dg] : 0u;
 			__builtin_amdgcn_wave_barrier();
 			if (valid && r == 0) {
