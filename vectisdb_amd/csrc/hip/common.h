@@ -35,21 +35,34 @@ __device__ __forceinline__ uint32_t CountBelow(u64 mask) {
 	return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+//! One step of DigitPeers for digit bit B: y = bit B of d sign-extended (all ones where set), the wave's ballot
+//! of it, and lo/hi &= ~(bal ^ y), the lanes agreeing with this one on bit B (one v_bitop3 per half, truth
+//! table 0x90 over mask, bal, y). The field extract is written as the instruction itself: left to the
+//! compiler, y != 0 becomes a shift and a sign test and y a second shift, one more operation per bit.
+template <int B>
+__device__ __forceinline__ void DigitPeerStep(uint32_t d, uint32_t &lo, uint32_t &hi) {
+	uint32_t y;
+	asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(y) : "v"(d), "n"(B));
+	const u64 bal = __ballot(y != 0u);
+	lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)bal, y, 0x90);
+	hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(bal >> 32), y, 0x90);
+}
+
+template <int B, int R>
+__device__ __forceinline__ void DigitPeerSteps(uint32_t d, uint32_t &lo, uint32_t &hi) {
+	if constexpr (B < R) {
+		DigitPeerStep<B>(d, lo, hi);
+		DigitPeerSteps<B + 1, R>(d, lo, hi);
+	}
+}
+
 //! Lanes of `active` whose R-bit digit equals this lane's (wave multisplit: one ballot per digit bit). Digits
 //! are masked to their width, so above a narrower digit every lane's bit is clear, the ballot is empty and
-//! the step keeps the mask: all R steps run without a per-bit width test. Each step is one compare into a
-//! lane mask and two 3-input bit operations (x = 0 where the bit is set, all ones where it is clear; the
-//! lanes agreeing with this one are bal ^ x).
+//! the step keeps the mask: all R steps run without a per-bit width test (four VALU operations per bit).
 template <int R>
 __device__ __forceinline__ u64 DigitPeers(uint32_t d, u64 active) {
 	uint32_t lo = (uint32_t)active, hi = (uint32_t)(active >> 32);
-#pragma unroll
-	for (int b = 0; b < R; b++) {
-		const uint32_t x = ((d >> b) & 1u) - 1u;
-		const u64 bal = __ballot(x == 0u);
-		lo &= (uint32_t)bal ^ x;
-		hi &= (uint32_t)(bal >> 32) ^ x;
-	}
+	DigitPeerSteps<0, R>(d, lo, hi);
 	return ((u64)hi << 32) | lo;
 }
 
