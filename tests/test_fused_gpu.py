@@ -180,6 +180,61 @@ def test_cross_filter_large_exact(dev, nl, nr):
     np.testing.assert_array_equal(outs[1].cpu().numpy(), ey)
 
 
+@pytest.mark.parametrize("nl,chunks", [(1, 1), (5, 1), (257, 2), (64, 3), (33, 8)])
+def test_cross_whole_tiles_against_oracle(dev, nl, chunks):
+    """Right sides of 1 to 8 whole 8,192-row chunks (the count, scan and write kernels over static tiles),
+    with one and with odd left-row counts: every kept pair in left-major order against the oracle."""
+    import oracle
+    rng = np.random.default_rng(nl * 31 + chunks)
+    nr = 8192 * chunks
+    x = rng.integers(0, 1 << 30, nl).astype(np.int32)
+    y = rng.integers(0, 1 << 30, nr).astype(np.int32)
+    pred = [(out(f"(({col(0, capi.INT32)} + {col(1, capi.INT32)}) < 1073741824)", capi.BOOLEAN), [0, 1],
+             capi.BOOLEAN)]
+    projs = [(copy_body(capi.INT32), [0], capi.INT32), (copy_body(capi.INT32), [1], capi.INT32)]
+    outs, k = capi.cross_filter([_dev(x, dev)], [_dev(y, dev)], pred, projs)
+    ex, ey = oracle.cross_filter_pairs(x, y, 1 << 30)
+    assert k == len(ex)
+    np.testing.assert_array_equal(outs[0].cpu().numpy(), ex)
+    np.testing.assert_array_equal(outs[1].cpu().numpy(), ey)
+
+
+def test_cross_whole_tiles_wide_outputs_and_extreme_selectivity(dev):
+    """8-byte outputs (the LDS stage at 8 bytes per pair), a projection over both sides, and predicates that
+    keep every pair and none, over a right side of two whole tiles."""
+    rng = np.random.default_rng(12)
+    nl, nr = 37, 8192 * 2
+    x = rng.integers(-(1 << 40), 1 << 40, nl).astype(np.int64)
+    y = rng.random(nr)
+    projs = [(copy_body(capi.INT64), [0], capi.INT64),
+             (out(f"((double){col(0, capi.INT64)} * {col(1, capi.FLOAT64)})", capi.FLOAT64), [0, 1], capi.FLOAT64),
+             (copy_body(capi.FLOAT64), [1], capi.FLOAT64)]
+    for thr, expect in [("0.5", None), ("2.0", nl * nr), ("-1.0", 0)]:
+        pred = [(out(f"({col(0, capi.FLOAT64)} < {thr})", capi.BOOLEAN), [1], capi.BOOLEAN)]
+        o, k = capi.cross_filter([_dev(x, dev)], [_dev(y, dev)], pred, projs)
+        keep = y < float(thr)
+        assert k == nl * int(keep.sum())
+        if expect is not None:
+            assert k == expect
+        ex = np.repeat(x, int(keep.sum()))
+        ey = np.tile(y[keep], nl)
+        np.testing.assert_array_equal(o[0].cpu().numpy(), ex)
+        np.testing.assert_array_equal(o[2].cpu().numpy(), ey)
+        np.testing.assert_array_equal(o[1].cpu().numpy(), ex.astype(np.float64) * ey)
+
+
+def test_cross_whole_tiles_capacity_guard(dev):
+    """Every pair kept over a whole tile, output capacity below the kept count: the kernels drop writes past
+    the capacity and the call reports the overflow."""
+    x = np.zeros(2, np.int32)
+    y = np.zeros(8192, np.int32)
+    pred = [(out(f"(({col(0, capi.INT32)} + {col(1, capi.INT32)}) < 5)", capi.BOOLEAN), [0, 1], capi.BOOLEAN)]
+    with pytest.raises(capi.VgpuError, match="more than the output capacity"):
+        capi.cross_filter([_dev(x, dev)], [_dev(y, dev)], pred,
+                          [(copy_body(capi.INT32), [0], capi.INT32), (copy_body(capi.INT32), [1], capi.INT32)],
+                          capacity=10000)
+
+
 def test_cross_product_unfiltered_left_major(dev):
     x = np.array([10, 20, 30], np.int64)
     y = np.array([1.5, 2.5], np.float64)

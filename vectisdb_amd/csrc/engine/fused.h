@@ -110,7 +110,7 @@ private:
 	RtcKernel *rtc_ = nullptr;
 	//! Filtered cross product with the right side a multiple of cross_tile_ pairs (its inputs come from
 	//! cache, so evaluating the predicate twice costs less than a tile ticket and a look-back per tile):
-	//! rtc_count_ counts each static tile's kept pairs, a one-block scan turns the counts into offsets,
+	//! rtc_count_ counts each static tile's kept pairs, a scan turns the counts into offsets,
 	//! rtc_write_ writes. Other cross products take the single-pass rtc_.
 	RtcKernel *rtc_count_ = nullptr;
 	RtcKernel *rtc_write_ = nullptr;

@@ -127,6 +127,7 @@ std::string ValidOf(const FusedSpec &s, const FusedExpr &e) {
 } // namespace
 
 std::string CrossAlignedKernelName(const FusedSpec &s, int tile_pairs, int phase, int rows_per_block) {
+	(void)s;
 	return "vectis_cross_filter_t" + std::to_string(tile_pairs) + "x" + std::to_string(rows_per_block) +
 	       (phase == 1 ? "_count" : "_write");
 }
@@ -148,26 +149,163 @@ std::string GenerateCrossAlignedSource(const FusedSpec &s, int tile_pairs, int p
 			}
 		}
 	}
+	for (auto &c : s.cols) {
+		if (c.segment_id) {
+			throw std::runtime_error("cross kernel: no segment-id columns");
+		}
+	}
 	const int rounds = tile_pairs / (kCrossBlock * kCrossVec);
 	const int wave_span = tile_pairs / (kCrossBlock / 64);
 	std::ostringstream o;
+
+	// The right side's tile-long chunk starting at pair __rj0 into register tiles (4 consecutive rows per lane
+	// and round), with 16-byte loads where the chunk is aligned.
+	auto emit_right_load = [&](const std::string &in) {
+		o << "#pragma unroll\n" << in << "for (int __r = 0; __r < " << rounds << "; __r++) {\n";
+		o << in << "  const vg_u64 __rj = __rj0 + (unsigned)wave * " << wave_span << "u + (unsigned)__r * "
+		  << (64 * kCrossVec) << "u + (unsigned)lane * " << kCrossVec << "u;\n";
+		for (size_t c = 0; c < s.cols.size(); c++) {
+			if (s.cols[c].side == 0) {
+				continue;
+			}
+			const char *t = FusedCType(s.cols[c].vtype);
+			const std::string cn = "__c" + std::to_string(c), vn = "__v" + std::to_string(c) + "[__r]";
+			o << in << "  if ((((vg_u64)(" << cn << " + __rj)) & (" << kCrossVec << " * sizeof(" << t
+			  << ") - 1)) == 0) {\n";
+			o << in << "    __builtin_memcpy(&" << vn << "[0], __builtin_assume_aligned((const void*)(" << cn
+			  << " + __rj), " << kCrossVec << " * sizeof(" << t << ")), " << kCrossVec << " * sizeof(" << t << "));\n";
+			o << in << "  } else {\n";
+			o << "#pragma unroll\n" << in << "    for (int __j = 0; __j < " << kCrossVec << "; __j++) " << vn
+			  << "[__j] = " << cn << "[__rj + __j];\n";
+			o << in << "  }\n";
+			if (s.cols[c].has_valid) {
+				o << "#pragma unroll\n" << in << "  for (int __j = 0; __j < " << kCrossVec << "; __j++) __m" << c
+				  << "[__r][__j] = a.valid[" << c << "] == 0 || a.valid[" << c << "][__rj + __j] != 0;\n";
+			}
+		}
+		o << in << "}\n";
+	};
+	// The left columns' values of left row __li.
+	auto emit_left_load = [&](const std::string &in) {
+		for (size_t c = 0; c < s.cols.size(); c++) {
+			if (s.cols[c].side != 0) {
+				continue;
+			}
+			const char *t = FusedCType(s.cols[c].vtype);
+			o << in << "const " << t << " __l" << c << " = __c" << c << "[__li];\n";
+			if (s.cols[c].has_valid) {
+				o << in << "const bool __lm" << c << " = a.valid[" << c << "] == 0 || a.valid[" << c << "][__li] != 0;\n";
+			}
+		}
+	};
+	// __ok = every predicate holds (and its inputs are valid) for pair (__li, right row of register [__r][__j]).
+	auto emit_predicates = [&](const std::string &in) {
+		o << in << "bool __ok = true;\n";
+		for (size_t p = 0; p < s.predicates.size(); p++) {
+			const auto &e = s.predicates[p];
+			o << in << "bool __p" << p << " = false;\n";
+			EmitBody(o, s, e, "row", "(void*)&__p" + std::to_string(p), "0", in);
+			const std::string v = ValidOf(s, e);
+			o << in << "__ok = __ok && __p" << p << (v.empty() ? "" : " && (" + v + ")") << ";\n";
+		}
+	};
+	// __wc = this wave's kept pairs of left row __li against the chunk in registers (ballots, no barrier).
+	auto emit_count = [&](const std::string &in) {
+		o << in << "unsigned int __wc = 0;\n";
+		o << "#pragma unroll\n" << in << "for (int __r = 0; __r < " << rounds << "; __r++) {\n";
+		o << "#pragma unroll\n" << in << "  for (int __j = 0; __j < " << kCrossVec << "; __j++) {\n";
+		emit_predicates(in + "    ");
+		o << in << "    __wc += (unsigned int)__popcll(__ballot(__ok));\n";
+		o << in << "  }\n";
+		o << in << "}\n";
+	};
+	// The kept pairs of left row __li against the chunk in registers, written in order from output row __gbase:
+	// left-only outputs as one fill, the others compacted through the LDS stage. Ends with a barrier.
+	auto emit_write_row = [&](const std::string &in) {
+		o << in << "unsigned int __keep[" << rounds << "];\n";
+		o << "#pragma unroll\n" << in << "for (int __r = 0; __r < " << rounds << "; __r++) {\n";
+		o << in << "  unsigned int __kb = 0;\n";
+		o << "#pragma unroll\n" << in << "  for (int __j = 0; __j < " << kCrossVec << "; __j++) {\n";
+		emit_predicates(in + "    ");
+		o << in << "    if (__ok) __kb |= 1u << __j;\n";
+		o << in << "  }\n";
+		o << in << "  __keep[__r] = __kb;\n";
+		o << in << "}\n";
+		// in-order positions: rounds within the wave's span, waves in order
+		o << in << "unsigned int __lpos0[" << rounds << "];\n";
+		o << in << "unsigned int __wrun = 0;\n";
+		o << "#pragma unroll\n" << in << "for (int __r = 0; __r < " << rounds << "; __r++) {\n";
+		o << in << "  unsigned int __below, __tot;\n";
+		o << in << "  vg_rank4(__keep[__r], __below, __tot);\n";
+		o << in << "  __lpos0[__r] = __wrun + __below;\n";
+		o << in << "  __wrun += __tot;\n";
+		o << in << "}\n";
+		o << in << "if (lane == 0) s_wtot[wave] = __wrun;\n";
+		o << in << "__syncthreads();\n";
+		o << in << "unsigned int __wexcl = 0, __ttot = 0;\n";
+		o << in << "for (int w = 0; w < 4; w++) { if (w < wave) __wexcl += s_wtot[w]; __ttot += s_wtot[w]; }\n";
+		o << "#pragma unroll\n" << in << "for (int __r = 0; __r < " << rounds << "; __r++) __lpos0[__r] += __wexcl;\n";
+		for (size_t e = 0; e < s.projections.size(); e++) {
+			const auto &ex = s.projections[e];
+			const char *ot = FusedCType(ex.out_type);
+			const std::string out = "((" + std::string(ot) + "*)a.outs[" + std::to_string(e) + "])";
+			const std::string v = ValidOf(s, ex);
+			if (LeftOnly(s, ex)) {
+				// one value for every kept pair of the tile
+				o << in << "{\n";
+				o << in << "  " << ot << " __u;\n";
+				EmitBody(o, s, ex, "left", "(void*)&__u", "0", in + "  ");
+				o << in << "  vg_fill_run<" << ot << ">(" << out << ", __gbase, __ttot, a.cap, __u);\n";
+				if (!v.empty()) {
+					o << in << "  if (a.out_valid[" << e << "] != 0) vg_fill_run<unsigned char>(a.out_valid[" << e
+					  << "], __gbase, __ttot, a.cap, (unsigned char)((" << v << ") ? 1 : 0));\n";
+				}
+				o << in << "}\n";
+				continue;
+			}
+			auto emit_staged = [&](const std::string &type, const std::string &target,
+			                       const std::function<void()> &value) {
+				o << in << "{\n";
+				o << in << "  " << type << "* __st = (" << type << "*)s_stage;\n";
+				o << in << "  const unsigned __sh = vg_stage_shift<" << type << ">(" << target << ", __gbase);\n";
+				o << "#pragma unroll\n" << in << "  for (int __r = 0; __r < " << rounds << "; __r++) {\n";
+				o << in << "    unsigned int __lp = __lpos0[__r] + __sh;\n";
+				o << "#pragma unroll\n" << in << "    for (int __j = 0; __j < " << kCrossVec << "; __j++) {\n";
+				o << in << "      if ((__keep[__r] >> __j) & 1u) {\n";
+				o << in << "        const unsigned __o = __lp++;\n";
+				value();
+				o << in << "      }\n";
+				o << in << "    }\n";
+				o << in << "  }\n";
+				o << in << "  __syncthreads();\n";
+				o << in << "  vg_copy_run<" << type << ">(" << target << ", __gbase, __ttot, a.cap, __st, __sh);\n";
+				o << in << "  __syncthreads();\n";
+				o << in << "}\n";
+			};
+			emit_staged(ot, out, [&]() { EmitBody(o, s, ex, "row", "(void*)__st", "__o", in + "        "); });
+			if (!v.empty()) {
+				o << in << "if (a.out_valid[" << e << "] != 0)\n";
+				emit_staged("unsigned char", "a.out_valid[" + std::to_string(e) + "]",
+				            [&]() { o << in << "        __st[__o] = (" << v << ") ? 1 : 0;\n"; });
+			}
+		}
+		o << in << "__syncthreads(); // s_wtot and the stage are reused by the next left row\n";
+	};
+
 	o << FusedPreamble() << kStore16 << kCrossHelpers;
 	o << "extern \"C\" __global__ void __launch_bounds__(" << kCrossBlock << ") "
 	  << CrossAlignedKernelName(s, tile_pairs, phase, rows_per_block) << "(VArgs a) {\n";
 	o << "  const int tid = threadIdx.x; const int lane = tid & 63; const int wave = tid >> 6;\n";
-	// block = (right chunk, group of rows_per_block left rows); tile t = left row x chunks + right chunk
 	o << "  const vg_u64 __chunks = a.m / " << tile_pairs << "ull;\n";
+	// block = (right chunk, group of rows_per_block left rows); tile t = left row x chunks + right chunk
 	o << "  const vg_u64 __chunk = blockIdx.x % __chunks;\n";
 	o << "  const vg_u64 __li0 = (blockIdx.x / __chunks) * " << rows_per_block << "ull;\n";
 	o << "  const vg_u64 __nl = a.n / a.m;\n";
 	o << "  const vg_u64 __rj0 = __chunk * " << tile_pairs << "ull;\n";
 	o << "  __shared__ unsigned int s_wtot[4];\n";
-	// right columns: register tiles of 4 consecutive rows per round, loaded once for all the block's left rows
+	// right columns: register tiles of 4 consecutive rows per round
 	for (size_t c = 0; c < s.cols.size(); c++) {
 		const char *t = FusedCType(s.cols[c].vtype);
-		if (s.cols[c].segment_id) {
-			throw std::runtime_error("cross kernel: no segment-id columns");
-		}
 		o << "  const " << t << "* __c" << c << " = (const " << t << "*)a.cols[" << c << "];\n";
 		if (s.cols[c].side != 0) {
 			o << "  " << t << " __v" << c << "[" << rounds << "][" << kCrossVec << "];\n";
@@ -176,28 +314,8 @@ std::string GenerateCrossAlignedSource(const FusedSpec &s, int tile_pairs, int p
 			}
 		}
 	}
-	o << "#pragma unroll\n  for (int __r = 0; __r < " << rounds << "; __r++) {\n";
-	o << "    const vg_u64 __rj = __rj0 + (unsigned)wave * " << wave_span << "u + (unsigned)__r * " << (64 * kCrossVec)
-	  << "u + (unsigned)lane * " << kCrossVec << "u;\n";
-	for (size_t c = 0; c < s.cols.size(); c++) {
-		if (s.cols[c].side == 0) {
-			continue;
-		}
-		const char *t = FusedCType(s.cols[c].vtype);
-		const std::string cn = "__c" + std::to_string(c), vn = "__v" + std::to_string(c) + "[__r]";
-		o << "    if ((((vg_u64)(" << cn << " + __rj)) & (" << kCrossVec << " * sizeof(" << t << ") - 1)) == 0) {\n";
-		o << "      __builtin_memcpy(&" << vn << "[0], __builtin_assume_aligned((const void*)(" << cn << " + __rj), "
-		  << kCrossVec << " * sizeof(" << t << ")), " << kCrossVec << " * sizeof(" << t << "));\n";
-		o << "    } else {\n";
-		o << "#pragma unroll\n      for (int __j = 0; __j < " << kCrossVec << "; __j++) " << vn << "[__j] = " << cn
-		  << "[__rj + __j];\n";
-		o << "    }\n";
-		if (s.cols[c].has_valid) {
-			o << "#pragma unroll\n    for (int __j = 0; __j < " << kCrossVec << "; __j++) __m" << c
-			  << "[__r][__j] = a.valid[" << c << "] == 0 || a.valid[" << c << "][__rj + __j] != 0;\n";
-		}
-	}
-	o << "  }\n";
+	// loaded once for all the block's left rows
+	emit_right_load("  ");
 	if (phase == 2) {
 		int stage_width = 1;
 		for (auto &e : s.projections) {
@@ -215,30 +333,8 @@ std::string GenerateCrossAlignedSource(const FusedSpec &s, int tile_pairs, int p
 		o << "#pragma unroll 1\n  for (int __k = 0; __k < " << rows_per_block << "; __k++) {\n";
 		o << "    const vg_u64 __li = __li0 + __k;\n";
 		o << "    if (__li >= __nl) break;\n";
-		for (size_t c = 0; c < s.cols.size(); c++) {
-			if (s.cols[c].side != 0) {
-				continue;
-			}
-			const char *t = FusedCType(s.cols[c].vtype);
-			o << "    const " << t << " __l" << c << " = __c" << c << "[__li];\n";
-			if (s.cols[c].has_valid) {
-				o << "    const bool __lm" << c << " = a.valid[" << c << "] == 0 || a.valid[" << c << "][__li] != 0;\n";
-			}
-		}
-		o << "    unsigned int __wc = 0;\n";
-		o << "#pragma unroll\n    for (int __r = 0; __r < " << rounds << "; __r++) {\n";
-		o << "#pragma unroll\n      for (int __j = 0; __j < " << kCrossVec << "; __j++) {\n";
-		o << "        bool __ok = true;\n";
-		for (size_t p = 0; p < s.predicates.size(); p++) {
-			const auto &e = s.predicates[p];
-			o << "        bool __p" << p << " = false;\n";
-			EmitBody(o, s, e, "row", "(void*)&__p" + std::to_string(p), "0", "        ");
-			const std::string v = ValidOf(s, e);
-			o << "        __ok = __ok && __p" << p << (v.empty() ? "" : " && (" + v + ")") << ";\n";
-		}
-		o << "        __wc += (unsigned int)__popcll(__ballot(__ok));\n";
-		o << "      }\n";
-		o << "    }\n";
+		emit_left_load("    ");
+		emit_count("    ");
 		o << "    if (lane == 0) s_cnt[wave][__k] = __wc;\n";
 		o << "  }\n";
 		o << "  __syncthreads();\n";
@@ -253,91 +349,9 @@ std::string GenerateCrossAlignedSource(const FusedSpec &s, int tile_pairs, int p
 	o << "  const vg_u64 __li = __li0 + __k;\n";
 	o << "  if (__li >= __nl) break;\n";
 	o << "  const vg_u64 tile = __li * __chunks + __chunk;\n";
-	for (size_t c = 0; c < s.cols.size(); c++) {
-		if (s.cols[c].side != 0) {
-			continue;
-		}
-		const char *t = FusedCType(s.cols[c].vtype);
-		o << "  const " << t << " __l" << c << " = __c" << c << "[__li];\n";
-		if (s.cols[c].has_valid) {
-			o << "  const bool __lm" << c << " = a.valid[" << c << "] == 0 || a.valid[" << c << "][__li] != 0;\n";
-		}
-	}
-	o << "  unsigned int __keep[" << rounds << "];\n";
-	o << "#pragma unroll\n  for (int __r = 0; __r < " << rounds << "; __r++) {\n";
-	o << "    unsigned int __kb = 0;\n";
-	o << "#pragma unroll\n    for (int __j = 0; __j < " << kCrossVec << "; __j++) {\n";
-	o << "      bool __ok = true;\n";
-	for (size_t p = 0; p < s.predicates.size(); p++) {
-		const auto &e = s.predicates[p];
-		o << "      bool __p" << p << " = false;\n";
-		EmitBody(o, s, e, "row", "(void*)&__p" + std::to_string(p), "0", "      ");
-		const std::string v = ValidOf(s, e);
-		o << "      __ok = __ok && __p" << p << (v.empty() ? "" : " && (" + v + ")") << ";\n";
-	}
-	o << "      if (__ok) __kb |= 1u << __j;\n";
-	o << "    }\n";
-	o << "    __keep[__r] = __kb;\n";
-	o << "  }\n";
-	// in-order positions: rounds within the wave's span, waves in order
-	o << "  unsigned int __lpos0[" << rounds << "];\n";
-	o << "  unsigned int __wrun = 0;\n";
-	o << "#pragma unroll\n  for (int __r = 0; __r < " << rounds << "; __r++) {\n";
-	o << "    unsigned int __below, __tot;\n";
-	o << "    vg_rank4(__keep[__r], __below, __tot);\n";
-	o << "    __lpos0[__r] = __wrun + __below;\n";
-	o << "    __wrun += __tot;\n";
-	o << "  }\n";
-	o << "  if (lane == 0) s_wtot[wave] = __wrun;\n";
-	o << "  __syncthreads();\n";
-	o << "  unsigned int __wexcl = 0, __ttot = 0;\n";
-	o << "  for (int w = 0; w < 4; w++) { if (w < wave) __wexcl += s_wtot[w]; __ttot += s_wtot[w]; }\n";
-	o << "#pragma unroll\n  for (int __r = 0; __r < " << rounds << "; __r++) __lpos0[__r] += __wexcl;\n";
+	emit_left_load("  ");
 	o << "  const vg_u64 __gbase = a.state[tile];\n";
-	for (size_t e = 0; e < s.projections.size(); e++) {
-		const auto &ex = s.projections[e];
-		const char *ot = FusedCType(ex.out_type);
-		const std::string out = "((" + std::string(ot) + "*)a.outs[" + std::to_string(e) + "])";
-		const std::string v = ValidOf(s, ex);
-		if (LeftOnly(s, ex)) {
-			// one value for every kept pair of the tile
-			o << "  {\n";
-			o << "    " << ot << " __u;\n";
-			EmitBody(o, s, ex, "left", "(void*)&__u", "0", "    ");
-			o << "    vg_fill_run<" << ot << ">(" << out << ", __gbase, __ttot, a.cap, __u);\n";
-			if (!v.empty()) {
-				o << "    if (a.out_valid[" << e << "] != 0) vg_fill_run<unsigned char>(a.out_valid[" << e
-				  << "], __gbase, __ttot, a.cap, (unsigned char)((" << v << ") ? 1 : 0));\n";
-			}
-			o << "  }\n";
-			continue;
-		}
-		auto emit_staged = [&](const std::string &type, const std::string &target, const std::function<void()> &value) {
-			o << "  {\n";
-			o << "    " << type << "* __st = (" << type << "*)s_stage;\n";
-			o << "    const unsigned __sh = vg_stage_shift<" << type << ">(" << target << ", __gbase);\n";
-			o << "#pragma unroll\n    for (int __r = 0; __r < " << rounds << "; __r++) {\n";
-			o << "      unsigned int __lp = __lpos0[__r] + __sh;\n";
-			o << "#pragma unroll\n      for (int __j = 0; __j < " << kCrossVec << "; __j++) {\n";
-			o << "        if ((__keep[__r] >> __j) & 1u) {\n";
-			o << "          const unsigned __o = __lp++;\n";
-			value();
-			o << "        }\n";
-			o << "      }\n";
-			o << "    }\n";
-			o << "    __syncthreads();\n";
-			o << "    vg_copy_run<" << type << ">(" << target << ", __gbase, __ttot, a.cap, __st, __sh);\n";
-			o << "    __syncthreads();\n";
-			o << "  }\n";
-		};
-		emit_staged(ot, out, [&]() { EmitBody(o, s, ex, "row", "(void*)__st", "__o", "          "); });
-		if (!v.empty()) {
-			o << "  if (a.out_valid[" << e << "] != 0)\n";
-			emit_staged("unsigned char", "a.out_valid[" + std::to_string(e) + "]",
-			            [&]() { o << "          __st[__o] = (" << v << ") ? 1 : 0;\n"; });
-		}
-	}
-	o << "  __syncthreads(); // s_wtot and the stage are reused by the next left row\n";
+	emit_write_row("  ");
 	o << "  }\n";
 	o << "}\n";
 	return o.str();

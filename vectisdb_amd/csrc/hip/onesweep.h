@@ -77,6 +77,79 @@ __device__ __forceinline__ void LoadWaveItems(const void *src, int width, uint64
 	}
 }
 
+//! Predecessor words read per step of the per-digit look-back (compile time: -DVGPU_LOOKBACK_WINDOW=n).
+#ifndef VGPU_LOOKBACK_WINDOW
+#define VGPU_LOOKBACK_WINDOW 4
+#endif
+constexpr int kLookBackWindow = VGPU_LOOKBACK_WINDOW;
+
+//! Decoupled look-back of D digits of one tile over per-(tile, digit) words [flag:2 | value:30] (flag 1 =
+//! the tile's own count, 2 = inclusive of every earlier tile, 0 = not yet written): excl[j] = digit[j]'s count
+//! in every earlier tile (0 for an inactive digit, or for tile 0). Each step reads the next W predecessor
+//! words of every digit together, so a walk past predecessors that have published only their own counts
+//! costs one round trip per W tiles instead of one per tile; it stops at the first inclusive word and waits
+//! at the first unwritten one.
+template <int D, int W = kLookBackWindow>
+__device__ __forceinline__ void LookBackDigits(const uint32_t *state, uint64_t tile, uint32_t radix,
+                                               const uint32_t (&digit)[D], const bool (&active)[D],
+                                               uint32_t (&excl)[D]) {
+	int64_t t[D];
+	bool done[D];
+#pragma unroll
+	for (int j = 0; j < D; j++) {
+		excl[j] = 0;
+		t[j] = (int64_t)tile - 1;
+		done[j] = !active[j] || tile == 0;
+	}
+	while (true) {
+		bool all = true;
+#pragma unroll
+		for (int j = 0; j < D; j++) {
+			all = all && done[j];
+		}
+		if (all) {
+			break;
+		}
+		uint32_t w[D][W];
+#pragma unroll
+		for (int j = 0; j < D; j++) {
+#pragma unroll
+			for (int q = 0; q < W; q++) {
+				// tile 0 publishes an inclusive word, so no walk reads past it; beyond it stands a zero inclusive
+				w[j][q] = (!done[j] && t[j] - q >= 0) ? LoadAgent(state + (uint64_t)(t[j] - q) * radix + digit[j])
+				                                      : kStFlagInc;
+			}
+		}
+		bool stall = false;
+#pragma unroll
+		for (int j = 0; j < D; j++) {
+			if (done[j]) {
+				continue;
+			}
+			bool go = true;
+			int used = 0;
+#pragma unroll
+			for (int q = 0; q < W; q++) {
+				const uint32_t flag = w[j][q] & (3u << 30);
+				if (go && flag != 0) {
+					excl[j] += w[j][q] & kStValue;
+					used++;
+					if (flag == kStFlagInc) {
+						done[j] = true;
+						go = false;
+					}
+				} else {
+					go = false;
+				}
+			}
+			t[j] -= used;
+			stall = stall || (!done[j] && used < W); // t[j] is unwritten: read it again
+		}
+		if (stall) {
+			__builtin_amdgcn_s_sleep(1);
+		}
+	}
+}
 
 //! Launches one pass of variant (R, ITEMS, PF, BLOCK) over `tiles` tiles of BLOCK x ITEMS rows; dispatches on
 //! the compact key width, FIRST and the payload count (0, 1, 2, or up to 4). With n_dev the kernel reads the

@@ -172,48 +172,15 @@ __global__ void __launch_bounds__(BLOCK) OnesweepKernel(KeyPack pack, const KeyT
 
 	// Look-back: all of this thread's digits advance together, so their loads are issued in parallel.
 	uint32_t excl[DPT];
-	bool done[DPT];
-#pragma unroll
-	for (int j = 0; j < DPT; j++) {
-		excl[j] = 0;
-		done[j] = tile == 0 || tid * DPT + j >= RADIX;
-	}
-	int64_t t = (int64_t)tile - 1;
-	while (true) {
-		bool all = true;
+	{
+		uint32_t dig[DPT];
+		bool act[DPT];
 #pragma unroll
 		for (int j = 0; j < DPT; j++) {
-			all = all && done[j];
+			dig[j] = (uint32_t)(tid * DPT + j);
+			act[j] = tid * DPT + j < RADIX;
 		}
-		if (all) {
-			break;
-		}
-		uint32_t word[DPT];
-#pragma unroll
-		for (int j = 0; j < DPT; j++) {
-			word[j] = done[j] ? 0u : LoadAgent(tile_state + (uint64_t)t * RADIX + tid * DPT + j);
-		}
-		bool any_pending = false;
-#pragma unroll
-		for (int j = 0; j < DPT; j++) {
-			if (!done[j] && (word[j] & (3u << 30)) == 0) {
-				any_pending = true;
-			}
-		}
-		if (any_pending) {
-			__builtin_amdgcn_s_sleep(1);
-			continue; // re-read the same predecessor
-		}
-#pragma unroll
-		for (int j = 0; j < DPT; j++) {
-			if (!done[j]) {
-				excl[j] += word[j] & kStValue;
-				if ((word[j] & (3u << 30)) == kStFlagInc) {
-					done[j] = true;
-				}
-			}
-		}
-		t--;
+		LookBackDigits<DPT>(tile_state, tile, RADIX, dig, act, excl);
 	}
 	uint32_t bstart = texcl;
 	uint32_t goff = gexcl;

@@ -572,20 +572,11 @@ RunReduceKernel(RunPassArgs a, CompDesc d) {
 			if (tid < RADIX) {
 				uint32_t excl = 0;
 				if (tile > 0) {
-					int64_t t = (int64_t)tile - 1;
-					while (true) {
-						const uint32_t word = LoadAgent(a.tile_state + (uint64_t)t * RADIX + tid);
-						const uint32_t flag = word & (3u << 30);
-						if (flag == 0) {
-							__builtin_amdgcn_s_sleep(1);
-							continue;
-						}
-						excl += word & sortpass::kStValue;
-						if (flag == sortpass::kStFlagInc) {
-							break;
-						}
-						t--;
-					}
+					const uint32_t dig[1] = {(uint32_t)tid};
+					const bool act[1] = {true};
+					uint32_t ex[1];
+					sortpass::LookBackDigits<1>(a.tile_state, tile, RADIX, dig, act, ex);
+					excl = ex[0];
 					StoreAgent(a.tile_state + (uint64_t)tile * RADIX + tid, sortpass::kStFlagInc | (excl + rc));
 				}
 				s_rbase[tid] = rstart + excl - rfirst;
