@@ -245,6 +245,46 @@ def test_groupby_under_sort_autotuner(dev):
         capi.sort_autotune(False)
 
 
+def test_autotuned_shape_reused_with_other_call_settings(dev):
+    """The autotuner caches a pass variant per sort shape, not the settings of the call that tuned it: a later
+    GROUP BY of the same shape with an aggregate mix the fused last pass does not cover must take the plain
+    last pass, and one whose INT64 column holds a value outside INT32 must flag its own narrowing overflow and
+    sort again with the column whole. Each result against numpy."""
+    import torch
+    rng = np.random.default_rng(29)
+    n = 2_000_000
+    # 17-bit keys: three 8-bit passes whose last digit has 2 values, so at most ~132K partial states (15x fewer
+    # than rows) and the fused last pass applies to the seven-aggregate mix
+    key = rng.integers(0, 1 << 17, n).astype(np.int64)
+    v1 = rng.integers(-10 ** 6, 10 ** 6 + 1, n).astype(np.int64)
+    v2 = rng.standard_normal(n)
+    order = np.argsort(key, kind="stable")
+    uk, first = np.unique(key[order], return_index=True)
+    capi.sort_autotune(True)
+    try:
+        capi.profile_reset()
+        capi.profile_enable(True)
+        _check(dev, key, v1, v2)  # tunes the shape; the fused last pass applies to this mix
+        capi.profile_enable(False)
+        assert any(e["name"] == "radix_pass_reduce" for e in capi.profile_read())
+        kd, ad, bd = (torch.from_numpy(x).to(dev) for x in (key, v1, v2))
+        capi.profile_reset()
+        capi.profile_enable(True)
+        keys, aggs, g = capi.group_by([kd], [ad, bd], [(capi.MIN, 1, capi.FLOAT64), (capi.MAX, 0, capi.INT64)], n)
+        capi.profile_enable(False)
+        assert not any(e["name"] == "radix_pass_reduce" for e in capi.profile_read())
+        assert g == len(uk)
+        np.testing.assert_array_equal(keys[0].cpu().numpy(), uk)
+        np.testing.assert_array_equal(aggs[0].cpu().numpy(), np.minimum.reduceat(v2[order], first))
+        np.testing.assert_array_equal(aggs[1].cpu().numpy(), np.maximum.reduceat(v1[order], first))
+        wide = v1.copy()
+        wide[n // 2] = 1 << 40
+        _check(dev, key, wide, v2)
+    finally:
+        capi.sort_autotune(False)
+        capi.profile_enable(False)
+
+
 @pytest.mark.parametrize("variant", [
     {"VGPU_SORT_BLOCK": "512"},
     {"VGPU_SORT_ITEMS": "32", "VGPU_SORT_PREFETCH": "0"},

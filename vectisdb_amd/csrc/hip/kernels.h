@@ -117,6 +117,17 @@ void PlanKeyPackAligned(const void *col, int vtype, uint64_t omin, uint64_t omax
 
 // ------------------------------------------------------------------ radix sort
 
+//! Up to four spans of 32-bit words that one launch clears (ZeroU32Spans): look-back words, tickets and flags
+//! of consecutive steps of an operator, zeroed together instead of by one fill launch each.
+struct ZeroSpans {
+	static constexpr int kMax = 4;
+	uint32_t *ptr[kMax] = {};
+	uint64_t words[kMax] = {};
+	int count = 0;
+	//! Adds a span (empty spans are skipped); more than kMax spans throw.
+	void Add(uint32_t *p, uint64_t n);
+};
+
 struct SortTuning {
 	int radix_bits = 8;      // 8 or 11 (digit width upper bound)
 	int passes_override = 0; // 0 = derive from total_bits
@@ -136,6 +147,9 @@ struct SortTuning {
 	//! Leave the last (most significant) digit pass to the caller when it is an 8-bit pass that is not the
 	//! first: SortedRun then holds the keys ordered by every bit below that digit (see RunReduceLastPass).
 	bool leave_last = false;
+	//! The caller's spans that the sort clears in the same launch as its look-back state, before its first
+	//! pass (the overflow word above, the fused last pass's state).
+	ZeroSpans zero_also;
 };
 
 //! Output of SortPacked: the compact keys (uint32 when total_bits <= 32, else uint64) in ascending order and
@@ -229,8 +243,12 @@ void SegmentedReduce(const SortedRun &run, uint64_t n, const ReduceSpec &spec, u
 //! (last_pending): each 8192-row tile is ranked by the digit in LDS, which leaves its rows in full-key order,
 //! and reduces its runs of equal keys to one partial state each; a small reduction then combines each key's
 //! partials into its group. Same outputs as SegmentedReduce over the fully sorted run.
+//! `state`, when given, is RunReduceStateWords(n) words already zero (cleared with the sort's own state);
+//! otherwise the pass allocates and clears it.
 void RunReduceLastPass(const SortedRun &run, uint64_t n, const ReduceSpec &spec, uint64_t *groups_device,
-                       Scratch &scratch, hipStream_t stream);
+                       Scratch &scratch, hipStream_t stream, uint32_t *state = nullptr);
+//! Look-back words and ticket of the fused last pass over n rows.
+uint64_t RunReduceStateWords(uint64_t n);
 //! The aggregate mixes the fused pass is compiled for (COUNT alone, or with one SUM, or the
 //! COUNT/SUM/MIN/MAX + DOUBLE SUM mix, with the integer column carried at 4 or 8 bytes).
 bool RunReduceSupported(const ReduceSpec &spec, int npayload, bool keys64);
@@ -277,6 +295,8 @@ void PackValidity(const uint8_t *dense, uint8_t *packed, uint64_t n, hipStream_t
 void FillU8(uint8_t *dst, uint8_t value, uint64_t n, hipStream_t stream);
 void FillU32(uint32_t *dst, uint32_t value, uint64_t n, hipStream_t stream);
 void FillU64(uint64_t *dst, uint64_t value, uint64_t n, hipStream_t stream);
+//! Clears every span of `spans` in one launch (nothing when it holds none).
+void ZeroU32Spans(const ZeroSpans &spans, hipStream_t stream);
 //! Small device-to-host read-back on `stream` that the host waits for: through a page-locked staging buffer
 //! of the calling thread (up to 4 KiB; larger reads copy directly), with the host polling the stream instead
 //! of sleeping in hipStreamSynchronize (a planning read-back sits between two kernels of one operator, so

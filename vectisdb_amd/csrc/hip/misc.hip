@@ -205,6 +205,44 @@ void FillU64(uint64_t *dst, uint64_t value, uint64_t n, hipStream_t stream) {
 	VGPU_HIP(hipGetLastError());
 }
 
+void ZeroSpans::Add(uint32_t *p, uint64_t n) {
+	if (p == nullptr || n == 0) {
+		return;
+	}
+	if (count == kMax) {
+		throw HipError("ZeroSpans: more than four spans");
+	}
+	ptr[count] = p;
+	words[count] = n;
+	count++;
+}
+
+namespace {
+__global__ void __launch_bounds__(256) ZeroSpansKernel(ZeroSpans spans) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	for (int k = 0; k < spans.count; k++) {
+		uint32_t *p = spans.ptr[k];
+		const uint64_t n = spans.words[k];
+		for (uint64_t i = first; i < n; i += stride) {
+			p[i] = 0;
+		}
+	}
+}
+} // namespace
+
+void ZeroU32Spans(const ZeroSpans &spans, hipStream_t stream) {
+	uint64_t most = 0;
+	for (int k = 0; k < spans.count; k++) {
+		most = std::max<uint64_t>(most, spans.words[k]);
+	}
+	if (most == 0) {
+		return;
+	}
+	hipLaunchKernelGGL(ZeroSpansKernel, dim3(GridFor(most)), dim3(256), 0, stream, spans);
+	VGPU_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ device copy
 
 namespace {

@@ -175,6 +175,17 @@ KeyRangeHist8Kernel(const T *col, int width, int is_signed, uint64_t n, u64 *min
 	}
 }
 
+//! Clears the byte-digit histograms and sets the min/max words (min all ones, max zero) in one launch.
+__global__ void __launch_bounds__(1024) KeyRangeInitKernel(uint32_t *hist8, u64 *mm) {
+	for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) {
+		hist8[i] = 0;
+	}
+	if (threadIdx.x == 0) {
+		mm[0] = ~0ull;
+		mm[1] = 0;
+	}
+}
+
 //! One read of the keys builds the digit histogram of every pass.
 template <int R>
 __global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, PassList passes, uint32_t *hist,
@@ -387,9 +398,8 @@ void KeyRangeHist8(const void *col, int vtype, uint64_t n, uint64_t *min_out, ui
                    uint32_t *hist8_dev, hipStream_t stream) {
 	// hist8_dev is followed by 2 words of min/max scratch (callers allocate 8 * 256 + 4 uint32).
 	u64 *mm = reinterpret_cast<u64 *>(hist8_dev + 8 * 256);
-	FillU32(hist8_dev, 0u, 8 * 256, stream);
-	FillU64(mm, ~0ull, 1, stream);
-	FillU64(mm + 1, 0ull, 1, stream);
+	hipLaunchKernelGGL(KeyRangeInitKernel, dim3(1), dim3(1024), 0, stream, hist8_dev, mm);
+	VGPU_HIP(hipGetLastError());
 	if (n > 0) {
 		const int width = TypeWidth(vtype);
 		// Every block merges its LDS histograms into the global ones with device-scope atomics at its end, so
@@ -550,7 +560,9 @@ void SortPackedWith(const KeyPack &pack, const void *const *payload_in, const in
 	                          : static_cast<uint32_t *>(scratch.Alloc((size_t)P * RADIX * sizeof(uint32_t)));
 	const uint64_t state_words = (uint64_t)P * tiles * RADIX + P;
 	auto *state = static_cast<uint32_t *>(scratch.Alloc(state_words * sizeof(uint32_t)));
-	FillU32(state, 0u, state_words, stream);
+	ZeroSpans zero = tuning.zero_also;
+	zero.Add(state, state_words);
+	ZeroU32Spans(zero, stream);
 	uint32_t *tickets = state + (uint64_t)P * tiles * RADIX;
 
 	// Several key columns with real payload columns: pack once, beside the histograms, and start the passes

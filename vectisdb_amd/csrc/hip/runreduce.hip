@@ -1140,8 +1140,14 @@ bool RunReduceSupported(const ReduceSpec &spec, int npayload, bool keys64) {
 	return RunSig(cd, npayload) != 0;
 }
 
+uint64_t RunReduceStateWords(uint64_t n) {
+	const RunShape block = RunShapeChoice();
+	const uint64_t tile_rows = (uint64_t)block.block * block.items;
+	return ((n + tile_rows - 1) / tile_rows) * kRunRadix + 1;
+}
+
 void RunReduceLastPass(const SortedRun &run, uint64_t n, const ReduceSpec &spec, uint64_t *groups_device,
-                       Scratch &scratch, hipStream_t stream) {
+                       Scratch &scratch, hipStream_t stream, uint32_t *state) {
 	if (!run.last_pending || run.last_bits > 8 || run.keys == nullptr) {
 		throw HipError("run reduce: the sort did not leave its last 8-bit digit pass");
 	}
@@ -1187,8 +1193,10 @@ void RunReduceLastPass(const SortedRun &run, uint64_t n, const ReduceSpec &spec,
 	a.region_cap = run.last_shift >= 31 ? 0xffffffffu
 	                                    : (uint32_t)std::min<uint64_t>((1ull << run.last_shift) + tiles - 1, 0xffffffffull);
 	a.ntiles = (uint32_t)tiles;
-	auto *state = static_cast<uint32_t *>(scratch.Alloc((tiles * kRunRadix + 1) * sizeof(uint32_t)));
-	FillU32(state, 0u, tiles * kRunRadix + 1, stream);
+	if (state == nullptr) {
+		state = static_cast<uint32_t *>(scratch.Alloc((tiles * kRunRadix + 1) * sizeof(uint32_t)));
+		FillU32(state, 0u, tiles * kRunRadix + 1, stream);
+	}
 	a.tile_state = state;
 	a.ticket = state + tiles * kRunRadix;
 	a.rec_keys = scratch.Alloc(std::max<uint64_t>(rec_bound, 1) * kw);

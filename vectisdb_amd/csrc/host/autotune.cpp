@@ -109,6 +109,19 @@ std::string SortAutotuneReport() {
 	return o.str();
 }
 
+namespace {
+//! The pass variant a tuning choice stands for (digit width, items per thread, threads per tile, payload
+//! prefetch), applied to a call's own tuning: what belongs to the call (its histograms, the narrowing mask and
+//! its overflow word, whether it leaves the last digit pass, the state it clears) is never taken from the call
+//! that tuned the shape.
+void ApplyVariant(SortTuning &dst, const SortTuning &variant) {
+	dst.radix_bits = variant.radix_bits;
+	dst.items = variant.items;
+	dst.block = variant.block;
+	dst.prefetch = variant.prefetch;
+}
+} // namespace
+
 void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *payload_width, int npayload,
                 uint64_t n, const SortTuning &tuning, SortedRun &out, Scratch &scratch, hipStream_t stream,
                 const uint64_t *n_dev) {
@@ -133,8 +146,8 @@ void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *p
 		std::lock_guard<std::mutex> lock(g_mutex);
 		auto it = g_cache.find(key);
 		if (it != g_cache.end()) {
-			SortTuning c = it->second.tuning;
-			c.hist8 = tuning.hist8;
+			SortTuning c = t;
+			ApplyVariant(c, it->second.tuning);
 			SortPackedWith(pack, payload_in, payload_width, npayload, n, c, out, scratch, stream);
 			return;
 		}
@@ -204,8 +217,8 @@ void SortPacked(const KeyPack &pack, const void *const *payload_in, const int *p
 		std::lock_guard<std::mutex> lock(g_mutex);
 		g_cache.emplace(key, best);
 	}
-	SortTuning c = best.tuning;
-	c.hist8 = tuning.hist8;
+	SortTuning c = t;
+	ApplyVariant(c, best.tuning);
 	SortPackedWith(pack, payload_in, payload_width, npayload, n, c, out, scratch, stream);
 }
 

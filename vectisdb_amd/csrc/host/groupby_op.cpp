@@ -367,17 +367,21 @@ uint64_t GroupByDevice(const std::vector<DevCol> &keys, const std::vector<DevCol
 			}
 			t.narrow = narrow_mask;
 			t.overflow = overflow;
-			FillU32(overflow, 0u, 1, stream);
+			t.zero_also.Add(overflow, 1); // cleared with the sort's look-back state
 		}
 		t.leave_last = rows_dev == nullptr && RunReduceWorthIt(pack, rs, widths, rows);
+		uint32_t *run_state = nullptr;
 		if (t.leave_last) {
 			t.radix_bits = 8; // the fused pass is an 8-bit digit pass
+			const uint64_t words = RunReduceStateWords(rows);
+			run_state = static_cast<uint32_t *>(scratch.Alloc(words * sizeof(uint32_t)));
+			t.zero_also.Add(run_state, words);
 		}
 		SortedRun run;
 		SortPacked(pack, payload_in.data(), widths.data(), (int)payload_in.size(), rows, t, run, scratch, stream,
 		           rows_dev);
 		if (run.last_pending) {
-			RunReduceLastPass(run, rows, rs, groups_dev, scratch, stream);
+			RunReduceLastPass(run, rows, rs, groups_dev, scratch, stream, run_state);
 		} else {
 			SegmentedReduce(run, rows, rs, groups_dev, scratch, stream, rows_dev);
 		}
