@@ -245,6 +245,29 @@ def test_groupby_under_sort_autotuner(dev):
         capi.sort_autotune(False)
 
 
+def test_prepared_group_by_matches_group_by(dev):
+    """The prepared call (argument arrays built once, as the benchmark's timed loop uses it) returns the same
+    groups as the plain call, on every one of several consecutive calls."""
+    import torch
+    rng = np.random.default_rng(31)
+    n = 300_000
+    key = rng.integers(-5_000, 5_000, n).astype(np.int64)
+    v1 = rng.integers(-10 ** 6, 10 ** 6 + 1, n).astype(np.int64)
+    v2 = rng.standard_normal(n)
+    kd, ad, bd = (torch.from_numpy(x).to(dev) for x in (key, v1, v2))
+    keys, aggs, g = capi.group_by([kd], [ad, bd], AGGS7, n)
+    cap = capi.group_by_bound([capi.column(kd)], n)
+    outs = capi.group_by_outputs([kd], AGGS7, cap)
+    step = capi.PreparedGroupBy([kd], [ad, bd], AGGS7, n, cap, outs)
+    for _ in range(3):
+        assert step() == g
+        pk, pa, pg = step.result()
+        assert pg == g
+        assert torch.equal(pk[0], keys[0])
+        for a, b in zip(pa, aggs):
+            assert torch.equal(a, b)
+
+
 def test_autotuned_shape_reused_with_other_call_settings(dev):
     """The autotuner caches a pass variant per sort shape, not the settings of the call that tuned it: a later
     GROUP BY of the same shape with an aggregate mix the fused last pass does not cover must take the plain

@@ -307,6 +307,48 @@ def group_by(keys, values, aggs, rows, capacity=None, stream=None, out=None):
     return [t[:g] for t in key_out], trimmed_aggs, g
 
 
+class PreparedGroupBy:
+    """vgpu_group_by over fixed columns, aggregates and caller-owned outputs, with its argument arrays built
+    once: each call is one C call and returns the group count (the outputs hold the groups). For loops that
+    run the same GROUP BY many times, where building the arrays per call would cost more host time than the
+    C entry point itself."""
+
+    def __init__(self, keys, values, aggs, rows, capacity, out, stream=None):
+        self._lib = load()
+        key_out, agg_out = out
+        self._keep = (list(keys), list(values), list(key_out), list(agg_out))  # the tensors the pointers name
+        key_cols = [column(k) for k in keys]
+        val_cols = [column(v) for v in values]
+        if len(key_out) != len(key_cols) or len(agg_out) != len(aggs):
+            raise ValueError("PreparedGroupBy: out holds one buffer per key column and per aggregate")
+        for t, a in zip(agg_out, aggs):
+            if t.numel() < (2 * capacity if a[2] == HUGEINT else capacity):
+                raise ValueError("PreparedGroupBy: an output buffer is smaller than the capacity")
+        for t in key_out:
+            if t.numel() < capacity:
+                raise ValueError("PreparedGroupBy: an output buffer is smaller than the capacity")
+        self._aggs = list(aggs)
+        self._args = ((Column * max(len(key_cols), 1))(*key_cols), len(key_cols),
+                      (Column * max(len(val_cols), 1))(*val_cols), len(val_cols),
+                      (Agg * max(len(aggs), 1))(*[Agg(*a) for a in aggs]), len(aggs), rows,
+                      (C.c_void_p * max(len(key_out), 1))(*[t.data_ptr() for t in key_out]),
+                      (C.c_void_p * max(len(agg_out), 1))(*[t.data_ptr() for t in agg_out]), capacity)
+        self._groups = C.c_uint64(0)
+        self._groups_ref = C.byref(self._groups)
+        self._stream = _stream_ptr(stream)
+
+    def __call__(self):
+        check(self._lib.vgpu_group_by(*self._args, self._groups_ref, self._stream))
+        return self._groups.value
+
+    def result(self):
+        """(key_tensors, agg_tensors, groups) of the last call, trimmed to its groups."""
+        g = self._groups.value
+        _, _, key_out, agg_out = self._keep
+        return ([t[:g] for t in key_out],
+                [t[: 2 * g] if a[2] == HUGEINT else t[:g] for t, a in zip(agg_out, self._aggs)], g)
+
+
 def sort_column(key, stream=None):
     import torch
     lib = load()
