@@ -81,18 +81,24 @@ __global__ void __launch_bounds__(256) KeyRangeKernel(RangeCols cols, uint64_t n
 //! latency with one wave generation on the chip).
 constexpr int kHistUnroll = 8;
 
+//! Counters per histogram bin in the key-range pass (spread over lanes; see KeyRangeHist8Kernel).
+constexpr int kHistCopies = 8;
+
 //! Min/max of the ordered key and the histograms of all 8 byte digits in one read. A digit that is the
 //! same for every key a wave reads in one step (the constant high bytes of a narrow key range) is counted
-//! with one LDS atomic instead of one per key.
+//! with one LDS atomic instead of one per key. Each bin has kHistCopies adjacent counters, lane l adding to
+//! copy l % kHistCopies: lanes of one wave that add to the same bin (equal keys of a skewed distribution, or
+//! a byte with few values) then mostly hit different banks instead of serializing on one word.
 template <typename T, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 KeyRangeHist8Kernel(const T *col, int width, int is_signed, uint64_t n, u64 *mins, u64 *maxs, uint32_t *hist8) {
-	__shared__ uint32_t s_hist[8 * 256];
-	for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) {
+	__shared__ uint32_t s_hist[8 * 256 * kHistCopies];
+	for (int i = threadIdx.x; i < 8 * 256 * kHistCopies; i += blockDim.x) {
 		s_hist[i] = 0;
 	}
 	__syncthreads();
 	const int lane = LaneId();
+	const uint32_t copy = (uint32_t)lane & (kHistCopies - 1);
 	u64 lo = ~0ull, hi = 0;
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	const uint64_t start = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -144,13 +150,13 @@ KeyRangeHist8Kernel(const T *col, int width, int is_signed, uint64_t n, u64 *min
 		for (int d = 0; d < 8; d++) {
 			if (((vary >> d) & 1u) == 0) {
 				if (lane == leader) {
-					atomicAdd(&s_hist[d * 256 + ((uint32_t)(ref >> (8 * d)) & 255u)], nvalid);
+					atomicAdd(&s_hist[(d * 256 + ((uint32_t)(ref >> (8 * d)) & 255u)) * kHistCopies], nvalid);
 				}
 			} else {
 #pragma unroll
 				for (int u = 0; u < kHistUnroll; u++) {
 					if (row0 + u * stride < n) {
-						atomicAdd(&s_hist[d * 256 + ((uint32_t)(ord[u] >> (8 * d)) & 255u)], 1u);
+						atomicAdd(&s_hist[(d * 256 + ((uint32_t)(ord[u] >> (8 * d)) & 255u)) * kHistCopies + copy], 1u);
 					}
 				}
 			}
@@ -158,7 +164,11 @@ KeyRangeHist8Kernel(const T *col, int width, int is_signed, uint64_t n, u64 *min
 	}
 	__syncthreads();
 	for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) {
-		uint32_t v = s_hist[i];
+		uint32_t v = 0;
+#pragma unroll
+		for (int c = 0; c < kHistCopies; c++) {
+			v += s_hist[i * kHistCopies + c];
+		}
 		if (v) {
 			FetchAddAgent(hist8 + i, v);
 		}
@@ -195,11 +205,13 @@ __global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, Pass
 		n = *n_dev; // row count written by an earlier kernel (n was its bound)
 	}
 	constexpr int PMAX = (64 + R - 1) / R;
-	__shared__ uint32_t s_hist[PMAX * RADIX];
-	for (int i = threadIdx.x; i < PMAX * RADIX; i += blockDim.x) {
+	constexpr int C = R == 8 ? 4 : 1; // lane copies per bin (as in KeyRangeHist8Kernel) where LDS allows
+	__shared__ uint32_t s_hist[PMAX * RADIX * C];
+	for (int i = threadIdx.x; i < PMAX * RADIX * C; i += blockDim.x) {
 		s_hist[i] = 0;
 	}
 	__syncthreads();
+	const uint32_t copy = (uint32_t)LaneId() & (C - 1);
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	for (uint64_t row0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; row0 < n; row0 += 4 * stride) {
 		u64 key[4];
@@ -213,14 +225,18 @@ __global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, Pass
 			if (row0 + u * stride < n) {
 				for (int p = 0; p < passes.npasses; p++) {
 					uint32_t d = (uint32_t)(key[u] >> passes.shift[p]) & ((1u << passes.bits[p]) - 1);
-					atomicAdd(&s_hist[p * RADIX + d], 1u);
+					atomicAdd(&s_hist[(p * RADIX + d) * C + copy], 1u);
 				}
 			}
 		}
 	}
 	__syncthreads();
 	for (int i = threadIdx.x; i < passes.npasses * RADIX; i += blockDim.x) {
-		uint32_t v = s_hist[i];
+		uint32_t v = 0;
+#pragma unroll
+		for (int c = 0; c < C; c++) {
+			v += s_hist[i * C + c];
+		}
 		if (v) {
 			FetchAddAgent(hist + i, v);
 		}
@@ -230,24 +246,24 @@ __global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, Pass
 //! Keys packed from several columns: one read of the key columns writes the packed keys (so the first radix
 //! pass reads one compact key instead of every key column, one column at a time) and builds the 8-bit digit
 //! histogram of every pass. A digit that a whole wave shares (a composite key's leading column, constant over
-//! long runs of rows) is counted by one lane with one LDS add instead of 64 colliding ones; each wave counts
-//! into its own copy of the histograms, summed at the end.
+//! long runs of rows) is counted by one lane with one LDS add instead of 64 colliding ones; the other digits
+//! go to one of kPackCopies counters per bin by lane, so lanes adding to one bin mostly hit different banks.
 template <typename KeyT>
 __global__ void __launch_bounds__(256) PackHistKernel(KeyPack pack, uint64_t n, PassList passes, KeyT *keys_out,
                                                       uint32_t *hist, const uint64_t *n_dev) {
 	constexpr int RADIX = 256;
 	constexpr int PMAX = 8;
-	constexpr int kWaves = 256 / kWave;
-	__shared__ uint32_t s_hist[kWaves][PMAX * RADIX];
+	constexpr int kPackCopies = 4;
+	__shared__ uint32_t s_hist[PMAX * RADIX * kPackCopies];
 	if (n_dev != nullptr) {
 		n = *n_dev; // row count written by an earlier kernel (n was its bound)
 	}
-	for (int i = threadIdx.x; i < kWaves * PMAX * RADIX; i += blockDim.x) {
-		(&s_hist[0][0])[i] = 0;
+	for (int i = threadIdx.x; i < PMAX * RADIX * kPackCopies; i += blockDim.x) {
+		s_hist[i] = 0;
 	}
 	__syncthreads();
 	const int lane = LaneId();
-	uint32_t *mine = s_hist[threadIdx.x / kWave];
+	const uint32_t copy = (uint32_t)lane & (kPackCopies - 1);
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	// every lane of a wave runs the same trip count (the ballots need the whole wave); four rows per lane are
 	// loaded before any is counted, to keep enough reads in flight
@@ -273,10 +289,10 @@ __global__ void __launch_bounds__(256) PackHistKernel(KeyPack pack, uint64_t n, 
 				if (__ballot(valid && d == d0) == live) {
 					// the whole wave holds one digit: one add instead of 64 to one address
 					if (lane == 0) {
-						atomicAdd(&mine[p * RADIX + d0], (uint32_t)Popc64(live));
+						atomicAdd(&s_hist[(p * RADIX + d0) * kPackCopies], (uint32_t)Popc64(live));
 					}
 				} else if (valid) {
-					atomicAdd(&mine[p * RADIX + d], 1u);
+					atomicAdd(&s_hist[(p * RADIX + d) * kPackCopies + copy], 1u);
 				}
 			}
 		}
@@ -285,8 +301,8 @@ __global__ void __launch_bounds__(256) PackHistKernel(KeyPack pack, uint64_t n, 
 	for (int i = threadIdx.x; i < passes.npasses * RADIX; i += blockDim.x) {
 		uint32_t v = 0;
 #pragma unroll
-		for (int w = 0; w < kWaves; w++) {
-			v += s_hist[w][i];
+		for (int c = 0; c < kPackCopies; c++) {
+			v += s_hist[i * kPackCopies + c];
 		}
 		if (v) {
 			FetchAddAgent(hist + i, v);
