@@ -367,6 +367,27 @@ def test_groupby_count_only_many_tiles(dev):
     np.testing.assert_array_equal(aggs[0].cpu().numpy()[:g], uc)
 
 
+@pytest.mark.parametrize("n", [1, 5, 2047, 2048, 2049, 4101, 100_003])
+@pytest.mark.parametrize("span_bits,distinct", [(30, 0), (30, 7), (40, 0), (40, 50)])
+def test_groupby_count_only_run_length_edges(dev, n, span_bits, distinct):
+    """COUNT-only GROUP BY through the run-length path around its 2,048-row tile and 256-row step edges, with
+    32-bit (30-bit span) and 64-bit (40-bit span) packed keys, all distinct or a few heavy groups: keys and
+    counts equal numpy's unique."""
+    import torch
+    rng = np.random.default_rng(n * 7 + span_bits + distinct)
+    if distinct:
+        vals = rng.integers(-(1 << (span_bits - 1)), 1 << (span_bits - 1), distinct, dtype=np.int64)
+        key = vals[rng.integers(0, distinct, n)]
+    else:
+        key = rng.integers(-(1 << (span_bits - 1)), 1 << (span_bits - 1), n, dtype=np.int64)
+    k = torch.from_numpy(key).to(dev)
+    keys, aggs, g = capi.group_by([k], [], [(capi.COUNT_STAR, -1, capi.INT64)], n)
+    uk, uc = np.unique(key, return_counts=True)
+    assert g == len(uk)
+    np.testing.assert_array_equal(keys[0].cpu().numpy()[:g], uk)
+    np.testing.assert_array_equal(aggs[0].cpu().numpy()[:g], uc)
+
+
 def test_groupby_int64_values_at_int32_bounds(dev):
     """INT64 aggregate inputs are carried through the sort as INT32 when they fit: values exactly at
     INT32_MIN / INT32_MAX fit, so SUM (HUGEINT), MIN and MAX come from the narrowed carry and must still be

@@ -401,65 +401,79 @@ __global__ void __launch_bounds__(256) HeadCountKernel(const KeyT *keys, uint64_
 
 
 //! Count-only reduce (every aggregate is a COUNT): a run-length encoding of the sorted keys. One wave per
-//! 2,048-row tile (the head-count tiles), each lane 32 consecutive keys in registers (vector loads; the
-//! key before them is one extra load that hits the neighbour's cache line); the tile's first group id comes
-//! from the head-count scan, so no tile takes a ticket or waits on another. Each group's key and first row
-//! are written; RleCountKernel turns first rows into counts.
+//! 2,048-row tile (the head-count tiles, so the tile's first group id comes from the head-count scan and no
+//! tile takes a ticket or waits on another). The tile is read in 8 steps of 256 consecutive rows, lane l
+//! holding rows 4l .. 4l + 3 of the step (one 16-byte load of 32-bit keys); a row's predecessor is the
+//! previous lane's last key (the previous step's last key for lane 0). A head's group id is the tile's first
+//! id + the heads before it: four ballots per step give the heads of the lower lanes, the lane's own bits
+//! those before it in the lane. Each group's key and first row are written in row order, so the stores of
+//! one step land in a few cache lines; RleCountKernel turns first rows into counts.
 template <typename KeyT>
 __global__ void __launch_bounds__(256) RleHeadsKernel(const KeyT *keys, uint64_t n, KeyOutDesc ko, u64 *starts,
                                                      const u64 *tile_prefix, const u64 *tile_carry,
                                                      uint64_t ntiles, const uint64_t *n_dev) {
+	constexpr int Q = 4;
+	constexpr int kSteps = kHeadLaneRows / Q;
+	constexpr uint64_t kTileRows = (uint64_t)kWave * kHeadLaneRows;
 	if (n_dev != nullptr) {
 		n = *n_dev;
-		ntiles = (n + (uint64_t)kWave * kHeadLaneRows - 1) / ((uint64_t)kWave * kHeadLaneRows);
+		ntiles = (n + kTileRows - 1) / kTileRows;
 	}
 	const uint64_t tile = (uint64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
 	if (tile >= ntiles) {
 		return;
 	}
 	const int lane = LaneId();
-	const uint64_t row0 = tile * ((uint64_t)kWave * kHeadLaneRows) + (uint64_t)lane * kHeadLaneRows;
-	KeyT k[kHeadLaneRows];
-	if (row0 + kHeadLaneRows <= n && (reinterpret_cast<uintptr_t>(keys) & 15) == 0) {
-		// 32 consecutive keys: 128 B (u32) or 256 B (u64), 16-byte aligned (row0 is a multiple of 32)
-		constexpr int kVecs = (int)(kHeadLaneRows * sizeof(KeyT) / 16);
-		const uint4 *src = reinterpret_cast<const uint4 *>(keys + row0);
-		uint4 v[kVecs];
+	const uint64_t tile_base = tile * kTileRows;
+	const bool vec = (reinterpret_cast<uintptr_t>(keys) & 15) == 0;
+	u64 gid = tile_prefix[tile] + tile_carry[tile / kScanTiles];
+	KeyT last = tile_base > 0 && tile_base <= n ? keys[tile_base - 1] : (KeyT)0; // the row before the step
 #pragma unroll
-		for (int j = 0; j < kVecs; j++) {
-			v[j] = src[j];
-		}
-		__builtin_memcpy(k, v, sizeof(k));
-	} else {
+	for (int j = 0; j < kSteps; j++) {
+		const uint64_t row0 = tile_base + (uint64_t)j * (kWave * Q) + (uint64_t)lane * Q;
+		KeyT k[Q];
+		if (vec && row0 + Q <= n) {
+			// 4 consecutive keys: 16 B (u32) or 32 B (u64), 16-byte aligned (row0 is a multiple of 4)
+			constexpr int kVecs = (int)(Q * sizeof(KeyT) / 16);
+			uint4 v[kVecs];
 #pragma unroll
-		for (int i = 0; i < kHeadLaneRows; i++) {
-			k[i] = row0 + i < n ? keys[row0 + i] : (KeyT)0;
-		}
-	}
-	KeyT before = row0 > 0 && row0 <= n ? keys[row0 - 1] : (KeyT)0;
-	uint32_t heads = 0; // bit i = row0 + i starts a group
+			for (int i = 0; i < kVecs; i++) {
+				v[i] = reinterpret_cast<const uint4 *>(keys + row0)[i];
+			}
+			__builtin_memcpy(k, v, sizeof(k));
+		} else {
 #pragma unroll
-	for (int i = 0; i < kHeadLaneRows; i++) {
-		const bool h = row0 + i < n && (row0 + i == 0 || k[i] != before);
-		before = k[i];
-		heads |= h ? (1u << i) : 0u;
-	}
-	const u64 hc = (u64)__popc(heads);
-	u64 incl = hc;
-	for (int off = 1; off < kWave; off <<= 1) {
-		const u64 up = __shfl_up(incl, off);
-		if (lane >= off) {
-			incl += up;
+			for (int q = 0; q < Q; q++) {
+				k[q] = row0 + q < n ? keys[row0 + q] : (KeyT)0;
+			}
 		}
-	}
-	u64 gid = tile_prefix[tile] + tile_carry[tile / kScanTiles] + incl - hc;
+		const KeyT up = __shfl_up(k[Q - 1], 1);
+		KeyT before = lane == 0 ? last : up;
+		uint32_t h = 0; // bit q: row0 + q starts a group
 #pragma unroll
-	for (int i = 0; i < kHeadLaneRows; i++) {
-		if ((heads >> i) & 1u) {
-			WriteKeys(ko, (u64)k[i], gid);
-			starts[gid] = row0 + i;
-			gid++;
+		for (int q = 0; q < Q; q++) {
+			const bool head = row0 + q < n && (row0 + q == 0 || k[q] != before);
+			before = k[q];
+			h |= head ? (1u << q) : 0u;
 		}
+		uint32_t below = 0, total = 0;
+#pragma unroll
+		for (int q = 0; q < Q; q++) {
+			const u64 b = __ballot((h >> q) & 1u);
+			below += __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+			total += (uint32_t)Popc64(b);
+		}
+		u64 g = gid + below;
+#pragma unroll
+		for (int q = 0; q < Q; q++) {
+			if ((h >> q) & 1u) {
+				WriteKeys(ko, (u64)k[q], g);
+				starts[g] = row0 + q;
+				g++;
+			}
+		}
+		gid += total;
+		last = __shfl(k[Q - 1], kWave - 1);
 	}
 }
 
