@@ -359,11 +359,14 @@ __global__ void __launch_bounds__(kRedBlock)
 
 //! Group heads per reduce tile (the same rows and the same head test as SegReduceKernel: row r starts a
 //! group when r == 0 or its key differs from row r - 1's; without keys only row 0 does). One wave per
-//! tile, each lane 32 consecutive keys read with vector loads; no barriers.
+//! tile, read in steps of 256 consecutive rows (lane l holds rows 4l .. 4l + 3 of a step: one 16-byte load
+//! of 32-bit keys, so each load instruction covers a contiguous 1 KiB); no barriers.
 constexpr int kHeadLaneRows = 32;
 template <typename KeyT>
 __global__ void __launch_bounds__(256) HeadCountKernel(const KeyT *keys, uint64_t n, uint64_t tile_rows,
                                                       uint64_t ntiles, uint32_t *counts, const uint64_t *n_dev) {
+	constexpr int Q = 4;
+	constexpr int kSteps = kHeadLaneRows / Q;
 	if (n_dev != nullptr) { // row count written by an earlier kernel; n was its bound
 		n = *n_dev;
 		ntiles = (n + tile_rows - 1) / tile_rows;
@@ -373,32 +376,49 @@ __global__ void __launch_bounds__(256) HeadCountKernel(const KeyT *keys, uint64_
 		return;
 	}
 	const int lane = LaneId();
-	u64 hc = 0;
+	uint32_t hc = 0;
 	if (keys == nullptr) {
-		hc = (tile == 0 && lane == 0 && n > 0) ? 1 : 0;
+		hc = (tile == 0 && lane == 0 && n > 0) ? 1u : 0u;
 	} else {
 		// tile_rows = 64 lanes x kHeadLaneRows rows
-		const uint64_t row0 = tile * tile_rows + (uint64_t)lane * kHeadLaneRows;
-		if (row0 < n) {
-			u64 kreg[kHeadLaneRows];
-			LoadRun<kHeadLaneRows, KeyT>(keys, row0, n, kreg);
-			KeyT before = row0 > 0 ? keys[row0 - 1] : (KeyT)0;
+		const uint64_t tile_base = tile * tile_rows;
+		const bool vec = (reinterpret_cast<uintptr_t>(keys) & 15) == 0;
+		KeyT last = tile_base > 0 && tile_base <= n ? keys[tile_base - 1] : (KeyT)0;
 #pragma unroll
-			for (int i = 0; i < kHeadLaneRows; i++) {
-				const KeyT k = (KeyT)kreg[i];
-				hc += (row0 + i < n && (row0 + i == 0 || k != before)) ? 1 : 0;
-				before = k;
+		for (int j = 0; j < kSteps; j++) {
+			const uint64_t row0 = tile_base + (uint64_t)j * (kWave * Q) + (uint64_t)lane * Q;
+			KeyT k[Q];
+			if (vec && row0 + Q <= n) {
+				constexpr int kVecs = (int)(Q * sizeof(KeyT) / 16);
+				uint4 v[kVecs];
+#pragma unroll
+				for (int i = 0; i < kVecs; i++) {
+					v[i] = reinterpret_cast<const uint4 *>(keys + row0)[i];
+				}
+				__builtin_memcpy(k, v, sizeof(k));
+			} else {
+#pragma unroll
+				for (int q = 0; q < Q; q++) {
+					k[q] = row0 + q < n ? keys[row0 + q] : (KeyT)0;
+				}
 			}
+			const KeyT up = __shfl_up(k[Q - 1], 1);
+			KeyT before = lane == 0 ? last : up;
+#pragma unroll
+			for (int q = 0; q < Q; q++) {
+				hc += (row0 + q < n && (row0 + q == 0 || k[q] != before)) ? 1u : 0u;
+				before = k[q];
+			}
+			last = __shfl(k[Q - 1], kWave - 1);
 		}
 	}
 	for (int off = kWave / 2; off > 0; off >>= 1) {
 		hc += __shfl_xor(hc, off);
 	}
 	if (lane == 0) {
-		counts[tile] = (uint32_t)hc;
+		counts[tile] = hc;
 	}
 }
-
 
 //! Count-only reduce (every aggregate is a COUNT): a run-length encoding of the sorted keys. One wave per
 //! 2,048-row tile (the head-count tiles, so the tile's first group id comes from the head-count scan and no
