@@ -246,24 +246,24 @@ __global__ void __launch_bounds__(256) HistKernel(KeyPack pack, uint64_t n, Pass
 //! Keys packed from several columns: one read of the key columns writes the packed keys (so the first radix
 //! pass reads one compact key instead of every key column, one column at a time) and builds the 8-bit digit
 //! histogram of every pass. A digit that a whole wave shares (a composite key's leading column, constant over
-//! long runs of rows) is counted by one lane with one LDS add instead of 64 colliding ones; the other digits
-//! go to one of kPackCopies counters per bin by lane, so lanes adding to one bin mostly hit different banks.
+//! long runs of rows) is counted by one lane with one LDS add instead of 64 colliding ones; each wave counts
+//! into its own copy of the histograms, summed at the end (lane copies in the same LDS measured 3% slower).
 template <typename KeyT>
 __global__ void __launch_bounds__(256) PackHistKernel(KeyPack pack, uint64_t n, PassList passes, KeyT *keys_out,
                                                       uint32_t *hist, const uint64_t *n_dev) {
 	constexpr int RADIX = 256;
 	constexpr int PMAX = 8;
-	constexpr int kPackCopies = 4;
-	__shared__ uint32_t s_hist[PMAX * RADIX * kPackCopies];
+	constexpr int kWaves = 256 / kWave;
+	__shared__ uint32_t s_hist[kWaves][PMAX * RADIX];
 	if (n_dev != nullptr) {
 		n = *n_dev; // row count written by an earlier kernel (n was its bound)
 	}
-	for (int i = threadIdx.x; i < PMAX * RADIX * kPackCopies; i += blockDim.x) {
-		s_hist[i] = 0;
+	for (int i = threadIdx.x; i < kWaves * PMAX * RADIX; i += blockDim.x) {
+		(&s_hist[0][0])[i] = 0;
 	}
 	__syncthreads();
 	const int lane = LaneId();
-	const uint32_t copy = (uint32_t)lane & (kPackCopies - 1);
+	uint32_t *mine = s_hist[threadIdx.x / kWave];
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	// every lane of a wave runs the same trip count (the ballots need the whole wave); four rows per lane are
 	// loaded before any is counted, to keep enough reads in flight
@@ -289,10 +289,10 @@ __global__ void __launch_bounds__(256) PackHistKernel(KeyPack pack, uint64_t n, 
 				if (__ballot(valid && d == d0) == live) {
 					// the whole wave holds one digit: one add instead of 64 to one address
 					if (lane == 0) {
-						atomicAdd(&s_hist[(p * RADIX + d0) * kPackCopies], (uint32_t)Popc64(live));
+						atomicAdd(&mine[p * RADIX + d0], (uint32_t)Popc64(live));
 					}
 				} else if (valid) {
-					atomicAdd(&s_hist[(p * RADIX + d) * kPackCopies + copy], 1u);
+					atomicAdd(&mine[p * RADIX + d], 1u);
 				}
 			}
 		}
@@ -301,8 +301,8 @@ __global__ void __launch_bounds__(256) PackHistKernel(KeyPack pack, uint64_t n, 
 	for (int i = threadIdx.x; i < passes.npasses * RADIX; i += blockDim.x) {
 		uint32_t v = 0;
 #pragma unroll
-		for (int c = 0; c < kPackCopies; c++) {
-			v += s_hist[i * kPackCopies + c];
+		for (int w = 0; w < kWaves; w++) {
+			v += s_hist[w][i];
 		}
 		if (v) {
 			FetchAddAgent(hist + i, v);
