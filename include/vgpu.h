@@ -430,6 +430,10 @@ int vgpu_server_run_open_loop(vgpu_server *server, const int32_t *tmpl, const ui
 
 /* When on, every kernel launch is bracketed by HIP events on its own stream. */
 int vgpu_profile_enable(int32_t on);
+/* Restricts the bracketing to launches whose profile name is `name` (NULL or "": every launch), so one
+ * kernel can be timed inside a timed loop without events around the others. No reference counterpart (a
+ * measurement control of this library). */
+int vgpu_profile_only(const char *name);
 int vgpu_profile_reset(void);
 /* JSON array [{"name":..,"ms":..,"launches":..,"bytes":..}, ...] into buf (synchronizes). Returns the
  * needed size in *needed when buf is too small. */

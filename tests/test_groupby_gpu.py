@@ -268,6 +268,32 @@ def test_prepared_group_by_matches_group_by(dev):
             assert torch.equal(a, b)
 
 
+def test_profile_only_times_the_named_kernel(dev):
+    """With a profile name filter, only launches under that name are bracketed and reported (the benchmark
+    times the dominant kernel this way inside its timed loop); without it every launch is."""
+    import torch
+    rng = np.random.default_rng(37)
+    n = 1 << 20
+    kd = torch.from_numpy(rng.integers(0, 1 << 18, n).astype(np.int64)).to(dev)
+    vd = torch.from_numpy(rng.integers(-100, 100, n).astype(np.int64)).to(dev)
+    aggs = [(capi.COUNT_STAR, -1, capi.INT64), (capi.SUM, 0, capi.HUGEINT)]
+    try:
+        capi.profile_reset()
+        capi.profile_enable(True)
+        capi.group_by([kd], [vd], aggs, n)
+        every = {e["name"] for e in capi.profile_read()}
+        assert "radix_pass_first" in every and len(every) > 1
+        capi.profile_reset()
+        capi.profile_only("radix_pass_first")
+        capi.group_by([kd], [vd], aggs, n)
+        capi.group_by([kd], [vd], aggs, n)
+        only = capi.profile_read()
+        assert [e["name"] for e in only] == ["radix_pass_first"] and only[0]["launches"] == 2
+    finally:
+        capi.profile_enable(False)
+        capi.profile_only(None)
+
+
 def test_autotuned_shape_reused_with_other_call_settings(dev):
     """The autotuner caches a pass variant per sort shape, not the settings of the call that tuned it: a later
     GROUP BY of the same shape with an aggregate mix the fused last pass does not cover must take the plain

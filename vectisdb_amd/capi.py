@@ -129,6 +129,7 @@ _PROTOS = {
     "vgpu_join_pairs": (C.c_int, [C.POINTER(Column), C.POINTER(Column), C.c_void_p, C.c_void_p, C.c_uint64,
                                   C.POINTER(C.c_uint64), C.c_void_p]),
     "vgpu_profile_enable": (C.c_int, [C.c_int32]),
+    "vgpu_profile_only": (C.c_int, [C.c_char_p]),
     "vgpu_profile_reset": (C.c_int, []),
     "vgpu_profile_read": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vgpu_pool_stats": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -678,6 +679,11 @@ def sort_autotune_report():
 
 def profile_enable(on=True):
     check(load().vgpu_profile_enable(1 if on else 0))
+
+
+def profile_only(name=None):
+    """Times only the launches profiled under `name` (None: every launch) while profiling is enabled."""
+    check(load().vgpu_profile_only(name.encode() if name else None))
 
 
 def profile_reset():

@@ -53,6 +53,9 @@ struct Scratch {
 //! with ProfileSnapshot. Used by the bench to time the dominant kernel on its own stream.
 void ProfileEnable(bool on);
 bool ProfileEnabled();
+//! Times only the scopes named `name` while enabled (nullptr or "": every scope). Each timed scope records
+//! two events on its stream, so timing one kernel of an operator leaves the others' launches unchanged.
+void ProfileOnly(const char *name);
 struct ProfileEntry {
 	std::string name;
 	double total_ms = 0;

@@ -58,6 +58,7 @@ struct ProfRecord {
 };
 std::mutex g_prof_mutex;
 bool g_prof_on = false;
+std::string g_prof_only; // when set, only scopes of this name are timed
 std::vector<ProfRecord> g_prof_pending;
 std::map<std::string, ProfileEntry> g_prof_totals;
 //! Events of read-out records, reused by later scopes (creating two events per launch cost microseconds of
@@ -89,10 +90,18 @@ bool ProfileEnabled() {
 	return g_prof_on;
 }
 
+void ProfileOnly(const char *name) {
+	std::lock_guard<std::mutex> lock(g_prof_mutex);
+	g_prof_only = name != nullptr ? name : "";
+}
+
 ProfileScope::ProfileScope(const char *name, hipStream_t stream, double bytes)
     : name_(name), stream_(stream), bytes_(bytes) {
-	if (!ProfileEnabled()) {
-		return;
+	{
+		std::lock_guard<std::mutex> lock(g_prof_mutex);
+		if (!g_prof_on || (!g_prof_only.empty() && g_prof_only != name_)) {
+			return;
+		}
 	}
 	// launches captured into a graph are not timed here (the events would never be recorded)
 	hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;

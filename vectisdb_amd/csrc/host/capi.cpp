@@ -249,6 +249,10 @@ int vgpu_profile_enable(int32_t on) {
 	return Guard([&] { ProfileEnable(on != 0); });
 }
 
+int vgpu_profile_only(const char *name) {
+	return Guard([&] { ProfileOnly(name); });
+}
+
 int vgpu_profile_reset(void) {
 	return Guard([&] { ProfileReset(); });
 }
