@@ -239,8 +239,8 @@ def test_groupby_under_sort_autotuner(dev):
         assert mine, report
         for r in mine:
             assert r["radix_bits"] in (8, 11) and r["items"] in (16, 32) and r["prefetch"] in (0, 1)
-            assert r["block"] in (256, 512)
-            assert r["candidates"] >= 5 and r["ms"] > 0
+            assert r["block"] in (256, 512, 1024)
+            assert r["candidates"] >= 6 and r["ms"] > 0
     finally:
         capi.sort_autotune(False)
 
@@ -336,11 +336,12 @@ def test_autotuned_shape_reused_with_other_call_settings(dev):
 
 @pytest.mark.parametrize("variant", [
     {"VGPU_SORT_BLOCK": "512"},
+    {"VGPU_SORT_BLOCK": "1024"},
     {"VGPU_SORT_ITEMS": "32", "VGPU_SORT_PREFETCH": "0"},
     {"VGPU_SORT_ITEMS": "32", "VGPU_SORT_PREFETCH": "1"},
     {"VGPU_SORT_ITEMS": "16", "VGPU_SORT_PREFETCH": "0"},
     {"VGPU_SORT_ITEMS": "16", "VGPU_SORT_PREFETCH": "1"},
-], ids=["b512_i16", "i32_load_late", "i32_prefetch", "i16_load_late", "i16_prefetch"])
+], ids=["b512_i16", "b1024_i16", "i32_load_late", "i32_prefetch", "i16_load_late", "i16_prefetch"])
 def test_sort_variants_forced(dev, monkeypatch, variant):
     """Every 8-bit radix-pass variant (threads per tile x items per thread x payload prefetch), forced through
     the tuning overrides: GROUP BY bit-exact against the oracle with row counts around the 4096- and

@@ -11,7 +11,7 @@ namespace vgpu {
 
 namespace {
 //! Optional overrides for sweeps: VGPU_SORT_ITEMS (16/32), VGPU_SORT_PREFETCH (0/1), VGPU_SORT_BLOCK
-//! (256/512; 512 runs with 16 items and no prefetch only).
+//! (256/512/1024; 512 and 1024 run with 16 items and no prefetch only).
 void ApplyVariantOverrides(SortTuning &t) {
 	if (const char *env = std::getenv("VGPU_SORT_BLOCK")) {
 		const int b = std::atoi(env);
